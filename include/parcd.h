@@ -1,0 +1,173 @@
+/*
+ * parcd.h -- C ABI of the MI355X (gfx950) community-detection hot path.
+ *
+ * Plain C types only: int32 ids and offsets, host pointers that are borrowed
+ * for the duration of a call, an opaque context that owns all device memory
+ * and one HIP stream.  Every function returns PCD_OK (0) or a negative
+ * PCD_E* status; pcd_last_error() gives the message of the last failure on
+ * the calling thread.  Nothing here exits the process or throws.
+ *
+ * Each entry point names the reference interface it replaces (stijnh/Par-CD,
+ * src/).  Semantics follow that code; differences are stated per function.
+ */
+#ifndef PARCD_H
+#define PARCD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCD_OK 0
+#define PCD_E_INVALID (-1)   /* bad argument */
+#define PCD_E_HIP (-2)       /* HIP runtime error */
+#define PCD_E_NOMEM (-3)     /* device or host allocation failed / over budget */
+#define PCD_E_IO (-4)        /* file error */
+#define PCD_E_FORMAT (-5)    /* malformed PCD file or edge list */
+#define PCD_E_STATE (-6)     /* call out of order (no graph, no labels, ...) */
+#define PCD_E_NODEVICE (-7)  /* no usable gfx950 device */
+
+#define PCD_NEW_LABEL (-1)   /* common.hpp:20 NEW_LABEL (~0) */
+
+/* Metric rules (rules.hpp:9-75). */
+#define PCD_RULE_MODULARITY 0
+#define PCD_RULE_CPM 1
+#define PCD_RULE_WCC 2
+#define PCD_RULE_CONDUCTANCE 3
+
+/* Tie order among equal best gains.  RANDOM is a uniform choice keyed by a
+ * counter hash of (seed, step, vertex|community, label) -- the counterpart of
+ * the reservoir sampling at refinement.cpp:200-204,291-294; LOWEST takes the
+ * lowest label (the deterministic synchronous mode). */
+#define PCD_TIE_RANDOM 0
+#define PCD_TIE_LOWEST 1
+
+typedef struct pcd_rule {
+    int kind;      /* PCD_RULE_* */
+    double lambda; /* CPM resolution (CPMRule(l), rules.hpp:52) */
+} pcd_rule;
+
+typedef struct pcd_ctx pcd_ctx;
+
+/* ---------------------------------------------------------------- context */
+/* Replaces framework-cuda.cuh:14-15 cuda_init/cuda_deinit. */
+int pcd_ctx_create(int device, pcd_ctx **out);
+int pcd_ctx_destroy(pcd_ctx *ctx);
+const char *pcd_last_error(void);
+int pcd_device_count(int *count);
+int pcd_synchronize(pcd_ctx *ctx);
+
+/* ------------------------------------------------------------------ graph */
+/* Uploads a CSR graph (graph.hpp:6-40 Graph{n, m, indices, adj}).  indices
+ * has n+1 entries, adj 2m entries, rows sorted ascending without loops or
+ * duplicates (what `convert` writes).  Replaces framework-cuda.cuh:16
+ * cuda_transfer_graph.  Previous graph, labels and statistics are dropped. */
+int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indices, const int32_t *adj);
+int pcd_graph_info(pcd_ctx *ctx, int *n, int *m, int *max_degree);
+
+/* ----------------------------------------------------------------- labels */
+/* Replaces framework-cuda.cuh:17-18 cuda_labels_to_device/_from_device.
+ * Labels are >= 0 or PCD_NEW_LABEL. */
+int pcd_labels_upload(pcd_ctx *ctx, const int32_t *labels);
+int pcd_labels_download(pcd_ctx *ctx, int32_t *labels);
+
+/* framework.cpp:146-170 compress_labels: dense relabel in order of first
+ * occurrence over vertex ids, each NEW_LABEL becomes a fresh id.  Bit-exact
+ * with the reference.  *num_labels (may be NULL) receives k. */
+int pcd_compress_labels(pcd_ctx *ctx, int *num_labels);
+
+/* framework.cpp:103-143 collect_statistics: sizes, int_volume, ext_volume
+ * for the current (compressed) labels.  k = max label + 1.  Bit-exact. */
+int pcd_collect_statistics(pcd_ctx *ctx, int *num_labels);
+/* Copies the statistics to host arrays of k entries each (any may be NULL). */
+int pcd_stats_download(pcd_ctx *ctx, int32_t *sizes, int32_t *int_volume, int32_t *ext_volume);
+
+/* Rule::calculate (rules.hpp:11; rules.cpp:9-22, 58-70, 121-129, 150-235)
+ * on the current labels and statistics.  Modularity and CPM use exact
+ * integer sums of the community totals; WCC and conductance sum in FP64. */
+int pcd_metric(pcd_ctx *ctx, pcd_rule rule, double *score);
+
+/* ------------------------------------------------------------- refinement */
+/* One label-propagation step, PropagationRefinement::refine
+ * (refinement.cpp:156-224): each vertex is active with probability frac
+ * (counter hash of (seed, step, v) against frac*RAND_MAX); an active vertex
+ * joins the neighbour community of best score_join if best + score_leave > 0
+ * and > score_leave, else leaves to a fresh singleton if score_leave > 0, else
+ * stays.  All reads see the labels and statistics before the step.  The new
+ * labels replace the current ones (NEW_LABEL for leavers); call
+ * pcd_compress_labels before the next statistics.  Needs current statistics. */
+int pcd_propagation_step(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie);
+
+/* One parallel-merging step, MergingRefinement::refine
+ * (refinement.cpp:227-330): per round every unmatched community picks the
+ * neighbour community of best score_merge (>= 0), matched pairs are found by
+ * handshake, and matched communities take min(l, partner).  Community-pair
+ * link counts come from the contraction below.  Needs current statistics. */
+int pcd_merging_step(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie);
+
+/* Contraction: the community graph of the current labels as the list of
+ * (l, k, links) for l != k, sorted by (l, k) -- the table MergingRefinement
+ * builds at refinement.cpp:270-280 and framework-cuda.cu:744-807 computes by
+ * sort + reduce-by-key.  *num_pairs receives U; arrays (may be NULL) need U
+ * entries; call first with NULL arrays to get U. */
+int pcd_contract(pcd_ctx *ctx, int64_t *num_pairs, int32_t *l, int32_t *k, int32_t *links);
+
+/* Graph::count_triangles (graph.cpp:150-218): per-vertex tri (twice the
+ * triangles at v), tri_deg (incident edges on a triangle) and the mean local
+ * clustering coefficient cc.  Results stay on the device for WCC; host
+ * arrays (n entries) and cc may be NULL. */
+int pcd_count_triangles(pcd_ctx *ctx, int32_t *tri, int32_t *tri_deg, double *cc);
+/* Overrides the clustering coefficient the WCC gains use (default: the one
+ * computed by pcd_count_triangles). */
+int pcd_set_clustering_coef(pcd_ctx *ctx, double cc);
+
+/* ------------------------------------------------------------ measurement */
+/* Per-phase device time (HIP events on the context stream) of the most
+ * recent call of each phase, in milliseconds. */
+#define PCD_PHASE_COMPRESS 0
+#define PCD_PHASE_STATS 1
+#define PCD_PHASE_METRIC 2
+#define PCD_PHASE_PROPAGATE 3
+#define PCD_PHASE_MERGE 4
+#define PCD_PHASE_TRIANGLES 5
+#define PCD_PHASE_COUNT 6
+int pcd_enable_timing(pcd_ctx *ctx, int enable);
+int pcd_phase_time(pcd_ctx *ctx, int phase, double *ms);
+/* Work counters of the most recent propagation step: active vertices,
+ * their edge entries E_A, and distinct neighbour labels D (for the
+ * algorithmic-bytes model).  Counting costs an extra pass; enable first. */
+int pcd_enable_counters(pcd_ctx *ctx, int enable);
+int pcd_propagation_counters(pcd_ctx *ctx, int64_t *active, int64_t *entries, int64_t *distinct);
+
+/* --------------------------------------------------------- host utilities */
+/* graph.cpp:40-105 Graph::read: reads a PCD file into malloc'ed arrays
+ * (free with pcd_free).  Same validation: magic, sizes, no trailing bytes. */
+int pcd_read_pcd(const char *path, int *n, int *m, int32_t **indices, int32_t **adj, int32_t **map);
+/* convert.cpp:244-278 write_edges (file bytes only). */
+int pcd_write_pcd(const char *path, int n, int m, const int32_t *indices, const int32_t *adj,
+                  const int32_t *map);
+/* convert.cpp:14-74 read_edges: text edge list -> arrays (free with pcd_free). */
+int pcd_read_edge_text(const char *path, int64_t *num_edges, int32_t **a, int32_t **b, int *num_nodes);
+/* convert.cpp:76-196: edge list -> degree-sorted, loop- and duplicate-free
+ * CSR identical to what `convert` writes.  Outputs malloc'ed.  Self-loop
+ * input edges are dropped (convert.cpp:94-103 leaves an uninitialised slot
+ * for them, so the reference output is undefined there). */
+int pcd_convert_edges(int num_nodes, int64_t num_edges, const int32_t *a, const int32_t *b, int *n, int *m,
+                      int32_t **indices, int32_t **adj, int32_t **map);
+void pcd_free(void *p);
+
+/* Seeded synthetic graphs (edge lists, loop-free; duplicates possible).
+ * SplitMix64 streams; same seed -> same edges on any host. */
+int pcd_gen_sbm(uint64_t seed, int num_blocks, const int32_t *block_sizes, double p_in, double p_out,
+                int64_t *num_edges, int32_t **a, int32_t **b, int *num_nodes);
+int pcd_gen_rmat(uint64_t seed, int scale, int64_t num_draws, double pa, double pb, double pc,
+                 int64_t *num_edges, int32_t **a, int32_t **b, int *num_nodes);
+int pcd_gen_chung_lu(uint64_t seed, int n, double exponent, int dmin, int dmax, int num_comms, int cmin,
+                     int cmax, double boost, int64_t *num_edges, int32_t **a, int32_t **b, int *num_nodes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARCD_H */

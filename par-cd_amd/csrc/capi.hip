@@ -1,0 +1,544 @@
+// C ABI entry points: context, graph and label transfer, phase dispatch.
+#include <algorithm>
+#include <cstring>
+
+#include "../host/hostutil.hpp"
+#include "prims.h"
+
+using namespace pcd;
+
+namespace pcd {
+
+int dev_alloc(pcd_ctx *ctx, void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(PCD_E_NOMEM, "hipMalloc(" + std::to_string(bytes) + " bytes) failed: " + hipGetErrorString(e));
+    }
+    ctx->allocations.push_back(*p);
+    return PCD_OK;
+}
+
+void dev_free(pcd_ctx *ctx, void *p) {
+    if (!p) return;
+    auto it = std::find(ctx->allocations.begin(), ctx->allocations.end(), p);
+    if (it != ctx->allocations.end()) ctx->allocations.erase(it);
+    (void)hipFree(p);
+}
+
+RuleParams make_rule(pcd_ctx *ctx, pcd_rule rule) {
+    RuleParams r;
+    r.kind = rule.kind;
+    r.lambda = rule.lambda;
+    r.m = ctx->m;
+    r.cc = ctx->cc;
+    return r;
+}
+
+void phase_begin(pcd_ctx *ctx, int phase) {
+    if (ctx->timing) (void)hipEventRecord(ctx->ev0[phase], ctx->stream);
+}
+
+void phase_end(pcd_ctx *ctx, int phase) {
+    if (ctx->timing) {
+        (void)hipEventRecord(ctx->ev1[phase], ctx->stream);
+        ctx->ev_valid[phase] = true;
+    }
+}
+
+}  // namespace pcd
+
+static int check_ctx(pcd_ctx *ctx) {
+    if (!ctx) return fail(PCD_E_INVALID, "null context");
+    PCD_HIP(hipSetDevice(ctx->device));
+    return PCD_OK;
+}
+
+static int check_graph(pcd_ctx *ctx) {
+    PCD_TRY(check_ctx(ctx));
+    if (!ctx->has_graph) return fail(PCD_E_STATE, "no graph uploaded");
+    return PCD_OK;
+}
+
+static int check_labels(pcd_ctx *ctx) {
+    PCD_TRY(check_graph(ctx));
+    if (!ctx->has_labels) return fail(PCD_E_STATE, "no labels uploaded");
+    return PCD_OK;
+}
+
+static int check_rule(pcd_rule rule) {
+    if (rule.kind < PCD_RULE_MODULARITY || rule.kind > PCD_RULE_CONDUCTANCE)
+        return fail(PCD_E_INVALID, "unknown rule kind " + std::to_string(rule.kind));
+    return PCD_OK;
+}
+
+extern "C" int pcd_device_count(int *count) {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    *count = c;
+    return PCD_OK;
+}
+
+extern "C" int pcd_ctx_create(int device, pcd_ctx **out) {
+    if (!out) return fail(PCD_E_INVALID, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return fail(PCD_E_NODEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= count) return fail(PCD_E_INVALID, "device index out of range");
+    hipDeviceProp_t prop;
+    PCD_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(PCD_E_NODEVICE, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
+    PCD_HIP(hipSetDevice(device));
+    pcd_ctx *ctx = new pcd_ctx();
+    ctx->device = device;
+    ctx->cu_count = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return fail(PCD_E_HIP, "hipStreamCreate failed");
+    }
+    for (int i = 0; i < PCD_PHASE_COUNT; i++) {
+        (void)hipEventCreate(&ctx->ev0[i]);
+        (void)hipEventCreate(&ctx->ev1[i]);
+    }
+    if (hipHostMalloc((void **)&ctx->h_result, 64, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->h_small, 256, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->h_small64, 256, hipHostMallocDefault) != hipSuccess) {
+        pcd_ctx_destroy(ctx);
+        return fail(PCD_E_NOMEM, "pinned host allocation failed");
+    }
+    int rc = dev_alloc_t(ctx, &ctx->d_k, 4);
+    if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_acc, 16);
+    if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_dacc, 4096);
+    if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_result, 8);
+    if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_counts, NUM_BINS + 16);
+    if (rc != PCD_OK) {
+        pcd_ctx_destroy(ctx);
+        return rc;
+    }
+    ctx->dacc_cap = 4096;
+    *out = ctx;
+    return PCD_OK;
+}
+
+static void free_graph_state(pcd_ctx *ctx) {
+    // everything except the per-context scalars allocated at creation
+    void *keep[] = {ctx->d_k, ctx->d_acc, ctx->d_dacc, ctx->d_result, ctx->d_counts};
+    std::vector<void *> all = ctx->allocations;
+    for (void *p : all) {
+        bool k = false;
+        for (void *q : keep) k = k || (p == q);
+        if (!k) dev_free(ctx, p);
+    }
+    ctx->d_indices = ctx->d_adj = ctx->d_order = nullptr;
+    ctx->d_hub_items = nullptr;
+    ctx->d_hub_table_off = nullptr;
+    ctx->d_hub_table_log = ctx->d_hub_keys = ctx->d_hub_cnts = nullptr;
+    ctx->d_stat_items = nullptr;
+    ctx->d_labels = ctx->d_labels_alt = nullptr;
+    ctx->d_stats = nullptr;
+    ctx->d_first = ctx->d_newid = ctx->d_active = ctx->d_hub_active = nullptr;
+    ctx->d_scan_part = nullptr;
+    ctx->scan_part_cap = 0;
+    ctx->d_tri = ctx->d_tri_deg = ctx->d_et = ctx->d_lowpos = ctx->d_it = ctx->d_itd = nullptr;
+    ctx->d_keys0 = ctx->d_keys1 = nullptr;
+    ctx->keys_cap = 0;
+    ctx->d_pair_links = ctx->d_pair_head = nullptr;
+    ctx->d_rs_hist = nullptr;
+    ctx->rs_hist_cap = 0;
+    ctx->d_bp = ctx->d_found = nullptr;
+    ctx->d_best_ord = ctx->d_best_key = nullptr;
+    ctx->first_cap = 0;
+    ctx->has_graph = ctx->has_labels = ctx->has_stats = ctx->has_triangles = false;
+    ctx->labels_compressed = false;
+    ctx->num_hubs = ctx->num_hub_items = ctx->num_stat_items = 0;
+    ctx->cc = 0.0;
+}
+
+extern "C" int pcd_ctx_destroy(pcd_ctx *ctx) {
+    if (!ctx) return PCD_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->allocations) (void)hipFree(p);
+    ctx->allocations.clear();
+    for (int i = 0; i < PCD_PHASE_COUNT; i++) {
+        if (ctx->ev0[i]) (void)hipEventDestroy(ctx->ev0[i]);
+        if (ctx->ev1[i]) (void)hipEventDestroy(ctx->ev1[i]);
+    }
+    if (ctx->h_result) (void)hipHostFree(ctx->h_result);
+    if (ctx->h_small) (void)hipHostFree(ctx->h_small);
+    if (ctx->h_small64) (void)hipHostFree(ctx->h_small64);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PCD_OK;
+}
+
+extern "C" int pcd_synchronize(pcd_ctx *ctx) {
+    PCD_TRY(check_ctx(ctx));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    return PCD_OK;
+}
+
+static int ceil_log2_host(int64_t x) {
+    int b = 0;
+    while (((int64_t)1 << b) < x) b++;
+    return b;
+}
+
+// graph.hpp:6-40 layout; builds degree bins and static work items.
+extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indices, const int32_t *adj) {
+    PCD_TRY(check_ctx(ctx));
+    if (n <= 0 || m < 0 || !indices || (m > 0 && !adj)) return fail(PCD_E_INVALID, "bad graph arguments");
+    if ((int64_t)2 * m > INT32_MAX) return fail(PCD_E_INVALID, "2m exceeds int32");
+    if (indices[0] != 0 || indices[n] != 2 * m) return fail(PCD_E_FORMAT, "indices[0] != 0 or indices[n] != 2m");
+    // validation: monotone offsets, ids in range, rows strictly ascending
+    std::vector<int> bad(host_threads() + 1, 0);
+    parallel_for(n, [&](int64_t lo, int64_t hi) {
+        int b = 0;
+        for (int64_t v = lo; v < hi && !b; v++) {
+            if (indices[v + 1] < indices[v]) { b = 1; break; }
+            for (int32_t j = indices[v]; j < indices[v + 1]; j++) {
+                int32_t u = adj[j];
+                if (u < 0 || u >= n || u == v || (j > indices[v] && adj[j - 1] >= u)) { b = 1; break; }
+            }
+        }
+        if (b) bad[0] = 1;
+    }, 4096);
+    if (bad[0]) return fail(PCD_E_FORMAT, "adjacency rows must be ascending, loop-free, with ids in [0,n)");
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    free_graph_state(ctx);
+    ctx->n = n;
+    ctx->m = m;
+    const int64_t E = (int64_t)2 * m;
+    // degree order
+    std::vector<int32_t> deg(n);
+    int maxd = 0;
+    bool sorted = true;
+    for (int v = 0; v < n; v++) {
+        deg[v] = indices[v + 1] - indices[v];
+        maxd = std::max(maxd, deg[v]);
+        if (v > 0 && deg[v] > deg[v - 1]) sorted = false;
+    }
+    ctx->max_degree = maxd;
+    std::vector<int32_t> order;
+    if (!sorted) {
+        // stable counting sort by degree, descending
+        std::vector<int64_t> bucket((size_t)maxd + 2, 0);
+        for (int v = 0; v < n; v++) bucket[maxd - deg[v] + 1]++;
+        for (int d = 0; d <= maxd; d++) bucket[d + 1] += bucket[d];
+        order.resize(n);
+        for (int v = 0; v < n; v++) order[bucket[maxd - deg[v]]++] = v;
+    }
+    auto deg_at = [&](int p) { return sorted ? deg[p] : deg[order[p]]; };
+    // bin boundaries by degree thresholds (positions are degree-descending)
+    const int thr[NUM_BINS] = {INT32_MAX, ctx->tune.block_max, BIN_W1024_MAX, BIN_W256_MAX, 64, 32, 16, 8, 4};
+    // bin b holds degrees in (thr[b+1], thr[b]]; HUB holds deg > block_max
+    auto first_pos_le = [&](int t) {  // first position with degree <= t
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            int mid = (lo + hi) / 2;
+            if (deg_at(mid) <= t) hi = mid;
+            else lo = mid + 1;
+        }
+        return lo;
+    };
+    ctx->bins.start[0] = 0;
+    for (int b = 1; b < NUM_BINS; b++) ctx->bins.start[b] = first_pos_le(thr[b]);
+    ctx->bins.start[NUM_BINS] = n;
+    // hub items and tables
+    const int hub_end = ctx->bins.start[BIN_BLOCK];
+    std::vector<int4> hub_items;
+    std::vector<int64_t> toff(hub_end + 1, 0);
+    std::vector<int32_t> tlog(hub_end + 1, 0);
+    for (int h = 0; h < hub_end; h++) {
+        int v = sorted ? h : order[h];
+        for (int b = indices[v]; b < indices[v + 1]; b += ctx->tune.hub_chunk)
+            hub_items.push_back(make_int4(h, v, b, std::min(indices[v + 1], b + ctx->tune.hub_chunk)));
+        tlog[h] = std::max(6, ceil_log2_host(2 * (int64_t)deg[v]));
+        toff[h + 1] = toff[h] + ((int64_t)1 << tlog[h]);
+    }
+    ctx->num_hubs = hub_end;
+    ctx->num_hub_items = (int)hub_items.size();
+    ctx->hub_table_slots = toff[hub_end];
+    // statistics items: rows longer than 64, in chunks
+    std::vector<int4> stat_items;
+    const int big_end = ctx->bins.start[BIN_G64];
+    for (int p = 0; p < big_end; p++) {
+        int v = sorted ? p : order[p];
+        for (int b = indices[v]; b < indices[v + 1]; b += ctx->tune.stat_chunk)
+            stat_items.push_back(make_int4(v, b, std::min(indices[v + 1], b + ctx->tune.stat_chunk), b == indices[v]));
+    }
+    ctx->num_stat_items = (int)stat_items.size();
+
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_indices, n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_adj, E + 1));
+    PCD_HIP(hipMemcpy(ctx->d_indices, indices, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
+    if (E > 0) PCD_HIP(hipMemcpy(ctx->d_adj, adj, sizeof(int32_t) * E, hipMemcpyHostToDevice));
+    if (!sorted) {
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_order, n));
+        PCD_HIP(hipMemcpy(ctx->d_order, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    }
+    if (!hub_items.empty()) {
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_items, hub_items.size()));
+        PCD_HIP(hipMemcpy(ctx->d_hub_items, hub_items.data(), sizeof(int4) * hub_items.size(), hipMemcpyHostToDevice));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_table_off, hub_end + 1));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_table_log, hub_end + 1));
+        PCD_HIP(hipMemcpy(ctx->d_hub_table_off, toff.data(), sizeof(int64_t) * (hub_end + 1), hipMemcpyHostToDevice));
+        PCD_HIP(hipMemcpy(ctx->d_hub_table_log, tlog.data(), sizeof(int32_t) * (hub_end + 1), hipMemcpyHostToDevice));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_keys, ctx->hub_table_slots));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_cnts, ctx->hub_table_slots));
+        PCD_TRY(fill_i32(ctx, ctx->d_hub_keys, EMPTY_KEY, ctx->hub_table_slots));
+        PCD_TRY(fill_i32(ctx, ctx->d_hub_cnts, 0, ctx->hub_table_slots));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_active, hub_end + 1));
+    }
+    if (!stat_items.empty()) {
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_stat_items, stat_items.size()));
+        PCD_HIP(hipMemcpy(ctx->d_stat_items, stat_items.data(), sizeof(int4) * stat_items.size(),
+                          hipMemcpyHostToDevice));
+    }
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_labels, n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_labels_alt, n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_stats, n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_active, n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_first, n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_newid, n + 1));
+    ctx->first_cap = n + 1;
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->has_graph = true;
+    return PCD_OK;
+}
+
+extern "C" int pcd_graph_info(pcd_ctx *ctx, int *n, int *m, int *max_degree) {
+    PCD_TRY(check_graph(ctx));
+    if (n) *n = ctx->n;
+    if (m) *m = ctx->m;
+    if (max_degree) *max_degree = ctx->max_degree;
+    return PCD_OK;
+}
+
+extern "C" int pcd_labels_upload(pcd_ctx *ctx, const int32_t *labels) {
+    PCD_TRY(check_graph(ctx));
+    if (!labels) return fail(PCD_E_INVALID, "null labels");
+    int mx = -1;
+    bool has_new = false;
+    for (int v = 0; v < ctx->n; v++) {
+        if (labels[v] < -1) return fail(PCD_E_INVALID, "labels must be >= 0 or NEW_LABEL");
+        if (labels[v] == -1) has_new = true;
+        mx = std::max(mx, labels[v]);
+    }
+    int64_t bound = (int64_t)mx + 1;
+    if (bound + 1 > ctx->first_cap) {
+        dev_free(ctx, ctx->d_first);
+        dev_free(ctx, ctx->d_newid);
+        ctx->d_first = ctx->d_newid = nullptr;
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_first, bound + 1));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_newid, bound + 1));
+        ctx->first_cap = bound + 1;
+    }
+    PCD_HIP(hipMemcpyAsync(ctx->d_labels, labels, sizeof(int32_t) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->label_bound = (int)bound;
+    ctx->has_labels = true;
+    ctx->labels_compressed = false;
+    ctx->labels_have_new = has_new;
+    ctx->has_stats = false;
+    return PCD_OK;
+}
+
+extern "C" int pcd_labels_download(pcd_ctx *ctx, int32_t *labels) {
+    PCD_TRY(check_labels(ctx));
+    PCD_HIP(hipMemcpyAsync(labels, ctx->d_labels, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    return PCD_OK;
+}
+
+extern "C" int pcd_compress_labels(pcd_ctx *ctx, int *num_labels) {
+    PCD_TRY(check_labels(ctx));
+    phase_begin(ctx, PCD_PHASE_COMPRESS);
+    PCD_TRY(compress_labels_dev(ctx));
+    phase_end(ctx, PCD_PHASE_COMPRESS);
+    ctx->labels_have_new = false;
+    if (num_labels) *num_labels = ctx->k;
+    return PCD_OK;
+}
+
+__global__ void k_max_label(int n, const int32_t *labels, int32_t *out) {
+    int mx = -1;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) mx = max(mx, labels[v]);
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) mx = max(mx, __shfl_xor(mx, d, WAVE));
+    if (__lane_id() == 0) atomicMax(out, mx + 1);
+}
+
+extern "C" int pcd_collect_statistics(pcd_ctx *ctx, int *num_labels) {
+    PCD_TRY(check_labels(ctx));
+    if (ctx->labels_have_new) return fail(PCD_E_STATE, "labels contain NEW_LABEL; compress them first");
+    phase_begin(ctx, PCD_PHASE_STATS);
+    if (!ctx->labels_compressed) {
+        // framework.cpp:110,348: k = max label + 1
+        PCD_TRY(fill_i32(ctx, ctx->d_k, 0, 1));
+        hipLaunchKernelGGL(k_max_label, dim3(grid_for(ctx, ctx->n, 256, 4)), dim3(256), 0, ctx->stream, ctx->n,
+                           ctx->d_labels, ctx->d_k);
+        PCD_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_k, 4, hipMemcpyDeviceToHost, ctx->stream));
+        PCD_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->k = ctx->h_small[0];
+        if (ctx->k > ctx->n + 1) {
+            // stats arrays hold n+1 communities; larger label ranges need compression
+            return fail(PCD_E_STATE, "labels exceed n; compress them first");
+        }
+    }
+    PCD_TRY(collect_statistics_dev(ctx));
+    phase_end(ctx, PCD_PHASE_STATS);
+    if (num_labels) *num_labels = ctx->k;
+    return PCD_OK;
+}
+
+__global__ void k_split_stats(int k, const CommStats *st, int32_t *s, int32_t *iv, int32_t *ev) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < k; c += gridDim.x * blockDim.x) {
+        CommStats x = st[c];
+        s[c] = x.size;
+        iv[c] = x.int_vol;
+        ev[c] = x.ext_vol;
+    }
+}
+
+extern "C" int pcd_stats_download(pcd_ctx *ctx, int32_t *sizes, int32_t *int_volume, int32_t *ext_volume) {
+    PCD_TRY(check_labels(ctx));
+    if (!ctx->has_stats) return fail(PCD_E_STATE, "no statistics; call pcd_collect_statistics");
+    const int k = ctx->k;
+    if (k == 0) return PCD_OK;
+    std::vector<CommStats> h(k);
+    PCD_HIP(hipMemcpyAsync(h.data(), ctx->d_stats, sizeof(CommStats) * k, hipMemcpyDeviceToHost, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < k; c++) {
+        if (sizes) sizes[c] = h[c].size;
+        if (int_volume) int_volume[c] = h[c].int_vol;
+        if (ext_volume) ext_volume[c] = h[c].ext_vol;
+    }
+    return PCD_OK;
+}
+
+extern "C" int pcd_metric(pcd_ctx *ctx, pcd_rule rule, double *score) {
+    PCD_TRY(check_labels(ctx));
+    PCD_TRY(check_rule(rule));
+    if (!ctx->has_stats) return fail(PCD_E_STATE, "no statistics; call pcd_collect_statistics");
+    if (rule.kind == PCD_RULE_WCC && !ctx->has_triangles) {
+        phase_begin(ctx, PCD_PHASE_TRIANGLES);
+        PCD_TRY(triangles_dev(ctx));
+        phase_end(ctx, PCD_PHASE_TRIANGLES);
+    }
+    phase_begin(ctx, PCD_PHASE_METRIC);
+    PCD_TRY(metric_dev(ctx, rule, score));
+    phase_end(ctx, PCD_PHASE_METRIC);
+    return PCD_OK;
+}
+
+extern "C" int pcd_propagation_step(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie) {
+    PCD_TRY(check_labels(ctx));
+    PCD_TRY(check_rule(rule));
+    if (!ctx->has_stats) return fail(PCD_E_STATE, "no statistics; call pcd_collect_statistics");
+    if (!(frac > 0.0)) return fail(PCD_E_INVALID, "frac must be > 0");
+    if (tie != PCD_TIE_RANDOM && tie != PCD_TIE_LOWEST) return fail(PCD_E_INVALID, "unknown tie order");
+    if (rule.kind == PCD_RULE_WCC && !ctx->has_triangles) {
+        phase_begin(ctx, PCD_PHASE_TRIANGLES);
+        PCD_TRY(triangles_dev(ctx));
+        phase_end(ctx, PCD_PHASE_TRIANGLES);
+    }
+    phase_begin(ctx, PCD_PHASE_PROPAGATE);
+    PCD_TRY(propagation_dev(ctx, rule, frac, seed, step, tie));
+    phase_end(ctx, PCD_PHASE_PROPAGATE);
+    ctx->labels_have_new = true;
+    return PCD_OK;
+}
+
+extern "C" int pcd_merging_step(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie) {
+    PCD_TRY(check_labels(ctx));
+    PCD_TRY(check_rule(rule));
+    if (!ctx->has_stats) return fail(PCD_E_STATE, "no statistics; call pcd_collect_statistics");
+    if (rounds <= 0) return fail(PCD_E_INVALID, "rounds must be positive");
+    if (tie != PCD_TIE_RANDOM && tie != PCD_TIE_LOWEST) return fail(PCD_E_INVALID, "unknown tie order");
+    if (rule.kind == PCD_RULE_WCC && !ctx->has_triangles) {
+        phase_begin(ctx, PCD_PHASE_TRIANGLES);
+        PCD_TRY(triangles_dev(ctx));
+        phase_end(ctx, PCD_PHASE_TRIANGLES);
+    }
+    phase_begin(ctx, PCD_PHASE_MERGE);
+    PCD_TRY(merging_dev(ctx, rule, rounds, seed, step, tie));
+    phase_end(ctx, PCD_PHASE_MERGE);
+    ctx->labels_have_new = false;
+    return PCD_OK;
+}
+
+extern "C" int pcd_contract(pcd_ctx *ctx, int64_t *num_pairs, int32_t *l, int32_t *k, int32_t *links) {
+    PCD_TRY(check_labels(ctx));
+    if (!ctx->has_stats) return fail(PCD_E_STATE, "no statistics; call pcd_collect_statistics");
+    if (!num_pairs) return fail(PCD_E_INVALID, "null num_pairs");
+    int64_t U = 0;
+    PCD_TRY(contract_dev(ctx, &U));
+    *num_pairs = U;
+    if (l || k || links) PCD_TRY(contract_download(ctx, U, l, k, links));
+    return PCD_OK;
+}
+
+extern "C" int pcd_count_triangles(pcd_ctx *ctx, int32_t *tri, int32_t *tri_deg, double *cc) {
+    PCD_TRY(check_graph(ctx));
+    phase_begin(ctx, PCD_PHASE_TRIANGLES);
+    PCD_TRY(triangles_dev(ctx));
+    phase_end(ctx, PCD_PHASE_TRIANGLES);
+    if (tri) PCD_HIP(hipMemcpyAsync(tri, ctx->d_tri, 4 * (size_t)ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    if (tri_deg)
+        PCD_HIP(hipMemcpyAsync(tri_deg, ctx->d_tri_deg, 4 * (size_t)ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    if (cc) *cc = ctx->cc;
+    return PCD_OK;
+}
+
+extern "C" int pcd_set_clustering_coef(pcd_ctx *ctx, double cc) {
+    PCD_TRY(check_graph(ctx));
+    if (!ctx->has_triangles) {
+        phase_begin(ctx, PCD_PHASE_TRIANGLES);
+        PCD_TRY(triangles_dev(ctx));
+        phase_end(ctx, PCD_PHASE_TRIANGLES);
+    }
+    ctx->cc = cc;
+    return PCD_OK;
+}
+
+extern "C" int pcd_enable_timing(pcd_ctx *ctx, int enable) {
+    PCD_TRY(check_ctx(ctx));
+    ctx->timing = enable != 0;
+    return PCD_OK;
+}
+
+extern "C" int pcd_phase_time(pcd_ctx *ctx, int phase, double *ms) {
+    PCD_TRY(check_ctx(ctx));
+    if (phase < 0 || phase >= PCD_PHASE_COUNT || !ms) return fail(PCD_E_INVALID, "bad phase");
+    if (!ctx->ev_valid[phase]) {
+        *ms = 0.0;
+        return PCD_OK;
+    }
+    PCD_HIP(hipEventSynchronize(ctx->ev1[phase]));
+    float f = 0;
+    PCD_HIP(hipEventElapsedTime(&f, ctx->ev0[phase], ctx->ev1[phase]));
+    *ms = f;
+    return PCD_OK;
+}
+
+extern "C" int pcd_enable_counters(pcd_ctx *ctx, int enable) {
+    PCD_TRY(check_ctx(ctx));
+    ctx->counters = enable != 0;
+    return PCD_OK;
+}
+
+extern "C" int pcd_propagation_counters(pcd_ctx *ctx, int64_t *active, int64_t *entries, int64_t *distinct) {
+    PCD_TRY(check_ctx(ctx));
+    if (active) *active = ctx->cnt_active;
+    if (entries) *entries = ctx->cnt_entries;
+    if (distinct) *distinct = ctx->cnt_distinct;
+    return PCD_OK;
+}

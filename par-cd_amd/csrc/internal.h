@@ -1,0 +1,180 @@
+// Internal declarations of the HIP side: the context and the launchers.
+#ifndef PARCD_INTERNAL_H
+#define PARCD_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "parcd.h"
+#include "rules_device.h"
+
+namespace pcd {
+
+int fail(int code, const std::string &msg);
+
+#define PCD_HIP(call)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return ::pcd::fail(PCD_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+#define PCD_TRY(expr)               \
+    do {                            \
+        int r_ = (expr);            \
+        if (r_ != PCD_OK) return r_; \
+    } while (0)
+
+constexpr int WAVE = 64;
+constexpr int EMPTY_KEY = -1;
+
+// Degree bins over vertex positions (positions are vertex ids when the
+// graph is degree-sorted descending, as `convert` writes it, else the
+// positions of a degree-descending permutation `order`).
+enum Bin { BIN_HUB = 0, BIN_BLOCK, BIN_W1024, BIN_W256, BIN_G64, BIN_G32, BIN_G16, BIN_G8, BIN_G4, NUM_BINS };
+// upper degree bound of each bin (HUB: unbounded, BLOCK: Tuning::block_max)
+constexpr int BIN_W1024_MAX = 1024, BIN_W256_MAX = 256;
+
+struct BinLayout {
+    int start[NUM_BINS + 1];  // position ranges, bin b = [start[b], start[b+1])
+};
+
+// Tunables (the autotuner rewrites these per graph shape).
+struct Tuning {
+    int block_max = 4096;   // deg <= block_max: one workgroup + LDS hash
+    int block_threads = 512;
+    int hub_chunk = 4096;   // entries per hub work item (LDS pre-aggregation)
+    int stat_chunk = 2048;  // entries per statistics work item for deg > 64
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct pcd_ctx_impl;
+
+}  // namespace pcd
+
+struct pcd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 256;
+
+    // graph
+    int n = 0, m = 0, max_degree = 0;
+    bool has_graph = false;
+    int32_t *d_indices = nullptr, *d_adj = nullptr;
+    int32_t *d_order = nullptr;  // nullptr when degree-sorted
+    std::vector<int32_t> h_degree_sorted_pos;
+    pcd::BinLayout bins{};
+    pcd::Tuning tune{};
+    // static work items: hub chunks (hub index, begin, end) and stats chunks
+    int num_hubs = 0;
+    int num_hub_items = 0;
+    int4 *d_hub_items = nullptr;       // {hub, v, begin, end}
+    int64_t *d_hub_table_off = nullptr;  // per hub
+    int32_t *d_hub_table_log = nullptr;  // log2 table size per hub
+    int32_t *d_hub_keys = nullptr, *d_hub_cnts = nullptr;
+    int64_t hub_table_slots = 0;
+    int num_stat_items = 0;
+    int4 *d_stat_items = nullptr;  // {v, begin, end, first}
+
+    // partition
+    bool has_labels = false;
+    bool labels_compressed = false;
+    bool labels_have_new = false;  // labels may hold NEW_LABEL
+    int64_t first_cap = 0;         // capacity of d_first / d_newid
+    bool has_stats = false;
+    int label_bound = 0;  // labels in [-1, label_bound)
+    int k = 0;
+    int32_t *d_labels = nullptr, *d_labels_alt = nullptr;
+    pcd::CommStats *d_stats = nullptr;
+    int32_t *d_first = nullptr, *d_newid = nullptr;
+    int32_t *d_active = nullptr;   // per-bin active lists (bin ranges)
+    int32_t *d_counts = nullptr;   // per-bin active counts + misc device counters
+    int32_t *d_hub_active = nullptr;
+    int32_t *d_scan_part = nullptr;  // scan partials
+    int scan_part_cap = 0;
+    int32_t *d_k = nullptr;          // device copy of k
+    int64_t *d_acc = nullptr;        // exact integer accumulators
+    double *d_dacc = nullptr;        // FP64 partial sums
+    int dacc_cap = 0;
+    double *d_result = nullptr;
+    double *h_result = nullptr;      // pinned
+    int32_t *h_small = nullptr;      // pinned scratch ints
+    int64_t *h_small64 = nullptr;    // pinned scratch int64
+
+    // triangles
+    bool has_triangles = false;
+    int32_t *d_tri = nullptr, *d_tri_deg = nullptr, *d_et = nullptr;
+    int32_t *d_lowpos = nullptr, *d_it = nullptr, *d_itd = nullptr;
+    double cc = 0.0;
+
+    // merge / contraction scratch
+    uint64_t *d_keys0 = nullptr, *d_keys1 = nullptr;
+    size_t keys_cap = 0;
+    int32_t *d_pair_links = nullptr, *d_pair_head = nullptr;
+    int32_t *d_rs_hist = nullptr;
+    size_t rs_hist_cap = 0;
+    int32_t *d_bp = nullptr, *d_found = nullptr;
+    uint64_t *d_best_ord = nullptr, *d_best_key = nullptr;
+
+    // measurement
+    bool timing = false;
+    bool counters = false;
+    hipEvent_t ev0[PCD_PHASE_COUNT]{}, ev1[PCD_PHASE_COUNT]{};
+    bool ev_valid[PCD_PHASE_COUNT]{};
+    int64_t cnt_active = 0, cnt_entries = 0, cnt_distinct = 0;
+
+    std::vector<void *> allocations;
+};
+
+namespace pcd {
+
+// allocation helpers (all device memory of a context is tracked)
+int dev_alloc(pcd_ctx *ctx, void **p, size_t bytes);
+void dev_free(pcd_ctx *ctx, void *p);
+template <typename T>
+int dev_alloc_t(pcd_ctx *ctx, T **p, size_t count) {
+    return dev_alloc(ctx, (void **)p, sizeof(T) * (count ? count : 1));
+}
+
+// launch-size helper: grid for a grid-stride kernel
+inline int grid_for(pcd_ctx *ctx, int64_t items, int per_block, int max_per_cu = 8) {
+    int64_t g = (items + per_block - 1) / per_block;
+    int64_t cap = (int64_t)ctx->cu_count * max_per_cu;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+// primitives (prims.hip)
+int fill_i32(pcd_ctx *ctx, int32_t *p, int32_t v, int64_t count);
+int fill_u64(pcd_ctx *ctx, uint64_t *p, uint64_t v, int64_t count);
+int ensure_scan_parts(pcd_ctx *ctx, int64_t n);
+int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int bits, uint64_t **out);
+
+// phases (partition.hip, propagate.hip, merge.hip, triangles.hip)
+int compress_labels_dev(pcd_ctx *ctx);
+int collect_statistics_dev(pcd_ctx *ctx);
+int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out);
+int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie);
+int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie);
+int contract_dev(pcd_ctx *ctx, int64_t *num_pairs);
+int contract_download(pcd_ctx *ctx, int64_t U, int32_t *l, int32_t *k, int32_t *links);
+int triangles_dev(pcd_ctx *ctx);
+int wcc_metric_dev(pcd_ctx *ctx, double *out);
+
+RuleParams make_rule(pcd_ctx *ctx, pcd_rule rule);
+
+// phase timing
+void phase_begin(pcd_ctx *ctx, int phase);
+void phase_end(pcd_ctx *ctx, int phase);
+
+}  // namespace pcd
+
+#endif

@@ -1,0 +1,341 @@
+// Parallel merging (MergingRefinement::refine, refinement.cpp:227-330) and
+// contraction on gfx950.
+//
+// Contraction: every CSR entry (v, u) writes the 2b-bit key (l_v << b | l_u)
+// (or an all-ones sentinel when l_v == l_u) into a buffer indexed by the
+// entry itself, the buffer is radix-sorted, and a run-length reduce-by-key
+// (scan of run heads) gives the community graph as unique (l, k, links)
+// sorted by (l, k): the table refinement.cpp:270-280 hashes per community
+// and framework-cuda.cu:744-807 sorts.
+//
+// Merging round: each unmatched, eligible community l takes the neighbour
+// community k (unmatched) of best score_merge >= 0 (refinement.cpp:282-302;
+// a zero score is accepted, as there) using two order-independent
+// atomics per l (max of the order-preserving score bits, then min of
+// (tie key, k)); handshake (refinement.cpp:309-316); relabel to
+// min(l, partner) (refinement.cpp:319-328).
+#include "prims.h"
+
+namespace pcd {
+
+// ---------------------------------------------------------------- keys
+template <int G>
+__global__ __launch_bounds__(256) void k_pair_keys_small(int p0, int p1, const int32_t *order, const int32_t *indices,
+                                                         const int32_t *adj, const int32_t *labels, int bits,
+                                                         uint64_t *keys) {
+    const int lane = __lane_id(), glane = lane & (G - 1);
+    const int per_wave = WAVE / G;
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    const int count = p1 - p0;
+    const uint64_t sentinel = (bits >= 64) ? ~0ULL : ((1ULL << bits) - 1);
+    const int half = bits / 2;
+    for (int base = wave_g * per_wave; base < count; base += nwaves * per_wave) {
+        int gi = base + lane / G;
+        if (gi >= count) continue;
+        int p = p0 + gi;
+        int v = order ? order[p] : p;
+        int start = indices[v], deg = indices[v + 1] - start, l = labels[v];
+        for (int j = glane; j < deg; j += G) {
+            int kk = labels[adj[start + j]];
+            keys[start + j] = kk == l ? sentinel : (((uint64_t)(uint32_t)l << half) | (uint32_t)kk);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pair_keys_items(int nitems, const int4 *items, const int32_t *adj,
+                                                         const int32_t *labels, int bits, uint64_t *keys) {
+    const int lane = __lane_id();
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    const uint64_t sentinel = (bits >= 64) ? ~0ULL : ((1ULL << bits) - 1);
+    const int half = bits / 2;
+    for (int it = wave_g; it < nitems; it += nwaves) {
+        int4 w = items[it];
+        int l = labels[w.x];
+        for (int j = w.y + lane; j < w.z; j += WAVE) {
+            int kk = labels[adj[j]];
+            keys[j] = kk == l ? sentinel : (((uint64_t)(uint32_t)l << half) | (uint32_t)kk);
+        }
+    }
+}
+
+// run heads of the first P sorted keys
+struct HeadFlag {
+    const uint64_t *keys;
+    __device__ int operator()(int64_t i) const { return (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0; }
+};
+struct HeadStore {
+    const uint64_t *keys;
+    uint64_t *ukeys;
+    int32_t *hpos;
+    __device__ void operator()(int64_t i, int ex, int flag) const {
+        if (flag) {
+            ukeys[ex] = keys[i];
+            hpos[ex] = (int)i;
+        }
+    }
+};
+
+__global__ void k_run_lengths(const int32_t *Up, const int32_t *hpos, int P, int32_t *links) {
+    const int U = *Up;
+    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < U; u += gridDim.x * blockDim.x)
+        links[u] = (u + 1 < U ? hpos[u + 1] : P) - hpos[u];
+}
+
+__global__ void k_sum_int_vol(const CommStats *st, const int32_t *kp, unsigned long long *acc) {
+    const int k = *kp;
+    long long a = 0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < k; c += gridDim.x * blockDim.x) a += st[c].int_vol;
+    a = wave_sum64(a);
+    if (__lane_id() == 0) atomicAdd(acc, (unsigned long long)a);
+}
+
+static int label_bits(int k) {
+    int b = 1;
+    while (((int64_t)1 << b) - 1 < (int64_t)k) b++;  // all-ones never a label
+    return b;
+}
+
+template <int G>
+static void launch_keys_small(pcd_ctx *ctx, int bin, int bits) {
+    const BinLayout &b = ctx->bins;
+    int p0 = b.start[bin], p1 = b.start[bin + 1];
+    if (p1 <= p0) return;
+    hipLaunchKernelGGL(k_pair_keys_small<G>, dim3(grid_for(ctx, p1 - p0, 256 / G, 16)), dim3(256), 0, ctx->stream, p0,
+                       p1, ctx->d_order, ctx->d_indices, ctx->d_adj, ctx->d_labels, bits, ctx->d_keys0);
+}
+
+// Builds the sorted unique (l, k) keys in ctx->d_keys1[0..U) with link
+// counts in ctx->d_pair_links.  Returns U and the key layout.
+static int build_pairs(pcd_ctx *ctx, int64_t *U_out, int *half_out) {
+    const int64_t E = (int64_t)2 * ctx->m;
+    if ((size_t)E > ctx->keys_cap) {
+        if (ctx->d_keys0) dev_free(ctx, ctx->d_keys0);
+        if (ctx->d_keys1) dev_free(ctx, ctx->d_keys1);
+        if (ctx->d_pair_links) dev_free(ctx, ctx->d_pair_links);
+        if (ctx->d_pair_head) dev_free(ctx, ctx->d_pair_head);
+        ctx->d_keys0 = ctx->d_keys1 = nullptr;
+        ctx->d_pair_links = ctx->d_pair_head = nullptr;
+        ctx->keys_cap = 0;
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_keys0, E + 1));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_keys1, E + 1));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_pair_links, E + 1));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_pair_head, E + 1));
+        ctx->keys_cap = (size_t)E;
+    }
+    const int half = label_bits(ctx->k);
+    const int bits = 2 * half;
+    if (ctx->num_stat_items > 0)
+        hipLaunchKernelGGL(k_pair_keys_items, dim3(grid_for(ctx, ctx->num_stat_items, 4, 16)), dim3(256), 0,
+                           ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_adj, ctx->d_labels, bits,
+                           ctx->d_keys0);
+    launch_keys_small<64>(ctx, BIN_G64, bits);
+    launch_keys_small<32>(ctx, BIN_G32, bits);
+    launch_keys_small<16>(ctx, BIN_G16, bits);
+    launch_keys_small<8>(ctx, BIN_G8, bits);
+    launch_keys_small<4>(ctx, BIN_G4, bits);
+    // P = E - sum(int_volume): entries between different communities
+    unsigned long long *acc = (unsigned long long *)ctx->d_acc;
+    PCD_TRY(fill_u64(ctx, (uint64_t *)acc, 0, 1));
+    hipLaunchKernelGGL(k_sum_int_vol, dim3(grid_for(ctx, ctx->k, 256, 4)), dim3(256), 0, ctx->stream, ctx->d_stats,
+                       ctx->d_k, acc);
+    PCD_HIP(hipGetLastError());
+    uint64_t *sorted = nullptr;
+    PCD_TRY(radix_sort_u64(ctx, ctx->d_keys0, ctx->d_keys1, E, bits, &sorted));
+    PCD_HIP(hipMemcpyAsync(ctx->h_small64, acc, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t P = E - ctx->h_small64[0];
+    uint64_t *ukeys = sorted == ctx->d_keys0 ? ctx->d_keys1 : ctx->d_keys0;
+    int32_t *total = nullptr;
+    PCD_TRY(device_scan(ctx, P, HeadFlag{sorted}, HeadStore{sorted, ukeys, ctx->d_pair_head}, &total));
+    hipLaunchKernelGGL(k_run_lengths, dim3(grid_for(ctx, P, 256)), dim3(256), 0, ctx->stream, total,
+                       ctx->d_pair_head, (int)P, ctx->d_pair_links);
+    PCD_HIP(hipGetLastError());
+    PCD_HIP(hipMemcpyAsync(ctx->h_small, total, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    int64_t U = P > 0 ? ctx->h_small[0] : 0;
+    // keep the unique keys in d_keys1
+    if (ukeys != ctx->d_keys1 && U > 0)
+        PCD_HIP(hipMemcpyAsync(ctx->d_keys1, ukeys, U * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
+    *U_out = U;
+    *half_out = half;
+    return PCD_OK;
+}
+
+__global__ void k_unpack_pairs(const uint64_t *ukeys, int64_t U, int half, int32_t *l, int32_t *k) {
+    const uint64_t mask = (1ULL << half) - 1;
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < U; u += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t key = ukeys[u];
+        l[u] = (int32_t)(key >> half);
+        k[u] = (int32_t)(key & mask);
+    }
+}
+
+// ---------------------------------------------------------------- rounds
+__device__ inline bool eligible(int l, const int32_t *found, const int32_t *bp) {
+    // refinement.cpp:266: skip matched communities and ones waiting on an
+    // unmatched partner
+    if (found[l]) return false;
+    int b = bp[l];
+    return !(b != -1 && !found[b]);
+}
+
+// order-preserving bits of a score >= 0 (-0.0 counts as +0.0); 0 = none
+__device__ inline uint64_t score_ord(double s) {
+    if (s == 0.0) s = 0.0;
+    return (uint64_t)__double_as_longlong(s) | 0x8000000000000000ULL;
+}
+
+struct MergeArgs {
+    const uint64_t *ukeys;
+    const int32_t *links;
+    int64_t U;
+    int half;
+    const CommStats *stats;
+    const int32_t *found, *bp;
+    RuleParams rule;
+    uint64_t seed;  // per-round seed
+    int step, tie;
+    unsigned long long *best_ord, *best_key;
+};
+
+template <int PHASE>
+__global__ __launch_bounds__(256) void k_merge_pass(MergeArgs a) {
+    const uint64_t mask = (1ULL << a.half) - 1;
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < a.U; u += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t key = a.ukeys[u];
+        int l = (int)(key >> a.half), kk = (int)(key & mask);
+        if (!eligible(l, a.found, a.bp) || a.found[kk]) continue;
+        double s = score_merge(a.rule, a.stats[l], a.stats[kk], a.links[u]);
+        if (!(s >= 0.0)) continue;  // negative or NaN: never beats best=0
+        uint64_t ord = score_ord(s);
+        if (PHASE == 0) {
+            atomicMax(&a.best_ord[l], (unsigned long long)ord);
+        } else if (ord == a.best_ord[l]) {
+            uint32_t tk = tie_key(a.tie, a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, l), kk);
+            atomicMin(&a.best_key[l], ((unsigned long long)tk << 32) | (uint32_t)kk);
+        }
+    }
+}
+
+__global__ void k_merge_finalize(int k, const int32_t *found, int32_t *bp, unsigned long long *best_ord,
+                                 unsigned long long *best_key) {
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) {
+        if (eligible(l, found, bp)) {
+            unsigned long long bk = best_key[l];
+            bp[l] = bk == ~0ULL ? -1 : (int32_t)(uint32_t)(bk & 0xFFFFFFFFULL);
+        }
+        best_ord[l] = 0;
+        best_key[l] = ~0ULL;
+    }
+}
+
+__global__ void k_handshake(int k, int32_t *found, const int32_t *bp, int32_t *unmatched) {
+    int local = 0;
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) {
+        if (!found[l] && (bp[l] == -1 || bp[bp[l]] == l)) found[l] = 1;
+        if (!found[l]) local++;
+    }
+    local = wave_sum(local);
+    if (__lane_id() == 0 && local) atomicAdd(unmatched, local);
+}
+
+__global__ void k_merge_relabel(int n, const int32_t *labels, const int32_t *found, const int32_t *bp, int32_t *out) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        int l = labels[v];
+        int b = bp[l];
+        out[v] = (found[l] && b != -1) ? min(l, b) : l;
+    }
+}
+
+static int ensure_merge_state(pcd_ctx *ctx) {
+    // sized for n communities at most (k <= n)
+    if (ctx->d_bp) return PCD_OK;
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_bp, ctx->n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_found, ctx->n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_best_ord, ctx->n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_best_key, ctx->n + 1));
+    return PCD_OK;
+}
+
+int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie) {
+    const int k = ctx->k;
+    int64_t U = 0;
+    int half = 0;
+    PCD_TRY(build_pairs(ctx, &U, &half));
+    PCD_TRY(ensure_merge_state(ctx));
+    PCD_TRY(fill_i32(ctx, ctx->d_bp, -1, k));
+    PCD_TRY(fill_i32(ctx, ctx->d_found, 0, k));
+    PCD_TRY(fill_u64(ctx, ctx->d_best_ord, 0, k));
+    PCD_TRY(fill_u64(ctx, ctx->d_best_key, ~0ULL, k));
+    MergeArgs a;
+    a.ukeys = ctx->d_keys1;
+    a.links = ctx->d_pair_links;
+    a.U = U;
+    a.half = half;
+    a.stats = ctx->d_stats;
+    a.found = ctx->d_found;
+    a.bp = ctx->d_bp;
+    a.rule = make_rule(ctx, rule);
+    a.step = step;
+    a.tie = tie;
+    a.best_ord = (unsigned long long *)ctx->d_best_ord;
+    a.best_key = (unsigned long long *)ctx->d_best_key;
+    int32_t *unmatched = ctx->d_counts + NUM_BINS;
+    for (int rnd = 0; rnd < rounds; rnd++) {
+        a.seed = merge_seed(seed, rnd);
+        if (U > 0) {
+            hipLaunchKernelGGL(k_merge_pass<0>, dim3(grid_for(ctx, U, 256)), dim3(256), 0, ctx->stream, a);
+            hipLaunchKernelGGL(k_merge_pass<1>, dim3(grid_for(ctx, U, 256)), dim3(256), 0, ctx->stream, a);
+        }
+        hipLaunchKernelGGL(k_merge_finalize, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_found,
+                           ctx->d_bp, a.best_ord, a.best_key);
+        PCD_TRY(fill_i32(ctx, unmatched, 0, 1));
+        hipLaunchKernelGGL(k_handshake, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_found,
+                           ctx->d_bp, unmatched);
+        PCD_HIP(hipGetLastError());
+        if (rnd + 1 < rounds) {
+            // refinement.cpp:262,306-316: stop once every community is matched
+            PCD_HIP(hipMemcpyAsync(ctx->h_small, unmatched, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+            PCD_HIP(hipStreamSynchronize(ctx->stream));
+            if (ctx->h_small[0] == 0) break;
+        }
+    }
+    hipLaunchKernelGGL(k_merge_relabel, dim3(grid_for(ctx, ctx->n, 256)), dim3(256), 0, ctx->stream, ctx->n,
+                       ctx->d_labels, ctx->d_found, ctx->d_bp, ctx->d_labels_alt);
+    PCD_HIP(hipGetLastError());
+    std::swap(ctx->d_labels, ctx->d_labels_alt);
+    ctx->labels_compressed = false;
+    ctx->has_stats = false;
+    ctx->label_bound = k;
+    return PCD_OK;
+}
+
+int contract_dev(pcd_ctx *ctx, int64_t *num_pairs) {
+    int64_t U = 0;
+    int half = 0;
+    PCD_TRY(build_pairs(ctx, &U, &half));
+    *num_pairs = U;
+    return PCD_OK;
+}
+
+int contract_download(pcd_ctx *ctx, int64_t U, int32_t *l, int32_t *k, int32_t *links) {
+    if (U <= 0) return PCD_OK;
+    int half = label_bits(ctx->k);
+    int32_t *dl = nullptr, *dk = nullptr;
+    PCD_TRY(dev_alloc_t(ctx, &dl, U));
+    PCD_TRY(dev_alloc_t(ctx, &dk, U));
+    hipLaunchKernelGGL(k_unpack_pairs, dim3(grid_for(ctx, U, 256)), dim3(256), 0, ctx->stream, ctx->d_keys1, U, half,
+                       dl, dk);
+    PCD_HIP(hipGetLastError());
+    if (l) PCD_HIP(hipMemcpyAsync(l, dl, U * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (k) PCD_HIP(hipMemcpyAsync(k, dk, U * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (links) PCD_HIP(hipMemcpyAsync(links, ctx->d_pair_links, U * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    dev_free(ctx, dl);
+    dev_free(ctx, dk);
+    return PCD_OK;
+}
+
+}  // namespace pcd

@@ -1,0 +1,248 @@
+// Partition phases on the device: label compression, community statistics
+// and the metric evaluation of modularity / CPM / conductance.
+#include "prims.h"
+
+namespace pcd {
+
+// ------------------------------------------------------------ compression
+// framework.cpp:146-170 numbers labels in order of first occurrence over v.
+// Here: first[l] = min v with label l (atomicMin), a vertex is a "first" if
+// it is the minimum of its label or has NEW_LABEL, and the exclusive scan of
+// those flags over v is exactly the reference's running counter.
+
+__global__ void k_first_occurrence(int n, const int32_t *labels, int32_t *first) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        int l = labels[v];
+        if (l >= 0) atomicMin(&first[l], v);
+    }
+}
+
+struct FirstFlag {
+    const int32_t *labels, *first;
+    __device__ int operator()(int64_t v) const {
+        int l = labels[v];
+        return (l < 0 || first[l] == (int)v) ? 1 : 0;
+    }
+};
+struct FirstStore {
+    const int32_t *labels;
+    int32_t *out, *newid;
+    __device__ void operator()(int64_t v, int ex, int flag) const {
+        if (!flag) return;
+        int l = labels[v];
+        if (l < 0) out[v] = ex;
+        else newid[l] = ex;
+    }
+};
+
+__global__ void k_relabel(int n, const int32_t *labels, const int32_t *newid, int32_t *out) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        int l = labels[v];
+        if (l >= 0) out[v] = newid[l];
+    }
+}
+
+__global__ void k_copy_i32(int32_t *dst, const int32_t *src) { *dst = *src; }
+
+int compress_labels_dev(pcd_ctx *ctx) {
+    const int n = ctx->n;
+    if (ctx->label_bound > 0) PCD_TRY(fill_i32(ctx, ctx->d_first, INT32_MAX, ctx->label_bound));
+    hipLaunchKernelGGL(k_first_occurrence, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_labels,
+                       ctx->d_first);
+    int32_t *total = nullptr;
+    PCD_TRY(device_scan(ctx, n, FirstFlag{ctx->d_labels, ctx->d_first},
+                        FirstStore{ctx->d_labels, ctx->d_labels_alt, ctx->d_newid}, &total));
+    hipLaunchKernelGGL(k_relabel, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_labels,
+                       ctx->d_newid, ctx->d_labels_alt);
+    hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(1), 0, ctx->stream, ctx->d_k, total);
+    PCD_HIP(hipGetLastError());
+    PCD_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_k, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    std::swap(ctx->d_labels, ctx->d_labels_alt);
+    ctx->k = ctx->h_small[0];
+    ctx->label_bound = ctx->k;
+    ctx->labels_compressed = true;
+    ctx->has_stats = false;
+    return PCD_OK;
+}
+
+// ------------------------------------------------------------ statistics
+// framework.cpp:69-143: per vertex, count neighbours with the same label;
+// sizes[l] += 1, int_volume[l] += same, ext_volume[l] += deg - same.
+
+__global__ void k_clear_stats(CommStats *st, const int32_t *kp) {
+    const int k = *kp;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < k; c += gridDim.x * blockDim.x)
+        st[c] = CommStats{0, 0, 0, 0};
+}
+
+// G lanes per vertex, vertices at positions [p0, p1) (all with degree <= G
+// is the fast case; larger degrees are still correct).
+template <int G>
+__global__ __launch_bounds__(256) void k_stats_small(int p0, int p1, const int32_t *order, const int32_t *indices,
+                                                     const int32_t *adj, const int32_t *labels, CommStats *st) {
+    const int lane = __lane_id(), glane = lane & (G - 1);
+    const int per_wave = WAVE / G;
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    const int count = p1 - p0;
+    for (int base = wave_g * per_wave; base < count; base += nwaves * per_wave) {
+        int gi = base + lane / G;
+        bool has = gi < count;
+        int v = 0, start = 0, deg = 0, l = 0;
+        if (has) {
+            int p = p0 + gi;
+            v = order ? order[p] : p;
+            start = indices[v];
+            deg = indices[v + 1] - start;
+            l = labels[v];
+        }
+        int same = 0;
+        for (int j = glane; j < deg; j += G) same += (labels[adj[start + j]] == l) ? 1 : 0;
+#pragma unroll
+        for (int d = G / 2; d > 0; d >>= 1) same += __shfl_xor(same, d, WAVE);
+        if (has && glane == 0) {
+            atomicAdd(&st[l].size, 1);
+            if (same) atomicAdd(&st[l].int_vol, same);
+            if (deg - same) atomicAdd(&st[l].ext_vol, deg - same);
+        }
+    }
+}
+
+// one wave per work item {v, begin, end, first} (rows longer than 64)
+__global__ __launch_bounds__(256) void k_stats_items(int nitems, const int4 *items, const int32_t *adj,
+                                                     const int32_t *labels, CommStats *st) {
+    const int lane = __lane_id();
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    for (int it = wave_g; it < nitems; it += nwaves) {
+        int4 w = items[it];
+        int l = labels[w.x];
+        int same = 0;
+        for (int j = w.y + lane; j < w.z; j += WAVE) same += (labels[adj[j]] == l) ? 1 : 0;
+        same = wave_sum(same);
+        if (lane == 0) {
+            if (w.w) atomicAdd(&st[l].size, 1);
+            if (same) atomicAdd(&st[l].int_vol, same);
+            int ext = (w.z - w.y) - same;
+            if (ext) atomicAdd(&st[l].ext_vol, ext);
+        }
+    }
+}
+
+template <int G>
+static void launch_stats_small(pcd_ctx *ctx, int p0, int p1) {
+    if (p1 <= p0) return;
+    int per_block = 256 / G;
+    hipLaunchKernelGGL(k_stats_small<G>, dim3(grid_for(ctx, p1 - p0, per_block, 16)), dim3(256), 0, ctx->stream, p0,
+                       p1, ctx->d_order, ctx->d_indices, ctx->d_adj, ctx->d_labels, ctx->d_stats);
+}
+
+int collect_statistics_dev(pcd_ctx *ctx) {
+    const BinLayout &b = ctx->bins;
+    hipLaunchKernelGGL(k_clear_stats, dim3(grid_for(ctx, ctx->k, 256)), dim3(256), 0, ctx->stream, ctx->d_stats,
+                       ctx->d_k);
+    if (ctx->num_stat_items > 0)
+        hipLaunchKernelGGL(k_stats_items, dim3(grid_for(ctx, ctx->num_stat_items, 4, 16)), dim3(256), 0,
+                           ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_adj, ctx->d_labels,
+                           ctx->d_stats);
+    launch_stats_small<64>(ctx, b.start[BIN_G64], b.start[BIN_G64 + 1]);
+    launch_stats_small<32>(ctx, b.start[BIN_G32], b.start[BIN_G32 + 1]);
+    launch_stats_small<16>(ctx, b.start[BIN_G16], b.start[BIN_G16 + 1]);
+    launch_stats_small<8>(ctx, b.start[BIN_G8], b.start[BIN_G8 + 1]);
+    launch_stats_small<4>(ctx, b.start[BIN_G4], b.start[BIN_G4 + 1]);
+    PCD_HIP(hipGetLastError());
+    ctx->has_stats = true;
+    return PCD_OK;
+}
+
+// ------------------------------------------------------------ metric
+// Modularity (rules.cpp:9-22) and CPM (rules.cpp:121-129) depend on the
+// statistics only through sum(int), sum(vol^2) and sum(size*(size-1)),
+// which are accumulated exactly in int64 (order independent); the final
+// double arithmetic runs on the device.  Conductance (rules.cpp:58-70) is
+// an FP64 sum in a fixed two-level order.
+
+__global__ __launch_bounds__(256) void k_metric_sums(const CommStats *st, const int32_t *kp,
+                                                     unsigned long long *acc) {
+    const int k = *kp;
+    long long a = 0, b = 0, c = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
+        CommStats s = st[i];
+        long long vol = (long long)s.int_vol + s.ext_vol;
+        a += s.int_vol;
+        b += vol * vol;
+        c += (long long)s.size * (s.size - 1);
+    }
+    a = wave_sum64(a); b = wave_sum64(b); c = wave_sum64(c);
+    if (__lane_id() == 0) {
+        atomicAdd(&acc[0], (unsigned long long)a);
+        atomicAdd(&acc[1], (unsigned long long)b);
+        atomicAdd(&acc[2], (unsigned long long)c);
+    }
+}
+
+__global__ void k_metric_final(const unsigned long long *acc, int kind, double lambda, int m, double *out) {
+    long long a = (long long)acc[0], b = (long long)acc[1], c = (long long)acc[2];
+    double two_m = 2.0 * m;
+    if (kind == PCD_RULE_MODULARITY) *out = a / two_m - (double)b / (two_m * two_m);
+    else *out = (a - lambda * (double)c) / (double)(2 * m);
+}
+
+constexpr int FSUM_BLOCKS = 1024;
+
+__global__ __launch_bounds__(256) void k_cond_partial(const CommStats *st, const int32_t *kp, double *part) {
+    __shared__ double lds[256 / WAVE];
+    const int k = *kp;
+    double acc = 0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < k; c += gridDim.x * blockDim.x) {
+        CommStats s = st[c];
+        acc += s.size * (s.int_vol / (double)(s.int_vol + s.ext_vol));
+    }
+    acc = wave_sum_f64(acc);
+    if (__lane_id() == 0) lds[threadIdx.x / WAVE] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int i = 0; i < 256 / WAVE; i++) t += lds[i];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double *part, int np, double div, double *out) {
+    __shared__ double lds[256 / WAVE];
+    double acc = 0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[i];
+    acc = wave_sum_f64(acc);
+    if (__lane_id() == 0) lds[threadIdx.x / WAVE] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int i = 0; i < 256 / WAVE; i++) t += lds[i];
+        *out = t / div;
+    }
+}
+
+int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out) {
+    if (rule.kind == PCD_RULE_WCC) return wcc_metric_dev(ctx, out);
+    if (rule.kind == PCD_RULE_CONDUCTANCE) {
+        hipLaunchKernelGGL(k_cond_partial, dim3(FSUM_BLOCKS), dim3(256), 0, ctx->stream, ctx->d_stats, ctx->d_k,
+                           ctx->d_dacc);
+        // rules.cpp:69 divides the sum by n
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->d_dacc, FSUM_BLOCKS,
+                           (double)ctx->n, ctx->d_result);
+    } else {
+        PCD_TRY(fill_u64(ctx, (uint64_t *)ctx->d_acc, 0, 4));
+        hipLaunchKernelGGL(k_metric_sums, dim3(grid_for(ctx, ctx->k, 256, 4)), dim3(256), 0, ctx->stream,
+                           ctx->d_stats, ctx->d_k, (unsigned long long *)ctx->d_acc);
+        hipLaunchKernelGGL(k_metric_final, dim3(1), dim3(1), 0, ctx->stream, (const unsigned long long *)ctx->d_acc,
+                           rule.kind, rule.lambda, ctx->m, ctx->d_result);
+    }
+    PCD_HIP(hipGetLastError());
+    PCD_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ctx->h_result[0];
+    return PCD_OK;
+}
+
+}  // namespace pcd

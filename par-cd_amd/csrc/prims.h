@@ -1,0 +1,125 @@
+// Device-wide primitives written for CDNA4 wave64: exclusive scan with
+// user load/store functors, and wave/block reductions.
+#ifndef PARCD_PRIMS_H
+#define PARCD_PRIMS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "internal.h"
+
+namespace pcd {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ROUNDS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ROUNDS;
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+__device__ inline int wave_incl_scan(int x) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        int y = __shfl_up(x, d, WAVE);
+        if (lane_id() >= d) x += y;
+    }
+    return x;
+}
+
+__device__ inline int wave_sum(int x) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, WAVE);
+    return x;
+}
+
+__device__ inline long long wave_sum64(long long x) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, WAVE);
+    return x;
+}
+
+__device__ inline double wave_sum_f64(double x) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, WAVE);
+    return x;
+}
+
+// Block-wide exclusive scan of one value per thread (blockDim == SCAN_THREADS);
+// returns the exclusive prefix, *total receives the block sum.
+__device__ inline int block_excl_scan(int x, int *total, int *lds /* >= SCAN_THREADS/WAVE + 1 */) {
+    const int lane = lane_id(), w = threadIdx.x / WAVE;
+    int inc = wave_incl_scan(x);
+    if (lane == WAVE - 1) lds[w] = inc;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_THREADS / WAVE; i++) {
+        int t = lds[i];
+        if (i < w) base += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - x;
+}
+
+// Phase 1: per-tile sums.
+template <class Load>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(int64_t n, Load load, int32_t *parts) {
+    __shared__ int lds[SCAN_THREADS / WAVE + 1];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int acc = 0;
+#pragma unroll 4
+    for (int r = 0; r < SCAN_ROUNDS; r++) {
+        int64_t i = base + (int64_t)r * SCAN_THREADS + threadIdx.x;
+        if (i < n) acc += load(i);
+    }
+    acc = wave_sum(acc);
+    if (lane_id() == 0) lds[threadIdx.x / WAVE] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < SCAN_THREADS / WAVE; i++) s += lds[i];
+        parts[blockIdx.x] = s;
+    }
+}
+
+// Phase 2: exclusive scan of the tile sums by one block; parts[nparts] = total.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_parts(int nparts, int32_t *parts);
+
+// Phase 3: rescan each tile with its offset; store(i, exclusive, value).
+template <class Load, class Store>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int64_t n, Load load, Store store, const int32_t *parts) {
+    __shared__ int lds[SCAN_THREADS / WAVE + 1];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int run = parts[blockIdx.x];
+    for (int r = 0; r < SCAN_ROUNDS; r++) {
+        int64_t i = base + (int64_t)r * SCAN_THREADS + threadIdx.x;
+        int x = i < n ? load(i) : 0;
+        int tot;
+        int ex = block_excl_scan(x, &tot, lds);
+        if (i < n) store(i, run + ex, x);
+        run += tot;
+    }
+}
+
+// Exclusive scan over [0, n): load(i) gives the value, store(i, prefix,
+// value) consumes it.  The total lands in ctx->d_scan_part[num_tiles].
+// Returns the device address of the total.
+template <class Load, class Store>
+int device_scan(pcd_ctx *ctx, int64_t n, Load load, Store store, int32_t **total_dev) {
+    PCD_TRY(ensure_scan_parts(ctx, n));
+    int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles < 1) tiles = 1;
+    hipLaunchKernelGGL(k_scan_tiles<Load>, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, ctx->stream, n, load,
+                       ctx->d_scan_part);
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, (int)tiles, ctx->d_scan_part);
+    hipLaunchKernelGGL((k_scan_apply<Load, Store>), dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, ctx->stream, n,
+                       load, store, ctx->d_scan_part);
+    PCD_HIP(hipGetLastError());
+    if (total_dev) *total_dev = ctx->d_scan_part + tiles;
+    return PCD_OK;
+}
+
+}  // namespace pcd
+
+#endif

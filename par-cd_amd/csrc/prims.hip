@@ -1,0 +1,155 @@
+// Primitives: fills, scan support and an LSD radix sort for 64-bit keys.
+// No vendor primitive library is used; everything is wave64 code.
+#include "prims.h"
+
+namespace pcd {
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_parts(int nparts, int32_t *parts) {
+    __shared__ int lds[SCAN_THREADS / WAVE + 1];
+    int run = 0;
+    for (int base = 0; base < nparts; base += SCAN_THREADS) {
+        int i = base + threadIdx.x;
+        int x = i < nparts ? parts[i] : 0;
+        int tot;
+        int ex = block_excl_scan(x, &tot, lds);
+        if (i < nparts) parts[i] = run + ex;
+        run += tot;
+    }
+    if (threadIdx.x == 0) parts[nparts] = run;
+}
+
+__global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u64(uint64_t *p, uint64_t v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+int fill_i32(pcd_ctx *ctx, int32_t *p, int32_t v, int64_t count) {
+    if (count <= 0) return PCD_OK;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(ctx, count, 256 * 4)), dim3(256), 0, ctx->stream, p, v, count);
+    PCD_HIP(hipGetLastError());
+    return PCD_OK;
+}
+
+int fill_u64(pcd_ctx *ctx, uint64_t *p, uint64_t v, int64_t count) {
+    if (count <= 0) return PCD_OK;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx, count, 256 * 4)), dim3(256), 0, ctx->stream, p, v, count);
+    PCD_HIP(hipGetLastError());
+    return PCD_OK;
+}
+
+int ensure_scan_parts(pcd_ctx *ctx, int64_t n) {
+    int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles < 1) tiles = 1;
+    if (tiles + 1 <= ctx->scan_part_cap) return PCD_OK;
+    if (ctx->d_scan_part) dev_free(ctx, ctx->d_scan_part);
+    ctx->d_scan_part = nullptr;
+    int cap = (int)(tiles + 1 + 1024);
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_scan_part, cap));
+    ctx->scan_part_cap = cap;
+    return PCD_OK;
+}
+
+// ------------------------------------------------------------ radix sort
+// LSD, 8-bit digits.  Per pass: tile histograms (digit-major), one
+// exclusive scan over all (digit, tile) counters, and a stable scatter that
+// ranks keys inside a tile with wave ballots (8 ballots give the lanes that
+// share a digit) and per-round LDS digit counters.
+constexpr int RS_THREADS = 256;
+constexpr int RS_ROUNDS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, int64_t n, int shift, int ntiles,
+                                                        int32_t *hist) {
+    __shared__ int cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(int)((keys[i] >> shift) & 0xFF)], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys, uint64_t *out, int64_t n, int shift,
+                                                           int ntiles, const int32_t *hist) {
+    __shared__ int gbase[256];
+    __shared__ int run[256];
+    __shared__ int wcnt[RS_THREADS / WAVE][256];
+    const int t = threadIdx.x, lane = __lane_id(), w = t / WAVE;
+    gbase[t] = hist[(int64_t)t * ntiles + blockIdx.x];
+    run[t] = 0;
+    for (int i = 0; i < RS_THREADS / WAVE; i++) wcnt[i][t] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const uint64_t lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        int64_t i = base + (int64_t)r * RS_THREADS + t;
+        bool valid = i < n;
+        uint64_t key = valid ? keys[i] : 0;
+        int d = (int)((key >> shift) & 0xFF);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint64_t bb = __ballot(valid && ((d >> b) & 1));
+            peers &= ((d >> b) & 1) ? bb : ~bb;
+        }
+        int rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int off = run[d] + rank;
+            for (int j = 0; j < w; j++) off += wcnt[j][d];
+            out[gbase[d] + off] = key;
+        }
+        __syncthreads();
+        int add = 0;
+        for (int j = 0; j < RS_THREADS / WAVE; j++) { add += wcnt[j][t]; wcnt[j][t] = 0; }
+        run[t] += add;
+        __syncthreads();
+    }
+}
+
+struct RsLoad {
+    const int32_t *h;
+    __device__ int operator()(int64_t i) const { return h[i]; }
+};
+struct RsStore {
+    int32_t *h;
+    __device__ void operator()(int64_t i, int ex, int) const { h[i] = ex; }
+};
+
+// Sorts n keys whose significant bits are [0, bits).  Ping-pongs between
+// keys and tmp; *out receives the buffer holding the result.
+int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int bits, uint64_t **out) {
+    *out = keys;
+    if (n <= 1) return PCD_OK;
+    int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
+    size_t need = (size_t)ntiles * 256;
+    if (need > ctx->rs_hist_cap) {
+        if (ctx->d_rs_hist) dev_free(ctx, ctx->d_rs_hist);
+        ctx->d_rs_hist = nullptr;
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_rs_hist, need));
+        ctx->rs_hist_cap = need;
+    }
+    uint64_t *src = keys, *dst = tmp;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, ctx->stream, src, n, shift, ntiles,
+                           ctx->d_rs_hist);
+        PCD_TRY(device_scan(ctx, (int64_t)need, RsLoad{ctx->d_rs_hist}, RsStore{ctx->d_rs_hist}, nullptr));
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, ctx->stream, src, dst, n, shift, ntiles,
+                           ctx->d_rs_hist);
+        PCD_HIP(hipGetLastError());
+        uint64_t *t = src; src = dst; dst = t;
+    }
+    *out = src;
+    return PCD_OK;
+}
+
+}  // namespace pcd

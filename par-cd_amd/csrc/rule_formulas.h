@@ -1,0 +1,185 @@
+// Gain formulas of the metric rules and the counter-based RNG, as plain
+// scalar functions shared by the device kernels and the host C++ API.  The
+// expression trees follow rules.cpp operator for operator, and everything
+// is compiled with -ffp-contract=off, so each value is the IEEE-754 double
+// the reference expression yields (int32 sub-products of the WCC gains wrap
+// as they do there).
+#ifndef PARCD_RULE_FORMULAS_H
+#define PARCD_RULE_FORMULAS_H
+
+#include <stdint.h>
+
+#include "parcd.h"
+
+#if defined(__HIPCC__)
+#define PCD_HD __host__ __device__
+#define PCD_ALIGN16 __align__(16)
+#else
+#define PCD_HD
+#define PCD_ALIGN16 alignas(16)
+#endif
+
+namespace pcd {
+
+// Community statistics as one 16-byte record: one gather per community.
+struct PCD_ALIGN16 CommStats {
+    int32_t size;
+    int32_t int_vol;  // Partition::int_volume (2x internal edges)
+    int32_t ext_vol;  // Partition::ext_volume
+    int32_t pad;
+};
+
+struct RuleParams {
+    int kind;
+    double lambda;
+    int m;       // Graph::m
+    double cc;   // Graph::clustering_coef (WCC)
+};
+
+PCD_HD inline int wrap_mul(int a, int b) { return (int)((unsigned)a * (unsigned)b); }
+
+// rules.cpp:316-465 calculate_wcc_improvement
+PCD_HD inline double wcc_improvement(int n_x, int n_y, int c_x, int c_y, int m_x, int m_y, int m_xy,
+                                                  int b_x, int b_y, double cc) {
+    double p_x = n_x > 1 ? m_x / (n_x * (n_x - 1.0) / 2.0) : 0.0;
+    double p_y = n_y > 1 ? m_y / (n_y * (n_y - 1.0) / 2.0) : 0.0;
+    double p_xy = c_x > 0 && c_y > 0 ? m_xy / (double)wrap_mul(c_x, c_y) : 0.0;
+    double q_x = n_x > 0 ? (b_x - m_xy) / (double)n_x : 0;
+    double q_y = n_y > 0 ? (b_y - m_xy) / (double)n_y : 0;
+    double score = 0.0;
+    const double t_x = (wrap_mul(n_x - 1, n_x - 2) / 2) * p_x * p_x * p_x;
+    const double t_y = (wrap_mul(n_y - 1, n_y - 2) / 2) * p_y * p_y * p_y;
+    if (c_x > 0) {
+        double t_xy = t_x + wrap_mul(c_x - 1, c_y) * p_x * p_y * p_xy + (wrap_mul(c_y, c_y - 1) / 2) * p_xy * p_xy * p_y;
+        double t_v = t_xy + q_x * (q_x - 1) / 2 * cc + q_x * (n_x - 1) * p_x * cc + c_y * q_x * p_xy * cc;
+        double vt_xy = q_x;
+        double vt_x = vt_xy + p_xy * c_y;
+        double vt_v = vt_x + p_x * (n_x - 1);
+        double tr = (vt_v / t_v) * (t_xy / (n_x + n_y - 1 + vt_xy)) - (vt_v / t_v) * (t_x / (n_x - 1 + vt_x));
+        if (t_v > 0) score += c_x * tr;
+    }
+    if (c_x < n_x) {
+        double t_xy = t_x;
+        double t_v = t_xy + q_x * (q_x - 1) / 2 * cc + q_x * (n_x - 1) * p_x * cc;
+        double vt_xy = q_x;
+        double vt_x = vt_xy;
+        double vt_v = vt_x + p_x * (n_x - 1);
+        double tr = (vt_v / t_v) * (t_xy / (n_x + n_y - 1 + vt_xy)) - (vt_v / t_v) * (t_x / (n_x - 1 + vt_x));
+        if (t_v > 0) score += (n_x - c_x) * tr;
+    }
+    if (c_y > 0) {
+        double t_xy = t_y + wrap_mul(c_y - 1, c_x) * p_y * p_x * p_xy + (wrap_mul(c_x, c_x - 1) / 2) * p_xy * p_xy * p_x;
+        double t_v = t_xy + q_y * (q_y - 1) / 2 * cc + q_y * (n_y - 1) * p_y * cc + c_x * q_y * p_xy * cc;
+        double vt_xy = q_y;
+        double vt_y = vt_xy + p_xy * c_x;
+        double vt_v = vt_y + p_y * (n_y - 1);
+        double tr = (vt_v / t_v) * (t_xy / (n_y + n_x - 1 + vt_xy)) - (vt_v / t_v) * (t_y / (n_y - 1 + vt_y));
+        if (t_v > 0) score += c_y * tr;
+    }
+    if (c_y < n_y) {
+        double t_xy = t_y;
+        double t_v = t_xy + q_y * (q_y - 1) / 2 * cc + q_y * (n_y - 1) * p_y * cc;
+        double vt_xy = q_y;
+        double vt_y = vt_xy;
+        double vt_v = vt_y + p_y * (n_y - 1);
+        double tr = (vt_v / t_v) * (t_xy / (n_y + n_x - 1 + vt_xy)) - (vt_v / t_v) * (t_y / (n_y - 1 + vt_y));
+        if (t_v > 0) score += (n_y - c_y) * tr;
+    }
+    return score;
+}
+
+// Rule::score_join (rules.cpp:110-119, 212-215, 479-493, 169-178)
+PCD_HD inline double score_join(const RuleParams &r, int deg, CommStats c, int num) {
+    switch (r.kind) {
+    case PCD_RULE_MODULARITY: {
+        double e_ij = num / (double)r.m;
+        double a_i = (c.int_vol + c.ext_vol) / (2.0 * r.m);
+        double a_j = deg / (2.0 * r.m);
+        return e_ij - 2 * a_i * a_j;
+    }
+    case PCD_RULE_CPM:
+        return num - r.lambda * c.size;
+    case PCD_RULE_WCC:
+        return wcc_improvement(1, c.size, 1, num, 0, c.int_vol / 2, num, deg, c.ext_vol, r.cc);
+    default: {
+        int s = c.size, i = c.int_vol, v = i + c.ext_vol;
+        return wrap_mul(s + 1, i + 2 * num) / (double)(v + deg) - s * (i / (double)v);
+    }
+    }
+}
+
+// Rule::score_leave (rules.cpp:121-132, 217-220, 495-512, 180-190)
+PCD_HD inline double score_leave(const RuleParams &r, int deg, CommStats c, int num) {
+    switch (r.kind) {
+    case PCD_RULE_MODULARITY: {
+        double e_ij = num / (double)r.m;
+        double a_i = (c.int_vol + c.ext_vol - deg) / (2.0 * r.m);
+        double a_j = deg / (2.0 * r.m);
+        return -(e_ij - 2 * a_i * a_j);
+    }
+    case PCD_RULE_CPM:
+        return -(num - r.lambda * (c.size - 1));
+    case PCD_RULE_WCC:
+        return -wcc_improvement(1, c.size - 1, 1, num, 0, c.int_vol / 2 - num, num, deg, c.ext_vol - deg + 2 * num,
+                                r.cc);
+    default: {
+        int s = c.size, i = c.int_vol, v = i + c.ext_vol;
+        return wrap_mul(s - 1, i - 2 * num) / (double)(v - deg) - s * (i / (double)v);
+    }
+    }
+}
+
+// Rule::score_merge (rules.cpp:100-108, 207-210, 467-477, 152-167)
+PCD_HD inline double score_merge(const RuleParams &r, CommStats a, CommStats b, int links) {
+    switch (r.kind) {
+    case PCD_RULE_MODULARITY: {
+        double e_ij = links / (double)r.m;
+        double a_i = (a.int_vol + a.ext_vol) / (2.0 * r.m);
+        double a_j = (b.int_vol + b.ext_vol) / (2.0 * r.m);
+        return e_ij - 2 * a_i * a_j;
+    }
+    case PCD_RULE_CPM:
+        return links - r.lambda * a.size * b.size;
+    case PCD_RULE_WCC: {
+        int cx = links < a.size ? links : a.size;
+        int cy = links < b.size ? links : b.size;
+        return wcc_improvement(a.size, b.size, cx, cy, a.int_vol / 2, b.int_vol / 2, links, a.ext_vol, b.ext_vol, r.cc);
+    }
+    default: {
+        int si = a.size, sj = b.size, ii = a.int_vol, ij = b.int_vol, vi = ii + a.ext_vol, vj = ij + b.ext_vol;
+        return wrap_mul(si + sj, ii + ij + links) / (double)(vi + vj + links) - wrap_mul(si, ii) / (double)vi -
+               wrap_mul(sj, ij) / (double)vj;
+    }
+    }
+}
+
+// ------------------------------------------------------ counter-based RNG
+PCD_HD inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+PCD_HD inline uint64_t hash3(uint64_t seed, uint32_t a, uint32_t b) {
+    uint64_t x = (((uint64_t)a << 32) | (uint64_t)b) + 1ULL;
+    return mix64(seed + 0x9E3779B97F4A7C15ULL * x);
+}
+// refinement.cpp:158,171: active iff rand <= (int)(min(frac,1) * RAND_MAX);
+// rand is the top 31 bits of hash3(seed, step, v).
+PCD_HD inline bool selected(uint64_t seed, int step, int v, int f) {
+    return (int)(uint32_t)(hash3(seed, (uint32_t)step, (uint32_t)v) >> 33) <= f;
+}
+// Tie key: per-(step, vertex) base hash, then mixed with the label.
+PCD_HD inline uint64_t tie_base(uint64_t seed, int step, int v) {
+    return hash3(seed ^ 0x5851F42D4C957F2DULL, (uint32_t)step, (uint32_t)v);
+}
+PCD_HD inline uint32_t tie_key(int tie, uint64_t base, int label) {
+    if (tie == PCD_TIE_LOWEST) return (uint32_t)label;
+    return (uint32_t)(mix64(base ^ ((uint64_t)(uint32_t)label * 0xD6E8FEB86659FD93ULL)) >> 32);
+}
+PCD_HD inline uint64_t merge_seed(uint64_t seed, int round) {
+    return mix64(seed ^ (0xA0761D6478BD642FULL + (uint64_t)round));
+}
+
+}  // namespace pcd
+
+#endif  // PARCD_RULE_FORMULAS_H

@@ -1,0 +1,288 @@
+"""ctypes binding of libparcd.so (the C ABI in include/parcd.h).
+
+Used by the tests and the benchmark to drive the HIP path.  There is no
+fallback: if the library is missing or cannot run, every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "libparcd.so")
+
+RULE_MODULARITY, RULE_CPM, RULE_WCC, RULE_CONDUCTANCE = 0, 1, 2, 3
+TIE_RANDOM, TIE_LOWEST = 0, 1
+NEW_LABEL = -1
+PHASES = ["compress", "stats", "metric", "propagate", "merge", "triangles"]
+
+EXPORTS = [
+    "pcd_ctx_create", "pcd_ctx_destroy", "pcd_last_error", "pcd_device_count", "pcd_synchronize",
+    "pcd_graph_upload", "pcd_graph_info", "pcd_labels_upload", "pcd_labels_download", "pcd_compress_labels",
+    "pcd_collect_statistics", "pcd_stats_download", "pcd_metric", "pcd_propagation_step", "pcd_merging_step",
+    "pcd_contract", "pcd_count_triangles", "pcd_set_clustering_coef", "pcd_enable_timing", "pcd_phase_time",
+    "pcd_enable_counters", "pcd_propagation_counters", "pcd_read_pcd", "pcd_write_pcd", "pcd_read_edge_text",
+    "pcd_convert_edges", "pcd_free", "pcd_gen_sbm", "pcd_gen_rmat", "pcd_gen_chung_lu",
+]
+
+
+class PcdError(RuntimeError):
+    pass
+
+
+class Rule(C.Structure):
+    _fields_ = [("kind", C.c_int), ("lam", C.c_double)]
+
+
+_lib = None
+I32P = C.POINTER(C.c_int32)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PcdError("libparcd.so not built (run __graft_entry__.build() or make -C par-cd_amd)")
+    L = C.CDLL(LIB_PATH)
+    L.pcd_last_error.restype = C.c_char_p
+    vp = C.c_void_p
+    L.pcd_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.pcd_ctx_destroy.argtypes = [vp]
+    L.pcd_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.pcd_synchronize.argtypes = [vp]
+    L.pcd_graph_upload.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+    L.pcd_graph_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.pcd_labels_upload.argtypes = [vp, vp]
+    L.pcd_labels_download.argtypes = [vp, vp]
+    L.pcd_compress_labels.argtypes = [vp, C.POINTER(C.c_int)]
+    L.pcd_collect_statistics.argtypes = [vp, C.POINTER(C.c_int)]
+    L.pcd_stats_download.argtypes = [vp, vp, vp, vp]
+    L.pcd_metric.argtypes = [vp, Rule, C.POINTER(C.c_double)]
+    L.pcd_propagation_step.argtypes = [vp, Rule, C.c_double, C.c_uint64, C.c_int, C.c_int]
+    L.pcd_merging_step.argtypes = [vp, Rule, C.c_int, C.c_uint64, C.c_int, C.c_int]
+    L.pcd_contract.argtypes = [vp, C.POINTER(C.c_int64), vp, vp, vp]
+    L.pcd_count_triangles.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
+    L.pcd_set_clustering_coef.argtypes = [vp, C.c_double]
+    L.pcd_enable_timing.argtypes = [vp, C.c_int]
+    L.pcd_phase_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    L.pcd_enable_counters.argtypes = [vp, C.c_int]
+    L.pcd_propagation_counters.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.pcd_read_pcd.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(I32P),
+                               C.POINTER(I32P), C.POINTER(I32P)]
+    L.pcd_write_pcd.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, vp, vp]
+    L.pcd_read_edge_text.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(I32P), C.POINTER(I32P),
+                                     C.POINTER(C.c_int)]
+    L.pcd_convert_edges.argtypes = [C.c_int, C.c_int64, vp, vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(I32P), C.POINTER(I32P), C.POINTER(I32P)]
+    L.pcd_free.argtypes = [vp]
+    L.pcd_free.restype = None
+    L.pcd_gen_sbm.argtypes = [C.c_uint64, C.c_int, vp, C.c_double, C.c_double, C.POINTER(C.c_int64),
+                              C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.c_int)]
+    L.pcd_gen_rmat.argtypes = [C.c_uint64, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double,
+                               C.POINTER(C.c_int64), C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.c_int)]
+    L.pcd_gen_chung_lu.argtypes = [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_double, C.POINTER(C.c_int64), C.POINTER(I32P), C.POINTER(I32P),
+                                   C.POINTER(C.c_int)]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise PcdError("pcd error %d: %s" % (rc, lib().pcd_last_error().decode(errors="replace")))
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _take(ptr, count):
+    out = np.ctypeslib.as_array(ptr, (max(int(count), 1),))[: int(count)].copy()
+    lib().pcd_free(C.cast(ptr, C.c_void_p))
+    return out
+
+
+def device_count():
+    c = C.c_int()
+    lib().pcd_device_count(C.byref(c))
+    return c.value
+
+
+def rule(kind, lam=0.0):
+    return Rule(int(kind), float(lam))
+
+
+# ------------------------------------------------------------ host helpers
+def read_pcd(path):
+    n, m = C.c_int(), C.c_int()
+    pi, pa, pm = I32P(), I32P(), I32P()
+    check(lib().pcd_read_pcd(path.encode(), C.byref(n), C.byref(m), C.byref(pi), C.byref(pa), C.byref(pm)))
+    return n.value, m.value, _take(pi, n.value + 1), _take(pa, 2 * m.value), _take(pm, n.value)
+
+
+def write_pcd(path, n, m, indices, adj, mp):
+    check(lib().pcd_write_pcd(path.encode(), n, m, _p(_i32(indices)), _p(_i32(adj)), _p(_i32(mp))))
+
+
+def read_edge_text(path):
+    ne, nn = C.c_int64(), C.c_int()
+    pa, pb = I32P(), I32P()
+    check(lib().pcd_read_edge_text(path.encode(), C.byref(ne), C.byref(pa), C.byref(pb), C.byref(nn)))
+    return _take(pa, ne.value), _take(pb, ne.value), nn.value
+
+
+def convert_edges(num_nodes, a, b):
+    a, b = _i32(a), _i32(b)
+    n, m = C.c_int(), C.c_int()
+    pi, pa, pm = I32P(), I32P(), I32P()
+    check(lib().pcd_convert_edges(num_nodes, a.size, _p(a), _p(b), C.byref(n), C.byref(m), C.byref(pi),
+                                  C.byref(pa), C.byref(pm)))
+    return n.value, m.value, _take(pi, n.value + 1), _take(pa, 2 * m.value), _take(pm, n.value)
+
+
+def gen_sbm(seed, block_sizes, p_in, p_out):
+    bs = _i32(block_sizes)
+    ne, nn = C.c_int64(), C.c_int()
+    pa, pb = I32P(), I32P()
+    check(lib().pcd_gen_sbm(seed, bs.size, _p(bs), p_in, p_out, C.byref(ne), C.byref(pa), C.byref(pb), C.byref(nn)))
+    return _take(pa, ne.value), _take(pb, ne.value), nn.value
+
+
+def gen_rmat(seed, scale, draws, a=0.57, b=0.19, c=0.19):
+    ne, nn = C.c_int64(), C.c_int()
+    pa, pb = I32P(), I32P()
+    check(lib().pcd_gen_rmat(seed, scale, draws, a, b, c, C.byref(ne), C.byref(pa), C.byref(pb), C.byref(nn)))
+    return _take(pa, ne.value), _take(pb, ne.value), nn.value
+
+
+def gen_chung_lu(seed, n, exponent, dmin, dmax, num_comms, cmin, cmax, boost):
+    ne, nn = C.c_int64(), C.c_int()
+    pa, pb = I32P(), I32P()
+    check(lib().pcd_gen_chung_lu(seed, n, exponent, dmin, dmax, num_comms, cmin, cmax, boost, C.byref(ne),
+                                 C.byref(pa), C.byref(pb), C.byref(nn)))
+    return _take(pa, ne.value), _take(pb, ne.value), nn.value
+
+
+def make_graph(kind, seed, **kw):
+    """Generator + converter: returns (n, m, indices, adj, map)."""
+    if kind == "sbm":
+        a, b, nn = gen_sbm(seed, kw["block_sizes"], kw["p_in"], kw["p_out"])
+    elif kind == "rmat":
+        a, b, nn = gen_rmat(seed, kw["scale"], kw.get("draws", 16 << kw["scale"]), kw.get("a", 0.57),
+                            kw.get("b", 0.19), kw.get("c", 0.19))
+    elif kind == "chung_lu":
+        a, b, nn = gen_chung_lu(seed, kw["n"], kw.get("exponent", 2.3), kw.get("dmin", 4), kw.get("dmax", 5000),
+                                kw.get("num_comms", 32), kw.get("cmin", 100), kw.get("cmax", 50000),
+                                kw.get("boost", 20.0))
+    else:
+        raise ValueError(kind)
+    return convert_edges(nn, a, b)
+
+
+# ------------------------------------------------------------ device context
+class Context:
+    def __init__(self, device=0):
+        self._ctx = C.c_void_p()
+        check(lib().pcd_ctx_create(device, C.byref(self._ctx)))
+        self.n = 0
+
+    def close(self):
+        if self._ctx:
+            lib().pcd_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_graph(self, n, m, indices, adj):
+        self._ind, self._adj = _i32(indices), _i32(adj)
+        check(lib().pcd_graph_upload(self._ctx, n, m, _p(self._ind), _p(self._adj)))
+        self.n = n
+
+    def upload_labels(self, labels):
+        lab = _i32(labels)
+        assert lab.size == self.n
+        check(lib().pcd_labels_upload(self._ctx, _p(lab)))
+
+    def labels(self):
+        out = np.zeros(self.n, np.int32)
+        check(lib().pcd_labels_download(self._ctx, _p(out)))
+        return out
+
+    def compress(self):
+        k = C.c_int()
+        check(lib().pcd_compress_labels(self._ctx, C.byref(k)))
+        return k.value
+
+    def statistics(self):
+        k = C.c_int()
+        check(lib().pcd_collect_statistics(self._ctx, C.byref(k)))
+        return k.value
+
+    def stats(self, k):
+        s, iv, ev = (np.zeros(max(k, 1), np.int32) for _ in range(3))
+        check(lib().pcd_stats_download(self._ctx, _p(s), _p(iv), _p(ev)))
+        return s[:k], iv[:k], ev[:k]
+
+    def metric(self, kind, lam=0.0):
+        out = C.c_double()
+        check(lib().pcd_metric(self._ctx, rule(kind, lam), C.byref(out)))
+        return out.value
+
+    def propagate(self, kind, lam=0.0, frac=0.5, seed=0, step=0, tie=TIE_RANDOM):
+        check(lib().pcd_propagation_step(self._ctx, rule(kind, lam), frac, seed, step, tie))
+
+    def merge(self, kind, lam=0.0, rounds=1, seed=0, step=0, tie=TIE_RANDOM):
+        check(lib().pcd_merging_step(self._ctx, rule(kind, lam), rounds, seed, step, tie))
+
+    def contract(self):
+        U = C.c_int64()
+        check(lib().pcd_contract(self._ctx, C.byref(U), None, None, None))
+        u = U.value
+        l, k, w = (np.zeros(max(u, 1), np.int32) for _ in range(3))
+        check(lib().pcd_contract(self._ctx, C.byref(U), _p(l), _p(k), _p(w)))
+        return l[:u], k[:u], w[:u]
+
+    def triangles(self):
+        tri, td = np.zeros(self.n, np.int32), np.zeros(self.n, np.int32)
+        cc = C.c_double()
+        check(lib().pcd_count_triangles(self._ctx, _p(tri), _p(td), C.byref(cc)))
+        return tri, td, cc.value
+
+    def set_cc(self, cc):
+        check(lib().pcd_set_clustering_coef(self._ctx, cc))
+
+    def enable_timing(self, on=True):
+        check(lib().pcd_enable_timing(self._ctx, 1 if on else 0))
+
+    def phase_ms(self, phase):
+        ms = C.c_double()
+        check(lib().pcd_phase_time(self._ctx, PHASES.index(phase) if isinstance(phase, str) else phase,
+                                   C.byref(ms)))
+        return ms.value
+
+    def enable_counters(self, on=True):
+        check(lib().pcd_enable_counters(self._ctx, 1 if on else 0))
+
+    def counters(self):
+        a, e, d = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().pcd_propagation_counters(self._ctx, C.byref(a), C.byref(e), C.byref(d)))
+        return a.value, e.value, d.value
+
+    def synchronize(self):
+        check(lib().pcd_synchronize(self._ctx))

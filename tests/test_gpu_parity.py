@@ -1,0 +1,281 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Integer results (compressed labels, statistics, triangles, contraction
+pairs, refinement decisions in counter mode) must be bit-exact; metrics
+must agree to 1e-9 relative.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import gpu_graphs as GG
+from conftest import golden, load
+
+pytestmark = pytest.mark.gpu
+
+parcd = pytest.importorskip("parcd")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if parcd.device_count() < 1:
+        pytest.fail("no GPU visible to the HIP runtime")
+    c = parcd.Context(0)
+    yield c
+    c.close()
+
+
+def graphs():
+    out = {}
+    for p in golden("graph_*.npz"):
+        d = load(p)
+        out["golden_" + os.path.basename(p)[6:-4]] = O.parse_pcd(d["pcd"].tobytes())
+    return out
+
+
+GOLDEN = graphs()
+RULES = [(O.MOD, 0.0), (O.CPM, 0.05), (O.CPM, 0.01)]
+
+
+def upload(ctx, g):
+    n, m, ind, adj, mp = g
+    ctx.upload_graph(n, m, ind, adj)
+    return n, m, ind, adj
+
+
+def oracle_stats(g, lab):
+    n, m, ind, adj = g[:4]
+    return O.collect_statistics(n, ind, adj, lab)
+
+
+def check_partition(ctx, g, lab_in, lam=0.05):
+    n, m, ind, adj = upload(ctx, g)
+    ctx.upload_labels(lab_in)
+    k = ctx.compress()
+    lab_o, k_o = O.compress_labels(lab_in)
+    assert k == k_o
+    lab = ctx.labels()
+    assert np.array_equal(lab, lab_o)
+    k2 = ctx.statistics()
+    assert k2 == k_o
+    s, iv, ev = ctx.stats(k2)
+    _, so, ivo, evo = oracle_stats(g, lab_o)
+    assert np.array_equal(s, so) and np.array_equal(iv, ivo) and np.array_equal(ev, evo)
+    for kind, lm in [(O.MOD, 0.0), (O.CPM, lam), (O.COND, 0.0)]:
+        got = ctx.metric(kind, lm)
+        want = O.calculate(kind, lm, n, m, ind, adj, lab_o, so, ivo, evo)
+        if np.isnan(want):
+            assert np.isnan(got)
+        else:
+            assert got == pytest.approx(want, rel=1e-9, abs=1e-12), kind
+    return lab_o, so, ivo, evo
+
+
+@pytest.mark.parametrize("name", sorted(GOLDEN))
+def test_partition_golden_graphs(ctx, name):
+    g = GOLDEN[name]
+    n = g[0]
+    rng = np.random.default_rng(1)
+    for lab in [np.arange(n)[rng.permutation(n)], rng.integers(0, max(2, n // 7), n),
+                np.where(rng.random(n) < 0.3, -1, rng.integers(0, n, n) * 2 + 5), np.zeros(n)]:
+        check_partition(ctx, g, lab)
+
+
+def test_partition_against_golden_fixtures(ctx):
+    for path in golden("part_*.npz"):
+        d = load(path)
+        name = "golden_" + os.path.basename(path)[5:-4].rsplit("_", 1)[0]
+        g = GOLDEN[name]
+        upload(ctx, g)
+        ctx.upload_labels(d["in_labels"])
+        k = ctx.compress()
+        assert k == int(d["k"][0])
+        assert np.array_equal(ctx.labels(), d["labels"])
+        ctx.statistics()
+        s, iv, ev = ctx.stats(k)
+        assert np.array_equal(s, d["sizes"]) and np.array_equal(iv, d["int_volume"])
+        assert np.array_equal(ev, d["ext_volume"])
+        lam = float(d["lam"][0])
+        for kind, lm in [(0, 0.0), (1, lam), (3, 0.0)]:
+            want = float(d["calculate"][kind])
+            got = ctx.metric(kind, lm)
+            assert (np.isnan(got) and np.isnan(want)) or got == pytest.approx(want, rel=1e-9, abs=1e-12)
+
+
+def test_partition_bin_zoo_and_unsorted(ctx):
+    g = GG.bin_zoo()
+    rng = np.random.default_rng(5)
+    n = g[0]
+    for gg in [g, GG.unsorted_copy(g)]:
+        for lab in [GG.singleton_labels(n), rng.integers(0, 300, n), rng.integers(0, 5, n)]:
+            check_partition(ctx, gg, lab)
+
+
+def _lp_compare(ctx, g, lab_in, kind, lam, frac, seed, step, tie, cc=None):
+    n, m, ind, adj = g[:4]
+    lab_o, so, ivo, evo = check_partition(ctx, g, lab_in)
+    if cc is not None:
+        ctx.set_cc(cc)
+    ctx.propagate(kind, lam, frac=frac, seed=seed, step=step, tie=tie)
+    got = ctx.labels()
+    want = O.propagation_refine(kind, lam, n, m, ind, adj, lab_o, so, ivo, evo, cc or 0.0, frac, 1, seed, step, tie)
+    assert np.array_equal(got, want), (np.nonzero(got != want)[0][:10], got[got != want][:10], want[got != want][:10])
+    return got
+
+
+@pytest.mark.parametrize("tie", [O.TIE_RANDOM, O.TIE_LOWEST])
+@pytest.mark.parametrize("kind,lam", RULES)
+def test_propagation_golden(ctx, kind, lam, tie):
+    rng = np.random.default_rng(2)
+    for name, g in sorted(GOLDEN.items()):
+        n = g[0]
+        for lab in [GG.singleton_labels(n), rng.integers(0, max(2, n // 9), n)]:
+            for frac in (0.5, 1.0):
+                _lp_compare(ctx, g, lab, kind, lam, frac, seed=1234, step=3, tie=tie)
+
+
+@pytest.mark.parametrize("tie", [O.TIE_RANDOM, O.TIE_LOWEST])
+@pytest.mark.parametrize("kind,lam", RULES)
+def test_propagation_bin_zoo(ctx, kind, lam, tie):
+    g = GG.bin_zoo()
+    n = g[0]
+    rng = np.random.default_rng(3)
+    for lab in [GG.singleton_labels(n), rng.integers(0, 200, n), rng.integers(0, 20000, n)]:
+        _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=99, step=1, tie=tie)
+        _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=7, step=0, tie=tie)
+    gu = GG.unsorted_copy(g)
+    _lp_compare(ctx, gu, rng.integers(0, 500, n), kind, lam, 0.7, seed=5, step=2, tie=tie)
+
+
+def test_propagation_wcc(ctx):
+    for name in ["golden_sbm200", "golden_sbm210", "golden_rmat9"]:
+        g = GOLDEN[name]
+        n, m, ind, adj = g[:4]
+        upload(ctx, g)
+        tri, td, cc = ctx.triangles()
+        et, tri_o, td_o, cc_o = O.count_triangles(n, ind, adj)
+        assert cc == pytest.approx(cc_o, rel=1e-12)
+        rng = np.random.default_rng(4)
+        lab = rng.integers(0, max(2, n // 10), n)
+        for tie in (O.TIE_RANDOM, O.TIE_LOWEST):
+            _lp_compare(ctx, g, lab, O.WCC, 0.0, 1.0, seed=3, step=1, tie=tie, cc=cc)
+
+
+def test_triangles(ctx):
+    cases = list(GOLDEN.items()) + [("zoo", GG.bin_zoo()), ("rmat14", GG.rmat(14))]
+    for name, g in cases:
+        n, m, ind, adj = upload(ctx, g)
+        tri, td, cc = ctx.triangles()
+        et, tri_o, td_o, cc_o = O.count_triangles(n, ind, adj)
+        assert np.array_equal(tri, tri_o), name
+        assert np.array_equal(td, td_o), name
+        assert cc == pytest.approx(cc_o, rel=1e-12, abs=1e-15), name
+
+
+def test_triangles_golden_fixture(ctx):
+    for p in golden("graph_*.npz"):
+        d = load(p)
+        g = O.parse_pcd(d["pcd"].tobytes())
+        upload(ctx, g)
+        tri, td, cc = ctx.triangles()
+        assert np.array_equal(tri, d["tri"]) and np.array_equal(td, d["tri_deg"])
+        assert cc == pytest.approx(float(d["cc"][0]), rel=1e-12, abs=1e-15)
+
+
+def test_wcc_metric(ctx):
+    cases = [GOLDEN["golden_sbm200"], GOLDEN["golden_rmat9"], GG.bin_zoo()]
+    rng = np.random.default_rng(8)
+    for g in cases:
+        n, m, ind, adj = g[:4]
+        et, tri_o, td_o, cc_o = O.count_triangles(n, ind, adj)
+        for lab in [rng.integers(0, max(2, n // 20), n), np.zeros(n, np.int32)]:
+            lab_o, so, ivo, evo = check_partition(ctx, g, lab)
+            got = ctx.metric(O.WCC)
+            want = O.calculate(O.WCC, 0, n, m, ind, adj, lab_o, so, ivo, evo, tri_o, td_o)
+            assert got == pytest.approx(want, rel=1e-9, abs=1e-12)
+
+
+def test_contraction_pairs(ctx):
+    cases = [GOLDEN["golden_sbm200"], GOLDEN["golden_rmat9"], GG.bin_zoo(), GG.rmat(13)]
+    rng = np.random.default_rng(9)
+    for g in cases:
+        n, m, ind, adj = g[:4]
+        for lab in [GG.singleton_labels(n), rng.integers(0, max(2, n // 30), n)]:
+            lab_o, so, ivo, evo = check_partition(ctx, g, lab)
+            l, k, w = ctx.contract()
+            lo, ko, wo = O.community_pairs(n, ind, adj, lab_o, len(so))
+            assert np.array_equal(l, lo) and np.array_equal(k, ko) and np.array_equal(w, wo)
+
+
+@pytest.mark.parametrize("tie", [O.TIE_RANDOM, O.TIE_LOWEST])
+@pytest.mark.parametrize("kind,lam", RULES + [(O.WCC, 0.0)])
+def test_merging(ctx, kind, lam, tie):
+    cases = [GOLDEN["golden_sbm200"], GOLDEN["golden_rmat9"], GOLDEN["golden_starclique"], GG.bin_zoo()]
+    rng = np.random.default_rng(10)
+    for g in cases:
+        n, m, ind, adj = g[:4]
+        cc = 0.0
+        if kind == O.WCC:
+            upload(ctx, g)
+            cc = ctx.triangles()[2]
+        for lab in [GG.singleton_labels(n), rng.integers(0, max(2, n // 5), n)]:
+            for rounds in (1, 3):
+                lab_o, so, ivo, evo = check_partition(ctx, g, lab)
+                if kind == O.WCC:
+                    ctx.set_cc(cc)
+                ctx.merge(kind, lam, rounds=rounds, seed=77, step=2, tie=tie)
+                got = ctx.labels()
+                want, bp, fp = O.merging_refine(kind, lam, n, m, ind, adj, lab_o, so, ivo, evo, cc, rounds, 1, 77, 2,
+                                                tie)
+                assert np.array_equal(got, want)
+
+
+def gpu_run(ctx, g, kind, lam, strategy, max_iter, seed, tie, frac=0.5, rounds=1, init=None):
+    """Outer loop of main.cpp:226-266 driven through the C ABI."""
+    n, m, ind, adj = upload(ctx, g)
+    ctx.upload_labels(GG.singleton_labels(n) if init is None else init)
+    prev, it, trace = 0.0, 0, []
+    while True:
+        ctx.compress()
+        k = ctx.statistics()
+        score = ctx.metric(kind, lam)
+        trace.append(score)
+        done = it >= max_iter if max_iter != -1 else (prev > 0 and score > 0 and score / prev - 1 <= 0)
+        if done:
+            break
+        if strategy == 0 or (strategy == 2 and it % 2 == 0):
+            ctx.propagate(kind, lam, frac=frac, seed=seed, step=it, tie=tie)
+        else:
+            ctx.merge(kind, lam, rounds=rounds, seed=seed, step=it, tie=tie)
+        it += 1
+        prev = score
+    return ctx.labels(), trace[-1], it + 1, k, trace
+
+
+@pytest.mark.parametrize("strategy", [0, 1, 2])
+@pytest.mark.parametrize("kind,lam", [(O.MOD, 0.0), (O.CPM, 0.01)])
+def test_full_run_replay(ctx, strategy, kind, lam):
+    """Deterministic mode: the GPU loop equals the serial replay label for label."""
+    for g in [GOLDEN["golden_sbm200"], GOLDEN["golden_rmat9"], GG.sbm(20, 100, 0.1, 0.002)]:
+        n, m, ind, adj = g[:4]
+        init = GG.singleton_labels(n)
+        for tie in (O.TIE_LOWEST, O.TIE_RANDOM):
+            lab, score, iters, k, trace = gpu_run(ctx, g, kind, lam, strategy, -1, 11, tie, init=init)
+            lab_o, score_o, iters_o, k_o, trace_o = O.run(n, m, ind, adj, kind=kind, lam=lam, strategy=strategy,
+                                                          mode=1, seed=11, tie=tie, init_labels=init)
+            assert iters == iters_o and k == k_o
+            assert np.array_equal(lab, lab_o)
+            assert np.allclose(trace, trace_o, rtol=1e-9, atol=1e-12)
+
+
+def test_rmat_large_properties(ctx):
+    """R-MAT scale 18: statistics exact vs oracle; LP step exact vs replay."""
+    g = GG.rmat(18)
+    n, m, ind, adj = g[:4]
+    init = GG.singleton_labels(n)
+    lab = _lp_compare(ctx, g, init, O.MOD, 0.0, 0.5, seed=2, step=0, tie=O.TIE_RANDOM)
+    assert lab.min() >= -1
+    rng = np.random.default_rng(1)
+    _lp_compare(ctx, g, rng.integers(0, 5000, n), O.CPM, 0.01, 0.5, seed=2, step=5, tie=O.TIE_LOWEST)
