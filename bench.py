@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain GPU refinement path on R-MAT scale 22.
+
+Workload (the roofline graph): R-MAT scale 22, seed 2, 16 x 2^22 draws,
+a/b/c/d = 0.57/0.19/0.19/0.05, symmetrised, loops and duplicates dropped,
+degree-sorted CSR exactly as `convert` writes it.  One step is one
+refinement iteration of the plain path -- compress_labels, statistics,
+metric, one label-propagation step (frac 0.5) -- run from the singleton
+partition; W untimed iterations, then K timed ones with the stream
+synchronised on both sides.  Run once for modularity (the reported value)
+and once for CPM with resolution 0.01.
+
+value = 2m CSR entries per iteration / seconds per iteration / 1e9 (GTEPS,
+entries convention: 2 per undirected edge).  The roofline entry is the
+dominant propagation kernel: its algorithmic bytes (from exact per-bin work
+counters of a deterministic replay of the same iterations) over its HIP-event
+duration.  cpu_baseline is the reference OpenMP binary (built from the
+reference sources into oracle/_ref) on the same graph file, same metric,
+timed over 2 iterations on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 22]
+"""
+import argparse
+import ctypes
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "par-cd_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def libc_srand(seed):
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(ctypes.c_uint(seed))
+
+
+def run_iterations(ctx, parcd, kind, lam, init, steps, warmup, seed, timers=True, counters=False):
+    """Refinement iterations of main.cpp:226-266 (plain path, -s 0)."""
+    ctx.upload_labels(init)
+    ctx.enable_timing(timers)
+    ctx.enable_counters(False)
+    scores = []
+    it = 0
+
+    def one(it):
+        ctx.compress()
+        ctx.statistics()
+        scores.append(ctx.metric(kind, lam))
+        ctx.propagate(kind, lam, frac=0.5, seed=seed, step=it, tie=parcd.TIE_RANDOM)
+
+    for _ in range(warmup):
+        one(it)
+        it += 1
+    ctx.synchronize()
+    ctx.kernel_timer_reset()
+    if counters:
+        ctx.enable_counters(True)
+        ctx.bin_counters_reset()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one(it)
+        it += 1
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.enable_counters(False)
+    return (t1 - t0), scores
+
+
+def lp_bin_bytes(active, entries, distinct, S):
+    """Algorithmic bytes of one propagation bin kernel: per active vertex the
+    list entry, two offsets, its own label, its community record and the new
+    label; per entry the neighbour id and its label; per distinct neighbour
+    label its community record (S bytes: what the rule reads)."""
+    return active * (4 + 8 + 4 + S + 4) + 8 * entries + S * distinct
+
+
+SLOT_OF_BIN = {"hub": "lp_hub", "b8192": "lp_b8192", "b4096": "lp_b4096", "b1024": "lp_b1024", "w256": "lp_w256",
+               "g64": "lp_g64", "g32": "lp_g32", "g16": "lp_g16", "g8": "lp_g8", "g4": "lp_g4"}
+
+
+def cpu_baseline(pcd_path, E, kind_arg, iters=2):
+    exe = os.path.join(ROOT, "oracle", "_ref", "main")
+    if not os.path.exists(exe):
+        log("reference binary missing (oracle/_ref/main); cpu_baseline skipped")
+        return None
+    cores = min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(cores))
+    t0 = time.time()
+    r = subprocess.run([exe, pcd_path, "-m", kind_arg, "-n", str(iters), "-p", str(cores)], env=env,
+                       stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=600)
+    wall = time.time() - t0
+    out = r.stdout
+    stats = [float(x) for x in re.findall(r"collecting statistics: ([0-9.e+-]+) sec", out)]
+    metric = [float(x) for x in re.findall(r"calculate metric: ([0-9.e+-]+) sec", out)]
+    lp = [float(x) for x in re.findall(r"label propagation: ([0-9.e+-]+) sec", out)]
+    n_it = min(len(lp), iters)
+    if n_it == 0:
+        log("reference run produced no iterations:", out[-400:])
+        return None
+    per_iter = sum(stats[i] + metric[i] + lp[i] for i in range(n_it)) / n_it
+    cpu = ""
+    try:
+        cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        pass
+    return {"value": E / per_iter / 1e9, "unit": "GTEPS", "cores": cores, "kind": "reference",
+            "sample": "reference main (oracle/_ref/main, -O3 -fopenmp) on the same R-MAT-22 PCD file, "
+                      "-m %s -n %d -p %d: mean of (statistics + metric + label propagation) over %d "
+                      "iterations from the singleton partition; wall %.1f s incl. load" % (
+                          kind_arg, iters, cores, n_it, wall),
+            "sec_per_iteration": per_iter, "lp_sec": lp[:n_it], "stats_sec": stats[:n_it], "host_cpu": cpu}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
+    ap.add_argument("--no-cpm", action="store_true", help="skip the CPM run")
+    ap.add_argument("--no-replay", action="store_true",
+                    help="skip the counter replay (for profiler runs: only the timed iterations execute)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import parcd
+
+    if parcd.device_count() < 1:
+        raise SystemExit("no GPU visible to the HIP runtime")
+    t = time.time()
+    n, m, ind, adj, mp = parcd.make_graph("rmat", 2, scale=args.scale, draws=16 << args.scale)
+    E = 2 * m
+    log("R-MAT-%d: n=%d m=%d E=%d max_deg=%d (host gen+convert %.1fs)" % (args.scale, n, m, E, ind[1] - ind[0],
+                                                                          time.time() - t))
+    libc_srand(1)
+    init = parcd.singleton_partition(n)
+
+    ctx = parcd.Context(0)
+    ctx.upload_graph(n, m, ind, adj)
+    sizes = ctx.bin_sizes()
+    log("bins:", {k: v for k, v in sizes.items() if v[0]})
+
+    # ---------------------------------------------------------- modularity
+    secs, scores = run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup, args.seed)
+    timers = ctx.kernel_timers()
+    labels_timed = ctx.labels()
+    ms_per_step = secs * 1000.0 / args.steps
+    gteps = E / (secs / args.steps) / 1e9
+    log("mod: %.3f ms/iteration, %.2f GTEPS, Q %s" % (ms_per_step, gteps, ["%.6f" % s for s in scores]))
+    log("kernel timers (ms total over %d steps):" % args.steps,
+        {k: round(v[0], 3) for k, v in sorted(timers.items(), key=lambda x: -x[1][0])})
+
+    # deterministic replay of the same iterations with work counters
+    if args.no_replay:
+        counters, replay_identical = {}, None
+    else:
+        _, scores2 = run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup,
+                                    args.seed, timers=False, counters=True)
+        counters = ctx.bin_counters()
+        replay_identical = bool(np.array_equal(ctx.labels(), labels_timed)) and scores2 == scores
+    log("replay identical:", replay_identical, "counters:", counters)
+
+    S = 8  # modularity reads int_volume + ext_volume per community record
+    per_kernel = {}
+    for b, (a_, e_, d_) in counters.items():
+        slot = SLOT_OF_BIN[b]
+        if b == "hub":
+            tms = timers.get("lp_hub_partition", (0, 0))[0] + timers.get("lp_hub_count", (0, 0))[0]
+            launches = timers.get("lp_hub_partition", (0, 0))[1]
+        else:
+            tms, launches = timers.get(slot, (0.0, 0))
+        if tms <= 0 or launches == 0:
+            continue
+        byt = lp_bin_bytes(a_, e_, d_, S)
+        per_kernel[slot] = {
+            "bytes_per_launch": byt / launches, "avg_ms": tms / launches,
+            "gbs": byt / (tms / 1000.0) / 1e9, "entries_per_launch": e_ / launches}
+    dom_name = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"]) if per_kernel else None
+    lp_ms = sum(v[0] for k, v in timers.items() if k.startswith("lp_"))
+    tot_a = sum(c[0] for c in counters.values())
+    tot_e = sum(c[1] for c in counters.values())
+    tot_d = sum(c[2] for c in counters.values())
+    lp_bytes = 8 * n * args.steps + 4 * tot_a + 8 * tot_e + S * (tot_d + tot_a)
+    lp_phase = {"ms_per_step": lp_ms / args.steps, "gbs": lp_bytes / (lp_ms / 1000.0) / 1e9 if lp_ms else None,
+                "gteps_EA": tot_e / (lp_ms / 1000.0) / 1e9 if lp_ms else None,
+                "E_A_per_step": tot_e / args.steps, "D_per_step": tot_d / args.steps}
+    stats_ms = timers.get("stats_items", (0, 0))[0] + timers.get("stats_small", (0, 0))[0]
+    stats_bytes = (8 * E + 8 * n + 12 * n) * args.steps
+    stats_phase = {"ms_per_step": stats_ms / args.steps,
+                   "gbs": stats_bytes / (stats_ms / 1000.0) / 1e9 if stats_ms else None}
+    roofline = None
+    if dom_name:
+        d = per_kernel[dom_name]
+        roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(d["gbs"], 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                    "bytes_per_launch": int(d["bytes_per_launch"]), "avg_launch_ms": round(d["avg_ms"], 4)}
+
+    # ---------------------------------------------------------- CPM 0.01
+    cpm = None
+    if not args.no_cpm:
+        secs_c, scores_c = run_iterations(ctx, parcd, parcd.RULE_CPM, 0.01, init, args.steps, args.warmup,
+                                          args.seed)
+        cpm = {"ms_per_step": secs_c * 1000 / args.steps, "gteps": E / (secs_c / args.steps) / 1e9,
+               "cpm_scores": scores_c}
+        log("cpm-0.01: %.3f ms/iteration, %.2f GTEPS" % (cpm["ms_per_step"], cpm["gteps"]))
+    ctx.close()
+
+    # ---------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+            pcd_path = os.path.join(tmp, "rmat.pcd")
+            parcd.write_pcd(pcd_path, n, m, ind, adj, mp)
+            cpu = cpu_baseline(pcd_path, E, "mod")
+        if cpu:
+            log("cpu baseline: %.4f GTEPS (%.3f s/iteration on %d cores)" % (cpu["value"], cpu["sec_per_iteration"],
+                                                                             cpu["cores"]))
+
+    out = {
+        "metric": "GTEPS & %HBM roofline; batched graphs/s + p99 vs plain GPU at equal quality",
+        "value": round(gteps, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32+f64", "data": "synthetic",
+        "config": {"workload": "rmat%d-plain-mod" % args.scale, "graph": "R-MAT scale %d seed 2, 16x draws, "
+                   "a/b/c/d=.57/.19/.19/.05" % args.scale, "n": n, "m": m, "E": E, "frac": 0.5,
+                   "step": "compress+statistics+metric+label propagation", "teps": "2m entries per iteration"},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(gteps / cpu["value"], 2) if cpu else None,
+        "phases": {"propagate": lp_phase, "statistics": stats_phase, "kernels": per_kernel},
+        "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -135,11 +135,36 @@ int pcd_set_clustering_coef(pcd_ctx *ctx, double cc);
 #define PCD_PHASE_COUNT 6
 int pcd_enable_timing(pcd_ctx *ctx, int enable);
 int pcd_phase_time(pcd_ctx *ctx, int phase, double *ms);
-/* Work counters of the most recent propagation step: active vertices,
- * their edge entries E_A, and distinct neighbour labels D (for the
- * algorithmic-bytes model).  Counting costs an extra pass; enable first. */
+/* Work counters of the propagation steps since the last
+ * pcd_bin_counters_reset, summed over degree bins: active vertices, their
+ * edge entries E_A, and distinct neighbour labels D (for the
+ * algorithmic-bytes model).  Counting costs extra atomics; enable first. */
 int pcd_enable_counters(pcd_ctx *ctx, int enable);
 int pcd_propagation_counters(pcd_ctx *ctx, int64_t *active, int64_t *entries, int64_t *distinct);
+
+/* Per-kernel timers: with timing enabled, every launch of the hot kernels
+ * is bracketed by HIP events on the context stream.  pcd_kernel_timer sums
+ * the durations (ms) and counts the launches of one kernel slot since the
+ * last reset.  Slots are numbered 0 .. pcd_kernel_slot_count()-1. */
+int pcd_kernel_slot_count(void);
+const char *pcd_kernel_slot_name(int slot);
+int pcd_kernel_timer_reset(pcd_ctx *ctx);
+int pcd_kernel_timer(pcd_ctx *ctx, int slot, double *total_ms, int64_t *launches);
+/* Work counters of the propagation step per degree bin (summed since the
+ * last reset): active vertices, their edge entries and distinct neighbour
+ * labels.  Bins are numbered 0 .. pcd_bin_count()-1 (hub first). */
+int pcd_bin_count(void);
+const char *pcd_bin_name(int bin);
+int pcd_bin_counters_reset(pcd_ctx *ctx);
+int pcd_bin_counters(pcd_ctx *ctx, int bin, int64_t *active, int64_t *entries, int64_t *distinct);
+/* Tunables of the kernel schedule: "hub_table_slots" (LDS slots of a hub
+ * partition table, power of two in [64, 16384]; takes effect at once),
+ * "hub_part_entries" (target entries per hub label partition) and
+ * "stat_chunk" (entries per statistics work item), which take effect at the
+ * next pcd_graph_upload.  Results do not depend on them. */
+int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value);
+/* Vertices per degree bin of the uploaded graph (bin layout). */
+int pcd_bin_size(pcd_ctx *ctx, int bin, int64_t *vertices, int64_t *entries);
 
 /* --------------------------------------------------------- host utilities */
 /* graph.cpp:40-105 Graph::read: reads a PCD file into malloc'ed arrays
@@ -157,6 +182,10 @@ int pcd_read_edge_text(const char *path, int64_t *num_edges, int32_t **a, int32_
 int pcd_convert_edges(int num_nodes, int64_t num_edges, const int32_t *a, const int32_t *b, int *n, int *m,
                       int32_t **indices, int32_t **adj, int32_t **map);
 void pcd_free(void *p);
+
+/* framework.cpp:14-24 create_singleton_partition: identity permutation
+ * shuffled with glibc rand() (the caller's generator state, as there). */
+int pcd_singleton_partition(int n, int32_t *labels);
 
 /* Seeded synthetic graphs (edge lists, loop-free; duplicates possible).
  * SplitMix64 streams; same seed -> same edges on any host. */

@@ -47,6 +47,38 @@ void phase_end(pcd_ctx *ctx, int phase) {
     }
 }
 
+static hipEvent_t pool_get(pcd_ctx *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void kt_begin(pcd_ctx *ctx) {
+    if (!ctx->timing) return;
+    ctx->kt_open = pool_get(ctx);
+    (void)hipEventRecord(ctx->kt_open, ctx->stream);
+}
+
+void kt_end(pcd_ctx *ctx, int slot) {
+    if (!ctx->timing || !ctx->kt_open) return;
+    hipEvent_t e1 = pool_get(ctx);
+    (void)hipEventRecord(e1, ctx->stream);
+    ctx->ktimes[slot].push_back(std::make_pair(ctx->kt_open, e1));
+    ctx->kt_open = nullptr;
+}
+
+static const char *kslot_names[NUM_KSLOTS] = {
+    "compress", "stats_items", "stats_small", "metric", "lp_select", "lp_hub_partition", "lp_hub_count",
+    "lp_b8192", "lp_b4096", "lp_b1024", "lp_w256", "lp_g64", "lp_g32", "lp_g16", "lp_g8", "lp_g4",
+    "merge_keys", "merge_sort", "merge_rle", "merge_rounds", "triangles"};
+
+static const char *bin_names[NUM_BINS] = {"hub", "b8192", "b4096", "b1024", "w256", "g64", "g32", "g16", "g8", "g4"};
+
 }  // namespace pcd
 
 static int check_ctx(pcd_ctx *ctx) {
@@ -117,6 +149,8 @@ extern "C" int pcd_ctx_create(int device, pcd_ctx **out) {
     if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_dacc, 4096);
     if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_result, 8);
     if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_counts, NUM_BINS + 16);
+    if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_bin_counters, 3 * NUM_BINS);
+    if (rc == PCD_OK) rc = fill_u64(ctx, (uint64_t *)ctx->d_bin_counters, 0, 3 * NUM_BINS);
     if (rc != PCD_OK) {
         pcd_ctx_destroy(ctx);
         return rc;
@@ -128,7 +162,7 @@ extern "C" int pcd_ctx_create(int device, pcd_ctx **out) {
 
 static void free_graph_state(pcd_ctx *ctx) {
     // everything except the per-context scalars allocated at creation
-    void *keep[] = {ctx->d_k, ctx->d_acc, ctx->d_dacc, ctx->d_result, ctx->d_counts};
+    void *keep[] = {ctx->d_k, ctx->d_acc, ctx->d_dacc, ctx->d_result, ctx->d_counts, ctx->d_bin_counters};
     std::vector<void *> all = ctx->allocations;
     for (void *p : all) {
         bool k = false;
@@ -137,6 +171,12 @@ static void free_graph_state(pcd_ctx *ctx) {
     }
     ctx->d_indices = ctx->d_adj = ctx->d_order = nullptr;
     ctx->d_hub_items = nullptr;
+    ctx->d_hub_item_off = nullptr;
+    ctx->d_hub_part_items = ctx->d_hub_info = nullptr;
+    ctx->d_hub_poff = ctx->d_hub_buf = ctx->d_hub_over = nullptr;
+    ctx->d_hub_buf_base = nullptr;
+    ctx->d_hub_partial = nullptr;
+    ctx->num_hub_parts = 0;
     ctx->d_hub_table_off = nullptr;
     ctx->d_hub_table_log = ctx->d_hub_keys = ctx->d_hub_cnts = nullptr;
     ctx->d_stat_items = nullptr;
@@ -170,6 +210,13 @@ extern "C" int pcd_ctx_destroy(pcd_ctx *ctx) {
         if (ctx->ev0[i]) (void)hipEventDestroy(ctx->ev0[i]);
         if (ctx->ev1[i]) (void)hipEventDestroy(ctx->ev1[i]);
     }
+    for (int s = 0; s < NUM_KSLOTS; s++)
+        for (auto &pr : ctx->ktimes[s]) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->kt_open) (void)hipEventDestroy(ctx->kt_open);
     if (ctx->h_result) (void)hipHostFree(ctx->h_result);
     if (ctx->h_small) (void)hipHostFree(ctx->h_small);
     if (ctx->h_small64) (void)hipHostFree(ctx->h_small64);
@@ -235,9 +282,8 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
         for (int v = 0; v < n; v++) order[bucket[maxd - deg[v]]++] = v;
     }
     auto deg_at = [&](int p) { return sorted ? deg[p] : deg[order[p]]; };
-    // bin boundaries by degree thresholds (positions are degree-descending)
-    const int thr[NUM_BINS] = {INT32_MAX, ctx->tune.block_max, BIN_W1024_MAX, BIN_W256_MAX, 64, 32, 16, 8, 4};
-    // bin b holds degrees in (thr[b+1], thr[b]]; HUB holds deg > block_max
+    // bin boundaries by degree thresholds (positions are degree-descending);
+    // bin b holds degrees in (BIN_MAX_DEG[b+1], BIN_MAX_DEG[b]]
     auto first_pos_le = [&](int t) {  // first position with degree <= t
         int lo = 0, hi = n;
         while (lo < hi) {
@@ -248,22 +294,42 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
         return lo;
     };
     ctx->bins.start[0] = 0;
-    for (int b = 1; b < NUM_BINS; b++) ctx->bins.start[b] = first_pos_le(thr[b]);
+    for (int b = 1; b < NUM_BINS; b++) ctx->bins.start[b] = first_pos_le(BIN_MAX_DEG[b]);
     ctx->bins.start[NUM_BINS] = n;
-    // hub items and tables
-    const int hub_end = ctx->bins.start[BIN_BLOCK];
-    std::vector<int4> hub_items;
-    std::vector<int64_t> toff(hub_end + 1, 0);
+    for (int b = 0; b < NUM_BINS; b++) {
+        int64_t e = 0;
+        for (int p = ctx->bins.start[b]; p < ctx->bins.start[b + 1]; p++) e += deg_at(p);
+        ctx->bin_entries[b] = e;
+    }
+    // hub layout: chunks of HUB_CHUNK entries, P label partitions per hub
+    const int hub_end = ctx->bins.start[BIN_HUB + 1];
+    std::vector<int4> hub_chunks, hub_parts, hub_info(hub_end);
+    std::vector<int32_t> chunk_off;
+    std::vector<int64_t> buf_base(hub_end + 1, 0), toff(hub_end + 1, 0);
     std::vector<int32_t> tlog(hub_end + 1, 0);
+    int64_t poff_total = 0;
     for (int h = 0; h < hub_end; h++) {
         int v = sorted ? h : order[h];
-        for (int b = indices[v]; b < indices[v + 1]; b += ctx->tune.hub_chunk)
-            hub_items.push_back(make_int4(h, v, b, std::min(indices[v + 1], b + ctx->tune.hub_chunk)));
-        tlog[h] = std::max(6, ceil_log2_host(2 * (int64_t)deg[v]));
+        int d = deg[v];
+        int P = (int)std::min<int64_t>(HUB_MAX_PARTS, std::max<int64_t>(1, ((int64_t)d + ctx->tune.hub_part_entries - 1) /
+                                                                          ctx->tune.hub_part_entries));
+        int first_chunk = (int)hub_chunks.size();
+        for (int b0 = indices[v]; b0 < indices[v + 1]; b0 += HUB_CHUNK) {
+            hub_chunks.push_back(make_int4(h, v, b0, std::min(indices[v + 1], b0 + HUB_CHUNK)));
+            chunk_off.push_back((int32_t)poff_total);
+            poff_total += P + 1;
+        }
+        int nch = (int)hub_chunks.size() - first_chunk;
+        hub_info[h] = make_int4(v, P, (int)hub_parts.size(), first_chunk);
+        for (int q = 0; q < P; q++) hub_parts.push_back(make_int4(h, q, first_chunk, nch));
+        buf_base[h + 1] = buf_base[h] + d;
+        tlog[h] = std::max(6, ceil_log2_host(2 * (int64_t)d));
         toff[h + 1] = toff[h] + ((int64_t)1 << tlog[h]);
     }
+    if (poff_total > INT32_MAX) return fail(PCD_E_INVALID, "hub layout too large");
     ctx->num_hubs = hub_end;
-    ctx->num_hub_items = (int)hub_items.size();
+    ctx->num_hub_items = (int)hub_chunks.size();
+    ctx->num_hub_parts = (int)hub_parts.size();
     ctx->hub_table_slots = toff[hub_end];
     // statistics items: rows longer than 64, in chunks
     std::vector<int4> stat_items;
@@ -283,18 +349,28 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
         PCD_TRY(dev_alloc_t(ctx, &ctx->d_order, n));
         PCD_HIP(hipMemcpy(ctx->d_order, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     }
-    if (!hub_items.empty()) {
-        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_items, hub_items.size()));
-        PCD_HIP(hipMemcpy(ctx->d_hub_items, hub_items.data(), sizeof(int4) * hub_items.size(), hipMemcpyHostToDevice));
-        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_table_off, hub_end + 1));
-        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_table_log, hub_end + 1));
-        PCD_HIP(hipMemcpy(ctx->d_hub_table_off, toff.data(), sizeof(int64_t) * (hub_end + 1), hipMemcpyHostToDevice));
-        PCD_HIP(hipMemcpy(ctx->d_hub_table_log, tlog.data(), sizeof(int32_t) * (hub_end + 1), hipMemcpyHostToDevice));
+    if (hub_end > 0) {
+        auto up = [&](auto **dst, const auto &vec) -> int {
+            PCD_TRY(dev_alloc_t(ctx, dst, vec.size()));
+            PCD_HIP(hipMemcpy(*dst, vec.data(), sizeof(vec[0]) * vec.size(), hipMemcpyHostToDevice));
+            return PCD_OK;
+        };
+        PCD_TRY(up(&ctx->d_hub_items, hub_chunks));
+        PCD_TRY(up(&ctx->d_hub_item_off, chunk_off));
+        PCD_TRY(up(&ctx->d_hub_part_items, hub_parts));
+        PCD_TRY(up(&ctx->d_hub_info, hub_info));
+        PCD_TRY(up(&ctx->d_hub_buf_base, buf_base));
+        PCD_TRY(up(&ctx->d_hub_table_off, toff));
+        PCD_TRY(up(&ctx->d_hub_table_log, tlog));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_poff, poff_total));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_buf, buf_base[hub_end]));
+        PCD_TRY(dev_alloc(ctx, &ctx->d_hub_partial, (size_t)HUB_PARTIAL_BYTES * hub_parts.size()));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_over, hub_end));
+        PCD_TRY(fill_i32(ctx, ctx->d_hub_over, 0, hub_end));
         PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_keys, ctx->hub_table_slots));
         PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_cnts, ctx->hub_table_slots));
         PCD_TRY(fill_i32(ctx, ctx->d_hub_keys, EMPTY_KEY, ctx->hub_table_slots));
         PCD_TRY(fill_i32(ctx, ctx->d_hub_cnts, 0, ctx->hub_table_slots));
-        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_active, hub_end + 1));
     }
     if (!stat_items.empty()) {
         PCD_TRY(dev_alloc_t(ctx, &ctx->d_stat_items, stat_items.size()));
@@ -529,6 +605,25 @@ extern "C" int pcd_phase_time(pcd_ctx *ctx, int phase, double *ms) {
     return PCD_OK;
 }
 
+extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
+    PCD_TRY(check_ctx(ctx));
+    if (!key) return fail(PCD_E_INVALID, "null key");
+    std::string k(key);
+    if (k == "hub_table_slots") {
+        if (value < 64 || value > 16384 || (value & (value - 1))) return fail(PCD_E_INVALID, "hub_table_slots: power of two in [64, 16384]");
+        ctx->tune.hub_table_slots = value;
+    } else if (k == "hub_part_entries") {
+        if (value < 64) return fail(PCD_E_INVALID, "hub_part_entries must be >= 64");
+        ctx->tune.hub_part_entries = value;  // applies to the next graph upload
+    } else if (k == "stat_chunk") {
+        if (value < 64) return fail(PCD_E_INVALID, "stat_chunk must be >= 64");
+        ctx->tune.stat_chunk = value;  // applies to the next graph upload
+    } else {
+        return fail(PCD_E_INVALID, "unknown tuning key " + k);
+    }
+    return PCD_OK;
+}
+
 extern "C" int pcd_enable_counters(pcd_ctx *ctx, int enable) {
     PCD_TRY(check_ctx(ctx));
     ctx->counters = enable != 0;
@@ -536,9 +631,79 @@ extern "C" int pcd_enable_counters(pcd_ctx *ctx, int enable) {
 }
 
 extern "C" int pcd_propagation_counters(pcd_ctx *ctx, int64_t *active, int64_t *entries, int64_t *distinct) {
+    int64_t a = 0, e = 0, d = 0;
+    for (int b = 0; b < NUM_BINS; b++) {
+        int64_t x = 0, y = 0, z = 0;
+        PCD_TRY(pcd_bin_counters(ctx, b, &x, &y, &z));
+        a += x; e += y; d += z;
+    }
+    if (active) *active = a;
+    if (entries) *entries = e;
+    if (distinct) *distinct = d;
+    return PCD_OK;
+}
+
+extern "C" int pcd_kernel_slot_count(void) { return NUM_KSLOTS; }
+
+extern "C" const char *pcd_kernel_slot_name(int slot) {
+    return (slot >= 0 && slot < NUM_KSLOTS) ? kslot_names[slot] : "";
+}
+
+extern "C" int pcd_kernel_timer_reset(pcd_ctx *ctx) {
     PCD_TRY(check_ctx(ctx));
-    if (active) *active = ctx->cnt_active;
-    if (entries) *entries = ctx->cnt_entries;
-    if (distinct) *distinct = ctx->cnt_distinct;
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    for (int s = 0; s < NUM_KSLOTS; s++) {
+        for (auto &pr : ctx->ktimes[s]) {
+            ctx->event_pool.push_back(pr.first);
+            ctx->event_pool.push_back(pr.second);
+        }
+        ctx->ktimes[s].clear();
+    }
+    return PCD_OK;
+}
+
+extern "C" int pcd_kernel_timer(pcd_ctx *ctx, int slot, double *total_ms, int64_t *launches) {
+    PCD_TRY(check_ctx(ctx));
+    if (slot < 0 || slot >= NUM_KSLOTS) return fail(PCD_E_INVALID, "bad kernel slot");
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    double t = 0;
+    for (auto &pr : ctx->ktimes[slot]) {
+        float f = 0;
+        PCD_HIP(hipEventElapsedTime(&f, pr.first, pr.second));
+        t += f;
+    }
+    if (total_ms) *total_ms = t;
+    if (launches) *launches = (int64_t)ctx->ktimes[slot].size();
+    return PCD_OK;
+}
+
+extern "C" int pcd_bin_count(void) { return NUM_BINS; }
+
+extern "C" const char *pcd_bin_name(int bin) { return (bin >= 0 && bin < NUM_BINS) ? bin_names[bin] : ""; }
+
+extern "C" int pcd_bin_counters_reset(pcd_ctx *ctx) {
+    PCD_TRY(check_ctx(ctx));
+    PCD_TRY(fill_u64(ctx, (uint64_t *)ctx->d_bin_counters, 0, 3 * NUM_BINS));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    return PCD_OK;
+}
+
+extern "C" int pcd_bin_counters(pcd_ctx *ctx, int bin, int64_t *active, int64_t *entries, int64_t *distinct) {
+    PCD_TRY(check_ctx(ctx));
+    if (bin < 0 || bin >= NUM_BINS) return fail(PCD_E_INVALID, "bad bin");
+    unsigned long long h[3];
+    PCD_HIP(hipMemcpyAsync(h, ctx->d_bin_counters + 3 * bin, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    if (active) *active = (int64_t)h[0];
+    if (entries) *entries = (int64_t)h[1];
+    if (distinct) *distinct = (int64_t)h[2];
+    return PCD_OK;
+}
+
+extern "C" int pcd_bin_size(pcd_ctx *ctx, int bin, int64_t *vertices, int64_t *entries) {
+    PCD_TRY(check_graph(ctx));
+    if (bin < 0 || bin >= NUM_BINS) return fail(PCD_E_INVALID, "bad bin");
+    if (vertices) *vertices = ctx->bins.start[bin + 1] - ctx->bins.start[bin];
+    if (entries) *entries = ctx->bin_entries[bin];
     return PCD_OK;
 }

@@ -34,9 +34,9 @@ constexpr int EMPTY_KEY = -1;
 // Degree bins over vertex positions (positions are vertex ids when the
 // graph is degree-sorted descending, as `convert` writes it, else the
 // positions of a degree-descending permutation `order`).
-enum Bin { BIN_HUB = 0, BIN_BLOCK, BIN_W1024, BIN_W256, BIN_G64, BIN_G32, BIN_G16, BIN_G8, BIN_G4, NUM_BINS };
-// upper degree bound of each bin (HUB: unbounded, BLOCK: Tuning::block_max)
-constexpr int BIN_W1024_MAX = 1024, BIN_W256_MAX = 256;
+enum Bin { BIN_HUB = 0, BIN_B8192, BIN_B4096, BIN_B1024, BIN_W256, BIN_G64, BIN_G32, BIN_G16, BIN_G8, BIN_G4, NUM_BINS };
+// degree upper bound of each bin (HUB: unbounded)
+constexpr int BIN_MAX_DEG[NUM_BINS] = {0x7fffffff, 8192, 4096, 1024, 256, 64, 32, 16, 8, 4};
 
 struct BinLayout {
     int start[NUM_BINS + 1];  // position ranges, bin b = [start[b], start[b+1])
@@ -44,10 +44,22 @@ struct BinLayout {
 
 // Tunables (the autotuner rewrites these per graph shape).
 struct Tuning {
-    int block_max = 4096;   // deg <= block_max: one workgroup + LDS hash
-    int block_threads = 512;
-    int hub_chunk = 4096;   // entries per hub work item (LDS pre-aggregation)
-    int stat_chunk = 2048;  // entries per statistics work item for deg > 64
+    int hub_chunk = 4096;        // entries per hub partition work item
+    int hub_part_entries = 4096; // target entries per hub label partition
+    int hub_table_slots = 16384; // LDS table slots of a hub partition (<= 16384)
+    int stat_chunk = 2048;       // entries per statistics work item for deg > 64
+};
+constexpr int HUB_MAX_PARTS = 1024;
+constexpr int HUB_CHUNK = 4096;      // entries per hub chunk item (pass 1 holds 512 x 8)
+constexpr int HUB_PARTIAL_BYTES = 32; // storage per (hub, partition) partial result
+
+// Kernel timer slots (HIP events around each launch when timing is on).
+enum KSlot {
+    KS_COMPRESS = 0, KS_STATS_ITEMS, KS_STATS_SMALL, KS_METRIC,
+    KS_LP_SELECT, KS_LP_HUB_INSERT, KS_LP_HUB_DECIDE, KS_LP_B8192, KS_LP_B4096, KS_LP_B1024, KS_LP_W256,
+    KS_LP_G64, KS_LP_G32, KS_LP_G16, KS_LP_G8, KS_LP_G4,
+    KS_MERGE_KEYS, KS_MERGE_SORT, KS_MERGE_RLE, KS_MERGE_ROUNDS, KS_TRIANGLES,
+    NUM_KSLOTS
 };
 
 struct DevBuf {
@@ -71,12 +83,22 @@ struct pcd_ctx {
     int32_t *d_order = nullptr;  // nullptr when degree-sorted
     std::vector<int32_t> h_degree_sorted_pos;
     pcd::BinLayout bins{};
+    int64_t bin_entries[pcd::NUM_BINS]{};
     pcd::Tuning tune{};
     // static work items: hub chunks (hub index, begin, end) and stats chunks
     int num_hubs = 0;
-    int num_hub_items = 0;
-    int4 *d_hub_items = nullptr;       // {hub, v, begin, end}
-    int64_t *d_hub_table_off = nullptr;  // per hub
+    int num_hub_items = 0;               // chunk items
+    int num_hub_parts = 0;               // (hub, partition) items
+    int4 *d_hub_items = nullptr;         // chunk: {hub, v, begin, end}
+    int32_t *d_hub_item_off = nullptr;   // chunk -> offset of its (P+1) partition offsets
+    int4 *d_hub_part_items = nullptr;    // {hub, part, first chunk, number of chunks}
+    int4 *d_hub_info = nullptr;          // per hub: {v, P, first part item, first chunk}
+    int32_t *d_hub_poff = nullptr;       // partition offsets of all chunks
+    int32_t *d_hub_buf = nullptr;        // partitioned labels (sum of hub degrees)
+    int64_t *d_hub_buf_base = nullptr;   // per hub: offset into d_hub_buf
+    void *d_hub_partial = nullptr;       // per (hub, part) partial argmax
+    int32_t *d_hub_over = nullptr;       // per hub: partition overflow flag
+    int64_t *d_hub_table_off = nullptr;  // per hub (fallback global tables)
     int32_t *d_hub_table_log = nullptr;  // log2 table size per hub
     int32_t *d_hub_keys = nullptr, *d_hub_cnts = nullptr;
     int64_t hub_table_slots = 0;
@@ -129,6 +151,10 @@ struct pcd_ctx {
     hipEvent_t ev0[PCD_PHASE_COUNT]{}, ev1[PCD_PHASE_COUNT]{};
     bool ev_valid[PCD_PHASE_COUNT]{};
     int64_t cnt_active = 0, cnt_entries = 0, cnt_distinct = 0;
+    unsigned long long *d_bin_counters = nullptr;  // NUM_BINS x {active, entries, distinct}
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ktimes[pcd::NUM_KSLOTS];
+    std::vector<hipEvent_t> event_pool;
+    hipEvent_t kt_open = nullptr;
 
     std::vector<void *> allocations;
 };
@@ -174,6 +200,9 @@ RuleParams make_rule(pcd_ctx *ctx, pcd_rule rule);
 // phase timing
 void phase_begin(pcd_ctx *ctx, int phase);
 void phase_end(pcd_ctx *ctx, int phase);
+// kernel timing: bracket one or more launches of a slot
+void kt_begin(pcd_ctx *ctx);
+void kt_end(pcd_ctx *ctx, int slot);
 
 }  // namespace pcd
 

@@ -126,6 +126,7 @@ static int build_pairs(pcd_ctx *ctx, int64_t *U_out, int *half_out) {
     }
     const int half = label_bits(ctx->k);
     const int bits = 2 * half;
+    kt_begin(ctx);
     if (ctx->num_stat_items > 0)
         hipLaunchKernelGGL(k_pair_keys_items, dim3(grid_for(ctx, ctx->num_stat_items, 4, 16)), dim3(256), 0,
                            ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_adj, ctx->d_labels, bits,
@@ -140,17 +141,22 @@ static int build_pairs(pcd_ctx *ctx, int64_t *U_out, int *half_out) {
     PCD_TRY(fill_u64(ctx, (uint64_t *)acc, 0, 1));
     hipLaunchKernelGGL(k_sum_int_vol, dim3(grid_for(ctx, ctx->k, 256, 4)), dim3(256), 0, ctx->stream, ctx->d_stats,
                        ctx->d_k, acc);
+    kt_end(ctx, KS_MERGE_KEYS);
     PCD_HIP(hipGetLastError());
     uint64_t *sorted = nullptr;
+    kt_begin(ctx);
     PCD_TRY(radix_sort_u64(ctx, ctx->d_keys0, ctx->d_keys1, E, bits, &sorted));
+    kt_end(ctx, KS_MERGE_SORT);
     PCD_HIP(hipMemcpyAsync(ctx->h_small64, acc, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     PCD_HIP(hipStreamSynchronize(ctx->stream));
     const int64_t P = E - ctx->h_small64[0];
     uint64_t *ukeys = sorted == ctx->d_keys0 ? ctx->d_keys1 : ctx->d_keys0;
     int32_t *total = nullptr;
+    kt_begin(ctx);
     PCD_TRY(device_scan(ctx, P, HeadFlag{sorted}, HeadStore{sorted, ukeys, ctx->d_pair_head}, &total));
     hipLaunchKernelGGL(k_run_lengths, dim3(grid_for(ctx, P, 256)), dim3(256), 0, ctx->stream, total,
                        ctx->d_pair_head, (int)P, ctx->d_pair_links);
+    kt_end(ctx, KS_MERGE_RLE);
     PCD_HIP(hipGetLastError());
     PCD_HIP(hipMemcpyAsync(ctx->h_small, total, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     PCD_HIP(hipStreamSynchronize(ctx->stream));
@@ -284,6 +290,7 @@ int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step
     a.best_key = (unsigned long long *)ctx->d_best_key;
     int32_t *unmatched = ctx->d_counts + NUM_BINS;
     for (int rnd = 0; rnd < rounds; rnd++) {
+        kt_begin(ctx);
         a.seed = merge_seed(seed, rnd);
         if (U > 0) {
             hipLaunchKernelGGL(k_merge_pass<0>, dim3(grid_for(ctx, U, 256)), dim3(256), 0, ctx->stream, a);
@@ -294,6 +301,7 @@ int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step
         PCD_TRY(fill_i32(ctx, unmatched, 0, 1));
         hipLaunchKernelGGL(k_handshake, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_found,
                            ctx->d_bp, unmatched);
+        kt_end(ctx, KS_MERGE_ROUNDS);
         PCD_HIP(hipGetLastError());
         if (rnd + 1 < rounds) {
             // refinement.cpp:262,306-316: stop once every community is matched

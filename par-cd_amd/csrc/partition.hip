@@ -46,6 +46,7 @@ __global__ void k_copy_i32(int32_t *dst, const int32_t *src) { *dst = *src; }
 
 int compress_labels_dev(pcd_ctx *ctx) {
     const int n = ctx->n;
+    kt_begin(ctx);
     if (ctx->label_bound > 0) PCD_TRY(fill_i32(ctx, ctx->d_first, INT32_MAX, ctx->label_bound));
     hipLaunchKernelGGL(k_first_occurrence, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_labels,
                        ctx->d_first);
@@ -55,6 +56,7 @@ int compress_labels_dev(pcd_ctx *ctx) {
     hipLaunchKernelGGL(k_relabel, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_labels,
                        ctx->d_newid, ctx->d_labels_alt);
     hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(1), 0, ctx->stream, ctx->d_k, total);
+    kt_end(ctx, KS_COMPRESS);
     PCD_HIP(hipGetLastError());
     PCD_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_k, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     PCD_HIP(hipStreamSynchronize(ctx->stream));
@@ -69,6 +71,13 @@ int compress_labels_dev(pcd_ctx *ctx) {
 // ------------------------------------------------------------ statistics
 // framework.cpp:69-143: per vertex, count neighbours with the same label;
 // sizes[l] += 1, int_volume[l] += same, ext_volume[l] += deg - same.
+
+// size and int_vol are adjacent 32-bit fields: one 64-bit atomic adds both
+// (neither field can carry past 2^31).
+__device__ inline void add_stats(CommStats *st, int l, int size, int same, int ext) {
+    atomicAdd((unsigned long long *)&st[l].size, ((unsigned long long)(unsigned)same << 32) | (unsigned)size);
+    if (ext) atomicAdd(&st[l].ext_vol, ext);
+}
 
 __global__ void k_clear_stats(CommStats *st, const int32_t *kp) {
     const int k = *kp;
@@ -101,17 +110,15 @@ __global__ __launch_bounds__(256) void k_stats_small(int p0, int p1, const int32
         for (int j = glane; j < deg; j += G) same += (labels[adj[start + j]] == l) ? 1 : 0;
 #pragma unroll
         for (int d = G / 2; d > 0; d >>= 1) same += __shfl_xor(same, d, WAVE);
-        if (has && glane == 0) {
-            atomicAdd(&st[l].size, 1);
-            if (same) atomicAdd(&st[l].int_vol, same);
-            if (deg - same) atomicAdd(&st[l].ext_vol, deg - same);
-        }
+        if (has && glane == 0) add_stats(st, l, 1, same, deg - same);
     }
 }
 
-// one wave per work item {v, begin, end, first} (rows longer than 64)
+// one wave per work item {v, begin, end, first} (rows longer than 64); each
+// lane keeps PF gathers in flight
 __global__ __launch_bounds__(256) void k_stats_items(int nitems, const int4 *items, const int32_t *adj,
                                                      const int32_t *labels, CommStats *st) {
+    constexpr int PF = 8;
     const int lane = __lane_id();
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int nwaves = gridDim.x * blockDim.x / WAVE;
@@ -119,14 +126,18 @@ __global__ __launch_bounds__(256) void k_stats_items(int nitems, const int4 *ite
         int4 w = items[it];
         int l = labels[w.x];
         int same = 0;
-        for (int j = w.y + lane; j < w.z; j += WAVE) same += (labels[adj[j]] == l) ? 1 : 0;
-        same = wave_sum(same);
-        if (lane == 0) {
-            if (w.w) atomicAdd(&st[l].size, 1);
-            if (same) atomicAdd(&st[l].int_vol, same);
-            int ext = (w.z - w.y) - same;
-            if (ext) atomicAdd(&st[l].ext_vol, ext);
+        for (int j0 = w.y + lane; j0 < w.z; j0 += PF * WAVE) {
+            int nb[PF];
+#pragma unroll
+            for (int e = 0; e < PF; e++) {
+                int j = j0 + e * WAVE;
+                nb[e] = j < w.z ? adj[j] : -1;
+            }
+#pragma unroll
+            for (int e = 0; e < PF; e++) same += (nb[e] >= 0 && labels[nb[e]] == l) ? 1 : 0;
         }
+        same = wave_sum(same);
+        if (lane == 0) add_stats(st, l, w.w ? 1 : 0, same, (w.z - w.y) - same);
     }
 }
 
@@ -142,15 +153,20 @@ int collect_statistics_dev(pcd_ctx *ctx) {
     const BinLayout &b = ctx->bins;
     hipLaunchKernelGGL(k_clear_stats, dim3(grid_for(ctx, ctx->k, 256)), dim3(256), 0, ctx->stream, ctx->d_stats,
                        ctx->d_k);
-    if (ctx->num_stat_items > 0)
+    if (ctx->num_stat_items > 0) {
+        kt_begin(ctx);
         hipLaunchKernelGGL(k_stats_items, dim3(grid_for(ctx, ctx->num_stat_items, 4, 16)), dim3(256), 0,
                            ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_adj, ctx->d_labels,
                            ctx->d_stats);
+        kt_end(ctx, KS_STATS_ITEMS);
+    }
+    kt_begin(ctx);
     launch_stats_small<64>(ctx, b.start[BIN_G64], b.start[BIN_G64 + 1]);
     launch_stats_small<32>(ctx, b.start[BIN_G32], b.start[BIN_G32 + 1]);
     launch_stats_small<16>(ctx, b.start[BIN_G16], b.start[BIN_G16 + 1]);
     launch_stats_small<8>(ctx, b.start[BIN_G8], b.start[BIN_G8 + 1]);
     launch_stats_small<4>(ctx, b.start[BIN_G4], b.start[BIN_G4 + 1]);
+    kt_end(ctx, KS_STATS_SMALL);
     PCD_HIP(hipGetLastError());
     ctx->has_stats = true;
     return PCD_OK;
@@ -165,6 +181,7 @@ int collect_statistics_dev(pcd_ctx *ctx) {
 
 __global__ __launch_bounds__(256) void k_metric_sums(const CommStats *st, const int32_t *kp,
                                                      unsigned long long *acc) {
+    __shared__ long long lds[3][256 / WAVE];
     const int k = *kp;
     long long a = 0, b = 0, c = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
@@ -175,10 +192,13 @@ __global__ __launch_bounds__(256) void k_metric_sums(const CommStats *st, const 
         c += (long long)s.size * (s.size - 1);
     }
     a = wave_sum64(a); b = wave_sum64(b); c = wave_sum64(c);
-    if (__lane_id() == 0) {
-        atomicAdd(&acc[0], (unsigned long long)a);
-        atomicAdd(&acc[1], (unsigned long long)b);
-        atomicAdd(&acc[2], (unsigned long long)c);
+    const int w = threadIdx.x / WAVE;
+    if (__lane_id() == 0) { lds[0][w] = a; lds[1][w] = b; lds[2][w] = c; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        long long t = 0;
+        for (int i = 0; i < 256 / WAVE; i++) t += lds[threadIdx.x][i];
+        atomicAdd(&acc[threadIdx.x], (unsigned long long)t);
     }
 }
 
@@ -225,6 +245,7 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *part, int np
 
 int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out) {
     if (rule.kind == PCD_RULE_WCC) return wcc_metric_dev(ctx, out);
+    kt_begin(ctx);
     if (rule.kind == PCD_RULE_CONDUCTANCE) {
         hipLaunchKernelGGL(k_cond_partial, dim3(FSUM_BLOCKS), dim3(256), 0, ctx->stream, ctx->d_stats, ctx->d_k,
                            ctx->d_dacc);
@@ -233,11 +254,12 @@ int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out) {
                            (double)ctx->n, ctx->d_result);
     } else {
         PCD_TRY(fill_u64(ctx, (uint64_t *)ctx->d_acc, 0, 4));
-        hipLaunchKernelGGL(k_metric_sums, dim3(grid_for(ctx, ctx->k, 256, 4)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_metric_sums, dim3(grid_for(ctx, ctx->k, 256 * 8, 2)), dim3(256), 0, ctx->stream,
                            ctx->d_stats, ctx->d_k, (unsigned long long *)ctx->d_acc);
         hipLaunchKernelGGL(k_metric_final, dim3(1), dim3(1), 0, ctx->stream, (const unsigned long long *)ctx->d_acc,
                            rule.kind, rule.lambda, ctx->m, ctx->d_result);
     }
+    kt_end(ctx, KS_METRIC);
     PCD_HIP(hipGetLastError());
     PCD_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     PCD_HIP(hipStreamSynchronize(ctx->stream));

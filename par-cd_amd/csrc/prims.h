@@ -62,6 +62,26 @@ __device__ inline int block_excl_scan(int x, int *total, int *lds /* >= SCAN_THR
     return base + inc - x;
 }
 
+// Block-wide exclusive scan for a block of NT threads (all threads call).
+template <int NT>
+__device__ inline int block_excl_scan_n(int x, int *total, int *lds /* >= NT/WAVE */) {
+    const int lane = lane_id(), w = threadIdx.x / WAVE;
+    int inc = wave_incl_scan(x);
+    __syncthreads();
+    if (lane == WAVE - 1) lds[w] = inc;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / WAVE; i++) {
+        int t = lds[i];
+        if (i < w) base += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - x;
+}
+
 // Phase 1: per-tile sums.
 template <class Load>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(int64_t n, Load load, int32_t *parts) {

@@ -4,20 +4,27 @@
 // Per active vertex v the step needs, for every distinct neighbour label c,
 // the count of neighbours in c, then score_join(v, c, count) for c != own,
 // the best of those (score, then tie key, then label), score_leave(v, own
-// count) and the move decision.  Vertices are processed by degree bin:
+// count) and the move decision.  Vertex selection (counter hash against
+// frac) is evaluated inline by every kernel; an inactive vertex writes its
+// own label.  Vertices are processed by degree bin (positions are degree-
+// descending, so every bin is a contiguous position range):
 //
-//   G4..G64   deg <= G: G lanes per vertex, one neighbour label per lane,
-//             bitonic sort across the lanes (shuffles), run heads found by
-//             ballot, butterfly argmax.  No LDS.
+//   G4..G64   deg <= G: each wave compacts the active vertices of 64
+//             consecutive positions (ballot), then G lanes per vertex load
+//             one neighbour label each, bitonic-sort them by shuffles, find
+//             run heads by ballot and reduce the best candidate by butterfly.
 //   W256      one wave per vertex, LDS hash table of nextpow2(2 deg) slots.
-//   W1024     as W256 with a larger per-wave table.
-//   BLOCK     one workgroup per vertex, LDS hash table (deg <= block_max).
-//   HUB       rows split into chunks; each chunk is pre-aggregated in an LDS
-//             table and flushed into a per-hub global hash table; one
-//             workgroup per hub then scores the table.
-//
-// Inactive vertices (counter hash > frac) copy their label in the select
-// pass, which also builds the per-bin active lists.
+//   B1024     one 256-thread workgroup per vertex, 2048-slot LDS table.
+//   B4096     one 1024-thread workgroup per vertex, 8192-slot LDS table.
+//   B8192     one 1024-thread workgroup per vertex, 16384-slot LDS table.
+//   HUB       deg > 8192.  Pass 1: every 4096-entry chunk of a hub row
+//             gathers its labels and writes them grouped by label-hash
+//             partition (LDS counting sort).  Pass 2: one workgroup per
+//             (hub, partition) counts the partition's labels in a 16384-slot
+//             LDS table and reduces its best candidate.  Pass 3: one wave
+//             per hub combines the partitions and decides.  A partition
+//             whose distinct labels do not fit the table flags its hub, and
+//             the hub is then recounted in a global-memory hash table.
 #include "prims.h"
 
 namespace pcd {
@@ -45,16 +52,26 @@ __device__ inline Cand cand_shfl_xor(const Cand &c, int d) {
     return o;
 }
 
+// partial result of one hub partition
+struct HubPartial {
+    Cand c;
+    int own_cnt;
+    int distinct;
+};
+
 struct LPArgs {
-    const int32_t *indices, *adj, *labels;
+    const int32_t *indices, *adj, *labels, *order;
     const CommStats *stats;
     int32_t *new_labels;
     RuleParams rule;
     uint64_t seed;
     int step;
     int tie;
-    unsigned long long *counters;  // {active, entries, distinct} or nullptr
+    int f;                         // selection threshold (refinement.cpp:158)
+    unsigned long long *counters;  // {active, entries, distinct} of the bin, or nullptr
 };
+
+__device__ inline int vertex_at(const LPArgs &a, int p) { return a.order ? a.order[p] : p; }
 
 // refinement.cpp:211-221: best_score (0 if no join candidate beat 0) plus
 // score_leave decides between moving, leaving to a new singleton, staying.
@@ -74,103 +91,84 @@ __device__ inline Cand lp_candidate(const LPArgs &a, int deg, uint64_t tb, int c
     return cand_none();
 }
 
-// ------------------------------------------------------------- select
-__global__ __launch_bounds__(256) void k_lp_select(int n, const int32_t *order, const int32_t *labels,
-                                                   int32_t *new_labels, uint64_t seed, int step, int f, BinLayout bins,
-                                                   int32_t *active, int32_t *counts, int32_t *hub_active) {
-    const int lane = __lane_id();
-    const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
-    for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x; p0 < n; p0 += (int64_t)gridDim.x * blockDim.x) {
-        int p = (int)(p0 + threadIdx.x);
-        bool valid = p < n;
-        int v = 0, b = NUM_BINS;
-        bool act = false;
-        if (valid) {
-            v = order ? order[p] : p;
-            act = selected(seed, step, v, f);
-            if (!act) new_labels[v] = labels[v];
-            b = 0;
-            while (p >= bins.start[b + 1]) b++;
-            if (b == BIN_HUB) {
-                hub_active[p - bins.start[BIN_HUB]] = act ? 1 : 0;
-                act = false;
-            }
-        }
-        uint64_t pend = __ballot(act);
-        while (pend) {
-            int leader = __builtin_ctzll(pend);
-            int b0 = __shfl(b, leader, WAVE);
-            uint64_t mine = __ballot(act && b == b0);
-            int base = 0;
-            if (lane == leader) base = atomicAdd(&counts[b0], __popcll(mine));
-            base = __shfl(base, leader, WAVE);
-            if (act && b == b0) active[bins.start[b0] + base + __popcll(mine & lt)] = v;
-            pend &= ~mine;
-        }
-    }
+__device__ inline void count3(const LPArgs &a, long long act, long long ent, long long dist) {
+    atomicAdd(&a.counters[0], (unsigned long long)act);
+    atomicAdd(&a.counters[1], (unsigned long long)ent);
+    atomicAdd(&a.counters[2], (unsigned long long)dist);
 }
 
 // ------------------------------------------------------------- G lanes / vertex
-template <int G>
-__global__ __launch_bounds__(256) void k_lp_small(const int32_t *list, const int32_t *count_ptr, LPArgs a) {
-    const int lane = __lane_id(), glane = lane & (G - 1);
+template <int G, int CP>
+__global__ __launch_bounds__(256) void k_lp_small(int p0, int p1, LPArgs a) {
+    __shared__ int s_list[256 / WAVE][WAVE];
+    const int lane = __lane_id(), glane = lane & (G - 1), w = threadIdx.x / WAVE;
+    const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
     const int per_wave = WAVE / G;
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int nwaves = gridDim.x * blockDim.x / WAVE;
-    const int count = *count_ptr;
-    for (int base = wave_g * per_wave; base < count; base += nwaves * per_wave) {
-        int gi = base + lane / G;
-        bool has = gi < count;
-        int v = 0, start = 0, deg = 0, own = 0;
-        if (has) {
-            v = list[gi];
-            start = a.indices[v];
-            deg = a.indices[v + 1] - start;
-            own = a.labels[v];
-        }
-        int x = (has && glane < deg) ? a.labels[a.adj[start + glane]] : INT32_MAX;
-        // bitonic sort (ascending) across the G lanes of the group
+    int *list = s_list[w];
+    for (int cbase = p0 + wave_g * CP; cbase < p1; cbase += nwaves * CP) {
+        // compaction of the active vertices among CP consecutive positions
+        int p = cbase + lane;
+        bool valid = lane < CP && p < p1;
+        int v = valid ? vertex_at(a, p) : 0;
+        bool act = valid && selected(a.seed, a.step, v, a.f);
+        if (valid && !act) a.new_labels[v] = a.labels[v];
+        uint64_t am = __ballot(act);
+        int nact = __popcll(am);
+        if (act) list[__popcll(am & lt)] = v;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int gb = 0; gb < nact; gb += per_wave) {
+            int gi = gb + lane / G;
+            bool has = gi < nact;
+            int u = 0, start = 0, deg = 0, own = 0;
+            if (has) {
+                u = list[gi];
+                start = a.indices[u];
+                deg = a.indices[u + 1] - start;
+                own = a.labels[u];
+            }
+            int x = (has && glane < deg) ? a.labels[a.adj[start + glane]] : INT32_MAX;
 #pragma unroll
-        for (int k = 2; k <= G; k <<= 1) {
+            for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                int y = __shfl_xor(x, j, WAVE);
-                bool up = (glane & k) == 0;
-                bool lower = (glane & j) == 0;
-                x = (lower == up) ? min(x, y) : max(x, y);
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    int y = __shfl_xor(x, j, WAVE);
+                    bool up = (glane & k) == 0;
+                    bool lower = (glane & j) == 0;
+                    x = (lower == up) ? min(x, y) : max(x, y);
+                }
+            }
+            int prev = __shfl(x, lane == 0 ? 0 : lane - 1, WAVE);
+            bool head = x != INT32_MAX && (glane == 0 || x != prev);
+            uint64_t heads = __ballot(head);
+            Cand c = cand_none();
+            int own_cnt = 0;
+            if (head) {
+                uint64_t above = lane == 63 ? 0ULL : (heads >> (lane + 1));
+                int nxt = above ? lane + 1 + __builtin_ctzll(above) : WAVE;
+                int gend = (lane & ~(G - 1)) + deg;
+                int cnt = min(nxt, gend) - lane;
+                if (x == own) own_cnt = cnt;
+                else c = lp_candidate(a, deg, a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, u), x, cnt);
+            }
+#pragma unroll
+            for (int d = G / 2; d > 0; d >>= 1) {
+                Cand o = cand_shfl_xor(c, d);
+                if (cand_better(o, c)) c = o;
+                own_cnt = max(own_cnt, __shfl_xor(own_cnt, d, WAVE));
+            }
+            if (has && glane == 0) a.new_labels[u] = lp_decide(a, deg, own, own_cnt, c);
+            if (a.counters) {
+                int ent = wave_sum((has && glane < deg) ? 1 : 0);
+                if (lane == 0) count3(a, __popcll(__ballot(has && glane == 0)), ent, __popcll(heads));
             }
         }
-        int prev = __shfl(x, lane == 0 ? 0 : lane - 1, WAVE);
-        bool head = x != INT32_MAX && (glane == 0 || x != prev);
-        uint64_t heads = __ballot(head);
-        Cand c = cand_none();
-        int own_cnt = 0;
-        if (head) {
-            uint64_t above = lane == 63 ? 0ULL : (heads >> (lane + 1));
-            int nxt = above ? lane + 1 + __builtin_ctzll(above) : WAVE;
-            int gend = (lane & ~(G - 1)) + deg;
-            int cnt = min(nxt, gend) - lane;
-            if (x == own) own_cnt = cnt;
-            else c = lp_candidate(a, deg, a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v), x, cnt);
-        }
-#pragma unroll
-        for (int d = G / 2; d > 0; d >>= 1) {
-            Cand o = cand_shfl_xor(c, d);
-            if (cand_better(o, c)) c = o;
-            own_cnt = max(own_cnt, __shfl_xor(own_cnt, d, WAVE));
-        }
-        if (has && glane == 0) a.new_labels[v] = lp_decide(a, deg, own, own_cnt, c);
-        if (a.counters) {
-            int nh = __popcll(heads);
-            int nact = __popcll(__ballot(has && glane == 0));
-            int ent = (has && glane < deg) ? 1 : 0;
-            ent = wave_sum(ent);
-            if (lane == 0) {
-                atomicAdd(&a.counters[0], (unsigned long long)nact);
-                atomicAdd(&a.counters[1], (unsigned long long)ent);
-                atomicAdd(&a.counters[2], (unsigned long long)nh);
-            }
-        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -179,16 +177,19 @@ __device__ inline uint32_t hslot(int key, int log_t) {
     return ((uint32_t)key * 0x9E3779B1u) >> (32 - log_t);
 }
 
-__device__ inline void lds_insert(int *keys, int *cnts, int mask, int log_t, int key, int add) {
+// Inserts key with count add.  Probes at most T slots; returns false when
+// the table is full (the key was not inserted).
+__device__ inline bool lds_insert(int *keys, int *cnts, int mask, int log_t, int key, int add) {
     uint32_t h = hslot(key, log_t);
-    while (true) {
+    for (int probe = 0; probe <= mask; probe++) {
         int old = atomicCAS(&keys[h], EMPTY_KEY, key);
         if (old == EMPTY_KEY || old == key) {
             atomicAdd(&cnts[h], add);
-            return;
+            return true;
         }
         h = (h + 1) & mask;
     }
+    return false;
 }
 
 __device__ inline void wave_sync() {
@@ -199,7 +200,6 @@ __device__ inline void wave_sync() {
 
 __device__ inline int ceil_log2(int x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
 
-// Reduce (Cand, own_cnt, distinct) across a wave; result valid in all lanes.
 __device__ inline void wave_reduce_cand(Cand &c, int &own_cnt) {
 #pragma unroll
     for (int d = WAVE / 2; d > 0; d >>= 1) {
@@ -209,48 +209,93 @@ __device__ inline void wave_reduce_cand(Cand &c, int &own_cnt) {
     }
 }
 
+// Gathers the labels of row entries [start + lo, start + deg) with stride
+// `stride`, PF at a time (independent loads in flight), and inserts them.
+template <int PF>
+__device__ inline void insert_row(const LPArgs &a, int *keys, int *cnts, int mask, int log_t, int start, int deg,
+                                  int lo, int stride) {
+    for (int j0 = lo; j0 < deg; j0 += PF * stride) {
+        int nb[PF];
+#pragma unroll
+        for (int e = 0; e < PF; e++) {
+            int j = j0 + e * stride;
+            nb[e] = j < deg ? a.adj[start + j] : -1;
+        }
+        int lab[PF];
+#pragma unroll
+        for (int e = 0; e < PF; e++) lab[e] = nb[e] >= 0 ? a.labels[nb[e]] : EMPTY_KEY;
+#pragma unroll
+        for (int e = 0; e < PF; e++)
+            if (lab[e] != EMPTY_KEY) lds_insert(keys, cnts, mask, log_t, lab[e], 1);
+    }
+}
+
+// Scans table slots [lo, T) with stride `stride` for candidates.  Slots are
+// taken SB at a time so that their community-record gathers are in flight
+// together.
+constexpr int SB = 4;
+__device__ inline void scan_table(const LPArgs &a, const int *keys, const int *cnts, int T, int lo, int stride,
+                                  int deg, int own, uint64_t tb, Cand &c, int &own_cnt, int &distinct) {
+    for (int s0 = lo; s0 < T; s0 += SB * stride) {
+        int key[SB], cnt[SB];
+        CommStats st[SB];
+#pragma unroll
+        for (int e = 0; e < SB; e++) {
+            int s = s0 + e * stride;
+            key[e] = s < T ? keys[s] : EMPTY_KEY;
+            cnt[e] = key[e] != EMPTY_KEY ? cnts[s] : 0;
+        }
+#pragma unroll
+        for (int e = 0; e < SB; e++)
+            if (key[e] != EMPTY_KEY && key[e] != own) st[e] = a.stats[key[e]];
+#pragma unroll
+        for (int e = 0; e < SB; e++) {
+            if (key[e] == EMPTY_KEY) continue;
+            distinct++;
+            if (key[e] == own) {
+                own_cnt = cnt[e];
+            } else {
+                double sc = score_join(a.rule, deg, st[e], cnt[e]);
+                if (sc > 0) {
+                    Cand o{sc, tie_key(a.tie, tb, key[e]), key[e]};
+                    if (cand_better(o, c)) c = o;
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------- wave / vertex
 template <int TW>
-__global__ __launch_bounds__(256) void k_lp_wave(const int32_t *list, const int32_t *count_ptr, LPArgs a) {
-    __shared__ int s_keys[4][TW];
-    __shared__ int s_cnts[4][TW];
+__global__ __launch_bounds__(256) void k_lp_wave(int p0, int p1, LPArgs a) {
+    __shared__ int s_keys[256 / WAVE][TW];
+    __shared__ int s_cnts[256 / WAVE][TW];
     const int lane = __lane_id(), w = threadIdx.x / WAVE;
     int *keys = s_keys[w], *cnts = s_cnts[w];
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int nwaves = gridDim.x * blockDim.x / WAVE;
-    const int count = *count_ptr;
-    for (int gi = wave_g; gi < count; gi += nwaves) {
-        int v = list[gi];
+    for (int p = p0 + wave_g; p < p1; p += nwaves) {
+        int v = vertex_at(a, p);
+        if (!selected(a.seed, a.step, v, a.f)) {
+            if (lane == 0) a.new_labels[v] = a.labels[v];
+            continue;
+        }
         int start = a.indices[v], deg = a.indices[v + 1] - start, own = a.labels[v];
         int log_t = max(6, ceil_log2(2 * deg));
         int T = 1 << log_t, mask = T - 1;
         for (int s = lane; s < T; s += WAVE) { keys[s] = EMPTY_KEY; cnts[s] = 0; }
         wave_sync();
-        for (int j = lane; j < deg; j += WAVE) lds_insert(keys, cnts, mask, log_t, a.labels[a.adj[start + j]], 1);
+        insert_row<4>(a, keys, cnts, mask, log_t, start, deg, lane, WAVE);
         wave_sync();
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        for (int s = lane; s < T; s += WAVE) {
-            int key = keys[s];
-            if (key == EMPTY_KEY) continue;
-            distinct++;
-            int cnt = cnts[s];
-            if (key == own) own_cnt = cnt;
-            else {
-                Cand o = lp_candidate(a, deg, tb, key, cnt);
-                if (cand_better(o, c)) c = o;
-            }
-        }
+        scan_table(a, keys, cnts, T, lane, WAVE, deg, own, tb, c, own_cnt, distinct);
         wave_reduce_cand(c, own_cnt);
         if (lane == 0) a.new_labels[v] = lp_decide(a, deg, own, own_cnt, c);
         if (a.counters) {
             distinct = wave_sum(distinct);
-            if (lane == 0) {
-                atomicAdd(&a.counters[0], 1ULL);
-                atomicAdd(&a.counters[1], (unsigned long long)deg);
-                atomicAdd(&a.counters[2], (unsigned long long)distinct);
-            }
+            if (lane == 0) count3(a, 1, deg, distinct);
         }
         wave_sync();
     }
@@ -276,108 +321,244 @@ __device__ inline void block_reduce_cand(Cand &c, int &own_cnt, int &distinct, C
 
 // ------------------------------------------------------------- block / vertex
 template <int NT, int TB>
-__global__ __launch_bounds__(NT) void k_lp_block(const int32_t *list, const int32_t *count_ptr, LPArgs a) {
+__global__ __launch_bounds__(NT) void k_lp_block(int p0, int p1, LPArgs a) {
     __shared__ int keys[TB];
     __shared__ int cnts[TB];
     __shared__ Cand s_c[NT / WAVE];
     __shared__ int s_i[2 * NT / WAVE];
-    const int count = *count_ptr;
-    for (int gi = blockIdx.x; gi < count; gi += gridDim.x) {
-        int v = list[gi];
+    for (int p = p0 + blockIdx.x; p < p1; p += gridDim.x) {
+        int v = vertex_at(a, p);
+        if (!selected(a.seed, a.step, v, a.f)) {
+            if (threadIdx.x == 0) a.new_labels[v] = a.labels[v];
+            continue;
+        }
         int start = a.indices[v], deg = a.indices[v + 1] - start, own = a.labels[v];
         int log_t = max(6, ceil_log2(2 * deg));
         int T = 1 << log_t, mask = T - 1;
         for (int s = threadIdx.x; s < T; s += NT) { keys[s] = EMPTY_KEY; cnts[s] = 0; }
         __syncthreads();
-        for (int j = threadIdx.x; j < deg; j += NT) lds_insert(keys, cnts, mask, log_t, a.labels[a.adj[start + j]], 1);
+        insert_row<(TB / 2 / NT > 8 ? 8 : (TB / 2 / NT < 1 ? 1 : TB / 2 / NT))>(a, keys, cnts, mask, log_t, start,
+                                                                                  deg, threadIdx.x, NT);
         __syncthreads();
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        for (int s = threadIdx.x; s < T; s += NT) {
-            int key = keys[s];
-            if (key == EMPTY_KEY) continue;
-            distinct++;
-            int cnt = cnts[s];
-            if (key == own) own_cnt = cnt;
-            else {
-                Cand o = lp_candidate(a, deg, tb, key, cnt);
-                if (cand_better(o, c)) c = o;
-            }
-        }
+        scan_table(a, keys, cnts, T, threadIdx.x, NT, deg, own, tb, c, own_cnt, distinct);
         block_reduce_cand<NT>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
             a.new_labels[v] = lp_decide(a, deg, own, own_cnt, c);
-            if (a.counters) {
-                atomicAdd(&a.counters[0], 1ULL);
-                atomicAdd(&a.counters[1], (unsigned long long)deg);
-                atomicAdd(&a.counters[2], (unsigned long long)distinct);
-            }
+            if (a.counters) count3(a, 1, deg, distinct);
         }
     }
 }
 
 // ------------------------------------------------------------- hubs
-// items: {hub index, v, begin, end}.  LDS pre-aggregation of one chunk,
-// then flush into the hub's global table (open addressing, CAS on keys).
-template <int NT, int TB>
-__global__ __launch_bounds__(NT) void k_hub_insert(int nitems, const int4 *items, const int32_t *hub_active,
-                                                   const int64_t *toff, const int32_t *tlog, int32_t *gkeys,
-                                                   int32_t *gcnts, const int32_t *adj, const int32_t *labels) {
-    __shared__ int keys[TB];
-    __shared__ int cnts[TB];
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        int4 w = items[it];
-        if (!hub_active[w.x]) continue;
-        const int len = w.w - w.z;
-        const int log_t = max(6, ceil_log2(2 * len));
-        const int T = 1 << log_t, mask = T - 1;
-        for (int s = threadIdx.x; s < T; s += NT) { keys[s] = EMPTY_KEY; cnts[s] = 0; }
+struct HubArgs {
+    const int4 *chunks;        // {hub, v, begin, end}
+    const int32_t *chunk_off;  // chunk -> first of its P+1 offsets in poff
+    const int4 *parts;         // {hub, part, first chunk, number of chunks}
+    const int4 *info;          // per hub {v, P, first part item, first chunk}
+    const int64_t *buf_base;   // per hub
+    int32_t *poff;
+    int32_t *buf;
+    HubPartial *partial;
+    int32_t *over;
+    int nchunks, nparts, nhubs;
+    int table_slots;  // LDS slots used in pass 2 (power of two <= HUB_TB)
+};
+
+// Partition of a label: a 32-bit finaliser (murmur3 fmix32) independent of
+// the multiplicative slot hash, so the keys of one partition still spread
+// over the whole LDS table.
+__device__ inline int hub_part_of(int label, int P) {
+    uint32_t x = (uint32_t)label;
+    x ^= x >> 16;
+    x *= 0x85EBCA6Bu;
+    x ^= x >> 13;
+    x *= 0xC2B2AE35u;
+    x ^= x >> 16;
+    return (int)(((uint64_t)x * (uint32_t)P) >> 32);
+}
+
+constexpr int HUB1_NT = 512, HUB1_PER = 8;  // 4096 entries per chunk item
+constexpr int HUB2_NT = 1024, HUB_TB = 16384;
+
+// pass 1: gather the chunk's labels and group them by partition
+__global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) {
+    __shared__ int cnt[HUB_MAX_PARTS + 1];
+    for (int it = blockIdx.x; it < h.nchunks; it += gridDim.x) {
+        const int4 ck = h.chunks[it];
+        const int v = ck.y;
+        if (!selected(a.seed, a.step, v, a.f)) continue;
+        const int4 inf = h.info[ck.x];
+        const int P = inf.y;
+        for (int i = threadIdx.x; i <= P; i += HUB1_NT) cnt[i] = 0;
         __syncthreads();
-        for (int j = w.z + threadIdx.x; j < w.w; j += NT) lds_insert(keys, cnts, mask, log_t, labels[adj[j]], 1);
-        __syncthreads();
-        const int64_t off = toff[w.x];
-        const int glog = tlog[w.x];
-        const int gmask = (1 << glog) - 1;
-        for (int s = threadIdx.x; s < T; s += NT) {
-            int key = keys[s];
-            if (key == EMPTY_KEY) continue;
-            int add = cnts[s];
-            uint32_t h = hslot(key, glog);
-            while (true) {
-                int old = atomicCAS(&gkeys[off + h], EMPTY_KEY, key);
-                if (old == EMPTY_KEY || old == key) {
-                    atomicAdd(&gcnts[off + h], add);
-                    break;
-                }
-                h = (h + 1) & gmask;
+        int lab[HUB1_PER], part[HUB1_PER], rank[HUB1_PER];
+#pragma unroll
+        for (int e = 0; e < HUB1_PER; e++) {
+            int j = ck.z + e * HUB1_NT + threadIdx.x;
+            part[e] = -1;
+            if (j < ck.w) {
+                lab[e] = a.labels[a.adj[j]];
+                part[e] = hub_part_of(lab[e], P);
+                rank[e] = atomicAdd(&cnt[part[e]], 1);
             }
         }
+        __syncthreads();
+        if (threadIdx.x == 0) {  // exclusive scan of the partition counts
+            int run = 0;
+            for (int q = 0; q < P; q++) {
+                int t = cnt[q];
+                cnt[q] = run;
+                run += t;
+            }
+            cnt[P] = run;
+        }
+        __syncthreads();
+        const int64_t base = h.buf_base[ck.x] + (ck.z - a.indices[v]);
+        int32_t *po = h.poff + h.chunk_off[it];
+        for (int q = threadIdx.x; q <= P; q += HUB1_NT) po[q] = (int32_t)(base + cnt[q] - h.buf_base[ck.x]);
+#pragma unroll
+        for (int e = 0; e < HUB1_PER; e++)
+            if (part[e] >= 0) h.buf[base + cnt[part[e]] + rank[e]] = lab[e];
         __syncthreads();
     }
 }
 
-template <int NT>
-__global__ __launch_bounds__(NT) void k_hub_decide(int p0, int nhubs, const int32_t *order, const int32_t *hub_active,
-                                                   const int64_t *toff, const int32_t *tlog, int32_t *gkeys,
-                                                   int32_t *gcnts, LPArgs a) {
-    __shared__ Cand s_c[NT / WAVE];
-    __shared__ int s_i[2 * NT / WAVE];
-    for (int h = blockIdx.x; h < nhubs; h += gridDim.x) {
-        if (!hub_active[h]) continue;
-        int p = p0 + h;
-        int v = order ? order[p] : p;
-        int start = a.indices[v], deg = a.indices[v + 1] - start, own = a.labels[v];
-        const int64_t off = toff[h];
-        const int T = 1 << tlog[h];
+// pass 2: count one partition of one hub in LDS, reduce its best candidate
+__global__ __launch_bounds__(HUB2_NT) void k_hub_count(LPArgs a, HubArgs h) {
+    __shared__ int keys[HUB_TB];
+    __shared__ int cnts[HUB_TB];
+    __shared__ int seg[HUB2_NT + 1];
+    __shared__ int segbase[HUB2_NT];
+    __shared__ Cand s_c[HUB2_NT / WAVE];
+    __shared__ int s_i[2 * HUB2_NT / WAVE];
+    __shared__ int s_over;
+    const int T = h.table_slots, log_t = 31 - __clz(T), mask = T - 1;
+    for (int it = blockIdx.x; it < h.nparts; it += gridDim.x) {
+        const int4 pi = h.parts[it];
+        const int4 inf = h.info[pi.x];
+        const int v = inf.x, P = inf.y, part = pi.y;
+        if (!selected(a.seed, a.step, v, a.f)) continue;
+        for (int s = threadIdx.x; s < T; s += HUB2_NT) { keys[s] = EMPTY_KEY; cnts[s] = 0; }
+        if (threadIdx.x == 0) s_over = 0;
+        __syncthreads();
+        const int64_t hb = h.buf_base[pi.x];
+        for (int c0 = 0; c0 < pi.w; c0 += HUB2_NT) {
+            int nc = min(HUB2_NT, pi.w - c0);
+            int len = 0, b0 = 0;
+            if (threadIdx.x < nc) {
+                const int32_t *po = h.poff + h.chunk_off[pi.z + c0 + threadIdx.x];
+                b0 = po[part];
+                len = po[part + 1] - b0;
+            }
+            int tot;
+            int ex = block_excl_scan_n<HUB2_NT>(len, &tot, s_i);
+            if (threadIdx.x < nc) { seg[threadIdx.x] = ex; segbase[threadIdx.x] = b0; }
+            if (threadIdx.x == 0) seg[nc] = tot;
+            __syncthreads();
+            for (int i = threadIdx.x; i < tot; i += HUB2_NT) {
+                int lo = 0, hi = nc;  // last segment with seg[s] <= i
+                while (hi - lo > 1) {
+                    int mid = (lo + hi) >> 1;
+                    if (seg[mid] <= i) lo = mid;
+                    else hi = mid;
+                }
+                int key = h.buf[hb + segbase[lo] + (i - seg[lo])];
+                if (!lds_insert(keys, cnts, mask, log_t, key, 1)) s_over = 1;
+            }
+            __syncthreads();
+        }
+        if (s_over) {
+            if (threadIdx.x == 0) h.over[pi.x] = 1;
+            __syncthreads();
+            continue;
+        }
+        const int deg = a.indices[v + 1] - a.indices[v], own = a.labels[v];
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        for (int s = threadIdx.x; s < T; s += NT) {
+        scan_table(a, keys, cnts, T, threadIdx.x, HUB2_NT, deg, own, tb, c, own_cnt, distinct);
+        block_reduce_cand<HUB2_NT>(c, own_cnt, distinct, s_c, s_i);
+        if (threadIdx.x == 0) h.partial[it] = HubPartial{c, own_cnt, distinct};
+        (void)P;
+    }
+}
+
+// pass 3: one wave per hub combines its partitions and decides
+__global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
+    const int lane = __lane_id();
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    for (int hub = wave_g; hub < h.nhubs; hub += nwaves) {
+        const int4 inf = h.info[hub];
+        const int v = inf.x, P = inf.y;
+        if (!selected(a.seed, a.step, v, a.f)) {
+            if (lane == 0) a.new_labels[v] = a.labels[v];
+            continue;
+        }
+        if (h.over[hub]) continue;  // recounted by the global-table path
+        Cand c = cand_none();
+        int own_cnt = 0, distinct = 0;
+        for (int q = lane; q < P; q += WAVE) {
+            HubPartial hp = h.partial[inf.z + q];
+            if (cand_better(hp.c, c)) c = hp.c;
+            own_cnt = max(own_cnt, hp.own_cnt);
+            distinct += hp.distinct;
+        }
+        wave_reduce_cand(c, own_cnt);
+        distinct = wave_sum(distinct);
+        if (lane == 0) {
+            const int deg = a.indices[v + 1] - a.indices[v];
+            a.new_labels[v] = lp_decide(a, deg, a.labels[v], own_cnt, c);
+            if (a.counters) count3(a, 1, deg, distinct);
+        }
+    }
+}
+
+// Fallback for flagged hubs: the whole row counted in a per-hub global
+// open-addressing table of nextpow2(2 deg) slots (never full).
+__global__ __launch_bounds__(512) void k_hub_global_insert(LPArgs a, HubArgs h, const int64_t *toff,
+                                                           const int32_t *tlog, int32_t *gkeys, int32_t *gcnts) {
+    for (int it = blockIdx.x; it < h.nchunks; it += gridDim.x) {
+        const int4 ck = h.chunks[it];
+        if (!h.over[ck.x]) continue;
+        const int64_t off = toff[ck.x];
+        const int glog = tlog[ck.x], gmask = (1 << glog) - 1;
+        for (int j = ck.z + threadIdx.x; j < ck.w; j += blockDim.x) {
+            int key = a.labels[a.adj[j]];
+            uint32_t s = hslot(key, glog);
+            while (true) {
+                int old = atomicCAS(&gkeys[off + s], EMPTY_KEY, key);
+                if (old == EMPTY_KEY || old == key) {
+                    atomicAdd(&gcnts[off + s], 1);
+                    break;
+                }
+                s = (s + 1) & gmask;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h, const int64_t *toff,
+                                                            const int32_t *tlog, int32_t *gkeys, int32_t *gcnts) {
+    __shared__ Cand s_c[1024 / WAVE];
+    __shared__ int s_i[2 * 1024 / WAVE];
+    for (int hub = blockIdx.x; hub < h.nhubs; hub += gridDim.x) {
+        if (!h.over[hub]) continue;
+        const int v = h.info[hub].x;
+        const int deg = a.indices[v + 1] - a.indices[v], own = a.labels[v];
+        const int64_t off = toff[hub];
+        const int T = 1 << tlog[hub];
+        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
+        Cand c = cand_none();
+        int own_cnt = 0, distinct = 0;
+        for (int s = threadIdx.x; s < T; s += 1024) {
             int key = gkeys[off + s];
             if (key == EMPTY_KEY) continue;
             int cnt = gcnts[off + s];
-            gkeys[off + s] = EMPTY_KEY;  // leave the table clean for the next step
+            gkeys[off + s] = EMPTY_KEY;
             gcnts[off + s] = 0;
             distinct++;
             if (key == own) own_cnt = cnt;
@@ -386,97 +567,110 @@ __global__ __launch_bounds__(NT) void k_hub_decide(int p0, int nhubs, const int3
                 if (cand_better(o, c)) c = o;
             }
         }
-        block_reduce_cand<NT>(c, own_cnt, distinct, s_c, s_i);
+        block_reduce_cand<1024>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
             a.new_labels[v] = lp_decide(a, deg, own, own_cnt, c);
-            if (a.counters) {
-                atomicAdd(&a.counters[0], 1ULL);
-                atomicAdd(&a.counters[1], (unsigned long long)deg);
-                atomicAdd(&a.counters[2], (unsigned long long)distinct);
-            }
+            if (a.counters) count3(a, 1, deg, distinct);
+            h.over[hub] = 0;
         }
     }
 }
 
 // ------------------------------------------------------------- driver
-template <int G>
-static void launch_small(pcd_ctx *ctx, int bin, const LPArgs &a) {
-    const BinLayout &b = ctx->bins;
-    int cap = b.start[bin + 1] - b.start[bin];
-    if (cap <= 0) return;
-    hipLaunchKernelGGL(k_lp_small<G>, dim3(grid_for(ctx, cap, 256 / G, 16)), dim3(256), 0, ctx->stream,
-                       ctx->d_active + b.start[bin], ctx->d_counts + bin, a);
-}
-
-constexpr int HUB_NT = 512, HUB_TB = 8192;
-constexpr int BLK_NT = 512, BLK_TB = 8192;
-
 int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie) {
-    const int n = ctx->n;
     const BinLayout &b = ctx->bins;
     double fr = frac < 1.0 ? frac : 1.0;
-    int f = (int)(fr * 2147483647.0);  // refinement.cpp:158
     LPArgs a;
     a.indices = ctx->d_indices;
     a.adj = ctx->d_adj;
     a.labels = ctx->d_labels;
+    a.order = ctx->d_order;
     a.stats = ctx->d_stats;
     a.new_labels = ctx->d_labels_alt;
     a.rule = make_rule(ctx, rule);
     a.seed = seed;
     a.step = step;
     a.tie = tie;
+    a.f = (int)(fr * 2147483647.0);  // refinement.cpp:158
     a.counters = nullptr;
-    unsigned long long *cnt64 = (unsigned long long *)(ctx->d_acc + 4);
-    if (ctx->counters) {
-        PCD_TRY(fill_u64(ctx, (uint64_t *)cnt64, 0, 3));
-        a.counters = cnt64;
-    }
-    PCD_TRY(fill_i32(ctx, ctx->d_counts, 0, NUM_BINS));
-    hipLaunchKernelGGL(k_lp_select, dim3(grid_for(ctx, n, 256, 16)), dim3(256), 0, ctx->stream, n, ctx->d_order,
-                       ctx->d_labels, ctx->d_labels_alt, seed, step, f, b, ctx->d_active, ctx->d_counts,
-                       ctx->d_hub_active);
+    auto args_for = [&](int bin) {
+        LPArgs x = a;
+        if (ctx->counters) x.counters = ctx->d_bin_counters + 3 * bin;
+        return x;
+    };
+    auto range = [&](int bin, int &p0, int &p1) {
+        p0 = b.start[bin];
+        p1 = b.start[bin + 1];
+        return p1 > p0;
+    };
+    int p0, p1;
     if (ctx->num_hubs > 0) {
-        hipLaunchKernelGGL((k_hub_insert<HUB_NT, HUB_TB>), dim3(grid_for(ctx, ctx->num_hub_items, 1, 2)),
-                           dim3(HUB_NT), 0, ctx->stream, ctx->num_hub_items, ctx->d_hub_items, ctx->d_hub_active,
-                           ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts, ctx->d_adj,
-                           ctx->d_labels);
-        hipLaunchKernelGGL(k_hub_decide<HUB_NT>, dim3(grid_for(ctx, ctx->num_hubs, 1, 2)), dim3(HUB_NT), 0,
-                           ctx->stream, b.start[BIN_HUB], ctx->num_hubs, ctx->d_order, ctx->d_hub_active,
-                           ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts, a);
+        HubArgs h;
+        h.chunks = ctx->d_hub_items;
+        h.chunk_off = ctx->d_hub_item_off;
+        h.parts = ctx->d_hub_part_items;
+        h.info = ctx->d_hub_info;
+        h.buf_base = ctx->d_hub_buf_base;
+        h.poff = ctx->d_hub_poff;
+        h.buf = ctx->d_hub_buf;
+        h.partial = (HubPartial *)ctx->d_hub_partial;
+        h.over = ctx->d_hub_over;
+        h.nchunks = ctx->num_hub_items;
+        h.nparts = ctx->num_hub_parts;
+        h.nhubs = ctx->num_hubs;
+        h.table_slots = ctx->tune.hub_table_slots;
+        LPArgs ah = args_for(BIN_HUB);
+        kt_begin(ctx);
+        hipLaunchKernelGGL(k_hub_partition, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0, ctx->stream, ah,
+                           h);
+        kt_end(ctx, KS_LP_HUB_INSERT);
+        kt_begin(ctx);
+        hipLaunchKernelGGL(k_hub_count, dim3(grid_for(ctx, h.nparts, 1, 1)), dim3(HUB2_NT), 0, ctx->stream, ah, h);
+        hipLaunchKernelGGL(k_hub_final, dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0, ctx->stream, ah, h);
+        hipLaunchKernelGGL(k_hub_global_insert, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(512), 0, ctx->stream, ah,
+                           h, ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts);
+        hipLaunchKernelGGL(k_hub_global_decide, dim3(grid_for(ctx, h.nhubs, 1, 1)), dim3(1024), 0, ctx->stream, ah,
+                           h, ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts);
+        kt_end(ctx, KS_LP_HUB_DECIDE);
     }
-    {
-        int cap = b.start[BIN_BLOCK + 1] - b.start[BIN_BLOCK];
-        if (cap > 0)
-            hipLaunchKernelGGL((k_lp_block<BLK_NT, BLK_TB>), dim3(grid_for(ctx, cap, 1, 2)), dim3(BLK_NT), 0,
-                               ctx->stream, ctx->d_active + b.start[BIN_BLOCK], ctx->d_counts + BIN_BLOCK, a);
+    if (range(BIN_B8192, p0, p1)) {
+        kt_begin(ctx);
+        hipLaunchKernelGGL((k_lp_block<1024, 16384>), dim3(grid_for(ctx, p1 - p0, 1, 1)), dim3(1024), 0, ctx->stream,
+                           p0, p1, args_for(BIN_B8192));
+        kt_end(ctx, KS_LP_B8192);
     }
-    {
-        int cap = b.start[BIN_W1024 + 1] - b.start[BIN_W1024];
-        if (cap > 0)
-            hipLaunchKernelGGL(k_lp_wave<2048>, dim3(grid_for(ctx, cap, 4, 2)), dim3(256), 0, ctx->stream,
-                               ctx->d_active + b.start[BIN_W1024], ctx->d_counts + BIN_W1024, a);
+    if (range(BIN_B4096, p0, p1)) {
+        kt_begin(ctx);
+        hipLaunchKernelGGL((k_lp_block<512, 8192>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0, ctx->stream,
+                           p0, p1, args_for(BIN_B4096));
+        kt_end(ctx, KS_LP_B4096);
     }
-    {
-        int cap = b.start[BIN_W256 + 1] - b.start[BIN_W256];
-        if (cap > 0)
-            hipLaunchKernelGGL(k_lp_wave<512>, dim3(grid_for(ctx, cap, 4, 8)), dim3(256), 0, ctx->stream,
-                               ctx->d_active + b.start[BIN_W256], ctx->d_counts + BIN_W256, a);
+    if (range(BIN_B1024, p0, p1)) {
+        kt_begin(ctx);
+        hipLaunchKernelGGL((k_lp_block<256, 2048>), dim3(grid_for(ctx, p1 - p0, 1, 10)), dim3(256), 0, ctx->stream,
+                           p0, p1, args_for(BIN_B1024));
+        kt_end(ctx, KS_LP_B1024);
     }
-    launch_small<64>(ctx, BIN_G64, a);
-    launch_small<32>(ctx, BIN_G32, a);
-    launch_small<16>(ctx, BIN_G16, a);
-    launch_small<8>(ctx, BIN_G8, a);
-    launch_small<4>(ctx, BIN_G4, a);
+    if (range(BIN_W256, p0, p1)) {
+        kt_begin(ctx);
+        hipLaunchKernelGGL(k_lp_wave<512>, dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256), 0, ctx->stream, p0, p1,
+                           args_for(BIN_W256));
+        kt_end(ctx, KS_LP_W256);
+    }
+#define PCD_SMALL(G, CP, BIN, SLOT)                                                                            \
+    if (range(BIN, p0, p1)) {                                                                                  \
+        kt_begin(ctx);                                                                                         \
+        hipLaunchKernelGGL((k_lp_small<G, CP>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)), dim3(256), 0,        \
+                           ctx->stream, p0, p1, args_for(BIN));                                                \
+        kt_end(ctx, SLOT);                                                                                     \
+    }
+    PCD_SMALL(64, 8, BIN_G64, KS_LP_G64)
+    PCD_SMALL(32, 16, BIN_G32, KS_LP_G32)
+    PCD_SMALL(16, 32, BIN_G16, KS_LP_G16)
+    PCD_SMALL(8, 64, BIN_G8, KS_LP_G8)
+    PCD_SMALL(4, 64, BIN_G4, KS_LP_G4)
+#undef PCD_SMALL
     PCD_HIP(hipGetLastError());
-    if (ctx->counters) {
-        // active counts of hubs are included by k_hub_decide
-        PCD_HIP(hipMemcpyAsync(ctx->h_small64, cnt64, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-        PCD_HIP(hipStreamSynchronize(ctx->stream));
-        ctx->cnt_active = ctx->h_small64[0];
-        ctx->cnt_entries = ctx->h_small64[1];
-        ctx->cnt_distinct = ctx->h_small64[2];
-    }
     std::swap(ctx->d_labels, ctx->d_labels_alt);
     ctx->labels_compressed = false;
     ctx->has_stats = false;

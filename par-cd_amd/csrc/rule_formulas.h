@@ -28,6 +28,7 @@ struct PCD_ALIGN16 CommStats {
     int32_t ext_vol;  // Partition::ext_volume
     int32_t pad;
 };
+static_assert(sizeof(CommStats) == 16, "community record is 16 bytes");
 
 struct RuleParams {
     int kind;

@@ -205,6 +205,7 @@ static int ensure_tri_buffers(pcd_ctx *ctx) {
 static int count_pass(pcd_ctx *ctx, const int32_t *labels, int32_t *t, int32_t *td) {
     const int n = ctx->n;
     const int64_t E = (int64_t)2 * ctx->m;
+    kt_begin(ctx);
     PCD_TRY(fill_i32(ctx, ctx->d_et, 0, E));
     PCD_TRY(fill_i32(ctx, t, 0, n));
     PCD_TRY(fill_i32(ctx, td, 0, n));
@@ -212,6 +213,7 @@ static int count_pass(pcd_ctx *ctx, const int32_t *labels, int32_t *t, int32_t *
                        ctx->d_adj, ctx->d_lowpos, labels, ctx->d_et);
     hipLaunchKernelGGL(k_tri_accumulate, dim3(grid_for(ctx, n, 4, 8)), dim3(256), 0, ctx->stream, n, ctx->d_indices,
                        ctx->d_adj, ctx->d_lowpos, ctx->d_et, t, td);
+    kt_end(ctx, KS_TRIANGLES);
     PCD_HIP(hipGetLastError());
     return PCD_OK;
 }

@@ -57,6 +57,19 @@ extern "C" const char *pcd_last_error(void) { return g_last_error.c_str(); }
 
 extern "C" void pcd_free(void *p) { free(p); }
 
+// framework.cpp:14-24
+extern "C" int pcd_singleton_partition(int n, int32_t *labels) {
+    if (n <= 0 || !labels) return fail(PCD_E_INVALID, "bad arguments");
+    for (int v = 0; v < n; v++) labels[v] = v;
+    for (int v = 0; v < n; v++) {
+        int j = rand() % n;
+        int32_t t = labels[v];
+        labels[v] = labels[j];
+        labels[j] = t;
+    }
+    return PCD_OK;
+}
+
 template <typename T>
 static T *xmalloc(size_t count) {
     return (T *)malloc(sizeof(T) * (count ? count : 1));

@@ -24,6 +24,9 @@ EXPORTS = [
     "pcd_contract", "pcd_count_triangles", "pcd_set_clustering_coef", "pcd_enable_timing", "pcd_phase_time",
     "pcd_enable_counters", "pcd_propagation_counters", "pcd_read_pcd", "pcd_write_pcd", "pcd_read_edge_text",
     "pcd_convert_edges", "pcd_free", "pcd_gen_sbm", "pcd_gen_rmat", "pcd_gen_chung_lu",
+    "pcd_kernel_slot_count", "pcd_kernel_slot_name", "pcd_kernel_timer_reset", "pcd_kernel_timer",
+    "pcd_bin_count", "pcd_bin_name", "pcd_bin_counters_reset", "pcd_bin_counters", "pcd_bin_size",
+    "pcd_singleton_partition", "pcd_set_tuning",
 ]
 
 
@@ -85,6 +88,17 @@ def lib():
     L.pcd_gen_chung_lu.argtypes = [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_double, C.POINTER(C.c_int64), C.POINTER(I32P), C.POINTER(I32P),
                                    C.POINTER(C.c_int)]
+    L.pcd_kernel_slot_name.restype = C.c_char_p
+    L.pcd_kernel_slot_name.argtypes = [C.c_int]
+    L.pcd_kernel_timer_reset.argtypes = [vp]
+    L.pcd_kernel_timer.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.pcd_bin_name.restype = C.c_char_p
+    L.pcd_bin_name.argtypes = [C.c_int]
+    L.pcd_bin_counters_reset.argtypes = [vp]
+    L.pcd_bin_counters.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.pcd_bin_size.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.pcd_singleton_partition.argtypes = [C.c_int, vp]
+    L.pcd_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
     _lib = L
     return L
 
@@ -183,6 +197,23 @@ def make_graph(kind, seed, **kw):
     else:
         raise ValueError(kind)
     return convert_edges(nn, a, b)
+
+
+def singleton_partition(n):
+    """framework.cpp:14-24 with the process's glibc rand() state."""
+    out = np.zeros(n, np.int32)
+    check(lib().pcd_singleton_partition(n, _p(out)))
+    return out
+
+
+def kernel_slots():
+    L = lib()
+    return [L.pcd_kernel_slot_name(i).decode() for i in range(L.pcd_kernel_slot_count())]
+
+
+def bin_names():
+    L = lib()
+    return [L.pcd_bin_name(i).decode() for i in range(L.pcd_bin_count())]
 
 
 # ------------------------------------------------------------ device context
@@ -286,3 +317,39 @@ class Context:
 
     def synchronize(self):
         check(lib().pcd_synchronize(self._ctx))
+
+    def set_tuning(self, key, value):
+        check(lib().pcd_set_tuning(self._ctx, key.encode(), int(value)))
+
+    def kernel_timer_reset(self):
+        check(lib().pcd_kernel_timer_reset(self._ctx))
+
+    def kernel_timers(self):
+        """{slot name: (total ms, launches)} since the last reset."""
+        out = {}
+        for i, name in enumerate(kernel_slots()):
+            t, c = C.c_double(), C.c_int64()
+            check(lib().pcd_kernel_timer(self._ctx, i, C.byref(t), C.byref(c)))
+            if c.value:
+                out[name] = (t.value, c.value)
+        return out
+
+    def bin_counters_reset(self):
+        check(lib().pcd_bin_counters_reset(self._ctx))
+
+    def bin_counters(self):
+        """{bin name: (active, entries, distinct)} since the last reset."""
+        out = {}
+        for i, name in enumerate(bin_names()):
+            a, e, d = C.c_int64(), C.c_int64(), C.c_int64()
+            check(lib().pcd_bin_counters(self._ctx, i, C.byref(a), C.byref(e), C.byref(d)))
+            out[name] = (a.value, e.value, d.value)
+        return out
+
+    def bin_sizes(self):
+        out = {}
+        for i, name in enumerate(bin_names()):
+            v, e = C.c_int64(), C.c_int64()
+            check(lib().pcd_bin_size(self._ctx, i, C.byref(v), C.byref(e)))
+            out[name] = (v.value, e.value)
+        return out

@@ -149,6 +149,25 @@ def test_propagation_bin_zoo(ctx, kind, lam, tie):
     _lp_compare(ctx, gu, rng.integers(0, 500, n), kind, lam, 0.7, seed=5, step=2, tie=tie)
 
 
+@pytest.mark.parametrize("slots,part_entries", [(16384, 4096), (64, 4096), (1024, 512), (64, 100000)])
+def test_propagation_hub_paths(ctx, slots, part_entries):
+    """Hub rows: partitioned LDS counting, and the global-table fallback when a
+    partition overflows its table (forced by tiny tables)."""
+    g = GG.bin_zoo()
+    n = g[0]
+    rng = np.random.default_rng(6)
+    try:
+        ctx.set_tuning("hub_part_entries", part_entries)
+        ctx.set_tuning("hub_table_slots", slots)
+        for lab in [GG.singleton_labels(n), rng.integers(0, 40, n), rng.integers(0, 3000, n)]:
+            for kind, lam in [(O.MOD, 0.0), (O.CPM, 0.02)]:
+                _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=4, step=3, tie=O.TIE_RANDOM)
+                _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=4, step=4, tie=O.TIE_LOWEST)
+    finally:
+        ctx.set_tuning("hub_part_entries", 4096)
+        ctx.set_tuning("hub_table_slots", 16384)
+
+
 def test_propagation_wcc(ctx):
     for name in ["golden_sbm200", "golden_sbm210", "golden_rmat9"]:
         g = GOLDEN[name]
