@@ -79,14 +79,14 @@ def run_iterations(ctx, parcd, kind, lam, init, steps, warmup, seed, timers=True
 
 
 def lp_bin_bytes(active, entries, distinct, S):
-    """Algorithmic bytes of one propagation bin kernel: per active vertex the
-    list entry, two offsets, its own label, its community record and the new
-    label; per entry the neighbour id and its label; per distinct neighbour
-    label its community record (S bytes: what the rule reads)."""
-    return active * (4 + 8 + 4 + S + 4) + 8 * entries + S * distinct
+    """Algorithmic bytes of one propagation bin kernel: per active vertex two
+    offsets, its own label, its community record and the new label; per
+    entry the neighbour id and its label; per distinct neighbour label its
+    community record (S bytes: what the rule reads)."""
+    return active * (8 + 4 + S + 4) + 8 * entries + S * distinct
 
 
-SLOT_OF_BIN = {"hub": "lp_hub", "b8192": "lp_b8192", "b4096": "lp_b4096", "b1024": "lp_b1024", "w256": "lp_w256",
+SLOT_OF_BIN = {"hub": "lp_hub", "b4096": "lp_b4096", "b1024": "lp_b1024", "w256": "lp_w256",
                "g64": "lp_g64", "g32": "lp_g32", "g16": "lp_g16", "g8": "lp_g8", "g4": "lp_g4"}
 
 
@@ -161,6 +161,7 @@ def main():
     secs, scores = run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup, args.seed)
     timers = ctx.kernel_timers()
     labels_timed = ctx.labels()
+    log("hub partitions overflowed:", ctx.hub_overflows())
     ms_per_step = secs * 1000.0 / args.steps
     gteps = E / (secs / args.steps) / 1e9
     log("mod: %.3f ms/iteration, %.2f GTEPS, Q %s" % (ms_per_step, gteps, ["%.6f" % s for s in scores]))

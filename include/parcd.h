@@ -158,11 +158,15 @@ const char *pcd_bin_name(int bin);
 int pcd_bin_counters_reset(pcd_ctx *ctx);
 int pcd_bin_counters(pcd_ctx *ctx, int bin, int64_t *active, int64_t *entries, int64_t *distinct);
 /* Tunables of the kernel schedule: "hub_table_slots" (LDS slots of a hub
- * partition table, power of two in [64, 16384]; takes effect at once),
+ * partition table, multiple of 64 in [64, 4096]; takes effect at once),
  * "hub_part_entries" (target entries per hub label partition) and
  * "stat_chunk" (entries per statistics work item), which take effect at the
  * next pcd_graph_upload.  Results do not depend on them. */
 int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value);
+/* Hub label partitions whose distinct labels overflowed their LDS table
+ * since the last call (those hubs were recounted in global memory); the
+ * counter is reset by the call. */
+int pcd_hub_overflows(pcd_ctx *ctx, int64_t *count);
 /* Vertices per degree bin of the uploaded graph (bin layout). */
 int pcd_bin_size(pcd_ctx *ctx, int bin, int64_t *vertices, int64_t *entries);
 

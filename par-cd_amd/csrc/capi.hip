@@ -73,11 +73,11 @@ void kt_end(pcd_ctx *ctx, int slot) {
 }
 
 static const char *kslot_names[NUM_KSLOTS] = {
-    "compress", "stats_items", "stats_small", "metric", "lp_select", "lp_hub_partition", "lp_hub_count",
-    "lp_b8192", "lp_b4096", "lp_b1024", "lp_w256", "lp_g64", "lp_g32", "lp_g16", "lp_g8", "lp_g4",
+    "compress", "stats_items", "stats_small", "metric", "lp_records", "lp_hub_partition", "lp_hub_count",
+    "lp_b4096", "lp_b1024", "lp_w256", "lp_g64", "lp_g32", "lp_g16", "lp_g8", "lp_g4",
     "merge_keys", "merge_sort", "merge_rle", "merge_rounds", "triangles"};
 
-static const char *bin_names[NUM_BINS] = {"hub", "b8192", "b4096", "b1024", "w256", "g64", "g32", "g16", "g8", "g4"};
+static const char *bin_names[NUM_BINS] = {"hub", "b4096", "b1024", "w256", "g64", "g32", "g16", "g8", "g4"};
 
 }  // namespace pcd
 
@@ -149,6 +149,7 @@ extern "C" int pcd_ctx_create(int device, pcd_ctx **out) {
     if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_dacc, 4096);
     if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_result, 8);
     if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_counts, NUM_BINS + 16);
+    if (rc == PCD_OK) rc = fill_i32(ctx, ctx->d_counts, 0, NUM_BINS + 16);
     if (rc == PCD_OK) rc = dev_alloc_t(ctx, &ctx->d_bin_counters, 3 * NUM_BINS);
     if (rc == PCD_OK) rc = fill_u64(ctx, (uint64_t *)ctx->d_bin_counters, 0, 3 * NUM_BINS);
     if (rc != PCD_OK) {
@@ -181,6 +182,7 @@ static void free_graph_state(pcd_ctx *ctx) {
     ctx->d_hub_table_log = ctx->d_hub_keys = ctx->d_hub_cnts = nullptr;
     ctx->d_stat_items = nullptr;
     ctx->d_labels = ctx->d_labels_alt = nullptr;
+    ctx->d_lv = nullptr;
     ctx->d_stats = nullptr;
     ctx->d_first = ctx->d_newid = ctx->d_active = ctx->d_hub_active = nullptr;
     ctx->d_scan_part = nullptr;
@@ -363,7 +365,7 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
         PCD_TRY(up(&ctx->d_hub_table_off, toff));
         PCD_TRY(up(&ctx->d_hub_table_log, tlog));
         PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_poff, poff_total));
-        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_buf, buf_base[hub_end]));
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_buf, 2 * buf_base[hub_end]));
         PCD_TRY(dev_alloc(ctx, &ctx->d_hub_partial, (size_t)HUB_PARTIAL_BYTES * hub_parts.size()));
         PCD_TRY(dev_alloc_t(ctx, &ctx->d_hub_over, hub_end));
         PCD_TRY(fill_i32(ctx, ctx->d_hub_over, 0, hub_end));
@@ -379,6 +381,7 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
     }
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_labels, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_labels_alt, n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_lv, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_stats, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_active, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_first, n + 1));
@@ -610,7 +613,7 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
     if (!key) return fail(PCD_E_INVALID, "null key");
     std::string k(key);
     if (k == "hub_table_slots") {
-        if (value < 64 || value > 16384 || (value & (value - 1))) return fail(PCD_E_INVALID, "hub_table_slots: power of two in [64, 16384]");
+        if (value < 64 || value > HUB_TB || (value % 64)) return fail(PCD_E_INVALID, "hub_table_slots: multiple of 64 in [64, 4096]");
         ctx->tune.hub_table_slots = value;
     } else if (k == "hub_part_entries") {
         if (value < 64) return fail(PCD_E_INVALID, "hub_part_entries must be >= 64");
@@ -674,6 +677,17 @@ extern "C" int pcd_kernel_timer(pcd_ctx *ctx, int slot, double *total_ms, int64_
     }
     if (total_ms) *total_ms = t;
     if (launches) *launches = (int64_t)ctx->ktimes[slot].size();
+    return PCD_OK;
+}
+
+extern "C" int pcd_hub_overflows(pcd_ctx *ctx, int64_t *count) {
+    PCD_TRY(check_ctx(ctx));
+    int32_t h = 0;
+    PCD_HIP(hipMemcpyAsync(&h, ctx->d_counts + NUM_BINS + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    if (count) *count = h;
+    PCD_TRY(fill_i32(ctx, ctx->d_counts + NUM_BINS + 2, 0, 1));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
     return PCD_OK;
 }
 

@@ -34,9 +34,9 @@ constexpr int EMPTY_KEY = -1;
 // Degree bins over vertex positions (positions are vertex ids when the
 // graph is degree-sorted descending, as `convert` writes it, else the
 // positions of a degree-descending permutation `order`).
-enum Bin { BIN_HUB = 0, BIN_B8192, BIN_B4096, BIN_B1024, BIN_W256, BIN_G64, BIN_G32, BIN_G16, BIN_G8, BIN_G4, NUM_BINS };
+enum Bin { BIN_HUB = 0, BIN_B4096, BIN_B1024, BIN_W256, BIN_G64, BIN_G32, BIN_G16, BIN_G8, BIN_G4, NUM_BINS };
 // degree upper bound of each bin (HUB: unbounded)
-constexpr int BIN_MAX_DEG[NUM_BINS] = {0x7fffffff, 8192, 4096, 1024, 256, 64, 32, 16, 8, 4};
+constexpr int BIN_MAX_DEG[NUM_BINS] = {0x7fffffff, 4096, 1024, 256, 64, 32, 16, 8, 4};
 
 struct BinLayout {
     int start[NUM_BINS + 1];  // position ranges, bin b = [start[b], start[b+1])
@@ -45,18 +45,21 @@ struct BinLayout {
 // Tunables (the autotuner rewrites these per graph shape).
 struct Tuning {
     int hub_chunk = 4096;        // entries per hub partition work item
-    int hub_part_entries = 4096; // target entries per hub label partition
-    int hub_table_slots = 16384; // LDS table slots of a hub partition (<= 16384)
+    int hub_part_entries = 2048; // target entries per hub label partition
+    int hub_table_slots = 4096;  // LDS table slots of a hub partition (multiple of 64, <= HUB_TB)
     int stat_chunk = 2048;       // entries per statistics work item for deg > 64
 };
 constexpr int HUB_MAX_PARTS = 1024;
+constexpr int HUB_TB = 4096;          // LDS table capacity of a hub partition
+static_assert(Tuning{}.hub_table_slots <= HUB_TB && Tuning{}.hub_table_slots % 64 == 0,
+              "default hub table must fit the LDS table");
 constexpr int HUB_CHUNK = 4096;      // entries per hub chunk item (pass 1 holds 512 x 8)
 constexpr int HUB_PARTIAL_BYTES = 32; // storage per (hub, partition) partial result
 
 // Kernel timer slots (HIP events around each launch when timing is on).
 enum KSlot {
     KS_COMPRESS = 0, KS_STATS_ITEMS, KS_STATS_SMALL, KS_METRIC,
-    KS_LP_SELECT, KS_LP_HUB_INSERT, KS_LP_HUB_DECIDE, KS_LP_B8192, KS_LP_B4096, KS_LP_B1024, KS_LP_W256,
+    KS_LP_SELECT, KS_LP_HUB_INSERT, KS_LP_HUB_DECIDE, KS_LP_B4096, KS_LP_B1024, KS_LP_W256,
     KS_LP_G64, KS_LP_G32, KS_LP_G16, KS_LP_G8, KS_LP_G4,
     KS_MERGE_KEYS, KS_MERGE_SORT, KS_MERGE_RLE, KS_MERGE_ROUNDS, KS_TRIANGLES,
     NUM_KSLOTS
@@ -94,7 +97,7 @@ struct pcd_ctx {
     int4 *d_hub_part_items = nullptr;    // {hub, part, first chunk, number of chunks}
     int4 *d_hub_info = nullptr;          // per hub: {v, P, first part item, first chunk}
     int32_t *d_hub_poff = nullptr;       // partition offsets of all chunks
-    int32_t *d_hub_buf = nullptr;        // partitioned labels (sum of hub degrees)
+    int32_t *d_hub_buf = nullptr;        // partitioned {label, aux} records (sum of hub degrees)
     int64_t *d_hub_buf_base = nullptr;   // per hub: offset into d_hub_buf
     void *d_hub_partial = nullptr;       // per (hub, part) partial argmax
     int32_t *d_hub_over = nullptr;       // per hub: partition overflow flag
@@ -114,6 +117,7 @@ struct pcd_ctx {
     int label_bound = 0;  // labels in [-1, label_bound)
     int k = 0;
     int32_t *d_labels = nullptr, *d_labels_alt = nullptr;
+    int2 *d_lv = nullptr;  // per vertex {label, aux} for the propagation gathers
     pcd::CommStats *d_stats = nullptr;
     int32_t *d_first = nullptr, *d_newid = nullptr;
     int32_t *d_active = nullptr;   // per-bin active lists (bin ranges)

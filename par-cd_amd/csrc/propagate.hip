@@ -4,27 +4,35 @@
 // Per active vertex v the step needs, for every distinct neighbour label c,
 // the count of neighbours in c, then score_join(v, c, count) for c != own,
 // the best of those (score, then tie key, then label), score_leave(v, own
-// count) and the move decision.  Vertex selection (counter hash against
-// frac) is evaluated inline by every kernel; an inactive vertex writes its
-// own label.  Vertices are processed by degree bin (positions are degree-
-// descending, so every bin is a contiguous position range):
+// count) and the move decision.
 //
-//   G4..G64   deg <= G: each wave compacts the active vertices of 64
+// Neighbour gathers read a per-vertex 8-byte record {label, aux} built at
+// the start of the step, where aux is the one community total the rule's
+// gains read (modularity: int_volume + ext_volume; CPM: size).  One random
+// gather per neighbour entry thus brings both the label and what its score
+// needs.  Rules that read more (WCC, conductance) gather the community
+// record per distinct label instead.
+//
+// Vertex selection (counter hash against frac) is evaluated inline; an
+// inactive vertex writes its own label.  Vertices are processed by degree
+// bin (positions are degree-descending, so every bin is a contiguous range):
+//
+//   G4..G64   deg <= G: each wave compacts the active vertices of CP
 //             consecutive positions (ballot), then G lanes per vertex load
-//             one neighbour label each, bitonic-sort them by shuffles, find
-//             run heads by ballot and reduce the best candidate by butterfly.
-//   W256      one wave per vertex, LDS hash table of nextpow2(2 deg) slots.
-//   B1024     one 256-thread workgroup per vertex, 2048-slot LDS table.
-//   B4096     one 1024-thread workgroup per vertex, 8192-slot LDS table.
-//   B8192     one 1024-thread workgroup per vertex, 16384-slot LDS table.
-//   HUB       deg > 8192.  Pass 1: every 4096-entry chunk of a hub row
-//             gathers its labels and writes them grouped by label-hash
+//             one neighbour record each, bitonic-sort them by label across
+//             the lanes (shuffles), find run heads by ballot and reduce the
+//             best candidate by butterfly.  No LDS table.
+//   W256      one wave per vertex, LDS hash table of 2 deg slots.
+//   B1024     one 256-thread workgroup per vertex, 2 deg slots.
+//   B4096     one 512-thread workgroup per vertex, 1.5 deg slots.
+//   HUB       deg > 4096.  Pass 1: each 4096-entry chunk of a hub row
+//             gathers its records and writes them grouped by label-hash
 //             partition (LDS counting sort).  Pass 2: one workgroup per
-//             (hub, partition) counts the partition's labels in a 16384-slot
-//             LDS table and reduces its best candidate.  Pass 3: one wave
-//             per hub combines the partitions and decides.  A partition
-//             whose distinct labels do not fit the table flags its hub, and
-//             the hub is then recounted in a global-memory hash table.
+//             (hub, partition) counts the partition in an LDS table and
+//             reduces its best candidate.  Pass 3: one wave per hub combines
+//             the partitions and decides.  A partition whose distinct labels
+//             overflow its table flags the hub, which is then recounted in a
+//             global-memory hash table (never full).
 #include "prims.h"
 
 namespace pcd {
@@ -58,25 +66,35 @@ struct HubPartial {
     int own_cnt;
     int distinct;
 };
+static_assert(sizeof(HubPartial) <= HUB_PARTIAL_BYTES, "hub partial record size");
 
 struct LPArgs {
     const int32_t *indices, *adj, *labels, *order;
+    const int2 *lv;  // per vertex {label, aux}
     const CommStats *stats;
     int32_t *new_labels;
     RuleParams rule;
     uint64_t seed;
     int step;
     int tie;
-    int f;                         // selection threshold (refinement.cpp:158)
+    int f;        // selection threshold (refinement.cpp:158)
+    int use_aux;  // the rule's gains read only aux
     unsigned long long *counters;  // {active, entries, distinct} of the bin, or nullptr
 };
 
 __device__ inline int vertex_at(const LPArgs &a, int p) { return a.order ? a.order[p] : p; }
 
+// Community record as the gains see it.  With aux: modularity reads
+// int_volume + ext_volume (= aux + 0), CPM reads size (= aux).
+__device__ inline CommStats comm_of(const LPArgs &a, int label, int aux) {
+    if (a.use_aux) return CommStats{aux, aux, 0, 0};
+    return a.stats[label];
+}
+
 // refinement.cpp:211-221: best_score (0 if no join candidate beat 0) plus
 // score_leave decides between moving, leaving to a new singleton, staying.
-__device__ inline int lp_decide(const LPArgs &a, int deg, int own, int own_cnt, const Cand &best) {
-    double leave = score_leave(a.rule, deg, a.stats[own], own_cnt);
+__device__ inline int lp_decide(const LPArgs &a, int deg, int own, int own_aux, int own_cnt, const Cand &best) {
+    double leave = score_leave(a.rule, deg, comm_of(a, own, own_aux), own_cnt);
     double bs = best.label != INT32_MAX ? best.s : 0.0;
     bs += leave;
     if (bs > 0 && bs > leave) return best.label;
@@ -84,9 +102,8 @@ __device__ inline int lp_decide(const LPArgs &a, int deg, int own, int own_cnt, 
     return own;
 }
 
-// A candidate for neighbour label c with count cnt (c != own).
-__device__ inline Cand lp_candidate(const LPArgs &a, int deg, uint64_t tb, int c, int cnt) {
-    double s = score_join(a.rule, deg, a.stats[c], cnt);
+__device__ inline Cand lp_candidate(const LPArgs &a, int deg, uint64_t tb, int c, int aux, int cnt) {
+    double s = score_join(a.rule, deg, comm_of(a, c, aux), cnt);
     if (s > 0) return Cand{s, tie_key(a.tie, tb, c), c};
     return cand_none();
 }
@@ -95,6 +112,22 @@ __device__ inline void count3(const LPArgs &a, long long act, long long ent, lon
     atomicAdd(&a.counters[0], (unsigned long long)act);
     atomicAdd(&a.counters[1], (unsigned long long)ent);
     atomicAdd(&a.counters[2], (unsigned long long)dist);
+}
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ------------------------------------------------------------- records
+__global__ void k_lp_records(int n, const int32_t *labels, const CommStats *st, int kind, int2 *lv) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        int l = labels[v];
+        CommStats c = st[l];
+        int aux = kind == PCD_RULE_MODULARITY ? c.int_vol + c.ext_vol : c.size;
+        lv[v] = make_int2(l, aux);
+    }
 }
 
 // ------------------------------------------------------------- G lanes / vertex
@@ -117,28 +150,31 @@ __global__ __launch_bounds__(256) void k_lp_small(int p0, int p1, LPArgs a) {
         uint64_t am = __ballot(act);
         int nact = __popcll(am);
         if (act) list[__popcll(am & lt)] = v;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync();
         for (int gb = 0; gb < nact; gb += per_wave) {
             int gi = gb + lane / G;
             bool has = gi < nact;
-            int u = 0, start = 0, deg = 0, own = 0;
+            int u = 0, start = 0, deg = 0;
+            int2 own = make_int2(0, 0);
             if (has) {
                 u = list[gi];
                 start = a.indices[u];
                 deg = a.indices[u + 1] - start;
-                own = a.labels[u];
+                own = a.lv[u];
             }
-            int x = (has && glane < deg) ? a.labels[a.adj[start + glane]] : INT32_MAX;
+            int2 r = (has && glane < deg) ? a.lv[a.adj[start + glane]] : make_int2(INT32_MAX, 0);
+            int x = r.x, xa = r.y;
 #pragma unroll
             for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
                 for (int j = k >> 1; j > 0; j >>= 1) {
                     int y = __shfl_xor(x, j, WAVE);
+                    int ya = __shfl_xor(xa, j, WAVE);
                     bool up = (glane & k) == 0;
                     bool lower = (glane & j) == 0;
-                    x = (lower == up) ? min(x, y) : max(x, y);
+                    bool take_min = lower == up;
+                    bool other = take_min ? (y < x) : (y > x);
+                    if (other) { x = y; xa = ya; }
                 }
             }
             int prev = __shfl(x, lane == 0 ? 0 : lane - 1, WAVE);
@@ -151,8 +187,8 @@ __global__ __launch_bounds__(256) void k_lp_small(int p0, int p1, LPArgs a) {
                 int nxt = above ? lane + 1 + __builtin_ctzll(above) : WAVE;
                 int gend = (lane & ~(G - 1)) + deg;
                 int cnt = min(nxt, gend) - lane;
-                if (x == own) own_cnt = cnt;
-                else c = lp_candidate(a, deg, a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, u), x, cnt);
+                if (x == own.x) own_cnt = cnt;
+                else c = lp_candidate(a, deg, a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, u), x, xa, cnt);
             }
 #pragma unroll
             for (int d = G / 2; d > 0; d >>= 1) {
@@ -160,60 +196,51 @@ __global__ __launch_bounds__(256) void k_lp_small(int p0, int p1, LPArgs a) {
                 if (cand_better(o, c)) c = o;
                 own_cnt = max(own_cnt, __shfl_xor(own_cnt, d, WAVE));
             }
-            if (has && glane == 0) a.new_labels[u] = lp_decide(a, deg, own, own_cnt, c);
+            if (has && glane == 0) a.new_labels[u] = lp_decide(a, deg, own.x, own.y, own_cnt, c);
             if (a.counters) {
                 int ent = wave_sum((has && glane < deg) ? 1 : 0);
                 if (lane == 0) count3(a, __popcll(__ballot(has && glane == 0)), ent, __popcll(heads));
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync();
     }
 }
 
-// ------------------------------------------------------------- LDS hash helpers
-__device__ inline uint32_t hslot(int key, int log_t) {
-    return ((uint32_t)key * 0x9E3779B1u) >> (32 - log_t);
+// ------------------------------------------------------------- LDS hash table
+// T slots (any multiple of 64), linear probing from (hash * T) >> 32.
+struct Table {
+    int *keys, *cnts, *auxs;
+    int T;
+};
+
+__device__ inline int tslot(int key, int T) {
+    return (int)(((uint64_t)((uint32_t)key * 0x9E3779B1u) * (uint32_t)T) >> 32);
 }
 
-// Inserts key with count add.  Probes at most T slots; returns false when
-// the table is full (the key was not inserted).
-__device__ inline bool lds_insert(int *keys, int *cnts, int mask, int log_t, int key, int add) {
-    uint32_t h = hslot(key, log_t);
-    for (int probe = 0; probe <= mask; probe++) {
-        int old = atomicCAS(&keys[h], EMPTY_KEY, key);
+// Inserts (key, aux) with count add; probes at most T slots and returns
+// false when the table is full.
+__device__ inline bool tinsert(const Table &t, int key, int aux, int add) {
+    int h = tslot(key, t.T);
+    for (int probe = 0; probe < t.T; probe++) {
+        int old = atomicCAS(&t.keys[h], EMPTY_KEY, key);
+        if (old == EMPTY_KEY) t.auxs[h] = aux;
         if (old == EMPTY_KEY || old == key) {
-            atomicAdd(&cnts[h], add);
+            atomicAdd(&t.cnts[h], add);
             return true;
         }
-        h = (h + 1) & mask;
+        if (++h == t.T) h = 0;
     }
     return false;
 }
 
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+__device__ inline void tclear(const Table &t, int lo, int stride) {
+    for (int s = lo; s < t.T; s += stride) { t.keys[s] = EMPTY_KEY; t.cnts[s] = 0; }
 }
 
-__device__ inline int ceil_log2(int x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
-
-__device__ inline void wave_reduce_cand(Cand &c, int &own_cnt) {
-#pragma unroll
-    for (int d = WAVE / 2; d > 0; d >>= 1) {
-        Cand o = cand_shfl_xor(c, d);
-        if (cand_better(o, c)) c = o;
-        own_cnt = max(own_cnt, __shfl_xor(own_cnt, d, WAVE));
-    }
-}
-
-// Gathers the labels of row entries [start + lo, start + deg) with stride
+// Gathers the records of row entries [start + lo, start + deg) with stride
 // `stride`, PF at a time (independent loads in flight), and inserts them.
 template <int PF>
-__device__ inline void insert_row(const LPArgs &a, int *keys, int *cnts, int mask, int log_t, int start, int deg,
-                                  int lo, int stride) {
+__device__ inline void insert_row(const LPArgs &a, const Table &t, int start, int deg, int lo, int stride) {
     for (int j0 = lo; j0 < deg; j0 += PF * stride) {
         int nb[PF];
 #pragma unroll
@@ -221,33 +248,41 @@ __device__ inline void insert_row(const LPArgs &a, int *keys, int *cnts, int mas
             int j = j0 + e * stride;
             nb[e] = j < deg ? a.adj[start + j] : -1;
         }
-        int lab[PF];
+        int2 r[PF];
 #pragma unroll
-        for (int e = 0; e < PF; e++) lab[e] = nb[e] >= 0 ? a.labels[nb[e]] : EMPTY_KEY;
+        for (int e = 0; e < PF; e++) r[e] = nb[e] >= 0 ? a.lv[nb[e]] : make_int2(EMPTY_KEY, 0);
 #pragma unroll
         for (int e = 0; e < PF; e++)
-            if (lab[e] != EMPTY_KEY) lds_insert(keys, cnts, mask, log_t, lab[e], 1);
+            if (r[e].x != EMPTY_KEY) tinsert(t, r[e].x, r[e].y, 1);
     }
 }
 
-// Scans table slots [lo, T) with stride `stride` for candidates.  Slots are
-// taken SB at a time so that their community-record gathers are in flight
-// together.
+// Scans table slots [lo, T) with stride `stride` for candidates, SB slots at
+// a time so that community-record gathers (rules without aux) overlap.
 constexpr int SB = 4;
-__device__ inline void scan_table(const LPArgs &a, const int *keys, const int *cnts, int T, int lo, int stride,
-                                  int deg, int own, uint64_t tb, Cand &c, int &own_cnt, int &distinct) {
-    for (int s0 = lo; s0 < T; s0 += SB * stride) {
+__device__ inline void scan_table(const LPArgs &a, const Table &t, int lo, int stride, int deg, int own, uint64_t tb,
+                                  Cand &c, int &own_cnt, int &distinct) {
+    for (int s0 = lo; s0 < t.T; s0 += SB * stride) {
         int key[SB], cnt[SB];
         CommStats st[SB];
 #pragma unroll
         for (int e = 0; e < SB; e++) {
             int s = s0 + e * stride;
-            key[e] = s < T ? keys[s] : EMPTY_KEY;
-            cnt[e] = key[e] != EMPTY_KEY ? cnts[s] : 0;
+            key[e] = s < t.T ? t.keys[s] : EMPTY_KEY;
+            cnt[e] = 0;
+            if (key[e] != EMPTY_KEY) {
+                cnt[e] = t.cnts[s];
+                if (a.use_aux) {
+                    int x = t.auxs[s];
+                    st[e] = CommStats{x, x, 0, 0};
+                }
+            }
         }
+        if (!a.use_aux) {
 #pragma unroll
-        for (int e = 0; e < SB; e++)
-            if (key[e] != EMPTY_KEY && key[e] != own) st[e] = a.stats[key[e]];
+            for (int e = 0; e < SB; e++)
+                if (key[e] != EMPTY_KEY && key[e] != own) st[e] = a.stats[key[e]];
+        }
 #pragma unroll
         for (int e = 0; e < SB; e++) {
             if (key[e] == EMPTY_KEY) continue;
@@ -265,43 +300,15 @@ __device__ inline void scan_table(const LPArgs &a, const int *keys, const int *c
     }
 }
 
-// ------------------------------------------------------------- wave / vertex
-template <int TW>
-__global__ __launch_bounds__(256) void k_lp_wave(int p0, int p1, LPArgs a) {
-    __shared__ int s_keys[256 / WAVE][TW];
-    __shared__ int s_cnts[256 / WAVE][TW];
-    const int lane = __lane_id(), w = threadIdx.x / WAVE;
-    int *keys = s_keys[w], *cnts = s_cnts[w];
-    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    const int nwaves = gridDim.x * blockDim.x / WAVE;
-    for (int p = p0 + wave_g; p < p1; p += nwaves) {
-        int v = vertex_at(a, p);
-        if (!selected(a.seed, a.step, v, a.f)) {
-            if (lane == 0) a.new_labels[v] = a.labels[v];
-            continue;
-        }
-        int start = a.indices[v], deg = a.indices[v + 1] - start, own = a.labels[v];
-        int log_t = max(6, ceil_log2(2 * deg));
-        int T = 1 << log_t, mask = T - 1;
-        for (int s = lane; s < T; s += WAVE) { keys[s] = EMPTY_KEY; cnts[s] = 0; }
-        wave_sync();
-        insert_row<4>(a, keys, cnts, mask, log_t, start, deg, lane, WAVE);
-        wave_sync();
-        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
-        Cand c = cand_none();
-        int own_cnt = 0, distinct = 0;
-        scan_table(a, keys, cnts, T, lane, WAVE, deg, own, tb, c, own_cnt, distinct);
-        wave_reduce_cand(c, own_cnt);
-        if (lane == 0) a.new_labels[v] = lp_decide(a, deg, own, own_cnt, c);
-        if (a.counters) {
-            distinct = wave_sum(distinct);
-            if (lane == 0) count3(a, 1, deg, distinct);
-        }
-        wave_sync();
+__device__ inline void wave_reduce_cand(Cand &c, int &own_cnt) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        Cand o = cand_shfl_xor(c, d);
+        if (cand_better(o, c)) c = o;
+        own_cnt = max(own_cnt, __shfl_xor(own_cnt, d, WAVE));
     }
 }
 
-// ------------------------------------------------------------- block reduce
 template <int NT>
 __device__ inline void block_reduce_cand(Cand &c, int &own_cnt, int &distinct, Cand *s_c, int *s_i) {
     const int lane = __lane_id(), w = threadIdx.x / WAVE;
@@ -319,11 +326,51 @@ __device__ inline void block_reduce_cand(Cand &c, int &own_cnt, int &distinct, C
     __syncthreads();
 }
 
+__device__ inline int round64(int x) { return (x + 63) & ~63; }
+
+// ------------------------------------------------------------- wave / vertex
+template <int TW>
+__global__ __launch_bounds__(256) void k_lp_wave(int p0, int p1, LPArgs a) {
+    __shared__ int s_keys[256 / WAVE][TW];
+    __shared__ int s_cnts[256 / WAVE][TW];
+    __shared__ int s_auxs[256 / WAVE][TW];
+    const int lane = __lane_id(), w = threadIdx.x / WAVE;
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    for (int p = p0 + wave_g; p < p1; p += nwaves) {
+        int v = vertex_at(a, p);
+        if (!selected(a.seed, a.step, v, a.f)) {
+            if (lane == 0) a.new_labels[v] = a.labels[v];
+            continue;
+        }
+        int start = a.indices[v], deg = a.indices[v + 1] - start;
+        int2 own = a.lv[v];
+        Table t{s_keys[w], s_cnts[w], s_auxs[w], min(TW, max(64, round64(2 * deg)))};
+        tclear(t, lane, WAVE);
+        wave_sync();
+        insert_row<4>(a, t, start, deg, lane, WAVE);
+        wave_sync();
+        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
+        Cand c = cand_none();
+        int own_cnt = 0, distinct = 0;
+        scan_table(a, t, lane, WAVE, deg, own.x, tb, c, own_cnt, distinct);
+        wave_reduce_cand(c, own_cnt);
+        if (lane == 0) a.new_labels[v] = lp_decide(a, deg, own.x, own.y, own_cnt, c);
+        if (a.counters) {
+            distinct = wave_sum(distinct);
+            if (lane == 0) count3(a, 1, deg, distinct);
+        }
+        wave_sync();
+    }
+}
+
 // ------------------------------------------------------------- block / vertex
-template <int NT, int TB>
+// Table slots: round64(LOAD_NUM * deg / 2), at most TB.
+template <int NT, int TB, int LOAD_NUM>
 __global__ __launch_bounds__(NT) void k_lp_block(int p0, int p1, LPArgs a) {
     __shared__ int keys[TB];
     __shared__ int cnts[TB];
+    __shared__ int auxs[TB];
     __shared__ Cand s_c[NT / WAVE];
     __shared__ int s_i[2 * NT / WAVE];
     for (int p = p0 + blockIdx.x; p < p1; p += gridDim.x) {
@@ -332,21 +379,20 @@ __global__ __launch_bounds__(NT) void k_lp_block(int p0, int p1, LPArgs a) {
             if (threadIdx.x == 0) a.new_labels[v] = a.labels[v];
             continue;
         }
-        int start = a.indices[v], deg = a.indices[v + 1] - start, own = a.labels[v];
-        int log_t = max(6, ceil_log2(2 * deg));
-        int T = 1 << log_t, mask = T - 1;
-        for (int s = threadIdx.x; s < T; s += NT) { keys[s] = EMPTY_KEY; cnts[s] = 0; }
+        int start = a.indices[v], deg = a.indices[v + 1] - start;
+        int2 own = a.lv[v];
+        Table t{keys, cnts, auxs, min(TB, max(64, round64(LOAD_NUM * deg / 2)))};
+        tclear(t, threadIdx.x, NT);
         __syncthreads();
-        insert_row<(TB / 2 / NT > 8 ? 8 : (TB / 2 / NT < 1 ? 1 : TB / 2 / NT))>(a, keys, cnts, mask, log_t, start,
-                                                                                  deg, threadIdx.x, NT);
+        insert_row<(TB / NT > 8 ? 8 : (TB / NT < 2 ? 2 : TB / NT))>(a, t, start, deg, threadIdx.x, NT);
         __syncthreads();
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_table(a, keys, cnts, T, threadIdx.x, NT, deg, own, tb, c, own_cnt, distinct);
+        scan_table(a, t, threadIdx.x, NT, deg, own.x, tb, c, own_cnt, distinct);
         block_reduce_cand<NT>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
-            a.new_labels[v] = lp_decide(a, deg, own, own_cnt, c);
+            a.new_labels[v] = lp_decide(a, deg, own.x, own.y, own_cnt, c);
             if (a.counters) count3(a, 1, deg, distinct);
         }
     }
@@ -360,16 +406,17 @@ struct HubArgs {
     const int4 *info;          // per hub {v, P, first part item, first chunk}
     const int64_t *buf_base;   // per hub
     int32_t *poff;
-    int32_t *buf;
+    int2 *buf;
     HubPartial *partial;
     int32_t *over;
+    int32_t *over_count;  // partitions that overflowed (diagnostic)
     int nchunks, nparts, nhubs;
-    int table_slots;  // LDS slots used in pass 2 (power of two <= HUB_TB)
+    int table_slots;  // LDS slots per partition (multiple of 64, <= HUB_TB)
 };
 
-// Partition of a label: a 32-bit finaliser (murmur3 fmix32) independent of
-// the multiplicative slot hash, so the keys of one partition still spread
-// over the whole LDS table.
+// Partition of a label: murmur3's 32-bit finaliser, independent of the
+// multiplicative slot hash, so one partition's keys still spread over the
+// whole table.
 __device__ inline int hub_part_of(int label, int P) {
     uint32_t x = (uint32_t)label;
     x ^= x >> 16;
@@ -380,28 +427,34 @@ __device__ inline int hub_part_of(int label, int P) {
     return (int)(((uint64_t)x * (uint32_t)P) >> 32);
 }
 
-constexpr int HUB1_NT = 512, HUB1_PER = 8;  // 4096 entries per chunk item
-constexpr int HUB2_NT = 1024, HUB_TB = 16384;
+constexpr int HUB1_NT = 512, HUB1_PER = HUB_CHUNK / HUB1_NT;
+constexpr int HUB2_NT = 512;
+static_assert(HUB1_NT * HUB1_PER == HUB_CHUNK, "hub chunk layout");
 
-// pass 1: gather the chunk's labels and group them by partition
+// pass 1: gather the chunk's records and group them by partition
 __global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) {
     __shared__ int cnt[HUB_MAX_PARTS + 1];
     for (int it = blockIdx.x; it < h.nchunks; it += gridDim.x) {
         const int4 ck = h.chunks[it];
         const int v = ck.y;
         if (!selected(a.seed, a.step, v, a.f)) continue;
-        const int4 inf = h.info[ck.x];
-        const int P = inf.y;
+        const int P = h.info[ck.x].y;
         for (int i = threadIdx.x; i <= P; i += HUB1_NT) cnt[i] = 0;
         __syncthreads();
-        int lab[HUB1_PER], part[HUB1_PER], rank[HUB1_PER];
+        int nb[HUB1_PER], part[HUB1_PER], rank[HUB1_PER];
+        int2 rec[HUB1_PER];
 #pragma unroll
         for (int e = 0; e < HUB1_PER; e++) {
             int j = ck.z + e * HUB1_NT + threadIdx.x;
+            nb[e] = j < ck.w ? a.adj[j] : -1;
+        }
+#pragma unroll
+        for (int e = 0; e < HUB1_PER; e++) rec[e] = nb[e] >= 0 ? a.lv[nb[e]] : make_int2(0, 0);
+#pragma unroll
+        for (int e = 0; e < HUB1_PER; e++) {
             part[e] = -1;
-            if (j < ck.w) {
-                lab[e] = a.labels[a.adj[j]];
-                part[e] = hub_part_of(lab[e], P);
+            if (nb[e] >= 0) {
+                part[e] = hub_part_of(rec[e].x, P);
                 rank[e] = atomicAdd(&cnt[part[e]], 1);
             }
         }
@@ -416,12 +469,13 @@ __global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) 
             cnt[P] = run;
         }
         __syncthreads();
-        const int64_t base = h.buf_base[ck.x] + (ck.z - a.indices[v]);
+        const int rel = ck.z - a.indices[v];  // chunk position inside the hub row
+        const int64_t base = h.buf_base[ck.x] + rel;
         int32_t *po = h.poff + h.chunk_off[it];
-        for (int q = threadIdx.x; q <= P; q += HUB1_NT) po[q] = (int32_t)(base + cnt[q] - h.buf_base[ck.x]);
+        for (int q = threadIdx.x; q <= P; q += HUB1_NT) po[q] = rel + cnt[q];
 #pragma unroll
         for (int e = 0; e < HUB1_PER; e++)
-            if (part[e] >= 0) h.buf[base + cnt[part[e]] + rank[e]] = lab[e];
+            if (part[e] >= 0) h.buf[base + cnt[part[e]] + rank[e]] = rec[e];
         __syncthreads();
     }
 }
@@ -430,18 +484,19 @@ __global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) 
 __global__ __launch_bounds__(HUB2_NT) void k_hub_count(LPArgs a, HubArgs h) {
     __shared__ int keys[HUB_TB];
     __shared__ int cnts[HUB_TB];
+    __shared__ int auxs[HUB_TB];
     __shared__ int seg[HUB2_NT + 1];
     __shared__ int segbase[HUB2_NT];
     __shared__ Cand s_c[HUB2_NT / WAVE];
     __shared__ int s_i[2 * HUB2_NT / WAVE];
     __shared__ int s_over;
-    const int T = h.table_slots, log_t = 31 - __clz(T), mask = T - 1;
+    const Table t{keys, cnts, auxs, h.table_slots};
     for (int it = blockIdx.x; it < h.nparts; it += gridDim.x) {
         const int4 pi = h.parts[it];
         const int4 inf = h.info[pi.x];
-        const int v = inf.x, P = inf.y, part = pi.y;
+        const int v = inf.x, part = pi.y;
         if (!selected(a.seed, a.step, v, a.f)) continue;
-        for (int s = threadIdx.x; s < T; s += HUB2_NT) { keys[s] = EMPTY_KEY; cnts[s] = 0; }
+        tclear(t, threadIdx.x, HUB2_NT);
         if (threadIdx.x == 0) s_over = 0;
         __syncthreads();
         const int64_t hb = h.buf_base[pi.x];
@@ -456,22 +511,24 @@ __global__ __launch_bounds__(HUB2_NT) void k_hub_count(LPArgs a, HubArgs h) {
             int tot;
             int ex = block_excl_scan_n<HUB2_NT>(len, &tot, s_i);
             if (threadIdx.x < nc) { seg[threadIdx.x] = ex; segbase[threadIdx.x] = b0; }
-            if (threadIdx.x == 0) seg[nc] = tot;
             __syncthreads();
             for (int i = threadIdx.x; i < tot; i += HUB2_NT) {
-                int lo = 0, hi = nc;  // last segment with seg[s] <= i
+                int lo = 0, hi = nc;  // last segment with seg[lo] <= i
                 while (hi - lo > 1) {
                     int mid = (lo + hi) >> 1;
                     if (seg[mid] <= i) lo = mid;
                     else hi = mid;
                 }
-                int key = h.buf[hb + segbase[lo] + (i - seg[lo])];
-                if (!lds_insert(keys, cnts, mask, log_t, key, 1)) s_over = 1;
+                int2 r = h.buf[hb + segbase[lo] + (i - seg[lo])];
+                if (!tinsert(t, r.x, r.y, 1)) s_over = 1;
             }
             __syncthreads();
         }
         if (s_over) {
-            if (threadIdx.x == 0) h.over[pi.x] = 1;
+            if (threadIdx.x == 0) {
+                h.over[pi.x] = 1;
+                atomicAdd(h.over_count, 1);
+            }
             __syncthreads();
             continue;
         }
@@ -479,10 +536,9 @@ __global__ __launch_bounds__(HUB2_NT) void k_hub_count(LPArgs a, HubArgs h) {
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_table(a, keys, cnts, T, threadIdx.x, HUB2_NT, deg, own, tb, c, own_cnt, distinct);
+        scan_table(a, t, threadIdx.x, HUB2_NT, deg, own, tb, c, own_cnt, distinct);
         block_reduce_cand<HUB2_NT>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) h.partial[it] = HubPartial{c, own_cnt, distinct};
-        (void)P;
     }
 }
 
@@ -511,14 +567,18 @@ __global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
         distinct = wave_sum(distinct);
         if (lane == 0) {
             const int deg = a.indices[v + 1] - a.indices[v];
-            a.new_labels[v] = lp_decide(a, deg, a.labels[v], own_cnt, c);
+            int2 own = a.lv[v];
+            a.new_labels[v] = lp_decide(a, deg, own.x, own.y, own_cnt, c);
             if (a.counters) count3(a, 1, deg, distinct);
         }
     }
 }
 
 // Fallback for flagged hubs: the whole row counted in a per-hub global
-// open-addressing table of nextpow2(2 deg) slots (never full).
+// open-addressing table of nextpow2(2 deg) slots (never full); community
+// records are read from the statistics.
+__device__ inline uint32_t gslot(int key, int log_t) { return ((uint32_t)key * 0x9E3779B1u) >> (32 - log_t); }
+
 __global__ __launch_bounds__(512) void k_hub_global_insert(LPArgs a, HubArgs h, const int64_t *toff,
                                                            const int32_t *tlog, int32_t *gkeys, int32_t *gcnts) {
     for (int it = blockIdx.x; it < h.nchunks; it += gridDim.x) {
@@ -528,7 +588,7 @@ __global__ __launch_bounds__(512) void k_hub_global_insert(LPArgs a, HubArgs h, 
         const int glog = tlog[ck.x], gmask = (1 << glog) - 1;
         for (int j = ck.z + threadIdx.x; j < ck.w; j += blockDim.x) {
             int key = a.labels[a.adj[j]];
-            uint32_t s = hslot(key, glog);
+            uint32_t s = gslot(key, glog);
             while (true) {
                 int old = atomicCAS(&gkeys[off + s], EMPTY_KEY, key);
                 if (old == EMPTY_KEY || old == key) {
@@ -563,13 +623,19 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
             distinct++;
             if (key == own) own_cnt = cnt;
             else {
-                Cand o = lp_candidate(a, deg, tb, key, cnt);
-                if (cand_better(o, c)) c = o;
+                double sc = score_join(a.rule, deg, a.stats[key], cnt);
+                if (sc > 0) {
+                    Cand o{sc, tie_key(a.tie, tb, key), key};
+                    if (cand_better(o, c)) c = o;
+                }
             }
         }
         block_reduce_cand<1024>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
-            a.new_labels[v] = lp_decide(a, deg, own, own_cnt, c);
+            double leave = score_leave(a.rule, deg, a.stats[own], own_cnt);
+            double bs = c.label != INT32_MAX ? c.s : 0.0;
+            bs += leave;
+            a.new_labels[v] = (bs > 0 && bs > leave) ? c.label : (leave > 0 ? PCD_NEW_LABEL : own);
             if (a.counters) count3(a, 1, deg, distinct);
             h.over[hub] = 0;
         }
@@ -585,6 +651,7 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
     a.adj = ctx->d_adj;
     a.labels = ctx->d_labels;
     a.order = ctx->d_order;
+    a.lv = ctx->d_lv;
     a.stats = ctx->d_stats;
     a.new_labels = ctx->d_labels_alt;
     a.rule = make_rule(ctx, rule);
@@ -592,6 +659,7 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
     a.step = step;
     a.tie = tie;
     a.f = (int)(fr * 2147483647.0);  // refinement.cpp:158
+    a.use_aux = rule.kind == PCD_RULE_MODULARITY || rule.kind == PCD_RULE_CPM;
     a.counters = nullptr;
     auto args_for = [&](int bin) {
         LPArgs x = a;
@@ -603,6 +671,10 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
         p1 = b.start[bin + 1];
         return p1 > p0;
     };
+    kt_begin(ctx);
+    hipLaunchKernelGGL(k_lp_records, dim3(grid_for(ctx, ctx->n, 256, 8)), dim3(256), 0, ctx->stream, ctx->n,
+                       ctx->d_labels, ctx->d_stats, rule.kind, ctx->d_lv);
+    kt_end(ctx, KS_LP_SELECT);
     int p0, p1;
     if (ctx->num_hubs > 0) {
         HubArgs h;
@@ -612,20 +684,22 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
         h.info = ctx->d_hub_info;
         h.buf_base = ctx->d_hub_buf_base;
         h.poff = ctx->d_hub_poff;
-        h.buf = ctx->d_hub_buf;
+        h.buf = (int2 *)ctx->d_hub_buf;
         h.partial = (HubPartial *)ctx->d_hub_partial;
         h.over = ctx->d_hub_over;
+        h.over_count = ctx->d_counts + NUM_BINS + 2;
         h.nchunks = ctx->num_hub_items;
         h.nparts = ctx->num_hub_parts;
         h.nhubs = ctx->num_hubs;
-        h.table_slots = ctx->tune.hub_table_slots;
+        // the LDS table holds HUB_TB slots: never index past it
+        h.table_slots = std::min(HUB_TB, std::max(64, ctx->tune.hub_table_slots & ~63));
         LPArgs ah = args_for(BIN_HUB);
         kt_begin(ctx);
         hipLaunchKernelGGL(k_hub_partition, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0, ctx->stream, ah,
                            h);
         kt_end(ctx, KS_LP_HUB_INSERT);
         kt_begin(ctx);
-        hipLaunchKernelGGL(k_hub_count, dim3(grid_for(ctx, h.nparts, 1, 1)), dim3(HUB2_NT), 0, ctx->stream, ah, h);
+        hipLaunchKernelGGL(k_hub_count, dim3(grid_for(ctx, h.nparts, 1, 3)), dim3(HUB2_NT), 0, ctx->stream, ah, h);
         hipLaunchKernelGGL(k_hub_final, dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0, ctx->stream, ah, h);
         hipLaunchKernelGGL(k_hub_global_insert, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(512), 0, ctx->stream, ah,
                            h, ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts);
@@ -633,27 +707,21 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
                            h, ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts);
         kt_end(ctx, KS_LP_HUB_DECIDE);
     }
-    if (range(BIN_B8192, p0, p1)) {
-        kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_block<1024, 16384>), dim3(grid_for(ctx, p1 - p0, 1, 1)), dim3(1024), 0, ctx->stream,
-                           p0, p1, args_for(BIN_B8192));
-        kt_end(ctx, KS_LP_B8192);
-    }
     if (range(BIN_B4096, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_block<512, 8192>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0, ctx->stream,
+        hipLaunchKernelGGL((k_lp_block<512, 6144, 3>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0, ctx->stream,
                            p0, p1, args_for(BIN_B4096));
         kt_end(ctx, KS_LP_B4096);
     }
     if (range(BIN_B1024, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_block<256, 2048>), dim3(grid_for(ctx, p1 - p0, 1, 10)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL((k_lp_block<256, 2048, 4>), dim3(grid_for(ctx, p1 - p0, 1, 6)), dim3(256), 0, ctx->stream,
                            p0, p1, args_for(BIN_B1024));
         kt_end(ctx, KS_LP_B1024);
     }
     if (range(BIN_W256, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL(k_lp_wave<512>, dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256), 0, ctx->stream, p0, p1,
+        hipLaunchKernelGGL(k_lp_wave<512>, dim3(grid_for(ctx, p1 - p0, 4, 6)), dim3(256), 0, ctx->stream, p0, p1,
                            args_for(BIN_W256));
         kt_end(ctx, KS_LP_W256);
     }

@@ -149,7 +149,7 @@ def test_propagation_bin_zoo(ctx, kind, lam, tie):
     _lp_compare(ctx, gu, rng.integers(0, 500, n), kind, lam, 0.7, seed=5, step=2, tie=tie)
 
 
-@pytest.mark.parametrize("slots,part_entries", [(16384, 4096), (64, 4096), (1024, 512), (64, 100000)])
+@pytest.mark.parametrize("slots,part_entries", [(4096, 2048), (64, 2048), (1024, 512), (64, 100000), (4096, 100000)])
 def test_propagation_hub_paths(ctx, slots, part_entries):
     """Hub rows: partitioned LDS counting, and the global-table fallback when a
     partition overflows its table (forced by tiny tables)."""
@@ -164,8 +164,24 @@ def test_propagation_hub_paths(ctx, slots, part_entries):
                 _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=4, step=3, tie=O.TIE_RANDOM)
                 _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=4, step=4, tie=O.TIE_LOWEST)
     finally:
-        ctx.set_tuning("hub_part_entries", 4096)
-        ctx.set_tuning("hub_table_slots", 16384)
+        ctx.set_tuning("hub_part_entries", 2048)
+        ctx.set_tuning("hub_table_slots", 4096)
+
+
+def test_hub_partitions_fit_with_default_tuning():
+    """A fresh context's default tuning counts hub partitions in LDS (no
+    overflow into the global-table fallback) on hubs of 5k-14k entries."""
+    g = GG.bin_zoo()
+    n, m, ind, adj = g[:4]
+    with parcd.Context(0) as c:
+        c.upload_graph(n, m, ind, adj)
+        assert c.bin_sizes()["hub"][0] == 3
+        lab_o, so, ivo, evo = check_partition(c, g, GG.singleton_labels(n))
+        c.hub_overflows()
+        c.propagate(O.MOD, 0.0, frac=1.0, seed=1, step=0, tie=O.TIE_RANDOM)
+        assert c.hub_overflows() == 0
+        want = O.propagation_refine(O.MOD, 0.0, n, m, ind, adj, lab_o, so, ivo, evo, 0.0, 1.0, 1, 1, 0, O.TIE_RANDOM)
+        assert np.array_equal(c.labels(), want)
 
 
 def test_propagation_wcc(ctx):
