@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
     ap.add_argument("--no-cpm", action="store_true", help="skip the CPM run")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="kernel-schedule tunable (pcd_set_tuning); results do not depend on it")
     ap.add_argument("--no-replay", action="store_true",
                     help="skip the counter replay (for profiler runs: only the timed iterations execute)")
     args = ap.parse_args()
@@ -153,6 +155,11 @@ def main():
     init = parcd.singleton_partition(n)
 
     ctx = parcd.Context(0)
+    for kv in args.tune:
+        k_, v_ = kv.split("=")
+        ctx.set_tuning(k_, int(v_))
+    copy_gbs = ctx.copy_probe(1 << 30, 10)
+    log("stream copy: %.0f GB/s (1 GiB, read+write)" % copy_gbs)
     ctx.upload_graph(n, m, ind, adj)
     sizes = ctx.bin_sizes()
     log("bins:", {k: v for k, v in sizes.items() if v[0]})
@@ -242,7 +249,7 @@ def main():
         "config": {"workload": "rmat%d-plain-mod" % args.scale, "graph": "R-MAT scale %d seed 2, 16x draws, "
                    "a/b/c/d=.57/.19/.19/.05" % args.scale, "n": n, "m": m, "E": E, "frac": 0.5,
                    "step": "compress+statistics+metric+label propagation", "teps": "2m entries per iteration"},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "stream_copy_gbs": round(copy_gbs, 1),
         "speedup_vs_cpu": round(gteps / cpu["value"], 2) if cpu else None,
         "phases": {"propagate": lp_phase, "statistics": stats_phase, "kernels": per_kernel},
         "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm,

@@ -161,12 +161,17 @@ int pcd_bin_counters(pcd_ctx *ctx, int bin, int64_t *active, int64_t *entries, i
  * partition table, multiple of 64 in [64, 4096]; takes effect at once),
  * "hub_part_entries" (target entries per hub label partition) and
  * "stat_chunk" (entries per statistics work item), which take effect at the
- * next pcd_graph_upload.  Results do not depend on them. */
+ * next pcd_graph_upload, and "small_aux" (0/1: the small-degree bins gather
+ * {label, community total} records instead of labels).  Results do not
+ * depend on them. */
 int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value);
 /* Hub label partitions whose distinct labels overflowed their LDS table
  * since the last call (those hubs were recounted in global memory); the
  * counter is reset by the call. */
 int pcd_hub_overflows(pcd_ctx *ctx, int64_t *count);
+/* Streaming device-to-device copy of `bytes` (16-byte accesses), `reps`
+ * times: *gbs receives the attained read+write bandwidth in GB/s. */
+int pcd_copy_probe(pcd_ctx *ctx, int64_t bytes, int reps, double *gbs);
 /* Vertices per degree bin of the uploaded graph (bin layout). */
 int pcd_bin_size(pcd_ctx *ctx, int bin, int64_t *vertices, int64_t *entries);
 

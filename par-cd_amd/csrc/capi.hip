@@ -618,6 +618,8 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
     } else if (k == "hub_part_entries") {
         if (value < 64) return fail(PCD_E_INVALID, "hub_part_entries must be >= 64");
         ctx->tune.hub_part_entries = value;  // applies to the next graph upload
+    } else if (k == "small_aux") {
+        ctx->tune.small_aux = value != 0;
     } else if (k == "stat_chunk") {
         if (value < 64) return fail(PCD_E_INVALID, "stat_chunk must be >= 64");
         ctx->tune.stat_chunk = value;  // applies to the next graph upload
@@ -688,6 +690,39 @@ extern "C" int pcd_hub_overflows(pcd_ctx *ctx, int64_t *count) {
     if (count) *count = h;
     PCD_TRY(fill_i32(ctx, ctx->d_counts + NUM_BINS + 2, 0, 1));
     PCD_HIP(hipStreamSynchronize(ctx->stream));
+    return PCD_OK;
+}
+
+namespace pcd {
+__global__ void k_stream_copy(const int4 *__restrict__ src, int4 *__restrict__ dst, int64_t n);
+}
+
+extern "C" int pcd_copy_probe(pcd_ctx *ctx, int64_t bytes, int reps, double *gbs) {
+    PCD_TRY(check_ctx(ctx));
+    if (bytes < 1024 || reps < 1 || !gbs) return fail(PCD_E_INVALID, "bad copy probe arguments");
+    const int64_t n = bytes / 16;
+    int4 *a = nullptr, *b = nullptr;
+    PCD_TRY(dev_alloc_t(ctx, &a, n));
+    int rc = dev_alloc_t(ctx, &b, n);
+    if (rc != PCD_OK) { dev_free(ctx, a); return rc; }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int grid = ctx->cu_count * 8;
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, a, b, n);  // warm-up
+    (void)hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, (r & 1) ? b : a, (r & 1) ? a : b, n);
+    (void)hipEventRecord(e1, ctx->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dev_free(ctx, a);
+    dev_free(ctx, b);
+    if (e != hipSuccess) return fail(PCD_E_HIP, std::string("copy probe: ") + hipGetErrorString(e));
+    *gbs = 2.0 * 16.0 * (double)n * reps / (ms / 1000.0) / 1e9;
     return PCD_OK;
 }
 

@@ -48,6 +48,7 @@ struct Tuning {
     int hub_part_entries = 2048; // target entries per hub label partition
     int hub_table_slots = 4096;  // LDS table slots of a hub partition (multiple of 64, <= HUB_TB)
     int stat_chunk = 2048;       // entries per statistics work item for deg > 64
+    int small_aux = 0;           // small bins gather {label, aux} records (1) or labels (0)
 };
 constexpr int HUB_MAX_PARTS = 1024;
 constexpr int HUB_TB = 4096;          // LDS table capacity of a hub partition

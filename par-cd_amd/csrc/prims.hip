@@ -42,6 +42,13 @@ int fill_u64(pcd_ctx *ctx, uint64_t *p, uint64_t v, int64_t count) {
     return PCD_OK;
 }
 
+// Streaming copy with 16-byte accesses: the attainable HBM bandwidth, and a
+// kernel of known traffic for calibrating memory counters.
+__global__ __launch_bounds__(256) void k_stream_copy(const int4 *__restrict__ src, int4 *__restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 int ensure_scan_parts(pcd_ctx *ctx, int64_t n) {
     int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (tiles < 1) tiles = 1;

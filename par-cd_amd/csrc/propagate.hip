@@ -102,6 +102,15 @@ __device__ inline int lp_decide(const LPArgs &a, int deg, int own, int own_aux, 
     return own;
 }
 
+__device__ inline int lp_decide_stats(const LPArgs &a, int deg, int own, int own_cnt, const Cand &best) {
+    double leave = score_leave(a.rule, deg, a.stats[own], own_cnt);
+    double bs = best.label != INT32_MAX ? best.s : 0.0;
+    bs += leave;
+    if (bs > 0 && bs > leave) return best.label;
+    if (leave > 0) return PCD_NEW_LABEL;
+    return own;
+}
+
 __device__ inline Cand lp_candidate(const LPArgs &a, int deg, uint64_t tb, int c, int aux, int cnt) {
     double s = score_join(a.rule, deg, comm_of(a, c, aux), cnt);
     if (s > 0) return Cand{s, tie_key(a.tie, tb, c), c};
@@ -131,7 +140,7 @@ __global__ void k_lp_records(int n, const int32_t *labels, const CommStats *st, 
 }
 
 // ------------------------------------------------------------- G lanes / vertex
-template <int G, int CP>
+template <int G, int CP, bool AUX>
 __global__ __launch_bounds__(256) void k_lp_small(int p0, int p1, LPArgs a) {
     __shared__ int s_list[256 / WAVE][WAVE];
     const int lane = __lane_id(), glane = lane & (G - 1), w = threadIdx.x / WAVE;
@@ -160,16 +169,20 @@ __global__ __launch_bounds__(256) void k_lp_small(int p0, int p1, LPArgs a) {
                 u = list[gi];
                 start = a.indices[u];
                 deg = a.indices[u + 1] - start;
-                own = a.lv[u];
+                own = AUX ? a.lv[u] : make_int2(a.labels[u], 0);
             }
-            int2 r = (has && glane < deg) ? a.lv[a.adj[start + glane]] : make_int2(INT32_MAX, 0);
+            int2 r = make_int2(INT32_MAX, 0);
+            if (has && glane < deg) {
+                int nb = a.adj[start + glane];
+                r = AUX ? a.lv[nb] : make_int2(a.labels[nb], 0);
+            }
             int x = r.x, xa = r.y;
 #pragma unroll
             for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
                 for (int j = k >> 1; j > 0; j >>= 1) {
                     int y = __shfl_xor(x, j, WAVE);
-                    int ya = __shfl_xor(xa, j, WAVE);
+                    int ya = AUX ? __shfl_xor(xa, j, WAVE) : 0;
                     bool up = (glane & k) == 0;
                     bool lower = (glane & j) == 0;
                     bool take_min = lower == up;
@@ -187,8 +200,13 @@ __global__ __launch_bounds__(256) void k_lp_small(int p0, int p1, LPArgs a) {
                 int nxt = above ? lane + 1 + __builtin_ctzll(above) : WAVE;
                 int gend = (lane & ~(G - 1)) + deg;
                 int cnt = min(nxt, gend) - lane;
-                if (x == own.x) own_cnt = cnt;
-                else c = lp_candidate(a, deg, a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, u), x, xa, cnt);
+                if (x == own.x) {
+                    own_cnt = cnt;
+                } else {
+                    uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, u);
+                    double sc = score_join(a.rule, deg, AUX ? CommStats{xa, xa, 0, 0} : a.stats[x], cnt);
+                    if (sc > 0) c = Cand{sc, tie_key(a.tie, tb, x), x};
+                }
             }
 #pragma unroll
             for (int d = G / 2; d > 0; d >>= 1) {
@@ -196,7 +214,9 @@ __global__ __launch_bounds__(256) void k_lp_small(int p0, int p1, LPArgs a) {
                 if (cand_better(o, c)) c = o;
                 own_cnt = max(own_cnt, __shfl_xor(own_cnt, d, WAVE));
             }
-            if (has && glane == 0) a.new_labels[u] = lp_decide(a, deg, own.x, own.y, own_cnt, c);
+            if (has && glane == 0)
+                a.new_labels[u] = AUX ? lp_decide(a, deg, own.x, own.y, own_cnt, c)
+                                      : lp_decide_stats(a, deg, own.x, own_cnt, c);
             if (a.counters) {
                 int ent = wave_sum((has && glane < deg) ? 1 : 0);
                 if (lane == 0) count3(a, __popcll(__ballot(has && glane == 0)), ent, __popcll(heads));
@@ -219,11 +239,12 @@ __device__ inline int tslot(int key, int T) {
 
 // Inserts (key, aux) with count add; probes at most T slots and returns
 // false when the table is full.
+template <bool AUX>
 __device__ inline bool tinsert(const Table &t, int key, int aux, int add) {
     int h = tslot(key, t.T);
     for (int probe = 0; probe < t.T; probe++) {
         int old = atomicCAS(&t.keys[h], EMPTY_KEY, key);
-        if (old == EMPTY_KEY) t.auxs[h] = aux;
+        if (AUX && old == EMPTY_KEY) t.auxs[h] = aux;
         if (old == EMPTY_KEY || old == key) {
             atomicAdd(&t.cnts[h], add);
             return true;
@@ -237,9 +258,10 @@ __device__ inline void tclear(const Table &t, int lo, int stride) {
     for (int s = lo; s < t.T; s += stride) { t.keys[s] = EMPTY_KEY; t.cnts[s] = 0; }
 }
 
-// Gathers the records of row entries [start + lo, start + deg) with stride
-// `stride`, PF at a time (independent loads in flight), and inserts them.
-template <int PF>
+// Gathers the labels (or {label, aux} records) of row entries
+// [start + lo, start + deg) with stride `stride`, PF at a time (independent
+// loads in flight), and inserts them.
+template <int PF, bool AUX>
 __device__ inline void insert_row(const LPArgs &a, const Table &t, int start, int deg, int lo, int stride) {
     for (int j0 = lo; j0 < deg; j0 += PF * stride) {
         int nb[PF];
@@ -250,16 +272,21 @@ __device__ inline void insert_row(const LPArgs &a, const Table &t, int start, in
         }
         int2 r[PF];
 #pragma unroll
-        for (int e = 0; e < PF; e++) r[e] = nb[e] >= 0 ? a.lv[nb[e]] : make_int2(EMPTY_KEY, 0);
+        for (int e = 0; e < PF; e++) {
+            if (AUX) r[e] = nb[e] >= 0 ? a.lv[nb[e]] : make_int2(EMPTY_KEY, 0);
+            else r[e] = make_int2(nb[e] >= 0 ? a.labels[nb[e]] : EMPTY_KEY, 0);
+        }
 #pragma unroll
         for (int e = 0; e < PF; e++)
-            if (r[e].x != EMPTY_KEY) tinsert(t, r[e].x, r[e].y, 1);
+            if (r[e].x != EMPTY_KEY) tinsert<AUX>(t, r[e].x, r[e].y, 1);
     }
 }
 
 // Scans table slots [lo, T) with stride `stride` for candidates, SB slots at
-// a time so that community-record gathers (rules without aux) overlap.
+// a time so that their community-record gathers overlap (tables without
+// aux records).
 constexpr int SB = 4;
+template <bool AUX>
 __device__ inline void scan_table(const LPArgs &a, const Table &t, int lo, int stride, int deg, int own, uint64_t tb,
                                   Cand &c, int &own_cnt, int &distinct) {
     for (int s0 = lo; s0 < t.T; s0 += SB * stride) {
@@ -272,13 +299,13 @@ __device__ inline void scan_table(const LPArgs &a, const Table &t, int lo, int s
             cnt[e] = 0;
             if (key[e] != EMPTY_KEY) {
                 cnt[e] = t.cnts[s];
-                if (a.use_aux) {
+                if (AUX) {
                     int x = t.auxs[s];
                     st[e] = CommStats{x, x, 0, 0};
                 }
             }
         }
-        if (!a.use_aux) {
+        if (!AUX) {
 #pragma unroll
             for (int e = 0; e < SB; e++)
                 if (key[e] != EMPTY_KEY && key[e] != own) st[e] = a.stats[key[e]];
@@ -328,13 +355,51 @@ __device__ inline void block_reduce_cand(Cand &c, int &own_cnt, int &distinct, C
 
 __device__ inline int round64(int x) { return (x + 63) & ~63; }
 
+// ------------------------------------------------------------- label tables
+// Tables of the per-vertex kernels hold nextpow2(2 deg) slots, more than the
+// row's distinct labels, so probing always ends at a free or matching slot.
+__device__ inline uint32_t pslot(int key, int log_t) { return ((uint32_t)key * 0x9E3779B1u) >> (32 - log_t); }
+
+__device__ inline void pinsert(int *keys, int *cnts, int mask, int log_t, int key) {
+    uint32_t h = pslot(key, log_t);
+    while (true) {
+        int old = atomicCAS(&keys[h], EMPTY_KEY, key);
+        if (old == EMPTY_KEY || old == key) {
+            atomicAdd(&cnts[h], 1);
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+template <int PF>
+__device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int mask, int log_t, int start, int deg,
+                                     int lo, int stride) {
+    for (int j0 = lo; j0 < deg; j0 += PF * stride) {
+        int nb[PF];
+#pragma unroll
+        for (int e = 0; e < PF; e++) {
+            int j = j0 + e * stride;
+            nb[e] = j < deg ? a.adj[start + j] : -1;
+        }
+        int lab[PF];
+#pragma unroll
+        for (int e = 0; e < PF; e++) lab[e] = nb[e] >= 0 ? a.labels[nb[e]] : EMPTY_KEY;
+#pragma unroll
+        for (int e = 0; e < PF; e++)
+            if (lab[e] != EMPTY_KEY) pinsert(keys, cnts, mask, log_t, lab[e]);
+    }
+}
+
+__device__ inline int ceil_log2(int x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
+
 // ------------------------------------------------------------- wave / vertex
 template <int TW>
 __global__ __launch_bounds__(256) void k_lp_wave(int p0, int p1, LPArgs a) {
     __shared__ int s_keys[256 / WAVE][TW];
     __shared__ int s_cnts[256 / WAVE][TW];
-    __shared__ int s_auxs[256 / WAVE][TW];
     const int lane = __lane_id(), w = threadIdx.x / WAVE;
+    int *keys = s_keys[w], *cnts = s_cnts[w];
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int nwaves = gridDim.x * blockDim.x / WAVE;
     for (int p = p0 + wave_g; p < p1; p += nwaves) {
@@ -343,19 +408,19 @@ __global__ __launch_bounds__(256) void k_lp_wave(int p0, int p1, LPArgs a) {
             if (lane == 0) a.new_labels[v] = a.labels[v];
             continue;
         }
-        int start = a.indices[v], deg = a.indices[v + 1] - start;
-        int2 own = a.lv[v];
-        Table t{s_keys[w], s_cnts[w], s_auxs[w], min(TW, max(64, round64(2 * deg)))};
+        int start = a.indices[v], deg = a.indices[v + 1] - start, own = a.labels[v];
+        int log_t = max(6, ceil_log2(2 * deg));
+        Table t{keys, cnts, nullptr, 1 << log_t};
         tclear(t, lane, WAVE);
         wave_sync();
-        insert_row<4>(a, t, start, deg, lane, WAVE);
+        insert_labels<4>(a, keys, cnts, t.T - 1, log_t, start, deg, lane, WAVE);
         wave_sync();
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_table(a, t, lane, WAVE, deg, own.x, tb, c, own_cnt, distinct);
+        scan_table<false>(a, t, lane, WAVE, deg, own, tb, c, own_cnt, distinct);
         wave_reduce_cand(c, own_cnt);
-        if (lane == 0) a.new_labels[v] = lp_decide(a, deg, own.x, own.y, own_cnt, c);
+        if (lane == 0) a.new_labels[v] = lp_decide_stats(a, deg, own, own_cnt, c);
         if (a.counters) {
             distinct = wave_sum(distinct);
             if (lane == 0) count3(a, 1, deg, distinct);
@@ -365,12 +430,11 @@ __global__ __launch_bounds__(256) void k_lp_wave(int p0, int p1, LPArgs a) {
 }
 
 // ------------------------------------------------------------- block / vertex
-// Table slots: round64(LOAD_NUM * deg / 2), at most TB.
-template <int NT, int TB, int LOAD_NUM>
+// Table slots: nextpow2(2 deg), at most TB.
+template <int NT, int TB>
 __global__ __launch_bounds__(NT) void k_lp_block(int p0, int p1, LPArgs a) {
     __shared__ int keys[TB];
     __shared__ int cnts[TB];
-    __shared__ int auxs[TB];
     __shared__ Cand s_c[NT / WAVE];
     __shared__ int s_i[2 * NT / WAVE];
     for (int p = p0 + blockIdx.x; p < p1; p += gridDim.x) {
@@ -379,20 +443,20 @@ __global__ __launch_bounds__(NT) void k_lp_block(int p0, int p1, LPArgs a) {
             if (threadIdx.x == 0) a.new_labels[v] = a.labels[v];
             continue;
         }
-        int start = a.indices[v], deg = a.indices[v + 1] - start;
-        int2 own = a.lv[v];
-        Table t{keys, cnts, auxs, min(TB, max(64, round64(LOAD_NUM * deg / 2)))};
+        int start = a.indices[v], deg = a.indices[v + 1] - start, own = a.labels[v];
+        int log_t = max(6, ceil_log2(2 * deg));
+        Table t{keys, cnts, nullptr, 1 << log_t};
         tclear(t, threadIdx.x, NT);
         __syncthreads();
-        insert_row<(TB / NT > 8 ? 8 : (TB / NT < 2 ? 2 : TB / NT))>(a, t, start, deg, threadIdx.x, NT);
+        insert_labels<4>(a, keys, cnts, t.T - 1, log_t, start, deg, threadIdx.x, NT);
         __syncthreads();
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_table(a, t, threadIdx.x, NT, deg, own.x, tb, c, own_cnt, distinct);
+        scan_table<false>(a, t, threadIdx.x, NT, deg, own, tb, c, own_cnt, distinct);
         block_reduce_cand<NT>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
-            a.new_labels[v] = lp_decide(a, deg, own.x, own.y, own_cnt, c);
+            a.new_labels[v] = lp_decide_stats(a, deg, own, own_cnt, c);
             if (a.counters) count3(a, 1, deg, distinct);
         }
     }
@@ -520,7 +584,7 @@ __global__ __launch_bounds__(HUB2_NT) void k_hub_count(LPArgs a, HubArgs h) {
                     else hi = mid;
                 }
                 int2 r = h.buf[hb + segbase[lo] + (i - seg[lo])];
-                if (!tinsert(t, r.x, r.y, 1)) s_over = 1;
+                if (!tinsert<true>(t, r.x, r.y, 1)) s_over = 1;
             }
             __syncthreads();
         }
@@ -536,7 +600,7 @@ __global__ __launch_bounds__(HUB2_NT) void k_hub_count(LPArgs a, HubArgs h) {
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_table(a, t, threadIdx.x, HUB2_NT, deg, own, tb, c, own_cnt, distinct);
+        scan_table<true>(a, t, threadIdx.x, HUB2_NT, deg, own, tb, c, own_cnt, distinct);
         block_reduce_cand<HUB2_NT>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) h.partial[it] = HubPartial{c, own_cnt, distinct};
     }
@@ -671,10 +735,15 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
         p1 = b.start[bin + 1];
         return p1 > p0;
     };
-    kt_begin(ctx);
-    hipLaunchKernelGGL(k_lp_records, dim3(grid_for(ctx, ctx->n, 256, 8)), dim3(256), 0, ctx->stream, ctx->n,
-                       ctx->d_labels, ctx->d_stats, rule.kind, ctx->d_lv);
-    kt_end(ctx, KS_LP_SELECT);
+    // {label, aux} records are read by the hub path and, when tuned so, by
+    // the small bins; rules without aux read the statistics instead
+    const bool small_aux = a.use_aux && ctx->tune.small_aux;
+    if (a.use_aux && (ctx->num_hubs > 0 || small_aux)) {
+        kt_begin(ctx);
+        hipLaunchKernelGGL(k_lp_records, dim3(grid_for(ctx, ctx->n, 256, 8)), dim3(256), 0, ctx->stream, ctx->n,
+                           ctx->d_labels, ctx->d_stats, rule.kind, ctx->d_lv);
+        kt_end(ctx, KS_LP_SELECT);
+    }
     int p0, p1;
     if (ctx->num_hubs > 0) {
         HubArgs h;
@@ -709,27 +778,31 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
     }
     if (range(BIN_B4096, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_block<512, 6144, 3>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0, ctx->stream,
+        hipLaunchKernelGGL((k_lp_block<512, 8192>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0, ctx->stream,
                            p0, p1, args_for(BIN_B4096));
         kt_end(ctx, KS_LP_B4096);
     }
     if (range(BIN_B1024, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_block<256, 2048, 4>), dim3(grid_for(ctx, p1 - p0, 1, 6)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL((k_lp_block<256, 2048>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256), 0, ctx->stream,
                            p0, p1, args_for(BIN_B1024));
         kt_end(ctx, KS_LP_B1024);
     }
     if (range(BIN_W256, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL(k_lp_wave<512>, dim3(grid_for(ctx, p1 - p0, 4, 6)), dim3(256), 0, ctx->stream, p0, p1,
+        hipLaunchKernelGGL(k_lp_wave<512>, dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256), 0, ctx->stream, p0, p1,
                            args_for(BIN_W256));
         kt_end(ctx, KS_LP_W256);
     }
 #define PCD_SMALL(G, CP, BIN, SLOT)                                                                            \
     if (range(BIN, p0, p1)) {                                                                                  \
         kt_begin(ctx);                                                                                         \
-        hipLaunchKernelGGL((k_lp_small<G, CP>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)), dim3(256), 0,        \
-                           ctx->stream, p0, p1, args_for(BIN));                                                \
+        if (small_aux)                                                                                         \
+            hipLaunchKernelGGL((k_lp_small<G, CP, true>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)), dim3(256), \
+                               0, ctx->stream, p0, p1, args_for(BIN));                                         \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_lp_small<G, CP, false>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)),           \
+                               dim3(256), 0, ctx->stream, p0, p1, args_for(BIN));                              \
         kt_end(ctx, SLOT);                                                                                     \
     }
     PCD_SMALL(64, 8, BIN_G64, KS_LP_G64)

@@ -26,7 +26,7 @@ EXPORTS = [
     "pcd_convert_edges", "pcd_free", "pcd_gen_sbm", "pcd_gen_rmat", "pcd_gen_chung_lu",
     "pcd_kernel_slot_count", "pcd_kernel_slot_name", "pcd_kernel_timer_reset", "pcd_kernel_timer",
     "pcd_bin_count", "pcd_bin_name", "pcd_bin_counters_reset", "pcd_bin_counters", "pcd_bin_size",
-    "pcd_singleton_partition", "pcd_set_tuning", "pcd_hub_overflows",
+    "pcd_singleton_partition", "pcd_set_tuning", "pcd_hub_overflows", "pcd_copy_probe",
 ]
 
 
@@ -100,6 +100,7 @@ def lib():
     L.pcd_singleton_partition.argtypes = [C.c_int, vp]
     L.pcd_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
     L.pcd_hub_overflows.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.pcd_copy_probe.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -321,6 +322,11 @@ class Context:
 
     def set_tuning(self, key, value):
         check(lib().pcd_set_tuning(self._ctx, key.encode(), int(value)))
+
+    def copy_probe(self, nbytes=1 << 30, reps=10):
+        g = C.c_double()
+        check(lib().pcd_copy_probe(self._ctx, int(nbytes), int(reps), C.byref(g)))
+        return g.value
 
     def hub_overflows(self):
         c = C.c_int64()

@@ -136,17 +136,22 @@ def test_propagation_golden(ctx, kind, lam, tie):
                 _lp_compare(ctx, g, lab, kind, lam, frac, seed=1234, step=3, tie=tie)
 
 
+@pytest.mark.parametrize("small_aux", [0, 1])
 @pytest.mark.parametrize("tie", [O.TIE_RANDOM, O.TIE_LOWEST])
 @pytest.mark.parametrize("kind,lam", RULES)
-def test_propagation_bin_zoo(ctx, kind, lam, tie):
+def test_propagation_bin_zoo(ctx, kind, lam, tie, small_aux):
     g = GG.bin_zoo()
     n = g[0]
     rng = np.random.default_rng(3)
-    for lab in [GG.singleton_labels(n), rng.integers(0, 200, n), rng.integers(0, 20000, n)]:
-        _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=99, step=1, tie=tie)
-        _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=7, step=0, tie=tie)
-    gu = GG.unsorted_copy(g)
-    _lp_compare(ctx, gu, rng.integers(0, 500, n), kind, lam, 0.7, seed=5, step=2, tie=tie)
+    ctx.set_tuning("small_aux", small_aux)
+    try:
+        for lab in [GG.singleton_labels(n), rng.integers(0, 200, n), rng.integers(0, 20000, n)]:
+            _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=99, step=1, tie=tie)
+            _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=7, step=0, tie=tie)
+        gu = GG.unsorted_copy(g)
+        _lp_compare(ctx, gu, rng.integers(0, 500, n), kind, lam, 0.7, seed=5, step=2, tie=tie)
+    finally:
+        ctx.set_tuning("small_aux", 0)
 
 
 @pytest.mark.parametrize("slots,part_entries", [(4096, 2048), (64, 2048), (1024, 512), (64, 100000), (4096, 100000)])
