@@ -89,6 +89,36 @@ def lp_bin_bytes(active, entries, distinct, S):
 SLOT_OF_BIN = {"hub": "lp_hub", "b4096": "lp_b4096", "b1024": "lp_b1024", "w256": "lp_w256",
                "g64": "lp_g64", "g32": "lp_g32", "g16": "lp_g16", "g8": "lp_g8", "g4": "lp_g4"}
 
+# kernel-timer slot -> kernel names in a rocprofv3 trace of this build
+ROCPROF_NAMES = {
+    "stats_items": ["pcd::k_stats_items"], "lp_b4096": ["pcd::k_lp_block<512, 8192>"],
+    "lp_b1024": ["pcd::k_lp_block<256, 2048>"], "lp_w256": ["pcd::k_lp_wave<512>"],
+    "lp_hub": ["pcd::k_hub_partition", "pcd::k_hub_count"], "lp_g64": ["pcd::k_lp_small<64, 8, false>"],
+    "lp_g32": ["pcd::k_lp_small<32, 16, false>"], "lp_g16": ["pcd::k_lp_small<16, 32, false>"],
+    "lp_g8": ["pcd::k_lp_small<8, 64, false>"], "lp_g4": ["pcd::k_lp_small<4, 64, false>"]}
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "rmat22_mod_pmc.md")
+FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0  # stream-copy calibration of FETCH_SIZE / WRITE_SIZE
+
+
+def pmc_traffic(slot):
+    """HBM-side traffic (bytes per dispatch) of one kernel from the committed
+    rocprofv3 counter summary of this workload: FETCH_SIZE x 2.0 + WRITE_SIZE
+    x 1.0 (KiB).  None when the summary or the kernel is absent."""
+    names = ROCPROF_NAMES.get(slot)
+    if not names or not os.path.exists(PMC_SUMMARY):
+        return None
+    rows = {}
+    for line in open(PMC_SUMMARY):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) > 4 and cells[0].startswith("pcd::"):
+            try:
+                rows[cells[0]] = float(cells[3]) * 1024 * FETCH_FACTOR + float(cells[4]) * 1024 * WRITE_FACTOR
+            except ValueError:
+                pass
+    if not all(n in rows for n in names):
+        return None
+    return sum(rows[n] for n in names)
+
 
 def cpu_baseline(pcd_path, E, kind_arg, iters=2):
     exe = os.path.join(ROOT, "oracle", "_ref", "main")
@@ -200,6 +230,17 @@ def main():
         per_kernel[slot] = {
             "bytes_per_launch": byt / launches, "avg_ms": tms / launches,
             "gbs": byt / (tms / 1000.0) / 1e9, "entries_per_launch": e_ / launches}
+    # statistics over rows longer than 64 (work items): per entry the neighbour
+    # id and its label, per row its extent and label and one 12-byte record
+    # update
+    deg = np.diff(ind)
+    big = deg > 64
+    st_ms, st_launches = timers.get("stats_items", (0.0, 0))
+    if st_ms > 0 and st_launches:
+        st_bytes = 8 * int(deg[big].sum()) + 20 * int(big.sum())
+        per_kernel["stats_items"] = {"bytes_per_launch": st_bytes, "avg_ms": st_ms / st_launches,
+                                     "gbs": st_bytes / (st_ms / st_launches / 1000.0) / 1e9,
+                                     "entries_per_launch": int(deg[big].sum())}
     dom_name = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"]) if per_kernel else None
     lp_ms = sum(v[0] for k, v in timers.items() if k.startswith("lp_"))
     tot_a = sum(c[0] for c in counters.values())
@@ -216,8 +257,11 @@ def main():
     roofline = None
     if dom_name:
         d = per_kernel[dom_name]
+        traffic = pmc_traffic(dom_name) if args.scale == 22 else None
         roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(d["gbs"], 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
+                    "traffic": int(traffic) if traffic else None,
+                    "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if traffic else None,
                     "bytes_per_launch": int(d["bytes_per_launch"]), "avg_launch_ms": round(d["avg_ms"], 4)}
 
     # ---------------------------------------------------------- CPM 0.01
