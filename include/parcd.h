@@ -175,6 +175,66 @@ int pcd_copy_probe(pcd_ctx *ctx, int64_t bytes, int reps, double *gbs);
 /* Vertices per degree bin of the uploaded graph (bin layout). */
 int pcd_bin_size(pcd_ctx *ctx, int bin, int64_t *vertices, int64_t *entries);
 
+/* ---------------------------------------------------------------- batches */
+/* Many independent graphs refined together: one round runs one iteration of
+ * the main.cpp:226-266 loop (compress, statistics, metric, stop rule,
+ * propagation with -s 0) for every running graph, as one captured hipGraph
+ * of persistent kernels over (graph, position-range) work items.  Graphs
+ * are admitted into a device arena between rounds and keep their state
+ * there until released.  Each graph's labels, statistics and scores equal
+ * those of the same graph run alone through a context (same job
+ * parameters), whatever else is in the batch.  No reference counterpart:
+ * the reference runs one graph per process. */
+typedef struct pcd_batch pcd_batch;
+
+/* Per-graph options: the ./main flags of one run (main.cpp:55-120). */
+typedef struct pcd_job {
+    pcd_rule rule;    /* -m: MODULARITY or CPM */
+    double frac;      /* -f (0.5) */
+    double threshold; /* -t (0) */
+    int max_iter;     /* -n (-1: stop on the threshold rule) */
+    int tie;          /* PCD_TIE_* */
+    uint64_t seed;    /* counter-hash seed of selection and ties */
+    int iter_cap;     /* safety bound on iterations (<= 0: 1000) */
+} pcd_job;
+
+#define PCD_BATCH_RUNNING 0
+#define PCD_BATCH_DONE 1
+#define PCD_BATCH_FREE 2
+
+/* `max_graphs` slots and an arena of `arena_bytes` device bytes. */
+int pcd_batch_create(int device, int max_graphs, int64_t arena_bytes, pcd_batch **out);
+int pcd_batch_destroy(pcd_batch *b);
+/* Arena bytes a graph of n vertices and m edges occupies (upper bound). */
+int pcd_batch_footprint(int n, int m, int64_t *bytes);
+/* Admits a graph (CSR as pcd_graph_upload, degrees <= 4096) with its
+ * initial labels (in [0,n) or PCD_NEW_LABEL).  PCD_E_NOMEM when no slot or
+ * arena space is free: retry after releasing finished graphs. */
+int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices, const int32_t *adj, const int32_t *labels,
+                  const pcd_job *job, int *slot);
+/* Runs `rounds` rounds; *running receives the number of graphs still
+ * running afterwards. */
+int pcd_batch_step(pcd_batch *b, int rounds, int *running);
+/* State after the last step: PCD_BATCH_*, iterations (main.cpp's final
+ * `it`), communities and the last score (the value main.cpp:273 prints). */
+int pcd_batch_status(pcd_batch *b, int slot, int *status, int *iterations, int *k, double *score);
+/* Slots that finished and are not yet released (*count may exceed cap). */
+int pcd_batch_finished(pcd_batch *b, int *slots, int cap, int *count);
+/* Final compressed labels / community statistics of a finished graph. */
+int pcd_batch_labels(pcd_batch *b, int slot, int32_t *labels);
+int pcd_batch_stats(pcd_batch *b, int slot, int32_t *sizes, int32_t *int_volume, int32_t *ext_volume);
+/* Frees a slot and its arena space (a running graph is abandoned). */
+int pcd_batch_release(pcd_batch *b, int slot);
+int pcd_batch_info(pcd_batch *b, int64_t *free_bytes, int *graphs, int *running, int64_t *rounds);
+/* Bin maps: which propagation kernel serves which degree range. */
+int pcd_batch_map_count(void);
+const char *pcd_batch_map_name(int map);
+int pcd_batch_set_map(pcd_batch *b, int map);
+/* Times the propagation phase under every bin map on the current state
+ * (`reps` runs each; needs one completed round; changes no results) and
+ * keeps the fastest; ms_per_map (pcd_batch_map_count entries) may be NULL. */
+int pcd_batch_autotune(pcd_batch *b, int reps, int *chosen, double *ms_per_map);
+
 /* --------------------------------------------------------- host utilities */
 /* graph.cpp:40-105 Graph::read: reads a PCD file into malloc'ed arrays
  * (free with pcd_free).  Same validation: magic, sizes, no trailing bytes. */

@@ -1,0 +1,1007 @@
+// Batched refinement: many independent graphs advance together, one
+// refinement iteration of main.cpp:226-266 per round for every running
+// graph, with the round's launches captured once in a hipGraph and
+// replayed.
+//
+// Layout.  Each graph owns a region of one device arena: its CSR, an
+// optional degree order, two label buffers, the compression scratch
+// (first occurrence, new ids) and its community records.  The kernels see
+// the batch through work-item lists:
+//   vertex tiles     2048 consecutive vertex ids of one graph (compression,
+//                    statistics clearing, metric sums)
+//   small rows       up to CP positions of one graph's degree class <= G
+//                    (statistics with G lanes per vertex)
+//   long rows        <= 2048 entries of one row of degree > 64 (statistics)
+//   propagation      per kernel variant, position ranges of one graph's
+//                    degree range (the bin map chooses which variant serves
+//                    which degree range)
+// Every kernel runs a fixed grid that strides over its list; the lists'
+// lengths are read on the device, so a captured round stays valid while
+// graphs are admitted and retired.  Per-graph state (iteration, score,
+// community count, status) stays on the device; the stop rule of
+// main.cpp:241-242 is evaluated there, so a round needs no host read-back.
+//
+// Identity with the plain path: a graph's labels, statistics and scores
+// in a batch equal those of the same graph run alone (same seeds, same
+// per-graph vertex and label numbering); the kernels only change who
+// computes what.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "work.h"
+
+namespace pcd {
+
+constexpr int VT = 2048;  // vertices per tile item
+constexpr int ROW_CHUNK = 2048;
+constexpr int STAT_G[5] = {64, 32, 16, 8, 4};
+constexpr int STAT_CP[5] = {8, 16, 32, 64, 64};
+
+// --------------------------------------------------------------- kernels
+__global__ __launch_bounds__(256) void k_b_fill_first(const WorkItem *tiles, const int *count, const BatchGraph *gs,
+                                                      const BatchState *st) {
+    const int nt = *count;
+    for (int u = blockIdx.x; u < nt; u += gridDim.x) {
+        const WorkItem w = tiles[u];
+        if (st[w.g].status != BG_RUNNING) continue;
+        const int hi = min(w.p1, st[w.g].label_bound);
+        int32_t *first = gs[w.g].first;
+        for (int v = w.p0 + (int)threadIdx.x; v < hi; v += blockDim.x) first[v] = INT32_MAX;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_b_first_occurrence(const WorkItem *tiles, const int *count,
+                                                            const BatchGraph *gs, const BatchState *st) {
+    const int nt = *count;
+    for (int u = blockIdx.x; u < nt; u += gridDim.x) {
+        const WorkItem w = tiles[u];
+        if (st[w.g].status != BG_RUNNING) continue;
+        const int32_t *next = gs[w.g].next;
+        int32_t *first = gs[w.g].first;
+        for (int v = w.p0 + (int)threadIdx.x; v < w.p1; v += blockDim.x) {
+            int l = next[v];
+            if (l >= 0) atomicMin(&first[l], v);
+        }
+    }
+}
+
+// framework.cpp:146-170: a vertex opens a new id if its label is NEW or it
+// is its label's first occurrence
+__device__ inline int first_flag(const BatchGraph &g, int v) {
+    int l = g.next[v];
+    return (l < 0 || g.first[l] == v) ? 1 : 0;
+}
+
+constexpr int VT_PER = VT / 256;
+
+__global__ __launch_bounds__(256) void k_b_tile_count(const WorkItem *tiles, const int *count, const BatchGraph *gs,
+                                                      const BatchState *st, int32_t *tile_sum) {
+    __shared__ int lds[256 / WAVE];
+    const int nt = *count;
+    for (int u = blockIdx.x; u < nt; u += gridDim.x) {
+        const WorkItem w = tiles[u];
+        int c = 0;
+        if (st[w.g].status == BG_RUNNING) {
+            const BatchGraph &g = gs[w.g];
+            for (int v = w.p0 + (int)threadIdx.x; v < w.p1; v += blockDim.x) c += first_flag(g, v);
+        }
+        c = wave_sum(c);
+        if (__lane_id() == 0) lds[threadIdx.x / WAVE] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int i = 0; i < 256 / WAVE; i++) t += lds[i];
+            tile_sum[u] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// one workgroup: exclusive scan of the tile sums (one graph's tiles are
+// consecutive, so a graph's local offsets are differences of this scan)
+__global__ __launch_bounds__(1024) void k_b_tile_scan(const int *count, const int32_t *tile_sum, int32_t *tile_pre) {
+    __shared__ int lds[1024 / WAVE];
+    const int nt = *count;
+    int run = 0;
+    for (int base = 0; base < nt; base += 1024) {
+        int i = base + threadIdx.x;
+        int x = i < nt ? tile_sum[i] : 0;
+        int tot;
+        int ex = block_excl_scan_n<1024>(x, &tot, lds);
+        if (i < nt) tile_pre[i] = run + ex;
+        run += tot;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_b_tile_apply(const WorkItem *tiles, const int *count, const int2 *gtiles,
+                                                      const BatchGraph *gs, BatchState *st, const int32_t *tile_sum,
+                                                      const int32_t *tile_pre) {
+    __shared__ int lds[256 / WAVE];
+    const int nt = *count;
+    for (int u = blockIdx.x; u < nt; u += gridDim.x) {
+        const WorkItem w = tiles[u];
+        if (st[w.g].status != BG_RUNNING) continue;
+        const BatchGraph &g = gs[w.g];
+        const int2 gt = gtiles[w.g];
+        const int gbase = tile_pre[u] - tile_pre[gt.x];
+        // each thread owns VT_PER consecutive vertices
+        const int v0 = w.p0 + (int)threadIdx.x * VT_PER;
+        int flags = 0, c = 0;
+#pragma unroll
+        for (int e = 0; e < VT_PER; e++) {
+            int v = v0 + e;
+            int f = (v < w.p1) ? first_flag(g, v) : 0;
+            flags |= f << e;
+            c += f;
+        }
+        int tot;
+        int ex = block_excl_scan_n<256>(c, &tot, lds);
+        int id = gbase + ex;
+#pragma unroll
+        for (int e = 0; e < VT_PER; e++) {
+            if ((flags >> e) & 1) {
+                int v = v0 + e;
+                int l = g.next[v];
+                if (l < 0) g.cur[v] = id;
+                else g.newid[l] = id;
+                id++;
+            }
+        }
+        if (threadIdx.x == 0 && u == gt.x + gt.y - 1) st[w.g].k = gbase + tile_sum[u];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_b_relabel(const WorkItem *tiles, const int *count, const BatchGraph *gs,
+                                                   const BatchState *st) {
+    const int nt = *count;
+    for (int u = blockIdx.x; u < nt; u += gridDim.x) {
+        const WorkItem w = tiles[u];
+        if (st[w.g].status != BG_RUNNING) continue;
+        const BatchGraph &g = gs[w.g];
+        for (int v = w.p0 + (int)threadIdx.x; v < w.p1; v += blockDim.x) {
+            int l = g.next[v];
+            if (l >= 0) g.cur[v] = g.newid[l];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_b_clear_stats(const WorkItem *tiles, const int *count, const BatchGraph *gs,
+                                                       const BatchState *st) {
+    const int nt = *count;
+    for (int u = blockIdx.x; u < nt; u += gridDim.x) {
+        const WorkItem w = tiles[u];
+        if (st[w.g].status != BG_RUNNING) continue;
+        const int hi = min(w.p1, st[w.g].k);
+        CommStats *s = gs[w.g].stats;
+        for (int c = w.p0 + (int)threadIdx.x; c < hi; c += blockDim.x) s[c] = CommStats{0, 0, 0, 0};
+    }
+}
+
+// framework.cpp:69-143 over rows of degree <= G: G lanes per vertex
+template <int G>
+__global__ __launch_bounds__(256) void k_b_stats_small(const WorkItem *items, const int *count, const BatchGraph *gs,
+                                                       const BatchState *st) {
+    const int lane = __lane_id(), glane = lane & (G - 1);
+    constexpr int per_wave = WAVE / G;
+    const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    const int nu = *count;
+    for (int u = wave_g; u < nu; u += nwaves) {
+        const WorkItem w = items[u];
+        if (st[w.g].status != BG_RUNNING) continue;
+        const BatchGraph &g = gs[w.g];
+        const LPArgs &a = g.lp;
+        for (int base = w.p0; base < w.p1; base += per_wave) {
+            int p = base + lane / G;
+            bool has = p < w.p1;
+            int v = 0, start = 0, deg = 0, l = 0;
+            if (has) {
+                v = a.order ? a.order[p] : p;
+                start = a.indices[v];
+                deg = a.indices[v + 1] - start;
+                l = g.cur[v];
+            }
+            int same = 0;
+            for (int j = glane; j < deg; j += G) same += (g.cur[a.adj[start + j]] == l) ? 1 : 0;
+#pragma unroll
+            for (int d = G / 2; d > 0; d >>= 1) same += __shfl_xor(same, d, WAVE);
+            if (has && glane == 0) add_stats(g.stats, l, 1, same, deg - same);
+        }
+    }
+}
+
+// rows of degree > 64: one wave per chunk of <= 2048 entries
+__global__ __launch_bounds__(256) void k_b_stats_rows(const RowItem *rows, const int *count, const BatchGraph *gs,
+                                                      const BatchState *st) {
+    constexpr int PF = 8;
+    const int lane = __lane_id();
+    const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    const int nu = *count;
+    for (int u = wave_g; u < nu; u += nwaves) {
+        const RowItem r = rows[u];
+        if (st[r.g].status != BG_RUNNING) continue;
+        const BatchGraph &g = gs[r.g];
+        const int l = g.cur[r.v];
+        int same = 0;
+        for (int j0 = r.begin + lane; j0 < r.end; j0 += PF * WAVE) {
+            int nb[PF];
+#pragma unroll
+            for (int e = 0; e < PF; e++) {
+                int j = j0 + e * WAVE;
+                nb[e] = j < r.end ? g.lp.adj[j] : -1;
+            }
+#pragma unroll
+            for (int e = 0; e < PF; e++) same += (nb[e] >= 0 && g.cur[nb[e]] == l) ? 1 : 0;
+        }
+        same = wave_sum(same);
+        if (lane == 0) add_stats(g.stats, l, r.begin == g.lp.indices[r.v] ? 1 : 0, same, (r.end - r.begin) - same);
+    }
+}
+
+// exact int64 sums of int_volume, volume^2 and size*(size-1) per graph
+__global__ __launch_bounds__(256) void k_b_metric(const WorkItem *tiles, const int *count, const BatchGraph *gs,
+                                                  BatchState *st) {
+    __shared__ long long lds[3][256 / WAVE];
+    const int nt = *count;
+    for (int u = blockIdx.x; u < nt; u += gridDim.x) {
+        const WorkItem w = tiles[u];
+        if (st[w.g].status != BG_RUNNING) continue;
+        const int hi = min(w.p1, st[w.g].k);
+        if (w.p0 >= hi) continue;
+        const CommStats *s = gs[w.g].stats;
+        long long a = 0, b = 0, c = 0;
+        for (int i = w.p0 + (int)threadIdx.x; i < hi; i += blockDim.x) {
+            CommStats x = s[i];
+            long long vol = (long long)x.int_vol + x.ext_vol;
+            a += x.int_vol;
+            b += vol * vol;
+            c += (long long)x.size * (x.size - 1);
+        }
+        a = wave_sum64(a);
+        b = wave_sum64(b);
+        c = wave_sum64(c);
+        const int wv = threadIdx.x / WAVE;
+        if (__lane_id() == 0) { lds[0][wv] = a; lds[1][wv] = b; lds[2][wv] = c; }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            long long t = 0;
+            for (int i = 0; i < 256 / WAVE; i++) t += lds[threadIdx.x][i];
+            atomicAdd(&st[w.g].acc[threadIdx.x], (unsigned long long)t);
+        }
+        __syncthreads();
+    }
+}
+
+// Score (rules.cpp:9-22, 121-129), stop rule (main.cpp:241-242) and the
+// iteration bookkeeping of main.cpp:262-264, one thread per graph.
+__global__ void k_b_finalize(const BatchGraph *gs, BatchState *st, int nslots) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    BatchState s = st[i];
+    if (s.status != BG_RUNNING) return;
+    const BatchGraph &g = gs[i];
+    const long long a = (long long)s.acc[0], b = (long long)s.acc[1], c = (long long)s.acc[2];
+    const int m = g.lp.rule.m;
+    double score;
+    if (g.lp.rule.kind == PCD_RULE_MODULARITY) {
+        double two_m = 2.0 * m;
+        score = a / two_m - (double)b / (two_m * two_m);
+    } else {
+        score = (a - g.lp.rule.lambda * (double)c) / (double)(2 * m);
+    }
+    const int it = s.iter;
+    bool done = g.max_iter != -1 ? it >= g.max_iter
+                                 : (s.prev_score > 0 && score > 0 && score / s.prev_score - 1 <= g.threshold);
+    if (!done && it >= g.iter_cap) {
+        done = true;
+        s.capped = 1;
+    }
+    s.acc[0] = s.acc[1] = s.acc[2] = 0;
+    s.score = score;
+    s.prev_score = score;
+    s.step = it;
+    s.iter = it + 1;
+    s.label_bound = s.k;
+    if (done) s.status = BG_DONE;
+    st[i] = s;
+}
+
+}  // namespace pcd
+
+// ===================================================================== host
+using namespace pcd;
+
+namespace {
+
+// degree thresholds of the class boundaries (positions are degree-descending)
+constexpr int NUM_TH = 11;
+constexpr int TH[NUM_TH] = {4096, 1024, 512, 256, 128, 64, 32, 16, 8, 4, 0};
+int th_index(int t) {
+    for (int i = 0; i < NUM_TH; i++)
+        if (TH[i] == t) return i;
+    return -1;
+}
+
+// A bin map: variants with the largest degree each serves, descending.
+struct BinMap {
+    const char *name;
+    int n;
+    int var[10];
+    int hi[10];
+};
+const BinMap MAPS[] = {
+    {"s4-s64,w512<=256,b256<=1024,b8192", 8,
+     {LPV_B512X8192, LPV_B256, LPV_W512, LPV_S64, LPV_S32, LPV_S16, LPV_S8, LPV_S4},
+     {4096, 1024, 256, 64, 32, 16, 8, 4}},
+    {"s4-s32,w512<=256,b256<=1024,b8192", 7,
+     {LPV_B512X8192, LPV_B256, LPV_W512, LPV_S32, LPV_S16, LPV_S8, LPV_S4}, {4096, 1024, 256, 32, 16, 8, 4}},
+    {"s4-s64,w1024<=512,b256<=1024,b8192", 8,
+     {LPV_B512X8192, LPV_B256, LPV_W1024, LPV_S64, LPV_S32, LPV_S16, LPV_S8, LPV_S4},
+     {4096, 1024, 512, 64, 32, 16, 8, 4}},
+    {"s4-s64,w512<=256,b512<=1024,b8192", 8,
+     {LPV_B512X8192, LPV_B512, LPV_W512, LPV_S64, LPV_S32, LPV_S16, LPV_S8, LPV_S4},
+     {4096, 1024, 256, 64, 32, 16, 8, 4}},
+    {"s4-s64,w512<=128,b256<=1024,b8192", 8,
+     {LPV_B512X8192, LPV_B256, LPV_W512, LPV_S64, LPV_S32, LPV_S16, LPV_S8, LPV_S4},
+     {4096, 1024, 128, 64, 32, 16, 8, 4}},
+    {"s4-s64,w512<=256,w1024<=512,b256<=1024,b8192", 9,
+     {LPV_B512X8192, LPV_B256, LPV_W1024, LPV_W512, LPV_S64, LPV_S32, LPV_S16, LPV_S8, LPV_S4},
+     {4096, 1024, 512, 256, 64, 32, 16, 8, 4}},
+};
+constexpr int NUM_MAPS = sizeof(MAPS) / sizeof(MAPS[0]);
+
+struct Slot {
+    bool used = false;
+    int status = BG_FREE;
+    int n = 0, m = 0;
+    size_t off = 0, bytes = 0;
+    int64_t admitted = 0;      // rounds run before admission
+    int pos_le[NUM_TH] = {0};  // first position with degree <= TH[i]
+    std::vector<RowItem> rows; // rows of degree > 64 (g set when listed)
+    BatchGraph rec;
+};
+
+// item list layout inside one device buffer (and its pinned mirror)
+enum ListId { L_TILES = 0, L_SMALL0, L_ROWS = L_SMALL0 + 5, L_LP0, NUM_LISTS = L_LP0 + NUM_LPV };
+
+}  // namespace
+
+struct pcd_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 256;
+    int cap = 0;  // graph slots
+    char *arena = nullptr;
+    size_t arena_bytes = 0;
+    std::map<size_t, size_t> free_list;  // offset -> bytes
+    std::vector<Slot> slots;
+    BatchGraph *d_graphs = nullptr;
+    BatchState *d_state = nullptr;
+    BatchState *h_state = nullptr;  // pinned
+    int2 *d_gtiles = nullptr;
+    int2 *h_gtiles = nullptr;       // pinned
+    // item lists
+    void *d_items = nullptr, *h_items = nullptr;
+    size_t items_cap = 0;           // bytes of each
+    int32_t *d_counts = nullptr, *h_counts = nullptr;
+    size_t list_off[NUM_LISTS] = {0};
+    int list_len[NUM_LISTS] = {0};
+    int32_t *d_tile_sum = nullptr, *d_tile_pre = nullptr;
+    size_t tile_cap = 0;
+    bool items_dirty = true;
+    int64_t listed_done_vertices = 0, listed_vertices = 0;
+    // bin map and captured round
+    int map = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t rounds = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double lp_ms_last = 0;
+};
+
+static int bfail(int code, const std::string &msg) { return fail(code, msg); }
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static size_t footprint(int n, int64_t E, bool sorted) {
+    size_t nb = align256(4 * ((size_t)n + 1));
+    return nb /*indices*/ + align256(4 * (size_t)std::max<int64_t>(E, 1)) /*adj*/ + (sorted ? 0 : nb) /*order*/ +
+           4 * nb /*cur next first newid*/ + align256(16 * ((size_t)n + 1)) /*stats*/;
+}
+
+static bool arena_alloc(pcd_batch *b, size_t bytes, size_t *off) {
+    for (auto it = b->free_list.begin(); it != b->free_list.end(); ++it) {
+        if (it->second >= bytes) {
+            *off = it->first;
+            size_t rest = it->second - bytes;
+            size_t roff = it->first + bytes;
+            b->free_list.erase(it);
+            if (rest) b->free_list[roff] = rest;
+            return true;
+        }
+    }
+    return false;
+}
+
+static void arena_free(pcd_batch *b, size_t off, size_t bytes) {
+    auto it = b->free_list.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != b->free_list.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        b->free_list.erase(nx);
+    }
+    if (it != b->free_list.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            b->free_list.erase(it);
+        }
+    }
+}
+
+static void drop_graph(pcd_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+extern "C" int pcd_batch_map_count(void) { return NUM_MAPS; }
+extern "C" const char *pcd_batch_map_name(int map) { return (map >= 0 && map < NUM_MAPS) ? MAPS[map].name : ""; }
+
+extern "C" int pcd_batch_footprint(int n, int m, int64_t *bytes) {
+    if (!bytes || n <= 0 || m < 0) return bfail(PCD_E_INVALID, "bad footprint arguments");
+    *bytes = (int64_t)footprint(n, 2 * (int64_t)m, false);
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_destroy(pcd_batch *b) {
+    if (!b) return PCD_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    if (b->arena) (void)hipFree(b->arena);
+    if (b->d_graphs) (void)hipFree(b->d_graphs);
+    if (b->d_state) (void)hipFree(b->d_state);
+    if (b->d_gtiles) (void)hipFree(b->d_gtiles);
+    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_counts) (void)hipFree(b->d_counts);
+    if (b->d_tile_sum) (void)hipFree(b->d_tile_sum);
+    if (b->d_tile_pre) (void)hipFree(b->d_tile_pre);
+    if (b->h_state) (void)hipHostFree(b->h_state);
+    if (b->h_gtiles) (void)hipHostFree(b->h_gtiles);
+    if (b->h_items) (void)hipHostFree(b->h_items);
+    if (b->h_counts) (void)hipHostFree(b->h_counts);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_create(int device, int max_graphs, int64_t arena_bytes, pcd_batch **out) {
+    if (!out) return bfail(PCD_E_INVALID, "null output pointer");
+    *out = nullptr;
+    if (max_graphs <= 0 || arena_bytes <= 0) return bfail(PCD_E_INVALID, "max_graphs and arena_bytes must be > 0");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return bfail(PCD_E_NODEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= count) return bfail(PCD_E_INVALID, "device index out of range");
+    hipDeviceProp_t prop;
+    PCD_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return bfail(PCD_E_NODEVICE, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
+    PCD_HIP(hipSetDevice(device));
+    pcd_batch *b = new pcd_batch();
+    b->device = device;
+    b->cu_count = prop.multiProcessorCount;
+    b->cap = max_graphs;
+    b->slots.resize(max_graphs);
+    auto bad = [&](int code, const char *what) {
+        pcd_batch_destroy(b);
+        return bfail(code, what);
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bad(PCD_E_HIP, "stream");
+    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) return bad(PCD_E_HIP, "event");
+    b->arena_bytes = align256((size_t)arena_bytes);
+    if (hipMalloc((void **)&b->arena, b->arena_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return bad(PCD_E_NOMEM, "arena allocation failed");
+    }
+    b->free_list[0] = b->arena_bytes;
+    if (hipMalloc((void **)&b->d_graphs, sizeof(BatchGraph) * max_graphs) != hipSuccess ||
+        hipMalloc((void **)&b->d_state, sizeof(BatchState) * max_graphs) != hipSuccess ||
+        hipMalloc((void **)&b->d_gtiles, sizeof(int2) * max_graphs) != hipSuccess ||
+        hipMalloc((void **)&b->d_counts, sizeof(int32_t) * NUM_LISTS) != hipSuccess) {
+        (void)hipGetLastError();
+        return bad(PCD_E_NOMEM, "batch tables");
+    }
+    if (hipHostMalloc((void **)&b->h_state, sizeof(BatchState) * max_graphs, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_gtiles, sizeof(int2) * max_graphs, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_counts, sizeof(int32_t) * NUM_LISTS, hipHostMallocDefault) != hipSuccess)
+        return bad(PCD_E_NOMEM, "pinned batch tables");
+    for (int i = 0; i < max_graphs; i++) {
+        std::memset(&b->h_state[i], 0, sizeof(BatchState));
+        b->h_state[i].status = BG_FREE;
+    }
+    std::memset(b->h_gtiles, 0, sizeof(int2) * max_graphs);
+    std::memset(b->h_counts, 0, sizeof(int32_t) * NUM_LISTS);
+    if (hipMemcpyAsync(b->d_state, b->h_state, sizeof(BatchState) * max_graphs, hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(b->d_gtiles, b->h_gtiles, sizeof(int2) * max_graphs, hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(b->d_counts, b->h_counts, sizeof(int32_t) * NUM_LISTS, hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return bad(PCD_E_HIP, "batch table upload");
+    *out = b;
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices, const int32_t *adj,
+                             const int32_t *labels, const pcd_job *job, int *slot_out) {
+    if (!b || !job || !slot_out) return bfail(PCD_E_INVALID, "null argument");
+    *slot_out = -1;
+    if (n <= 0 || m <= 0 || !indices || !adj || !labels) return bfail(PCD_E_INVALID, "bad graph arguments");
+    if ((int64_t)2 * m > INT32_MAX) return bfail(PCD_E_INVALID, "2m exceeds int32");
+    if (job->rule.kind != PCD_RULE_MODULARITY && job->rule.kind != PCD_RULE_CPM)
+        return bfail(PCD_E_INVALID, "batches run modularity or CPM");
+    if (!(job->frac > 0.0)) return bfail(PCD_E_INVALID, "frac must be > 0");
+    if (job->tie != PCD_TIE_RANDOM && job->tie != PCD_TIE_LOWEST) return bfail(PCD_E_INVALID, "unknown tie order");
+    if (indices[0] != 0 || indices[n] != 2 * m) return bfail(PCD_E_FORMAT, "indices[0] != 0 or indices[n] != 2m");
+    int maxd = 0;
+    bool sorted = true;
+    for (int v = 0; v < n; v++) {
+        int d = indices[v + 1] - indices[v];
+        if (d < 0) return bfail(PCD_E_FORMAT, "indices must be monotone");
+        if (v > 0 && d > indices[v] - indices[v - 1]) sorted = false;
+        maxd = std::max(maxd, d);
+        for (int j = indices[v]; j < indices[v + 1]; j++) {
+            int u = adj[j];
+            if (u < 0 || u >= n || u == v || (j > indices[v] && adj[j - 1] >= u))
+                return bfail(PCD_E_FORMAT, "adjacency rows must be ascending, loop-free, with ids in [0,n)");
+        }
+        if (labels[v] < -1 || labels[v] >= n) return bfail(PCD_E_INVALID, "labels must be in [0,n) or NEW_LABEL");
+    }
+    if (maxd > 4096) return bfail(PCD_E_INVALID, "degree above 4096: run this graph in a plain context");
+    int s = -1;
+    for (int i = 0; i < b->cap; i++)
+        if (!b->slots[i].used) { s = i; break; }
+    if (s < 0) return bfail(PCD_E_NOMEM, "no free graph slot");
+    const int64_t E = 2 * (int64_t)m;
+    const size_t bytes = footprint(n, E, sorted);
+    size_t off = 0;
+    if (!arena_alloc(b, bytes, &off)) return bfail(PCD_E_NOMEM, "batch arena full");
+    Slot &sl = b->slots[s];
+    sl = Slot();
+    sl.used = true;
+    sl.status = BG_RUNNING;
+    sl.n = n;
+    sl.m = m;
+    sl.off = off;
+    sl.bytes = bytes;
+    sl.admitted = b->rounds;
+    // degree order (stable, descending) when the graph is not degree-sorted
+    std::vector<int32_t> order;
+    if (!sorted) {
+        std::vector<int64_t> bucket((size_t)maxd + 2, 0);
+        for (int v = 0; v < n; v++) bucket[maxd - (indices[v + 1] - indices[v]) + 1]++;
+        for (int d = 0; d <= maxd; d++) bucket[d + 1] += bucket[d];
+        order.resize(n);
+        for (int v = 0; v < n; v++) order[bucket[maxd - (indices[v + 1] - indices[v])]++] = v;
+    }
+    auto deg_at = [&](int p) {
+        int v = sorted ? p : order[p];
+        return indices[v + 1] - indices[v];
+    };
+    for (int i = 0; i < NUM_TH; i++) {
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            int mid = (lo + hi) / 2;
+            if (deg_at(mid) <= TH[i]) hi = mid;
+            else lo = mid + 1;
+        }
+        sl.pos_le[i] = lo;
+    }
+    for (int p = 0; p < sl.pos_le[th_index(64)]; p++) {
+        int v = sorted ? p : order[p];
+        for (int j = indices[v]; j < indices[v + 1]; j += ROW_CHUNK)
+            sl.rows.push_back(RowItem{s, v, j, std::min(indices[v + 1], j + ROW_CHUNK)});
+    }
+    // device layout
+    char *base = b->arena + off;
+    size_t nb = align256(4 * ((size_t)n + 1));
+    int32_t *d_ind = (int32_t *)base;
+    int32_t *d_adj = (int32_t *)(base + nb);
+    char *p = base + nb + align256(4 * (size_t)E);
+    int32_t *d_order = nullptr;
+    if (!sorted) {
+        d_order = (int32_t *)p;
+        p += nb;
+    }
+    int32_t *d_cur = (int32_t *)p;
+    int32_t *d_next = (int32_t *)(p + nb);
+    int32_t *d_first = (int32_t *)(p + 2 * nb);
+    int32_t *d_newid = (int32_t *)(p + 3 * nb);
+    CommStats *d_stats = (CommStats *)(p + 4 * nb);
+    PCD_HIP(hipMemcpyAsync(d_ind, indices, 4 * ((size_t)n + 1), hipMemcpyHostToDevice, b->stream));
+    PCD_HIP(hipMemcpyAsync(d_adj, adj, 4 * (size_t)E, hipMemcpyHostToDevice, b->stream));
+    if (!sorted) PCD_HIP(hipMemcpyAsync(d_order, order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    PCD_HIP(hipMemcpyAsync(d_next, labels, 4 * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    BatchGraph &g = sl.rec;
+    std::memset(&g, 0, sizeof(g));
+    g.lp.indices = d_ind;
+    g.lp.adj = d_adj;
+    g.lp.labels = d_cur;
+    g.lp.order = d_order;
+    g.lp.lv = nullptr;
+    g.lp.stats = d_stats;
+    g.lp.new_labels = d_next;
+    g.lp.rule.kind = job->rule.kind;
+    g.lp.rule.lambda = job->rule.lambda;
+    g.lp.rule.m = m;
+    g.lp.rule.cc = 0.0;
+    g.lp.seed = job->seed;
+    g.lp.step = 0;
+    g.lp.tie = job->tie;
+    g.lp.f = (int)((job->frac < 1.0 ? job->frac : 1.0) * 2147483647.0);  // refinement.cpp:158
+    g.lp.use_aux = 0;
+    g.lp.counters = nullptr;
+    g.next = d_next;
+    g.cur = d_cur;
+    g.stats = d_stats;
+    g.first = d_first;
+    g.newid = d_newid;
+    g.n = n;
+    g.max_iter = job->max_iter;
+    g.threshold = job->threshold;
+    g.iter_cap = job->iter_cap > 0 ? job->iter_cap : 1000;
+    BatchState &st = b->h_state[s];
+    std::memset(&st, 0, sizeof(st));
+    st.status = BG_RUNNING;
+    st.label_bound = n;
+    // pageable sources: the copies are staged before hipMemcpyAsync returns
+    PCD_HIP(hipMemcpyAsync(b->d_graphs + s, &g, sizeof(BatchGraph), hipMemcpyHostToDevice, b->stream));
+    PCD_HIP(hipMemcpyAsync(b->d_state + s, &st, sizeof(BatchState), hipMemcpyHostToDevice, b->stream));
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    b->items_dirty = true;
+    *slot_out = s;
+    return PCD_OK;
+}
+
+// Rebuilds the work-item lists from the running graphs and uploads them.
+// With settled_only, graphs admitted after the last round are left out
+// (their labels are not compressed yet).
+static int rebuild_items(pcd_batch *b, bool settled_only = false) {
+    std::vector<WorkItem> lists[NUM_LISTS];
+    std::vector<RowItem> rows;
+    const BinMap &mp = MAPS[b->map];
+    int64_t listed = 0;
+    for (int s = 0; s < b->cap; s++) {
+        Slot &sl = b->slots[s];
+        if (!sl.used || sl.status != BG_RUNNING || (settled_only && sl.admitted >= b->rounds)) {
+            b->h_gtiles[s] = make_int2(0, 0);
+            continue;
+        }
+        listed += sl.n;
+        const int nt = (sl.n + VT - 1) / VT;
+        b->h_gtiles[s] = make_int2((int)lists[L_TILES].size(), nt);
+        for (int t = 0; t < nt; t++) lists[L_TILES].push_back(WorkItem{s, t * VT, std::min(sl.n, (t + 1) * VT)});
+        auto push_range = [&](std::vector<WorkItem> &out, int p0, int p1, int per) {
+            for (int p = p0; p < p1; p += per) out.push_back(WorkItem{s, p, std::min(p1, p + per)});
+        };
+        // statistics classes: degree in (G/2, G] (G4 takes 0..4)
+        for (int i = 0; i < 5; i++) {
+            int G = STAT_G[i];
+            int p0 = sl.pos_le[th_index(G)];
+            int p1 = G == 4 ? sl.n : sl.pos_le[th_index(G / 2)];
+            push_range(lists[L_SMALL0 + i], p0, p1, STAT_CP[i]);
+        }
+        rows.insert(rows.end(), sl.rows.begin(), sl.rows.end());
+        // propagation by bin map
+        for (int i = 0; i < mp.n; i++) {
+            int p0 = sl.pos_le[th_index(mp.hi[i])];
+            int p1 = i + 1 < mp.n ? sl.pos_le[th_index(mp.hi[i + 1])] : sl.n;
+            push_range(lists[L_LP0 + mp.var[i]], p0, p1, LPV_ITEM[mp.var[i]]);
+        }
+    }
+    // layout
+    size_t off = 0;
+    for (int l = 0; l < NUM_LISTS; l++) {
+        size_t cnt = l == L_ROWS ? rows.size() : lists[l].size();
+        size_t esz = l == L_ROWS ? sizeof(RowItem) : sizeof(WorkItem);
+        b->list_off[l] = off;
+        b->list_len[l] = (int)cnt;
+        off = align256(off + std::max<size_t>(cnt, 1) * esz);
+    }
+    const size_t ntiles = std::max<size_t>(lists[L_TILES].size(), 1);
+    bool realloc_items = off > b->items_cap, realloc_tiles = ntiles > b->tile_cap;
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    if (realloc_items) {
+        drop_graph(b);
+        if (b->d_items) (void)hipFree(b->d_items);
+        if (b->h_items) (void)hipHostFree(b->h_items);
+        b->d_items = b->h_items = nullptr;
+        size_t cap = off + off / 4 + 4096;
+        if (hipMalloc(&b->d_items, cap) != hipSuccess || hipHostMalloc(&b->h_items, cap, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            b->items_cap = 0;
+            return bfail(PCD_E_NOMEM, "work item lists");
+        }
+        b->items_cap = cap;
+    }
+    if (realloc_tiles) {
+        drop_graph(b);
+        if (b->d_tile_sum) (void)hipFree(b->d_tile_sum);
+        if (b->d_tile_pre) (void)hipFree(b->d_tile_pre);
+        size_t cap = ntiles + ntiles / 4 + 64;
+        if (hipMalloc((void **)&b->d_tile_sum, 4 * cap) != hipSuccess ||
+            hipMalloc((void **)&b->d_tile_pre, 4 * cap) != hipSuccess) {
+            (void)hipGetLastError();
+            b->tile_cap = 0;
+            return bfail(PCD_E_NOMEM, "tile scan buffers");
+        }
+        b->tile_cap = cap;
+    }
+    // list offsets are fixed by the layout above, so a change of list
+    // lengths moves the buffers the captured launches point into
+    static_assert(NUM_LISTS < 64, "list count");
+    for (int l = 0; l < NUM_LISTS; l++) {
+        char *dst = (char *)b->h_items + b->list_off[l];
+        if (l == L_ROWS) std::memcpy(dst, rows.data(), rows.size() * sizeof(RowItem));
+        else std::memcpy(dst, lists[l].data(), lists[l].size() * sizeof(WorkItem));
+        b->h_counts[l] = b->list_len[l];
+    }
+    PCD_HIP(hipMemcpyAsync(b->d_items, b->h_items, off, hipMemcpyHostToDevice, b->stream));
+    PCD_HIP(hipMemcpyAsync(b->d_counts, b->h_counts, sizeof(int32_t) * NUM_LISTS, hipMemcpyHostToDevice, b->stream));
+    PCD_HIP(hipMemcpyAsync(b->d_gtiles, b->h_gtiles, sizeof(int2) * b->cap, hipMemcpyHostToDevice, b->stream));
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    b->items_dirty = false;
+    b->listed_vertices = listed;
+    b->listed_done_vertices = 0;
+    // offsets inside the item buffer moved: the captured round is stale
+    drop_graph(b);
+    return PCD_OK;
+}
+
+template <class T>
+static T *list_ptr(pcd_batch *b, int l) {
+    return (T *)((char *)b->d_items + b->list_off[l]);
+}
+
+static BatchLPLaunch lp_launch(pcd_batch *b) {
+    BatchLPLaunch l;
+    std::memset(&l, 0, sizeof(l));
+    const BinMap &mp = MAPS[b->map];
+    for (int v = 0; v < NUM_LPV; v++) {
+        l.items[v] = list_ptr<WorkItem>(b, L_LP0 + v);
+        l.count[v] = b->d_counts + L_LP0 + v;
+        l.used[v] = false;
+    }
+    for (int i = 0; i < mp.n; i++) l.used[mp.var[i]] = true;
+    l.graphs = b->d_graphs;
+    l.state = b->d_state;
+    l.cu_count = b->cu_count;
+    return l;
+}
+
+// The launches of one round (main.cpp:226-266 for every running graph).
+static int launch_round(pcd_batch *b) {
+    hipStream_t s = b->stream;
+    const int cu = b->cu_count;
+    const WorkItem *tiles = list_ptr<WorkItem>(b, L_TILES);
+    const int *ntile = b->d_counts + L_TILES;
+    const BatchGraph *gs = b->d_graphs;
+    BatchState *st = b->d_state;
+    // compress_labels (framework.cpp:146-170)
+    hipLaunchKernelGGL(k_b_fill_first, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
+    hipLaunchKernelGGL(k_b_first_occurrence, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
+    hipLaunchKernelGGL(k_b_tile_count, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st, b->d_tile_sum);
+    hipLaunchKernelGGL(k_b_tile_scan, dim3(1), dim3(1024), 0, s, ntile, b->d_tile_sum, b->d_tile_pre);
+    hipLaunchKernelGGL(k_b_tile_apply, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, b->d_gtiles, gs, st,
+                       b->d_tile_sum, b->d_tile_pre);
+    hipLaunchKernelGGL(k_b_relabel, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
+    // collect_statistics (framework.cpp:69-143)
+    hipLaunchKernelGGL(k_b_clear_stats, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
+    hipLaunchKernelGGL(k_b_stats_rows, dim3(cu * 8), dim3(256), 0, s, list_ptr<RowItem>(b, L_ROWS),
+                       b->d_counts + L_ROWS, gs, st);
+#define PCD_BSTATS(I, G)                                                                                    \
+    hipLaunchKernelGGL(k_b_stats_small<G>, dim3(cu * 8), dim3(256), 0, s, list_ptr<WorkItem>(b, L_SMALL0 + I), \
+                       b->d_counts + L_SMALL0 + I, gs, st);
+    PCD_BSTATS(0, 64)
+    PCD_BSTATS(1, 32)
+    PCD_BSTATS(2, 16)
+    PCD_BSTATS(3, 8)
+    PCD_BSTATS(4, 4)
+#undef PCD_BSTATS
+    // Rule::calculate and the stop rule
+    hipLaunchKernelGGL(k_b_metric, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
+    hipLaunchKernelGGL(k_b_finalize, dim3((b->cap + 255) / 256), dim3(256), 0, s, gs, st, b->cap);
+    // PropagationRefinement::refine for the graphs still running
+    PCD_TRY(batch_propagation_launch(s, lp_launch(b)));
+    PCD_HIP(hipGetLastError());
+    return PCD_OK;
+}
+
+static int ensure_graph(pcd_batch *b) {
+    if (b->items_dirty) PCD_TRY(rebuild_items(b));
+    if (b->exec) return PCD_OK;
+    PCD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_round(b);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc != PCD_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return bfail(PCD_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    b->graph = g;
+    PCD_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return PCD_OK;
+}
+
+static int poll_state(pcd_batch *b, int *running) {
+    PCD_HIP(hipMemcpyAsync(b->h_state, b->d_state, sizeof(BatchState) * b->cap, hipMemcpyDeviceToHost, b->stream));
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    int r = 0;
+    for (int s = 0; s < b->cap; s++) {
+        Slot &sl = b->slots[s];
+        if (!sl.used) continue;
+        int now = b->h_state[s].status;
+        if (sl.status == BG_RUNNING && now != BG_RUNNING) b->listed_done_vertices += sl.n;
+        sl.status = now;
+        if (now == BG_RUNNING) r++;
+    }
+    // retire finished graphs from the lists once they are a quarter of the work
+    if (b->listed_done_vertices * 4 > b->listed_vertices) b->items_dirty = true;
+    if (running) *running = r;
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_step(pcd_batch *b, int rounds, int *running) {
+    if (!b) return bfail(PCD_E_INVALID, "null batch");
+    if (rounds <= 0) return bfail(PCD_E_INVALID, "rounds must be positive");
+    PCD_HIP(hipSetDevice(b->device));
+    PCD_TRY(ensure_graph(b));
+    for (int r = 0; r < rounds; r++) PCD_HIP(hipGraphLaunch(b->exec, b->stream));
+    b->rounds += rounds;
+    return poll_state(b, running);
+}
+
+extern "C" int pcd_batch_status(pcd_batch *b, int slot, int *status, int *iterations, int *k, double *score) {
+    if (!b || slot < 0 || slot >= b->cap) return bfail(PCD_E_INVALID, "bad slot");
+    const Slot &sl = b->slots[slot];
+    const BatchState &st = b->h_state[slot];
+    if (status) *status = sl.used ? sl.status : BG_FREE;
+    if (iterations) *iterations = st.iter;
+    if (k) *k = st.k;
+    if (score) *score = st.score;
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_finished(pcd_batch *b, int *slots, int cap, int *count) {
+    if (!b || !count || (cap > 0 && !slots)) return bfail(PCD_E_INVALID, "bad arguments");
+    int c = 0;
+    for (int s = 0; s < b->cap; s++)
+        if (b->slots[s].used && b->slots[s].status == BG_DONE) {
+            if (c < cap) slots[c] = s;
+            c++;
+        }
+    *count = c;
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_labels(pcd_batch *b, int slot, int32_t *labels) {
+    if (!b || slot < 0 || slot >= b->cap || !labels) return bfail(PCD_E_INVALID, "bad arguments");
+    const Slot &sl = b->slots[slot];
+    if (!sl.used) return bfail(PCD_E_STATE, "slot is free");
+    if (sl.status != BG_DONE) return bfail(PCD_E_STATE, "graph is still running");
+    PCD_HIP(hipSetDevice(b->device));
+    PCD_HIP(hipMemcpyAsync(labels, sl.rec.cur, 4 * (size_t)sl.n, hipMemcpyDeviceToHost, b->stream));
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_stats(pcd_batch *b, int slot, int32_t *sizes, int32_t *int_volume, int32_t *ext_volume) {
+    if (!b || slot < 0 || slot >= b->cap) return bfail(PCD_E_INVALID, "bad arguments");
+    const Slot &sl = b->slots[slot];
+    if (!sl.used || sl.status != BG_DONE) return bfail(PCD_E_STATE, "graph is not finished");
+    const int k = b->h_state[slot].k;
+    std::vector<CommStats> st(std::max(k, 1));
+    PCD_HIP(hipSetDevice(b->device));
+    PCD_HIP(hipMemcpyAsync(st.data(), sl.rec.stats, sizeof(CommStats) * k, hipMemcpyDeviceToHost, b->stream));
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    for (int c = 0; c < k; c++) {
+        if (sizes) sizes[c] = st[c].size;
+        if (int_volume) int_volume[c] = st[c].int_vol;
+        if (ext_volume) ext_volume[c] = st[c].ext_vol;
+    }
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_release(pcd_batch *b, int slot) {
+    if (!b || slot < 0 || slot >= b->cap) return bfail(PCD_E_INVALID, "bad slot");
+    Slot &sl = b->slots[slot];
+    if (!sl.used) return bfail(PCD_E_STATE, "slot is free");
+    PCD_HIP(hipSetDevice(b->device));
+    BatchState &st = b->h_state[slot];
+    std::memset(&st, 0, sizeof(st));
+    st.status = BG_FREE;
+    // stream order: earlier rounds finish before the slot is reused
+    PCD_HIP(hipMemcpyAsync(b->d_state + slot, &st, sizeof(BatchState), hipMemcpyHostToDevice, b->stream));
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    if (sl.status == BG_RUNNING) b->items_dirty = true;
+    arena_free(b, sl.off, sl.bytes);
+    sl = Slot();
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_info(pcd_batch *b, int64_t *free_bytes, int *graphs, int *running, int64_t *rounds) {
+    if (!b) return bfail(PCD_E_INVALID, "null batch");
+    size_t fb = 0;
+    for (auto &kv : b->free_list) fb += kv.second;
+    int g = 0, r = 0;
+    for (auto &sl : b->slots) {
+        if (!sl.used) continue;
+        g++;
+        if (sl.status == BG_RUNNING) r++;
+    }
+    if (free_bytes) *free_bytes = (int64_t)fb;
+    if (graphs) *graphs = g;
+    if (running) *running = r;
+    if (rounds) *rounds = b->rounds;
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_set_map(pcd_batch *b, int map) {
+    if (!b || map < 0 || map >= NUM_MAPS) return bfail(PCD_E_INVALID, "bad bin map");
+    if (map != b->map) {
+        b->map = map;
+        b->items_dirty = true;
+        drop_graph(b);
+    }
+    return PCD_OK;
+}
+
+// Times the propagation phase of the current round under every bin map
+// and keeps the fastest.  Propagation reads the compressed labels and
+// statistics of the last completed round and writes only the next labels,
+// which it fully determines, so repeating it changes no state.  Needs one
+// completed round.
+extern "C" int pcd_batch_autotune(pcd_batch *b, int reps, int *chosen, double *ms_per_map) {
+    if (!b || reps <= 0) return bfail(PCD_E_INVALID, "bad arguments");
+    if (b->rounds == 0) return bfail(PCD_E_STATE, "autotune needs one completed round");
+    PCD_HIP(hipSetDevice(b->device));
+    int best = b->map;
+    double best_ms = 1e300;
+    for (int mp = 0; mp < NUM_MAPS; mp++) {
+        b->map = mp;
+        PCD_TRY(rebuild_items(b, true));
+        PCD_TRY(batch_propagation_launch(b->stream, lp_launch(b)));  // warm
+        double t = 1e300;
+        for (int r = 0; r < reps; r++) {
+            PCD_HIP(hipEventRecord(b->ev0, b->stream));
+            PCD_TRY(batch_propagation_launch(b->stream, lp_launch(b)));
+            PCD_HIP(hipEventRecord(b->ev1, b->stream));
+            PCD_HIP(hipEventSynchronize(b->ev1));
+            float ms = 0;
+            PCD_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            t = std::min(t, (double)ms);
+        }
+        if (ms_per_map) ms_per_map[mp] = t;
+        if (t < best_ms) {
+            best_ms = t;
+            best = mp;
+        }
+    }
+    b->map = best;
+    b->items_dirty = true;
+    drop_graph(b);
+    if (chosen) *chosen = best;
+    return PCD_OK;
+}

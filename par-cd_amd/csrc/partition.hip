@@ -72,12 +72,6 @@ int compress_labels_dev(pcd_ctx *ctx) {
 // framework.cpp:69-143: per vertex, count neighbours with the same label;
 // sizes[l] += 1, int_volume[l] += same, ext_volume[l] += deg - same.
 
-// size and int_vol are adjacent 32-bit fields: one 64-bit atomic adds both
-// (neither field can carry past 2^31).
-__device__ inline void add_stats(CommStats *st, int l, int size, int same, int ext) {
-    atomicAdd((unsigned long long *)&st[l].size, ((unsigned long long)(unsigned)same << 32) | (unsigned)size);
-    if (ext) atomicAdd(&st[l].ext_vol, ext);
-}
 
 __global__ void k_clear_stats(CommStats *st, const int32_t *kp) {
     const int k = *kp;

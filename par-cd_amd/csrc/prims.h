@@ -10,6 +10,13 @@
 
 namespace pcd {
 
+// Community statistics update: size and int_vol are adjacent 32-bit fields,
+// so one 64-bit atomic adds both (neither field can carry past 2^31).
+__device__ inline void add_stats(CommStats *st, int l, int size, int same, int ext) {
+    atomicAdd((unsigned long long *)&st[l].size, ((unsigned long long)(unsigned)same << 32) | (unsigned)size);
+    if (ext) atomicAdd(&st[l].ext_vol, ext);
+}
+
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ROUNDS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ROUNDS;

@@ -33,7 +33,7 @@
 //             the partitions and decides.  A partition whose distinct labels
 //             overflow its table flags the hub, which is then recounted in a
 //             global-memory hash table (never full).
-#include "prims.h"
+#include "work.h"
 
 namespace pcd {
 
@@ -67,22 +67,6 @@ struct HubPartial {
     int distinct;
 };
 static_assert(sizeof(HubPartial) <= HUB_PARTIAL_BYTES, "hub partial record size");
-
-struct LPArgs {
-    const int32_t *indices, *adj, *labels, *order;
-    const int2 *lv;  // per vertex {label, aux}
-    const CommStats *stats;
-    int32_t *new_labels;
-    RuleParams rule;
-    uint64_t seed;
-    int step;
-    int tie;
-    int f;        // selection threshold (refinement.cpp:158)
-    int use_aux;  // the rule's gains read only aux
-    unsigned long long *counters;  // {active, entries, distinct} of the bin, or nullptr
-};
-
-__device__ inline int vertex_at(const LPArgs &a, int p) { return a.order ? a.order[p] : p; }
 
 // Community record as the gains see it.  With aux: modularity reads
 // int_volume + ext_volume (= aux + 0), CPM reads size (= aux).
@@ -140,19 +124,23 @@ __global__ void k_lp_records(int n, const int32_t *labels, const CommStats *st, 
 }
 
 // ------------------------------------------------------------- G lanes / vertex
-template <int G, int CP, bool AUX>
-__global__ __launch_bounds__(256) void k_lp_small(int p0, int p1, LPArgs a) {
+template <int G, int CP, bool AUX, class Src>
+__global__ __launch_bounds__(256) void k_lp_small(Src src) {
     __shared__ int s_list[256 / WAVE][WAVE];
     const int lane = __lane_id(), glane = lane & (G - 1), w = threadIdx.x / WAVE;
     const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
     const int per_wave = WAVE / G;
-    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
     const int nwaves = gridDim.x * blockDim.x / WAVE;
+    const int nunits = src.units();
     int *list = s_list[w];
-    for (int cbase = p0 + wave_g * CP; cbase < p1; cbase += nwaves * CP) {
-        // compaction of the active vertices among CP consecutive positions
-        int p = cbase + lane;
-        bool valid = lane < CP && p < p1;
+    for (int u = wave_g; u < nunits; u += nwaves) {
+        LPArgs a;
+        int q0, q1;
+        if (!src.unit(u, a, q0, q1)) continue;
+        // compaction of the active vertices among (at most CP) consecutive positions
+        int p = q0 + lane;
+        bool valid = lane < CP && p < q1;
         int v = valid ? vertex_at(a, p) : 0;
         bool act = valid && selected(a.seed, a.step, v, a.f);
         if (valid && !act) a.new_labels[v] = a.labels[v];
@@ -394,15 +382,20 @@ __device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int 
 __device__ inline int ceil_log2(int x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
 
 // ------------------------------------------------------------- wave / vertex
-template <int TW>
-__global__ __launch_bounds__(256) void k_lp_wave(int p0, int p1, LPArgs a) {
+template <int TW, class Src>
+__global__ __launch_bounds__(256) void k_lp_wave(Src src) {
     __shared__ int s_keys[256 / WAVE][TW];
     __shared__ int s_cnts[256 / WAVE][TW];
     const int lane = __lane_id(), w = threadIdx.x / WAVE;
     int *keys = s_keys[w], *cnts = s_cnts[w];
-    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
     const int nwaves = gridDim.x * blockDim.x / WAVE;
-    for (int p = p0 + wave_g; p < p1; p += nwaves) {
+    const int nunits = src.units();
+    for (int u = wave_g; u < nunits; u += nwaves) {
+      LPArgs a;
+      int q0, q1;
+      if (!src.unit(u, a, q0, q1)) continue;
+      for (int p = q0; p < q1; p++) {
         int v = vertex_at(a, p);
         if (!selected(a.seed, a.step, v, a.f)) {
             if (lane == 0) a.new_labels[v] = a.labels[v];
@@ -426,18 +419,24 @@ __global__ __launch_bounds__(256) void k_lp_wave(int p0, int p1, LPArgs a) {
             if (lane == 0) count3(a, 1, deg, distinct);
         }
         wave_sync();
+      }
     }
 }
 
 // ------------------------------------------------------------- block / vertex
 // Table slots: nextpow2(2 deg), at most TB.
-template <int NT, int TB>
-__global__ __launch_bounds__(NT) void k_lp_block(int p0, int p1, LPArgs a) {
+template <int NT, int TB, class Src>
+__global__ __launch_bounds__(NT) void k_lp_block(Src src) {
     __shared__ int keys[TB];
     __shared__ int cnts[TB];
     __shared__ Cand s_c[NT / WAVE];
     __shared__ int s_i[2 * NT / WAVE];
-    for (int p = p0 + blockIdx.x; p < p1; p += gridDim.x) {
+    const int nunits = src.units();
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+      LPArgs a;
+      int q0, q1;
+      if (!src.unit(u, a, q0, q1)) continue;
+      for (int p = q0; p < q1; p++) {
         int v = vertex_at(a, p);
         if (!selected(a.seed, a.step, v, a.f)) {
             if (threadIdx.x == 0) a.new_labels[v] = a.labels[v];
@@ -459,6 +458,7 @@ __global__ __launch_bounds__(NT) void k_lp_block(int p0, int p1, LPArgs a) {
             a.new_labels[v] = lp_decide_stats(a, deg, own, own_cnt, c);
             if (a.counters) count3(a, 1, deg, distinct);
         }
+      }
     }
 }
 
@@ -778,31 +778,31 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
     }
     if (range(BIN_B4096, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_block<512, 8192>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0, ctx->stream,
-                           p0, p1, args_for(BIN_B4096));
+        hipLaunchKernelGGL((k_lp_block<512, 8192, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0,
+                           ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B4096)});
         kt_end(ctx, KS_LP_B4096);
     }
     if (range(BIN_B1024, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_block<256, 2048>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256), 0, ctx->stream,
-                           p0, p1, args_for(BIN_B1024));
+        hipLaunchKernelGGL((k_lp_block<256, 2048, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256), 0,
+                           ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
         kt_end(ctx, KS_LP_B1024);
     }
     if (range(BIN_W256, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL(k_lp_wave<512>, dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256), 0, ctx->stream, p0, p1,
-                           args_for(BIN_W256));
+        hipLaunchKernelGGL((k_lp_wave<512, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256), 0,
+                           ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_W256)});
         kt_end(ctx, KS_LP_W256);
     }
 #define PCD_SMALL(G, CP, BIN, SLOT)                                                                            \
     if (range(BIN, p0, p1)) {                                                                                  \
         kt_begin(ctx);                                                                                         \
         if (small_aux)                                                                                         \
-            hipLaunchKernelGGL((k_lp_small<G, CP, true>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)), dim3(256), \
-                               0, ctx->stream, p0, p1, args_for(BIN));                                         \
+            hipLaunchKernelGGL((k_lp_small<G, CP, true, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)),  \
+                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, CP, args_for(BIN)});                \
         else                                                                                                   \
-            hipLaunchKernelGGL((k_lp_small<G, CP, false>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)),           \
-                               dim3(256), 0, ctx->stream, p0, p1, args_for(BIN));                              \
+            hipLaunchKernelGGL((k_lp_small<G, CP, false, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)), \
+                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, CP, args_for(BIN)});                \
         kt_end(ctx, SLOT);                                                                                     \
     }
     PCD_SMALL(64, 8, BIN_G64, KS_LP_G64)
@@ -816,6 +816,38 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
     ctx->labels_compressed = false;
     ctx->has_stats = false;
     ctx->label_bound = ctx->k;
+    return PCD_OK;
+}
+
+// ------------------------------------------------------------- batches
+// One persistent launch per variant in use: a fixed grid whose waves or
+// workgroups stride over the variant's work items (the item count is read
+// on the device, so the launches can be captured once and replayed while
+// the lists change).  Small bins read labels and statistics (no records).
+int batch_propagation_launch(hipStream_t stream, const BatchLPLaunch &l) {
+    const int cu = l.cu_count;
+    auto src = [&](int v) { return ItemSrc{l.items[v], l.count[v], l.graphs, l.state}; };
+    if (l.used[LPV_B512X8192])
+        hipLaunchKernelGGL((k_lp_block<512, 8192, ItemSrc>), dim3(cu * 2), dim3(512), 0, stream, src(LPV_B512X8192));
+    if (l.used[LPV_B512])
+        hipLaunchKernelGGL((k_lp_block<512, 2048, ItemSrc>), dim3(cu * 4), dim3(512), 0, stream, src(LPV_B512));
+    if (l.used[LPV_B256])
+        hipLaunchKernelGGL((k_lp_block<256, 2048, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_B256));
+    if (l.used[LPV_W1024])
+        hipLaunchKernelGGL((k_lp_wave<1024, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W1024));
+    if (l.used[LPV_W512])
+        hipLaunchKernelGGL((k_lp_wave<512, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W512));
+    if (l.used[LPV_S64])
+        hipLaunchKernelGGL((k_lp_small<64, 8, false, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S64));
+    if (l.used[LPV_S32])
+        hipLaunchKernelGGL((k_lp_small<32, 16, false, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S32));
+    if (l.used[LPV_S16])
+        hipLaunchKernelGGL((k_lp_small<16, 32, false, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S16));
+    if (l.used[LPV_S8])
+        hipLaunchKernelGGL((k_lp_small<8, 64, false, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S8));
+    if (l.used[LPV_S4])
+        hipLaunchKernelGGL((k_lp_small<4, 64, false, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S4));
+    PCD_HIP(hipGetLastError());
     return PCD_OK;
 }
 

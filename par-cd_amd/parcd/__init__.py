@@ -27,7 +27,11 @@ EXPORTS = [
     "pcd_kernel_slot_count", "pcd_kernel_slot_name", "pcd_kernel_timer_reset", "pcd_kernel_timer",
     "pcd_bin_count", "pcd_bin_name", "pcd_bin_counters_reset", "pcd_bin_counters", "pcd_bin_size",
     "pcd_singleton_partition", "pcd_set_tuning", "pcd_hub_overflows", "pcd_copy_probe",
+    "pcd_batch_create", "pcd_batch_destroy", "pcd_batch_footprint", "pcd_batch_add", "pcd_batch_step",
+    "pcd_batch_status", "pcd_batch_finished", "pcd_batch_labels", "pcd_batch_stats", "pcd_batch_release",
+    "pcd_batch_info", "pcd_batch_map_count", "pcd_batch_map_name", "pcd_batch_set_map", "pcd_batch_autotune",
 ]
+BATCH_RUNNING, BATCH_DONE, BATCH_FREE = 0, 1, 2
 
 
 class PcdError(RuntimeError):
@@ -36,6 +40,17 @@ class PcdError(RuntimeError):
 
 class Rule(C.Structure):
     _fields_ = [("kind", C.c_int), ("lam", C.c_double)]
+
+
+class Job(C.Structure):
+    """pcd_job: the ./main options of one graph in a batch."""
+    _fields_ = [("rule", Rule), ("frac", C.c_double), ("threshold", C.c_double), ("max_iter", C.c_int),
+                ("tie", C.c_int), ("seed", C.c_uint64), ("iter_cap", C.c_int)]
+
+
+def job(kind=0, lam=0.0, frac=0.5, threshold=0.0, max_iter=-1, tie=0, seed=0, iter_cap=0):
+    return Job(Rule(int(kind), float(lam)), float(frac), float(threshold), int(max_iter), int(tie), int(seed),
+               int(iter_cap))
 
 
 _lib = None
@@ -101,6 +116,22 @@ def lib():
     L.pcd_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
     L.pcd_hub_overflows.argtypes = [vp, C.POINTER(C.c_int64)]
     L.pcd_copy_probe.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    ip, dp, lp = C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int64)
+    L.pcd_batch_create.argtypes = [C.c_int, C.c_int, C.c_int64, C.POINTER(vp)]
+    L.pcd_batch_destroy.argtypes = [vp]
+    L.pcd_batch_footprint.argtypes = [C.c_int, C.c_int, lp]
+    L.pcd_batch_add.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.POINTER(Job), ip]
+    L.pcd_batch_step.argtypes = [vp, C.c_int, ip]
+    L.pcd_batch_status.argtypes = [vp, C.c_int, ip, ip, ip, dp]
+    L.pcd_batch_finished.argtypes = [vp, vp, C.c_int, ip]
+    L.pcd_batch_labels.argtypes = [vp, C.c_int, vp]
+    L.pcd_batch_stats.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.pcd_batch_release.argtypes = [vp, C.c_int]
+    L.pcd_batch_info.argtypes = [vp, lp, ip, ip, lp]
+    L.pcd_batch_map_name.argtypes = [C.c_int]
+    L.pcd_batch_map_name.restype = C.c_char_p
+    L.pcd_batch_set_map.argtypes = [vp, C.c_int]
+    L.pcd_batch_autotune.argtypes = [vp, C.c_int, ip, dp]
     _lib = L
     return L
 
@@ -280,6 +311,30 @@ class Context:
     def propagate(self, kind, lam=0.0, frac=0.5, seed=0, step=0, tie=TIE_RANDOM):
         check(lib().pcd_propagation_step(self._ctx, rule(kind, lam), frac, seed, step, tie))
 
+    def run(self, n, m, indices, adj, labels, job_):
+        """The refinement loop of main.cpp:226-266 with -s 0 on one graph:
+        returns (labels, last score, iterations, communities)."""
+        self.upload_graph(n, m, indices, adj)
+        self.upload_labels(labels)
+        kind, lam = job_.rule.kind, job_.rule.lam
+        prev, it, k, score = 0.0, 0, 0, 0.0
+        cap = job_.iter_cap if job_.iter_cap > 0 else 1000
+        while True:
+            k = self.compress()
+            self.statistics()
+            score = self.metric(kind, lam)
+            if job_.max_iter != -1:
+                done = it >= job_.max_iter
+            else:
+                done = prev > 0 and score > 0 and score / prev - 1 <= job_.threshold
+            done = done or it >= cap
+            if not done:
+                self.propagate(kind, lam, frac=job_.frac, seed=job_.seed, step=it, tie=job_.tie)
+            it += 1
+            prev = score
+            if done:
+                return self.labels(), score, it, k
+
     def merge(self, kind, lam=0.0, rounds=1, seed=0, step=0, tie=TIE_RANDOM):
         check(lib().pcd_merging_step(self._ctx, rule(kind, lam), rounds, seed, step, tie))
 
@@ -365,3 +420,99 @@ class Context:
             check(lib().pcd_bin_size(self._ctx, i, C.byref(v), C.byref(e)))
             out[name] = (v.value, e.value)
         return out
+
+
+# ------------------------------------------------------------ batches
+def batch_map_names():
+    L = lib()
+    return [L.pcd_batch_map_name(i).decode() for i in range(L.pcd_batch_map_count())]
+
+
+def batch_footprint(n, m):
+    b = C.c_int64()
+    check(lib().pcd_batch_footprint(n, m, C.byref(b)))
+    return b.value
+
+
+class Batch:
+    """Many graphs refined together (pcd_batch_*)."""
+
+    def __init__(self, device=0, max_graphs=64, arena_bytes=1 << 30):
+        self._b = C.c_void_p()
+        check(lib().pcd_batch_create(device, int(max_graphs), int(arena_bytes), C.byref(self._b)))
+        self.sizes = {}
+
+    def close(self):
+        if self._b:
+            lib().pcd_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, n, m, indices, adj, labels, job_):
+        """Returns the slot, or None when the batch is full (PCD_E_NOMEM)."""
+        indices, adj, labels = _i32(indices), _i32(adj), _i32(labels)
+        slot = C.c_int()
+        rc = lib().pcd_batch_add(self._b, int(n), int(m), _p(indices), _p(adj), _p(labels), C.byref(job_),
+                                 C.byref(slot))
+        if rc == -3:
+            return None
+        check(rc)
+        self.sizes[slot.value] = int(n)
+        return slot.value
+
+    def step(self, rounds=1):
+        r = C.c_int()
+        check(lib().pcd_batch_step(self._b, int(rounds), C.byref(r)))
+        return r.value
+
+    def status(self, slot):
+        st, it, k, sc = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(lib().pcd_batch_status(self._b, slot, C.byref(st), C.byref(it), C.byref(k), C.byref(sc)))
+        return st.value, it.value, k.value, sc.value
+
+    def finished(self):
+        cnt = C.c_int()
+        check(lib().pcd_batch_finished(self._b, None, 0, C.byref(cnt)))
+        out = np.zeros(max(cnt.value, 1), np.int32)
+        check(lib().pcd_batch_finished(self._b, _p(out), len(out), C.byref(cnt)))
+        return [int(x) for x in out[: cnt.value]]
+
+    def labels(self, slot):
+        out = np.zeros(self.sizes[slot], np.int32)
+        check(lib().pcd_batch_labels(self._b, slot, _p(out)))
+        return out
+
+    def stats(self, slot):
+        k = self.status(slot)[2]
+        s, i, e = (np.zeros(max(k, 1), np.int32) for _ in range(3))
+        check(lib().pcd_batch_stats(self._b, slot, _p(s), _p(i), _p(e)))
+        return s[:k], i[:k], e[:k]
+
+    def release(self, slot):
+        check(lib().pcd_batch_release(self._b, slot))
+        self.sizes.pop(slot, None)
+
+    def info(self):
+        fb, g, r, rd = C.c_int64(), C.c_int(), C.c_int(), C.c_int64()
+        check(lib().pcd_batch_info(self._b, C.byref(fb), C.byref(g), C.byref(r), C.byref(rd)))
+        return {"free_bytes": fb.value, "graphs": g.value, "running": r.value, "rounds": rd.value}
+
+    def set_map(self, m):
+        check(lib().pcd_batch_set_map(self._b, int(m)))
+
+    def autotune(self, reps=3):
+        ch = C.c_int()
+        ms = (C.c_double * lib().pcd_batch_map_count())()
+        check(lib().pcd_batch_autotune(self._b, int(reps), C.byref(ch), ms))
+        return ch.value, list(ms)

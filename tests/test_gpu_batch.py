@@ -1,0 +1,155 @@
+"""Batched refinement on the GPU: every graph of a batch ends exactly where
+the same graph run alone ends (plain context loop and the serial replay),
+whatever the batch's composition, admission time or bin map."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_graphs import rmat, sbm, singleton_labels, unsorted_copy
+
+pytestmark = pytest.mark.gpu
+parcd = pytest.importorskip("parcd")
+
+
+def graphs():
+    """Mixed shapes: SBMs of several sizes, a small R-MAT (degrees up to
+    a few hundred), a graph that is not degree-sorted."""
+    out = [sbm(4, 50, 0.3, 0.01, seed=11), sbm(10, 100, 0.08, 0.002, seed=12), sbm(2, 400, 0.2, 0.01, seed=13),
+           rmat(10, seed=5), unsorted_copy(sbm(6, 80, 0.2, 0.01, seed=14), seed=9), sbm(3, 20, 0.5, 0.05, seed=15)]
+    return out
+
+
+JOBS = [
+    dict(kind=O.MOD, seed=1), dict(kind=O.CPM, lam=0.02, seed=2), dict(kind=O.MOD, seed=3, tie=O.TIE_LOWEST),
+    dict(kind=O.CPM, lam=0.005, seed=4, max_iter=3), dict(kind=O.MOD, seed=5, frac=0.8),
+    dict(kind=O.MOD, seed=6, threshold=0.01),
+]
+
+
+def plain(g, lab, jb):
+    n, m, ind, adj, _ = g
+    with parcd.Context(0) as ctx:
+        return ctx.run(n, m, ind, adj, lab, jb)
+
+
+def run_batch(items, map_=None, rounds=1):
+    with parcd.Batch(0, max_graphs=16, arena_bytes=256 << 20) as b:
+        if map_ is not None:
+            b.set_map(map_)
+        slots = [b.add(g[0], g[1], g[2], g[3], lab, jb) for g, lab, jb in items]
+        assert all(s is not None for s in slots)
+        while b.step(rounds):
+            pass
+        return [(b.labels(s),) + b.status(s)[1:] + (b.status(s)[0],) for s in slots]
+
+
+def test_batch_equals_plain_and_replay():
+    gs = graphs()
+    items = [(g, singleton_labels(g[0]), parcd.job(**jb)) for g, jb in zip(gs, JOBS)]
+    res = run_batch(items)
+    for (g, lab, jb), (bl, iters, k, score, status), jd in zip(items, res, JOBS):
+        assert status == parcd.BATCH_DONE
+        pl, ps, pit, pk = plain(g, lab, jb)
+        assert np.array_equal(bl, pl) and iters == pit and k == pk and score == ps
+        n, m, ind, adj, _ = g
+        ol, osc, oit, ok, _ = O.run(n, m, ind, adj, kind=jd["kind"], lam=jd.get("lam", 0.0),
+                                    frac=jd.get("frac", 0.5), max_iter=jd.get("max_iter", -1),
+                                    threshold=jd.get("threshold", 0.0), mode=1, seed=jd["seed"],
+                                    tie=jd.get("tie", O.TIE_RANDOM), init_labels=lab)
+        assert np.array_equal(bl, ol) and iters == oit and k == ok
+        assert abs(score - osc) <= 1e-9 * max(1.0, abs(osc))
+
+
+def test_result_independent_of_the_mix():
+    gs = graphs()
+    lab = [singleton_labels(g[0]) for g in gs]
+    jobs = [parcd.job(**jb) for jb in JOBS]
+    alone = run_batch([(gs[1], lab[1], jobs[1])])[0]
+    mixed = run_batch([(g, l, j) for g, l, j in zip(gs, lab, jobs)])[1]
+    assert np.array_equal(alone[0], mixed[0]) and alone[1:] == mixed[1:]
+
+
+def test_every_bin_map_gives_the_same_result():
+    gs = graphs()
+    items = [(g, singleton_labels(g[0]), parcd.job(**jb)) for g, jb in zip(gs, JOBS)]
+    ref = run_batch(items, map_=0)
+    for mp in range(1, len(parcd.batch_map_names())):
+        got = run_batch(items, map_=mp, rounds=3)
+        for a, b in zip(ref, got):
+            assert np.array_equal(a[0], b[0]) and a[1:] == b[1:], parcd.batch_map_names()[mp]
+
+
+def test_admission_between_rounds_and_slot_reuse():
+    gs = graphs()
+    lab = [singleton_labels(g[0]) for g in gs]
+    jobs = [parcd.job(**jb) for jb in JOBS]
+    expect = [plain(g, l, j) for g, l, j in zip(gs, lab, jobs)]
+    with parcd.Batch(0, max_graphs=3, arena_bytes=64 << 20) as b:
+        pending = list(range(len(gs)))
+        live = {}
+        done = {}
+        rounds = 0
+        while pending or live:
+            while pending:
+                i = pending[0]
+                s = b.add(gs[i][0], gs[i][1], gs[i][2], gs[i][3], lab[i], jobs[i])
+                if s is None:
+                    break
+                live[s] = i
+                pending.pop(0)
+            b.step(1)
+            rounds += 1
+            for s in b.finished():
+                done[live.pop(s)] = (b.labels(s), b.status(s))
+                b.release(s)
+            assert rounds < 10000
+    for i, (pl, ps, pit, pk) in enumerate(expect):
+        bl, (st, it, k, sc) = done[i]
+        assert np.array_equal(bl, pl) and it == pit and k == pk and sc == ps
+
+
+def test_autotune_keeps_results():
+    gs = graphs()
+    items = [(g, singleton_labels(g[0]), parcd.job(**jb)) for g, jb in zip(gs, JOBS)]
+    ref = run_batch(items)
+    with parcd.Batch(0, max_graphs=16, arena_bytes=256 << 20) as b:
+        slots = [b.add(g[0], g[1], g[2], g[3], lab, jb) for g, lab, jb in items]
+        b.step(1)
+        chosen, ms = b.autotune(reps=2)
+        assert 0 <= chosen < len(ms) and all(t > 0 for t in ms)
+        while b.step(2):
+            pass
+        for s, r in zip(slots, ref):
+            assert np.array_equal(b.labels(s), r[0]) and b.status(s)[1:] == r[1:4]
+
+
+def test_batch_statistics_match_plain():
+    g = sbm(10, 100, 0.08, 0.002, seed=12)
+    lab = singleton_labels(g[0])
+    jb = parcd.job(kind=O.MOD, seed=1)
+    with parcd.Batch(0, max_graphs=2, arena_bytes=64 << 20) as b:
+        s = b.add(g[0], g[1], g[2], g[3], lab, jb)
+        while b.step(1):
+            pass
+        bs = b.stats(s)
+        bl = b.labels(s)
+    k, sz, iv, ev = O.collect_statistics(g[0], g[2], g[3], bl)
+    assert k == len(bs[0])
+    assert np.array_equal(bs[0], sz) and np.array_equal(bs[1], iv) and np.array_equal(bs[2], ev)
+
+
+def test_batch_rejections():
+    with parcd.Batch(0, max_graphs=1, arena_bytes=1 << 20) as b:
+        g = sbm(4, 50, 0.3, 0.01, seed=11)
+        lab = singleton_labels(g[0])
+        with pytest.raises(parcd.PcdError):
+            b.add(g[0], g[1], g[2], g[3], lab, parcd.job(kind=O.WCC))
+        bad = lab.copy()
+        bad[0] = g[0]
+        with pytest.raises(parcd.PcdError):
+            b.add(g[0], g[1], g[2], g[3], bad, parcd.job())
+        big = sbm(2, 2000, 0.3, 0.01, seed=3)  # ~64 MB footprint > 1 MB arena
+        assert b.add(big[0], big[1], big[2], big[3], singleton_labels(big[0]), parcd.job()) is None
+        s = b.add(g[0], g[1], g[2], g[3], lab, parcd.job())
+        assert s == 0
+        assert b.add(g[0], g[1], g[2], g[3], lab, parcd.job()) is None  # no free slot
