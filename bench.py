@@ -153,6 +153,93 @@ def cpu_baseline(pcd_path, E, kind_arg, iters=2):
             "sec_per_iteration": per_iter, "lp_sec": lp[:n_it], "stats_sec": stats[:n_it], "host_cpu": cpu}
 
 
+ITER_CAP = 100
+
+
+def batch_leg(parcd, np, graphs_count, seed=4):
+    """Batched form: graphs_count SBM graphs (pcd_gen_sbm_batch, seed 4:
+    n log-uniform in [50, 20000], 2-64 equal blocks, p_in U[0.05, 0.3],
+    p_out U[0.001, 0.01], modularity or CPM with resolution U[0.005, 0.05]),
+    each refined to convergence (-s 0, default threshold) from the singleton
+    partition a fresh ./main process draws (srand(1)).  Timed end to end from
+    host CSR to host labels: (a) one batch -- admission, one round, the bin-map
+    autotuner, rounds until every graph stopped, label download; (b) the same
+    graphs one after another through one plain context (upload, pcd_refine,
+    download).  Quality: every graph's labels and score from (a) and (b) are
+    compared."""
+    t = time.time()
+    gs, prm = parcd.gen_sbm_batch(seed, graphs_count, pinned=True)
+    # graphs left without edges (tiny n with many blocks) have no score to
+    # refine (0/0 modularity): they are counted as skipped
+    keep = [i for i, g in enumerate(gs) if g[1] > 0]
+    skipped = len(gs) - len(keep)
+    gs = [gs[i] for i in keep]
+    prm = prm[keep]
+    E = sum(2 * g[1] for g in gs)
+    log("batch workload: %d graphs, %d vertices, %d entries (host gen %.1fs)" % (
+        len(gs), sum(g[0] for g in gs), E, time.time() - t))
+    labs = []
+    for g in gs:
+        libc_srand(1)
+        labs.append(parcd.singleton_partition(g[0]))
+    # graphs whose score never turns positive never meet the stop rule of
+    # main.cpp:241-242 (the reference loops forever); both paths stop them
+    # after ITER_CAP iterations
+    jobs = [parcd.job(kind=int(p[4]), lam=float(p[5]) if int(p[4]) == parcd.RULE_CPM else 0.0, seed=i,
+                      iter_cap=ITER_CAP) for i, p in zip(keep, prm)]
+    arena = sum(parcd.batch_footprint(g[0], g[1]) for g in gs) + (256 << 20)
+    # (a) batched
+    t0 = time.perf_counter()
+    b = parcd.Batch(0, max_graphs=len(gs), arena_bytes=arena)
+    slots = [b.add(g[0], g[1], g[2], g[3], lab, jb) for g, lab, jb in zip(gs, labs, jobs)]
+    assert all(s_ is not None for s_ in slots)
+    t_adm = time.perf_counter()
+    running = b.step(1)
+    chosen, map_ms = b.autotune(reps=2)
+    t_rounds0 = time.perf_counter()
+    while running:
+        running = b.step(1)
+    t_rounds1 = time.perf_counter()
+    res_b = [(b.labels(s_), b.status(s_)) for s_ in slots]
+    t1 = time.perf_counter()
+    rounds = b.info()["rounds"]
+    b.close()
+    log("batched: %.3f s (admission %.3f s, %d rounds, rounds after tuning %.3f s), bin map %s" % (
+        t1 - t0, t_adm - t0, rounds, t_rounds1 - t_rounds0, parcd.batch_map_names()[chosen]))
+    # (b) plain, one graph after another
+    ctx = parcd.Context(0)
+    t2 = time.perf_counter()
+    res_p = []
+    for g, lab, jb in zip(gs, labs, jobs):
+        ctx.upload_graph(g[0], g[1], g[2], g[3])
+        ctx.upload_labels(lab)
+        sc, it, k = ctx.refine(jb)
+        res_p.append((ctx.labels(), sc, it, k))
+    t3 = time.perf_counter()
+    ctx.close()
+    log("plain per graph: %.3f s" % (t3 - t2))
+    same = sum(1 for (bl, bs), (pl, ps, pit, pk) in zip(res_b, res_p)
+               if np.array_equal(bl, pl) and bs[0] == parcd.BATCH_DONE and bs[1:] == (pit, pk, ps))
+    mod_b = [bs[3] for (bl, bs), jb in zip(res_b, jobs) if jb.rule.kind == parcd.RULE_MODULARITY]
+    mod_p = [r[1] for r, jb in zip(res_p, jobs) if jb.rule.kind == parcd.RULE_MODULARITY]
+    iters = [r[2] for r in res_p]
+    return {
+        "graphs": len(gs), "skipped_without_edges": skipped, "vertices": int(sum(g[0] for g in gs)),
+        "entries": int(E),
+        "graphs_per_s": round(len(gs) / (t1 - t0), 2), "plain_graphs_per_s": round(len(gs) / (t3 - t2), 2),
+        "speedup_vs_plain": round((t3 - t2) / (t1 - t0), 2), "batched_s": round(t1 - t0, 3),
+        "plain_s": round(t3 - t2, 3), "admission_s": round(t_adm - t0, 3), "rounds": int(rounds),
+        "rounds_after_tuning_s": round(t_rounds1 - t_rounds0, 3),
+        "gteps_rounds": round(sum(2 * g[1] * r[2] for g, r in zip(gs, res_p)) / (t_rounds1 - t_rounds0) / 1e9, 2),
+        "bin_map": parcd.batch_map_names()[chosen], "autotune_ms": [round(x, 3) for x in map_ms],
+        "identical_results": same, "mean_modularity_batched": float(np.mean(mod_b)) if mod_b else None,
+        "mean_modularity_plain": float(np.mean(mod_p)) if mod_p else None,
+        "iterations_min_mean_max": [int(min(iters)), round(float(np.mean(iters)), 2), int(max(iters))],
+        "capped_at_%d_iterations" % ITER_CAP: int(sum(1 for x in iters if x > ITER_CAP)),
+        "timing": "end to end, host CSR in page-locked memory to labels on the host, both paths",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +251,8 @@ def main():
     ap.add_argument("--no-cpm", action="store_true", help="skip the CPM run")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="kernel-schedule tunable (pcd_set_tuning); results do not depend on it")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-form leg")
+    ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--no-replay", action="store_true",
                     help="skip the counter replay (for profiler runs: only the timed iterations execute)")
     args = ap.parse_args()
@@ -274,6 +363,12 @@ def main():
         log("cpm-0.01: %.3f ms/iteration, %.2f GTEPS" % (cpm["ms_per_step"], cpm["gteps"]))
     ctx.close()
 
+    # ---------------------------------------------------------- batched form
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(parcd, np, args.batch_graphs)
+        log("batch:", json.dumps(batch))
+
     # ---------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
@@ -296,7 +391,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "stream_copy_gbs": round(copy_gbs, 1),
         "speedup_vs_cpu": round(gteps / cpu["value"], 2) if cpu else None,
         "phases": {"propagate": lp_phase, "statistics": stats_phase, "kernels": per_kernel},
-        "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm,
+        "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm, "batch": batch,
     }
     print(json.dumps(out), flush=True)
 

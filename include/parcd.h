@@ -202,6 +202,12 @@ typedef struct pcd_job {
 #define PCD_BATCH_DONE 1
 #define PCD_BATCH_FREE 2
 
+/* The whole main.cpp:226-266 loop on the context's graph and current labels
+ * (-s strategy 0/1/2, -r rounds): on return the context holds the final
+ * compressed labels and statistics; *score receives the last score (what
+ * main.cpp:273 prints), *iterations the final `it`, *k the communities. */
+int pcd_refine(pcd_ctx *ctx, const pcd_job *job, int strategy, int rounds, double *score, int *iterations, int *k);
+
 /* `max_graphs` slots and an arena of `arena_bytes` device bytes. */
 int pcd_batch_create(int device, int max_graphs, int64_t arena_bytes, pcd_batch **out);
 int pcd_batch_destroy(pcd_batch *b);
@@ -236,6 +242,10 @@ int pcd_batch_set_map(pcd_batch *b, int map);
 int pcd_batch_autotune(pcd_batch *b, int reps, int *chosen, double *ms_per_map);
 
 /* --------------------------------------------------------- host utilities */
+/* Page-locked host memory (hipHostMalloc): graph arrays placed here upload
+ * at full link rate.  Free with pcd_host_free. */
+int pcd_host_alloc(int64_t bytes, void **p);
+int pcd_host_free(void *p);
 /* graph.cpp:40-105 Graph::read: reads a PCD file into malloc'ed arrays
  * (free with pcd_free).  Same validation: magic, sizes, no trailing bytes. */
 int pcd_read_pcd(const char *path, int *n, int *m, int32_t **indices, int32_t **adj, int32_t **map);
@@ -264,6 +274,15 @@ int pcd_gen_rmat(uint64_t seed, int scale, int64_t num_draws, double pa, double 
                  int64_t *num_edges, int32_t **a, int32_t **b, int *num_nodes);
 int pcd_gen_chung_lu(uint64_t seed, int n, double exponent, int dmin, int dmax, int num_comms, int cmin,
                      int cmax, double boost, int64_t *num_edges, int32_t **a, int32_t **b, int *num_nodes);
+
+/* The batched-form workload: `count` SBM graphs built in parallel (graph i
+ * from streams of (seed, i): log-uniform n, uniform block count, p_in,
+ * p_out, metric and CPM resolution), each converted like pcd_convert_edges.
+ * Concatenated CSR arrays with per-graph offsets; params holds 6 doubles per
+ * graph {nominal n, blocks, p_in, p_out, kind, lambda}.  Free with pcd_free. */
+int pcd_gen_sbm_batch(uint64_t seed, int count, int nmin, int nmax, int bmin, int bmax, double pin_lo, double pin_hi,
+                      double pout_lo, double pout_hi, double lam_lo, double lam_hi, int32_t **n, int32_t **m,
+                      int64_t **ind_off, int64_t **adj_off, int32_t **indices, int32_t **adj, double **params);
 
 #ifdef __cplusplus
 }

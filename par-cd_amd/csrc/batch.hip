@@ -26,10 +26,12 @@
 // per-graph vertex and label numbering); the kernels only change who
 // computes what.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <vector>
 
+#include "hostutil.hpp"
 #include "work.h"
 
 namespace pcd {
@@ -561,13 +563,20 @@ extern "C" int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices,
         if (d < 0) return bfail(PCD_E_FORMAT, "indices must be monotone");
         if (v > 0 && d > indices[v] - indices[v - 1]) sorted = false;
         maxd = std::max(maxd, d);
-        for (int j = indices[v]; j < indices[v + 1]; j++) {
-            int u = adj[j];
-            if (u < 0 || u >= n || u == v || (j > indices[v] && adj[j - 1] >= u))
-                return bfail(PCD_E_FORMAT, "adjacency rows must be ascending, loop-free, with ids in [0,n)");
-        }
         if (labels[v] < -1 || labels[v] >= n) return bfail(PCD_E_INVALID, "labels must be in [0,n) or NEW_LABEL");
     }
+    std::atomic<int> bad{0};
+    parallel_for(n, [&](int64_t lo, int64_t hi) {
+        for (int64_t v = lo; v < hi; v++)
+            for (int j = indices[v]; j < indices[v + 1]; j++) {
+                int u = adj[j];
+                if (u < 0 || u >= n || u == v || (j > indices[v] && adj[j - 1] >= u)) {
+                    bad = 1;
+                    return;
+                }
+            }
+    }, 2048);
+    if (bad) return bfail(PCD_E_FORMAT, "adjacency rows must be ascending, loop-free, with ids in [0,n)");
     if (maxd > 4096) return bfail(PCD_E_INVALID, "degree above 4096: run this graph in a plain context");
     int s = -1;
     for (int i = 0; i < b->cap; i++)
@@ -674,55 +683,65 @@ extern "C" int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices,
     return PCD_OK;
 }
 
-// Rebuilds the work-item lists from the running graphs and uploads them.
-// With settled_only, graphs admitted after the last round are left out
-// (their labels are not compressed yet).
-static int rebuild_items(pcd_batch *b, bool settled_only = false) {
-    std::vector<WorkItem> lists[NUM_LISTS];
-    std::vector<RowItem> rows;
+// Rebuilds the work-item lists from the running graphs and uploads them:
+// one counting pass fixes the layout, a second writes the items straight
+// into the pinned staging buffer.  With settled_only, graphs admitted after
+// the last round are left out (their labels are not compressed yet); with
+// lp_only, only the propagation lists (which depend on the bin map) are
+// rewritten, behind unchanged base lists.
+static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only = false, int stride = 1) {
     const BinMap &mp = MAPS[b->map];
+    auto listed_slot = [&](const Slot &sl) {
+        return sl.used && sl.status == BG_RUNNING && !(settled_only && sl.admitted >= b->rounds) &&
+               (stride == 1 || (&sl - b->slots.data()) % stride == 0);
+    };
+    auto nitems = [](int p0, int p1, int per) { return p1 > p0 ? (p1 - p0 + per - 1) / per : 0; };
+    auto lp_range = [&](const Slot &sl, int i, int &p0, int &p1) {
+        p0 = sl.pos_le[th_index(mp.hi[i])];
+        p1 = i + 1 < mp.n ? sl.pos_le[th_index(mp.hi[i + 1])] : sl.n;
+    };
+    auto small_range = [&](const Slot &sl, int i, int &p0, int &p1) {  // degree in (G/2, G], G4 takes 0..4
+        int G = STAT_G[i];
+        p0 = sl.pos_le[th_index(G)];
+        p1 = G == 4 ? sl.n : sl.pos_le[th_index(G / 2)];
+    };
+    // pass 1: counts
+    int64_t cnt[NUM_LISTS] = {0};
     int64_t listed = 0;
     for (int s = 0; s < b->cap; s++) {
-        Slot &sl = b->slots[s];
-        if (!sl.used || sl.status != BG_RUNNING || (settled_only && sl.admitted >= b->rounds)) {
-            b->h_gtiles[s] = make_int2(0, 0);
-            continue;
-        }
+        const Slot &sl = b->slots[s];
+        if (!listed_slot(sl)) continue;
         listed += sl.n;
-        const int nt = (sl.n + VT - 1) / VT;
-        b->h_gtiles[s] = make_int2((int)lists[L_TILES].size(), nt);
-        for (int t = 0; t < nt; t++) lists[L_TILES].push_back(WorkItem{s, t * VT, std::min(sl.n, (t + 1) * VT)});
-        auto push_range = [&](std::vector<WorkItem> &out, int p0, int p1, int per) {
-            for (int p = p0; p < p1; p += per) out.push_back(WorkItem{s, p, std::min(p1, p + per)});
-        };
-        // statistics classes: degree in (G/2, G] (G4 takes 0..4)
-        for (int i = 0; i < 5; i++) {
-            int G = STAT_G[i];
-            int p0 = sl.pos_le[th_index(G)];
-            int p1 = G == 4 ? sl.n : sl.pos_le[th_index(G / 2)];
-            push_range(lists[L_SMALL0 + i], p0, p1, STAT_CP[i]);
+        if (!lp_only) {
+            cnt[L_TILES] += (sl.n + VT - 1) / VT;
+            for (int i = 0; i < 5; i++) {
+                int p0, p1;
+                small_range(sl, i, p0, p1);
+                cnt[L_SMALL0 + i] += nitems(p0, p1, STAT_CP[i]);
+            }
+            cnt[L_ROWS] += (int64_t)sl.rows.size();
         }
-        rows.insert(rows.end(), sl.rows.begin(), sl.rows.end());
-        // propagation by bin map
         for (int i = 0; i < mp.n; i++) {
-            int p0 = sl.pos_le[th_index(mp.hi[i])];
-            int p1 = i + 1 < mp.n ? sl.pos_le[th_index(mp.hi[i + 1])] : sl.n;
-            push_range(lists[L_LP0 + mp.var[i]], p0, p1, LPV_ITEM[mp.var[i]]);
+            int p0, p1;
+            lp_range(sl, i, p0, p1);
+            cnt[L_LP0 + mp.var[i]] += nitems(p0, p1, LPV_ITEM[mp.var[i]]);
         }
     }
     // layout
-    size_t off = 0;
-    for (int l = 0; l < NUM_LISTS; l++) {
-        size_t cnt = l == L_ROWS ? rows.size() : lists[l].size();
+    const int first_list = lp_only ? L_LP0 : 0;
+    size_t off = lp_only ? align256(b->list_off[L_ROWS] + std::max<size_t>(b->list_len[L_ROWS], 1) * sizeof(RowItem))
+                         : 0;
+    size_t list_off[NUM_LISTS];
+    for (int l = first_list; l < NUM_LISTS; l++) {
+        if (cnt[l] > INT32_MAX) return bfail(PCD_E_NOMEM, "work item list too long");
         size_t esz = l == L_ROWS ? sizeof(RowItem) : sizeof(WorkItem);
-        b->list_off[l] = off;
-        b->list_len[l] = (int)cnt;
-        off = align256(off + std::max<size_t>(cnt, 1) * esz);
+        list_off[l] = off;
+        off = align256(off + std::max<int64_t>(cnt[l], 1) * esz);
     }
-    const size_t ntiles = std::max<size_t>(lists[L_TILES].size(), 1);
-    bool realloc_items = off > b->items_cap, realloc_tiles = ntiles > b->tile_cap;
-    PCD_HIP(hipStreamSynchronize(b->stream));
-    if (realloc_items) {
+    const size_t ntiles = lp_only ? b->tile_cap : std::max<int64_t>(cnt[L_TILES], 1);
+    PCD_HIP(hipStreamSynchronize(b->stream));  // the staging buffer may still feed a copy
+    if (off > b->items_cap) {
+        if (lp_only) return rebuild_items(b, settled_only, false, stride);
         drop_graph(b);
         if (b->d_items) (void)hipFree(b->d_items);
         if (b->h_items) (void)hipHostFree(b->h_items);
@@ -735,7 +754,7 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false) {
         }
         b->items_cap = cap;
     }
-    if (realloc_tiles) {
+    if (ntiles > b->tile_cap) {
         drop_graph(b);
         if (b->d_tile_sum) (void)hipFree(b->d_tile_sum);
         if (b->d_tile_pre) (void)hipFree(b->d_tile_pre);
@@ -748,22 +767,58 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false) {
         }
         b->tile_cap = cap;
     }
-    // list offsets are fixed by the layout above, so a change of list
-    // lengths moves the buffers the captured launches point into
-    static_assert(NUM_LISTS < 64, "list count");
-    for (int l = 0; l < NUM_LISTS; l++) {
-        char *dst = (char *)b->h_items + b->list_off[l];
-        if (l == L_ROWS) std::memcpy(dst, rows.data(), rows.size() * sizeof(RowItem));
-        else std::memcpy(dst, lists[l].data(), lists[l].size() * sizeof(WorkItem));
-        b->h_counts[l] = b->list_len[l];
+    for (int l = first_list; l < NUM_LISTS; l++) {
+        b->list_off[l] = list_off[l];
+        b->list_len[l] = (int)cnt[l];
+        b->h_counts[l] = (int)cnt[l];
     }
-    PCD_HIP(hipMemcpyAsync(b->d_items, b->h_items, off, hipMemcpyHostToDevice, b->stream));
+    // pass 2: fill
+    char *hb = (char *)b->h_items;
+    WorkItem *wp[NUM_LISTS];
+    for (int l = first_list; l < NUM_LISTS; l++) wp[l] = (WorkItem *)(hb + b->list_off[l]);
+    RowItem *rp = lp_only ? nullptr : (RowItem *)(hb + b->list_off[L_ROWS]);
+    int ntile = 0;
+    for (int s = 0; s < b->cap; s++) {
+        const Slot &sl = b->slots[s];
+        if (!listed_slot(sl)) {
+            if (!lp_only) b->h_gtiles[s] = make_int2(0, 0);
+            continue;
+        }
+        auto push_range = [&](int l, int p0, int p1, int per) {
+            for (int p = p0; p < p1; p += per) *wp[l]++ = WorkItem{s, p, std::min(p1, p + per)};
+        };
+        if (!lp_only) {
+            const int nt = (sl.n + VT - 1) / VT;
+            b->h_gtiles[s] = make_int2(ntile, nt);
+            ntile += nt;
+            push_range(L_TILES, 0, sl.n, VT);
+            for (int i = 0; i < 5; i++) {
+                int p0, p1;
+                small_range(sl, i, p0, p1);
+                push_range(L_SMALL0 + i, p0, p1, STAT_CP[i]);
+            }
+            if (!sl.rows.empty()) {
+                std::memcpy(rp, sl.rows.data(), sl.rows.size() * sizeof(RowItem));
+                rp += sl.rows.size();
+            }
+        }
+        for (int i = 0; i < mp.n; i++) {
+            int p0, p1;
+            lp_range(sl, i, p0, p1);
+            push_range(L_LP0 + mp.var[i], p0, p1, LPV_ITEM[mp.var[i]]);
+        }
+    }
+    const size_t up0 = lp_only ? b->list_off[L_LP0] : 0;
+    PCD_HIP(hipMemcpyAsync((char *)b->d_items + up0, hb + up0, off - up0, hipMemcpyHostToDevice, b->stream));
     PCD_HIP(hipMemcpyAsync(b->d_counts, b->h_counts, sizeof(int32_t) * NUM_LISTS, hipMemcpyHostToDevice, b->stream));
-    PCD_HIP(hipMemcpyAsync(b->d_gtiles, b->h_gtiles, sizeof(int2) * b->cap, hipMemcpyHostToDevice, b->stream));
+    if (!lp_only)
+        PCD_HIP(hipMemcpyAsync(b->d_gtiles, b->h_gtiles, sizeof(int2) * b->cap, hipMemcpyHostToDevice, b->stream));
     PCD_HIP(hipStreamSynchronize(b->stream));
-    b->items_dirty = false;
-    b->listed_vertices = listed;
-    b->listed_done_vertices = 0;
+    if (!lp_only) {
+        b->items_dirty = false;
+        b->listed_vertices = listed;
+        b->listed_done_vertices = 0;
+    }
     // offsets inside the item buffer moved: the captured round is stale
     drop_graph(b);
     return PCD_OK;
@@ -972,17 +1027,20 @@ extern "C" int pcd_batch_set_map(pcd_batch *b, int map) {
 // and keeps the fastest.  Propagation reads the compressed labels and
 // statistics of the last completed round and writes only the next labels,
 // which it fully determines, so repeating it changes no state.  Needs one
-// completed round.
+// completed round.  Large batches are timed on a sample of about 512 of
+// their running graphs (every stride-th slot).
 extern "C" int pcd_batch_autotune(pcd_batch *b, int reps, int *chosen, double *ms_per_map) {
     if (!b || reps <= 0) return bfail(PCD_E_INVALID, "bad arguments");
     if (b->rounds == 0) return bfail(PCD_E_STATE, "autotune needs one completed round");
     PCD_HIP(hipSetDevice(b->device));
+    int running = 0;
+    for (auto &sl : b->slots) running += sl.used && sl.status == BG_RUNNING;
+    const int stride = std::max(1, running / 512);
     int best = b->map;
     double best_ms = 1e300;
     for (int mp = 0; mp < NUM_MAPS; mp++) {
         b->map = mp;
-        PCD_TRY(rebuild_items(b, true));
-        PCD_TRY(batch_propagation_launch(b->stream, lp_launch(b)));  // warm
+        PCD_TRY(rebuild_items(b, true, mp > 0, stride));
         double t = 1e300;
         for (int r = 0; r < reps; r++) {
             PCD_HIP(hipEventRecord(b->ev0, b->stream));

@@ -726,6 +726,22 @@ extern "C" int pcd_copy_probe(pcd_ctx *ctx, int64_t bytes, int reps, double *gbs
     return PCD_OK;
 }
 
+extern "C" int pcd_host_alloc(int64_t bytes, void **p) {
+    if (!p || bytes <= 0) return fail(PCD_E_INVALID, "bad arguments");
+    *p = nullptr;
+    if (hipHostMalloc(p, (size_t)bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(PCD_E_NOMEM, "pinned host allocation failed");
+    }
+    return PCD_OK;
+}
+
+extern "C" int pcd_host_free(void *p) {
+    if (p) PCD_HIP(hipHostFree(p));
+    return PCD_OK;
+}
+
 extern "C" int pcd_bin_count(void) { return NUM_BINS; }
 
 extern "C" const char *pcd_bin_name(int bin) { return (bin >= 0 && bin < NUM_BINS) ? bin_names[bin] : ""; }

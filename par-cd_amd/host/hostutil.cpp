@@ -4,6 +4,7 @@
 // The converter produces the same bytes as the reference `convert`
 // (convert.cpp:76-278) but runs its per-row work on all host cores.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -18,6 +19,8 @@
 namespace pcd {
 
 thread_local std::string g_last_error;
+// set in workers that already run one task per thread: nested loops run serially
+static thread_local bool g_serial = false;
 
 int fail(int code, const std::string &msg) {
     g_last_error = msg;
@@ -35,7 +38,7 @@ int host_threads() {
 }
 
 void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &fn, int64_t grain) {
-    int nt = host_threads();
+    int nt = g_serial ? 1 : host_threads();
     if (n <= grain || nt <= 1) {
         if (n > 0) fn(0, n);
         return;
@@ -445,4 +448,88 @@ extern "C" int pcd_gen_chung_lu(uint64_t seed, int n, double expo, int dmin, int
     }
     *num_nodes = n;
     return emit(a, b, ne, oa, ob);
+}
+
+// Batched-form workload: `count` planted-partition SBM graphs.  Graph i
+// draws its shape from stream_seed(seed, i, 0): n log-uniform in
+// [nmin, nmax], a block count uniform in [bmin, bmax] (equal blocks, the
+// first n mod B one larger), p_in uniform in [pin_lo, pin_hi], p_out in
+// [pout_lo, pout_hi], then the metric (modularity or CPM, 1/2 each) and a
+// CPM resolution uniform in [lam_lo, lam_hi]; its edges come from
+// pcd_gen_sbm with seed stream_seed(seed, i, 1), converted as pcd_convert_edges
+// does.  Graphs are built in parallel, one per worker.  Outputs (malloc'ed,
+// free with pcd_free): per graph n, m, the offsets of its rows and entries in
+// the concatenated indices / adj arrays, and params[6 i ..] = {nominal n, B,
+// p_in, p_out, kind, lambda}.
+extern "C" int pcd_gen_sbm_batch(uint64_t seed, int count, int nmin, int nmax, int bmin, int bmax, double pin_lo,
+                                 double pin_hi, double pout_lo, double pout_hi, double lam_lo, double lam_hi,
+                                 int32_t **n_out, int32_t **m_out, int64_t **ind_off, int64_t **adj_off,
+                                 int32_t **indices, int32_t **adj, double **params) {
+    if (count <= 0 || nmin < 1 || nmax < nmin || bmin < 1 || bmax < bmin) return fail(PCD_E_INVALID, "bad batch shape");
+    struct G { int n = 0, m = 0; int32_t *ind = nullptr, *adj = nullptr; double prm[6]; int rc = PCD_OK; };
+    std::vector<G> gs(count);
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        g_serial = true;
+        for (int i; (i = next.fetch_add(1)) < count;) {
+            G &g = gs[i];
+            SplitMix r(stream_seed(seed, (uint64_t)i, 0));
+            double ln = std::log((double)nmin) + r.uniform() * (std::log((double)nmax) - std::log((double)nmin));
+            int nn = std::min(nmax, std::max(nmin, (int)std::lround(std::exp(ln))));
+            int B = bmin + (int)(r.next() % (uint64_t)(bmax - bmin + 1));
+            double pin = pin_lo + r.uniform() * (pin_hi - pin_lo);
+            double pout = pout_lo + r.uniform() * (pout_hi - pout_lo);
+            int kind = (r.next() >> 63) ? PCD_RULE_CPM : PCD_RULE_MODULARITY;
+            double lam = lam_lo + r.uniform() * (lam_hi - lam_lo);
+            g.prm[0] = nn; g.prm[1] = B; g.prm[2] = pin; g.prm[3] = pout; g.prm[4] = kind; g.prm[5] = lam;
+            std::vector<int32_t> bs(B, nn / B);
+            for (int b = 0; b < nn % B; b++) bs[b]++;
+            int64_t ne = 0;
+            int32_t *ea = nullptr, *eb = nullptr, *mp = nullptr;
+            int nodes = 0;
+            g.rc = pcd_gen_sbm(stream_seed(seed, (uint64_t)i, 1), B, bs.data(), pin, pout, &ne, &ea, &eb, &nodes);
+            if (g.rc == PCD_OK) g.rc = pcd_convert_edges(nodes, ne, ea, eb, &g.n, &g.m, &g.ind, &g.adj, &mp);
+            free(ea);
+            free(eb);
+            free(mp);
+        }
+        g_serial = false;
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < host_threads(); t++) th.emplace_back(worker);
+    for (auto &t : th) t.join();
+    int64_t tn = 0, te = 0;
+    int rc = PCD_OK;
+    for (auto &g : gs) {
+        if (g.rc != PCD_OK) rc = g.rc;
+        tn += g.n + 1;
+        te += 2 * (int64_t)g.m;
+    }
+    int32_t *on = xmalloc<int32_t>(count), *om = xmalloc<int32_t>(count), *oi = xmalloc<int32_t>((size_t)tn),
+            *oa = xmalloc<int32_t>((size_t)std::max<int64_t>(te, 1));
+    int64_t *oio = xmalloc<int64_t>(count), *oao = xmalloc<int64_t>(count);
+    double *op = xmalloc<double>((size_t)count * 6);
+    if (rc == PCD_OK && (!on || !om || !oi || !oa || !oio || !oao || !op)) rc = fail(PCD_E_NOMEM, "out of host memory");
+    if (rc == PCD_OK) {
+        int64_t pi = 0, pa = 0;
+        for (int i = 0; i < count; i++) {
+            G &g = gs[i];
+            on[i] = g.n;
+            om[i] = g.m;
+            oio[i] = pi;
+            oao[i] = pa;
+            memcpy(oi + pi, g.ind, 4 * ((size_t)g.n + 1));
+            if (g.m) memcpy(oa + pa, g.adj, 8 * (size_t)g.m);
+            memcpy(op + 6 * (size_t)i, g.prm, sizeof(g.prm));
+            pi += g.n + 1;
+            pa += 2 * (int64_t)g.m;
+        }
+    }
+    for (auto &g : gs) { free(g.ind); free(g.adj); }
+    if (rc != PCD_OK) {
+        free(on); free(om); free(oi); free(oa); free(oio); free(oao); free(op);
+        return rc;
+    }
+    *n_out = on; *m_out = om; *ind_off = oio; *adj_off = oao; *indices = oi; *adj = oa; *params = op;
+    return PCD_OK;
 }

@@ -30,6 +30,7 @@ EXPORTS = [
     "pcd_batch_create", "pcd_batch_destroy", "pcd_batch_footprint", "pcd_batch_add", "pcd_batch_step",
     "pcd_batch_status", "pcd_batch_finished", "pcd_batch_labels", "pcd_batch_stats", "pcd_batch_release",
     "pcd_batch_info", "pcd_batch_map_count", "pcd_batch_map_name", "pcd_batch_set_map", "pcd_batch_autotune",
+    "pcd_gen_sbm_batch", "pcd_refine", "pcd_host_alloc", "pcd_host_free",
 ]
 BATCH_RUNNING, BATCH_DONE, BATCH_FREE = 0, 1, 2
 
@@ -132,6 +133,12 @@ def lib():
     L.pcd_batch_map_name.restype = C.c_char_p
     L.pcd_batch_set_map.argtypes = [vp, C.c_int]
     L.pcd_batch_autotune.argtypes = [vp, C.c_int, ip, dp]
+    L.pcd_host_alloc.argtypes = [C.c_int64, C.POINTER(vp)]
+    L.pcd_host_free.argtypes = [vp]
+    L.pcd_refine.argtypes = [vp, C.POINTER(Job), C.c_int, C.c_int, dp, ip, ip]
+    L.pcd_gen_sbm_batch.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + \
+        [C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.POINTER(C.c_int64)), C.POINTER(C.POINTER(C.c_int64)),
+         C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.POINTER(C.c_double))]
     _lib = L
     return L
 
@@ -230,6 +237,60 @@ def make_graph(kind, seed, **kw):
     else:
         raise ValueError(kind)
     return convert_edges(nn, a, b)
+
+
+def gen_sbm_batch(seed, count, nmin=50, nmax=20000, bmin=2, bmax=64, pin=(0.05, 0.3), pout=(0.001, 0.01),
+                  lam=(0.005, 0.05), pinned=False):
+    """The batched-form workload (pcd_gen_sbm_batch): a list of
+    (n, m, indices, adj) views into two concatenated arrays (page-locked
+    with pinned=True), and the per-graph parameter rows {nominal n, blocks,
+    p_in, p_out, kind, lambda}."""
+    pn, pm, pio, pao, pi, pa = I32P(), I32P(), C.POINTER(C.c_int64)(), C.POINTER(C.c_int64)(), I32P(), I32P()
+    pp = C.POINTER(C.c_double)()
+    check(lib().pcd_gen_sbm_batch(seed, count, nmin, nmax, bmin, bmax, pin[0], pin[1], pout[0], pout[1], lam[0],
+                                  lam[1], C.byref(pn), C.byref(pm), C.byref(pio), C.byref(pao), C.byref(pi),
+                                  C.byref(pa), C.byref(pp)))
+    n = _take(pn, count)
+    m = _take(pm, count)
+    io = _take(pio, count)
+    ao = _take(pao, count)
+    params = _take(pp, 6 * count).reshape(count, 6)
+    tn = int(io[-1]) + int(n[-1]) + 1
+    te = int(ao[-1]) + 2 * int(m[-1])
+    if pinned:
+        ind, adj = pinned_empty(tn), pinned_empty(te)
+        ind[:] = np.ctypeslib.as_array(pi, (tn,))
+        adj[:] = np.ctypeslib.as_array(pa, (max(te, 1),))[:te]
+        lib().pcd_free(C.cast(pi, C.c_void_p))
+        lib().pcd_free(C.cast(pa, C.c_void_p))
+    else:
+        ind = _take(pi, tn)
+        adj = _take(pa, te)
+    graphs = [(int(n[i]), int(m[i]), ind[io[i]: io[i] + n[i] + 1], adj[ao[i]: ao[i] + 2 * m[i]]) for i in range(count)]
+    return graphs, params
+
+
+class _Pinned:
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def __del__(self):
+        try:
+            lib().pcd_host_free(self.ptr)
+        except Exception:
+            pass
+
+
+def pinned_empty(count, dtype=np.int32):
+    """A numpy array in page-locked host memory (pcd_host_alloc); the memory
+    lives as long as the array or any view of it."""
+    dtype = np.dtype(dtype)
+    p = C.c_void_p()
+    check(lib().pcd_host_alloc(max(1, int(count)) * dtype.itemsize, C.byref(p)))
+    holder = _Pinned(p)
+    buf = (C.c_char * (max(1, int(count)) * dtype.itemsize)).from_address(p.value)
+    buf._holder = holder
+    return np.frombuffer(buf, dtype=dtype, count=int(count))
 
 
 def singleton_partition(n):
@@ -334,6 +395,12 @@ class Context:
             prev = score
             if done:
                 return self.labels(), score, it, k
+
+    def refine(self, job_, strategy=0, rounds=1):
+        """pcd_refine on the uploaded graph and labels: (score, iterations, k)."""
+        sc, it, k = C.c_double(), C.c_int(), C.c_int()
+        check(lib().pcd_refine(self._ctx, C.byref(job_), strategy, rounds, C.byref(sc), C.byref(it), C.byref(k)))
+        return sc.value, it.value, k.value
 
     def merge(self, kind, lam=0.0, rounds=1, seed=0, step=0, tie=TIE_RANDOM):
         check(lib().pcd_merging_step(self._ctx, rule(kind, lam), rounds, seed, step, tie))

@@ -153,3 +153,20 @@ def test_batch_rejections():
         s = b.add(g[0], g[1], g[2], g[3], lab, parcd.job())
         assert s == 0
         assert b.add(g[0], g[1], g[2], g[3], lab, parcd.job()) is None  # no free slot
+
+
+@pytest.mark.parametrize("strategy", [0, 1, 2])
+def test_refine_call_equals_replay(strategy):
+    g = sbm(6, 60, 0.25, 0.01, seed=21)
+    n, m, ind, adj, _ = g
+    lab = singleton_labels(n)
+    jb = parcd.job(kind=O.MOD, seed=3, tie=O.TIE_LOWEST)
+    with parcd.Context(0) as ctx:
+        ctx.upload_graph(n, m, ind, adj)
+        ctx.upload_labels(lab)
+        score, iters, k = ctx.refine(jb, strategy=strategy)
+        got = ctx.labels()
+    ol, osc, oit, ok, _ = O.run(n, m, ind, adj, kind=O.MOD, strategy=strategy, mode=1, seed=3, tie=O.TIE_LOWEST,
+                                init_labels=lab)
+    assert np.array_equal(got, ol) and iters == oit and k == ok
+    assert abs(score - osc) <= 1e-9 * max(1.0, abs(osc))
