@@ -241,6 +241,40 @@ int pcd_batch_set_map(pcd_batch *b, int map);
  * keeps the fastest; ms_per_map (pcd_batch_map_count entries) may be NULL. */
 int pcd_batch_autotune(pcd_batch *b, int reps, int *chosen, double *ms_per_map);
 
+/* ---------------------------------------------------------------- serving */
+/* A request loop with continuous batching: a scheduler thread admits
+ * requests into a batch (pcd_batch_*) as slots and arena bytes free up, runs
+ * one round at a time for everything resident and retires finished graphs;
+ * requests the batch does not take (WCC, initial partition 1, degrees above
+ * 4096, graphs larger than the arena) run on worker threads with their own
+ * contexts and streams, admitted against plain_bytes.  Each result equals
+ * pcd_refine (-s 0) of the same request alone.  No reference counterpart. */
+typedef struct pcd_server pcd_server;
+typedef struct pcd_server_config {
+    int device;
+    int64_t batch_arena_bytes; /* device arena of the batch: its admission budget */
+    int batch_slots;           /* graphs resident in the batch at once */
+    int plain_workers;         /* worker threads, one context and stream each */
+    int64_t plain_bytes;       /* device-byte budget of the workers together */
+} pcd_server_config;
+
+int pcd_server_create(const pcd_server_config *cfg, pcd_server **out);
+/* Stops the server after the admitted work; unfinished requests are dropped. */
+int pcd_server_destroy(pcd_server *s);
+/* Queues a request.  indices/adj are borrowed until its result is taken;
+ * labels (n entries, copied) are the initial partition when initial == 0;
+ * initial == 1 builds create_clustering_partition (framework.cpp:27-59). */
+int pcd_server_submit(pcd_server *s, int n, int m, const int32_t *indices, const int32_t *adj, const int32_t *labels,
+                      const pcd_job *job, int initial, int64_t *id);
+/* Ids of finished requests (blocks up to timeout_ms for the first one). */
+int pcd_server_wait(pcd_server *s, int timeout_ms, int64_t *ids, int cap, int *count);
+/* Result of a finished request (returns its error code if it failed) and
+ * forgets it: labels (n entries), last score, iterations, communities,
+ * latency from submission to completion, route (0 batch, 1 worker). */
+int pcd_server_result(pcd_server *s, int64_t id, int32_t *labels, double *score, int *iterations, int *k,
+                      double *latency_ms, int *route);
+int pcd_server_info(pcd_server *s, int *queued, int *running_batch, int *running_plain, int64_t *rounds);
+
 /* --------------------------------------------------------- host utilities */
 /* Page-locked host memory (hipHostMalloc): graph arrays placed here upload
  * at full link rate.  Free with pcd_host_free. */
@@ -265,6 +299,11 @@ void pcd_free(void *p);
 /* framework.cpp:14-24 create_singleton_partition: identity permutation
  * shuffled with glibc rand() (the caller's generator state, as there). */
 int pcd_singleton_partition(int n, int32_t *labels);
+
+/* framework.cpp:27-59 create_clustering_partition from per-vertex triangle
+ * counts tri (as pcd_count_triangles returns them). */
+int pcd_clustering_partition(int n, const int32_t *indices, const int32_t *adj, const int32_t *tri,
+                             int32_t *labels);
 
 /* Seeded synthetic graphs (edge lists, loop-free; duplicates possible).
  * SplitMix64 streams; same seed -> same edges on any host. */

@@ -533,3 +533,38 @@ extern "C" int pcd_gen_sbm_batch(uint64_t seed, int count, int nmin, int nmax, i
     *n_out = on; *m_out = om; *ind_off = oio; *adj_off = oao; *indices = oi; *adj = oa; *params = op;
     return PCD_OK;
 }
+
+// framework.cpp:27-59 create_clustering_partition: vertices in order of
+// descending local clustering coefficient tri/(d(d-1)) (int32 product as
+// there), then descending degree, then id; each unlabelled vertex opens a
+// community and takes its unlabelled neighbours.
+extern "C" int pcd_clustering_partition(int n, const int32_t *indices, const int32_t *adj, const int32_t *tri,
+                                        int32_t *labels) {
+    if (n <= 0 || !indices || !adj || !tri || !labels) return fail(PCD_E_INVALID, "bad arguments");
+    struct Key {
+        double ncc;
+        int ndeg, v;
+    };
+    std::vector<Key> keys(n);
+    for (int v = 0; v < n; v++) {
+        int deg = indices[v + 1] - indices[v];
+        double cc = deg > 1 ? tri[v] / (double)(int)((unsigned)deg * (unsigned)(deg - 1)) : 0.0;
+        keys[v] = Key{-cc, -deg, v};
+    }
+    std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
+        if (a.ncc < b.ncc) return true;
+        if (b.ncc < a.ncc) return false;
+        if (a.ndeg != b.ndeg) return a.ndeg < b.ndeg;
+        return a.v < b.v;
+    });
+    int nc = 0;
+    for (int v = 0; v < n; v++) labels[v] = PCD_NEW_LABEL;
+    for (const Key &k : keys) {
+        if (labels[k.v] != PCD_NEW_LABEL) continue;
+        labels[k.v] = nc;
+        for (int j = indices[k.v]; j < indices[k.v + 1]; j++)
+            if (labels[adj[j]] == PCD_NEW_LABEL) labels[adj[j]] = nc;
+        nc++;
+    }
+    return PCD_OK;
+}

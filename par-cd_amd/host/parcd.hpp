@@ -128,31 +128,7 @@ inline void create_singleton_partition(const Graph &g, int32_t *labels) {
 // opens a community with its unlabelled neighbours.  Reads g.tri, computed
 // on the device by count_triangles (deg*(deg-1) is an int32 product there).
 inline void create_clustering_partition(const Graph &g, int32_t *labels) {
-    struct Key {
-        double ncc;
-        int ndeg, v;
-    };
-    std::vector<Key> keys(g.n);
-    for (int v = 0; v < g.n; v++) {
-        int deg = g.degree(v);
-        double cc = deg > 1 ? g.tri[v] / (double)pcd::wrap_mul(deg, deg - 1) : 0.0;
-        keys[v] = Key{-cc, -deg, v};
-    }
-    std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
-        if (a.ncc < b.ncc) return true;
-        if (b.ncc < a.ncc) return false;
-        if (a.ndeg != b.ndeg) return a.ndeg < b.ndeg;
-        return a.v < b.v;
-    });
-    int nc = 0;
-    for (int v = 0; v < g.n; v++) labels[v] = PCD_NEW_LABEL;
-    for (const Key &k : keys) {
-        if (labels[k.v] != PCD_NEW_LABEL) continue;
-        labels[k.v] = nc;
-        for (const int32_t *u = g.begin_neighbors(k.v); u != g.end_neighbors(k.v); u++)
-            if (labels[*u] == PCD_NEW_LABEL) labels[*u] = nc;
-        nc++;
-    }
+    check(pcd_clustering_partition(g.n, g.indices.data(), g.adj.data(), g.tri.data(), labels));
 }
 
 inline int compress_labels(const Graph &g, int32_t *labels) {

@@ -30,7 +30,9 @@ EXPORTS = [
     "pcd_batch_create", "pcd_batch_destroy", "pcd_batch_footprint", "pcd_batch_add", "pcd_batch_step",
     "pcd_batch_status", "pcd_batch_finished", "pcd_batch_labels", "pcd_batch_stats", "pcd_batch_release",
     "pcd_batch_info", "pcd_batch_map_count", "pcd_batch_map_name", "pcd_batch_set_map", "pcd_batch_autotune",
-    "pcd_gen_sbm_batch", "pcd_refine", "pcd_host_alloc", "pcd_host_free",
+    "pcd_gen_sbm_batch", "pcd_refine", "pcd_host_alloc", "pcd_host_free", "pcd_clustering_partition",
+    "pcd_server_create", "pcd_server_destroy", "pcd_server_submit", "pcd_server_wait", "pcd_server_result",
+    "pcd_server_info",
 ]
 BATCH_RUNNING, BATCH_DONE, BATCH_FREE = 0, 1, 2
 
@@ -41,6 +43,11 @@ class PcdError(RuntimeError):
 
 class Rule(C.Structure):
     _fields_ = [("kind", C.c_int), ("lam", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("batch_arena_bytes", C.c_int64), ("batch_slots", C.c_int),
+                ("plain_workers", C.c_int), ("plain_bytes", C.c_int64)]
 
 
 class Job(C.Structure):
@@ -135,6 +142,14 @@ def lib():
     L.pcd_batch_autotune.argtypes = [vp, C.c_int, ip, dp]
     L.pcd_host_alloc.argtypes = [C.c_int64, C.POINTER(vp)]
     L.pcd_host_free.argtypes = [vp]
+    L.pcd_clustering_partition.argtypes = [C.c_int, vp, vp, vp, vp]
+    L.pcd_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(vp)]
+    L.pcd_server_destroy.argtypes = [vp]
+    L.pcd_server_submit.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.POINTER(Job), C.c_int,
+                                    C.POINTER(C.c_int64)]
+    L.pcd_server_wait.argtypes = [vp, C.c_int, vp, C.c_int, ip]
+    L.pcd_server_result.argtypes = [vp, C.c_int64, vp, dp, ip, ip, dp, ip]
+    L.pcd_server_info.argtypes = [vp, ip, ip, ip, lp]
     L.pcd_refine.argtypes = [vp, C.POINTER(Job), C.c_int, C.c_int, dp, ip, ip]
     L.pcd_gen_sbm_batch.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + \
         [C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.POINTER(C.c_int64)), C.POINTER(C.POINTER(C.c_int64)),
@@ -583,3 +598,71 @@ class Batch:
         ms = (C.c_double * lib().pcd_batch_map_count())()
         check(lib().pcd_batch_autotune(self._b, int(reps), C.byref(ch), ms))
         return ch.value, list(ms)
+
+
+def clustering_partition(n, indices, adj, tri):
+    """framework.cpp:27-59 from triangle counts (pcd_clustering_partition)."""
+    out = np.zeros(n, np.int32)
+    check(lib().pcd_clustering_partition(n, _p(_i32(indices)), _p(_i32(adj)), _p(_i32(tri)), _p(out)))
+    return out
+
+
+# ------------------------------------------------------------ serving
+class Server:
+    """Continuous-batching request loop (pcd_server_*).  Keeps the arrays of
+    every request alive until its result is taken."""
+
+    def __init__(self, device=0, batch_arena_bytes=1 << 30, batch_slots=256, plain_workers=2, plain_bytes=8 << 30):
+        self._s = C.c_void_p()
+        cfg = ServerConfig(device, int(batch_arena_bytes), int(batch_slots), int(plain_workers), int(plain_bytes))
+        check(lib().pcd_server_create(C.byref(cfg), C.byref(self._s)))
+        self._keep = {}
+
+    def close(self):
+        if self._s:
+            lib().pcd_server_destroy(self._s)
+            self._s = C.c_void_p()
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, n, m, indices, adj, labels, job_, initial=0):
+        ind, adj_ = _i32(indices), _i32(adj)
+        lab = None if labels is None else _i32(labels)
+        rid = C.c_int64()
+        check(lib().pcd_server_submit(self._s, int(n), int(m), _p(ind), _p(adj_), _p(lab), C.byref(job_),
+                                      int(initial), C.byref(rid)))
+        self._keep[rid.value] = (ind, adj_, int(n))
+        return rid.value
+
+    def wait(self, timeout_ms=100, cap=4096):
+        ids = np.zeros(cap, np.int64)
+        cnt = C.c_int()
+        check(lib().pcd_server_wait(self._s, int(timeout_ms), _p(ids), cap, C.byref(cnt)))
+        return [int(x) for x in ids[: cnt.value]]
+
+    def result(self, rid):
+        """(labels, score, iterations, k, latency_ms, route)."""
+        n = self._keep[rid][2]
+        out = np.zeros(n, np.int32)
+        sc, it, k, lat, route = C.c_double(), C.c_int(), C.c_int(), C.c_double(), C.c_int()
+        rc = lib().pcd_server_result(self._s, rid, _p(out), C.byref(sc), C.byref(it), C.byref(k), C.byref(lat),
+                                     C.byref(route))
+        self._keep.pop(rid, None)
+        check(rc)
+        return out, sc.value, it.value, k.value, lat.value, route.value
+
+    def info(self):
+        q, rb, rp, rd = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+        check(lib().pcd_server_info(self._s, C.byref(q), C.byref(rb), C.byref(rp), C.byref(rd)))
+        return {"queued": q.value, "running_batch": rb.value, "running_plain": rp.value, "rounds": rd.value}
