@@ -240,6 +240,96 @@ def batch_leg(parcd, np, graphs_count, seed=4):
     }
 
 
+def serve_leg(parcd, np, count, m_lo, m_hi, qps, seed=5, sample=24):
+    """Serving form: `count` requests (pcd_gen_requests, seed 5: SBM or
+    R-MAT 1/2 each, target edges log-uniform in [m_lo, m_hi], modularity 60 %
+    / CPM 0.01 30 % / WCC with the triangle-based initial partition 10 %)
+    arriving as a Poisson stream at `qps` into one server (continuous
+    batching plus worker contexts).  Latency is submission to completion
+    inside the server.  A random sample of requests is re-run alone and
+    compared."""
+    import threading
+    t = time.time()
+    gs, prm = parcd.gen_requests(seed, count, m_lo, m_hi, pinned=True)
+    log("serve workload: %d requests, %d edges (host gen %.1fs)" % (count, sum(g[1] for g in gs), time.time() - t))
+    labs, jobs = [], []
+    for i, (g, p) in enumerate(zip(gs, prm)):
+        initial = int(p[4])
+        if initial == 0:
+            libc_srand(1)
+            labs.append(parcd.singleton_partition(g[0]))
+        else:
+            labs.append(None)
+        jobs.append(parcd.job(kind=int(p[2]), lam=float(p[3]), seed=i, iter_cap=ITER_CAP))
+    rng = np.random.default_rng(seed)
+    arrive = np.cumsum(rng.exponential(1.0 / qps, count))
+    srv = parcd.Server(0, batch_arena_bytes=24 << 30, batch_slots=2048, plain_workers=4, plain_bytes=64 << 30)
+    results = {}
+    keep_labels = set(int(x) for x in rng.choice(count, size=min(sample, count), replace=False))
+    done_t = {}
+    lock = threading.Lock()
+    rid_of = {}
+
+    def collect():
+        while len(results) < count:
+            for rid in srv.wait(100):
+                with lock:
+                    i = rid_of[rid]
+                try:
+                    lab, sc, it, k, lat, route = srv.result(rid)
+                except parcd.PcdError as e:
+                    log("request %d failed: %s" % (i, e))
+                    lab, sc, it, k, lat, route = None, float("nan"), -1, -1, float("nan"), -1
+                results[i] = (lab if i in keep_labels else None, sc, it, k, lat, route)
+                done_t[i] = time.perf_counter()
+
+    th = threading.Thread(target=collect)
+    t0 = time.perf_counter()
+    th.start()
+    for i, (g, lab, jb) in enumerate(zip(gs, labs, jobs)):
+        wait = t0 + arrive[i] - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        with lock:
+            rid = srv.submit(g[0], g[1], g[2], g[3], lab, jb, initial=0 if lab is not None else 1)
+            rid_of[rid] = i
+    t_sub = time.perf_counter()
+    th.join()
+    t1 = max(done_t.values())
+    info = srv.info()
+    srv.close()
+    failed = sum(1 for i in range(count) if results[i][5] < 0)
+    lat = np.array([results[i][4] for i in range(count) if results[i][5] >= 0])
+    routes = np.array([results[i][5] for i in range(count)])
+    # sample re-run alone
+    same = 0
+    with parcd.Context(0) as ctx:
+        for i in sorted(keep_labels):
+            g, jb = gs[i], jobs[i]
+            ctx.upload_graph(g[0], g[1], g[2], g[3])
+            lab = labs[i]
+            if lab is None or jb.rule.kind == parcd.RULE_WCC:
+                tri = ctx.triangles()[0]
+                if lab is None:
+                    lab = parcd.clustering_partition(g[0], g[2], g[3], tri)
+            ctx.upload_labels(lab)
+            sc, it, k = ctx.refine(jb)
+            r = results[i]
+            same += int(r[0] is not None and np.array_equal(ctx.labels(), r[0]) and (sc, it, k) == r[1:4])
+    edges = sum(g[1] for g in gs)
+    return {
+        "requests": count, "failed": failed, "target_qps": qps, "offered_span_s": round(float(arrive[-1]), 3),
+        "edges": int(edges), "m_range": [int(m_lo), int(m_hi)],
+        "graphs_per_s": round(count / (t1 - t0), 2), "edges_per_s": round(edges / (t1 - t0), 1),
+        "makespan_s": round(t1 - t0, 3), "submit_span_s": round(t_sub - t0, 3),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p90": round(float(np.percentile(lat, 90)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3), "max": round(float(lat.max()), 3)},
+        "routed_to_batch": int((routes == 0).sum()), "routed_to_workers": int((routes == 1).sum()),
+        "batch_rounds": int(info["rounds"]), "sample_identical_alone": "%d/%d" % (same, len(keep_labels)),
+        "capped_at_%d_iterations" % ITER_CAP: int(sum(1 for i in range(count) if results[i][2] > ITER_CAP)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +343,11 @@ def main():
                     help="kernel-schedule tunable (pcd_set_tuning); results do not depend on it")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--batch-graphs", type=int, default=4096)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-qps", type=float, default=1000.0)
+    ap.add_argument("--serve-m", type=float, nargs=2, default=[1e3, 1e6], metavar=("LO", "HI"),
+                    help="edge-count range of the requests (the full stream is 20000 requests, 1e3..5e7)")
     ap.add_argument("--no-replay", action="store_true",
                     help="skip the counter replay (for profiler runs: only the timed iterations execute)")
     args = ap.parse_args()
@@ -369,6 +464,12 @@ def main():
         batch = batch_leg(parcd, np, args.batch_graphs)
         log("batch:", json.dumps(batch))
 
+    # ---------------------------------------------------------- serving form
+    serve = None
+    if not args.no_serve:
+        serve = serve_leg(parcd, np, args.serve_requests, int(args.serve_m[0]), int(args.serve_m[1]), args.serve_qps)
+        log("serve:", json.dumps(serve))
+
     # ---------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
@@ -391,7 +492,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "stream_copy_gbs": round(copy_gbs, 1),
         "speedup_vs_cpu": round(gteps / cpu["value"], 2) if cpu else None,
         "phases": {"propagate": lp_phase, "statistics": stats_phase, "kernels": per_kernel},
-        "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm, "batch": batch,
+        "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm, "batch": batch, "serve": serve,
     }
     print(json.dumps(out), flush=True)
 

@@ -300,6 +300,14 @@ void pcd_free(void *p);
  * shuffled with glibc rand() (the caller's generator state, as there). */
 int pcd_singleton_partition(int n, int32_t *labels);
 
+/* The serving-form workload: `count` requests, each an SBM or R-MAT graph
+ * (1/2 each) with a log-uniform target edge count in [m_lo, m_hi] and a
+ * metric (modularity 0.6, CPM 0.01 0.3, WCC with initial partition 1 0.1),
+ * built in parallel as pcd_gen_sbm_batch builds its graphs.  params holds
+ * {family, target m, kind, lambda, initial, 0} per request. */
+int pcd_gen_requests(uint64_t seed, int count, int64_t m_lo, int64_t m_hi, int32_t **n, int32_t **m,
+                     int64_t **ind_off, int64_t **adj_off, int32_t **indices, int32_t **adj, double **params);
+
 /* framework.cpp:27-59 create_clustering_partition from per-vertex triangle
  * counts tri (as pcd_count_triangles returns them). */
 int pcd_clustering_partition(int n, const int32_t *indices, const int32_t *adj, const int32_t *tri,

@@ -391,6 +391,9 @@ struct pcd_batch {
     int32_t *d_counts = nullptr, *h_counts = nullptr;
     size_t list_off[NUM_LISTS] = {0};
     int list_len[NUM_LISTS] = {0};
+    int64_t list_cap[NUM_LISTS] = {0};  // items each list's region holds
+    std::vector<int> pending_append;    // admitted since the lists were built
+    std::vector<char> slot_listed;      // the slot has items in the lists
     int32_t *d_tile_sum = nullptr, *d_tile_pre = nullptr;
     size_t tile_cap = 0;
     bool items_dirty = true;
@@ -504,6 +507,7 @@ extern "C" int pcd_batch_create(int device, int max_graphs, int64_t arena_bytes,
     b->cu_count = prop.multiProcessorCount;
     b->cap = max_graphs;
     b->slots.resize(max_graphs);
+    b->slot_listed.assign(max_graphs, 0);
     auto bad = [&](int code, const char *what) {
         pcd_batch_destroy(b);
         return bfail(code, what);
@@ -678,32 +682,79 @@ extern "C" int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices,
     PCD_HIP(hipMemcpyAsync(b->d_graphs + s, &g, sizeof(BatchGraph), hipMemcpyHostToDevice, b->stream));
     PCD_HIP(hipMemcpyAsync(b->d_state + s, &st, sizeof(BatchState), hipMemcpyHostToDevice, b->stream));
     PCD_HIP(hipStreamSynchronize(b->stream));
-    b->items_dirty = true;
+    // items of the slot's previous graph may still be listed: rebuild then
+    if (b->items_cap == 0 || b->slot_listed[s]) b->items_dirty = true;
+    if (!b->items_dirty) b->pending_append.push_back(s);
     *slot_out = s;
     return PCD_OK;
 }
 
+// Items of one graph: vertex tiles, statistics classes (degree in (G/2, G],
+// G4 takes 0..4), long rows, and the propagation ranges of the bin map.
+static void lp_range(const BinMap &mp, const Slot &sl, int i, int &p0, int &p1) {
+    p0 = sl.pos_le[th_index(mp.hi[i])];
+    p1 = i + 1 < mp.n ? sl.pos_le[th_index(mp.hi[i + 1])] : sl.n;
+}
+static void small_range(const Slot &sl, int i, int &p0, int &p1) {
+    int G = STAT_G[i];
+    p0 = sl.pos_le[th_index(G)];
+    p1 = G == 4 ? sl.n : sl.pos_le[th_index(G / 2)];
+}
+static int64_t nitems(int p0, int p1, int per) { return p1 > p0 ? (p1 - p0 + per - 1) / per : 0; }
+
+static void slot_counts(const BinMap &mp, const Slot &sl, bool lp_only, int64_t *cnt) {
+    int p0, p1;
+    if (!lp_only) {
+        cnt[L_TILES] += (sl.n + VT - 1) / VT;
+        for (int i = 0; i < 5; i++) {
+            small_range(sl, i, p0, p1);
+            cnt[L_SMALL0 + i] += nitems(p0, p1, STAT_CP[i]);
+        }
+        cnt[L_ROWS] += (int64_t)sl.rows.size();
+    }
+    for (int i = 0; i < mp.n; i++) {
+        lp_range(mp, sl, i, p0, p1);
+        cnt[L_LP0 + mp.var[i]] += nitems(p0, p1, LPV_ITEM[mp.var[i]]);
+    }
+}
+
+static void slot_fill(const BinMap &mp, const Slot &sl, int s, bool lp_only, WorkItem **wp, RowItem *&rp) {
+    auto push_range = [&](int l, int p0, int p1, int per) {
+        for (int p = p0; p < p1; p += per) *wp[l]++ = WorkItem{s, p, std::min(p1, p + per)};
+    };
+    int p0, p1;
+    if (!lp_only) {
+        push_range(L_TILES, 0, sl.n, VT);
+        for (int i = 0; i < 5; i++) {
+            small_range(sl, i, p0, p1);
+            push_range(L_SMALL0 + i, p0, p1, STAT_CP[i]);
+        }
+        if (!sl.rows.empty()) {
+            std::memcpy(rp, sl.rows.data(), sl.rows.size() * sizeof(RowItem));
+            rp += sl.rows.size();
+        }
+    }
+    for (int i = 0; i < mp.n; i++) {
+        lp_range(mp, sl, i, p0, p1);
+        push_range(L_LP0 + mp.var[i], p0, p1, LPV_ITEM[mp.var[i]]);
+    }
+}
+
+static size_t list_esz(int l) { return l == L_ROWS ? sizeof(RowItem) : sizeof(WorkItem); }
+
 // Rebuilds the work-item lists from the running graphs and uploads them:
-// one counting pass fixes the layout, a second writes the items straight
+// one counting pass fixes the layout (every list with room to grow, so
+// later admissions append in place), a second writes the items straight
 // into the pinned staging buffer.  With settled_only, graphs admitted after
 // the last round are left out (their labels are not compressed yet); with
 // lp_only, only the propagation lists (which depend on the bin map) are
-// rewritten, behind unchanged base lists.
+// rewritten, behind unchanged base lists; stride > 1 lists every
+// stride-th slot only.
 static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only = false, int stride = 1) {
     const BinMap &mp = MAPS[b->map];
     auto listed_slot = [&](const Slot &sl) {
         return sl.used && sl.status == BG_RUNNING && !(settled_only && sl.admitted >= b->rounds) &&
                (stride == 1 || (&sl - b->slots.data()) % stride == 0);
-    };
-    auto nitems = [](int p0, int p1, int per) { return p1 > p0 ? (p1 - p0 + per - 1) / per : 0; };
-    auto lp_range = [&](const Slot &sl, int i, int &p0, int &p1) {
-        p0 = sl.pos_le[th_index(mp.hi[i])];
-        p1 = i + 1 < mp.n ? sl.pos_le[th_index(mp.hi[i + 1])] : sl.n;
-    };
-    auto small_range = [&](const Slot &sl, int i, int &p0, int &p1) {  // degree in (G/2, G], G4 takes 0..4
-        int G = STAT_G[i];
-        p0 = sl.pos_le[th_index(G)];
-        p1 = G == 4 ? sl.n : sl.pos_le[th_index(G / 2)];
     };
     // pass 1: counts
     int64_t cnt[NUM_LISTS] = {0};
@@ -712,33 +763,21 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
         const Slot &sl = b->slots[s];
         if (!listed_slot(sl)) continue;
         listed += sl.n;
-        if (!lp_only) {
-            cnt[L_TILES] += (sl.n + VT - 1) / VT;
-            for (int i = 0; i < 5; i++) {
-                int p0, p1;
-                small_range(sl, i, p0, p1);
-                cnt[L_SMALL0 + i] += nitems(p0, p1, STAT_CP[i]);
-            }
-            cnt[L_ROWS] += (int64_t)sl.rows.size();
-        }
-        for (int i = 0; i < mp.n; i++) {
-            int p0, p1;
-            lp_range(sl, i, p0, p1);
-            cnt[L_LP0 + mp.var[i]] += nitems(p0, p1, LPV_ITEM[mp.var[i]]);
-        }
+        slot_counts(mp, sl, lp_only, cnt);
     }
-    // layout
+    // layout with growth room
     const int first_list = lp_only ? L_LP0 : 0;
-    size_t off = lp_only ? align256(b->list_off[L_ROWS] + std::max<size_t>(b->list_len[L_ROWS], 1) * sizeof(RowItem))
+    size_t off = lp_only ? align256(b->list_off[L_ROWS] + std::max<int64_t>(b->list_cap[L_ROWS], 1) * sizeof(RowItem))
                          : 0;
     size_t list_off[NUM_LISTS];
+    int64_t list_cap[NUM_LISTS];
     for (int l = first_list; l < NUM_LISTS; l++) {
-        if (cnt[l] > INT32_MAX) return bfail(PCD_E_NOMEM, "work item list too long");
-        size_t esz = l == L_ROWS ? sizeof(RowItem) : sizeof(WorkItem);
+        if (cnt[l] > INT32_MAX / 2) return bfail(PCD_E_NOMEM, "work item list too long");
+        list_cap[l] = cnt[l] + std::max<int64_t>(cnt[l] / 2, 1024);
         list_off[l] = off;
-        off = align256(off + std::max<int64_t>(cnt[l], 1) * esz);
+        off = align256(off + list_cap[l] * list_esz(l));
     }
-    const size_t ntiles = lp_only ? b->tile_cap : std::max<int64_t>(cnt[L_TILES], 1);
+    const size_t ntiles = lp_only ? b->tile_cap : list_cap[L_TILES];
     PCD_HIP(hipStreamSynchronize(b->stream));  // the staging buffer may still feed a copy
     if (off > b->items_cap) {
         if (lp_only) return rebuild_items(b, settled_only, false, stride);
@@ -758,7 +797,7 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
         drop_graph(b);
         if (b->d_tile_sum) (void)hipFree(b->d_tile_sum);
         if (b->d_tile_pre) (void)hipFree(b->d_tile_pre);
-        size_t cap = ntiles + ntiles / 4 + 64;
+        size_t cap = ntiles + 64;
         if (hipMalloc((void **)&b->d_tile_sum, 4 * cap) != hipSuccess ||
             hipMalloc((void **)&b->d_tile_pre, 4 * cap) != hipSuccess) {
             (void)hipGetLastError();
@@ -769,6 +808,7 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
     }
     for (int l = first_list; l < NUM_LISTS; l++) {
         b->list_off[l] = list_off[l];
+        b->list_cap[l] = l == L_TILES ? std::min<int64_t>(list_cap[l], (int64_t)b->tile_cap) : list_cap[l];
         b->list_len[l] = (int)cnt[l];
         b->h_counts[l] = (int)cnt[l];
     }
@@ -780,33 +820,17 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
     int ntile = 0;
     for (int s = 0; s < b->cap; s++) {
         const Slot &sl = b->slots[s];
+        if (!lp_only) b->slot_listed[s] = listed_slot(sl) ? 1 : 0;
         if (!listed_slot(sl)) {
             if (!lp_only) b->h_gtiles[s] = make_int2(0, 0);
             continue;
         }
-        auto push_range = [&](int l, int p0, int p1, int per) {
-            for (int p = p0; p < p1; p += per) *wp[l]++ = WorkItem{s, p, std::min(p1, p + per)};
-        };
         if (!lp_only) {
             const int nt = (sl.n + VT - 1) / VT;
             b->h_gtiles[s] = make_int2(ntile, nt);
             ntile += nt;
-            push_range(L_TILES, 0, sl.n, VT);
-            for (int i = 0; i < 5; i++) {
-                int p0, p1;
-                small_range(sl, i, p0, p1);
-                push_range(L_SMALL0 + i, p0, p1, STAT_CP[i]);
-            }
-            if (!sl.rows.empty()) {
-                std::memcpy(rp, sl.rows.data(), sl.rows.size() * sizeof(RowItem));
-                rp += sl.rows.size();
-            }
         }
-        for (int i = 0; i < mp.n; i++) {
-            int p0, p1;
-            lp_range(sl, i, p0, p1);
-            push_range(L_LP0 + mp.var[i], p0, p1, LPV_ITEM[mp.var[i]]);
-        }
+        slot_fill(mp, sl, s, lp_only, wp, rp);
     }
     const size_t up0 = lp_only ? b->list_off[L_LP0] : 0;
     PCD_HIP(hipMemcpyAsync((char *)b->d_items + up0, hb + up0, off - up0, hipMemcpyHostToDevice, b->stream));
@@ -818,9 +842,48 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
         b->items_dirty = false;
         b->listed_vertices = listed;
         b->listed_done_vertices = 0;
+        b->pending_append.clear();
     }
     // offsets inside the item buffer moved: the captured round is stale
     drop_graph(b);
+    return PCD_OK;
+}
+
+// Appends the items of the graphs admitted since the last rebuild behind
+// the current lists (offsets unchanged, so the captured round stays valid);
+// falls back to a rebuild when a list is out of room.
+static int append_items(pcd_batch *b) {
+    const BinMap &mp = MAPS[b->map];
+    int64_t add[NUM_LISTS] = {0};
+    for (int s : b->pending_append) slot_counts(mp, b->slots[s], false, add);
+    for (int l = 0; l < NUM_LISTS; l++)
+        if (b->list_len[l] + add[l] > b->list_cap[l]) return rebuild_items(b);
+    PCD_HIP(hipStreamSynchronize(b->stream));  // the staging buffer may still feed a copy
+    char *hb = (char *)b->h_items;
+    WorkItem *wp[NUM_LISTS];
+    for (int l = 0; l < NUM_LISTS; l++) wp[l] = (WorkItem *)(hb + b->list_off[l] + b->list_len[l] * list_esz(l));
+    RowItem *rp = (RowItem *)(hb + b->list_off[L_ROWS] + (size_t)b->list_len[L_ROWS] * sizeof(RowItem));
+    int ntile = b->list_len[L_TILES];
+    for (int s : b->pending_append) {
+        const Slot &sl = b->slots[s];
+        const int nt = (sl.n + VT - 1) / VT;
+        b->h_gtiles[s] = make_int2(ntile, nt);
+        ntile += nt;
+        slot_fill(mp, sl, s, false, wp, rp);
+        b->slot_listed[s] = 1;
+        b->listed_vertices += sl.n;
+        PCD_HIP(hipMemcpyAsync(b->d_gtiles + s, b->h_gtiles + s, sizeof(int2), hipMemcpyHostToDevice, b->stream));
+    }
+    for (int l = 0; l < NUM_LISTS; l++) {
+        if (!add[l]) continue;
+        size_t o = b->list_off[l] + (size_t)b->list_len[l] * list_esz(l);
+        PCD_HIP(hipMemcpyAsync((char *)b->d_items + o, hb + o, add[l] * list_esz(l), hipMemcpyHostToDevice, b->stream));
+        b->list_len[l] += (int)add[l];
+        b->h_counts[l] = b->list_len[l];
+    }
+    PCD_HIP(hipMemcpyAsync(b->d_counts, b->h_counts, sizeof(int32_t) * NUM_LISTS, hipMemcpyHostToDevice, b->stream));
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    b->pending_append.clear();
     return PCD_OK;
 }
 
@@ -885,8 +948,11 @@ static int launch_round(pcd_batch *b) {
 
 static int ensure_graph(pcd_batch *b) {
     if (b->items_dirty) PCD_TRY(rebuild_items(b));
+    else if (!b->pending_append.empty()) PCD_TRY(append_items(b));
     if (b->exec) return PCD_OK;
-    PCD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    // relaxed: worker threads of a server keep allocating and copying on
+    // their own streams while this stream captures
+    PCD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
     int rc = launch_round(b);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);

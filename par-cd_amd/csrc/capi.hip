@@ -345,16 +345,16 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
 
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_indices, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_adj, E + 1));
-    PCD_HIP(hipMemcpy(ctx->d_indices, indices, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
-    if (E > 0) PCD_HIP(hipMemcpy(ctx->d_adj, adj, sizeof(int32_t) * E, hipMemcpyHostToDevice));
+    PCD_HIP(hipMemcpyAsync(ctx->d_indices, indices, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream));
+    if (E > 0) PCD_HIP(hipMemcpyAsync(ctx->d_adj, adj, sizeof(int32_t) * E, hipMemcpyHostToDevice, ctx->stream));
     if (!sorted) {
         PCD_TRY(dev_alloc_t(ctx, &ctx->d_order, n));
-        PCD_HIP(hipMemcpy(ctx->d_order, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+        PCD_HIP(hipMemcpyAsync(ctx->d_order, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
     }
     if (hub_end > 0) {
         auto up = [&](auto **dst, const auto &vec) -> int {
             PCD_TRY(dev_alloc_t(ctx, dst, vec.size()));
-            PCD_HIP(hipMemcpy(*dst, vec.data(), sizeof(vec[0]) * vec.size(), hipMemcpyHostToDevice));
+            PCD_HIP(hipMemcpyAsync(*dst, vec.data(), sizeof(vec[0]) * vec.size(), hipMemcpyHostToDevice, ctx->stream));
             return PCD_OK;
         };
         PCD_TRY(up(&ctx->d_hub_items, hub_chunks));
@@ -376,8 +376,8 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
     }
     if (!stat_items.empty()) {
         PCD_TRY(dev_alloc_t(ctx, &ctx->d_stat_items, stat_items.size()));
-        PCD_HIP(hipMemcpy(ctx->d_stat_items, stat_items.data(), sizeof(int4) * stat_items.size(),
-                          hipMemcpyHostToDevice));
+        PCD_HIP(hipMemcpyAsync(ctx->d_stat_items, stat_items.data(), sizeof(int4) * stat_items.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
     }
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_labels, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_labels_alt, n + 1));

@@ -5,6 +5,7 @@
 // (convert.cpp:76-278) but runs its per-row work on all host cores.
 #include <algorithm>
 #include <atomic>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -461,38 +462,23 @@ extern "C" int pcd_gen_chung_lu(uint64_t seed, int n, double expo, int dmin, int
 // free with pcd_free): per graph n, m, the offsets of its rows and entries in
 // the concatenated indices / adj arrays, and params[6 i ..] = {nominal n, B,
 // p_in, p_out, kind, lambda}.
-extern "C" int pcd_gen_sbm_batch(uint64_t seed, int count, int nmin, int nmax, int bmin, int bmax, double pin_lo,
-                                 double pin_hi, double pout_lo, double pout_hi, double lam_lo, double lam_hi,
-                                 int32_t **n_out, int32_t **m_out, int64_t **ind_off, int64_t **adj_off,
-                                 int32_t **indices, int32_t **adj, double **params) {
-    if (count <= 0 || nmin < 1 || nmax < nmin || bmin < 1 || bmax < bmin) return fail(PCD_E_INVALID, "bad batch shape");
-    struct G { int n = 0, m = 0; int32_t *ind = nullptr, *adj = nullptr; double prm[6]; int rc = PCD_OK; };
-    std::vector<G> gs(count);
+namespace {
+struct GenGraph {
+    int n = 0, m = 0;
+    int32_t *ind = nullptr, *adj = nullptr;
+    double prm[6] = {0, 0, 0, 0, 0, 0};
+    int rc = PCD_OK;
+};
+
+// Builds `count` graphs on all host threads (one graph per worker at a
+// time; nested loops run serially) and concatenates them.
+int gen_many(int count, const std::function<void(int, GenGraph &)> &make, int32_t **n_out, int32_t **m_out,
+             int64_t **ind_off, int64_t **adj_off, int32_t **indices, int32_t **adj, double **params) {
+    std::vector<GenGraph> gs(count);
     std::atomic<int> next{0};
     auto worker = [&]() {
         g_serial = true;
-        for (int i; (i = next.fetch_add(1)) < count;) {
-            G &g = gs[i];
-            SplitMix r(stream_seed(seed, (uint64_t)i, 0));
-            double ln = std::log((double)nmin) + r.uniform() * (std::log((double)nmax) - std::log((double)nmin));
-            int nn = std::min(nmax, std::max(nmin, (int)std::lround(std::exp(ln))));
-            int B = bmin + (int)(r.next() % (uint64_t)(bmax - bmin + 1));
-            double pin = pin_lo + r.uniform() * (pin_hi - pin_lo);
-            double pout = pout_lo + r.uniform() * (pout_hi - pout_lo);
-            int kind = (r.next() >> 63) ? PCD_RULE_CPM : PCD_RULE_MODULARITY;
-            double lam = lam_lo + r.uniform() * (lam_hi - lam_lo);
-            g.prm[0] = nn; g.prm[1] = B; g.prm[2] = pin; g.prm[3] = pout; g.prm[4] = kind; g.prm[5] = lam;
-            std::vector<int32_t> bs(B, nn / B);
-            for (int b = 0; b < nn % B; b++) bs[b]++;
-            int64_t ne = 0;
-            int32_t *ea = nullptr, *eb = nullptr, *mp = nullptr;
-            int nodes = 0;
-            g.rc = pcd_gen_sbm(stream_seed(seed, (uint64_t)i, 1), B, bs.data(), pin, pout, &ne, &ea, &eb, &nodes);
-            if (g.rc == PCD_OK) g.rc = pcd_convert_edges(nodes, ne, ea, eb, &g.n, &g.m, &g.ind, &g.adj, &mp);
-            free(ea);
-            free(eb);
-            free(mp);
-        }
+        for (int i; (i = next.fetch_add(1)) < count;) make(i, gs[i]);
         g_serial = false;
     };
     std::vector<std::thread> th;
@@ -513,12 +499,13 @@ extern "C" int pcd_gen_sbm_batch(uint64_t seed, int count, int nmin, int nmax, i
     if (rc == PCD_OK) {
         int64_t pi = 0, pa = 0;
         for (int i = 0; i < count; i++) {
-            G &g = gs[i];
+            GenGraph &g = gs[i];
             on[i] = g.n;
             om[i] = g.m;
             oio[i] = pi;
             oao[i] = pa;
-            memcpy(oi + pi, g.ind, 4 * ((size_t)g.n + 1));
+            if (g.ind) memcpy(oi + pi, g.ind, 4 * ((size_t)g.n + 1));
+            else oi[pi] = 0;
             if (g.m) memcpy(oa + pa, g.adj, 8 * (size_t)g.m);
             memcpy(op + 6 * (size_t)i, g.prm, sizeof(g.prm));
             pi += g.n + 1;
@@ -532,6 +519,99 @@ extern "C" int pcd_gen_sbm_batch(uint64_t seed, int count, int nmin, int nmax, i
     }
     *n_out = on; *m_out = om; *ind_off = oio; *adj_off = oao; *indices = oi; *adj = oa; *params = op;
     return PCD_OK;
+}
+
+void convert_into(GenGraph &g, int nodes, int64_t ne, int32_t *ea, int32_t *eb) {
+    int32_t *mp = nullptr;
+    g.rc = pcd_convert_edges(nodes, ne, ea, eb, &g.n, &g.m, &g.ind, &g.adj, &mp);
+    free(mp);
+}
+}  // namespace
+
+// Batched-form workload: `count` planted-partition SBM graphs.  Graph i
+// draws its shape from stream_seed(seed, i, 0): n log-uniform in
+// [nmin, nmax], a block count uniform in [bmin, bmax] (equal blocks, the
+// first n mod B one larger), p_in uniform in [pin_lo, pin_hi], p_out in
+// [pout_lo, pout_hi], then the metric (modularity or CPM, 1/2 each) and a
+// CPM resolution uniform in [lam_lo, lam_hi]; its edges come from
+// pcd_gen_sbm with seed stream_seed(seed, i, 1), converted as pcd_convert_edges
+// does.  Outputs (malloc'ed, free with pcd_free): per graph n, m, the offsets
+// of its rows and entries in the concatenated indices / adj arrays, and
+// params[6 i ..] = {nominal n, B, p_in, p_out, kind, lambda}.
+extern "C" int pcd_gen_sbm_batch(uint64_t seed, int count, int nmin, int nmax, int bmin, int bmax, double pin_lo,
+                                 double pin_hi, double pout_lo, double pout_hi, double lam_lo, double lam_hi,
+                                 int32_t **n_out, int32_t **m_out, int64_t **ind_off, int64_t **adj_off,
+                                 int32_t **indices, int32_t **adj, double **params) {
+    if (count <= 0 || nmin < 1 || nmax < nmin || bmin < 1 || bmax < bmin) return fail(PCD_E_INVALID, "bad batch shape");
+    auto make = [&](int i, GenGraph &g) {
+        SplitMix r(stream_seed(seed, (uint64_t)i, 0));
+        double ln = std::log((double)nmin) + r.uniform() * (std::log((double)nmax) - std::log((double)nmin));
+        int nn = std::min(nmax, std::max(nmin, (int)std::lround(std::exp(ln))));
+        int B = bmin + (int)(r.next() % (uint64_t)(bmax - bmin + 1));
+        double pin = pin_lo + r.uniform() * (pin_hi - pin_lo);
+        double pout = pout_lo + r.uniform() * (pout_hi - pout_lo);
+        int kind = (r.next() >> 63) ? PCD_RULE_CPM : PCD_RULE_MODULARITY;
+        double lam = lam_lo + r.uniform() * (lam_hi - lam_lo);
+        g.prm[0] = nn; g.prm[1] = B; g.prm[2] = pin; g.prm[3] = pout; g.prm[4] = kind; g.prm[5] = lam;
+        std::vector<int32_t> bs(B, nn / B);
+        for (int b = 0; b < nn % B; b++) bs[b]++;
+        int64_t ne = 0;
+        int32_t *ea = nullptr, *eb = nullptr;
+        int nodes = 0;
+        g.rc = pcd_gen_sbm(stream_seed(seed, (uint64_t)i, 1), B, bs.data(), pin, pout, &ne, &ea, &eb, &nodes);
+        if (g.rc == PCD_OK) convert_into(g, nodes, ne, ea, eb);
+        free(ea);
+        free(eb);
+    };
+    return gen_many(count, make, n_out, m_out, ind_off, adj_off, indices, adj, params);
+}
+
+// Serving-form workload: request i draws from stream_seed(seed, i, 0) its
+// graph family (SBM or R-MAT, 1/2 each), a target edge count log-uniform in
+// [m_lo, m_hi] and its metric (modularity 0.6, CPM with resolution 0.01
+// 0.3, WCC with the triangle-based initial partition 0.1).  SBM: n = m/8
+// (average degree 16), blocks of 200, 80 % of the edges inside blocks.
+// R-MAT: scale ceil(log2(m/8)), m draws, a/b/c = .57/.19/.19.  Edges from
+// stream_seed(seed, i, 1), converted as pcd_convert_edges does.
+// params[6 i ..] = {family (0 SBM, 1 R-MAT), target m, kind, lambda,
+// initial partition, 0}.
+extern "C" int pcd_gen_requests(uint64_t seed, int count, int64_t m_lo, int64_t m_hi, int32_t **n_out,
+                                int32_t **m_out, int64_t **ind_off, int64_t **adj_off, int32_t **indices,
+                                int32_t **adj, double **params) {
+    if (count <= 0 || m_lo < 16 || m_hi < m_lo) return fail(PCD_E_INVALID, "bad request shape");
+    auto make = [&](int i, GenGraph &g) {
+        SplitMix r(stream_seed(seed, (uint64_t)i, 0));
+        const int fam = (int)(r.next() >> 63);
+        double lm = std::log((double)m_lo) + r.uniform() * (std::log((double)m_hi) - std::log((double)m_lo));
+        const int64_t mt = std::max<int64_t>(m_lo, std::min<int64_t>(m_hi, std::llround(std::exp(lm))));
+        const double u = r.uniform();
+        const int kind = u < 0.6 ? PCD_RULE_MODULARITY : (u < 0.9 ? PCD_RULE_CPM : PCD_RULE_WCC);
+        g.prm[0] = fam; g.prm[1] = (double)mt; g.prm[2] = kind; g.prm[3] = kind == PCD_RULE_CPM ? 0.01 : 0.0;
+        g.prm[4] = kind == PCD_RULE_WCC ? 1 : 0;
+        int64_t ne = 0;
+        int32_t *ea = nullptr, *eb = nullptr;
+        int nodes = 0;
+        const uint64_t es = stream_seed(seed, (uint64_t)i, 1);
+        if (fam == 0) {
+            int64_t nn = std::max<int64_t>(32, mt / 8);
+            int B = (int)std::max<int64_t>(2, nn / 200);
+            std::vector<int32_t> bs(B, (int32_t)(nn / B));
+            for (int b = 0; b < nn % B; b++) bs[b]++;
+            double pairs_in = 0;
+            for (int b = 0; b < B; b++) pairs_in += 0.5 * (double)bs[b] * (bs[b] - 1);
+            double pairs_out = 0.5 * (double)nn * (nn - 1) - pairs_in;
+            double pin = std::min(1.0, 0.8 * mt / std::max(1.0, pairs_in));
+            double pout = std::min(1.0, 0.2 * mt / std::max(1.0, pairs_out));
+            g.rc = pcd_gen_sbm(es, B, bs.data(), pin, pout, &ne, &ea, &eb, &nodes);
+        } else {
+            int scale = std::max(5, (int)std::ceil(std::log2(std::max(2.0, mt / 8.0))));
+            g.rc = pcd_gen_rmat(es, scale, mt, 0.57, 0.19, 0.19, &ne, &ea, &eb, &nodes);
+        }
+        if (g.rc == PCD_OK) convert_into(g, nodes, ne, ea, eb);
+        free(ea);
+        free(eb);
+    };
+    return gen_many(count, make, n_out, m_out, ind_off, adj_off, indices, adj, params);
 }
 
 // framework.cpp:27-59 create_clustering_partition: vertices in order of

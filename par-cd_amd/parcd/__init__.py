@@ -32,7 +32,7 @@ EXPORTS = [
     "pcd_batch_info", "pcd_batch_map_count", "pcd_batch_map_name", "pcd_batch_set_map", "pcd_batch_autotune",
     "pcd_gen_sbm_batch", "pcd_refine", "pcd_host_alloc", "pcd_host_free", "pcd_clustering_partition",
     "pcd_server_create", "pcd_server_destroy", "pcd_server_submit", "pcd_server_wait", "pcd_server_result",
-    "pcd_server_info",
+    "pcd_server_info", "pcd_gen_requests",
 ]
 BATCH_RUNNING, BATCH_DONE, BATCH_FREE = 0, 1, 2
 
@@ -150,6 +150,9 @@ def lib():
     L.pcd_server_wait.argtypes = [vp, C.c_int, vp, C.c_int, ip]
     L.pcd_server_result.argtypes = [vp, C.c_int64, vp, dp, ip, ip, dp, ip]
     L.pcd_server_info.argtypes = [vp, ip, ip, ip, lp]
+    L.pcd_gen_requests.argtypes = [C.c_uint64, C.c_int, C.c_int64, C.c_int64] + \
+        [C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.POINTER(C.c_int64)), C.POINTER(C.POINTER(C.c_int64)),
+         C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.POINTER(C.c_double))]
     L.pcd_refine.argtypes = [vp, C.POINTER(Job), C.c_int, C.c_int, dp, ip, ip]
     L.pcd_gen_sbm_batch.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + \
         [C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.POINTER(C.c_int64)), C.POINTER(C.POINTER(C.c_int64)),
@@ -254,17 +257,7 @@ def make_graph(kind, seed, **kw):
     return convert_edges(nn, a, b)
 
 
-def gen_sbm_batch(seed, count, nmin=50, nmax=20000, bmin=2, bmax=64, pin=(0.05, 0.3), pout=(0.001, 0.01),
-                  lam=(0.005, 0.05), pinned=False):
-    """The batched-form workload (pcd_gen_sbm_batch): a list of
-    (n, m, indices, adj) views into two concatenated arrays (page-locked
-    with pinned=True), and the per-graph parameter rows {nominal n, blocks,
-    p_in, p_out, kind, lambda}."""
-    pn, pm, pio, pao, pi, pa = I32P(), I32P(), C.POINTER(C.c_int64)(), C.POINTER(C.c_int64)(), I32P(), I32P()
-    pp = C.POINTER(C.c_double)()
-    check(lib().pcd_gen_sbm_batch(seed, count, nmin, nmax, bmin, bmax, pin[0], pin[1], pout[0], pout[1], lam[0],
-                                  lam[1], C.byref(pn), C.byref(pm), C.byref(pio), C.byref(pao), C.byref(pi),
-                                  C.byref(pa), C.byref(pp)))
+def _unpack_many(count, pn, pm, pio, pao, pi, pa, pp, pinned):
     n = _take(pn, count)
     m = _take(pm, count)
     io = _take(pio, count)
@@ -283,6 +276,31 @@ def gen_sbm_batch(seed, count, nmin=50, nmax=20000, bmin=2, bmax=64, pin=(0.05, 
         adj = _take(pa, te)
     graphs = [(int(n[i]), int(m[i]), ind[io[i]: io[i] + n[i] + 1], adj[ao[i]: ao[i] + 2 * m[i]]) for i in range(count)]
     return graphs, params
+
+
+def _many_ptrs():
+    return (I32P(), I32P(), C.POINTER(C.c_int64)(), C.POINTER(C.c_int64)(), I32P(), I32P(),
+            C.POINTER(C.c_double)())
+
+
+def gen_sbm_batch(seed, count, nmin=50, nmax=20000, bmin=2, bmax=64, pin=(0.05, 0.3), pout=(0.001, 0.01),
+                  lam=(0.005, 0.05), pinned=False):
+    """The batched-form workload (pcd_gen_sbm_batch): a list of
+    (n, m, indices, adj) views into two concatenated arrays (page-locked
+    with pinned=True), and the per-graph parameter rows {nominal n, blocks,
+    p_in, p_out, kind, lambda}."""
+    ps = _many_ptrs()
+    check(lib().pcd_gen_sbm_batch(seed, count, nmin, nmax, bmin, bmax, pin[0], pin[1], pout[0], pout[1], lam[0],
+                                  lam[1], *[C.byref(x) for x in ps]))
+    return _unpack_many(count, *ps, pinned)
+
+
+def gen_requests(seed, count, m_lo=1000, m_hi=50000000, pinned=False):
+    """The serving-form workload (pcd_gen_requests): graphs as gen_sbm_batch
+    returns them and rows {family, target m, kind, lambda, initial, 0}."""
+    ps = _many_ptrs()
+    check(lib().pcd_gen_requests(seed, count, int(m_lo), int(m_hi), *[C.byref(x) for x in ps]))
+    return _unpack_many(count, *ps, pinned)
 
 
 class _Pinned:
