@@ -120,26 +120,37 @@ def pmc_traffic(slot):
     return sum(rows[n] for n in names)
 
 
-def cpu_baseline(pcd_path, E, kind_arg, iters=2):
+def _ref_run(exe, pcd_path, kind_arg, iters, cores):
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(cores))
+    t0 = time.time()
+    r = subprocess.run([exe, pcd_path, "-m", kind_arg, "-n", str(iters), "-p", str(cores)], env=env,
+                       stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=600)
+    out = r.stdout
+    stats = [float(x) for x in re.findall(r"collecting statistics: ([0-9.e+-]+) sec", out)]
+    metric = [float(x) for x in re.findall(r"calculate metric: ([0-9.e+-]+) sec", out)]
+    lp = [float(x) for x in re.findall(r"label propagation: ([0-9.e+-]+) sec", out)]
+    n_it = min(len(lp), len(stats), len(metric))
+    return [stats[i] + metric[i] + lp[i] for i in range(n_it)], lp[:n_it], time.time() - t0, out
+
+
+def cpu_baseline(pcd_path, E, kind_arg, first, count):
+    """The reference OpenMP binary (oracle/_ref/main) on the same graph file:
+    seconds of (statistics + metric + label propagation) per iteration over
+    the same iterations the GPU times (first .. first+count-1), on all host
+    cores (at most 16, OMP_PROC_BIND=close), plus one core for the first
+    iterations."""
     exe = os.path.join(ROOT, "oracle", "_ref", "main")
     if not os.path.exists(exe):
         log("reference binary missing (oracle/_ref/main); cpu_baseline skipped")
         return None
     cores = min(16, os.cpu_count() or 1)
-    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(cores))
-    t0 = time.time()
-    r = subprocess.run([exe, pcd_path, "-m", kind_arg, "-n", str(iters), "-p", str(cores)], env=env,
-                       stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=600)
-    wall = time.time() - t0
-    out = r.stdout
-    stats = [float(x) for x in re.findall(r"collecting statistics: ([0-9.e+-]+) sec", out)]
-    metric = [float(x) for x in re.findall(r"calculate metric: ([0-9.e+-]+) sec", out)]
-    lp = [float(x) for x in re.findall(r"label propagation: ([0-9.e+-]+) sec", out)]
-    n_it = min(len(lp), iters)
-    if n_it == 0:
-        log("reference run produced no iterations:", out[-400:])
+    per_it, lp, wall, out = _ref_run(exe, pcd_path, kind_arg, first + count, cores)
+    sel = per_it[first: first + count]
+    if not sel:
+        log("reference run produced too few iterations:", out[-400:])
         return None
-    per_iter = sum(stats[i] + metric[i] + lp[i] for i in range(n_it)) / n_it
+    per_iter = sum(sel) / len(sel)
+    one, _, wall1, _ = _ref_run(exe, pcd_path, kind_arg, 1, 1)
     cpu = ""
     try:
         cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
@@ -147,10 +158,12 @@ def cpu_baseline(pcd_path, E, kind_arg, iters=2):
         pass
     return {"value": E / per_iter / 1e9, "unit": "GTEPS", "cores": cores, "kind": "reference",
             "sample": "reference main (oracle/_ref/main, -O3 -fopenmp) on the same R-MAT-22 PCD file, "
-                      "-m %s -n %d -p %d: mean of (statistics + metric + label propagation) over %d "
-                      "iterations from the singleton partition; wall %.1f s incl. load" % (
-                          kind_arg, iters, cores, n_it, wall),
-            "sec_per_iteration": per_iter, "lp_sec": lp[:n_it], "stats_sec": stats[:n_it], "host_cpu": cpu}
+                      "-m %s -n %d -p %d: mean of (statistics + metric + label propagation) over iterations "
+                      "%d-%d from the singleton partition (the iterations the GPU times); wall %.1f s incl. load"
+                      % (kind_arg, first + count, cores, first + 1, first + count, wall),
+            "sec_per_iteration": per_iter, "per_iteration_sec": [round(x, 4) for x in per_it],
+            "one_core_sec_first_iteration": round(one[0], 3) if one else None,
+            "one_core_wall_s": round(wall1, 1), "host_cpu": cpu}
 
 
 ITER_CAP = 100
@@ -476,7 +489,7 @@ def main():
         with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
             pcd_path = os.path.join(tmp, "rmat.pcd")
             parcd.write_pcd(pcd_path, n, m, ind, adj, mp)
-            cpu = cpu_baseline(pcd_path, E, "mod")
+            cpu = cpu_baseline(pcd_path, E, "mod", args.warmup, args.steps)
         if cpu:
             log("cpu baseline: %.4f GTEPS (%.3f s/iteration on %d cores)" % (cpu["value"], cpu["sec_per_iteration"],
                                                                              cpu["cores"]))

@@ -798,7 +798,9 @@ long orc_community_pairs(int n, const int *indices, const int *adj, const int *l
 /* strategy: 0 = propagation, 1 = merging, 2 = alternate (main.cpp:247).
  * initial: 0 singleton (rand()), 1 triangle clustering (main.cpp:193-212).
  * The glibc generator is reset to its default state first, as in a fresh
- * `main` process.  `init_labels` (if non-NULL) gives the initial labels
+ * `main` process (srand(1)); in libc mode a non-zero seed s resets it with
+ * srand(s) instead, which samples the reference's run-to-run spread as if
+ * it had a seed option.  `init_labels` (if non-NULL) gives the initial labels
  * instead of the rand()-based singleton partition.
  * Outputs: labels_out[n] (final compressed labels), score, iterations, k. */
 int orc_run(int n, int m, const int *indices, const int *adj, int kind, double lambda,
@@ -809,7 +811,7 @@ int orc_run(int n, int m, const int *indices, const int *adj, int kind, double l
     int *labels = labels_out;
     int *tri = NULL, *tri_deg = NULL;
     double cc = 0.0;
-    srand(1);
+    srand(mode == 0 && seed != 0 ? (unsigned)seed : 1u);
     if (initial == 1 || kind == ORC_WCC) {
         long E = indices[n];
         int *et = (int *)malloc(sizeof(int) * ((size_t)E + 1));

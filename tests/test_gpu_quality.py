@@ -1,0 +1,67 @@
+"""Partition quality of the GPU path against the reference algorithm on
+SBM-1 (100 blocks of 100, p_in 0.08, p_out 0.0005, generator seed 1).
+
+The GPU draws vertex selection and ties from a counter hash; the reference
+draws them from glibc rand/rand_r, so single runs differ and only the
+distributions can be compared.  The reference side is the oracle in glibc
+mode (label-exact with the reference binary at -p 1), reset with srand(s)
+for seeds s = 1..SEEDS; the GPU side is pcd_refine with seeds 1..SEEDS.
+Final scores: the means must agree within 0.5 % of the reference mean, or
+within three standard errors of the difference where the runs' own spread
+is larger than that (label propagation alone, -s 0, spreads by ~2 %)."""
+import multiprocessing as mp
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_graphs import sbm, singleton_labels
+
+pytestmark = pytest.mark.gpu
+parcd = pytest.importorskip("parcd")
+
+SEEDS = 12
+CASES = [("modularity", O.MOD, 0.0, 0), ("cpm-0.01", O.CPM, 0.01, 0), ("wcc -i 1", O.WCC, 0.0, 1)]
+
+
+def _ref(a):
+    n, m, ind, adj, kind, lam, initial, strategy, seed = a
+    return O.run(n, m, ind, adj, kind=kind, lam=lam, initial=initial, strategy=strategy, mode=0, seed=seed)[1]
+
+
+@pytest.fixture(scope="module")
+def sbm1():
+    return sbm(100, 100, 0.08, 0.0005, seed=1)
+
+
+@pytest.fixture(scope="module")
+def reference_scores(sbm1):
+    n, m, ind, adj, _ = sbm1
+    work = [(n, m, ind, adj, kind, lam, ini, st, sd) for _, kind, lam, ini in CASES for st in (0, 2)
+            for sd in range(1, SEEDS + 1)]
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
+        scores = pool.map(_ref, work)
+    return np.array(scores).reshape(len(CASES), 2, SEEDS)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("strategy", [0, 2])
+def test_quality_matches_reference(sbm1, reference_scores, case, strategy):
+    name, kind, lam, ini = CASES[case]
+    n, m, ind, adj, _ = sbm1
+    ours = []
+    with parcd.Context(0) as ctx:
+        ctx.upload_graph(n, m, ind, adj)
+        tri = ctx.triangles()[0]
+        for sd in range(1, SEEDS + 1):
+            lab = parcd.clustering_partition(n, ind, adj, tri) if ini else singleton_labels(n)
+            ctx.upload_labels(lab)
+            ours.append(ctx.refine(parcd.job(kind=kind, lam=lam, seed=sd), strategy=strategy)[0])
+    ours = np.array(ours)
+    ref = reference_scores[case, 0 if strategy == 0 else 1]
+    diff = ours.mean() - ref.mean()
+    se = np.sqrt(ours.var(ddof=1) / SEEDS + ref.var(ddof=1) / SEEDS)
+    print("%s -s %d: gpu %.6f +- %.6f, reference %.6f +- %.6f, rel diff %+.3f %%, diff/se %+.2f" % (
+        name, strategy, ours.mean(), ours.std(), ref.mean(), ref.std(), 100 * diff / abs(ref.mean()), diff / se))
+    assert abs(diff) <= max(0.005 * abs(ref.mean()), 3 * se)
