@@ -169,6 +169,81 @@ def cpu_baseline(pcd_path, E, kind_arg, first, count):
 ITER_CAP = 100
 
 
+def wcc_leg(parcd, np, steps, warmup):
+    """WCC on the Chung-Lu graph of config 2 (seed 3: 10^6 vertices, expected
+    degrees from a power law with exponent 2.3 on [4, 5000], 32 planted
+    communities of 100-50,000 vertices boosted x20), as `main -m wcc -i 1`:
+    triangle counting, the triangle-based initial partition, then refinement
+    iterations (compress, statistics, WCC metric with its internal-triangle
+    pass, propagation).  Returns the graph arrays too (for the reference run)."""
+    t = time.time()
+    n, m, ind, adj, mp = parcd.make_graph("chung_lu", 3, n=1000000, exponent=2.3, dmin=4, dmax=5000, num_comms=32,
+                                          cmin=100, cmax=50000, boost=20.0)
+    E = 2 * m
+    log("Chung-Lu-1M: n=%d m=%d max_deg=%d (host gen+convert %.1fs)" % (n, m, ind[1] - ind[0], time.time() - t))
+    ctx = parcd.Context(0)
+    ctx.upload_graph(n, m, ind, adj)
+    ctx.enable_timing(True)
+    ctx.kernel_timer_reset()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    tri, td, cc = ctx.triangles()
+    t1 = time.perf_counter()
+    tri_ms = ctx.kernel_timers().get("triangles", (0.0, 0))[0]
+    lab = parcd.clustering_partition(n, ind, adj, tri)
+    t2 = time.perf_counter()
+    ctx.upload_labels(lab)
+    scores = []
+    for it in range(warmup):
+        ctx.compress(); ctx.statistics(); scores.append(ctx.metric(parcd.RULE_WCC))
+        ctx.propagate(parcd.RULE_WCC, frac=0.5, seed=1, step=it)
+    ctx.synchronize()
+    ctx.kernel_timer_reset()
+    t3 = time.perf_counter()
+    for it in range(warmup, warmup + steps):
+        ctx.compress(); ctx.statistics(); scores.append(ctx.metric(parcd.RULE_WCC))
+        ctx.propagate(parcd.RULE_WCC, frac=0.5, seed=1, step=it)
+    ctx.synchronize()
+    t4 = time.perf_counter()
+    timers = ctx.kernel_timers()
+    ctx.close()
+    it_ms = (t4 - t3) * 1000 / steps
+    return {
+        "graph": "Chung-Lu seed 3, n=%d m=%d" % (n, m), "E": int(E), "cc": round(cc, 6),
+        "triangles_ms": round((t1 - t0) * 1000, 3), "triangle_kernels_ms": round(tri_ms, 3),
+        "triangles_gteps": round(E / (t1 - t0) / 1e9, 2), "clustering_partition_host_ms": round((t2 - t1) * 1000, 1),
+        "ms_per_iteration": round(it_ms, 3), "gteps": round(E / (it_ms / 1000) / 1e9, 2),
+        "internal_triangle_ms_per_iteration": round(timers.get("triangles", (0.0, 0))[0] / steps, 3),
+        "wcc_scores": [round(x, 6) for x in scores],
+        "kernel_ms_per_iteration": {k: round(v[0] / steps, 3) for k, v in timers.items()},
+    }, (n, m, ind, adj, mp)
+
+
+def cpu_reference_wcc(pcd_path, E, iters=3):
+    """The reference binary on the Chung-Lu graph, -m wcc -i 1: its serial
+    count_triangles and the per-iteration phases (WCC calculate included)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "main")
+    if not os.path.exists(exe):
+        return None
+    cores = min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(cores))
+    t0 = time.time()
+    r = subprocess.run([exe, pcd_path, "-m", "wcc", "-i", "1", "-n", str(iters), "-p", str(cores)], env=env,
+                       stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=900)
+    out = r.stdout
+    tri = [float(x) for x in re.findall(r"count triangles: ([0-9.e+-]+)", out)]
+    stats = [float(x) for x in re.findall(r"collecting statistics: ([0-9.e+-]+) sec", out)]
+    metric = [float(x) for x in re.findall(r"calculate metric: ([0-9.e+-]+) sec", out)]
+    lp = [float(x) for x in re.findall(r"label propagation: ([0-9.e+-]+) sec", out)]
+    k = min(len(stats), len(metric), len(lp))
+    if not tri or k == 0:
+        return None
+    per = [stats[i] + metric[i] + lp[i] for i in range(k)]
+    return {"count_triangles_sec": tri[0], "cores": cores, "sec_per_iteration": sum(per) / k,
+            "gteps": E / (sum(per) / k) / 1e9, "iterations": k, "wall_s": round(time.time() - t0, 1),
+            "note": "count_triangles is serial in the reference (graph.hpp:23)"}
+
+
 def batch_leg(parcd, np, graphs_count, seed=4):
     """Batched form: graphs_count SBM graphs (pcd_gen_sbm_batch, seed 4:
     n log-uniform in [50, 20000], 2-64 equal blocks, p_in U[0.05, 0.3],
@@ -357,6 +432,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-wcc", action="store_true", help="skip the WCC / Chung-Lu leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--serve-m", type=float, nargs=2, default=[1e3, 1e6], metavar=("LO", "HI"),
@@ -471,6 +547,12 @@ def main():
         log("cpm-0.01: %.3f ms/iteration, %.2f GTEPS" % (cpm["ms_per_step"], cpm["gteps"]))
     ctx.close()
 
+    # ---------------------------------------------------------- WCC, Chung-Lu
+    wcc = wcc_graph = None
+    if not args.no_wcc:
+        wcc, wcc_graph = wcc_leg(parcd, np, 3, 1)
+        log("wcc:", json.dumps(wcc))
+
     # ---------------------------------------------------------- batched form
     batch = None
     if not args.no_batch:
@@ -490,6 +572,11 @@ def main():
             pcd_path = os.path.join(tmp, "rmat.pcd")
             parcd.write_pcd(pcd_path, n, m, ind, adj, mp)
             cpu = cpu_baseline(pcd_path, E, "mod", args.warmup, args.steps)
+            if cpu and wcc_graph:
+                wn, wm, wi, wa, wmp = wcc_graph
+                wpath = os.path.join(tmp, "chung_lu.pcd")
+                parcd.write_pcd(wpath, wn, wm, wi, wa, wmp)
+                cpu["wcc_chung_lu"] = cpu_reference_wcc(wpath, 2 * wm)
         if cpu:
             log("cpu baseline: %.4f GTEPS (%.3f s/iteration on %d cores)" % (cpu["value"], cpu["sec_per_iteration"],
                                                                              cpu["cores"]))
@@ -505,7 +592,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "stream_copy_gbs": round(copy_gbs, 1),
         "speedup_vs_cpu": round(gteps / cpu["value"], 2) if cpu else None,
         "phases": {"propagate": lp_phase, "statistics": stats_phase, "kernels": per_kernel},
-        "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm, "batch": batch, "serve": serve,
+        "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm, "wcc_chung_lu": wcc, "batch": batch, "serve": serve,
     }
     print(json.dumps(out), flush=True)
 

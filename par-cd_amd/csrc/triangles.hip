@@ -30,6 +30,12 @@ __global__ void k_lowpos(int n, const int32_t *indices, const int32_t *adj, int3
     }
 }
 
+__device__ inline void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline int find_sorted(const int32_t *a, int len, int x) {
     int lo = 0, hi = len;
     while (lo < hi) {
@@ -40,8 +46,11 @@ __device__ inline int find_sorted(const int32_t *a, int len, int x) {
     return (lo < len && a[lo] == x) ? lo : -1;
 }
 
+// Vertices whose prefix exceeds FLAT_TS: one wave walks u over the prefix
+// and its lanes take the entries of N(u) ∩ [0,u).
 __global__ __launch_bounds__(256) void k_triangles(int n, const int32_t *indices, const int32_t *adj,
-                                                   const int32_t *lowpos, const int32_t *labels, int32_t *et) {
+                                                   const int32_t *lowpos, const int32_t *labels, int32_t *et,
+                                                   int min_len) {
     __shared__ int32_t s_val[4][TRI_STAGE];
     __shared__ int32_t s_cnt[4][TRI_STAGE];
     const int lane = __lane_id(), w = threadIdx.x / WAVE;
@@ -50,7 +59,7 @@ __global__ __launch_bounds__(256) void k_triangles(int n, const int32_t *indices
     const int nwaves = gridDim.x * blockDim.x / WAVE;
     for (int v = wave_g; v < n; v += nwaves) {
         const int lenv = lowpos[v];
-        if (lenv < 2) continue;  // a triangle needs two smaller neighbours
+        if (lenv < 2 || lenv <= min_len) continue;  // a triangle needs two smaller neighbours
         const int sv = indices[v];
         const int l = labels ? labels[v] : 0;
         const bool staged = lenv <= TRI_STAGE;
@@ -92,6 +101,74 @@ __global__ __launch_bounds__(256) void k_triangles(int n, const int32_t *indices
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// Vertices with prefixes up to TS: the wedge checks (r, y) of vertex v --
+// u = the r-th prefix entry, w = the y-th entry of N(u) ∩ [0,u) -- are
+// flattened over the lanes (an exclusive scan of the per-r lengths in LDS,
+// a binary search of the lane's item in it), so a wave stays busy however
+// short the rows are.  Same checks and counts as k_triangles.
+constexpr int FLAT_TS = 512;
+
+template <int TS>
+__global__ __launch_bounds__(256) void k_triangles_flat(int n, const int32_t *indices, const int32_t *adj,
+                                                        const int32_t *lowpos, const int32_t *labels, int32_t *et) {
+    __shared__ int32_t s_val[4][TS];
+    __shared__ int32_t s_cnt[4][TS];
+    __shared__ int32_t s_su[4][TS];
+    __shared__ int32_t s_pre[4][TS + 1];
+    const int lane = __lane_id(), w = threadIdx.x / WAVE;
+    int32_t *sval = s_val[w], *scnt = s_cnt[w], *ssu = s_su[w], *spre = s_pre[w];
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    for (int v = wave_g; v < n; v += nwaves) {
+        const int lenv = lowpos[v];
+        if (lenv < 2 || lenv > TS) continue;
+        const int sv = indices[v];
+        const int l = labels ? labels[v] : 0;
+        int run = 0;
+        for (int base = 0; base < lenv; base += WAVE) {
+            const int i = base + lane;
+            int len = 0;
+            if (i < lenv) {
+                const int u = adj[sv + i];
+                int su = 0;
+                if (i >= 1 && (!labels || labels[u] == l)) {
+                    su = indices[u];
+                    len = lowpos[u];
+                }
+                sval[i] = u;
+                scnt[i] = 0;
+                ssu[i] = su;
+            }
+            const int inc = wave_incl_scan(len);
+            if (i < lenv) spre[i] = run + inc - len;
+            run += __shfl(inc, WAVE - 1, WAVE);
+        }
+        if (lane == 0) spre[lenv] = run;
+        wave_sync_lds();
+        for (int t = lane; t < run; t += WAVE) {
+            int lo = 0, hi = lenv;  // largest r with spre[r] <= t; its range holds t
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (spre[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            const int r = lo, y = t - spre[r];
+            const int su = ssu[r];
+            const int wv = adj[su + y];
+            const int x = find_sorted(sval, r, wv);
+            if (x >= 0 && (!labels || labels[wv] == l)) {
+                atomicAdd(&scnt[x], 1);
+                atomicAdd(&scnt[r], 1);
+                atomicAdd(&et[su + y], 1);
+            }
+        }
+        wave_sync_lds();
+        for (int i = lane; i < lenv; i += WAVE)
+            if (scnt[i]) atomicAdd(&et[sv + i], scnt[i]);
+        wave_sync_lds();
     }
 }
 
@@ -209,8 +286,10 @@ static int count_pass(pcd_ctx *ctx, const int32_t *labels, int32_t *t, int32_t *
     PCD_TRY(fill_i32(ctx, ctx->d_et, 0, E));
     PCD_TRY(fill_i32(ctx, t, 0, n));
     PCD_TRY(fill_i32(ctx, td, 0, n));
+    hipLaunchKernelGGL(k_triangles_flat<FLAT_TS>, dim3(grid_for(ctx, n, 4, 4)), dim3(256), 0, ctx->stream, n,
+                       ctx->d_indices, ctx->d_adj, ctx->d_lowpos, labels, ctx->d_et);
     hipLaunchKernelGGL(k_triangles, dim3(grid_for(ctx, n, 4, 2)), dim3(256), 0, ctx->stream, n, ctx->d_indices,
-                       ctx->d_adj, ctx->d_lowpos, labels, ctx->d_et);
+                       ctx->d_adj, ctx->d_lowpos, labels, ctx->d_et, FLAT_TS);
     hipLaunchKernelGGL(k_tri_accumulate, dim3(grid_for(ctx, n, 4, 8)), dim3(256), 0, ctx->stream, n, ctx->d_indices,
                        ctx->d_adj, ctx->d_lowpos, ctx->d_et, t, td);
     kt_end(ctx, KS_TRIANGLES);

@@ -70,3 +70,11 @@ def singleton_labels(n):
     lab = np.zeros(n, np.int32)
     L.orc_singleton_partition(n, lab)
     return lab
+
+
+@functools.lru_cache(maxsize=None)
+def dense_blocks(seed=5):
+    """A near-clique of 2200 vertices beside 30 dense blocks of 100: vertex
+    prefixes N(v) ∩ [0, v) from a few up to above 2048 entries."""
+    import parcd
+    return parcd.make_graph("sbm", seed, block_sizes=[2200] + [100] * 30, p_in=0.97, p_out=0.001)

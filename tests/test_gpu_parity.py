@@ -204,7 +204,7 @@ def test_propagation_wcc(ctx):
 
 
 def test_triangles(ctx):
-    cases = list(GOLDEN.items()) + [("zoo", GG.bin_zoo()), ("rmat14", GG.rmat(14))]
+    cases = list(GOLDEN.items()) + [("zoo", GG.bin_zoo()), ("rmat14", GG.rmat(14)), ("dense", GG.dense_blocks())]
     for name, g in cases:
         n, m, ind, adj = upload(ctx, g)
         tri, td, cc = ctx.triangles()
@@ -225,7 +225,7 @@ def test_triangles_golden_fixture(ctx):
 
 
 def test_wcc_metric(ctx):
-    cases = [GOLDEN["golden_sbm200"], GOLDEN["golden_rmat9"], GG.bin_zoo()]
+    cases = [GOLDEN["golden_sbm200"], GOLDEN["golden_rmat9"], GG.bin_zoo(), GG.dense_blocks()]
     rng = np.random.default_rng(8)
     for g in cases:
         n, m, ind, adj = g[:4]
