@@ -135,6 +135,41 @@ __global__ __launch_bounds__(256) void k_stats_items(int nitems, const int4 *ite
     }
 }
 
+// Same work with the labels of vertices [0, ncache) staged in LDS: ids are
+// degree-descending, so those are the most frequent gather targets.  One
+// 1024-thread workgroup per CU pair of slots; each wave takes items as above.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_stats_items_cached(int nitems, const int4 *items, const int32_t *adj,
+                                                           const int32_t *labels, CommStats *st, int ncache) {
+    constexpr int PF = 8;
+    __shared__ int32_t hot[STATS_CACHE];
+    for (int i = threadIdx.x; i < ncache; i += NT) hot[i] = labels[i];
+    __syncthreads();
+    const int lane = __lane_id();
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    for (int it = wave_g; it < nitems; it += nwaves) {
+        int4 w = items[it];
+        int l = labels[w.x];
+        int same = 0;
+        for (int j0 = w.y + lane; j0 < w.z; j0 += PF * WAVE) {
+            int nb[PF];
+#pragma unroll
+            for (int e = 0; e < PF; e++) {
+                int j = j0 + e * WAVE;
+                nb[e] = j < w.z ? adj[j] : -1;
+            }
+            int lab[PF];
+#pragma unroll
+            for (int e = 0; e < PF; e++) lab[e] = nb[e] < 0 ? -2 : (nb[e] < ncache ? hot[nb[e]] : labels[nb[e]]);
+#pragma unroll
+            for (int e = 0; e < PF; e++) same += lab[e] == l ? 1 : 0;
+        }
+        same = wave_sum(same);
+        if (lane == 0) add_stats(st, l, w.w ? 1 : 0, same, (w.z - w.y) - same);
+    }
+}
+
 template <int G>
 static void launch_stats_small(pcd_ctx *ctx, int p0, int p1) {
     if (p1 <= p0) return;
@@ -149,9 +184,14 @@ int collect_statistics_dev(pcd_ctx *ctx) {
                        ctx->d_k);
     if (ctx->num_stat_items > 0) {
         kt_begin(ctx);
-        hipLaunchKernelGGL(k_stats_items, dim3(grid_for(ctx, ctx->num_stat_items, 4, 16)), dim3(256), 0,
-                           ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_adj, ctx->d_labels,
-                           ctx->d_stats);
+        if (ctx->tune.stats_cache)
+            hipLaunchKernelGGL(k_stats_items_cached<1024>, dim3(std::min(ctx->cu_count * 2, ctx->num_stat_items)),
+                               dim3(1024), 0, ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_adj,
+                               ctx->d_labels, ctx->d_stats, std::min(ctx->n, STATS_CACHE));
+        else
+            hipLaunchKernelGGL(k_stats_items, dim3(grid_for(ctx, ctx->num_stat_items, 4, 16)), dim3(256), 0,
+                               ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_adj, ctx->d_labels,
+                               ctx->d_stats);
         kt_end(ctx, KS_STATS_ITEMS);
     }
     kt_begin(ctx);
