@@ -77,8 +77,9 @@ __device__ inline CommStats comm_of(const LPArgs &a, int label, int aux) {
 
 // refinement.cpp:211-221: best_score (0 if no join candidate beat 0) plus
 // score_leave decides between moving, leaving to a new singleton, staying.
+template <unsigned KM>
 __device__ inline int lp_decide(const LPArgs &a, int deg, int own, int own_aux, int own_cnt, const Cand &best) {
-    double leave = score_leave(a.rule, deg, comm_of(a, own, own_aux), own_cnt);
+    double leave = score_leave<KM>(a.rule, deg, comm_of(a, own, own_aux), own_cnt);
     double bs = best.label != INT32_MAX ? best.s : 0.0;
     bs += leave;
     if (bs > 0 && bs > leave) return best.label;
@@ -86,19 +87,14 @@ __device__ inline int lp_decide(const LPArgs &a, int deg, int own, int own_aux, 
     return own;
 }
 
+template <unsigned KM>
 __device__ inline int lp_decide_stats(const LPArgs &a, int deg, int own, int own_cnt, const Cand &best) {
-    double leave = score_leave(a.rule, deg, a.stats[own], own_cnt);
+    double leave = score_leave<KM>(a.rule, deg, a.stats[own], own_cnt);
     double bs = best.label != INT32_MAX ? best.s : 0.0;
     bs += leave;
     if (bs > 0 && bs > leave) return best.label;
     if (leave > 0) return PCD_NEW_LABEL;
     return own;
-}
-
-__device__ inline Cand lp_candidate(const LPArgs &a, int deg, uint64_t tb, int c, int aux, int cnt) {
-    double s = score_join(a.rule, deg, comm_of(a, c, aux), cnt);
-    if (s > 0) return Cand{s, tie_key(a.tie, tb, c), c};
-    return cand_none();
 }
 
 __device__ inline void count3(const LPArgs &a, long long act, long long ent, long long dist) {
@@ -124,7 +120,7 @@ __global__ void k_lp_records(int n, const int32_t *labels, const CommStats *st, 
 }
 
 // ------------------------------------------------------------- G lanes / vertex
-template <int G, int CP, bool AUX, class Src>
+template <int G, int CP, bool AUX, unsigned KM, class Src>
 __global__ __launch_bounds__(256) void k_lp_small(Src src) {
     __shared__ int s_list[256 / WAVE][WAVE];
     const int lane = __lane_id(), glane = lane & (G - 1), w = threadIdx.x / WAVE;
@@ -192,7 +188,7 @@ __global__ __launch_bounds__(256) void k_lp_small(Src src) {
                     own_cnt = cnt;
                 } else {
                     uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, u);
-                    double sc = score_join(a.rule, deg, AUX ? CommStats{xa, xa, 0, 0} : a.stats[x], cnt);
+                    double sc = score_join<KM>(a.rule, deg, AUX ? CommStats{xa, xa, 0, 0} : a.stats[x], cnt);
                     if (sc > 0) c = Cand{sc, tie_key(a.tie, tb, x), x};
                 }
             }
@@ -203,8 +199,8 @@ __global__ __launch_bounds__(256) void k_lp_small(Src src) {
                 own_cnt = max(own_cnt, __shfl_xor(own_cnt, d, WAVE));
             }
             if (has && glane == 0)
-                a.new_labels[u] = AUX ? lp_decide(a, deg, own.x, own.y, own_cnt, c)
-                                      : lp_decide_stats(a, deg, own.x, own_cnt, c);
+                a.new_labels[u] = AUX ? lp_decide<KM>(a, deg, own.x, own.y, own_cnt, c)
+                                      : lp_decide_stats<KM>(a, deg, own.x, own_cnt, c);
             if (a.counters) {
                 int ent = wave_sum((has && glane < deg) ? 1 : 0);
                 if (lane == 0) count3(a, __popcll(__ballot(has && glane == 0)), ent, __popcll(heads));
@@ -274,7 +270,7 @@ __device__ inline void insert_row(const LPArgs &a, const Table &t, int start, in
 // a time so that their community-record gathers overlap (tables without
 // aux records).
 constexpr int SB = 4;
-template <bool AUX>
+template <bool AUX, unsigned KM>
 __device__ inline void scan_table(const LPArgs &a, const Table &t, int lo, int stride, int deg, int own, uint64_t tb,
                                   Cand &c, int &own_cnt, int &distinct) {
     for (int s0 = lo; s0 < t.T; s0 += SB * stride) {
@@ -305,7 +301,7 @@ __device__ inline void scan_table(const LPArgs &a, const Table &t, int lo, int s
             if (key[e] == own) {
                 own_cnt = cnt[e];
             } else {
-                double sc = score_join(a.rule, deg, st[e], cnt[e]);
+                double sc = score_join<KM>(a.rule, deg, st[e], cnt[e]);
                 if (sc > 0) {
                     Cand o{sc, tie_key(a.tie, tb, key[e]), key[e]};
                     if (cand_better(o, c)) c = o;
@@ -382,7 +378,7 @@ __device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int 
 __device__ inline int ceil_log2(int x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
 
 // ------------------------------------------------------------- wave / vertex
-template <int TW, class Src>
+template <int TW, unsigned KM, class Src>
 __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
     __shared__ int s_keys[256 / WAVE][TW];
     __shared__ int s_cnts[256 / WAVE][TW];
@@ -411,9 +407,9 @@ __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_table<false>(a, t, lane, WAVE, deg, own, tb, c, own_cnt, distinct);
+        scan_table<false, KM>(a, t, lane, WAVE, deg, own, tb, c, own_cnt, distinct);
         wave_reduce_cand(c, own_cnt);
-        if (lane == 0) a.new_labels[v] = lp_decide_stats(a, deg, own, own_cnt, c);
+        if (lane == 0) a.new_labels[v] = lp_decide_stats<KM>(a, deg, own, own_cnt, c);
         if (a.counters) {
             distinct = wave_sum(distinct);
             if (lane == 0) count3(a, 1, deg, distinct);
@@ -425,7 +421,7 @@ __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
 
 // ------------------------------------------------------------- block / vertex
 // Table slots: nextpow2(2 deg), at most TB.
-template <int NT, int TB, class Src>
+template <int NT, int TB, unsigned KM, class Src>
 __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
     __shared__ int keys[TB];
     __shared__ int cnts[TB];
@@ -452,10 +448,10 @@ __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_table<false>(a, t, threadIdx.x, NT, deg, own, tb, c, own_cnt, distinct);
+        scan_table<false, KM>(a, t, threadIdx.x, NT, deg, own, tb, c, own_cnt, distinct);
         block_reduce_cand<NT>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
-            a.new_labels[v] = lp_decide_stats(a, deg, own, own_cnt, c);
+            a.new_labels[v] = lp_decide_stats<KM>(a, deg, own, own_cnt, c);
             if (a.counters) count3(a, 1, deg, distinct);
         }
       }
@@ -545,6 +541,7 @@ __global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) 
 }
 
 // pass 2: count one partition of one hub in LDS, reduce its best candidate
+template <unsigned KM>
 __global__ __launch_bounds__(HUB2_NT) void k_hub_count(LPArgs a, HubArgs h) {
     __shared__ int keys[HUB_TB];
     __shared__ int cnts[HUB_TB];
@@ -600,13 +597,14 @@ __global__ __launch_bounds__(HUB2_NT) void k_hub_count(LPArgs a, HubArgs h) {
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_table<true>(a, t, threadIdx.x, HUB2_NT, deg, own, tb, c, own_cnt, distinct);
+        scan_table<true, KM>(a, t, threadIdx.x, HUB2_NT, deg, own, tb, c, own_cnt, distinct);
         block_reduce_cand<HUB2_NT>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) h.partial[it] = HubPartial{c, own_cnt, distinct};
     }
 }
 
 // pass 3: one wave per hub combines its partitions and decides
+template <unsigned KM>
 __global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
     const int lane = __lane_id();
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
@@ -632,7 +630,7 @@ __global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
         if (lane == 0) {
             const int deg = a.indices[v + 1] - a.indices[v];
             int2 own = a.lv[v];
-            a.new_labels[v] = lp_decide(a, deg, own.x, own.y, own_cnt, c);
+            a.new_labels[v] = lp_decide<KM>(a, deg, own.x, own.y, own_cnt, c);
             if (a.counters) count3(a, 1, deg, distinct);
         }
     }
@@ -665,6 +663,7 @@ __global__ __launch_bounds__(512) void k_hub_global_insert(LPArgs a, HubArgs h, 
     }
 }
 
+template <unsigned KM>
 __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h, const int64_t *toff,
                                                             const int32_t *tlog, int32_t *gkeys, int32_t *gcnts) {
     __shared__ Cand s_c[1024 / WAVE];
@@ -687,7 +686,7 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
             distinct++;
             if (key == own) own_cnt = cnt;
             else {
-                double sc = score_join(a.rule, deg, a.stats[key], cnt);
+                double sc = score_join<KM>(a.rule, deg, a.stats[key], cnt);
                 if (sc > 0) {
                     Cand o{sc, tie_key(a.tie, tb, key), key};
                     if (cand_better(o, c)) c = o;
@@ -696,7 +695,7 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
         }
         block_reduce_cand<1024>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
-            double leave = score_leave(a.rule, deg, a.stats[own], own_cnt);
+            double leave = score_leave<KM>(a.rule, deg, a.stats[own], own_cnt);
             double bs = c.label != INT32_MAX ? c.s : 0.0;
             bs += leave;
             a.new_labels[v] = (bs > 0 && bs > leave) ? c.label : (leave > 0 ? PCD_NEW_LABEL : own);
@@ -707,24 +706,10 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
 }
 
 // ------------------------------------------------------------- driver
-int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie) {
+// Launches the propagation kernels of one step, compiled for the rule kinds KM.
+template <unsigned KM>
+static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
     const BinLayout &b = ctx->bins;
-    double fr = frac < 1.0 ? frac : 1.0;
-    LPArgs a;
-    a.indices = ctx->d_indices;
-    a.adj = ctx->d_adj;
-    a.labels = ctx->d_labels;
-    a.order = ctx->d_order;
-    a.lv = ctx->d_lv;
-    a.stats = ctx->d_stats;
-    a.new_labels = ctx->d_labels_alt;
-    a.rule = make_rule(ctx, rule);
-    a.seed = seed;
-    a.step = step;
-    a.tie = tie;
-    a.f = (int)(fr * 2147483647.0);  // refinement.cpp:158
-    a.use_aux = rule.kind == PCD_RULE_MODULARITY || rule.kind == PCD_RULE_CPM;
-    a.counters = nullptr;
     auto args_for = [&](int bin) {
         LPArgs x = a;
         if (ctx->counters) x.counters = ctx->d_bin_counters + 3 * bin;
@@ -741,7 +726,7 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
     if (a.use_aux && (ctx->num_hubs > 0 || small_aux)) {
         kt_begin(ctx);
         hipLaunchKernelGGL(k_lp_records, dim3(grid_for(ctx, ctx->n, 256, 8)), dim3(256), 0, ctx->stream, ctx->n,
-                           ctx->d_labels, ctx->d_stats, rule.kind, ctx->d_lv);
+                           ctx->d_labels, ctx->d_stats, a.rule.kind, ctx->d_lv);
         kt_end(ctx, KS_LP_SELECT);
     }
     int p0, p1;
@@ -768,29 +753,29 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
                            h);
         kt_end(ctx, KS_LP_HUB_INSERT);
         kt_begin(ctx);
-        hipLaunchKernelGGL(k_hub_count, dim3(grid_for(ctx, h.nparts, 1, 3)), dim3(HUB2_NT), 0, ctx->stream, ah, h);
-        hipLaunchKernelGGL(k_hub_final, dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0, ctx->stream, ah, h);
+        hipLaunchKernelGGL(k_hub_count<KM>, dim3(grid_for(ctx, h.nparts, 1, 3)), dim3(HUB2_NT), 0, ctx->stream, ah, h);
+        hipLaunchKernelGGL(k_hub_final<KM>, dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0, ctx->stream, ah, h);
         hipLaunchKernelGGL(k_hub_global_insert, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(512), 0, ctx->stream, ah,
                            h, ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts);
-        hipLaunchKernelGGL(k_hub_global_decide, dim3(grid_for(ctx, h.nhubs, 1, 1)), dim3(1024), 0, ctx->stream, ah,
+        hipLaunchKernelGGL(k_hub_global_decide<KM>, dim3(grid_for(ctx, h.nhubs, 1, 1)), dim3(1024), 0, ctx->stream, ah,
                            h, ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts);
         kt_end(ctx, KS_LP_HUB_DECIDE);
     }
     if (range(BIN_B4096, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_block<512, 8192, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0,
+        hipLaunchKernelGGL((k_lp_block<512, 8192, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0,
                            ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B4096)});
         kt_end(ctx, KS_LP_B4096);
     }
     if (range(BIN_B1024, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_block<256, 2048, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256), 0,
+        hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256), 0,
                            ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
         kt_end(ctx, KS_LP_B1024);
     }
     if (range(BIN_W256, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_wave<512, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256), 0,
+        hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256), 0,
                            ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_W256)});
         kt_end(ctx, KS_LP_W256);
     }
@@ -798,10 +783,10 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
     if (range(BIN, p0, p1)) {                                                                                  \
         kt_begin(ctx);                                                                                         \
         if (small_aux)                                                                                         \
-            hipLaunchKernelGGL((k_lp_small<G, CP, true, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)),  \
+            hipLaunchKernelGGL((k_lp_small<G, CP, true, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)),  \
                                dim3(256), 0, ctx->stream, RangeSrc{p0, p1, CP, args_for(BIN)});                \
         else                                                                                                   \
-            hipLaunchKernelGGL((k_lp_small<G, CP, false, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)), \
+            hipLaunchKernelGGL((k_lp_small<G, CP, false, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)), \
                                dim3(256), 0, ctx->stream, RangeSrc{p0, p1, CP, args_for(BIN)});                \
         kt_end(ctx, SLOT);                                                                                     \
     }
@@ -812,6 +797,31 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
     PCD_SMALL(4, 64, BIN_G4, KS_LP_G4)
 #undef PCD_SMALL
     PCD_HIP(hipGetLastError());
+    return PCD_OK;
+}
+
+int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie) {
+    double fr = frac < 1.0 ? frac : 1.0;
+    LPArgs a;
+    a.indices = ctx->d_indices;
+    a.adj = ctx->d_adj;
+    a.labels = ctx->d_labels;
+    a.order = ctx->d_order;
+    a.lv = ctx->d_lv;
+    a.stats = ctx->d_stats;
+    a.new_labels = ctx->d_labels_alt;
+    a.rule = make_rule(ctx, rule);
+    a.seed = seed;
+    a.step = step;
+    a.tie = tie;
+    a.f = (int)(fr * 2147483647.0);  // refinement.cpp:158
+    a.use_aux = rule.kind == PCD_RULE_MODULARITY || rule.kind == PCD_RULE_CPM;
+    a.counters = nullptr;
+    // kernels specialised for modularity or CPM hold fewer registers (more
+    // waves per SIMD); WCC and conductance use the general build
+    if (rule.kind == PCD_RULE_MODULARITY) PCD_TRY(launch_propagation<RK_MOD>(ctx, a));
+    else if (rule.kind == PCD_RULE_CPM) PCD_TRY(launch_propagation<RK_CPM>(ctx, a));
+    else PCD_TRY(launch_propagation<RK_ALL>(ctx, a));
     std::swap(ctx->d_labels, ctx->d_labels_alt);
     ctx->labels_compressed = false;
     ctx->has_stats = false;
@@ -828,25 +838,25 @@ int batch_propagation_launch(hipStream_t stream, const BatchLPLaunch &l) {
     const int cu = l.cu_count;
     auto src = [&](int v) { return ItemSrc{l.items[v], l.count[v], l.graphs, l.state}; };
     if (l.used[LPV_B512X8192])
-        hipLaunchKernelGGL((k_lp_block<512, 8192, ItemSrc>), dim3(cu * 2), dim3(512), 0, stream, src(LPV_B512X8192));
+        hipLaunchKernelGGL((k_lp_block<512, 8192, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 2), dim3(512), 0, stream, src(LPV_B512X8192));
     if (l.used[LPV_B512])
-        hipLaunchKernelGGL((k_lp_block<512, 2048, ItemSrc>), dim3(cu * 4), dim3(512), 0, stream, src(LPV_B512));
+        hipLaunchKernelGGL((k_lp_block<512, 2048, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 4), dim3(512), 0, stream, src(LPV_B512));
     if (l.used[LPV_B256])
-        hipLaunchKernelGGL((k_lp_block<256, 2048, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_B256));
+        hipLaunchKernelGGL((k_lp_block<256, 2048, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_B256));
     if (l.used[LPV_W1024])
-        hipLaunchKernelGGL((k_lp_wave<1024, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W1024));
+        hipLaunchKernelGGL((k_lp_wave<1024, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W1024));
     if (l.used[LPV_W512])
-        hipLaunchKernelGGL((k_lp_wave<512, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W512));
+        hipLaunchKernelGGL((k_lp_wave<512, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W512));
     if (l.used[LPV_S64])
-        hipLaunchKernelGGL((k_lp_small<64, 8, false, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S64));
+        hipLaunchKernelGGL((k_lp_small<64, 8, false, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S64));
     if (l.used[LPV_S32])
-        hipLaunchKernelGGL((k_lp_small<32, 16, false, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S32));
+        hipLaunchKernelGGL((k_lp_small<32, 16, false, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S32));
     if (l.used[LPV_S16])
-        hipLaunchKernelGGL((k_lp_small<16, 32, false, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S16));
+        hipLaunchKernelGGL((k_lp_small<16, 32, false, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S16));
     if (l.used[LPV_S8])
-        hipLaunchKernelGGL((k_lp_small<8, 64, false, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S8));
+        hipLaunchKernelGGL((k_lp_small<8, 64, false, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S8));
     if (l.used[LPV_S4])
-        hipLaunchKernelGGL((k_lp_small<4, 64, false, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S4));
+        hipLaunchKernelGGL((k_lp_small<4, 64, false, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S4));
     PCD_HIP(hipGetLastError());
     return PCD_OK;
 }

@@ -89,45 +89,52 @@ PCD_HD inline double wcc_improvement(int n_x, int n_y, int c_x, int c_y, int m_x
     return score;
 }
 
+// Rule kinds a caller is compiled for, one bit per kind.  Kernels built for
+// modularity or CPM alone leave the WCC and conductance formulas (and the
+// registers they take) out; RK_ALL dispatches on r.kind at run time.
+constexpr unsigned RK_MOD = 1u << PCD_RULE_MODULARITY;
+constexpr unsigned RK_CPM = 1u << PCD_RULE_CPM;
+constexpr unsigned RK_WCC = 1u << PCD_RULE_WCC;
+constexpr unsigned RK_COND = 1u << PCD_RULE_CONDUCTANCE;
+constexpr unsigned RK_ALL = RK_MOD | RK_CPM | RK_WCC | RK_COND;
+
 // Rule::score_join (rules.cpp:110-119, 212-215, 479-493, 169-178)
+template <unsigned KM = RK_ALL>
 PCD_HD inline double score_join(const RuleParams &r, int deg, CommStats c, int num) {
-    switch (r.kind) {
-    case PCD_RULE_MODULARITY: {
+    if ((KM & RK_MOD) && (KM == RK_MOD || r.kind == PCD_RULE_MODULARITY)) {
         double e_ij = num / (double)r.m;
         double a_i = (c.int_vol + c.ext_vol) / (2.0 * r.m);
         double a_j = deg / (2.0 * r.m);
         return e_ij - 2 * a_i * a_j;
     }
-    case PCD_RULE_CPM:
-        return num - r.lambda * c.size;
-    case PCD_RULE_WCC:
+    if ((KM & RK_CPM) && (KM == RK_CPM || r.kind == PCD_RULE_CPM)) return num - r.lambda * c.size;
+    if ((KM & RK_WCC) && (KM == RK_WCC || r.kind == PCD_RULE_WCC))
         return wcc_improvement(1, c.size, 1, num, 0, c.int_vol / 2, num, deg, c.ext_vol, r.cc);
-    default: {
+    if (KM & RK_COND) {
         int s = c.size, i = c.int_vol, v = i + c.ext_vol;
         return wrap_mul(s + 1, i + 2 * num) / (double)(v + deg) - s * (i / (double)v);
     }
-    }
+    return 0.0;  // kind outside KM: never called so
 }
 
 // Rule::score_leave (rules.cpp:121-132, 217-220, 495-512, 180-190)
+template <unsigned KM = RK_ALL>
 PCD_HD inline double score_leave(const RuleParams &r, int deg, CommStats c, int num) {
-    switch (r.kind) {
-    case PCD_RULE_MODULARITY: {
+    if ((KM & RK_MOD) && (KM == RK_MOD || r.kind == PCD_RULE_MODULARITY)) {
         double e_ij = num / (double)r.m;
         double a_i = (c.int_vol + c.ext_vol - deg) / (2.0 * r.m);
         double a_j = deg / (2.0 * r.m);
         return -(e_ij - 2 * a_i * a_j);
     }
-    case PCD_RULE_CPM:
-        return -(num - r.lambda * (c.size - 1));
-    case PCD_RULE_WCC:
+    if ((KM & RK_CPM) && (KM == RK_CPM || r.kind == PCD_RULE_CPM)) return -(num - r.lambda * (c.size - 1));
+    if ((KM & RK_WCC) && (KM == RK_WCC || r.kind == PCD_RULE_WCC))
         return -wcc_improvement(1, c.size - 1, 1, num, 0, c.int_vol / 2 - num, num, deg, c.ext_vol - deg + 2 * num,
                                 r.cc);
-    default: {
+    if (KM & RK_COND) {
         int s = c.size, i = c.int_vol, v = i + c.ext_vol;
         return wrap_mul(s - 1, i - 2 * num) / (double)(v - deg) - s * (i / (double)v);
     }
-    }
+    return 0.0;
 }
 
 // Rule::score_merge (rules.cpp:100-108, 207-210, 467-477, 152-167)
