@@ -620,6 +620,8 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.hub_part_entries = value;  // applies to the next graph upload
     } else if (k == "small_aux") {
         ctx->tune.small_aux = value != 0;
+    } else if (k == "block_aux") {
+        ctx->tune.block_aux = value != 0;
     } else if (k == "stats_cache") {
         ctx->tune.stats_cache = value != 0;
     } else if (k == "stat_chunk") {

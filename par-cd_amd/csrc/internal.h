@@ -49,6 +49,7 @@ struct Tuning {
     int hub_table_slots = 4096;  // LDS table slots of a hub partition (multiple of 64, <= HUB_TB)
     int stat_chunk = 2048;       // entries per statistics work item for deg > 64
     int small_aux = 0;           // small bins gather {label, aux} records (1) or labels (0)
+    int block_aux = 0;           // wave/workgroup-per-vertex bins gather {label, aux} records
     int stats_cache = 1;         // statistics over long rows read the labels of the first
                                  // STATS_CACHE vertices (the highest degrees) from LDS
 };

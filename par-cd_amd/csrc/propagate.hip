@@ -356,9 +356,25 @@ __device__ inline void pinsert(int *keys, int *cnts, int mask, int log_t, int ke
     }
 }
 
-template <int PF>
-__device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int mask, int log_t, int start, int deg,
-                                     int lo, int stride) {
+// As pinsert; the lane that claims a slot also stores the label's aux.
+__device__ inline void pinsert_aux(int *keys, int *cnts, int *auxs, int mask, int log_t, int key, int aux) {
+    uint32_t h = pslot(key, log_t);
+    while (true) {
+        int old = atomicCAS(&keys[h], EMPTY_KEY, key);
+        if (old == EMPTY_KEY) auxs[h] = aux;
+        if (old == EMPTY_KEY || old == key) {
+            atomicAdd(&cnts[h], 1);
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+// With AUX the neighbour records {label, aux} are gathered (one 8-byte
+// gather brings what the gains read) and aux lands in the table.
+template <int PF, bool AUX = false>
+__device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int *auxs, int mask, int log_t, int start,
+                                     int deg, int lo, int stride) {
     for (int j0 = lo; j0 < deg; j0 += PF * stride) {
         int nb[PF];
 #pragma unroll
@@ -366,24 +382,29 @@ __device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int 
             int j = j0 + e * stride;
             nb[e] = j < deg ? a.adj[start + j] : -1;
         }
-        int lab[PF];
-#pragma unroll
-        for (int e = 0; e < PF; e++) lab[e] = nb[e] >= 0 ? a.labels[nb[e]] : EMPTY_KEY;
+        int2 rec[PF];
 #pragma unroll
         for (int e = 0; e < PF; e++)
-            if (lab[e] != EMPTY_KEY) pinsert(keys, cnts, mask, log_t, lab[e]);
+            rec[e] = nb[e] < 0 ? make_int2(EMPTY_KEY, 0) : (AUX ? a.lv[nb[e]] : make_int2(a.labels[nb[e]], 0));
+#pragma unroll
+        for (int e = 0; e < PF; e++) {
+            if (rec[e].x == EMPTY_KEY) continue;
+            if (AUX) pinsert_aux(keys, cnts, auxs, mask, log_t, rec[e].x, rec[e].y);
+            else pinsert(keys, cnts, mask, log_t, rec[e].x);
+        }
     }
 }
 
 __device__ inline int ceil_log2(int x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
 
 // ------------------------------------------------------------- wave / vertex
-template <int TW, unsigned KM, class Src>
+template <int TW, unsigned KM, class Src, bool AUX = false>
 __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
     __shared__ int s_keys[256 / WAVE][TW];
     __shared__ int s_cnts[256 / WAVE][TW];
+    __shared__ int s_auxs[256 / WAVE][AUX ? TW : 1];
     const int lane = __lane_id(), w = threadIdx.x / WAVE;
-    int *keys = s_keys[w], *cnts = s_cnts[w];
+    int *keys = s_keys[w], *cnts = s_cnts[w], *auxs = s_auxs[w];
     const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
     const int nwaves = gridDim.x * blockDim.x / WAVE;
     const int nunits = src.units();
@@ -397,19 +418,23 @@ __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
             if (lane == 0) a.new_labels[v] = a.labels[v];
             continue;
         }
-        int start = a.indices[v], deg = a.indices[v + 1] - start, own = a.labels[v];
+        int start = a.indices[v], deg = a.indices[v + 1] - start;
+        int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
+        int own = ownr.x;
         int log_t = max(6, ceil_log2(2 * deg));
-        Table t{keys, cnts, nullptr, 1 << log_t};
+        Table t{keys, cnts, AUX ? auxs : nullptr, 1 << log_t};
         tclear(t, lane, WAVE);
         wave_sync();
-        insert_labels<4>(a, keys, cnts, t.T - 1, log_t, start, deg, lane, WAVE);
+        insert_labels<4, AUX>(a, keys, cnts, auxs, t.T - 1, log_t, start, deg, lane, WAVE);
         wave_sync();
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_table<false, KM>(a, t, lane, WAVE, deg, own, tb, c, own_cnt, distinct);
+        scan_table<AUX, KM>(a, t, lane, WAVE, deg, own, tb, c, own_cnt, distinct);
         wave_reduce_cand(c, own_cnt);
-        if (lane == 0) a.new_labels[v] = lp_decide_stats<KM>(a, deg, own, own_cnt, c);
+        if (lane == 0)
+            a.new_labels[v] = AUX ? lp_decide<KM>(a, deg, own, ownr.y, own_cnt, c)
+                                  : lp_decide_stats<KM>(a, deg, own, own_cnt, c);
         if (a.counters) {
             distinct = wave_sum(distinct);
             if (lane == 0) count3(a, 1, deg, distinct);
@@ -421,10 +446,11 @@ __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
 
 // ------------------------------------------------------------- block / vertex
 // Table slots: nextpow2(2 deg), at most TB.
-template <int NT, int TB, unsigned KM, class Src>
+template <int NT, int TB, unsigned KM, class Src, bool AUX = false>
 __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
     __shared__ int keys[TB];
     __shared__ int cnts[TB];
+    __shared__ int auxs[AUX ? TB : 1];
     __shared__ Cand s_c[NT / WAVE];
     __shared__ int s_i[2 * NT / WAVE];
     const int nunits = src.units();
@@ -438,20 +464,23 @@ __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
             if (threadIdx.x == 0) a.new_labels[v] = a.labels[v];
             continue;
         }
-        int start = a.indices[v], deg = a.indices[v + 1] - start, own = a.labels[v];
+        int start = a.indices[v], deg = a.indices[v + 1] - start;
+        int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
+        int own = ownr.x;
         int log_t = max(6, ceil_log2(2 * deg));
-        Table t{keys, cnts, nullptr, 1 << log_t};
+        Table t{keys, cnts, AUX ? auxs : nullptr, 1 << log_t};
         tclear(t, threadIdx.x, NT);
         __syncthreads();
-        insert_labels<4>(a, keys, cnts, t.T - 1, log_t, start, deg, threadIdx.x, NT);
+        insert_labels<4, AUX>(a, keys, cnts, auxs, t.T - 1, log_t, start, deg, threadIdx.x, NT);
         __syncthreads();
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_table<false, KM>(a, t, threadIdx.x, NT, deg, own, tb, c, own_cnt, distinct);
+        scan_table<AUX, KM>(a, t, threadIdx.x, NT, deg, own, tb, c, own_cnt, distinct);
         block_reduce_cand<NT>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
-            a.new_labels[v] = lp_decide_stats<KM>(a, deg, own, own_cnt, c);
+            a.new_labels[v] = AUX ? lp_decide<KM>(a, deg, own, ownr.y, own_cnt, c)
+                                  : lp_decide_stats<KM>(a, deg, own, own_cnt, c);
             if (a.counters) count3(a, 1, deg, distinct);
         }
       }
@@ -723,7 +752,8 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
     // {label, aux} records are read by the hub path and, when tuned so, by
     // the small bins; rules without aux read the statistics instead
     const bool small_aux = a.use_aux && ctx->tune.small_aux;
-    if (a.use_aux && (ctx->num_hubs > 0 || small_aux)) {
+    const bool block_aux = a.use_aux && ctx->tune.block_aux;
+    if (a.use_aux && (ctx->num_hubs > 0 || small_aux || block_aux)) {
         kt_begin(ctx);
         hipLaunchKernelGGL(k_lp_records, dim3(grid_for(ctx, ctx->n, 256, 8)), dim3(256), 0, ctx->stream, ctx->n,
                            ctx->d_labels, ctx->d_stats, a.rule.kind, ctx->d_lv);
@@ -763,20 +793,29 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
     }
     if (range(BIN_B4096, p0, p1)) {
         kt_begin(ctx);
+        // (a 12-byte-slot table of 8192 entries would leave one workgroup per CU)
         hipLaunchKernelGGL((k_lp_block<512, 8192, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0,
                            ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B4096)});
         kt_end(ctx, KS_LP_B4096);
     }
     if (range(BIN_B1024, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256), 0,
-                           ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
+        if (block_aux)
+            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - p0, 1, 8)),
+                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
+        else
+            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256),
+                               0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
         kt_end(ctx, KS_LP_B1024);
     }
     if (range(BIN_W256, p0, p1)) {
         kt_begin(ctx);
-        hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256), 0,
-                           ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_W256)});
+        if (block_aux)
+            hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256),
+                               0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_W256)});
+        else
+            hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256), 0,
+                               ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_W256)});
         kt_end(ctx, KS_LP_W256);
     }
 #define PCD_SMALL(G, CP, BIN, SLOT)                                                                            \

@@ -136,14 +136,17 @@ def test_propagation_golden(ctx, kind, lam, tie):
                 _lp_compare(ctx, g, lab, kind, lam, frac, seed=1234, step=3, tie=tie)
 
 
-@pytest.mark.parametrize("small_aux", [0, 1])
+@pytest.mark.parametrize("aux", [0, 1])
 @pytest.mark.parametrize("tie", [O.TIE_RANDOM, O.TIE_LOWEST])
 @pytest.mark.parametrize("kind,lam", RULES)
-def test_propagation_bin_zoo(ctx, kind, lam, tie, small_aux):
+def test_propagation_bin_zoo(ctx, kind, lam, tie, aux):
+    """Every degree bin; aux=1 switches the small, wave and block bins to
+    {label, aux} record gathers (modularity and CPM)."""
     g = GG.bin_zoo()
     n = g[0]
     rng = np.random.default_rng(3)
-    ctx.set_tuning("small_aux", small_aux)
+    ctx.set_tuning("small_aux", aux)
+    ctx.set_tuning("block_aux", aux)
     try:
         for lab in [GG.singleton_labels(n), rng.integers(0, 200, n), rng.integers(0, 20000, n)]:
             _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=99, step=1, tie=tie)
@@ -152,6 +155,7 @@ def test_propagation_bin_zoo(ctx, kind, lam, tie, small_aux):
         _lp_compare(ctx, gu, rng.integers(0, 500, n), kind, lam, 0.7, seed=5, step=2, tie=tie)
     finally:
         ctx.set_tuning("small_aux", 0)
+        ctx.set_tuning("block_aux", 0)
 
 
 @pytest.mark.parametrize("slots,part_entries", [(4096, 2048), (64, 2048), (1024, 512), (64, 100000), (4096, 100000)])
