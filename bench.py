@@ -89,13 +89,17 @@ def lp_bin_bytes(active, entries, distinct, S):
 SLOT_OF_BIN = {"hub": "lp_hub", "b4096": "lp_b4096", "b1024": "lp_b1024", "w256": "lp_w256",
                "g64": "lp_g64", "g32": "lp_g32", "g16": "lp_g16", "g8": "lp_g8", "g4": "lp_g4"}
 
-# kernel-timer slot -> kernel names in a rocprofv3 trace of this build
+# kernel-timer slot -> patterns of its kernel names in a rocprofv3 trace of
+# this build (modularity build, template argument 1u)
 ROCPROF_NAMES = {
-    "stats_items": ["pcd::k_stats_items"], "lp_b4096": ["pcd::k_lp_block<512, 8192>"],
-    "lp_b1024": ["pcd::k_lp_block<256, 2048>"], "lp_w256": ["pcd::k_lp_wave<512>"],
-    "lp_hub": ["pcd::k_hub_partition", "pcd::k_hub_count"], "lp_g64": ["pcd::k_lp_small<64, 8, false>"],
-    "lp_g32": ["pcd::k_lp_small<32, 16, false>"], "lp_g16": ["pcd::k_lp_small<16, 32, false>"],
-    "lp_g8": ["pcd::k_lp_small<8, 64, false>"], "lp_g4": ["pcd::k_lp_small<4, 64, false>"]}
+    "stats_items": [r"pcd::k_stats_items(_cached<\d+>)?$"],
+    "lp_b4096": [r"pcd::k_lp_block<512, 8192, 1u, pcd::RangeSrc"],
+    "lp_b1024": [r"pcd::k_lp_block<256, 2048, 1u, pcd::RangeSrc"],
+    "lp_w256": [r"pcd::k_lp_wave<512, 1u, pcd::RangeSrc"],
+    "lp_hub": [r"pcd::k_hub_partition$", r"pcd::k_hub_count<1u>$"],
+    "lp_g64": [r"pcd::k_lp_small<64, 8, false, 1u"], "lp_g32": [r"pcd::k_lp_small<32, 16, false, 1u"],
+    "lp_g16": [r"pcd::k_lp_small<16, 32, false, 1u"], "lp_g8": [r"pcd::k_lp_small<8, 64, false, 1u"],
+    "lp_g4": [r"pcd::k_lp_small<4, 64, false, 1u"]}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "rmat22_mod_pmc.md")
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0  # stream-copy calibration of FETCH_SIZE / WRITE_SIZE
 
@@ -104,8 +108,8 @@ def pmc_traffic(slot):
     """HBM-side traffic (bytes per dispatch) of one kernel from the committed
     rocprofv3 counter summary of this workload: FETCH_SIZE x 2.0 + WRITE_SIZE
     x 1.0 (KiB).  None when the summary or the kernel is absent."""
-    names = ROCPROF_NAMES.get(slot)
-    if not names or not os.path.exists(PMC_SUMMARY):
+    pats = ROCPROF_NAMES.get(slot)
+    if not pats or not os.path.exists(PMC_SUMMARY):
         return None
     rows = {}
     for line in open(PMC_SUMMARY):
@@ -115,9 +119,13 @@ def pmc_traffic(slot):
                 rows[cells[0]] = float(cells[3]) * 1024 * FETCH_FACTOR + float(cells[4]) * 1024 * WRITE_FACTOR
             except ValueError:
                 pass
-    if not all(n in rows for n in names):
-        return None
-    return sum(rows[n] for n in names)
+    total = 0.0
+    for pat in pats:
+        hit = [v for k, v in rows.items() if re.match(pat, k)]
+        if len(hit) != 1:
+            return None
+        total += hit[0]
+    return total
 
 
 def _ref_run(exe, pcd_path, kind_arg, iters, cores):
