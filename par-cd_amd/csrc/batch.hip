@@ -277,6 +277,19 @@ __global__ __launch_bounds__(256) void k_b_metric(const WorkItem *tiles, const i
     }
 }
 
+// next = cur for the graphs still running: propagation then writes the
+// active vertices only
+__global__ __launch_bounds__(256) void k_b_copy_next(const WorkItem *tiles, const int *count, const BatchGraph *gs,
+                                                     const BatchState *st) {
+    const int nt = *count;
+    for (int u = blockIdx.x; u < nt; u += gridDim.x) {
+        const WorkItem w = tiles[u];
+        if (st[w.g].status != BG_RUNNING) continue;
+        const BatchGraph &g = gs[w.g];
+        for (int v = w.p0 + (int)threadIdx.x; v < w.p1; v += blockDim.x) g.next[v] = g.cur[v];
+    }
+}
+
 // Score (rules.cpp:9-22, 121-129), stop rule (main.cpp:241-242) and the
 // iteration bookkeeping of main.cpp:262-264, one thread per graph.
 __global__ void k_b_finalize(const BatchGraph *gs, BatchState *st, int nslots) {
@@ -941,6 +954,7 @@ static int launch_round(pcd_batch *b) {
     hipLaunchKernelGGL(k_b_metric, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
     hipLaunchKernelGGL(k_b_finalize, dim3((b->cap + 255) / 256), dim3(256), 0, s, gs, st, b->cap);
     // PropagationRefinement::refine for the graphs still running
+    hipLaunchKernelGGL(k_b_copy_next, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
     PCD_TRY(batch_propagation_launch(s, lp_launch(b)));
     PCD_HIP(hipGetLastError());
     return PCD_OK;

@@ -13,8 +13,9 @@
 // needs.  Rules that read more (WCC, conductance) gather the community
 // record per distinct label instead.
 //
-// Vertex selection (counter hash against frac) is evaluated inline; an
-// inactive vertex writes its own label.  Vertices are processed by degree
+// Vertex selection (counter hash against frac) is evaluated inline.  The
+// next labels start as a copy of the current ones (one bulk copy before the
+// kernels), so kernels write active vertices only.  Vertices are processed by degree
 // bin (positions are degree-descending, so every bin is a contiguous range):
 //
 //   G4..G64   deg <= G: each wave compacts the active vertices of CP
@@ -139,7 +140,6 @@ __global__ __launch_bounds__(256) void k_lp_small(Src src) {
         bool valid = lane < CP && p < q1;
         int v = valid ? vertex_at(a, p) : 0;
         bool act = valid && selected(a.seed, a.step, v, a.f);
-        if (valid && !act) a.new_labels[v] = a.labels[v];
         uint64_t am = __ballot(act);
         int nact = __popcll(am);
         if (act) list[__popcll(am & lt)] = v;
@@ -414,10 +414,7 @@ __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
       if (!src.unit(u, a, q0, q1)) continue;
       for (int p = q0; p < q1; p++) {
         int v = vertex_at(a, p);
-        if (!selected(a.seed, a.step, v, a.f)) {
-            if (lane == 0) a.new_labels[v] = a.labels[v];
-            continue;
-        }
+        if (!selected(a.seed, a.step, v, a.f)) continue;
         int start = a.indices[v], deg = a.indices[v + 1] - start;
         int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
         int own = ownr.x;
@@ -460,10 +457,7 @@ __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
       if (!src.unit(u, a, q0, q1)) continue;
       for (int p = q0; p < q1; p++) {
         int v = vertex_at(a, p);
-        if (!selected(a.seed, a.step, v, a.f)) {
-            if (threadIdx.x == 0) a.new_labels[v] = a.labels[v];
-            continue;
-        }
+        if (!selected(a.seed, a.step, v, a.f)) continue;
         int start = a.indices[v], deg = a.indices[v + 1] - start;
         int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
         int own = ownr.x;
@@ -641,10 +635,7 @@ __global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
     for (int hub = wave_g; hub < h.nhubs; hub += nwaves) {
         const int4 inf = h.info[hub];
         const int v = inf.x, P = inf.y;
-        if (!selected(a.seed, a.step, v, a.f)) {
-            if (lane == 0) a.new_labels[v] = a.labels[v];
-            continue;
-        }
+        if (!selected(a.seed, a.step, v, a.f)) continue;
         if (h.over[hub]) continue;  // recounted by the global-table path
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
@@ -751,6 +742,9 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
     };
     // {label, aux} records are read by the hub path and, when tuned so, by
     // the small bins; rules without aux read the statistics instead
+    // inactive vertices keep their label: copy all, the kernels overwrite the active ones
+    PCD_HIP(hipMemcpyAsync(a.new_labels, a.labels, sizeof(int32_t) * (size_t)ctx->n, hipMemcpyDeviceToDevice,
+                           ctx->stream));
     const bool small_aux = a.use_aux && ctx->tune.small_aux;
     const bool block_aux = a.use_aux && ctx->tune.block_aux;
     if (a.use_aux && (ctx->num_hubs > 0 || small_aux || block_aux)) {
