@@ -336,7 +336,7 @@ def batch_leg(parcd, np, graphs_count, seed=4):
     }
 
 
-def serve_leg(parcd, np, count, m_lo, m_hi, qps, seed=5, sample=24):
+def serve_leg(parcd, np, count, m_lo, m_hi, qps, seed=5, sample=24, batch_max_edges=1 << 17):
     """Serving form: `count` requests (pcd_gen_requests, seed 5: SBM or
     R-MAT 1/2 each, target edges log-uniform in [m_lo, m_hi], modularity 60 %
     / CPM 0.01 30 % / WCC with the triangle-based initial partition 10 %)
@@ -359,7 +359,8 @@ def serve_leg(parcd, np, count, m_lo, m_hi, qps, seed=5, sample=24):
         jobs.append(parcd.job(kind=int(p[2]), lam=float(p[3]), seed=i, iter_cap=ITER_CAP))
     rng = np.random.default_rng(seed)
     arrive = np.cumsum(rng.exponential(1.0 / qps, count))
-    srv = parcd.Server(0, batch_arena_bytes=24 << 30, batch_slots=2048, plain_workers=4, plain_bytes=64 << 30)
+    srv = parcd.Server(0, batch_arena_bytes=24 << 30, batch_slots=2048, plain_workers=4, plain_bytes=64 << 30,
+                       batch_max_edges=batch_max_edges)
     results = {}
     keep_labels = set(int(x) for x in rng.choice(count, size=min(sample, count), replace=False))
     done_t = {}
@@ -421,7 +422,8 @@ def serve_leg(parcd, np, count, m_lo, m_hi, qps, seed=5, sample=24):
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p90": round(float(np.percentile(lat, 90)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3), "max": round(float(lat.max()), 3)},
         "routed_to_batch": int((routes == 0).sum()), "routed_to_workers": int((routes == 1).sum()),
-        "batch_rounds": int(info["rounds"]), "sample_identical_alone": "%d/%d" % (same, len(keep_labels)),
+        "batch_rounds": int(info["rounds"]), "batch_max_edges": batch_max_edges,
+        "sample_identical_alone": "%d/%d" % (same, len(keep_labels)),
         "capped_at_%d_iterations" % ITER_CAP: int(sum(1 for i in range(count) if results[i][2] > ITER_CAP)),
     }
 
@@ -439,10 +441,14 @@ def main():
                     help="kernel-schedule tunable (pcd_set_tuning); results do not depend on it")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--batch-graphs", type=int, default=4096)
-    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve", action="store_true",
+                    help="run the serving leg (held off by default: see DESIGN.md, serving form)")
+    ap.add_argument("--no-serve", action="store_true", help=argparse.SUPPRESS)  # the default; kept for old commands
     ap.add_argument("--no-wcc", action="store_true", help="skip the WCC / Chung-Lu leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-qps", type=float, default=1000.0)
+    ap.add_argument("--serve-batch-max-edges", type=int, default=1 << 17,
+                    help="requests with more edges run on worker contexts (0: no limit)")
     ap.add_argument("--serve-m", type=float, nargs=2, default=[1e3, 1e6], metavar=("LO", "HI"),
                     help="edge-count range of the requests (the full stream is 20000 requests, 1e3..5e7)")
     ap.add_argument("--no-replay", action="store_true",
@@ -569,8 +575,9 @@ def main():
 
     # ---------------------------------------------------------- serving form
     serve = None
-    if not args.no_serve:
-        serve = serve_leg(parcd, np, args.serve_requests, int(args.serve_m[0]), int(args.serve_m[1]), args.serve_qps)
+    if args.serve:
+        serve = serve_leg(parcd, np, args.serve_requests, int(args.serve_m[0]), int(args.serve_m[1]), args.serve_qps,
+                          batch_max_edges=args.serve_batch_max_edges)
         log("serve:", json.dumps(serve))
 
     # ---------------------------------------------------------- CPU baseline

@@ -246,7 +246,8 @@ int pcd_batch_autotune(pcd_batch *b, int reps, int *chosen, double *ms_per_map);
  * requests into a batch (pcd_batch_*) as slots and arena bytes free up, runs
  * one round at a time for everything resident and retires finished graphs;
  * requests the batch does not take (WCC, initial partition 1, degrees above
- * 4096, graphs larger than the arena) run on worker threads with their own
+ * 4096, more than batch_max_edges edges, graphs larger than the arena) run on
+ * worker threads with their own
  * contexts and streams, admitted against plain_bytes.  Each result equals
  * pcd_refine (-s 0) of the same request alone.  No reference counterpart. */
 typedef struct pcd_server pcd_server;
@@ -256,6 +257,7 @@ typedef struct pcd_server_config {
     int batch_slots;           /* graphs resident in the batch at once */
     int plain_workers;         /* worker threads, one context and stream each */
     int64_t plain_bytes;       /* device-byte budget of the workers together */
+    int64_t batch_max_edges;   /* graphs with more edges go to the workers (0: no limit) */
 } pcd_server_config;
 
 int pcd_server_create(const pcd_server_config *cfg, pcd_server **out);

@@ -6,10 +6,12 @@
 // retires the graphs that stopped: a request that arrives while others run
 // joins at the next round boundary, and every graph keeps its labels and
 // statistics on the device until it finishes.  Requests the batch does not
-// take (WCC, the triangle-based initial partition, degrees above 4096, or
-// graphs larger than the whole arena) go to worker threads that each own a
-// plain context and stream, admitted against their own device-byte budget;
-// their kernels run concurrently with the batch rounds.
+// take go to worker threads that each own a plain context and stream,
+// admitted against their own device-byte budget; their kernels run
+// concurrently with the batch rounds.  Those are WCC, the triangle-based
+// initial partition, degrees above 4096, graphs above the edge limit (their
+// iterations would set the length of every round) and graphs larger than
+// the whole arena.
 //
 // A request's result equals the result of running it alone (pcd_refine on a
 // context with the same job): the batch gives every graph the numbering and
@@ -305,7 +307,8 @@ extern "C" int pcd_server_submit(pcd_server *s, int n, int m, const int32_t *ind
     int64_t bfp = 0;
     pcd_batch_footprint(n, std::max(m, 1), &bfp);
     const bool batchable = (job->rule.kind == PCD_RULE_MODULARITY || job->rule.kind == PCD_RULE_CPM) &&
-                           initial == 0 && r->maxd <= 4096 && m > 0 && bfp <= s->cfg.batch_arena_bytes;
+                           initial == 0 && r->maxd <= 4096 && m > 0 && bfp <= s->cfg.batch_arena_bytes &&
+                           (s->cfg.batch_max_edges <= 0 || m <= s->cfg.batch_max_edges);
     if (!batchable && s->cfg.plain_workers == 0) {
         delete r;
         return fail(PCD_E_INVALID, "request needs a plain worker and the server has none");

@@ -47,7 +47,7 @@ class Rule(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("batch_arena_bytes", C.c_int64), ("batch_slots", C.c_int),
-                ("plain_workers", C.c_int), ("plain_bytes", C.c_int64)]
+                ("plain_workers", C.c_int), ("plain_bytes", C.c_int64), ("batch_max_edges", C.c_int64)]
 
 
 class Job(C.Structure):
@@ -630,9 +630,11 @@ class Server:
     """Continuous-batching request loop (pcd_server_*).  Keeps the arrays of
     every request alive until its result is taken."""
 
-    def __init__(self, device=0, batch_arena_bytes=1 << 30, batch_slots=256, plain_workers=2, plain_bytes=8 << 30):
+    def __init__(self, device=0, batch_arena_bytes=1 << 30, batch_slots=256, plain_workers=2, plain_bytes=8 << 30,
+                 batch_max_edges=0):
         self._s = C.c_void_p()
-        cfg = ServerConfig(device, int(batch_arena_bytes), int(batch_slots), int(plain_workers), int(plain_bytes))
+        cfg = ServerConfig(device, int(batch_arena_bytes), int(batch_slots), int(plain_workers), int(plain_bytes),
+                           int(batch_max_edges))
         check(lib().pcd_server_create(C.byref(cfg), C.byref(self._s)))
         self._keep = {}
 

@@ -9,7 +9,11 @@ import pytest
 import oracle_lib as O
 from gpu_graphs import bin_zoo, rmat, sbm, singleton_labels
 
-pytestmark = pytest.mark.gpu
+# A GPU call that began with these tests ended with the box faulted before
+# any output (no cause found in the code): kept off the GPU
+# until a cause is shown.
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skip(reason="serving path held off the GPU after an unexplained box fault")]
 parcd = pytest.importorskip("parcd")
 
 
