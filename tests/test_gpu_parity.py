@@ -104,13 +104,20 @@ def test_partition_against_golden_fixtures(ctx):
             assert (np.isnan(got) and np.isnan(want)) or got == pytest.approx(want, rel=1e-9, abs=1e-12)
 
 
-def test_partition_bin_zoo_and_unsorted(ctx):
+@pytest.mark.parametrize("stats_cache", [0, 1])
+def test_partition_bin_zoo_and_unsorted(ctx, stats_cache):
+    """Statistics over long rows with and without the LDS label cache."""
     g = GG.bin_zoo()
     rng = np.random.default_rng(5)
     n = g[0]
-    for gg in [g, GG.unsorted_copy(g)]:
-        for lab in [GG.singleton_labels(n), rng.integers(0, 300, n), rng.integers(0, 5, n)]:
-            check_partition(ctx, gg, lab)
+    ctx.set_tuning("stats_cache", stats_cache)
+    try:
+        for gg in [g, GG.unsorted_copy(g), GG.dense_blocks()]:
+            nn = gg[0]
+            for lab in [GG.singleton_labels(nn), rng.integers(0, 300, nn), rng.integers(0, 5, nn)]:
+                check_partition(ctx, gg, lab)
+    finally:
+        ctx.set_tuning("stats_cache", 1)
 
 
 def _lp_compare(ctx, g, lab_in, kind, lam, frac, seed, step, tie, cc=None):
