@@ -177,6 +177,46 @@ def cpu_baseline(pcd_path, E, kind_arg, first, count):
 ITER_CAP = 100
 
 
+def merge_leg(parcd, np, ctx, n, m, init, steps, warmup, seed):
+    """Parallel merging (-s 1, one round) on R-MAT-22 from the singleton
+    partition: compress + statistics + metric + merging step per iteration,
+    the same counting as the propagation iterations.  The contraction inside
+    the step (community-pair keys, LSD radix sort, run-length reduce) is
+    timed per phase; its sort streams 16 bytes per key and pass."""
+    E = 2 * m
+    ctx.upload_labels(init)
+    ctx.enable_timing(True)
+    ks = []
+    it = 0
+    for _ in range(warmup):
+        ctx.compress(); ctx.statistics(); ctx.metric(parcd.RULE_MODULARITY)
+        ctx.merge(parcd.RULE_MODULARITY, rounds=1, seed=seed, step=it)
+        it += 1
+    ctx.synchronize()
+    ctx.kernel_timer_reset()
+    scores = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ks.append(ctx.compress()); ctx.statistics(); scores.append(ctx.metric(parcd.RULE_MODULARITY))
+        ctx.merge(parcd.RULE_MODULARITY, rounds=1, seed=seed, step=it)
+        it += 1
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    timers = ctx.kernel_timers()
+    ms = (t1 - t0) * 1000 / steps
+    sort_ms = timers.get("merge_sort", (0.0, 0))[0] / steps
+    # LSD passes of 8 bits over 2 x ceil(log2(k+1)) key bits, every entry a key
+    passes = [((2 * max(1, int(np.ceil(np.log2(k + 2))))) + 7) // 8 for k in ks]
+    sort_bytes = sum(16 * E * p for p in passes) / steps
+    return {
+        "ms_per_iteration": round(ms, 3), "gteps": round(E / (ms / 1000) / 1e9, 2),
+        "modularity_scores": [round(x, 6) for x in scores], "communities": ks,
+        "phase_ms_per_iteration": {k: round(v[0] / steps, 3) for k, v in timers.items() if k.startswith("merge")},
+        "sort_passes": passes, "sort_gbs": round(sort_bytes / (sort_ms / 1000) / 1e9, 1) if sort_ms else None,
+        "sort_hbm_frac": round(sort_bytes / (sort_ms / 1000) / 1e9 / HBM_PEAK_GBS, 3) if sort_ms else None,
+    }
+
+
 def wcc_leg(parcd, np, steps, warmup):
     """WCC on the Chung-Lu graph of config 2 (seed 3: 10^6 vertices, expected
     degrees from a power law with exponent 2.3 on [4, 5000], 32 planted
@@ -445,6 +485,7 @@ def main():
                     help="run the serving leg (held off by default: see DESIGN.md, serving form)")
     ap.add_argument("--no-serve", action="store_true", help=argparse.SUPPRESS)  # the default; kept for old commands
     ap.add_argument("--no-wcc", action="store_true", help="skip the WCC / Chung-Lu leg")
+    ap.add_argument("--no-merge", action="store_true", help="skip the merging (-s 1) leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--serve-batch-max-edges", type=int, default=1 << 17,
@@ -551,6 +592,12 @@ def main():
                     "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if traffic else None,
                     "bytes_per_launch": int(d["bytes_per_launch"]), "avg_launch_ms": round(d["avg_ms"], 4)}
 
+    # ---------------------------------------------------------- merging (-s 1)
+    merge = None
+    if not args.no_merge:
+        merge = merge_leg(parcd, np, ctx, n, m, init, args.steps, args.warmup, args.seed)
+        log("merge:", json.dumps(merge))
+
     # ---------------------------------------------------------- CPM 0.01
     cpm = None
     if not args.no_cpm:
@@ -607,7 +654,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "stream_copy_gbs": round(copy_gbs, 1),
         "speedup_vs_cpu": round(gteps / cpu["value"], 2) if cpu else None,
         "phases": {"propagate": lp_phase, "statistics": stats_phase, "kernels": per_kernel},
-        "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm, "wcc_chung_lu": wcc, "batch": batch, "serve": serve,
+        "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm, "merge_s1": merge, "wcc_chung_lu": wcc, "batch": batch, "serve": serve,
     }
     print(json.dumps(out), flush=True)
 

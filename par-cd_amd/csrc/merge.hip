@@ -206,21 +206,63 @@ struct MergeArgs {
     unsigned long long *best_ord, *best_key;
 };
 
+// Unique pairs arrive sorted by l, so neighbouring lanes mostly share l.
+// Each wave first reduces within its runs of equal l (segmented shuffle
+// scan from the run heads); only the last lane of a run updates memory.
+__device__ inline uint64_t shfl_up_u64(uint64_t x, int d) {
+    int lo = __shfl_up((int)(uint32_t)x, d, WAVE), hi = __shfl_up((int)(uint32_t)(x >> 32), d, WAVE);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+template <bool MAX>
+__device__ inline uint64_t segmented_scan(uint64_t v, int lane, int seg_start) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        uint64_t o = shfl_up_u64(v, d);
+        if (lane - d >= seg_start) v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+    }
+    return v;
+}
+
 template <int PHASE>
 __global__ __launch_bounds__(256) void k_merge_pass(MergeArgs a) {
     const uint64_t mask = (1ULL << a.half) - 1;
-    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < a.U; u += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t key = a.ukeys[u];
-        int l = (int)(key >> a.half), kk = (int)(key & mask);
-        if (!eligible(l, a.found, a.bp) || a.found[kk]) continue;
-        double s = score_merge(a.rule, a.stats[l], a.stats[kk], a.links[u]);
-        if (!(s >= 0.0)) continue;  // negative or NaN: never beats best=0
-        uint64_t ord = score_ord(s);
-        if (PHASE == 0) {
-            atomicMax(&a.best_ord[l], (unsigned long long)ord);
-        } else if (ord == a.best_ord[l]) {
-            uint32_t tk = tie_key(a.tie, a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, l), kk);
-            atomicMin(&a.best_key[l], ((unsigned long long)tk << 32) | (uint32_t)kk);
+    const int lane = __lane_id();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // whole waves iterate together (the scan needs every lane)
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < a.U; base += stride) {
+        const int64_t u = base + lane;
+        const bool in = u < a.U;
+        int l = -1, kk = 0;
+        uint64_t val = PHASE == 0 ? 0ULL : ~0ULL;
+        if (in) {
+            uint64_t key = a.ukeys[u];
+            l = (int)(key >> a.half);
+            kk = (int)(key & mask);
+            if (eligible(l, a.found, a.bp) && !a.found[kk]) {
+                double s = score_merge(a.rule, a.stats[l], a.stats[kk], a.links[u]);
+                if (s >= 0.0) {  // negative or NaN never beats best = 0
+                    uint64_t ord = score_ord(s);
+                    if (PHASE == 0) {
+                        val = ord;
+                    } else if (ord == a.best_ord[l]) {
+                        uint32_t tk = tie_key(a.tie, a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, l), kk);
+                        val = ((uint64_t)tk << 32) | (uint32_t)kk;
+                    }
+                }
+            }
+        }
+        const int prev = __shfl_up(l, 1, WAVE);
+        const uint64_t heads = __ballot(lane == 0 || l != prev);
+        const int seg_start = 63 - __clzll(heads & (lane == 63 ? ~0ULL : ((2ULL << lane) - 1)));
+        val = segmented_scan<PHASE == 0>(val, lane, seg_start);
+        const bool tail = lane == WAVE - 1 || ((heads >> (lane + 1)) & 1);
+        if (in && tail) {
+            if (PHASE == 0) {
+                if (val) atomicMax(&a.best_ord[l], (unsigned long long)val);
+            } else if (val != ~0ULL) {
+                atomicMin(&a.best_key[l], (unsigned long long)val);
+            }
         }
     }
 }
