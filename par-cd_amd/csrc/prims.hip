@@ -62,16 +62,19 @@ int ensure_scan_parts(pcd_ctx *ctx, int64_t n) {
 }
 
 // ------------------------------------------------------------ radix sort
-// LSD, 8-bit digits.  Per pass: tile histograms (digit-major), one
-// exclusive scan over all (digit, tile) counters, and a stable scatter that
-// ranks keys inside a tile with wave ballots (8 ballots give the lanes that
-// share a digit) and per-round LDS digit counters.
+// LSD, 8-bit digits.  Per pass: tile histograms (digit-major, kept twice:
+// raw counts and, after one exclusive scan over all (digit, tile)
+// counters, global offsets), then a scatter that first sorts its tile
+// stably by digit in LDS -- ranks from wave ballots (8 ballots give the
+// lanes that share a digit) and per-round digit counters -- and then writes
+// each digit's run to its global offset with consecutive threads on
+// consecutive addresses.
 constexpr int RS_THREADS = 256;
 constexpr int RS_ROUNDS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, int64_t n, int shift, int ntiles,
-                                                        int32_t *hist) {
+                                                        int32_t *hist, int32_t *raw) {
     __shared__ int cnt[256];
     cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -81,20 +84,28 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, in
         if (i < n) atomicAdd(&cnt[(int)((keys[i] >> shift) & 0xFF)], 1);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+    const int64_t o = (int64_t)threadIdx.x * ntiles + blockIdx.x;
+    hist[o] = cnt[threadIdx.x];
+    raw[o] = cnt[threadIdx.x];
 }
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys, uint64_t *out, int64_t n, int shift,
-                                                           int ntiles, const int32_t *hist) {
+                                                           int ntiles, const int32_t *hist, const int32_t *raw) {
+    __shared__ uint64_t sk[RS_TILE];
     __shared__ int gbase[256];
+    __shared__ int lstart[256];
     __shared__ int run[256];
     __shared__ int wcnt[RS_THREADS / WAVE][256];
+    __shared__ int scan_lds[RS_THREADS / WAVE + 1];
     const int t = threadIdx.x, lane = __lane_id(), w = t / WAVE;
     gbase[t] = hist[(int64_t)t * ntiles + blockIdx.x];
+    int tot;
+    lstart[t] = block_excl_scan(raw[(int64_t)t * ntiles + blockIdx.x], &tot, scan_lds);
     run[t] = 0;
     for (int i = 0; i < RS_THREADS / WAVE; i++) wcnt[i][t] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int count = (int)min((int64_t)RS_TILE, n - base);
     const uint64_t lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
     for (int r = 0; r < RS_ROUNDS; r++) {
         int64_t i = base + (int64_t)r * RS_THREADS + t;
@@ -113,13 +124,19 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys,
         if (valid) {
             int off = run[d] + rank;
             for (int j = 0; j < w; j++) off += wcnt[j][d];
-            out[gbase[d] + off] = key;
+            sk[lstart[d] + off] = key;
         }
         __syncthreads();
         int add = 0;
         for (int j = 0; j < RS_THREADS / WAVE; j++) { add += wcnt[j][t]; wcnt[j][t] = 0; }
         run[t] += add;
         __syncthreads();
+    }
+    // the tile is now sorted by digit: write each run to its global place
+    for (int i = t; i < count; i += RS_THREADS) {
+        uint64_t key = sk[i];
+        int d = (int)((key >> shift) & 0xFF);
+        out[gbase[d] + (i - lstart[d])] = key;
     }
 }
 
@@ -139,19 +156,20 @@ int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int b
     if (n <= 1) return PCD_OK;
     int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
     size_t need = (size_t)ntiles * 256;
-    if (need > ctx->rs_hist_cap) {
+    if (2 * need > ctx->rs_hist_cap) {
         if (ctx->d_rs_hist) dev_free(ctx, ctx->d_rs_hist);
         ctx->d_rs_hist = nullptr;
-        PCD_TRY(dev_alloc_t(ctx, &ctx->d_rs_hist, need));
-        ctx->rs_hist_cap = need;
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_rs_hist, 2 * need));
+        ctx->rs_hist_cap = 2 * need;
     }
+    int32_t *raw = ctx->d_rs_hist + need;
     uint64_t *src = keys, *dst = tmp;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, ctx->stream, src, n, shift, ntiles,
-                           ctx->d_rs_hist);
+                           ctx->d_rs_hist, raw);
         PCD_TRY(device_scan(ctx, (int64_t)need, RsLoad{ctx->d_rs_hist}, RsStore{ctx->d_rs_hist}, nullptr));
         hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, ctx->stream, src, dst, n, shift, ntiles,
-                           ctx->d_rs_hist);
+                           ctx->d_rs_hist, raw);
         PCD_HIP(hipGetLastError());
         uint64_t *t = src; src = dst; dst = t;
     }
