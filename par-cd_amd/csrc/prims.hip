@@ -79,9 +79,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, in
     cnt[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    uint64_t kv[RS_ROUNDS];
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
         int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(int)((keys[i] >> shift) & 0xFF)], 1);
+        kv[r] = i < n ? keys[i] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(int)((kv[r] >> shift) & 0xFF)], 1);
     }
     __syncthreads();
     const int64_t o = (int64_t)threadIdx.x * ntiles + blockIdx.x;
@@ -107,10 +114,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys,
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     const int count = (int)min((int64_t)RS_TILE, n - base);
     const uint64_t lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    uint64_t kv[RS_ROUNDS];  // all of the thread's keys in flight at once
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        int64_t i = base + (int64_t)r * RS_THREADS + t;
+        kv[r] = i < n ? keys[i] : 0;
+    }
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
         int64_t i = base + (int64_t)r * RS_THREADS + t;
         bool valid = i < n;
-        uint64_t key = valid ? keys[i] : 0;
+        uint64_t key = kv[r];
         int d = (int)((key >> shift) & 0xFF);
         uint64_t peers = __ballot(valid);
 #pragma unroll
