@@ -195,6 +195,8 @@ static void free_graph_state(pcd_ctx *ctx) {
     ctx->rs_hist_cap = 0;
     ctx->d_bp = ctx->d_found = nullptr;
     ctx->d_best_ord = ctx->d_best_key = nullptr;
+    ctx->d_mcand = nullptr;
+    ctx->mcand_cap = 0;
     ctx->first_cap = 0;
     ctx->has_graph = ctx->has_labels = ctx->has_stats = ctx->has_triangles = false;
     ctx->labels_compressed = false;

@@ -153,6 +153,8 @@ struct pcd_ctx {
     size_t rs_hist_cap = 0;
     int32_t *d_bp = nullptr, *d_found = nullptr;
     uint64_t *d_best_ord = nullptr, *d_best_key = nullptr;
+    void *d_mcand = nullptr;         // merge candidates of runs crossing waves
+    size_t mcand_cap = 0;
 
     // measurement
     bool timing = false;

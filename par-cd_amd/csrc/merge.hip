@@ -206,26 +206,25 @@ struct MergeArgs {
     unsigned long long *best_ord, *best_key;
 };
 
-// Unique pairs arrive sorted by l, so neighbouring lanes mostly share l.
-// Each wave first reduces within its runs of equal l (segmented shuffle
-// scan from the run heads); only the last lane of a run updates memory.
+// Unique pairs arrive sorted by l, so the pairs of one community are a
+// contiguous run.  One pass: each wave reduces its runs lexicographically
+// (best score, then smallest tie key) with a segmented shuffle scan from the
+// run heads.  A run that lies inside the wave is complete: its last lane
+// stores the result.  A run that touches the wave's first or last lane may
+// continue in a neighbouring wave: its partial result goes to best_ord with
+// atomicMax and to one of two candidate slots of its 64 pairs, and a second
+// pass over the slots keeps the smallest key among the maxima.
 __device__ inline uint64_t shfl_up_u64(uint64_t x, int d) {
     int lo = __shfl_up((int)(uint32_t)x, d, WAVE), hi = __shfl_up((int)(uint32_t)(x >> 32), d, WAVE);
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
-template <bool MAX>
-__device__ inline uint64_t segmented_scan(uint64_t v, int lane, int seg_start) {
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        uint64_t o = shfl_up_u64(v, d);
-        if (lane - d >= seg_start) v = MAX ? (o > v ? o : v) : (o < v ? o : v);
-    }
-    return v;
-}
+struct MergeCand {
+    uint64_t ord, key;
+    int l, pad;
+};
 
-template <int PHASE>
-__global__ __launch_bounds__(256) void k_merge_pass(MergeArgs a) {
+__global__ __launch_bounds__(256) void k_merge_pass(MergeArgs a, MergeCand *cand) {
     const uint64_t mask = (1ULL << a.half) - 1;
     const int lane = __lane_id();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -234,36 +233,58 @@ __global__ __launch_bounds__(256) void k_merge_pass(MergeArgs a) {
         const int64_t u = base + lane;
         const bool in = u < a.U;
         int l = -1, kk = 0;
-        uint64_t val = PHASE == 0 ? 0ULL : ~0ULL;
+        uint64_t ord = 0, key = ~0ULL;  // "no candidate"
         if (in) {
-            uint64_t key = a.ukeys[u];
-            l = (int)(key >> a.half);
-            kk = (int)(key & mask);
+            uint64_t k64 = a.ukeys[u];
+            l = (int)(k64 >> a.half);
+            kk = (int)(k64 & mask);
             if (eligible(l, a.found, a.bp) && !a.found[kk]) {
                 double s = score_merge(a.rule, a.stats[l], a.stats[kk], a.links[u]);
                 if (s >= 0.0) {  // negative or NaN never beats best = 0
-                    uint64_t ord = score_ord(s);
-                    if (PHASE == 0) {
-                        val = ord;
-                    } else if (ord == a.best_ord[l]) {
-                        uint32_t tk = tie_key(a.tie, a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, l), kk);
-                        val = ((uint64_t)tk << 32) | (uint32_t)kk;
-                    }
+                    ord = score_ord(s);
+                    uint32_t tk = tie_key(a.tie, a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, l), kk);
+                    key = ((uint64_t)tk << 32) | (uint32_t)kk;
                 }
             }
         }
         const int prev = __shfl_up(l, 1, WAVE);
         const uint64_t heads = __ballot(lane == 0 || l != prev);
         const int seg_start = 63 - __clzll(heads & (lane == 63 ? ~0ULL : ((2ULL << lane) - 1)));
-        val = segmented_scan<PHASE == 0>(val, lane, seg_start);
-        const bool tail = lane == WAVE - 1 || ((heads >> (lane + 1)) & 1);
-        if (in && tail) {
-            if (PHASE == 0) {
-                if (val) atomicMax(&a.best_ord[l], (unsigned long long)val);
-            } else if (val != ~0ULL) {
-                atomicMin(&a.best_key[l], (unsigned long long)val);
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            uint64_t o = shfl_up_u64(ord, d), ok = shfl_up_u64(key, d);
+            if (lane - d >= seg_start && (o > ord || (o == ord && ok < key))) {
+                ord = o;
+                key = ok;
             }
         }
+        const bool tail = lane == WAVE - 1 || ((heads >> (lane + 1)) & 1);
+        const bool open = seg_start == 0 || lane == WAVE - 1;  // may continue in another wave
+        const bool has = in && tail && ord != 0;
+        if (has && !open) {
+            a.best_ord[l] = ord;
+            a.best_key[l] = key;
+        }
+        // slot 2w: the run holding lane 0; slot 2w+1: the run holding lane 63
+        // when that is another run; a slot without a candidate is marked empty
+        const int64_t w2 = 2 * (base / WAVE);
+        const bool emit_a = has && open && seg_start == 0;
+        const bool emit_b = has && open && seg_start > 0;
+        const uint64_t ea = __ballot(emit_a), eb = __ballot(emit_b);
+        if (emit_a || emit_b) {
+            atomicMax(&a.best_ord[l], (unsigned long long)ord);
+            cand[w2 + (emit_a ? 0 : 1)] = MergeCand{ord, key, l, 0};
+        }
+        if (lane == 0 && !ea) cand[w2].ord = 0;
+        if (lane == 0 && !eb) cand[w2 + 1].ord = 0;
+    }
+}
+
+// among the partial results of runs that cross waves: smallest key at the maximum score
+__global__ void k_merge_resolve(MergeArgs a, const MergeCand *cand, int64_t nc) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += (int64_t)gridDim.x * blockDim.x) {
+        MergeCand c = cand[i];
+        if (c.ord != 0 && c.ord == a.best_ord[c.l]) atomicMin(&a.best_key[c.l], (unsigned long long)c.key);
     }
 }
 
@@ -313,6 +334,15 @@ int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step
     int half = 0;
     PCD_TRY(build_pairs(ctx, &U, &half));
     PCD_TRY(ensure_merge_state(ctx));
+    // candidate slots: two per 64 pairs (the runs that may cross a wave boundary)
+    const size_t ncap = (size_t)(2 * ((U + WAVE - 1) / WAVE) + 2);
+    if (ncap > ctx->mcand_cap) {
+        if (ctx->d_mcand) dev_free(ctx, ctx->d_mcand);
+        ctx->d_mcand = nullptr;
+        ctx->mcand_cap = 0;
+        PCD_TRY(dev_alloc(ctx, &ctx->d_mcand, sizeof(MergeCand) * ncap));
+        ctx->mcand_cap = ncap;
+    }
     PCD_TRY(fill_i32(ctx, ctx->d_bp, -1, k));
     PCD_TRY(fill_i32(ctx, ctx->d_found, 0, k));
     PCD_TRY(fill_u64(ctx, ctx->d_best_ord, 0, k));
@@ -335,8 +365,11 @@ int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step
         kt_begin(ctx);
         a.seed = merge_seed(seed, rnd);
         if (U > 0) {
-            hipLaunchKernelGGL(k_merge_pass<0>, dim3(grid_for(ctx, U, 256)), dim3(256), 0, ctx->stream, a);
-            hipLaunchKernelGGL(k_merge_pass<1>, dim3(grid_for(ctx, U, 256)), dim3(256), 0, ctx->stream, a);
+            const int64_t nslots = 2 * ((U + WAVE - 1) / WAVE);
+            hipLaunchKernelGGL(k_merge_pass, dim3(grid_for(ctx, U, 256)), dim3(256), 0, ctx->stream, a,
+                               (MergeCand *)ctx->d_mcand);
+            hipLaunchKernelGGL(k_merge_resolve, dim3(grid_for(ctx, nslots, 256)), dim3(256), 0, ctx->stream, a,
+                               (const MergeCand *)ctx->d_mcand, nslots);
         }
         hipLaunchKernelGGL(k_merge_finalize, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_found,
                            ctx->d_bp, a.best_ord, a.best_key);
