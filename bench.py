@@ -267,6 +267,30 @@ def wcc_leg(parcd, np, steps, warmup):
     }, (n, m, ind, adj, mp)
 
 
+def cpu_reference_merge(pcd_path, E, first, count):
+    """The reference binary with parallel merging only (-s 1 -r 1) on the
+    R-MAT-22 file: seconds per iteration of (statistics + metric + merging)
+    over the iterations the GPU's merging leg times."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "main")
+    if not os.path.exists(exe):
+        return None
+    cores = min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(cores))
+    t0 = time.time()
+    r = subprocess.run([exe, pcd_path, "-m", "mod", "-s", "1", "-r", "1", "-n", str(first + count), "-p", str(cores)],
+                       env=env, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=900)
+    out = r.stdout
+    stats = [float(x) for x in re.findall(r"collecting statistics: ([0-9.e+-]+) sec", out)]
+    metric = [float(x) for x in re.findall(r"calculate metric: ([0-9.e+-]+) sec", out)]
+    mg = [float(x) for x in re.findall(r"parallel merging: ([0-9.e+-]+) sec", out)]
+    k = min(len(stats), len(metric), len(mg))
+    per = [stats[i] + metric[i] + mg[i] for i in range(k)][first: first + count]
+    if not per:
+        return None
+    return {"cores": cores, "sec_per_iteration": sum(per) / len(per), "gteps": E / (sum(per) / len(per)) / 1e9,
+            "iterations": "%d-%d" % (first + 1, first + len(per)), "wall_s": round(time.time() - t0, 1)}
+
+
 def cpu_reference_wcc(pcd_path, E, iters=3):
     """The reference binary on the Chung-Lu graph, -m wcc -i 1: its serial
     count_triangles and the per-iteration phases (WCC calculate included)."""
@@ -634,6 +658,8 @@ def main():
             pcd_path = os.path.join(tmp, "rmat.pcd")
             parcd.write_pcd(pcd_path, n, m, ind, adj, mp)
             cpu = cpu_baseline(pcd_path, E, "mod", args.warmup, args.steps)
+            if cpu and merge:
+                cpu["merge_s1"] = cpu_reference_merge(pcd_path, E, args.warmup, args.steps)
             if cpu and wcc_graph:
                 wn, wm, wi, wa, wmp = wcc_graph
                 wpath = os.path.join(tmp, "chung_lu.pcd")
