@@ -322,8 +322,9 @@ def batch_leg(parcd, np, graphs_count, seed=4):
     p_out U[0.001, 0.01], modularity or CPM with resolution U[0.005, 0.05]),
     each refined to convergence (-s 0, default threshold) from the singleton
     partition a fresh ./main process draws (srand(1)).  Timed end to end from
-    host CSR to host labels: (a) one batch -- admission, one round, the bin-map
-    autotuner, rounds until every graph stopped, label download; (b) the same
+    host CSR to host labels: (a) one batch -- admission (pcd_batch_add_many),
+    one round, the bin-map autotuner, rounds until every graph stopped, label
+    download (pcd_batch_labels_many); (b) the same
     graphs one after another through one plain context (upload, pcd_refine,
     download).  Quality: every graph's labels and score from (a) and (b) are
     compared."""
@@ -351,7 +352,7 @@ def batch_leg(parcd, np, graphs_count, seed=4):
     # (a) batched
     t0 = time.perf_counter()
     b = parcd.Batch(0, max_graphs=len(gs), arena_bytes=arena)
-    slots = [b.add(g[0], g[1], g[2], g[3], lab, jb) for g, lab, jb in zip(gs, labs, jobs)]
+    slots = b.add_many([g[:4] for g in gs], labs, jobs)
     assert all(s_ is not None for s_ in slots)
     t_adm = time.perf_counter()
     running = b.step(1)
@@ -360,7 +361,7 @@ def batch_leg(parcd, np, graphs_count, seed=4):
     while running:
         running = b.step(1)
     t_rounds1 = time.perf_counter()
-    res_b = [(b.labels(s_), b.status(s_)) for s_ in slots]
+    res_b = list(zip(b.labels_many(slots), [b.status(s_) for s_ in slots]))
     t1 = time.perf_counter()
     rounds = b.info()["rounds"]
     b.close()

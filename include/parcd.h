@@ -218,6 +218,20 @@ int pcd_batch_footprint(int n, int m, int64_t *bytes);
  * arena space is free: retry after releasing finished graphs. */
 int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices, const int32_t *adj, const int32_t *labels,
                   const pcd_job *job, int *slot);
+/* One graph of pcd_batch_add_many: the arguments of pcd_batch_add. */
+typedef struct pcd_graph_ref {
+    int n, m;
+    const int32_t *indices, *adj, *labels;
+} pcd_graph_ref;
+/* Admits `count` graphs (graphs[i] with jobs[i]) in one call: the host
+ * checks run on several threads and the stream is synchronised once.
+ * slots[i] receives graph i's slot or -1; codes[i] (optional) its
+ * pcd_batch_add return code.  Returns PCD_OK when every graph was admitted,
+ * else the code of the first one that was not (pcd_last_error names it);
+ * the others stay admitted.  Stands for one pcd_batch_add per graph, that
+ * is, one reference main.cpp run per graph (main.cpp:170-225). */
+int pcd_batch_add_many(pcd_batch *b, int count, const pcd_graph_ref *graphs, const pcd_job *jobs, int *slots,
+                       int *codes);
 /* Runs `rounds` rounds; *running receives the number of graphs still
  * running afterwards. */
 int pcd_batch_step(pcd_batch *b, int rounds, int *running);
@@ -228,6 +242,9 @@ int pcd_batch_status(pcd_batch *b, int slot, int *status, int *iterations, int *
 int pcd_batch_finished(pcd_batch *b, int *slots, int cap, int *count);
 /* Final compressed labels / community statistics of a finished graph. */
 int pcd_batch_labels(pcd_batch *b, int slot, int32_t *labels);
+/* Labels of `count` finished graphs in one copy, concatenated in the order
+ * of `slots` (graph i's n labels follow graph i-1's). */
+int pcd_batch_labels_many(pcd_batch *b, int count, const int *slots, int32_t *labels);
 int pcd_batch_stats(pcd_batch *b, int slot, int32_t *sizes, int32_t *int_volume, int32_t *ext_volume);
 /* Frees a slot and its arena space (a running graph is abandoned). */
 int pcd_batch_release(pcd_batch *b, int slot);

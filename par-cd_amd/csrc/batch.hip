@@ -324,6 +324,19 @@ __global__ void k_b_finalize(const BatchGraph *gs, BatchState *st, int nslots) {
     st[i] = s;
 }
 
+
+// Labels of several finished graphs, graph after graph: out[offs[i] + v] =
+// cur[v] of the graph in slots[i].  One workgroup per graph at a time.
+__global__ void __launch_bounds__(256) k_b_gather_labels(const BatchGraph *graphs, const int32_t *slots,
+                                                         const int64_t *offs, int count, int32_t *out) {
+    for (int i = blockIdx.x; i < count; i += gridDim.x) {
+        const BatchGraph &g = graphs[slots[i]];
+        const int32_t *src = g.cur;
+        int32_t *dst = out + offs[i];
+        for (int v = threadIdx.x; v < g.n; v += blockDim.x) dst[v] = src[v];
+    }
+}
+
 }  // namespace pcd
 
 // ===================================================================== host
@@ -409,6 +422,8 @@ struct pcd_batch {
     std::vector<char> slot_listed;      // the slot has items in the lists
     int32_t *d_tile_sum = nullptr, *d_tile_pre = nullptr;
     size_t tile_cap = 0;
+    char *d_gather = nullptr;  // label download staging (pcd_batch_labels_many)
+    size_t gather_cap = 0;
     bool items_dirty = true;
     int64_t listed_done_vertices = 0, listed_vertices = 0;
     // bin map and captured round
@@ -489,6 +504,7 @@ extern "C" int pcd_batch_destroy(pcd_batch *b) {
     if (b->d_counts) (void)hipFree(b->d_counts);
     if (b->d_tile_sum) (void)hipFree(b->d_tile_sum);
     if (b->d_tile_pre) (void)hipFree(b->d_tile_pre);
+    if (b->d_gather) (void)hipFree(b->d_gather);
     if (b->h_state) (void)hipHostFree(b->h_state);
     if (b->h_gtiles) (void)hipHostFree(b->h_gtiles);
     if (b->h_items) (void)hipHostFree(b->h_items);
@@ -562,67 +578,75 @@ extern "C" int pcd_batch_create(int device, int max_graphs, int64_t arena_bytes,
     return PCD_OK;
 }
 
-extern "C" int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices, const int32_t *adj,
-                             const int32_t *labels, const pcd_job *job, int *slot_out) {
-    if (!b || !job || !slot_out) return bfail(PCD_E_INVALID, "null argument");
-    *slot_out = -1;
-    if (n <= 0 || m <= 0 || !indices || !adj || !labels) return bfail(PCD_E_INVALID, "bad graph arguments");
-    if ((int64_t)2 * m > INT32_MAX) return bfail(PCD_E_INVALID, "2m exceeds int32");
+namespace {
+// Host-side preparation of one graph: argument and format checks, the
+// degree order (when the ids are not degree-sorted), the bin boundaries and
+// the statistics row items.  Touches no batch or device state, so graphs
+// can be prepared on several threads at once.
+struct Prepared {
+    int code = PCD_OK;
+    const char *msg = "";
+    bool sorted = true;
+    int pos_le[NUM_TH] = {0};
+    std::vector<int32_t> order;
+    std::vector<RowItem> rows;  // graph index set at admission
+};
+}  // namespace
+
+static void prepare_graph(int n, int m, const int32_t *indices, const int32_t *adj, const int32_t *labels,
+                          const pcd_job *job, bool parallel, Prepared &pr) {
+    auto reject = [&](int code, const char *msg) {
+        pr.code = code;
+        pr.msg = msg;
+    };
+    if (!job) return reject(PCD_E_INVALID, "null job");
+    if (n <= 0 || m <= 0 || !indices || !adj || !labels) return reject(PCD_E_INVALID, "bad graph arguments");
+    if ((int64_t)2 * m > INT32_MAX) return reject(PCD_E_INVALID, "2m exceeds int32");
     if (job->rule.kind != PCD_RULE_MODULARITY && job->rule.kind != PCD_RULE_CPM)
-        return bfail(PCD_E_INVALID, "batches run modularity or CPM");
-    if (!(job->frac > 0.0)) return bfail(PCD_E_INVALID, "frac must be > 0");
-    if (job->tie != PCD_TIE_RANDOM && job->tie != PCD_TIE_LOWEST) return bfail(PCD_E_INVALID, "unknown tie order");
-    if (indices[0] != 0 || indices[n] != 2 * m) return bfail(PCD_E_FORMAT, "indices[0] != 0 or indices[n] != 2m");
+        return reject(PCD_E_INVALID, "batches run modularity or CPM");
+    if (!(job->frac > 0.0)) return reject(PCD_E_INVALID, "frac must be > 0");
+    if (job->tie != PCD_TIE_RANDOM && job->tie != PCD_TIE_LOWEST) return reject(PCD_E_INVALID, "unknown tie order");
+    if (indices[0] != 0 || indices[n] != 2 * m) return reject(PCD_E_FORMAT, "indices[0] != 0 or indices[n] != 2m");
     int maxd = 0;
     bool sorted = true;
     for (int v = 0; v < n; v++) {
         int d = indices[v + 1] - indices[v];
-        if (d < 0) return bfail(PCD_E_FORMAT, "indices must be monotone");
+        if (d < 0) return reject(PCD_E_FORMAT, "indices must be monotone");
         if (v > 0 && d > indices[v] - indices[v - 1]) sorted = false;
         maxd = std::max(maxd, d);
-        if (labels[v] < -1 || labels[v] >= n) return bfail(PCD_E_INVALID, "labels must be in [0,n) or NEW_LABEL");
+        if (labels[v] < -1 || labels[v] >= n) return reject(PCD_E_INVALID, "labels must be in [0,n) or NEW_LABEL");
     }
-    std::atomic<int> bad{0};
-    parallel_for(n, [&](int64_t lo, int64_t hi) {
+    auto rows_ok = [&](int64_t lo, int64_t hi) {
         for (int64_t v = lo; v < hi; v++)
             for (int j = indices[v]; j < indices[v + 1]; j++) {
                 int u = adj[j];
-                if (u < 0 || u >= n || u == v || (j > indices[v] && adj[j - 1] >= u)) {
-                    bad = 1;
-                    return;
-                }
+                if (u < 0 || u >= n || u == v || (j > indices[v] && adj[j - 1] >= u)) return false;
             }
-    }, 2048);
-    if (bad) return bfail(PCD_E_FORMAT, "adjacency rows must be ascending, loop-free, with ids in [0,n)");
-    if (maxd > 4096) return bfail(PCD_E_INVALID, "degree above 4096: run this graph in a plain context");
-    int s = -1;
-    for (int i = 0; i < b->cap; i++)
-        if (!b->slots[i].used) { s = i; break; }
-    if (s < 0) return bfail(PCD_E_NOMEM, "no free graph slot");
-    const int64_t E = 2 * (int64_t)m;
-    const size_t bytes = footprint(n, E, sorted);
-    size_t off = 0;
-    if (!arena_alloc(b, bytes, &off)) return bfail(PCD_E_NOMEM, "batch arena full");
-    Slot &sl = b->slots[s];
-    sl = Slot();
-    sl.used = true;
-    sl.status = BG_RUNNING;
-    sl.n = n;
-    sl.m = m;
-    sl.off = off;
-    sl.bytes = bytes;
-    sl.admitted = b->rounds;
+        return true;
+    };
+    bool ok = true;
+    if (parallel) {
+        std::atomic<int> bad{0};
+        parallel_for(n, [&](int64_t lo, int64_t hi) {
+            if (!rows_ok(lo, hi)) bad = 1;
+        }, 2048);
+        ok = !bad;
+    } else {
+        ok = rows_ok(0, n);
+    }
+    if (!ok) return reject(PCD_E_FORMAT, "adjacency rows must be ascending, loop-free, with ids in [0,n)");
+    if (maxd > 4096) return reject(PCD_E_INVALID, "degree above 4096: run this graph in a plain context");
+    pr.sorted = sorted;
     // degree order (stable, descending) when the graph is not degree-sorted
-    std::vector<int32_t> order;
     if (!sorted) {
         std::vector<int64_t> bucket((size_t)maxd + 2, 0);
         for (int v = 0; v < n; v++) bucket[maxd - (indices[v + 1] - indices[v]) + 1]++;
         for (int d = 0; d <= maxd; d++) bucket[d + 1] += bucket[d];
-        order.resize(n);
-        for (int v = 0; v < n; v++) order[bucket[maxd - (indices[v + 1] - indices[v])]++] = v;
+        pr.order.resize(n);
+        for (int v = 0; v < n; v++) pr.order[bucket[maxd - (indices[v + 1] - indices[v])]++] = v;
     }
     auto deg_at = [&](int p) {
-        int v = sorted ? p : order[p];
+        int v = sorted ? p : pr.order[p];
         return indices[v + 1] - indices[v];
     };
     for (int i = 0; i < NUM_TH; i++) {
@@ -632,13 +656,43 @@ extern "C" int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices,
             if (deg_at(mid) <= TH[i]) hi = mid;
             else lo = mid + 1;
         }
-        sl.pos_le[i] = lo;
+        pr.pos_le[i] = lo;
     }
-    for (int p = 0; p < sl.pos_le[th_index(64)]; p++) {
-        int v = sorted ? p : order[p];
+    for (int p = 0; p < pr.pos_le[th_index(64)]; p++) {
+        int v = sorted ? p : pr.order[p];
         for (int j = indices[v]; j < indices[v + 1]; j += ROW_CHUNK)
-            sl.rows.push_back(RowItem{s, v, j, std::min(indices[v + 1], j + ROW_CHUNK)});
+            pr.rows.push_back(RowItem{-1, v, j, std::min(indices[v + 1], j + ROW_CHUNK)});
     }
+}
+
+// Places a prepared graph in the first free slot at or after *hint and
+// queues its uploads on the batch stream.  The slot's records (graph and
+// state) are written on the host only; upload_records sends them.  The
+// caller synchronises the stream before the host buffers may change.
+static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, const int32_t *adj,
+                          const int32_t *labels, const pcd_job *job, Prepared &pr, int *hint, int *slot_out) {
+    int s = -1;
+    for (int i = *hint; i < b->cap; i++)
+        if (!b->slots[i].used) { s = i; break; }
+    if (s < 0) return bfail(PCD_E_NOMEM, "no free graph slot");
+    const int64_t E = 2 * (int64_t)m;
+    const bool sorted = pr.sorted;
+    const size_t bytes = footprint(n, E, sorted);
+    size_t off = 0;
+    if (!arena_alloc(b, bytes, &off)) return bfail(PCD_E_NOMEM, "batch arena full");
+    *hint = s + 1;
+    Slot &sl = b->slots[s];
+    sl = Slot();
+    sl.used = true;
+    sl.status = BG_RUNNING;
+    sl.n = n;
+    sl.m = m;
+    sl.off = off;
+    sl.bytes = bytes;
+    sl.admitted = b->rounds;
+    std::memcpy(sl.pos_le, pr.pos_le, sizeof(sl.pos_le));
+    sl.rows = std::move(pr.rows);
+    for (RowItem &r : sl.rows) r.g = s;
     // device layout
     char *base = b->arena + off;
     size_t nb = align256(4 * ((size_t)n + 1));
@@ -657,7 +711,7 @@ extern "C" int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices,
     CommStats *d_stats = (CommStats *)(p + 4 * nb);
     PCD_HIP(hipMemcpyAsync(d_ind, indices, 4 * ((size_t)n + 1), hipMemcpyHostToDevice, b->stream));
     PCD_HIP(hipMemcpyAsync(d_adj, adj, 4 * (size_t)E, hipMemcpyHostToDevice, b->stream));
-    if (!sorted) PCD_HIP(hipMemcpyAsync(d_order, order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    if (!sorted) PCD_HIP(hipMemcpyAsync(d_order, pr.order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, b->stream));
     PCD_HIP(hipMemcpyAsync(d_next, labels, 4 * (size_t)n, hipMemcpyHostToDevice, b->stream));
     BatchGraph &g = sl.rec;
     std::memset(&g, 0, sizeof(g));
@@ -691,14 +745,108 @@ extern "C" int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices,
     std::memset(&st, 0, sizeof(st));
     st.status = BG_RUNNING;
     st.label_bound = n;
-    // pageable sources: the copies are staged before hipMemcpyAsync returns
-    PCD_HIP(hipMemcpyAsync(b->d_graphs + s, &g, sizeof(BatchGraph), hipMemcpyHostToDevice, b->stream));
-    PCD_HIP(hipMemcpyAsync(b->d_state + s, &st, sizeof(BatchState), hipMemcpyHostToDevice, b->stream));
-    PCD_HIP(hipStreamSynchronize(b->stream));
     // items of the slot's previous graph may still be listed: rebuild then
     if (b->items_cap == 0 || b->slot_listed[s]) b->items_dirty = true;
     if (!b->items_dirty) b->pending_append.push_back(s);
     *slot_out = s;
+    return PCD_OK;
+}
+
+// Uploads the graph and state records of `slots` (ascending), one copy per
+// run of consecutive slots.  `staging` holds the graph records until the
+// caller has synchronised the stream.
+static int upload_records(pcd_batch *b, const std::vector<int> &slots, std::vector<BatchGraph> &staging) {
+    staging.resize(slots.size());
+    for (size_t i = 0; i < slots.size(); i++) staging[i] = b->slots[slots[i]].rec;
+    for (size_t i = 0; i < slots.size();) {
+        size_t j = i + 1;
+        while (j < slots.size() && slots[j] == slots[j - 1] + 1) j++;
+        const int s0 = slots[i];
+        const size_t cnt = j - i;
+        PCD_HIP(hipMemcpyAsync(b->d_graphs + s0, staging.data() + i, sizeof(BatchGraph) * cnt, hipMemcpyHostToDevice,
+                               b->stream));
+        PCD_HIP(hipMemcpyAsync(b->d_state + s0, b->h_state + s0, sizeof(BatchState) * cnt, hipMemcpyHostToDevice,
+                               b->stream));
+        i = j;
+    }
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices, const int32_t *adj,
+                             const int32_t *labels, const pcd_job *job, int *slot_out) {
+    if (!b || !job || !slot_out) return bfail(PCD_E_INVALID, "null argument");
+    *slot_out = -1;
+    Prepared pr;
+    prepare_graph(n, m, indices, adj, labels, job, true, pr);
+    if (pr.code != PCD_OK) return bfail(pr.code, pr.msg);
+    PCD_HIP(hipSetDevice(b->device));
+    int hint = 0, s = -1;
+    int rc = admit_prepared(b, n, m, indices, adj, labels, job, pr, &hint, &s);
+    if (rc != PCD_OK) return rc;
+    std::vector<BatchGraph> staging;
+    rc = upload_records(b, std::vector<int>{s}, staging);
+    if (rc != PCD_OK) return rc;
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    *slot_out = s;
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_add_many(pcd_batch *b, int count, const pcd_graph_ref *graphs, const pcd_job *jobs,
+                                  int *slots_out, int *codes_out) {
+    if (!b || count < 0 || (count > 0 && (!graphs || !jobs || !slots_out)))
+        return bfail(PCD_E_INVALID, "bad arguments");
+    for (int i = 0; i < count; i++) {
+        slots_out[i] = -1;
+        if (codes_out) codes_out[i] = PCD_OK;
+    }
+    if (count == 0) return PCD_OK;
+    PCD_HIP(hipSetDevice(b->device));
+    // Graphs are prepared CHUNK at a time on the host threads (one graph per
+    // thread); the uploads of one chunk run while the next is prepared.
+    constexpr int CHUNK = 256;
+    std::vector<Prepared> pr;
+    std::vector<int> admitted;
+    int first_code = PCD_OK, first_index = -1;
+    std::string first_msg;
+    int hint = 0;
+    for (int c0 = 0; c0 < count; c0 += CHUNK) {
+        const int c1 = std::min(count, c0 + CHUNK);
+        pr.assign(c1 - c0, Prepared());
+        parallel_for(c1 - c0, [&](int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; i++) {
+                const pcd_graph_ref &g = graphs[c0 + i];
+                prepare_graph(g.n, g.m, g.indices, g.adj, g.labels, jobs + c0 + i, false, pr[i]);
+            }
+        }, 1);
+        for (int i = c0; i < c1; i++) {
+            const pcd_graph_ref &g = graphs[i];
+            Prepared &p = pr[i - c0];
+            int rc = p.code;
+            std::string msg = p.msg;
+            if (rc == PCD_OK) {
+                int s = -1;
+                rc = admit_prepared(b, g.n, g.m, g.indices, g.adj, g.labels, jobs + i, p, &hint, &s);
+                if (rc == PCD_OK) {
+                    slots_out[i] = s;
+                    admitted.push_back(s);
+                } else {
+                    msg = pcd_last_error();
+                }
+            }
+            if (rc == PCD_E_HIP) return bfail(rc, "graph " + std::to_string(i) + ": " + msg);
+            if (codes_out) codes_out[i] = rc;
+            if (rc != PCD_OK && first_code == PCD_OK) {
+                first_code = rc;
+                first_index = i;
+                first_msg = msg;
+            }
+        }
+    }
+    std::vector<BatchGraph> staging;
+    int rc = upload_records(b, admitted, staging);
+    if (rc != PCD_OK) return rc;
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    if (first_code != PCD_OK) return bfail(first_code, "graph " + std::to_string(first_index) + ": " + first_msg);
     return PCD_OK;
 }
 
@@ -1038,6 +1186,48 @@ extern "C" int pcd_batch_labels(pcd_batch *b, int slot, int32_t *labels) {
     if (sl.status != BG_DONE) return bfail(PCD_E_STATE, "graph is still running");
     PCD_HIP(hipSetDevice(b->device));
     PCD_HIP(hipMemcpyAsync(labels, sl.rec.cur, 4 * (size_t)sl.n, hipMemcpyDeviceToHost, b->stream));
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_labels_many(pcd_batch *b, int count, const int *slots, int32_t *labels) {
+    if (!b || count < 0 || (count > 0 && (!slots || !labels))) return bfail(PCD_E_INVALID, "bad arguments");
+    if (count == 0) return PCD_OK;
+    std::vector<int64_t> head((size_t)count + (count + 1) / 2);  // offsets, then the slots as int32
+    int32_t *hslots = (int32_t *)(head.data() + count);
+    int64_t total = 0;
+    for (int i = 0; i < count; i++) {
+        const int s = slots[i];
+        if (s < 0 || s >= b->cap) return bfail(PCD_E_INVALID, "bad slot " + std::to_string(s));
+        const Slot &sl = b->slots[s];
+        if (!sl.used) return bfail(PCD_E_STATE, "slot " + std::to_string(s) + " is free");
+        if (sl.status != BG_DONE) return bfail(PCD_E_STATE, "graph in slot " + std::to_string(s) + " is still running");
+        head[i] = total;
+        hslots[i] = s;
+        total += sl.n;
+    }
+    const size_t hbytes = align256(sizeof(int64_t) * head.size());
+    const size_t need = hbytes + 4 * (size_t)total;
+    PCD_HIP(hipSetDevice(b->device));
+    if (need > b->gather_cap) {
+        if (b->d_gather) PCD_HIP(hipFree(b->d_gather));
+        b->d_gather = nullptr;
+        b->gather_cap = 0;
+        if (hipMalloc((void **)&b->d_gather, need) != hipSuccess) {
+            (void)hipGetLastError();
+            return bfail(PCD_E_NOMEM, "label staging allocation failed");
+        }
+        b->gather_cap = need;
+    }
+    const int64_t *d_offs = (const int64_t *)b->d_gather;
+    const int32_t *d_slots = (const int32_t *)(d_offs + count);
+    int32_t *d_out = (int32_t *)(b->d_gather + hbytes);
+    PCD_HIP(hipMemcpyAsync(b->d_gather, head.data(), sizeof(int64_t) * head.size(), hipMemcpyHostToDevice, b->stream));
+    const int grid = std::min(count, b->cu_count * 8);
+    hipLaunchKernelGGL(k_b_gather_labels, dim3(grid), dim3(256), 0, b->stream, (const BatchGraph *)b->d_graphs,
+                       d_slots, d_offs, count, d_out);
+    PCD_HIP(hipGetLastError());
+    PCD_HIP(hipMemcpyAsync(labels, d_out, 4 * (size_t)total, hipMemcpyDeviceToHost, b->stream));
     PCD_HIP(hipStreamSynchronize(b->stream));
     return PCD_OK;
 }

@@ -27,7 +27,8 @@ EXPORTS = [
     "pcd_kernel_slot_count", "pcd_kernel_slot_name", "pcd_kernel_timer_reset", "pcd_kernel_timer",
     "pcd_bin_count", "pcd_bin_name", "pcd_bin_counters_reset", "pcd_bin_counters", "pcd_bin_size",
     "pcd_singleton_partition", "pcd_set_tuning", "pcd_hub_overflows", "pcd_copy_probe",
-    "pcd_batch_create", "pcd_batch_destroy", "pcd_batch_footprint", "pcd_batch_add", "pcd_batch_step",
+    "pcd_batch_create", "pcd_batch_destroy", "pcd_batch_footprint", "pcd_batch_add", "pcd_batch_add_many",
+    "pcd_batch_labels_many", "pcd_batch_step",
     "pcd_batch_status", "pcd_batch_finished", "pcd_batch_labels", "pcd_batch_stats", "pcd_batch_release",
     "pcd_batch_info", "pcd_batch_map_count", "pcd_batch_map_name", "pcd_batch_set_map", "pcd_batch_autotune",
     "pcd_gen_sbm_batch", "pcd_refine", "pcd_host_alloc", "pcd_host_free", "pcd_clustering_partition",
@@ -54,6 +55,12 @@ class Job(C.Structure):
     """pcd_job: the ./main options of one graph in a batch."""
     _fields_ = [("rule", Rule), ("frac", C.c_double), ("threshold", C.c_double), ("max_iter", C.c_int),
                 ("tie", C.c_int), ("seed", C.c_uint64), ("iter_cap", C.c_int)]
+
+
+class GraphRef(C.Structure):
+    """pcd_graph_ref: one graph of pcd_batch_add_many."""
+    _fields_ = [("n", C.c_int), ("m", C.c_int), ("indices", C.c_void_p), ("adj", C.c_void_p),
+                ("labels", C.c_void_p)]
 
 
 def job(kind=0, lam=0.0, frac=0.5, threshold=0.0, max_iter=-1, tie=0, seed=0, iter_cap=0):
@@ -129,6 +136,8 @@ def lib():
     L.pcd_batch_destroy.argtypes = [vp]
     L.pcd_batch_footprint.argtypes = [C.c_int, C.c_int, lp]
     L.pcd_batch_add.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.POINTER(Job), ip]
+    L.pcd_batch_add_many.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+    L.pcd_batch_labels_many.argtypes = [vp, C.c_int, vp, vp]
     L.pcd_batch_step.argtypes = [vp, C.c_int, ip]
     L.pcd_batch_status.argtypes = [vp, C.c_int, ip, ip, ip, dp]
     L.pcd_batch_finished.argtypes = [vp, vp, C.c_int, ip]
@@ -570,6 +579,46 @@ class Batch:
         check(rc)
         self.sizes[slot.value] = int(n)
         return slot.value
+
+    def add_many(self, graphs, labels, jobs):
+        """Admits graphs[i] = (n, m, indices, adj) with labels[i] and jobs[i]
+        in one call (pcd_batch_add_many).  Returns the slots, None where a
+        graph was not admitted for lack of room (PCD_E_NOMEM); the codes are
+        kept in self.last_codes.  Raises PcdError when a graph was refused
+        for another reason (the admitted ones stay admitted)."""
+        count = len(graphs)
+        refs = (GraphRef * max(count, 1))()
+        keep = []
+        for i, (g, lab) in enumerate(zip(graphs, labels)):
+            ind, adj, lab = _i32(g[2]), _i32(g[3]), _i32(lab)
+            keep.append((ind, adj, lab))
+            refs[i] = GraphRef(int(g[0]), int(g[1]), ind.ctypes.data, adj.ctypes.data, lab.ctypes.data)
+        jarr = (Job * max(count, 1))(*jobs)
+        slots = np.full(max(count, 1), -1, np.int32)
+        codes = np.zeros(max(count, 1), np.int32)
+        rc = lib().pcd_batch_add_many(self._b, count, C.cast(refs, C.c_void_p), C.cast(jarr, C.c_void_p),
+                                      _p(slots), _p(codes))
+        msg = "pcd error %d: %s" % (rc, lib().pcd_last_error().decode(errors="replace")) if rc else ""
+        out = []
+        for i in range(count):
+            if slots[i] >= 0:
+                self.sizes[int(slots[i])] = int(graphs[i][0])
+                out.append(int(slots[i]))
+            else:
+                out.append(None)
+        self.last_codes = [int(c) for c in codes[:count]]
+        if any(c not in (0, -3) for c in self.last_codes):
+            raise PcdError(msg)
+        return out
+
+    def labels_many(self, slots):
+        """Labels of finished graphs in one download: a list of arrays (views
+        of one buffer), in the order of `slots`."""
+        slots = np.ascontiguousarray(slots, np.int32)
+        sizes = [self.sizes[int(s_)] for s_ in slots]
+        out = np.empty(max(sum(sizes), 1), np.int32)
+        check(lib().pcd_batch_labels_many(self._b, len(slots), _p(slots), _p(out)))
+        return np.split(out[: sum(sizes)], np.cumsum(sizes)[:-1]) if len(sizes) else []
 
     def step(self, rounds=1):
         r = C.c_int()

@@ -155,6 +155,65 @@ def test_batch_rejections():
         assert b.add(g[0], g[1], g[2], g[3], lab, parcd.job()) is None  # no free slot
 
 
+def test_add_many_and_labels_many_equal_one_by_one():
+    """pcd_batch_add_many admits what one pcd_batch_add per graph admits, in
+    the same slots, with the same results; pcd_batch_labels_many returns
+    every graph's labels in the order asked for."""
+    gs = graphs() * 3  # 18 graphs: several runs of consecutive slots
+    items = [(g, singleton_labels(g[0]), parcd.job(**JOBS[i % len(JOBS)])) for i, g in enumerate(gs)]
+    ref = run_batch(items[: len(JOBS)])
+    with parcd.Batch(0, max_graphs=32, arena_bytes=512 << 20) as b:
+        # a hole at slot 1 makes the admitted slots two runs
+        s0 = b.add(gs[0][0], gs[0][1], gs[0][2], gs[0][3], items[0][1], items[0][2])
+        s1 = b.add(gs[1][0], gs[1][1], gs[1][2], gs[1][3], items[1][1], items[1][2])
+        b.release(s1)
+        slots = b.add_many([g[:4] for g, _, _ in items[1:]], [x[1] for x in items[1:]], [x[2] for x in items[1:]])
+        assert slots[0] == s1 and all(s is not None for s in slots)
+        slots = [s0] + slots
+        assert len(set(slots)) == len(slots)
+        while b.step(1):
+            pass
+        order = slots[::-1]
+        many = b.labels_many(order)
+        for s, lab in zip(order, many):
+            assert np.array_equal(lab, b.labels(s))
+        for i, s in enumerate(slots):
+            r = ref[i % len(JOBS)]
+            assert np.array_equal(b.labels(s), r[0]) and b.status(s)[1:] == r[1:4], i
+
+
+def test_add_many_rejections_leave_the_rest_admitted():
+    g = sbm(4, 50, 0.3, 0.01, seed=11)
+    lab = singleton_labels(g[0])
+    bad = lab.copy()
+    bad[0] = g[0]
+    big = sbm(2, 2000, 0.3, 0.01, seed=3)  # footprint above the arena
+    with parcd.Batch(0, max_graphs=3, arena_bytes=4 << 20) as b:
+        gl = [g[:4], big[:4], g[:4], g[:4], g[:4]]
+        ll = [lab, singleton_labels(big[0]), lab, lab, lab]
+        slots = b.add_many(gl, ll, [parcd.job()] * 5)
+        # the big graph finds no arena room; the fifth graph no slot
+        assert slots[1] is None and slots[4] is None
+        assert b.last_codes[1] == -3 and b.last_codes[4] == -3
+        assert sorted(s for s in slots if s is not None) == [0, 1, 2]
+        b.release(slots[2])
+        with pytest.raises(parcd.PcdError):
+            b.add_many([g[:4], g[:4]], [bad, lab], [parcd.job(), parcd.job()])
+        assert b.last_codes[0] == -1  # PCD_E_INVALID
+        assert b.last_codes[1] == 0
+        with pytest.raises(parcd.PcdError):
+            b.add_many([g[:4]], [lab], [parcd.job(kind=O.WCC)])
+        with pytest.raises(parcd.PcdError):
+            b.labels_many([slots[0]])  # still running
+        while b.step(1):
+            pass
+        live = [s for s in b.finished()]
+        assert len(live) == 3
+        ref = run_batch([(g, lab, parcd.job())])[0]
+        for s, l in zip(live, b.labels_many(live)):
+            assert np.array_equal(l, ref[0])
+
+
 @pytest.mark.parametrize("strategy", [0, 1, 2])
 def test_refine_call_equals_replay(strategy):
     g = sbm(6, 60, 0.25, 0.01, seed=21)
