@@ -75,7 +75,7 @@ void kt_end(pcd_ctx *ctx, int slot) {
 static const char *kslot_names[NUM_KSLOTS] = {
     "compress", "stats_items", "stats_small", "metric", "lp_records", "lp_hub_partition", "lp_hub_count",
     "lp_b4096", "lp_b1024", "lp_w256", "lp_g64", "lp_g32", "lp_g16", "lp_g8", "lp_g4",
-    "merge_keys", "merge_sort", "merge_rle", "merge_rounds", "triangles"};
+    "merge_keys", "merge_sort", "merge_rle", "merge_rounds", "triangles", "stats_delta"};
 
 static const char *bin_names[NUM_BINS] = {"hub", "b4096", "b1024", "w256", "g64", "g32", "g16", "g8", "g4"};
 
@@ -183,7 +183,7 @@ static void free_graph_state(pcd_ctx *ctx) {
     ctx->d_stat_items = nullptr;
     ctx->d_labels = ctx->d_labels_alt = nullptr;
     ctx->d_lv = nullptr;
-    ctx->d_stats = nullptr;
+    ctx->d_stats = ctx->d_stats_alt = nullptr;
     ctx->d_first = ctx->d_newid = ctx->d_active = ctx->d_hub_active = nullptr;
     ctx->d_scan_part = nullptr;
     ctx->scan_part_cap = 0;
@@ -199,6 +199,7 @@ static void free_graph_state(pcd_ctx *ctx) {
     ctx->mcand_cap = 0;
     ctx->first_cap = 0;
     ctx->has_graph = ctx->has_labels = ctx->has_stats = ctx->has_triangles = false;
+    ctx->stats_pending = false;
     ctx->labels_compressed = false;
     ctx->num_hubs = ctx->num_hub_items = ctx->num_stat_items = 0;
     ctx->cc = 0.0;
@@ -385,6 +386,7 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_labels_alt, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_lv, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_stats, n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_stats_alt, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_active, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_first, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_newid, n + 1));
@@ -428,6 +430,7 @@ extern "C" int pcd_labels_upload(pcd_ctx *ctx, const int32_t *labels) {
     ctx->labels_compressed = false;
     ctx->labels_have_new = has_new;
     ctx->has_stats = false;
+    ctx->stats_pending = false;
     return PCD_OK;
 }
 
@@ -460,6 +463,13 @@ extern "C" int pcd_collect_statistics(pcd_ctx *ctx, int *num_labels) {
     PCD_TRY(check_labels(ctx));
     if (ctx->labels_have_new) return fail(PCD_E_STATE, "labels contain NEW_LABEL; compress them first");
     phase_begin(ctx, PCD_PHASE_STATS);
+    if (ctx->has_stats && ctx->labels_compressed) {
+        // the last compression renumbered statistics that a propagation step
+        // had updated: they are those of the current labels already
+        phase_end(ctx, PCD_PHASE_STATS);
+        if (num_labels) *num_labels = ctx->k;
+        return PCD_OK;
+    }
     if (!ctx->labels_compressed) {
         // framework.cpp:110,348: k = max label + 1
         PCD_TRY(fill_i32(ctx, ctx->d_k, 0, 1));
@@ -626,6 +636,8 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.block_aux = value != 0;
     } else if (k == "stats_cache") {
         ctx->tune.stats_cache = value != 0;
+    } else if (k == "inc_stats") {
+        ctx->tune.inc_stats = value != 0;
     } else if (k == "stat_chunk") {
         if (value < 64) return fail(PCD_E_INVALID, "stat_chunk must be >= 64");
         ctx->tune.stat_chunk = value;  // applies to the next graph upload

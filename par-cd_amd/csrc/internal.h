@@ -52,6 +52,8 @@ struct Tuning {
     int block_aux = 0;           // wave/workgroup-per-vertex bins gather {label, aux} records
     int stats_cache = 1;         // statistics over long rows read the labels of the first
                                  // STATS_CACHE vertices (the highest degrees) from LDS
+    int inc_stats = 1;           // a propagation step updates the statistics from the rows
+                                 // of the vertices it moved (else a full recount)
 };
 constexpr int STATS_CACHE = 16384;
 constexpr int HUB_MAX_PARTS = 1024;
@@ -66,7 +68,7 @@ enum KSlot {
     KS_COMPRESS = 0, KS_STATS_ITEMS, KS_STATS_SMALL, KS_METRIC,
     KS_LP_SELECT, KS_LP_HUB_INSERT, KS_LP_HUB_DECIDE, KS_LP_B4096, KS_LP_B1024, KS_LP_W256,
     KS_LP_G64, KS_LP_G32, KS_LP_G16, KS_LP_G8, KS_LP_G4,
-    KS_MERGE_KEYS, KS_MERGE_SORT, KS_MERGE_RLE, KS_MERGE_ROUNDS, KS_TRIANGLES,
+    KS_MERGE_KEYS, KS_MERGE_SORT, KS_MERGE_RLE, KS_MERGE_ROUNDS, KS_TRIANGLES, KS_STATS_DELTA,
     NUM_KSLOTS
 };
 
@@ -119,11 +121,16 @@ struct pcd_ctx {
     bool labels_have_new = false;  // labels may hold NEW_LABEL
     int64_t first_cap = 0;         // capacity of d_first / d_newid
     bool has_stats = false;
+    // d_stats holds the statistics of d_labels under the label ids of the
+    // last compression, vertices with NEW_LABEL not counted (after a
+    // propagation step); the next compression renumbers them
+    bool stats_pending = false;
     int label_bound = 0;  // labels in [-1, label_bound)
     int k = 0;
     int32_t *d_labels = nullptr, *d_labels_alt = nullptr;
     int2 *d_lv = nullptr;  // per vertex {label, aux} for the propagation gathers
     pcd::CommStats *d_stats = nullptr;
+    pcd::CommStats *d_stats_alt = nullptr;  // compression writes the renumbered statistics here
     int32_t *d_first = nullptr, *d_newid = nullptr;
     int32_t *d_active = nullptr;   // per-bin active lists (bin ranges)
     int32_t *d_counts = nullptr;   // per-bin active counts + misc device counters
@@ -198,6 +205,7 @@ int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int b
 // phases (partition.hip, propagate.hip, merge.hip, triangles.hip)
 int compress_labels_dev(pcd_ctx *ctx);
 int collect_statistics_dev(pcd_ctx *ctx);
+int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *after);
 int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out);
 int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie);
 int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie);

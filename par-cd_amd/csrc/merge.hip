@@ -391,6 +391,7 @@ int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step
     std::swap(ctx->d_labels, ctx->d_labels_alt);
     ctx->labels_compressed = false;
     ctx->has_stats = false;
+    ctx->stats_pending = false;
     ctx->label_bound = k;
     return PCD_OK;
 }

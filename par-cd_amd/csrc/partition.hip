@@ -24,14 +24,21 @@ struct FirstFlag {
         return (l < 0 || first[l] == (int)v) ? 1 : 0;
     }
 };
+// Each first occurrence gets its new id.  With st_in set, the statistics
+// move to the new ids as well: st_out[new id] = st_in[old label], and a
+// NEW_LABEL vertex opens its singleton record {1, 0, deg}.
 struct FirstStore {
     const int32_t *labels;
     int32_t *out, *newid;
+    const CommStats *st_in;
+    CommStats *st_out;
+    const int32_t *indices;
     __device__ void operator()(int64_t v, int ex, int flag) const {
         if (!flag) return;
         int l = labels[v];
         if (l < 0) out[v] = ex;
         else newid[l] = ex;
+        if (st_in) st_out[ex] = l < 0 ? CommStats{1, 0, indices[v + 1] - indices[v], 0} : st_in[l];
     }
 };
 
@@ -50,9 +57,14 @@ int compress_labels_dev(pcd_ctx *ctx) {
     if (ctx->label_bound > 0) PCD_TRY(fill_i32(ctx, ctx->d_first, INT32_MAX, ctx->label_bound));
     hipLaunchKernelGGL(k_first_occurrence, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_labels,
                        ctx->d_first);
+    // statistics of these labels (full count, or updated by the last
+    // propagation step) are renumbered with them
+    const bool renumber = ctx->stats_pending || ctx->has_stats;
     int32_t *total = nullptr;
     PCD_TRY(device_scan(ctx, n, FirstFlag{ctx->d_labels, ctx->d_first},
-                        FirstStore{ctx->d_labels, ctx->d_labels_alt, ctx->d_newid}, &total));
+                        FirstStore{ctx->d_labels, ctx->d_labels_alt, ctx->d_newid, renumber ? ctx->d_stats : nullptr,
+                                   ctx->d_stats_alt, ctx->d_indices},
+                        &total));
     hipLaunchKernelGGL(k_relabel, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_labels,
                        ctx->d_newid, ctx->d_labels_alt);
     hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(1), 0, ctx->stream, ctx->d_k, total);
@@ -61,10 +73,12 @@ int compress_labels_dev(pcd_ctx *ctx) {
     PCD_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_k, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     PCD_HIP(hipStreamSynchronize(ctx->stream));
     std::swap(ctx->d_labels, ctx->d_labels_alt);
+    if (renumber) std::swap(ctx->d_stats, ctx->d_stats_alt);
     ctx->k = ctx->h_small[0];
     ctx->label_bound = ctx->k;
     ctx->labels_compressed = true;
-    ctx->has_stats = false;
+    ctx->has_stats = renumber;
+    ctx->stats_pending = false;
     return PCD_OK;
 }
 
@@ -203,6 +217,146 @@ int collect_statistics_dev(pcd_ctx *ctx) {
     kt_end(ctx, KS_STATS_SMALL);
     PCD_HIP(hipGetLastError());
     ctx->has_stats = true;
+    return PCD_OK;
+}
+
+// ------------------------------------------------------------ statistics after a propagation step
+// The counts of framework.cpp:69-143, brought from the labels before a
+// propagation step (L) to those after it (N) through the rows of the
+// vertices that moved.  Moving v from lo = L(v) to ln = N(v) touches the
+// records of lo and ln alone.  For each neighbour w of v, the ordered pair
+// (v, w) -- and (w, v) too when w kept its label, weight f = 2 -- was
+// internal to lo when L(w) = lo and is internal to ln when N(w) = ln; a
+// pair whose ends both moved is counted once from each end's row.  The
+// volume moves by deg(v), and ext = volume - int.  A vertex with NEW_LABEL
+// only leaves lo here: compression opens its singleton record.  The sums are
+// exact integer additions, so the records equal a full recount.
+
+// G lanes per vertex, positions [p0, p1); rows of unmoved vertices are not read
+template <int G>
+__global__ __launch_bounds__(256) void k_delta_small(int p0, int p1, const int32_t *order, const int32_t *indices,
+                                                     const int32_t *adj, const int32_t *L, const int32_t *N,
+                                                     CommStats *st) {
+    const int lane = __lane_id(), glane = lane & (G - 1);
+    const int per_wave = WAVE / G;
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    const int count = p1 - p0;
+    for (int base = wave_g * per_wave; base < count; base += nwaves * per_wave) {
+        const int gi = base + lane / G;
+        int start = 0, deg = 0, lo = 0, ln = 0;
+        bool moved = false;
+        if (gi < count) {
+            const int p = p0 + gi;
+            const int v = order ? order[p] : p;
+            lo = L[v];
+            ln = N[v];
+            moved = lo != ln;
+            if (moved) {
+                start = indices[v];
+                deg = indices[v + 1] - start;
+            }
+        }
+        if (!__ballot(moved)) continue;
+        int a = 0, b = 0;
+        for (int j = glane; j < deg; j += G) {
+            const int w = adj[start + j];
+            const int lw = L[w], nw = N[w];
+            const int f = lw != nw ? 1 : 2;
+            a += lw == lo ? f : 0;
+            b += nw == ln ? f : 0;
+        }
+#pragma unroll
+        for (int d = G / 2; d > 0; d >>= 1) {
+            a += __shfl_xor(a, d, WAVE);
+            b += __shfl_xor(b, d, WAVE);
+        }
+        if (moved && glane == 0) {
+            add_stats(st, lo, -1, -a, a - deg);
+            if (ln >= 0) add_stats(st, ln, 1, b, deg - b);
+        }
+    }
+}
+
+// Work items {v, begin, end, first} (rows longer than 64): each wave checks
+// 64 items at once and runs the row chunks of the moved ones together.
+__global__ __launch_bounds__(256) void k_delta_items(int nitems, const int4 *items, const int32_t *adj,
+                                                     const int32_t *L, const int32_t *N, CommStats *st) {
+    constexpr int PF = 4;
+    const int lane = __lane_id();
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    for (int base = wave_g * WAVE; base < nitems; base += nwaves * WAVE) {
+        const int i = base + lane;
+        int4 w = make_int4(0, 0, 0, 0);
+        int lo = 0, ln = 0;
+        if (i < nitems) {
+            w = items[i];
+            lo = L[w.x];
+            ln = N[w.x];
+        }
+        uint64_t mv = __ballot(i < nitems && lo != ln);
+        while (mv) {
+            const int src = __builtin_ctzll(mv);
+            mv &= mv - 1;
+            const int xb = __builtin_amdgcn_readlane(w.y, src), xe = __builtin_amdgcn_readlane(w.z, src);
+            const int xf = __builtin_amdgcn_readlane(w.w, src);
+            const int xlo = __builtin_amdgcn_readlane(lo, src), xln = __builtin_amdgcn_readlane(ln, src);
+            int a = 0, b = 0;
+            for (int j0 = xb + lane; j0 < xe; j0 += PF * WAVE) {
+                int nb[PF];
+#pragma unroll
+                for (int e = 0; e < PF; e++) {
+                    const int j = j0 + e * WAVE;
+                    nb[e] = j < xe ? adj[j] : -1;
+                }
+                int lw[PF], nw[PF];
+#pragma unroll
+                for (int e = 0; e < PF; e++) {
+                    lw[e] = nb[e] < 0 ? -2 : L[nb[e]];
+                    nw[e] = nb[e] < 0 ? -2 : N[nb[e]];
+                }
+#pragma unroll
+                for (int e = 0; e < PF; e++) {
+                    if (nb[e] < 0) continue;
+                    const int f = lw[e] != nw[e] ? 1 : 2;
+                    a += lw[e] == xlo ? f : 0;
+                    b += nw[e] == xln ? f : 0;
+                }
+            }
+            a = wave_sum(a);
+            b = wave_sum(b);
+            if (lane == 0) {
+                const int len = xe - xb;
+                add_stats(st, xlo, xf ? -1 : 0, -a, a - len);
+                if (xln >= 0) add_stats(st, xln, xf ? 1 : 0, b, len - b);
+            }
+        }
+    }
+}
+
+template <int G>
+static void launch_delta_small(pcd_ctx *ctx, int p0, int p1, const int32_t *L, const int32_t *N) {
+    if (p1 <= p0) return;
+    int per_block = 256 / G;
+    hipLaunchKernelGGL(k_delta_small<G>, dim3(grid_for(ctx, p1 - p0, per_block, 16)), dim3(256), 0, ctx->stream, p0,
+                       p1, ctx->d_order, ctx->d_indices, ctx->d_adj, L, N, ctx->d_stats);
+}
+
+int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *after) {
+    const BinLayout &b = ctx->bins;
+    kt_begin(ctx);
+    if (ctx->num_stat_items > 0)
+        hipLaunchKernelGGL(k_delta_items, dim3(grid_for(ctx, ctx->num_stat_items, 4 * WAVE, 8)), dim3(256), 0,
+                           ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_adj, before, after,
+                           ctx->d_stats);
+    launch_delta_small<64>(ctx, b.start[BIN_G64], b.start[BIN_G64 + 1], before, after);
+    launch_delta_small<32>(ctx, b.start[BIN_G32], b.start[BIN_G32 + 1], before, after);
+    launch_delta_small<16>(ctx, b.start[BIN_G16], b.start[BIN_G16 + 1], before, after);
+    launch_delta_small<8>(ctx, b.start[BIN_G8], b.start[BIN_G8 + 1], before, after);
+    launch_delta_small<4>(ctx, b.start[BIN_G4], b.start[BIN_G4 + 1], before, after);
+    kt_end(ctx, KS_STATS_DELTA);
+    PCD_HIP(hipGetLastError());
     return PCD_OK;
 }
 

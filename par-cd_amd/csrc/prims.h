@@ -12,8 +12,13 @@ namespace pcd {
 
 // Community statistics update: size and int_vol are adjacent 32-bit fields,
 // so one 64-bit atomic adds both (neither field can carry past 2^31).
+// Adds (size, same, ext) to record l; size and same share one 64-bit
+// atomic.  The deltas may be negative: the packed word changes by
+// same * 2^32 + size mod 2^64, so a transient borrow between the halves is
+// undone by the later additions and the final halves are exact.
 __device__ inline void add_stats(CommStats *st, int l, int size, int same, int ext) {
-    atomicAdd((unsigned long long *)&st[l].size, ((unsigned long long)(unsigned)same << 32) | (unsigned)size);
+    atomicAdd((unsigned long long *)&st[l].size,
+              (unsigned long long)((long long)same * 4294967296LL + (long long)size));
     if (ext) atomicAdd(&st[l].ext_vol, ext);
 }
 

@@ -855,9 +855,14 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
     if (rule.kind == PCD_RULE_MODULARITY) PCD_TRY(launch_propagation<RK_MOD>(ctx, a));
     else if (rule.kind == PCD_RULE_CPM) PCD_TRY(launch_propagation<RK_CPM>(ctx, a));
     else PCD_TRY(launch_propagation<RK_ALL>(ctx, a));
+    // the statistics follow the moved vertices; the next compression
+    // renumbers them
+    const bool follow = ctx->tune.inc_stats != 0;
+    if (follow) PCD_TRY(statistics_delta_dev(ctx, ctx->d_labels, ctx->d_labels_alt));
     std::swap(ctx->d_labels, ctx->d_labels_alt);
     ctx->labels_compressed = false;
     ctx->has_stats = false;
+    ctx->stats_pending = follow;
     ctx->label_bound = ctx->k;
     return PCD_OK;
 }

@@ -165,6 +165,39 @@ def test_propagation_bin_zoo(ctx, kind, lam, tie, aux):
         ctx.set_tuning("block_aux", 0)
 
 
+@pytest.mark.parametrize("inc", [1, 0])
+def test_statistics_follow_propagation(ctx, inc):
+    """After each propagation step the statistics are updated from the moved
+    vertices' rows (inc=1) or recounted (inc=0); after the next compression
+    they must equal the oracle's full count of the new labels, including
+    vertices that left to NEW_LABEL singletons (CPM at a high resolution)."""
+    ctx.set_tuning("inc_stats", inc)
+    left = 0
+    try:
+        for g in [GOLDEN["golden_rmat9"], GG.bin_zoo(), GG.unsorted_copy(GG.bin_zoo()), GG.dense_blocks(),
+                  GG.rmat(12)]:
+            n, m, ind, adj = upload(ctx, g)
+            rng = np.random.default_rng(n)
+            for kind, lam, init in [(O.MOD, 0.0, GG.singleton_labels(n)), (O.CPM, 0.3, rng.integers(0, 40, n)),
+                                    (O.CPM, 0.01, GG.singleton_labels(n))]:
+                ctx.upload_labels(init)
+                for it in range(6):
+                    k = ctx.compress()
+                    lab = ctx.labels()
+                    assert ctx.statistics() == k
+                    s, iv, ev = ctx.stats(k)
+                    ko, so, ivo, evo = O.collect_statistics(n, ind, adj, lab)
+                    assert ko == k
+                    assert np.array_equal(s, so) and np.array_equal(iv, ivo) and np.array_equal(ev, evo), it
+                    assert ctx.metric(kind, lam) == pytest.approx(
+                        O.calculate(kind, lam, n, m, ind, adj, lab, so, ivo, evo), rel=1e-9, abs=1e-12)
+                    ctx.propagate(kind, lam, frac=0.6, seed=3, step=it)
+                    left += int((ctx.labels() < 0).sum())
+    finally:
+        ctx.set_tuning("inc_stats", 1)
+    assert left > 0  # NEW_LABEL moves were exercised
+
+
 @pytest.mark.parametrize("slots,part_entries", [(4096, 2048), (64, 2048), (1024, 512), (64, 100000), (4096, 100000)])
 def test_propagation_hub_paths(ctx, slots, part_entries):
     """Hub rows: partitioned LDS counting, and the global-table fallback when a
