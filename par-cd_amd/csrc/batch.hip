@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void k_b_first_occurrence(const WorkItem *tile
         int32_t *first = gs[w.g].first;
         for (int v = w.p0 + (int)threadIdx.x; v < w.p1; v += blockDim.x) {
             int l = next[v];
-            if (l >= 0) atomicMin(&first[l], v);
+            if (l >= 0 && first[l] > v) atomicMin(&first[l], v);  // read first: see k_first_occurrence
         }
     }
 }

@@ -184,6 +184,7 @@ static void free_graph_state(pcd_ctx *ctx) {
     ctx->d_labels = ctx->d_labels_alt = nullptr;
     ctx->d_lv = nullptr;
     ctx->d_stats = ctx->d_stats_alt = nullptr;
+    ctx->d_moved = nullptr;
     ctx->d_first = ctx->d_newid = ctx->d_active = ctx->d_hub_active = nullptr;
     ctx->d_scan_part = nullptr;
     ctx->scan_part_cap = 0;
@@ -387,6 +388,7 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_lv, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_stats, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_stats_alt, n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_moved, 8 + (size_t)n + ctx->num_stat_items + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_active, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_first, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_newid, n + 1));

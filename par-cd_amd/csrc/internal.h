@@ -131,6 +131,7 @@ struct pcd_ctx {
     int2 *d_lv = nullptr;  // per vertex {label, aux} for the propagation gathers
     pcd::CommStats *d_stats = nullptr;
     pcd::CommStats *d_stats_alt = nullptr;  // compression writes the renumbered statistics here
+    int32_t *d_moved = nullptr;  // 8 list bounds, then the moved small-bin vertices and stats items
     int32_t *d_first = nullptr, *d_newid = nullptr;
     int32_t *d_active = nullptr;   // per-bin active lists (bin ranges)
     int32_t *d_counts = nullptr;   // per-bin active counts + misc device counters

@@ -10,10 +10,13 @@ namespace pcd {
 // it is the minimum of its label or has NEW_LABEL, and the exclusive scan of
 // those flags over v is exactly the reference's running counter.
 
+// A plain read of first[l] skips the atomic when it already holds a smaller
+// vertex: the value only decreases, so a stale read costs at most an
+// atomic, and large communities do not serialise every member's update.
 __global__ void k_first_occurrence(int n, const int32_t *labels, int32_t *first) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         int l = labels[v];
-        if (l >= 0) atomicMin(&first[l], v);
+        if (l >= 0 && first[l] > v) atomicMin(&first[l], v);
     }
 }
 
@@ -232,32 +235,80 @@ int collect_statistics_dev(pcd_ctx *ctx) {
 // only leaves lo here: compression opens its singleton record.  The sums are
 // exact integer additions, so the records equal a full recount.
 
-// G lanes per vertex, positions [p0, p1); rows of unmoved vertices are not read
+// One list of the moved vertices, built by an exclusive scan of "moved"
+// flags over index t: t < nsmall is small-bin position s[0] + t (its entry
+// is the vertex), t in [nsmall, nsmall + nitems) is long-row item
+// t - nsmall (its entry is the item index), and a last index with flag 0
+// closes the list.  The scan keeps position order, so the small bins'
+// moved vertices come out grouped by bin; bounds[b] (b < 5) is where bin b
+// starts, bounds[5] where the items start and bounds[6] the total.
+struct MovedArgs {
+    int s[6];  // small bins G64..G4: positions [s[b], s[b+1])
+    int nitems;
+    const int32_t *order;
+    const int4 *items;
+    const int32_t *L, *N;
+};
+
+struct MovedFlag {
+    MovedArgs a;
+    __device__ int operator()(int64_t t) const {
+        const int nsmall = a.s[5] - a.s[0];
+        int v;
+        if (t < nsmall) {
+            const int p = a.s[0] + (int)t;
+            v = a.order ? a.order[p] : p;
+        } else if (t < nsmall + a.nitems) {
+            v = a.items[t - nsmall].x;
+        } else {
+            return 0;
+        }
+        return a.L[v] != a.N[v] ? 1 : 0;
+    }
+};
+
+struct MovedStore {
+    MovedArgs a;
+    int32_t *bounds, *list;
+    __device__ void operator()(int64_t t, int ex, int flag) const {
+        const int nsmall = a.s[5] - a.s[0];
+#pragma unroll
+        for (int b = 0; b < 5; b++)
+            if (t == a.s[b] - a.s[0]) bounds[b] = ex;
+        if (t == nsmall) bounds[5] = ex;
+        if (t == nsmall + a.nitems) bounds[6] = ex;
+        if (!flag) return;
+        if (t < nsmall) {
+            const int p = a.s[0] + (int)t;
+            list[ex] = a.order ? a.order[p] : p;
+        } else {
+            list[ex] = (int)(t - nsmall);
+        }
+    }
+};
+
+// G lanes per moved vertex of one small bin (degree <= 64)
 template <int G>
-__global__ __launch_bounds__(256) void k_delta_small(int p0, int p1, const int32_t *order, const int32_t *indices,
-                                                     const int32_t *adj, const int32_t *L, const int32_t *N,
-                                                     CommStats *st) {
+__global__ __launch_bounds__(256) void k_delta_small(const int32_t *list_all, const int32_t *bounds,
+                                                     const int32_t *indices, const int32_t *adj, const int32_t *L,
+                                                     const int32_t *N, CommStats *st) {
     const int lane = __lane_id(), glane = lane & (G - 1);
     const int per_wave = WAVE / G;
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int nwaves = gridDim.x * blockDim.x / WAVE;
-    const int count = p1 - p0;
-    for (int base = wave_g * per_wave; base < count; base += nwaves * per_wave) {
+    const int32_t *list = list_all + bounds[0];
+    const int cnt = bounds[1] - bounds[0];
+    for (int base = wave_g * per_wave; base < cnt; base += nwaves * per_wave) {
         const int gi = base + lane / G;
+        const bool has = gi < cnt;
         int start = 0, deg = 0, lo = 0, ln = 0;
-        bool moved = false;
-        if (gi < count) {
-            const int p = p0 + gi;
-            const int v = order ? order[p] : p;
+        if (has) {
+            const int v = list[gi];
             lo = L[v];
             ln = N[v];
-            moved = lo != ln;
-            if (moved) {
-                start = indices[v];
-                deg = indices[v + 1] - start;
-            }
+            start = indices[v];
+            deg = indices[v + 1] - start;
         }
-        if (!__ballot(moved)) continue;
         int a = 0, b = 0;
         for (int j = glane; j < deg; j += G) {
             const int w = adj[start + j];
@@ -271,31 +322,36 @@ __global__ __launch_bounds__(256) void k_delta_small(int p0, int p1, const int32
             a += __shfl_xor(a, d, WAVE);
             b += __shfl_xor(b, d, WAVE);
         }
-        if (moved && glane == 0) {
+        if (has && glane == 0) {
             add_stats(st, lo, -1, -a, a - deg);
             if (ln >= 0) add_stats(st, ln, 1, b, deg - b);
         }
     }
 }
 
-// Work items {v, begin, end, first} (rows longer than 64): each wave checks
-// 64 items at once and runs the row chunks of the moved ones together.
-__global__ __launch_bounds__(256) void k_delta_items(int nitems, const int4 *items, const int32_t *adj,
-                                                     const int32_t *L, const int32_t *N, CommStats *st) {
+// Moved items {v, begin, end, first} (rows longer than 64): each wave reads
+// 64 list entries at once, lane j taking entry wave + (64 r + j) * waves in
+// round r, and runs their row chunks one after another; one long row's
+// chunks are spread over many waves.
+__global__ __launch_bounds__(256) void k_delta_items(const int32_t *list_all, const int32_t *bounds,
+                                                     const int4 *items, const int32_t *adj, const int32_t *L,
+                                                     const int32_t *N, CommStats *st) {
     constexpr int PF = 4;
     const int lane = __lane_id();
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int nwaves = gridDim.x * blockDim.x / WAVE;
-    for (int base = wave_g * WAVE; base < nitems; base += nwaves * WAVE) {
-        const int i = base + lane;
+    const int32_t *list = list_all + bounds[0];
+    const int cnt = bounds[1] - bounds[0];
+    for (int64_t r = 0; wave_g + r * WAVE * nwaves < cnt; r++) {
+        const int64_t i = wave_g + (r * WAVE + lane) * nwaves;
         int4 w = make_int4(0, 0, 0, 0);
         int lo = 0, ln = 0;
-        if (i < nitems) {
-            w = items[i];
+        if (i < cnt) {
+            w = items[list[i]];
             lo = L[w.x];
             ln = N[w.x];
         }
-        uint64_t mv = __ballot(i < nitems && lo != ln);
+        uint64_t mv = __ballot(i < cnt);
         while (mv) {
             const int src = __builtin_ctzll(mv);
             mv &= mv - 1;
@@ -335,26 +391,33 @@ __global__ __launch_bounds__(256) void k_delta_items(int nitems, const int4 *ite
     }
 }
 
-template <int G>
-static void launch_delta_small(pcd_ctx *ctx, int p0, int p1, const int32_t *L, const int32_t *N) {
-    if (p1 <= p0) return;
-    int per_block = 256 / G;
-    hipLaunchKernelGGL(k_delta_small<G>, dim3(grid_for(ctx, p1 - p0, per_block, 16)), dim3(256), 0, ctx->stream, p0,
-                       p1, ctx->d_order, ctx->d_indices, ctx->d_adj, L, N, ctx->d_stats);
-}
-
 int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *after) {
     const BinLayout &b = ctx->bins;
     kt_begin(ctx);
-    if (ctx->num_stat_items > 0)
-        hipLaunchKernelGGL(k_delta_items, dim3(grid_for(ctx, ctx->num_stat_items, 4 * WAVE, 8)), dim3(256), 0,
-                           ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_adj, before, after,
-                           ctx->d_stats);
-    launch_delta_small<64>(ctx, b.start[BIN_G64], b.start[BIN_G64 + 1], before, after);
-    launch_delta_small<32>(ctx, b.start[BIN_G32], b.start[BIN_G32 + 1], before, after);
-    launch_delta_small<16>(ctx, b.start[BIN_G16], b.start[BIN_G16 + 1], before, after);
-    launch_delta_small<8>(ctx, b.start[BIN_G8], b.start[BIN_G8 + 1], before, after);
-    launch_delta_small<4>(ctx, b.start[BIN_G4], b.start[BIN_G4 + 1], before, after);
+    MovedArgs m;
+    for (int i = 0; i < 6; i++) m.s[i] = b.start[BIN_G64 + i];
+    m.nitems = ctx->num_stat_items;
+    m.order = ctx->d_order;
+    m.items = ctx->d_stat_items;
+    m.L = before;
+    m.N = after;
+    int32_t *bounds = ctx->d_moved;
+    int32_t *list = ctx->d_moved + 8;
+    const int64_t len = (int64_t)(m.s[5] - m.s[0]) + m.nitems + 1;
+    PCD_TRY(device_scan(ctx, len, MovedFlag{m}, MovedStore{m, bounds, list}, nullptr));
+    if (m.nitems > 0)
+        hipLaunchKernelGGL(k_delta_items, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, list, bounds + 5,
+                           ctx->d_stat_items, ctx->d_adj, before, after, ctx->d_stats);
+#define PCD_DELTA(I, G)                                                                                             \
+    if (m.s[I + 1] > m.s[I])                                                                                        \
+        hipLaunchKernelGGL(k_delta_small<G>, dim3(ctx->cu_count * 4), dim3(256), 0, ctx->stream, list, bounds + I, \
+                           ctx->d_indices, ctx->d_adj, before, after, ctx->d_stats);
+    PCD_DELTA(0, 64)
+    PCD_DELTA(1, 32)
+    PCD_DELTA(2, 16)
+    PCD_DELTA(3, 8)
+    PCD_DELTA(4, 4)
+#undef PCD_DELTA
     kt_end(ctx, KS_STATS_DELTA);
     PCD_HIP(hipGetLastError());
     return PCD_OK;
