@@ -141,6 +141,9 @@ __global__ __launch_bounds__(256) void k_b_tile_apply(const WorkItem *tiles, con
         int tot;
         int ex = block_excl_scan_n<256>(c, &tot, lds);
         int id = gbase + ex;
+        // graphs whose statistics follow the labels: the records move to
+        // the new ids, a NEW_LABEL vertex opens its singleton record
+        const bool ready = st[w.g].ready != 0;
 #pragma unroll
         for (int e = 0; e < VT_PER; e++) {
             if ((flags >> e) & 1) {
@@ -148,6 +151,8 @@ __global__ __launch_bounds__(256) void k_b_tile_apply(const WorkItem *tiles, con
                 int l = g.next[v];
                 if (l < 0) g.cur[v] = id;
                 else g.newid[l] = id;
+                if (ready)
+                    g.stats_alt[id] = l < 0 ? CommStats{1, 0, g.lp.indices[v + 1] - g.lp.indices[v], 0} : g.stats[l];
                 id++;
             }
         }
@@ -169,12 +174,27 @@ __global__ __launch_bounds__(256) void k_b_relabel(const WorkItem *tiles, const 
     }
 }
 
+// renumbered statistics back into place (graphs whose statistics follow
+// the labels)
+__global__ __launch_bounds__(256) void k_b_stats_copy(const WorkItem *tiles, const int *count, const BatchGraph *gs,
+                                                      const BatchState *st) {
+    const int nt = *count;
+    for (int u = blockIdx.x; u < nt; u += gridDim.x) {
+        const WorkItem w = tiles[u];
+        if (st[w.g].status != BG_RUNNING || !st[w.g].ready) continue;
+        const int hi = min(w.p1, st[w.g].k);
+        const BatchGraph &g = gs[w.g];
+        for (int c = w.p0 + (int)threadIdx.x; c < hi; c += blockDim.x) g.stats[c] = g.stats_alt[c];
+    }
+}
+
+// The full count runs in a graph's first round only
 __global__ __launch_bounds__(256) void k_b_clear_stats(const WorkItem *tiles, const int *count, const BatchGraph *gs,
                                                        const BatchState *st) {
     const int nt = *count;
     for (int u = blockIdx.x; u < nt; u += gridDim.x) {
         const WorkItem w = tiles[u];
-        if (st[w.g].status != BG_RUNNING) continue;
+        if (st[w.g].status != BG_RUNNING || st[w.g].ready) continue;
         const int hi = min(w.p1, st[w.g].k);
         CommStats *s = gs[w.g].stats;
         for (int c = w.p0 + (int)threadIdx.x; c < hi; c += blockDim.x) s[c] = CommStats{0, 0, 0, 0};
@@ -192,7 +212,7 @@ __global__ __launch_bounds__(256) void k_b_stats_small(const WorkItem *items, co
     const int nu = *count;
     for (int u = wave_g; u < nu; u += nwaves) {
         const WorkItem w = items[u];
-        if (st[w.g].status != BG_RUNNING) continue;
+        if (st[w.g].status != BG_RUNNING || st[w.g].ready) continue;
         const BatchGraph &g = gs[w.g];
         const LPArgs &a = g.lp;
         for (int base = w.p0; base < w.p1; base += per_wave) {
@@ -224,7 +244,7 @@ __global__ __launch_bounds__(256) void k_b_stats_rows(const RowItem *rows, const
     const int nu = *count;
     for (int u = wave_g; u < nu; u += nwaves) {
         const RowItem r = rows[u];
-        if (st[r.g].status != BG_RUNNING) continue;
+        if (st[r.g].status != BG_RUNNING || st[r.g].ready) continue;
         const BatchGraph &g = gs[r.g];
         const int l = g.cur[r.v];
         int same = 0;
@@ -240,6 +260,127 @@ __global__ __launch_bounds__(256) void k_b_stats_rows(const RowItem *rows, const
         }
         same = wave_sum(same);
         if (lane == 0) add_stats(g.stats, l, r.begin == g.lp.indices[r.v] ? 1 : 0, same, (r.end - r.begin) - same);
+    }
+}
+
+// Statistics after a propagation step, from the moved vertices only (the
+// update of partition.hip's k_delta_small / k_delta_items): cur holds the
+// labels the step read (L), next those it wrote (N), and the records of
+// graphs whose statistics follow the labels are brought from L to N under
+// L's ids; the compression that follows renumbers them.
+
+// rows of degree <= G: G lanes per vertex; unmoved vertices read no row
+template <int G>
+__global__ __launch_bounds__(256) void k_b_delta_small(const WorkItem *items, const int *count, const BatchGraph *gs,
+                                                       const BatchState *st) {
+    const int lane = __lane_id(), glane = lane & (G - 1);
+    constexpr int per_wave = WAVE / G;
+    const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    const int nu = *count;
+    for (int u = wave_g; u < nu; u += nwaves) {
+        const WorkItem w = items[u];
+        if (st[w.g].status != BG_RUNNING || !st[w.g].ready) continue;
+        const BatchGraph &g = gs[w.g];
+        const LPArgs &a = g.lp;
+        for (int base = w.p0; base < w.p1; base += per_wave) {
+            const int p = base + lane / G;
+            int start = 0, deg = 0, lo = 0, ln = 0;
+            bool moved = false;
+            if (p < w.p1) {
+                const int v = a.order ? a.order[p] : p;
+                lo = g.cur[v];
+                ln = g.next[v];
+                moved = lo != ln;
+                if (moved) {
+                    start = a.indices[v];
+                    deg = a.indices[v + 1] - start;
+                }
+            }
+            if (!__ballot(moved)) continue;
+            int sa = 0, sb = 0;
+            for (int j = glane; j < deg; j += G) {
+                const int x = a.adj[start + j];
+                const int lx = g.cur[x], nx = g.next[x];
+                const int f = lx != nx ? 1 : 2;
+                sa += lx == lo ? f : 0;
+                sb += nx == ln ? f : 0;
+            }
+#pragma unroll
+            for (int d = G / 2; d > 0; d >>= 1) {
+                sa += __shfl_xor(sa, d, WAVE);
+                sb += __shfl_xor(sb, d, WAVE);
+            }
+            if (moved && glane == 0) {
+                add_stats(g.stats, lo, -1, -sa, sa - deg);
+                if (ln >= 0) add_stats(g.stats, ln, 1, sb, deg - sb);
+            }
+        }
+    }
+}
+
+// rows of degree > 64: each wave checks 64 row items at once (lane j takes
+// item wave + (64 r + j) * waves in round r) and runs the moved ones
+__global__ __launch_bounds__(256) void k_b_delta_rows(const RowItem *rows, const int *count, const BatchGraph *gs,
+                                                      const BatchState *st) {
+    constexpr int PF = 4;
+    const int lane = __lane_id();
+    const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    const int nu = *count;
+    for (int64_t r = 0; wave_g + r * WAVE * nwaves < nu; r++) {
+        const int64_t i = wave_g + (r * WAVE + lane) * nwaves;
+        RowItem it{0, 0, 0, 0};
+        int lo = 0, ln = 0;
+        bool moved = false;
+        if (i < nu) {
+            it = rows[i];
+            if (st[it.g].status == BG_RUNNING && st[it.g].ready) {
+                const BatchGraph &g = gs[it.g];
+                lo = g.cur[it.v];
+                ln = g.next[it.v];
+                moved = lo != ln;
+            }
+        }
+        uint64_t mv = __ballot(moved);
+        while (mv) {
+            const int src = __builtin_ctzll(mv);
+            mv &= mv - 1;
+            const int xg = __builtin_amdgcn_readlane(it.g, src), xv = __builtin_amdgcn_readlane(it.v, src);
+            const int xb = __builtin_amdgcn_readlane(it.begin, src), xe = __builtin_amdgcn_readlane(it.end, src);
+            const int xlo = __builtin_amdgcn_readlane(lo, src), xln = __builtin_amdgcn_readlane(ln, src);
+            const BatchGraph &g = gs[xg];
+            int sa = 0, sb = 0;
+            for (int j0 = xb + lane; j0 < xe; j0 += PF * WAVE) {
+                int nb[PF];
+#pragma unroll
+                for (int e = 0; e < PF; e++) {
+                    const int j = j0 + e * WAVE;
+                    nb[e] = j < xe ? g.lp.adj[j] : -1;
+                }
+                int lx[PF], nx[PF];
+#pragma unroll
+                for (int e = 0; e < PF; e++) {
+                    lx[e] = nb[e] < 0 ? -2 : g.cur[nb[e]];
+                    nx[e] = nb[e] < 0 ? -2 : g.next[nb[e]];
+                }
+#pragma unroll
+                for (int e = 0; e < PF; e++) {
+                    if (nb[e] < 0) continue;
+                    const int f = lx[e] != nx[e] ? 1 : 2;
+                    sa += lx[e] == xlo ? f : 0;
+                    sb += nx[e] == xln ? f : 0;
+                }
+            }
+            sa = wave_sum(sa);
+            sb = wave_sum(sb);
+            if (lane == 0) {
+                const int len = xe - xb;
+                const int first = xb == g.lp.indices[xv] ? 1 : 0;
+                add_stats(g.stats, xlo, -first, -sa, sa - len);
+                if (xln >= 0) add_stats(g.stats, xln, first, sb, len - sb);
+            }
+        }
     }
 }
 
@@ -320,6 +461,7 @@ __global__ void k_b_finalize(const BatchGraph *gs, BatchState *st, int nslots) {
     s.step = it;
     s.iter = it + 1;
     s.label_bound = s.k;
+    s.ready = 1;  // counted in full this round; the propagation steps that follow update the counts
     if (done) s.status = BG_DONE;
     st[i] = s;
 }
@@ -442,7 +584,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static size_t footprint(int n, int64_t E, bool sorted) {
     size_t nb = align256(4 * ((size_t)n + 1));
     return nb /*indices*/ + align256(4 * (size_t)std::max<int64_t>(E, 1)) /*adj*/ + (sorted ? 0 : nb) /*order*/ +
-           4 * nb /*cur next first newid*/ + align256(16 * ((size_t)n + 1)) /*stats*/;
+           4 * nb /*cur next first newid*/ + 2 * align256(16 * ((size_t)n + 1)) /*stats, stats_alt*/;
 }
 
 static bool arena_alloc(pcd_batch *b, size_t bytes, size_t *off) {
@@ -709,6 +851,7 @@ static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, co
     int32_t *d_first = (int32_t *)(p + 2 * nb);
     int32_t *d_newid = (int32_t *)(p + 3 * nb);
     CommStats *d_stats = (CommStats *)(p + 4 * nb);
+    CommStats *d_stats_alt = (CommStats *)(p + 4 * nb + align256(16 * ((size_t)n + 1)));
     PCD_HIP(hipMemcpyAsync(d_ind, indices, 4 * ((size_t)n + 1), hipMemcpyHostToDevice, b->stream));
     PCD_HIP(hipMemcpyAsync(d_adj, adj, 4 * (size_t)E, hipMemcpyHostToDevice, b->stream));
     if (!sorted) PCD_HIP(hipMemcpyAsync(d_order, pr.order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, b->stream));
@@ -735,6 +878,7 @@ static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, co
     g.next = d_next;
     g.cur = d_cur;
     g.stats = d_stats;
+    g.stats_alt = d_stats_alt;
     g.first = d_first;
     g.newid = d_newid;
     g.n = n;
@@ -1077,6 +1221,19 @@ static int launch_round(pcd_batch *b) {
     const int *ntile = b->d_counts + L_TILES;
     const BatchGraph *gs = b->d_graphs;
     BatchState *st = b->d_state;
+    // statistics of the last round's propagation step (graphs past their
+    // first round)
+    hipLaunchKernelGGL(k_b_delta_rows, dim3(cu * 8), dim3(256), 0, s, list_ptr<RowItem>(b, L_ROWS),
+                       b->d_counts + L_ROWS, gs, st);
+#define PCD_BDELTA(I, G)                                                                                    \
+    hipLaunchKernelGGL(k_b_delta_small<G>, dim3(cu * 8), dim3(256), 0, s, list_ptr<WorkItem>(b, L_SMALL0 + I), \
+                       b->d_counts + L_SMALL0 + I, gs, st);
+    PCD_BDELTA(0, 64)
+    PCD_BDELTA(1, 32)
+    PCD_BDELTA(2, 16)
+    PCD_BDELTA(3, 8)
+    PCD_BDELTA(4, 4)
+#undef PCD_BDELTA
     // compress_labels (framework.cpp:146-170)
     hipLaunchKernelGGL(k_b_fill_first, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
     hipLaunchKernelGGL(k_b_first_occurrence, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
@@ -1085,7 +1242,8 @@ static int launch_round(pcd_batch *b) {
     hipLaunchKernelGGL(k_b_tile_apply, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, b->d_gtiles, gs, st,
                        b->d_tile_sum, b->d_tile_pre);
     hipLaunchKernelGGL(k_b_relabel, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
-    // collect_statistics (framework.cpp:69-143)
+    hipLaunchKernelGGL(k_b_stats_copy, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
+    // collect_statistics (framework.cpp:69-143) in a graph's first round
     hipLaunchKernelGGL(k_b_clear_stats, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
     hipLaunchKernelGGL(k_b_stats_rows, dim3(cu * 8), dim3(256), 0, s, list_ptr<RowItem>(b, L_ROWS),
                        b->d_counts + L_ROWS, gs, st);

@@ -66,6 +66,7 @@ struct BatchGraph {
     int32_t *next;        // == lp.new_labels
     int32_t *cur;         // == lp.labels
     CommStats *stats;     // == lp.stats
+    CommStats *stats_alt; // compression renumbers the statistics into it
     int32_t *first, *newid;
     int n, max_iter;      // max_iter -1: stop on the score threshold
     double threshold;
@@ -80,10 +81,12 @@ struct BatchState {
     int k;            // communities after this round's compression
     int label_bound;  // labels in `next` are < label_bound or NEW_LABEL
     int capped;       // stopped by iter_cap rather than the stop rule
+    int ready;        // stats follow the labels: each round updates them from the moved vertices
+    int pad;
     double score, prev_score;
     unsigned long long acc[3];
 };
-static_assert(sizeof(BatchState) == 64, "batch state record");
+static_assert(sizeof(BatchState) == 72, "batch state record");
 
 struct ItemSrc {
     const WorkItem *items;

@@ -123,19 +123,33 @@ def test_autotune_keeps_results():
             assert np.array_equal(b.labels(s), r[0]) and b.status(s)[1:] == r[1:4]
 
 
-def test_batch_statistics_match_plain():
+@pytest.mark.parametrize("case", ["mod", "cpm_leave", "capped"])
+def test_batch_statistics_match_plain(case):
+    """Statistics at the end (updated from the moved vertices every round
+    after the first) equal the oracle's full count; the CPM case starts from
+    a coarse random partition at a resolution where vertices leave to
+    NEW_LABEL singletons, the capped case stops after a few rounds while
+    many vertices still move."""
     g = sbm(10, 100, 0.08, 0.002, seed=12)
-    lab = singleton_labels(g[0])
-    jb = parcd.job(kind=O.MOD, seed=1)
+    n = g[0]
+    if case == "mod":
+        lab, jb = singleton_labels(n), parcd.job(kind=O.MOD, seed=1)
+    elif case == "cpm_leave":
+        lab, jb = np.random.default_rng(4).integers(0, 4, n).astype(np.int32), parcd.job(kind=O.CPM, lam=0.3, seed=2)
+    else:
+        lab, jb = singleton_labels(n), parcd.job(kind=O.MOD, seed=3, max_iter=3)
     with parcd.Batch(0, max_graphs=2, arena_bytes=64 << 20) as b:
         s = b.add(g[0], g[1], g[2], g[3], lab, jb)
         while b.step(1):
             pass
         bs = b.stats(s)
         bl = b.labels(s)
+        st = b.status(s)
     k, sz, iv, ev = O.collect_statistics(g[0], g[2], g[3], bl)
-    assert k == len(bs[0])
+    assert k == len(bs[0]) == st[2]
     assert np.array_equal(bs[0], sz) and np.array_equal(bs[1], iv) and np.array_equal(bs[2], ev)
+    pl, ps, pit, pk = plain(g, lab, jb)
+    assert np.array_equal(bl, pl) and st[1:] == (pit, pk, ps)
 
 
 def test_batch_rejections():
