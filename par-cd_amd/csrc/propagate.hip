@@ -344,42 +344,50 @@ __device__ inline int round64(int x) { return (x + 63) & ~63; }
 // row's distinct labels, so probing always ends at a free or matching slot.
 __device__ inline uint32_t pslot(int key, int log_t) { return ((uint32_t)key * 0x9E3779B1u) >> (32 - log_t); }
 
-__device__ inline void pinsert(int *keys, int *cnts, int mask, int log_t, int key) {
+// Inserts `key` (with AUX its aux, stored by the lane that claims the slot)
+// with count 1; returns true when this lane claimed a free slot, whose
+// index lands in `slot`.
+template <bool AUX>
+__device__ inline bool pinsert(int *keys, int *cnts, int *auxs, int mask, int log_t, int key, int aux, int &slot) {
     uint32_t h = pslot(key, log_t);
     while (true) {
         int old = atomicCAS(&keys[h], EMPTY_KEY, key);
+        if (AUX && old == EMPTY_KEY) auxs[h] = aux;
         if (old == EMPTY_KEY || old == key) {
             atomicAdd(&cnts[h], 1);
-            return;
+            slot = (int)h;
+            return old == EMPTY_KEY;
         }
         h = (h + 1) & mask;
     }
 }
 
-// As pinsert; the lane that claims a slot also stores the label's aux.
-__device__ inline void pinsert_aux(int *keys, int *cnts, int *auxs, int mask, int log_t, int key, int aux) {
-    uint32_t h = pslot(key, log_t);
-    while (true) {
-        int old = atomicCAS(&keys[h], EMPTY_KEY, key);
-        if (old == EMPTY_KEY) auxs[h] = aux;
-        if (old == EMPTY_KEY || old == key) {
-            atomicAdd(&cnts[h], 1);
-            return;
-        }
-        h = (h + 1) & mask;
-    }
+// Appends the slots this wave's lanes claimed to the table's slot list
+// (one LDS atomic per wave).  Called by every lane of the wave.
+__device__ inline void occ_append(bool claimed, int slot, uint16_t *occ, int *nocc) {
+    const uint64_t cm = __ballot(claimed);
+    if (!cm) return;
+    const int lane = __lane_id();
+    const int leader = __builtin_ctzll(cm);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(nocc, __popcll(cm));
+    base = __builtin_amdgcn_readlane(base, leader);
+    if (claimed) occ[base + __popcll(cm & ((1ULL << lane) - 1))] = (uint16_t)slot;
 }
 
-// With AUX the neighbour records {label, aux} are gathered (one 8-byte
-// gather brings what the gains read) and aux lands in the table.
+// Gathers the labels (with AUX the {label, aux} records: one 8-byte gather
+// brings what the gains read, and aux lands in the table) of row entries
+// base + lo + e * stride, PF at a time (independent loads in flight), and
+// inserts them; claimed slots go to the slot list.  The loop bounds are the
+// same for every lane of the workgroup.
 template <int PF, bool AUX = false>
-__device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int *auxs, int mask, int log_t, int start,
-                                     int deg, int lo, int stride) {
-    for (int j0 = lo; j0 < deg; j0 += PF * stride) {
+__device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int *auxs, uint16_t *occ, int *nocc,
+                                     int mask, int log_t, int start, int deg, int lo, int stride) {
+    for (int base = 0; base < deg; base += PF * stride) {
         int nb[PF];
 #pragma unroll
         for (int e = 0; e < PF; e++) {
-            int j = j0 + e * stride;
+            int j = base + lo + e * stride;
             nb[e] = j < deg ? a.adj[start + j] : -1;
         }
         int2 rec[PF];
@@ -388,9 +396,57 @@ __device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int 
             rec[e] = nb[e] < 0 ? make_int2(EMPTY_KEY, 0) : (AUX ? a.lv[nb[e]] : make_int2(a.labels[nb[e]], 0));
 #pragma unroll
         for (int e = 0; e < PF; e++) {
-            if (rec[e].x == EMPTY_KEY) continue;
-            if (AUX) pinsert_aux(keys, cnts, auxs, mask, log_t, rec[e].x, rec[e].y);
-            else pinsert(keys, cnts, mask, log_t, rec[e].x);
+            bool claimed = false;
+            int slot = 0;
+            if (rec[e].x != EMPTY_KEY) claimed = pinsert<AUX>(keys, cnts, auxs, mask, log_t, rec[e].x, rec[e].y, slot);
+            occ_append(claimed, slot, occ, nocc);
+        }
+    }
+}
+
+// Scans the D listed slots [lo, D) with stride `stride` for candidates, SB
+// at a time so that their community-record gathers overlap, and clears each
+// slot it read (the table is empty again for the next row).
+template <bool AUX, unsigned KM>
+__device__ inline void scan_listed(const LPArgs &a, int *keys, int *cnts, const int *auxs, const uint16_t *occ, int D,
+                                   int lo, int stride, int deg, int own, uint64_t tb, Cand &c, int &own_cnt) {
+    for (int i0 = lo; i0 < D; i0 += SB * stride) {
+        int key[SB], cnt[SB];
+        CommStats st[SB];
+#pragma unroll
+        for (int e = 0; e < SB; e++) {
+            const int i = i0 + e * stride;
+            key[e] = EMPTY_KEY;
+            cnt[e] = 0;
+            if (i < D) {
+                const int h = occ[i];
+                key[e] = keys[h];
+                cnt[e] = cnts[h];
+                if (AUX) {
+                    const int x = auxs[h];
+                    st[e] = CommStats{x, x, 0, 0};
+                }
+                keys[h] = EMPTY_KEY;
+                cnts[h] = 0;
+            }
+        }
+        if (!AUX) {
+#pragma unroll
+            for (int e = 0; e < SB; e++)
+                if (key[e] != EMPTY_KEY && key[e] != own) st[e] = a.stats[key[e]];
+        }
+#pragma unroll
+        for (int e = 0; e < SB; e++) {
+            if (key[e] == EMPTY_KEY) continue;
+            if (key[e] == own) {
+                own_cnt = cnt[e];
+            } else {
+                double sc = score_join<KM>(a.rule, deg, st[e], cnt[e]);
+                if (sc > 0) {
+                    Cand o{sc, tie_key(a.tie, tb, key[e]), key[e]};
+                    if (cand_better(o, c)) c = o;
+                }
+            }
         }
     }
 }
@@ -398,13 +454,24 @@ __device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int 
 __device__ inline int ceil_log2(int x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
 
 // ------------------------------------------------------------- wave / vertex
+// The table starts empty and each row's scan clears the slots it used.
 template <int TW, unsigned KM, class Src, bool AUX = false>
 __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
     __shared__ int s_keys[256 / WAVE][TW];
     __shared__ int s_cnts[256 / WAVE][TW];
     __shared__ int s_auxs[256 / WAVE][AUX ? TW : 1];
+    __shared__ uint16_t s_occ[256 / WAVE][TW / 2];
+    __shared__ int s_nocc[256 / WAVE];
     const int lane = __lane_id(), w = threadIdx.x / WAVE;
     int *keys = s_keys[w], *cnts = s_cnts[w], *auxs = s_auxs[w];
+    uint16_t *occ = s_occ[w];
+    int *nocc = &s_nocc[w];
+    for (int i = lane; i < TW; i += WAVE) {
+        keys[i] = EMPTY_KEY;
+        cnts[i] = 0;
+    }
+    if (lane == 0) *nocc = 0;
+    wave_sync();
     const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
     const int nwaves = gridDim.x * blockDim.x / WAVE;
     const int nunits = src.units();
@@ -419,22 +486,19 @@ __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
         int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
         int own = ownr.x;
         int log_t = max(6, ceil_log2(2 * deg));
-        Table t{keys, cnts, AUX ? auxs : nullptr, 1 << log_t};
-        tclear(t, lane, WAVE);
+        insert_labels<4, AUX>(a, keys, cnts, auxs, occ, nocc, (1 << log_t) - 1, log_t, start, deg, lane, WAVE);
         wave_sync();
-        insert_labels<4, AUX>(a, keys, cnts, auxs, t.T - 1, log_t, start, deg, lane, WAVE);
-        wave_sync();
+        const int D = *nocc;
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
-        int own_cnt = 0, distinct = 0;
-        scan_table<AUX, KM>(a, t, lane, WAVE, deg, own, tb, c, own_cnt, distinct);
+        int own_cnt = 0;
+        scan_listed<AUX, KM>(a, keys, cnts, auxs, occ, D, lane, WAVE, deg, own, tb, c, own_cnt);
         wave_reduce_cand(c, own_cnt);
-        if (lane == 0)
+        if (lane == 0) {
             a.new_labels[v] = AUX ? lp_decide<KM>(a, deg, own, ownr.y, own_cnt, c)
                                   : lp_decide_stats<KM>(a, deg, own, own_cnt, c);
-        if (a.counters) {
-            distinct = wave_sum(distinct);
-            if (lane == 0) count3(a, 1, deg, distinct);
+            if (a.counters) count3(a, 1, deg, D);
+            *nocc = 0;
         }
         wave_sync();
       }
@@ -448,8 +512,16 @@ __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
     __shared__ int keys[TB];
     __shared__ int cnts[TB];
     __shared__ int auxs[AUX ? TB : 1];
+    __shared__ uint16_t occ[TB / 2];
+    __shared__ int s_nocc;
     __shared__ Cand s_c[NT / WAVE];
     __shared__ int s_i[2 * NT / WAVE];
+    for (int i = threadIdx.x; i < TB; i += NT) {
+        keys[i] = EMPTY_KEY;
+        cnts[i] = 0;
+    }
+    if (threadIdx.x == 0) s_nocc = 0;
+    __syncthreads();
     const int nunits = src.units();
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
       LPArgs a;
@@ -462,21 +534,22 @@ __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
         int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
         int own = ownr.x;
         int log_t = max(6, ceil_log2(2 * deg));
-        Table t{keys, cnts, AUX ? auxs : nullptr, 1 << log_t};
-        tclear(t, threadIdx.x, NT);
+        insert_labels<4, AUX>(a, keys, cnts, auxs, occ, &s_nocc, (1 << log_t) - 1, log_t, start, deg, threadIdx.x,
+                              NT);
         __syncthreads();
-        insert_labels<4, AUX>(a, keys, cnts, auxs, t.T - 1, log_t, start, deg, threadIdx.x, NT);
-        __syncthreads();
+        const int D = s_nocc;
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_table<AUX, KM>(a, t, threadIdx.x, NT, deg, own, tb, c, own_cnt, distinct);
+        scan_listed<AUX, KM>(a, keys, cnts, auxs, occ, D, threadIdx.x, NT, deg, own, tb, c, own_cnt);
         block_reduce_cand<NT>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
             a.new_labels[v] = AUX ? lp_decide<KM>(a, deg, own, ownr.y, own_cnt, c)
                                   : lp_decide_stats<KM>(a, deg, own, own_cnt, c);
-            if (a.counters) count3(a, 1, deg, distinct);
+            if (a.counters) count3(a, 1, deg, D);
+            s_nocc = 0;
         }
+        __syncthreads();
       }
     }
 }
