@@ -371,7 +371,9 @@ def batch_leg(parcd, np, graphs_count, seed=4):
     ctx = parcd.Context(0)
     t2 = time.perf_counter()
     res_p = []
-    for g, lab, jb in zip(gs, labs, jobs):
+    for i, (g, lab, jb) in enumerate(zip(gs, labs, jobs)):
+        if i and i % 1024 == 0:
+            log("plain per graph: %d of %d" % (i, len(gs)))
         ctx.upload_graph(g[0], g[1], g[2], g[3])
         ctx.upload_labels(lab)
         sc, it, k = ctx.refine(jb)
