@@ -332,6 +332,18 @@ int pcd_gen_requests(uint64_t seed, int count, int64_t m_lo, int64_t m_hi, int32
 int pcd_clustering_partition(int n, const int32_t *indices, const int32_t *adj, const int32_t *tri,
                              int32_t *labels);
 
+/* Normalized mutual information of two labelings of the n vertices
+ * (common.cpp:170-176 compare_communities(g, a, b) with
+ * common.cpp:75-128 norm_mutual_info; labels >= 0).  As in the reference
+ * the count product n_i * n_j is int32 arithmetic: past 2^31 it wraps, and
+ * the result can exceed 1 or be NaN; both labelings with one community
+ * give 0/0 = NaN. */
+int pcd_nmi(int n, const int32_t *a, const int32_t *b, double *nmi);
+/* The same against a ground-truth file (common.cpp:130-168
+ * compare_communities(g, labels, filename)): each line that is not empty and
+ * does not start with '#' lists the vertices of one community; N stays n. */
+int pcd_nmi_file(int n, const int32_t *labels, const char *path, double *nmi);
+
 /* Seeded synthetic graphs (edge lists, loop-free; duplicates possible).
  * SplitMix64 streams; same seed -> same edges on any host. */
 int pcd_gen_sbm(uint64_t seed, int num_blocks, const int32_t *block_sizes, double p_in, double p_out,

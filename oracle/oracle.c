@@ -854,3 +854,102 @@ int orc_run(int n, int m, const int *indices, const int *adj, int kind, double l
     free(tri); free(tri_deg);
     return 0;
 }
+
+/* ------------------------------------------------------------------ NMI
+ * norm_mutual_info (common.cpp:75-128) over the confusion counts of
+ * compare_communities (common.cpp:130-176).  The reference keeps the counts
+ * in a std::map keyed by (i, j): nom sums over pairs in ascending (i, j)
+ * order, den over i ascending then j ascending; ni[i] * nj[j] is an int32
+ * product (it wraps for products of 2^31 and more) converted to double
+ * after the multiplication.  Labels and community indices must be >= 0. */
+static int cmp_pair64(const void *x, const void *y) {
+    uint64_t a = *(const uint64_t *)x, b = *(const uint64_t *)y;
+    return a < b ? -1 : a > b;
+}
+
+static double nmi_of_pairs(int n, uint64_t *keys, long cnt) {
+    qsort(keys, (size_t)cnt, sizeof(uint64_t), cmp_pair64);
+    /* counts per row i and column j: sorted unique values with their sums */
+    long ui = 0, uj = 0;
+    int *ri = malloc(sizeof(int) * (cnt + 1)), *ci = malloc(sizeof(int) * (cnt + 1));
+    int *rj = malloc(sizeof(int) * (cnt + 1)), *cj = malloc(sizeof(int) * (cnt + 1));
+    uint32_t *js = malloc(sizeof(uint32_t) * (cnt + 1));
+    for (long t = 0; t < cnt; t++) {
+        int i = (int)(keys[t] >> 32);
+        if (ui == 0 || ri[ui - 1] != i) { ri[ui] = i; ci[ui] = 0; ui++; }
+        ci[ui - 1]++;
+        js[t] = (uint32_t)keys[t];
+    }
+    uint32_t *jsorted = malloc(sizeof(uint32_t) * (cnt + 1));
+    memcpy(jsorted, js, sizeof(uint32_t) * cnt);
+    qsort(jsorted, (size_t)cnt, sizeof(uint32_t), cmp_int);  /* values < 2^31 */
+    for (long t = 0; t < cnt; t++) {
+        int j = (int)jsorted[t];
+        if (uj == 0 || rj[uj - 1] != j) { rj[uj] = j; cj[uj] = 0; uj++; }
+        cj[uj - 1]++;
+    }
+    double nom = 0.0, den = 0.0;
+    long a = 0, ib = 0;
+    while (a < cnt) {
+        long b = a;
+        while (b < cnt && keys[b] == keys[a]) b++;
+        int i = (int)(keys[a] >> 32), j = (int)(uint32_t)keys[a], val = (int)(b - a);
+        while (ri[ib] != i) ib++;
+        long lo = 0, hi = uj - 1;
+        while (lo < hi) { long mid = (lo + hi) / 2; if (rj[mid] < j) lo = mid + 1; else hi = mid; }
+        nom += val * log(((double)val * n) / wrap_mul(ci[ib], cj[lo]));
+        a = b;
+    }
+    for (long t = 0; t < ui; t++) den += ci[t] * log(ci[t] / ((double)n));
+    for (long t = 0; t < uj; t++) den += cj[t] * log(cj[t] / ((double)n));
+    free(ri); free(ci); free(rj); free(cj); free(js); free(jsorted);
+    return -2 * nom / den;
+}
+
+/* compare_communities(g, a, b) (common.cpp:170-176); -1 on negative labels */
+int orc_nmi(int n, const int *a, const int *b, double *out) {
+    uint64_t *keys = malloc(sizeof(uint64_t) * (n > 0 ? n : 1));
+    for (int v = 0; v < n; v++) {
+        if (a[v] < 0 || b[v] < 0) { free(keys); return -1; }
+        keys[v] = ((uint64_t)(uint32_t)a[v] << 32) | (uint32_t)b[v];
+    }
+    *out = nmi_of_pairs(n, keys, n);
+    free(keys);
+    return 0;
+}
+
+/* compare_communities(g, labels, filename) (common.cpp:130-168): community
+ * c (from 1, in line order; empty and '#' lines skipped) holds the integers
+ * of its line up to the first token that is not one.  -1: unreadable file,
+ * a vertex outside [0, n) or a negative label. */
+int orc_nmi_file(int n, const int *labels, const char *path, double *out) {
+    FILE *fp = fopen(path, "r");
+    if (!fp) return -1;
+    long cap = 1024, cnt = 0;
+    uint64_t *keys = malloc(sizeof(uint64_t) * cap);
+    char *line = NULL;
+    size_t lcap = 0;
+    ssize_t len;
+    int comm = 0, bad = 0;
+    while (!bad && (len = getline(&line, &lcap, fp)) >= 0) {
+        if (len > 0 && line[len - 1] == '\n') line[--len] = 0;
+        if (len == 0 || line[0] == '#') continue;
+        comm++;
+        char *p = line;
+        for (;;) {
+            char *e;
+            long v = strtol(p, &e, 10);
+            if (e == p) break;
+            p = e;
+            if (v < 0 || v >= n || labels[v] < 0) { bad = 1; break; }
+            if (cnt == cap) { cap *= 2; keys = realloc(keys, sizeof(uint64_t) * cap); }
+            keys[cnt++] = ((uint64_t)(uint32_t)comm << 32) | (uint32_t)labels[v];
+        }
+    }
+    free(line);
+    fclose(fp);
+    if (bad) { free(keys); return -1; }
+    *out = nmi_of_pairs(n, keys, cnt);
+    free(keys);
+    return 0;
+}

@@ -240,13 +240,45 @@ static int cmd_refine(int argc, char **argv) {
     return 0;
 }
 
+// nmi <pcd> <in> <out>: compare_communities(g, a, b) (common.cpp:170-176)
+static int cmd_nmi(int argc, char **argv) {
+    const Graph *g = load(argv[2]);
+    map<string, Rec> in = read_recs(argv[3]);
+    const int n = g->n;
+    int *a = new int[n], *b = new int[n];
+    for (int v = 0; v < n; v++) { a[v] = in["a"].i[v]; b[v] = in["b"].i[v]; }
+    double nmi = compare_communities(*g, a, b);
+    g_out = fopen(argv[4], "wb");
+    put_d("nmi", &nmi, 1);
+    fclose(g_out);
+    return 0;
+}
+
+// nmifile <pcd> <in> <truth> <out>: compare_communities(g, labels, file)
+// (common.cpp:130-168)
+static int cmd_nmifile(int argc, char **argv) {
+    if (argc < 6) return 2;
+    const Graph *g = load(argv[2]);
+    map<string, Rec> in = read_recs(argv[3]);
+    const int n = g->n;
+    int *labels = new int[n];
+    for (int v = 0; v < n; v++) labels[v] = in["labels"].i[v];
+    double nmi = compare_communities(*g, labels, string(argv[4]));
+    g_out = fopen(argv[5], "wb");
+    put_d("nmi", &nmi, 1);
+    fclose(g_out);
+    return 0;
+}
+
 int main(int argc, char **argv) {
-    if (argc < 4) { fprintf(stderr, "usage: ref_harness graph|part|synth|refine ...\n"); return 2; }
+    if (argc < 4) { fprintf(stderr, "usage: ref_harness graph|part|synth|refine|nmi|nmifile ...\n"); return 2; }
     string c = argv[1];
     if (c == "graph") return cmd_graph(argc, argv);
     if (c == "part") return cmd_part(argc, argv);
     if (c == "synth") return cmd_synth(argc, argv);
     if (c == "refine") return cmd_refine(argc, argv);
+    if (c == "nmi") return cmd_nmi(argc, argv);
+    if (c == "nmifile") return cmd_nmifile(argc, argv);
     fprintf(stderr, "unknown command %s\n", argv[1]);
     return 2;
 }

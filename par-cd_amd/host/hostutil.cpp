@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <thread>
 #include <vector>
@@ -646,5 +648,87 @@ extern "C" int pcd_clustering_partition(int n, const int32_t *indices, const int
             if (labels[adj[j]] == PCD_NEW_LABEL) labels[adj[j]] = nc;
         nc++;
     }
+    return PCD_OK;
+}
+
+// ----------------------------------------------------------------- NMI
+// norm_mutual_info (common.cpp:75-128) over the confusion counts n_ij of two
+// labelings: -2 sum n_ij log(n_ij N / (n_i n_j)) / (sum n_i log(n_i / N) +
+// sum n_j log(n_j / N)), N = the graph's vertex count.  The reference keeps
+// n_ij in a std::map, so the first sum runs over (i, j) ascending and the
+// second over i, then j ascending; here the (i, j) keys are sorted once.
+// n_i * n_j is an int32 product there and wraps from 2^31 on; it is kept so
+// (a result above 1, or NaN, flags a run beyond that range).
+static double nmi_sorted_pairs(int n, std::vector<uint64_t> &keys) {
+    std::sort(keys.begin(), keys.end());
+    struct Count {
+        int id, count;
+    };
+    std::vector<Count> rows, cols;
+    std::vector<int32_t> js(keys.size());
+    for (size_t t = 0; t < keys.size(); t++) {
+        const int i = (int)(keys[t] >> 32);
+        if (rows.empty() || rows.back().id != i) rows.push_back(Count{i, 0});
+        rows.back().count++;
+        js[t] = (int32_t)(uint32_t)keys[t];
+    }
+    std::sort(js.begin(), js.end());
+    for (int32_t j : js) {
+        if (cols.empty() || cols.back().id != j) cols.push_back(Count{j, 0});
+        cols.back().count++;
+    }
+    double nom = 0.0, den = 0.0;
+    size_t row = 0;
+    for (size_t a = 0; a < keys.size();) {
+        size_t b = a;
+        while (b < keys.size() && keys[b] == keys[a]) b++;
+        const int i = (int)(keys[a] >> 32), j = (int)(uint32_t)keys[a], val = (int)(b - a);
+        while (rows[row].id != i) row++;
+        const Count &col = *std::lower_bound(cols.begin(), cols.end(), j,
+                                             [](const Count &c, int x) { return c.id < x; });
+        const int prod = (int)((unsigned)rows[row].count * (unsigned)col.count);
+        nom += val * std::log(((double)val * n) / prod);
+        a = b;
+    }
+    for (const Count &r : rows) den += r.count * std::log(r.count / ((double)n));
+    for (const Count &c : cols) den += c.count * std::log(c.count / ((double)n));
+    return -2 * nom / den;
+}
+
+// compare_communities(g, a, b) (common.cpp:170-176)
+extern "C" int pcd_nmi(int n, const int32_t *a, const int32_t *b, double *nmi) {
+    if (n <= 0 || !a || !b || !nmi) return fail(PCD_E_INVALID, "bad arguments");
+    std::vector<uint64_t> keys((size_t)n);
+    for (int v = 0; v < n; v++) {
+        if (a[v] < 0 || b[v] < 0) return fail(PCD_E_INVALID, "labels must be >= 0");
+        keys[v] = ((uint64_t)(uint32_t)a[v] << 32) | (uint32_t)b[v];
+    }
+    *nmi = nmi_sorted_pairs(n, keys);
+    return PCD_OK;
+}
+
+// compare_communities(g, labels, filename) (common.cpp:130-168): line c
+// (from 1, empty and '#' lines not counted) lists the vertices of ground
+// truth community c, read as integers up to the first token that is not one.
+extern "C" int pcd_nmi_file(int n, const int32_t *labels, const char *path, double *nmi) {
+    if (n <= 0 || !labels || !path || !nmi) return fail(PCD_E_INVALID, "bad arguments");
+    std::ifstream f(path);
+    if (!f.is_open()) return fail(PCD_E_IO, std::string("failed to open ") + path);
+    std::vector<uint64_t> keys;
+    std::string line;
+    int comm = 0;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        comm++;
+        std::istringstream ss(line);
+        int v;
+        while (ss >> v) {
+            if (v < 0 || v >= n) return fail(PCD_E_FORMAT, "vertex " + std::to_string(v) + " outside [0, n)");
+            if (labels[v] < 0) return fail(PCD_E_INVALID, "labels must be >= 0");
+            keys.push_back(((uint64_t)(uint32_t)comm << 32) | (uint32_t)labels[v]);
+        }
+    }
+    if (f.fail() && !f.eof()) return fail(PCD_E_IO, std::string("error while reading ") + path);
+    *nmi = nmi_sorted_pairs(n, keys);
     return PCD_OK;
 }

@@ -28,7 +28,7 @@ EXPORTS = [
     "pcd_bin_count", "pcd_bin_name", "pcd_bin_counters_reset", "pcd_bin_counters", "pcd_bin_size",
     "pcd_singleton_partition", "pcd_set_tuning", "pcd_hub_overflows", "pcd_copy_probe",
     "pcd_batch_create", "pcd_batch_destroy", "pcd_batch_footprint", "pcd_batch_add", "pcd_batch_add_many",
-    "pcd_batch_labels_many", "pcd_batch_step",
+    "pcd_batch_labels_many", "pcd_batch_step", "pcd_nmi", "pcd_nmi_file",
     "pcd_batch_status", "pcd_batch_finished", "pcd_batch_labels", "pcd_batch_stats", "pcd_batch_release",
     "pcd_batch_info", "pcd_batch_map_count", "pcd_batch_map_name", "pcd_batch_set_map", "pcd_batch_autotune",
     "pcd_gen_sbm_batch", "pcd_refine", "pcd_host_alloc", "pcd_host_free", "pcd_clustering_partition",
@@ -138,6 +138,8 @@ def lib():
     L.pcd_batch_add.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.POINTER(Job), ip]
     L.pcd_batch_add_many.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     L.pcd_batch_labels_many.argtypes = [vp, C.c_int, vp, vp]
+    L.pcd_nmi.argtypes = [C.c_int, vp, vp, dp]
+    L.pcd_nmi_file.argtypes = [C.c_int, vp, C.c_char_p, dp]
     L.pcd_batch_step.argtypes = [vp, C.c_int, ip]
     L.pcd_batch_status.argtypes = [vp, C.c_int, ip, ip, ip, dp]
     L.pcd_batch_finished.argtypes = [vp, vp, C.c_int, ip]
@@ -541,6 +543,25 @@ def batch_footprint(n, m):
     b = C.c_int64()
     check(lib().pcd_batch_footprint(n, m, C.byref(b)))
     return b.value
+
+
+def nmi(a, b):
+    """Normalized mutual information of two labelings (pcd_nmi)."""
+    a, b = _i32(a), _i32(b)
+    if a.size != b.size:
+        raise PcdError("labelings differ in length")
+    out = C.c_double()
+    check(lib().pcd_nmi(a.size, _p(a), _p(b), C.byref(out)))
+    return out.value
+
+
+def nmi_file(labels, path):
+    """Normalized mutual information against a ground-truth community file
+    (pcd_nmi_file)."""
+    lab = _i32(labels)
+    out = C.c_double()
+    check(lib().pcd_nmi_file(lab.size, _p(lab), os.fsencode(path), C.byref(out)))
+    return out.value
 
 
 class Batch:

@@ -13,6 +13,10 @@ inputs and the reference's outputs:
   run_<...>.npz       full `main -p 1` runs: labels, score, iterations
   synth_scores.npz    score functions on fabricated statistics, including
                       the int32-overflow region of the WCC gains
+  nmi_<case>.npz      compare_communities on two labelings or on a labeling
+                      and a ground-truth community file (written by
+                      `make_golden.py nmi`, which leaves the other fixtures
+                      alone)
 
 Input graphs are seeded numpy draws (small SBM, G(n,p), R-MAT, star+clique
 and hand-written edge lists).
@@ -278,5 +282,84 @@ def main():
     print("done")
 
 
+def make_nmi():
+    """compare_communities (common.cpp:75-176) on seeded labelings of the
+    golden graphs and of a 150,000-vertex path, where the int32 product
+    ni * nj of norm_mutual_info wraps (finite but shifted values, and NaN
+    where the wrapped product is negative)."""
+    rng = np.random.default_rng(20240917)
+    with tempfile.TemporaryDirectory() as tmp:
+        pcds = {}
+        for name in ["sbm200", "rmat9", "gnp150"]:
+            d = np.load(os.path.join(HERE, "graph_%s.npz" % name))
+            path = os.path.join(tmp, name + ".pcd")
+            open(path, "wb").write(d["pcd"].tobytes())
+            pcds[name] = path
+        npath = 150000
+        text = "".join("%d %d\n" % (i, i + 1) for i in range(npath - 1))
+        _, out = convert(text, tmp)
+        pcds["path150k"] = os.path.join(tmp, "path.pcd")
+        os.rename(out, pcds["path150k"])
+
+        def pair(case, name, a, b):
+            n = read_pcd(pcds[name])[0]
+            assert len(a) == n and len(b) == n
+            inp, out = os.path.join(tmp, "n.in"), os.path.join(tmp, "n.out")
+            write_recs(inp, {"a": np.asarray(a), "b": np.asarray(b)})
+            r = run([os.path.join(REF, "ref_harness"), "nmi", pcds[name], inp, out])
+            assert r.returncode == 0, r.stderr
+            res = read_recs(out)
+            np.savez_compressed(os.path.join(HERE, "nmi_%s.npz" % case), n=n, a=np.asarray(a, np.int32),
+                                b=np.asarray(b, np.int32), nmi=res["nmi"])
+            print("nmi", case, res["nmi"])
+
+        def truth(case, name, labels, text):
+            n = read_pcd(pcds[name])[0]
+            inp, out = os.path.join(tmp, "n.in"), os.path.join(tmp, "n.out")
+            tpath = os.path.join(tmp, "truth.txt")
+            open(tpath, "w").write(text)
+            write_recs(inp, {"labels": np.asarray(labels)})
+            r = run([os.path.join(REF, "ref_harness"), "nmifile", pcds[name], inp, tpath, out])
+            assert r.returncode == 0, r.stderr
+            res = read_recs(out)
+            np.savez_compressed(os.path.join(HERE, "nmi_%s.npz" % case), n=n, labels=np.asarray(labels, np.int32),
+                                truth=np.frombuffer(text.encode(), np.uint8), nmi=res["nmi"])
+            print("nmi", case, res["nmi"])
+
+        for name in ["sbm200", "rmat9", "gnp150"]:
+            n = read_pcd(pcds[name])[0]
+            c1, c2 = rng.integers(0, 8, n), rng.integers(0, 12, n)
+            pair("%s_random" % name, name, c1, c2)
+            pair("%s_same" % name, name, c1, c1)
+            pair("%s_singletons" % name, name, rng.permutation(n), c2)
+            pair("%s_one" % name, name, np.zeros(n, np.int64), c1)
+            pair("%s_sparse_ids" % name, name, c1 * 1000 + 999983, c2 * 7 + 5)
+        n = read_pcd(pcds["sbm200"])[0]
+        pair("sbm200_both_one", "sbm200", np.zeros(n, np.int64), np.zeros(n, np.int64))
+        ids = np.arange(npath)
+        pair("path150k_wrap", "path150k", (ids >= 80000).astype(np.int64), (ids >= 70000).astype(np.int64))
+        pair("path150k_nan", "path150k", ids // 50000, ids // 50000)
+        pair("path150k_random", "path150k", rng.integers(0, 40, npath), rng.integers(0, 60, npath))
+        # ground-truth files: comments, blank lines, tabs, a line cut by a
+        # non-number, vertices listed twice or never
+        n = read_pcd(pcds["sbm200"])[0]
+        lab = rng.integers(0, 10, n)
+        groups = [[v for v in range(n) if v % 8 == g] for g in range(8)]
+        lines = ["# planted blocks", ""]
+        for g, vs in enumerate(groups):
+            sep = "\t" if g % 3 == 0 else " "
+            lines.append(sep.join(str(v) for v in vs))
+        lines.append("3 4 x 5")
+        lines.append("")
+        lines.append("7 7 9")
+        truth("sbm200_file", "sbm200", lab, "\n".join(lines) + "\n")
+        truth("sbm200_file_partial", "sbm200", lab, "\n".join(" ".join(str(v) for v in vs[: len(vs) // 2])
+                                                                for vs in groups[:5]) + "\n")
+    print("done")
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["nmi"]:
+        make_nmi()
+    else:
+        main()

@@ -81,6 +81,10 @@ def lib():
     L.orc_run.argtypes = [C.c_int, C.c_int, IP, IP, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                           C.c_double, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_void_p, C.c_double,
                           IP, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_int]
+    L.orc_nmi.restype = C.c_int
+    L.orc_nmi.argtypes = [C.c_int, IP, IP, C.POINTER(C.c_double)]
+    L.orc_nmi_file.restype = C.c_int
+    L.orc_nmi_file.argtypes = [C.c_int, IP, C.c_char_p, C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -215,3 +219,21 @@ def run(n, m, ind, adj, kind=MOD, lam=0.0, frac=0.5, rounds=1, max_iter=-1, thre
               seed, tie, _ptr(il), cc_override, labels, C.byref(score), C.byref(iters), C.byref(k),
               trace.ctypes.data_as(C.c_void_p), trace_cap)
     return labels[:n], score.value, iters.value, k.value, trace[: min(iters.value, trace_cap)]
+
+
+def nmi(a, b):
+    """compare_communities(g, a, b) with g.n = len(a); None on negative labels."""
+    a, b = i32(a), i32(b)
+    out = C.c_double()
+    if lib().orc_nmi(len(a), a, b, C.byref(out)) != 0:
+        return None
+    return out.value
+
+
+def nmi_file(n, labels, path):
+    """compare_communities(g, labels, path); None on a bad file."""
+    lab = i32(labels)
+    out = C.c_double()
+    if lib().orc_nmi_file(int(n), lab, path.encode(), C.byref(out)) != 0:
+        return None
+    return out.value

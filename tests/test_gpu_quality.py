@@ -8,7 +8,10 @@ mode (label-exact with the reference binary at -p 1), reset with srand(s)
 for seeds s = 1..SEEDS; the GPU side is pcd_refine with seeds 1..SEEDS.
 Final scores: the means must agree within 0.5 % of the reference mean, or
 within three standard errors of the difference where the runs' own spread
-is larger than that (label propagation alone, -s 0, spreads by ~2 %)."""
+is larger than that (label propagation alone, -s 0, spreads by ~2 %).
+Agreement with the planted blocks, as normalized mutual information
+(pcd_nmi, pinned to the reference's compare_communities), is held to the
+same bar."""
 import multiprocessing as mp
 import os
 
@@ -27,7 +30,23 @@ CASES = [("modularity", O.MOD, 0.0, 0), ("cpm-0.01", O.CPM, 0.01, 0), ("wcc -i 1
 
 def _ref(a):
     n, m, ind, adj, kind, lam, initial, strategy, seed = a
-    return O.run(n, m, ind, adj, kind=kind, lam=lam, initial=initial, strategy=strategy, mode=0, seed=seed)[1]
+    r = O.run(n, m, ind, adj, kind=kind, lam=lam, initial=initial, strategy=strategy, mode=0, seed=seed)
+    return r[1], r[0]
+
+
+def planted(g):
+    """Block of each vertex: SBM-1's generator numbers block b's vertices
+    100 b .. 100 b + 99, and the PCD map holds each vertex's generator id."""
+    return (np.asarray(g[4]) // 100).astype(np.int32)
+
+
+def same_bar(name, ours, ref):
+    diff = ours.mean() - ref.mean()
+    se = np.sqrt(ours.var(ddof=1) / SEEDS + ref.var(ddof=1) / SEEDS)
+    print("%s: gpu %.6f +- %.6f, reference %.6f +- %.6f, rel diff %+.3f %%, diff/se %+.2f" % (
+        name, ours.mean(), ours.std(), ref.mean(), ref.std(), 100 * diff / abs(ref.mean()),
+        diff / se if se > 0 else 0.0))
+    assert abs(diff) <= max(0.005 * abs(ref.mean()), 3 * se)
 
 
 @pytest.fixture(scope="module")
@@ -41,8 +60,11 @@ def reference_scores(sbm1):
     work = [(n, m, ind, adj, kind, lam, ini, st, sd) for _, kind, lam, ini in CASES for st in (0, 2)
             for sd in range(1, SEEDS + 1)]
     with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
-        scores = pool.map(_ref, work)
-    return np.array(scores).reshape(len(CASES), 2, SEEDS)
+        res = pool.map(_ref, work)
+    truth = planted(sbm1)
+    scores = np.array([r[0] for r in res]).reshape(len(CASES), 2, SEEDS)
+    nmis = np.array([parcd.nmi(r[1], truth) for r in res]).reshape(len(CASES), 2, SEEDS)
+    return scores, nmis
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))
@@ -50,7 +72,8 @@ def reference_scores(sbm1):
 def test_quality_matches_reference(sbm1, reference_scores, case, strategy):
     name, kind, lam, ini = CASES[case]
     n, m, ind, adj, _ = sbm1
-    ours = []
+    ours, ours_nmi = [], []
+    truth = planted(sbm1)
     with parcd.Context(0) as ctx:
         ctx.upload_graph(n, m, ind, adj)
         tri = ctx.triangles()[0]
@@ -58,10 +81,8 @@ def test_quality_matches_reference(sbm1, reference_scores, case, strategy):
             lab = parcd.clustering_partition(n, ind, adj, tri) if ini else singleton_labels(n)
             ctx.upload_labels(lab)
             ours.append(ctx.refine(parcd.job(kind=kind, lam=lam, seed=sd), strategy=strategy)[0])
-    ours = np.array(ours)
-    ref = reference_scores[case, 0 if strategy == 0 else 1]
-    diff = ours.mean() - ref.mean()
-    se = np.sqrt(ours.var(ddof=1) / SEEDS + ref.var(ddof=1) / SEEDS)
-    print("%s -s %d: gpu %.6f +- %.6f, reference %.6f +- %.6f, rel diff %+.3f %%, diff/se %+.2f" % (
-        name, strategy, ours.mean(), ours.std(), ref.mean(), ref.std(), 100 * diff / abs(ref.mean()), diff / se))
-    assert abs(diff) <= max(0.005 * abs(ref.mean()), 3 * se)
+            ours_nmi.append(parcd.nmi(ctx.labels(), truth))
+    scores, nmis = reference_scores
+    side = 0 if strategy == 0 else 1
+    same_bar("%s -s %d score" % (name, strategy), np.array(ours), scores[case, side])
+    same_bar("%s -s %d NMI vs planted" % (name, strategy), np.array(ours_nmi), nmis[case, side])

@@ -130,3 +130,33 @@ def test_graft_entry_imports():
     m = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(m)
     assert callable(m.build) and callable(m.smoke)
+
+
+@pytest.mark.parametrize("path", golden("nmi_*.npz"))
+def test_nmi_matches_reference(path, tmp_path):
+    """pcd_nmi / pcd_nmi_file (host code of the library) against the compiled
+    reference's compare_communities, NaN and int32-wrap cases included."""
+    d = load(path)
+    want = float(d["nmi"][0])
+    if "truth" in d:
+        t = tmp_path / "truth.txt"
+        t.write_bytes(d["truth"].tobytes())
+        got = parcd.nmi_file(d["labels"], str(t))
+    else:
+        got = parcd.nmi(d["a"], d["b"])
+    assert (np.isnan(got) and np.isnan(want)) or got == pytest.approx(want, rel=1e-13, abs=1e-15), (got, want)
+
+
+def test_nmi_random_and_errors(tmp_path):
+    rng = np.random.default_rng(21)
+    for n, ka, kb in [(10, 2, 3), (5000, 50, 8), (20000, 3000, 3000)]:
+        a, b = rng.integers(0, ka, n), rng.integers(0, kb, n)
+        assert parcd.nmi(a, b) == pytest.approx(O.nmi(a, b), rel=1e-13)
+    with pytest.raises(parcd.PcdError):
+        parcd.nmi(np.array([0, -1]), np.array([0, 0]))
+    p = tmp_path / "t.txt"
+    p.write_text("0 1\n2 9\n")
+    with pytest.raises(parcd.PcdError):  # vertex outside [0, n)
+        parcd.nmi_file(np.zeros(5, np.int32), str(p))
+    with pytest.raises(parcd.PcdError):
+        parcd.nmi_file(np.zeros(5, np.int32), str(tmp_path / "missing.txt"))

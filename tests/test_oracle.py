@@ -386,3 +386,57 @@ def test_full_run_matches_reference_p1(path, golden_graphs):
     assert np.array_equal(labels, d["labels"])
     assert score == pytest.approx(float(d["score"]), rel=1e-5)
     assert np.allclose(trace, d["trace"], rtol=1e-5, atol=1e-6)
+
+
+# ------------------------------------------------------------------ NMI
+def _same_float(x, y):
+    return (np.isnan(x) and np.isnan(y)) or x == pytest.approx(y, rel=1e-13, abs=1e-15)
+
+
+@pytest.mark.parametrize("path", golden("nmi_*.npz"))
+def test_nmi_matches_reference(path, tmp_path):
+    """compare_communities of the compiled reference (common.cpp:75-176),
+    including the int32 wrap of ni * nj (values above 1, and NaN)."""
+    d = load(path)
+    want = float(d["nmi"][0])
+    if "truth" in d:
+        t = tmp_path / "truth.txt"
+        t.write_bytes(d["truth"].tobytes())
+        got = O.nmi_file(int(d["n"]), d["labels"], str(t))
+    else:
+        got = O.nmi(d["a"], d["b"])
+    assert got is not None and _same_float(got, want), (got, want)
+
+
+def test_nmi_against_sklearn_and_invariants():
+    sk = pytest.importorskip("sklearn.metrics")
+    rng = np.random.default_rng(11)
+    for n, ka, kb in [(50, 3, 4), (1000, 10, 25), (3000, 200, 7), (2000, 2000, 40)]:
+        a, b = rng.integers(0, ka, n), rng.integers(0, kb, n)
+        got = O.nmi(a, b)
+        assert got == pytest.approx(sk.normalized_mutual_info_score(a, b), rel=1e-12, abs=1e-14)
+        # symmetric, invariant under renaming communities, 1 against itself
+        assert got == pytest.approx(O.nmi(b, a), rel=1e-12)
+        perm = rng.permutation(ka) * 3 + 1
+        assert got == pytest.approx(O.nmi(perm[a], b), rel=1e-12)
+        assert O.nmi(a, a) == pytest.approx(1.0, rel=1e-12)
+    # a planted partition recovered up to names: 1; one community on either side: 0
+    a = np.repeat(np.arange(10), 7)
+    assert O.nmi(a, (a * 13 + 5) % 97) == pytest.approx(1.0, rel=1e-12)
+    assert O.nmi(np.zeros(70, np.int32), a) == 0.0
+    assert O.nmi(np.full(5, -1), np.zeros(5)) is None
+
+
+def test_nmi_file_matches_pairs(tmp_path):
+    """A ground-truth file covering every vertex once equals the two-labeling
+    form with the file's line numbers (from 1) as labels."""
+    rng = np.random.default_rng(12)
+    n = 300
+    truth = rng.integers(0, 9, n)
+    lab = rng.integers(0, 14, n)
+    lines = ["# generated", ""] + [" ".join(str(v) for v in np.nonzero(truth == c)[0]) for c in range(9)]
+    p = tmp_path / "t.txt"
+    p.write_text("\n".join(lines) + "\n")
+    assert O.nmi_file(n, lab, str(p)) == pytest.approx(O.nmi(truth + 1, lab), rel=1e-13)
+    p.write_text("1 2 %d\n" % n)
+    assert O.nmi_file(n, lab, str(p)) is None
