@@ -327,8 +327,14 @@ int pcd_singleton_partition(int n, int32_t *labels);
 int pcd_gen_requests(uint64_t seed, int count, int64_t m_lo, int64_t m_hi, int32_t **n, int32_t **m,
                      int64_t **ind_off, int64_t **adj_off, int32_t **indices, int32_t **adj, double **params);
 
+/* framework.cpp:27-59 create_clustering_partition on the device: the
+ * context's labels become the triangle-based initial partition (`-i 1`),
+ * equal to the reference's serial greedy; triangles are counted first if
+ * needed.  *num_communities receives the number of communities. */
+int pcd_clustering_labels(pcd_ctx *ctx, int *num_communities);
+
 /* framework.cpp:27-59 create_clustering_partition from per-vertex triangle
- * counts tri (as pcd_count_triangles returns them). */
+ * counts tri (as pcd_count_triangles returns them), on the host. */
 int pcd_clustering_partition(int n, const int32_t *indices, const int32_t *adj, const int32_t *tri,
                              int32_t *labels);
 

@@ -436,6 +436,26 @@ extern "C" int pcd_labels_upload(pcd_ctx *ctx, const int32_t *labels) {
     return PCD_OK;
 }
 
+// framework.cpp:27-59 create_clustering_partition, on the device
+extern "C" int pcd_clustering_labels(pcd_ctx *ctx, int *num_communities) {
+    PCD_TRY(check_graph(ctx));
+    if (!ctx->has_triangles) {
+        phase_begin(ctx, PCD_PHASE_TRIANGLES);
+        PCD_TRY(triangles_dev(ctx));
+        phase_end(ctx, PCD_PHASE_TRIANGLES);
+    }
+    int nc = 0;
+    PCD_TRY(clustering_labels_dev(ctx, &nc));
+    ctx->label_bound = nc;
+    ctx->has_labels = true;
+    ctx->labels_compressed = false;
+    ctx->labels_have_new = false;
+    ctx->has_stats = false;
+    ctx->stats_pending = false;
+    if (num_communities) *num_communities = nc;
+    return PCD_OK;
+}
+
 extern "C" int pcd_labels_download(pcd_ctx *ctx, int32_t *labels) {
     PCD_TRY(check_labels(ctx));
     PCD_HIP(hipMemcpyAsync(labels, ctx->d_labels, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
@@ -640,6 +660,9 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.stats_cache = value != 0;
     } else if (k == "inc_stats") {
         ctx->tune.inc_stats = value != 0;
+    } else if (k == "cluster_rounds") {
+        if (value < 0) return fail(PCD_E_INVALID, "cluster_rounds must be >= 0");
+        ctx->tune.cluster_rounds = value;
     } else if (k == "stat_chunk") {
         if (value < 64) return fail(PCD_E_INVALID, "stat_chunk must be >= 64");
         ctx->tune.stat_chunk = value;  // applies to the next graph upload

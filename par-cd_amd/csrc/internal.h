@@ -54,6 +54,8 @@ struct Tuning {
                                  // STATS_CACHE vertices (the highest degrees) from LDS
     int inc_stats = 1;           // a propagation step updates the statistics from the rows
                                  // of the vertices it moved (else a full recount)
+    int cluster_rounds = 128;    // device rounds of the clustering partition before the
+                                 // remaining dependency chains are finished on the host
 };
 constexpr int STATS_CACHE = 16384;
 constexpr int HUB_MAX_PARTS = 1024;
@@ -207,6 +209,7 @@ int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int b
 int compress_labels_dev(pcd_ctx *ctx);
 int collect_statistics_dev(pcd_ctx *ctx);
 int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *after);
+int clustering_labels_dev(pcd_ctx *ctx, int *num_communities);
 int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out);
 int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie);
 int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie);

@@ -127,8 +127,11 @@ inline void create_singleton_partition(const Graph &g, int32_t *labels) {
 // framework.cpp:27-59: vertices by (-cc, -deg, v), each unlabelled vertex
 // opens a community with its unlabelled neighbours.  Reads g.tri, computed
 // on the device by count_triangles (deg*(deg-1) is an int32 product there).
+// framework.cpp:27-59, computed on the device (triangles included)
 inline void create_clustering_partition(const Graph &g, int32_t *labels) {
-    check(pcd_clustering_partition(g.n, g.indices.data(), g.adj.data(), g.tri.data(), labels));
+    int nc = 0;
+    check(pcd_clustering_labels(g.ctx(), &nc));
+    check(pcd_labels_download(g.ctx(), labels));
 }
 
 inline int compress_labels(const Graph &g, int32_t *labels) {

@@ -9,7 +9,7 @@
 // take go to worker threads that each own a plain context and stream,
 // admitted against their own device-byte budget; their kernels run
 // concurrently with the batch rounds.  Those are WCC, the triangle-based
-// initial partition, degrees above 4096, graphs above the edge limit (their
+// initial partition (built on the device), degrees above 4096, graphs above the edge limit (their
 // iterations would set the length of every round) and graphs larger than
 // the whole arena.
 //
@@ -94,7 +94,6 @@ static void finish(pcd_server *s, Req *r, int err, const std::string &msg) {
 static void plain_worker(pcd_server *s) {
     pcd_ctx *ctx = nullptr;
     int rc0 = pcd_ctx_create(s->cfg.device, &ctx);
-    std::vector<int32_t> tri;
     for (;;) {
         Req *r = nullptr;
         {
@@ -111,15 +110,11 @@ static void plain_worker(pcd_server *s) {
         std::string msg = rc0 != PCD_OK ? std::string(pcd_last_error()) : std::string();
         if (rc == PCD_OK) {
             rc = pcd_graph_upload(ctx, r->n, r->m, r->ind, r->adj);
-            if (rc == PCD_OK && (r->initial == 1 || r->job.rule.kind == PCD_RULE_WCC)) {
-                tri.resize(r->n);
-                rc = pcd_count_triangles(ctx, tri.data(), nullptr, nullptr);
-                if (rc == PCD_OK && r->initial == 1) {
-                    r->labels.resize(r->n);
-                    rc = pcd_clustering_partition(r->n, r->ind, r->adj, tri.data(), r->labels.data());
-                }
-            }
-            if (rc == PCD_OK) rc = pcd_labels_upload(ctx, r->labels.data());
+            // the triangle-based initial partition (-i 1) is built on the
+            // device; WCC counts its triangles inside the first metric call
+            int nc = 0;
+            if (rc == PCD_OK) rc = r->initial == 1 ? pcd_clustering_labels(ctx, &nc)
+                                                   : pcd_labels_upload(ctx, r->labels.data());
             if (rc == PCD_OK) rc = pcd_refine(ctx, &r->job, 0, 1, &r->score, &r->iters, &r->k);
             if (rc == PCD_OK) {
                 r->labels.resize(r->n);

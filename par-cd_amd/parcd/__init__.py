@@ -28,7 +28,7 @@ EXPORTS = [
     "pcd_bin_count", "pcd_bin_name", "pcd_bin_counters_reset", "pcd_bin_counters", "pcd_bin_size",
     "pcd_singleton_partition", "pcd_set_tuning", "pcd_hub_overflows", "pcd_copy_probe",
     "pcd_batch_create", "pcd_batch_destroy", "pcd_batch_footprint", "pcd_batch_add", "pcd_batch_add_many",
-    "pcd_batch_labels_many", "pcd_batch_step", "pcd_nmi", "pcd_nmi_file",
+    "pcd_batch_labels_many", "pcd_batch_step", "pcd_nmi", "pcd_nmi_file", "pcd_clustering_labels",
     "pcd_batch_status", "pcd_batch_finished", "pcd_batch_labels", "pcd_batch_stats", "pcd_batch_release",
     "pcd_batch_info", "pcd_batch_map_count", "pcd_batch_map_name", "pcd_batch_set_map", "pcd_batch_autotune",
     "pcd_gen_sbm_batch", "pcd_refine", "pcd_host_alloc", "pcd_host_free", "pcd_clustering_partition",
@@ -139,6 +139,7 @@ def lib():
     L.pcd_batch_add_many.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     L.pcd_batch_labels_many.argtypes = [vp, C.c_int, vp, vp]
     L.pcd_nmi.argtypes = [C.c_int, vp, vp, dp]
+    L.pcd_clustering_labels.argtypes = [vp, ip]
     L.pcd_nmi_file.argtypes = [C.c_int, vp, C.c_char_p, dp]
     L.pcd_batch_step.argtypes = [vp, C.c_int, ip]
     L.pcd_batch_status.argtypes = [vp, C.c_int, ip, ip, ip, dp]
@@ -412,6 +413,13 @@ class Context:
         out = C.c_double()
         check(lib().pcd_metric(self._ctx, rule(kind, lam), C.byref(out)))
         return out.value
+
+    def clustering(self):
+        """The triangle-based initial partition (-i 1) as the context's labels
+        (pcd_clustering_labels); returns the number of communities."""
+        nc = C.c_int()
+        check(lib().pcd_clustering_labels(self._ctx, C.byref(nc)))
+        return nc.value
 
     def propagate(self, kind, lam=0.0, frac=0.5, seed=0, step=0, tie=TIE_RANDOM):
         check(lib().pcd_propagation_step(self._ctx, rule(kind, lam), frac, seed, step, tie))

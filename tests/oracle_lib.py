@@ -237,3 +237,10 @@ def nmi_file(n, labels, path):
     if lib().orc_nmi_file(int(n), lab, path.encode(), C.byref(out)) != 0:
         return None
     return out.value
+
+
+def clustering_partition(n, ind, adj, tri):
+    """framework.cpp:27-59 create_clustering_partition."""
+    out = np.zeros(n, np.int32)
+    lib().orc_clustering_partition(int(n), i32(ind), i32(adj), i32(tri), out)
+    return out

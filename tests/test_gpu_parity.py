@@ -247,6 +247,54 @@ def test_propagation_wcc(ctx):
             _lp_compare(ctx, g, lab, O.WCC, 0.0, 1.0, seed=3, step=1, tie=tie, cc=cc)
 
 
+def _path(n):
+    a = np.arange(n - 1)
+    return GG.csr_from_edges(a, a + 1)
+
+
+def _star_with_triangles(leaves):
+    """A hub above 46,341 neighbours (the reference's int32 deg * (deg - 1)
+    wraps) whose leaves pair up into triangles, plus a clique."""
+    hub = np.zeros(leaves, np.int64)
+    lv = np.arange(1, leaves + 1)
+    a = [hub, lv[0::2][: leaves // 2]]
+    b = [lv, lv[1::2][: leaves // 2]]
+    i, j = np.triu_indices(12, 1)
+    a.append(i + leaves + 1)
+    b.append(j + leaves + 1)
+    return GG.csr_from_edges(np.concatenate(a), np.concatenate(b))
+
+
+@pytest.mark.parametrize("rounds", [128, 1, 0])
+def test_clustering_partition_on_device(ctx, rounds):
+    """pcd_clustering_labels equals the reference's serial greedy
+    (framework.cpp:27-59, as restated on the host and pinned by the -i 1 run
+    fixtures), also when the rounds stop early and the host finishes the
+    chains (rounds 1 and 0), on a path numbered in order and on a hub whose
+    cc divides by a wrapped int32."""
+    ctx.set_tuning("cluster_rounds", rounds)
+    try:
+        graphs = [GOLDEN[k] for k in sorted(GOLDEN)] + [GG.bin_zoo(), GG.unsorted_copy(GG.bin_zoo()),
+                                                       GG.dense_blocks(), GG.rmat(12), _path(3000),
+                                                       _star_with_triangles(50000)]
+        for g in graphs:
+            n, m, ind, adj = upload(ctx, g)
+            nc = ctx.clustering()
+            got = ctx.labels()
+            tri = ctx.triangles()[0]
+            want = parcd.clustering_partition(n, ind, adj, tri)
+            assert nc == int(want.max()) + 1
+            assert np.array_equal(got, want), (n, np.nonzero(got != want)[0][:10])
+        # the oracle's restatement on one graph as a second check
+        g = GOLDEN["golden_sbm200"]
+        n, m, ind, adj = upload(ctx, g)
+        ctx.clustering()
+        _, tri_o, _, _ = O.count_triangles(n, ind, adj)
+        assert np.array_equal(ctx.labels(), O.clustering_partition(n, ind, adj, tri_o))
+    finally:
+        ctx.set_tuning("cluster_rounds", 128)
+
+
 def test_triangles(ctx):
     cases = list(GOLDEN.items()) + [("zoo", GG.bin_zoo()), ("rmat14", GG.rmat(14)), ("dense", GG.dense_blocks())]
     for name, g in cases:
