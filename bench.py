@@ -555,14 +555,30 @@ def main():
     log("bins:", {k: v for k, v in sizes.items() if v[0]})
 
     # ---------------------------------------------------------- modularity
-    secs, scores = run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup, args.seed)
+    # probe pass (not timed): every kernel slot timed, to find the kernel
+    # with the longest average launch; the timed pass then brackets only
+    # that kernel with events (each timed launch adds two events to the
+    # stream, about 0.15 ms per iteration for all slots together)
+    run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup, args.seed)
     timers = ctx.kernel_timers()
+    probe_avg = {}
+    for k_, (ms_, cnt_) in timers.items():
+        name_ = "lp_hub" if k_ in ("lp_hub_partition", "lp_hub_count") else k_
+        if cnt_ and (name_ in SLOT_OF_BIN.values() or name_ == "stats_items"):
+            probe_avg[name_] = probe_avg.get(name_, 0.0) + ms_ / cnt_
+    dom_probe = max(probe_avg, key=probe_avg.get) if probe_avg else None
+    dom_slots = (["lp_hub_partition", "lp_hub_count"] if dom_probe == "lp_hub" else [dom_probe]) if dom_probe else []
+    ctx.kernel_timer_select(dom_slots)
+    secs, scores = run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup, args.seed)
+    timed_dom = ctx.kernel_timers()
+    ctx.kernel_timer_select(None)
+    log("timed pass: events around %s only" % (", ".join(dom_slots) or "nothing"))
     labels_timed = ctx.labels()
     log("hub partitions overflowed:", ctx.hub_overflows())
     ms_per_step = secs * 1000.0 / args.steps
     gteps = E / (secs / args.steps) / 1e9
     log("mod: %.3f ms/iteration, %.2f GTEPS, Q %s" % (ms_per_step, gteps, ["%.6f" % s for s in scores]))
-    log("kernel timers (ms total over %d steps):" % args.steps,
+    log("kernel timers, probe pass (ms total over %d steps):" % args.steps,
         {k: round(v[0], 3) for k, v in sorted(timers.items(), key=lambda x: -x[1][0])})
 
     # deterministic replay of the same iterations with work counters
@@ -602,6 +618,19 @@ def main():
                                      "gbs": st_bytes / (st_ms / st_launches / 1000.0) / 1e9,
                                      "entries_per_launch": int(deg[big].sum())}
     dom_name = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"]) if per_kernel else None
+    # the roofline kernel's average launch comes from the timed pass
+    if dom_name and dom_name == dom_probe:
+        if dom_name == "lp_hub":
+            t_ms = sum(timed_dom.get(k_, (0.0, 0))[0] for k_ in dom_slots)
+            t_launches = timed_dom.get("lp_hub_partition", (0.0, 0))[1]
+        else:
+            t_ms, t_launches = timed_dom.get(dom_name, (0.0, 0))
+        if t_launches:
+            d_ = per_kernel[dom_name]
+            d_["avg_ms_probe_pass"] = d_["avg_ms"]
+            d_["avg_ms"] = t_ms / t_launches
+            d_["gbs"] = d_["bytes_per_launch"] / (d_["avg_ms"] / 1000.0) / 1e9
+            d_["measured_in"] = "timed pass"
     lp_ms = sum(v[0] for k, v in timers.items() if k.startswith("lp_"))
     tot_a = sum(c[0] for c in counters.values())
     tot_e = sum(c[1] for c in counters.values())
@@ -628,7 +657,9 @@ def main():
                     "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
                     "traffic": int(traffic) if traffic else None,
                     "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if traffic else None,
-                    "bytes_per_launch": int(d["bytes_per_launch"]), "avg_launch_ms": round(d["avg_ms"], 4)}
+                    "bytes_per_launch": int(d["bytes_per_launch"]), "avg_launch_ms": round(d["avg_ms"], 4),
+                    "avg_launch_from": "HIP events around this kernel in the timed pass"
+                    if d.get("measured_in") == "timed pass" else "HIP events of the probe pass"}
 
     # ---------------------------------------------------------- merging (-s 1)
     merge = None
@@ -640,7 +671,7 @@ def main():
     cpm = None
     if not args.no_cpm:
         secs_c, scores_c = run_iterations(ctx, parcd, parcd.RULE_CPM, 0.01, init, args.steps, args.warmup,
-                                          args.seed)
+                                          args.seed, timers=False)
         cpm = {"ms_per_step": secs_c * 1000 / args.steps, "gteps": E / (secs_c / args.steps) / 1e9,
                "cpm_scores": scores_c}
         log("cpm-0.01: %.3f ms/iteration, %.2f GTEPS" % (cpm["ms_per_step"], cpm["gteps"]))

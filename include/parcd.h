@@ -150,6 +150,9 @@ int pcd_kernel_slot_count(void);
 const char *pcd_kernel_slot_name(int slot);
 int pcd_kernel_timer_reset(pcd_ctx *ctx);
 int pcd_kernel_timer(pcd_ctx *ctx, int slot, double *total_ms, int64_t *launches);
+/* Restricts the timers to the slots whose bit is set in mask (all by
+ * default): each timed launch adds two events to the stream. */
+int pcd_kernel_timer_select(pcd_ctx *ctx, uint32_t mask);
 /* Work counters of the propagation step per degree bin (summed since the
  * last reset): active vertices, their edge entries and distinct neighbour
  * labels.  Bins are numbered 0 .. pcd_bin_count()-1 (hub first). */

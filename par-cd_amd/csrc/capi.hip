@@ -58,14 +58,16 @@ static hipEvent_t pool_get(pcd_ctx *ctx) {
     return e;
 }
 
-void kt_begin(pcd_ctx *ctx) {
-    if (!ctx->timing) return;
+// Events around the launches of one timer slot (only the selected slots:
+// each event costs a few microseconds of stream time)
+void kt_begin(pcd_ctx *ctx, int slot) {
+    if (!ctx->timing || !((ctx->timer_mask >> slot) & 1u)) return;
     ctx->kt_open = pool_get(ctx);
     (void)hipEventRecord(ctx->kt_open, ctx->stream);
 }
 
 void kt_end(pcd_ctx *ctx, int slot) {
-    if (!ctx->timing || !ctx->kt_open) return;
+    if (!ctx->timing || !ctx->kt_open || !((ctx->timer_mask >> slot) & 1u)) return;
     hipEvent_t e1 = pool_get(ctx);
     (void)hipEventRecord(e1, ctx->stream);
     ctx->ktimes[slot].push_back(std::make_pair(ctx->kt_open, e1));
@@ -688,6 +690,12 @@ extern "C" int pcd_propagation_counters(pcd_ctx *ctx, int64_t *active, int64_t *
     if (active) *active = a;
     if (entries) *entries = e;
     if (distinct) *distinct = d;
+    return PCD_OK;
+}
+
+extern "C" int pcd_kernel_timer_select(pcd_ctx *ctx, uint32_t mask) {
+    PCD_TRY(check_ctx(ctx));
+    ctx->timer_mask = mask;
     return PCD_OK;
 }
 

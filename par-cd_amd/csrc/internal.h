@@ -168,6 +168,7 @@ struct pcd_ctx {
 
     // measurement
     bool timing = false;
+    uint32_t timer_mask = 0xFFFFFFFFu;  // kernel timer slots that record events
     bool counters = false;
     hipEvent_t ev0[PCD_PHASE_COUNT]{}, ev1[PCD_PHASE_COUNT]{};
     bool ev_valid[PCD_PHASE_COUNT]{};
@@ -224,7 +225,7 @@ RuleParams make_rule(pcd_ctx *ctx, pcd_rule rule);
 void phase_begin(pcd_ctx *ctx, int phase);
 void phase_end(pcd_ctx *ctx, int phase);
 // kernel timing: bracket one or more launches of a slot
-void kt_begin(pcd_ctx *ctx);
+void kt_begin(pcd_ctx *ctx, int slot);
 void kt_end(pcd_ctx *ctx, int slot);
 
 }  // namespace pcd

@@ -126,7 +126,7 @@ static int build_pairs(pcd_ctx *ctx, int64_t *U_out, int *half_out) {
     }
     const int half = label_bits(ctx->k);
     const int bits = 2 * half;
-    kt_begin(ctx);
+    kt_begin(ctx, KS_MERGE_KEYS);
     if (ctx->num_stat_items > 0)
         hipLaunchKernelGGL(k_pair_keys_items, dim3(grid_for(ctx, ctx->num_stat_items, 4, 16)), dim3(256), 0,
                            ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_adj, ctx->d_labels, bits,
@@ -144,7 +144,7 @@ static int build_pairs(pcd_ctx *ctx, int64_t *U_out, int *half_out) {
     kt_end(ctx, KS_MERGE_KEYS);
     PCD_HIP(hipGetLastError());
     uint64_t *sorted = nullptr;
-    kt_begin(ctx);
+    kt_begin(ctx, KS_MERGE_SORT);
     PCD_TRY(radix_sort_u64(ctx, ctx->d_keys0, ctx->d_keys1, E, bits, &sorted));
     kt_end(ctx, KS_MERGE_SORT);
     PCD_HIP(hipMemcpyAsync(ctx->h_small64, acc, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -152,7 +152,7 @@ static int build_pairs(pcd_ctx *ctx, int64_t *U_out, int *half_out) {
     const int64_t P = E - ctx->h_small64[0];
     uint64_t *ukeys = sorted == ctx->d_keys0 ? ctx->d_keys1 : ctx->d_keys0;
     int32_t *total = nullptr;
-    kt_begin(ctx);
+    kt_begin(ctx, KS_MERGE_RLE);
     PCD_TRY(device_scan(ctx, P, HeadFlag{sorted}, HeadStore{sorted, ukeys, ctx->d_pair_head}, &total));
     hipLaunchKernelGGL(k_run_lengths, dim3(grid_for(ctx, P, 256)), dim3(256), 0, ctx->stream, total,
                        ctx->d_pair_head, (int)P, ctx->d_pair_links);
@@ -362,7 +362,7 @@ int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step
     a.best_key = (unsigned long long *)ctx->d_best_key;
     int32_t *unmatched = ctx->d_counts + NUM_BINS;
     for (int rnd = 0; rnd < rounds; rnd++) {
-        kt_begin(ctx);
+        kt_begin(ctx, KS_MERGE_ROUNDS);
         a.seed = merge_seed(seed, rnd);
         if (U > 0) {
             const int64_t nslots = 2 * ((U + WAVE - 1) / WAVE);

@@ -56,7 +56,7 @@ __global__ void k_copy_i32(int32_t *dst, const int32_t *src) { *dst = *src; }
 
 int compress_labels_dev(pcd_ctx *ctx) {
     const int n = ctx->n;
-    kt_begin(ctx);
+    kt_begin(ctx, KS_COMPRESS);
     if (ctx->label_bound > 0) PCD_TRY(fill_i32(ctx, ctx->d_first, INT32_MAX, ctx->label_bound));
     hipLaunchKernelGGL(k_first_occurrence, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_labels,
                        ctx->d_first);
@@ -200,7 +200,7 @@ int collect_statistics_dev(pcd_ctx *ctx) {
     hipLaunchKernelGGL(k_clear_stats, dim3(grid_for(ctx, ctx->k, 256)), dim3(256), 0, ctx->stream, ctx->d_stats,
                        ctx->d_k);
     if (ctx->num_stat_items > 0) {
-        kt_begin(ctx);
+        kt_begin(ctx, KS_STATS_ITEMS);
         if (ctx->tune.stats_cache)
             hipLaunchKernelGGL(k_stats_items_cached<1024>, dim3(std::min(ctx->cu_count * 2, ctx->num_stat_items)),
                                dim3(1024), 0, ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_adj,
@@ -211,7 +211,7 @@ int collect_statistics_dev(pcd_ctx *ctx) {
                                ctx->d_stats);
         kt_end(ctx, KS_STATS_ITEMS);
     }
-    kt_begin(ctx);
+    kt_begin(ctx, KS_STATS_SMALL);
     launch_stats_small<64>(ctx, b.start[BIN_G64], b.start[BIN_G64 + 1]);
     launch_stats_small<32>(ctx, b.start[BIN_G32], b.start[BIN_G32 + 1]);
     launch_stats_small<16>(ctx, b.start[BIN_G16], b.start[BIN_G16 + 1]);
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(256) void k_delta_items(const int32_t *list_all, co
 
 int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *after) {
     const BinLayout &b = ctx->bins;
-    kt_begin(ctx);
+    kt_begin(ctx, KS_STATS_DELTA);
     MovedArgs m;
     for (int i = 0; i < 6; i++) m.s[i] = b.start[BIN_G64 + i];
     m.nitems = ctx->num_stat_items;
@@ -496,7 +496,7 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *part, int np
 
 int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out) {
     if (rule.kind == PCD_RULE_WCC) return wcc_metric_dev(ctx, out);
-    kt_begin(ctx);
+    kt_begin(ctx, KS_METRIC);
     if (rule.kind == PCD_RULE_CONDUCTANCE) {
         hipLaunchKernelGGL(k_cond_partial, dim3(FSUM_BLOCKS), dim3(256), 0, ctx->stream, ctx->d_stats, ctx->d_k,
                            ctx->d_dacc);

@@ -821,7 +821,7 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
     const bool small_aux = a.use_aux && ctx->tune.small_aux;
     const bool block_aux = a.use_aux && ctx->tune.block_aux;
     if (a.use_aux && (ctx->num_hubs > 0 || small_aux || block_aux)) {
-        kt_begin(ctx);
+        kt_begin(ctx, KS_LP_SELECT);
         hipLaunchKernelGGL(k_lp_records, dim3(grid_for(ctx, ctx->n, 256, 8)), dim3(256), 0, ctx->stream, ctx->n,
                            ctx->d_labels, ctx->d_stats, a.rule.kind, ctx->d_lv);
         kt_end(ctx, KS_LP_SELECT);
@@ -845,11 +845,11 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
         // the LDS table holds HUB_TB slots: never index past it
         h.table_slots = std::min(HUB_TB, std::max(64, ctx->tune.hub_table_slots & ~63));
         LPArgs ah = args_for(BIN_HUB);
-        kt_begin(ctx);
+        kt_begin(ctx, KS_LP_HUB_INSERT);
         hipLaunchKernelGGL(k_hub_partition, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0, ctx->stream, ah,
                            h);
         kt_end(ctx, KS_LP_HUB_INSERT);
-        kt_begin(ctx);
+        kt_begin(ctx, KS_LP_HUB_DECIDE);
         hipLaunchKernelGGL(k_hub_count<KM>, dim3(grid_for(ctx, h.nparts, 1, 3)), dim3(HUB2_NT), 0, ctx->stream, ah, h);
         hipLaunchKernelGGL(k_hub_final<KM>, dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0, ctx->stream, ah, h);
         hipLaunchKernelGGL(k_hub_global_insert, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(512), 0, ctx->stream, ah,
@@ -859,14 +859,14 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
         kt_end(ctx, KS_LP_HUB_DECIDE);
     }
     if (range(BIN_B4096, p0, p1)) {
-        kt_begin(ctx);
+        kt_begin(ctx, KS_LP_B4096);
         // (a 12-byte-slot table of 8192 entries would leave one workgroup per CU)
         hipLaunchKernelGGL((k_lp_block<512, 8192, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0,
                            ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B4096)});
         kt_end(ctx, KS_LP_B4096);
     }
     if (range(BIN_B1024, p0, p1)) {
-        kt_begin(ctx);
+        kt_begin(ctx, KS_LP_B1024);
         if (block_aux)
             hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - p0, 1, 8)),
                                dim3(256), 0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
@@ -876,7 +876,7 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
         kt_end(ctx, KS_LP_B1024);
     }
     if (range(BIN_W256, p0, p1)) {
-        kt_begin(ctx);
+        kt_begin(ctx, KS_LP_W256);
         if (block_aux)
             hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256),
                                0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_W256)});
@@ -887,7 +887,7 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
     }
 #define PCD_SMALL(G, CP, BIN, SLOT)                                                                            \
     if (range(BIN, p0, p1)) {                                                                                  \
-        kt_begin(ctx);                                                                                         \
+        kt_begin(ctx, SLOT);                                                                                         \
         if (small_aux)                                                                                         \
             hipLaunchKernelGGL((k_lp_small<G, CP, true, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)),  \
                                dim3(256), 0, ctx->stream, RangeSrc{p0, p1, CP, args_for(BIN)});                \

@@ -282,7 +282,7 @@ static int ensure_tri_buffers(pcd_ctx *ctx) {
 static int count_pass(pcd_ctx *ctx, const int32_t *labels, int32_t *t, int32_t *td) {
     const int n = ctx->n;
     const int64_t E = (int64_t)2 * ctx->m;
-    kt_begin(ctx);
+    kt_begin(ctx, KS_TRIANGLES);
     PCD_TRY(fill_i32(ctx, ctx->d_et, 0, E));
     PCD_TRY(fill_i32(ctx, t, 0, n));
     PCD_TRY(fill_i32(ctx, td, 0, n));

@@ -29,6 +29,7 @@ EXPORTS = [
     "pcd_singleton_partition", "pcd_set_tuning", "pcd_hub_overflows", "pcd_copy_probe",
     "pcd_batch_create", "pcd_batch_destroy", "pcd_batch_footprint", "pcd_batch_add", "pcd_batch_add_many",
     "pcd_batch_labels_many", "pcd_batch_step", "pcd_nmi", "pcd_nmi_file", "pcd_clustering_labels",
+    "pcd_kernel_timer_select",
     "pcd_batch_status", "pcd_batch_finished", "pcd_batch_labels", "pcd_batch_stats", "pcd_batch_release",
     "pcd_batch_info", "pcd_batch_map_count", "pcd_batch_map_name", "pcd_batch_set_map", "pcd_batch_autotune",
     "pcd_gen_sbm_batch", "pcd_refine", "pcd_host_alloc", "pcd_host_free", "pcd_clustering_partition",
@@ -140,6 +141,7 @@ def lib():
     L.pcd_batch_labels_many.argtypes = [vp, C.c_int, vp, vp]
     L.pcd_nmi.argtypes = [C.c_int, vp, vp, dp]
     L.pcd_clustering_labels.argtypes = [vp, ip]
+    L.pcd_kernel_timer_select.argtypes = [vp, C.c_uint32]
     L.pcd_nmi_file.argtypes = [C.c_int, vp, C.c_char_p, dp]
     L.pcd_batch_step.argtypes = [vp, C.c_int, ip]
     L.pcd_batch_status.argtypes = [vp, C.c_int, ip, ip, ip, dp]
@@ -506,6 +508,17 @@ class Context:
         c = C.c_int64()
         check(lib().pcd_hub_overflows(self._ctx, C.byref(c)))
         return c.value
+
+    def kernel_timer_select(self, names=None):
+        """Time only the named kernel slots (None: all of them)."""
+        if names is None:
+            mask = 0xFFFFFFFF
+        else:
+            slots = kernel_slots()
+            mask = 0
+            for nm in names:
+                mask |= 1 << slots.index(nm)
+        check(lib().pcd_kernel_timer_select(self._ctx, mask))
 
     def kernel_timer_reset(self):
         check(lib().pcd_kernel_timer_reset(self._ctx))
