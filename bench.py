@@ -357,10 +357,12 @@ def batch_leg(parcd, np, graphs_count, seed=4):
     # (a) batched
     t0 = time.perf_counter()
     b = parcd.Batch(0, max_graphs=len(gs), arena_bytes=arena)
+    t_create = time.perf_counter()
     slots = b.add_many([g[:4] for g in gs], labs, jobs)
     assert all(s_ is not None for s_ in slots)
     t_adm = time.perf_counter()
     running = b.step(1)
+    t_first = time.perf_counter()
     chosen, map_ms = b.autotune(reps=2)
     t_rounds0 = time.perf_counter()
     while running:
@@ -368,6 +370,8 @@ def batch_leg(parcd, np, graphs_count, seed=4):
     t_rounds1 = time.perf_counter()
     res_b = list(zip(b.labels_many(slots), [b.status(s_) for s_ in slots]))
     t1 = time.perf_counter()
+    phases = {"create": t_create - t0, "admission": t_adm - t_create, "first_round": t_first - t_adm, "autotune": t_rounds0 - t_first,
+              "rounds_after_tuning": t_rounds1 - t_rounds0, "download": t1 - t_rounds1}
     rounds = b.info()["rounds"]
     b.close()
     log("batched: %.3f s (admission %.3f s, %d rounds, rounds after tuning %.3f s), bin map %s" % (
@@ -398,6 +402,7 @@ def batch_leg(parcd, np, graphs_count, seed=4):
         "speedup_vs_plain": round((t3 - t2) / (t1 - t0), 2), "batched_s": round(t1 - t0, 3),
         "plain_s": round(t3 - t2, 3), "admission_s": round(t_adm - t0, 3), "rounds": int(rounds),
         "rounds_after_tuning_s": round(t_rounds1 - t_rounds0, 3),
+        "phases_s": {k_: round(v_, 4) for k_, v_ in phases.items()},
         "gteps_rounds": round(sum(2 * g[1] * r[2] for g, r in zip(gs, res_p)) / (t_rounds1 - t_rounds0) / 1e9, 2),
         "bin_map": parcd.batch_map_names()[chosen], "autotune_ms": [round(x, 3) for x in map_ms],
         "identical_results": same, "mean_modularity_batched": float(np.mean(mod_b)) if mod_b else None,

@@ -11,7 +11,8 @@
 //                    statistics clearing, metric sums)
 //   small rows       up to CP positions of one graph's degree class <= G
 //                    (statistics with G lanes per vertex)
-//   long rows        <= 2048 entries of one row of degree > 64 (statistics)
+//   long rows        up to ROWS_PER positions of one graph's rows of
+//                    degree > 64 (statistics)
 //   propagation      per kernel variant, position ranges of one graph's
 //                    degree range (the bin map chooses which variant serves
 //                    which degree range)
@@ -37,7 +38,8 @@
 namespace pcd {
 
 constexpr int VT = 2048;  // vertices per tile item
-constexpr int ROW_CHUNK = 2048;
+constexpr int ROWS_PER = 64;  // row positions per long-row item (one per lane of the update kernel)
+static_assert(ROWS_PER == WAVE, "k_b_delta_rows checks one row per lane");
 constexpr int STAT_G[5] = {64, 32, 16, 8, 4};
 constexpr int STAT_CP[5] = {8, 16, 32, 64, 64};
 
@@ -234,8 +236,9 @@ __global__ __launch_bounds__(256) void k_b_stats_small(const WorkItem *items, co
     }
 }
 
-// rows of degree > 64: one wave per chunk of <= 2048 entries
-__global__ __launch_bounds__(256) void k_b_stats_rows(const RowItem *rows, const int *count, const BatchGraph *gs,
+// rows of degree > 64 (at most 4096 entries in a batch): one wave per item
+// of up to ROWS_PER row positions, row after row
+__global__ __launch_bounds__(256) void k_b_stats_rows(const WorkItem *items, const int *count, const BatchGraph *gs,
                                                       const BatchState *st) {
     constexpr int PF = 8;
     const int lane = __lane_id();
@@ -243,23 +246,28 @@ __global__ __launch_bounds__(256) void k_b_stats_rows(const RowItem *rows, const
     const int nwaves = gridDim.x * blockDim.x / WAVE;
     const int nu = *count;
     for (int u = wave_g; u < nu; u += nwaves) {
-        const RowItem r = rows[u];
-        if (st[r.g].status != BG_RUNNING || st[r.g].ready) continue;
-        const BatchGraph &g = gs[r.g];
-        const int l = g.cur[r.v];
-        int same = 0;
-        for (int j0 = r.begin + lane; j0 < r.end; j0 += PF * WAVE) {
-            int nb[PF];
+        const WorkItem w = items[u];
+        if (st[w.g].status != BG_RUNNING || st[w.g].ready) continue;
+        const BatchGraph &g = gs[w.g];
+        const LPArgs &a = g.lp;
+        for (int p = w.p0; p < w.p1; p++) {
+            const int v = a.order ? a.order[p] : p;
+            const int l = g.cur[v];
+            const int b = a.indices[v], e = a.indices[v + 1];
+            int same = 0;
+            for (int j0 = b + lane; j0 < e; j0 += PF * WAVE) {
+                int nb[PF];
 #pragma unroll
-            for (int e = 0; e < PF; e++) {
-                int j = j0 + e * WAVE;
-                nb[e] = j < r.end ? g.lp.adj[j] : -1;
+                for (int k = 0; k < PF; k++) {
+                    const int j = j0 + k * WAVE;
+                    nb[k] = j < e ? a.adj[j] : -1;
+                }
+#pragma unroll
+                for (int k = 0; k < PF; k++) same += (nb[k] >= 0 && g.cur[nb[k]] == l) ? 1 : 0;
             }
-#pragma unroll
-            for (int e = 0; e < PF; e++) same += (nb[e] >= 0 && g.cur[nb[e]] == l) ? 1 : 0;
+            same = wave_sum(same);
+            if (lane == 0) add_stats(g.stats, l, 1, same, (e - b) - same);
         }
-        same = wave_sum(same);
-        if (lane == 0) add_stats(g.stats, l, r.begin == g.lp.indices[r.v] ? 1 : 0, same, (r.end - r.begin) - same);
     }
 }
 
@@ -319,66 +327,64 @@ __global__ __launch_bounds__(256) void k_b_delta_small(const WorkItem *items, co
     }
 }
 
-// rows of degree > 64: each wave checks 64 row items at once (lane j takes
-// item wave + (64 r + j) * waves in round r) and runs the moved ones
-__global__ __launch_bounds__(256) void k_b_delta_rows(const RowItem *rows, const int *count, const BatchGraph *gs,
+// rows of degree > 64: one wave per item of up to ROWS_PER (= 64) row
+// positions; lane j checks row p0 + j, then the wave runs the moved rows
+__global__ __launch_bounds__(256) void k_b_delta_rows(const WorkItem *items, const int *count, const BatchGraph *gs,
                                                       const BatchState *st) {
     constexpr int PF = 4;
     const int lane = __lane_id();
     const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
     const int nwaves = gridDim.x * blockDim.x / WAVE;
     const int nu = *count;
-    for (int64_t r = 0; wave_g + r * WAVE * nwaves < nu; r++) {
-        const int64_t i = wave_g + (r * WAVE + lane) * nwaves;
-        RowItem it{0, 0, 0, 0};
-        int lo = 0, ln = 0;
+    for (int u = wave_g; u < nu; u += nwaves) {
+        const WorkItem w = items[u];
+        if (st[w.g].status != BG_RUNNING || !st[w.g].ready) continue;
+        const BatchGraph &g = gs[w.g];
+        const LPArgs &a = g.lp;
+        int v = 0, lo = 0, ln = 0;
         bool moved = false;
-        if (i < nu) {
-            it = rows[i];
-            if (st[it.g].status == BG_RUNNING && st[it.g].ready) {
-                const BatchGraph &g = gs[it.g];
-                lo = g.cur[it.v];
-                ln = g.next[it.v];
-                moved = lo != ln;
-            }
+        if (w.p0 + lane < w.p1) {
+            const int p = w.p0 + lane;
+            v = a.order ? a.order[p] : p;
+            lo = g.cur[v];
+            ln = g.next[v];
+            moved = lo != ln;
         }
         uint64_t mv = __ballot(moved);
         while (mv) {
             const int src = __builtin_ctzll(mv);
             mv &= mv - 1;
-            const int xg = __builtin_amdgcn_readlane(it.g, src), xv = __builtin_amdgcn_readlane(it.v, src);
-            const int xb = __builtin_amdgcn_readlane(it.begin, src), xe = __builtin_amdgcn_readlane(it.end, src);
+            const int xv = __builtin_amdgcn_readlane(v, src);
             const int xlo = __builtin_amdgcn_readlane(lo, src), xln = __builtin_amdgcn_readlane(ln, src);
-            const BatchGraph &g = gs[xg];
+            const int xb = a.indices[xv], xe = a.indices[xv + 1];
             int sa = 0, sb = 0;
             for (int j0 = xb + lane; j0 < xe; j0 += PF * WAVE) {
                 int nb[PF];
 #pragma unroll
-                for (int e = 0; e < PF; e++) {
-                    const int j = j0 + e * WAVE;
-                    nb[e] = j < xe ? g.lp.adj[j] : -1;
+                for (int k = 0; k < PF; k++) {
+                    const int j = j0 + k * WAVE;
+                    nb[k] = j < xe ? a.adj[j] : -1;
                 }
                 int lx[PF], nx[PF];
 #pragma unroll
-                for (int e = 0; e < PF; e++) {
-                    lx[e] = nb[e] < 0 ? -2 : g.cur[nb[e]];
-                    nx[e] = nb[e] < 0 ? -2 : g.next[nb[e]];
+                for (int k = 0; k < PF; k++) {
+                    lx[k] = nb[k] < 0 ? -2 : g.cur[nb[k]];
+                    nx[k] = nb[k] < 0 ? -2 : g.next[nb[k]];
                 }
 #pragma unroll
-                for (int e = 0; e < PF; e++) {
-                    if (nb[e] < 0) continue;
-                    const int f = lx[e] != nx[e] ? 1 : 2;
-                    sa += lx[e] == xlo ? f : 0;
-                    sb += nx[e] == xln ? f : 0;
+                for (int k = 0; k < PF; k++) {
+                    if (nb[k] < 0) continue;
+                    const int f = lx[k] != nx[k] ? 1 : 2;
+                    sa += lx[k] == xlo ? f : 0;
+                    sb += nx[k] == xln ? f : 0;
                 }
             }
             sa = wave_sum(sa);
             sb = wave_sum(sb);
             if (lane == 0) {
                 const int len = xe - xb;
-                const int first = xb == g.lp.indices[xv] ? 1 : 0;
-                add_stats(g.stats, xlo, -first, -sa, sa - len);
-                if (xln >= 0) add_stats(g.stats, xln, first, sb, len - sb);
+                add_stats(g.stats, xlo, -1, -sa, sa - len);
+                if (xln >= 0) add_stats(g.stats, xln, 1, sb, len - sb);
             }
         }
     }
@@ -530,7 +536,6 @@ struct Slot {
     size_t off = 0, bytes = 0;
     int64_t admitted = 0;      // rounds run before admission
     int pos_le[NUM_TH] = {0};  // first position with degree <= TH[i]
-    std::vector<RowItem> rows; // rows of degree > 64 (g set when listed)
     BatchGraph rec;
 };
 
@@ -731,7 +736,6 @@ struct Prepared {
     bool sorted = true;
     int pos_le[NUM_TH] = {0};
     std::vector<int32_t> order;
-    std::vector<RowItem> rows;  // graph index set at admission
 };
 }  // namespace
 
@@ -800,11 +804,6 @@ static void prepare_graph(int n, int m, const int32_t *indices, const int32_t *a
         }
         pr.pos_le[i] = lo;
     }
-    for (int p = 0; p < pr.pos_le[th_index(64)]; p++) {
-        int v = sorted ? p : pr.order[p];
-        for (int j = indices[v]; j < indices[v + 1]; j += ROW_CHUNK)
-            pr.rows.push_back(RowItem{-1, v, j, std::min(indices[v + 1], j + ROW_CHUNK)});
-    }
 }
 
 // Places a prepared graph in the first free slot at or after *hint and
@@ -833,8 +832,6 @@ static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, co
     sl.bytes = bytes;
     sl.admitted = b->rounds;
     std::memcpy(sl.pos_le, pr.pos_le, sizeof(sl.pos_le));
-    sl.rows = std::move(pr.rows);
-    for (RowItem &r : sl.rows) r.g = s;
     // device layout
     char *base = b->arena + off;
     size_t nb = align256(4 * ((size_t)n + 1));
@@ -1015,7 +1012,7 @@ static void slot_counts(const BinMap &mp, const Slot &sl, bool lp_only, int64_t 
             small_range(sl, i, p0, p1);
             cnt[L_SMALL0 + i] += nitems(p0, p1, STAT_CP[i]);
         }
-        cnt[L_ROWS] += (int64_t)sl.rows.size();
+        cnt[L_ROWS] += nitems(0, sl.pos_le[th_index(64)], ROWS_PER);
     }
     for (int i = 0; i < mp.n; i++) {
         lp_range(mp, sl, i, p0, p1);
@@ -1023,7 +1020,7 @@ static void slot_counts(const BinMap &mp, const Slot &sl, bool lp_only, int64_t 
     }
 }
 
-static void slot_fill(const BinMap &mp, const Slot &sl, int s, bool lp_only, WorkItem **wp, RowItem *&rp) {
+static void slot_fill(const BinMap &mp, const Slot &sl, int s, bool lp_only, WorkItem **wp) {
     auto push_range = [&](int l, int p0, int p1, int per) {
         for (int p = p0; p < p1; p += per) *wp[l]++ = WorkItem{s, p, std::min(p1, p + per)};
     };
@@ -1034,10 +1031,7 @@ static void slot_fill(const BinMap &mp, const Slot &sl, int s, bool lp_only, Wor
             small_range(sl, i, p0, p1);
             push_range(L_SMALL0 + i, p0, p1, STAT_CP[i]);
         }
-        if (!sl.rows.empty()) {
-            std::memcpy(rp, sl.rows.data(), sl.rows.size() * sizeof(RowItem));
-            rp += sl.rows.size();
-        }
+        push_range(L_ROWS, 0, sl.pos_le[th_index(64)], ROWS_PER);
     }
     for (int i = 0; i < mp.n; i++) {
         lp_range(mp, sl, i, p0, p1);
@@ -1045,7 +1039,7 @@ static void slot_fill(const BinMap &mp, const Slot &sl, int s, bool lp_only, Wor
     }
 }
 
-static size_t list_esz(int l) { return l == L_ROWS ? sizeof(RowItem) : sizeof(WorkItem); }
+static size_t list_esz(int) { return sizeof(WorkItem); }
 
 // Rebuilds the work-item lists from the running graphs and uploads them:
 // one counting pass fixes the layout (every list with room to grow, so
@@ -1072,7 +1066,7 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
     }
     // layout with growth room
     const int first_list = lp_only ? L_LP0 : 0;
-    size_t off = lp_only ? align256(b->list_off[L_ROWS] + std::max<int64_t>(b->list_cap[L_ROWS], 1) * sizeof(RowItem))
+    size_t off = lp_only ? align256(b->list_off[L_ROWS] + std::max<int64_t>(b->list_cap[L_ROWS], 1) * sizeof(WorkItem))
                          : 0;
     size_t list_off[NUM_LISTS];
     int64_t list_cap[NUM_LISTS];
@@ -1121,7 +1115,6 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
     char *hb = (char *)b->h_items;
     WorkItem *wp[NUM_LISTS];
     for (int l = first_list; l < NUM_LISTS; l++) wp[l] = (WorkItem *)(hb + b->list_off[l]);
-    RowItem *rp = lp_only ? nullptr : (RowItem *)(hb + b->list_off[L_ROWS]);
     int ntile = 0;
     for (int s = 0; s < b->cap; s++) {
         const Slot &sl = b->slots[s];
@@ -1135,7 +1128,7 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
             b->h_gtiles[s] = make_int2(ntile, nt);
             ntile += nt;
         }
-        slot_fill(mp, sl, s, lp_only, wp, rp);
+        slot_fill(mp, sl, s, lp_only, wp);
     }
     const size_t up0 = lp_only ? b->list_off[L_LP0] : 0;
     PCD_HIP(hipMemcpyAsync((char *)b->d_items + up0, hb + up0, off - up0, hipMemcpyHostToDevice, b->stream));
@@ -1167,14 +1160,13 @@ static int append_items(pcd_batch *b) {
     char *hb = (char *)b->h_items;
     WorkItem *wp[NUM_LISTS];
     for (int l = 0; l < NUM_LISTS; l++) wp[l] = (WorkItem *)(hb + b->list_off[l] + b->list_len[l] * list_esz(l));
-    RowItem *rp = (RowItem *)(hb + b->list_off[L_ROWS] + (size_t)b->list_len[L_ROWS] * sizeof(RowItem));
     int ntile = b->list_len[L_TILES];
     for (int s : b->pending_append) {
         const Slot &sl = b->slots[s];
         const int nt = (sl.n + VT - 1) / VT;
         b->h_gtiles[s] = make_int2(ntile, nt);
         ntile += nt;
-        slot_fill(mp, sl, s, false, wp, rp);
+        slot_fill(mp, sl, s, false, wp);
         b->slot_listed[s] = 1;
         b->listed_vertices += sl.n;
         PCD_HIP(hipMemcpyAsync(b->d_gtiles + s, b->h_gtiles + s, sizeof(int2), hipMemcpyHostToDevice, b->stream));
@@ -1223,7 +1215,7 @@ static int launch_round(pcd_batch *b) {
     BatchState *st = b->d_state;
     // statistics of the last round's propagation step (graphs past their
     // first round)
-    hipLaunchKernelGGL(k_b_delta_rows, dim3(cu * 8), dim3(256), 0, s, list_ptr<RowItem>(b, L_ROWS),
+    hipLaunchKernelGGL(k_b_delta_rows, dim3(cu * 8), dim3(256), 0, s, list_ptr<WorkItem>(b, L_ROWS),
                        b->d_counts + L_ROWS, gs, st);
 #define PCD_BDELTA(I, G)                                                                                    \
     hipLaunchKernelGGL(k_b_delta_small<G>, dim3(cu * 8), dim3(256), 0, s, list_ptr<WorkItem>(b, L_SMALL0 + I), \
@@ -1245,7 +1237,7 @@ static int launch_round(pcd_batch *b) {
     hipLaunchKernelGGL(k_b_stats_copy, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
     // collect_statistics (framework.cpp:69-143) in a graph's first round
     hipLaunchKernelGGL(k_b_clear_stats, dim3(cu * 4), dim3(256), 0, s, tiles, ntile, gs, st);
-    hipLaunchKernelGGL(k_b_stats_rows, dim3(cu * 8), dim3(256), 0, s, list_ptr<RowItem>(b, L_ROWS),
+    hipLaunchKernelGGL(k_b_stats_rows, dim3(cu * 8), dim3(256), 0, s, list_ptr<WorkItem>(b, L_ROWS),
                        b->d_counts + L_ROWS, gs, st);
 #define PCD_BSTATS(I, G)                                                                                    \
     hipLaunchKernelGGL(k_b_stats_small<G>, dim3(cu * 8), dim3(256), 0, s, list_ptr<WorkItem>(b, L_SMALL0 + I), \

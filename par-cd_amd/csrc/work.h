@@ -50,11 +50,6 @@ struct WorkItem {
     int g, p0, p1;
 };
 
-// statistics row item: entries [begin, end) of vertex v of graph g
-struct RowItem {
-    int g, v, begin, end;
-};
-
 enum BatchStatus { BG_RUNNING = 0, BG_DONE = 1, BG_FREE = 2 };
 
 // Static description of one graph of a batch (written at admission).
@@ -116,7 +111,7 @@ enum LPVariant {
 // largest degree each variant handles (its table holds 2 x degree slots)
 constexpr int LPV_MAX_DEG[NUM_LPV] = {4, 8, 16, 32, 64, 256, 512, 1024, 1024, 4096};
 // positions per work item of each variant
-constexpr int LPV_ITEM[NUM_LPV] = {64, 64, 32, 16, 8, 1, 1, 1, 1, 1};
+constexpr int LPV_ITEM[NUM_LPV] = {64, 64, 32, 16, 8, 16, 8, 4, 4, 2};
 
 struct BatchLPLaunch {
     const WorkItem *items[NUM_LPV];
