@@ -94,7 +94,7 @@ SLOT_OF_BIN = {"hub": "lp_hub", "b4096": "lp_b4096", "b1024": "lp_b1024", "w256"
 ROCPROF_NAMES = {
     "stats_items": [r"pcd::k_stats_items(_cached<\d+>)?$"],
     "lp_b4096": [r"pcd::k_lp_block<512, 8192, 1u, pcd::RangeSrc"],
-    "lp_b1024": [r"pcd::k_lp_block<256, 2048, 1u, pcd::RangeSrc"],
+    "lp_b1024": [r"pcd::k_lp_block<256, 2048, 1u, pcd::RangeSrc", r"pcd::k_lp_wave<1024, 1u, pcd::RangeSrc"],
     "lp_w256": [r"pcd::k_lp_wave<512, 1u, pcd::RangeSrc"],
     "lp_hub": [r"pcd::k_hub_partition$", r"pcd::k_hub_count<1u>$"],
     "lp_g64": [r"pcd::k_lp_small<64, 8, false, 1u"], "lp_g32": [r"pcd::k_lp_small<32, 16, false, 1u"],

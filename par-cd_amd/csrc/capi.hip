@@ -304,6 +304,8 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
     ctx->bins.start[0] = 0;
     for (int b = 1; b < NUM_BINS; b++) ctx->bins.start[b] = first_pos_le(BIN_MAX_DEG[b]);
     ctx->bins.start[NUM_BINS] = n;
+    ctx->bins.split512 = first_pos_le(512);
+    ctx->bins.split2048 = first_pos_le(2048);
     for (int b = 0; b < NUM_BINS; b++) {
         int64_t e = 0;
         for (int p = ctx->bins.start[b]; p < ctx->bins.start[b + 1]; p++) e += deg_at(p);
@@ -662,6 +664,10 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.stats_cache = value != 0;
     } else if (k == "inc_stats") {
         ctx->tune.inc_stats = value != 0;
+    } else if (k == "b1024_wave") {
+        ctx->tune.b1024_wave = value != 0;
+    } else if (k == "b4096_split") {
+        ctx->tune.b4096_split = value != 0;
     } else if (k == "cluster_rounds") {
         if (value < 0) return fail(PCD_E_INVALID, "cluster_rounds must be >= 0");
         ctx->tune.cluster_rounds = value;

@@ -40,6 +40,8 @@ constexpr int BIN_MAX_DEG[NUM_BINS] = {0x7fffffff, 4096, 1024, 256, 64, 32, 16, 
 
 struct BinLayout {
     int start[NUM_BINS + 1];  // position ranges, bin b = [start[b], start[b+1])
+    int split512;             // first position of degree <= 512 (inside B1024)
+    int split2048;            // first position of degree <= 2048 (inside B4096)
 };
 
 // Tunables (the autotuner rewrites these per graph shape).
@@ -54,6 +56,10 @@ struct Tuning {
                                  // STATS_CACHE vertices (the highest degrees) from LDS
     int inc_stats = 1;           // a propagation step updates the statistics from the rows
                                  // of the vertices it moved (else a full recount)
+    int b1024_wave = 1;          // degrees 257-512 of B1024 run one wave per vertex with a
+                                 // 1024-slot table (twice the vertices in flight per CU)
+    int b4096_split = 0;         // degrees 1025-2048 of B4096 run 256-thread workgroups with a
+                                 // 4096-slot table (measured slower on R-MAT-22: off)
     int cluster_rounds = 128;    // device rounds of the clustering partition before the
                                  // remaining dependency chains are finished on the host
 };

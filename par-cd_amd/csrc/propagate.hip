@@ -859,20 +859,41 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
         kt_end(ctx, KS_LP_HUB_DECIDE);
     }
     if (range(BIN_B4096, p0, p1)) {
+        // degrees 2049-4096: 512-thread workgroups with 8192-slot tables (two
+        // per CU; a 12-byte-slot table of 8192 entries would leave one);
+        // 1025-2048: 256-thread workgroups with 4096-slot tables (four per CU)
+        const int ps = ctx->tune.b4096_split ? std::min(std::max(ctx->bins.split2048, p0), p1) : p1;
         kt_begin(ctx, KS_LP_B4096);
-        // (a 12-byte-slot table of 8192 entries would leave one workgroup per CU)
-        hipLaunchKernelGGL((k_lp_block<512, 8192, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 2)), dim3(512), 0,
-                           ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B4096)});
+        if (ps > p0)
+            hipLaunchKernelGGL((k_lp_block<512, 8192, KM, RangeSrc>), dim3(grid_for(ctx, ps - p0, 1, 2)), dim3(512),
+                               0, ctx->stream, RangeSrc{p0, ps, 1, args_for(BIN_B4096)});
+        if (p1 > ps)
+            hipLaunchKernelGGL((k_lp_block<256, 4096, KM, RangeSrc>), dim3(grid_for(ctx, p1 - ps, 1, 4)), dim3(256),
+                               0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B4096)});
         kt_end(ctx, KS_LP_B4096);
     }
     if (range(BIN_B1024, p0, p1)) {
+        // degrees 513-1024: one workgroup per vertex (2048-slot table);
+        // 257-512: one wave per vertex with a 1024-slot table, so a CU holds
+        // 16 vertices in flight instead of 8
+        const int ps = ctx->tune.b1024_wave ? std::min(std::max(ctx->bins.split512, p0), p1) : p1;
         kt_begin(ctx, KS_LP_B1024);
-        if (block_aux)
-            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - p0, 1, 8)),
-                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
-        else
-            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256),
-                               0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
+        if (ps > p0) {
+            if (block_aux)
+                hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc, true>), dim3(grid_for(ctx, ps - p0, 1, 8)),
+                                   dim3(256), 0, ctx->stream, RangeSrc{p0, ps, 1, args_for(BIN_B1024)});
+            else
+                hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid_for(ctx, ps - p0, 1, 8)),
+                                   dim3(256), 0, ctx->stream, RangeSrc{p0, ps, 1, args_for(BIN_B1024)});
+        }
+        if (p1 > ps) {
+            if (block_aux)
+                hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - ps, 4, 3)),
+                                   dim3(256), 0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B1024)});
+            else
+                hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc>), dim3(grid_for(ctx, p1 - ps, 4, 4)), dim3(256),
+                                   0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B1024)});
+        }
         kt_end(ctx, KS_LP_B1024);
     }
     if (range(BIN_W256, p0, p1)) {
