@@ -495,6 +495,30 @@ __global__ __launch_bounds__(256) void k_b_pull(const PullPiece *pieces, int cou
     }
 }
 
+// Format check of admitted graphs' rows in the arena (what prepare_graph
+// checks on the host for a single graph): ascending, loop-free, ids in
+// [0, n).  One wave per item of up to 256 rows; bad[slot] = 1 on failure.
+__global__ __launch_bounds__(256) void k_b_check_rows(const WorkItem *items, int count, const BatchGraph *gs,
+                                                      int32_t *bad) {
+    const int lane = __lane_id();
+    const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    for (int u = wave_g; u < count; u += nwaves) {
+        const WorkItem w = items[u];
+        const BatchGraph &g = gs[w.g];
+        const int n = g.n;
+        bool fail = false;
+        for (int v = w.p0; v < w.p1; v++) {
+            const int b = g.lp.indices[v], e = g.lp.indices[v + 1];
+            for (int j = b + lane; j < e; j += WAVE) {
+                const int x = g.lp.adj[j];
+                fail |= x < 0 || x >= n || x == v || (j > b && g.lp.adj[j - 1] >= x);
+            }
+        }
+        if (__ballot(fail) && lane == 0) bad[w.g] = 1;
+    }
+}
+
 // Labels of several finished graphs, graph after graph: out[offs[i] + v] =
 // cur[v] of the graph in slots[i].  One workgroup per graph at a time.
 __global__ void __launch_bounds__(256) k_b_gather_labels(const BatchGraph *graphs, const int32_t *slots,
@@ -596,6 +620,9 @@ struct pcd_batch {
     std::vector<PullPiece> pulls;  // admission copies the device reads from host memory
     PullPiece *d_pulls = nullptr;
     size_t pulls_cap = 0;
+    WorkItem *d_check = nullptr;   // row-check items of pcd_batch_add_many
+    size_t check_cap = 0;
+    int32_t *d_bad = nullptr;      // per slot: rows failed the check
     bool items_dirty = true;
     int64_t listed_done_vertices = 0, listed_vertices = 0;
     // bin map and captured round
@@ -678,6 +705,8 @@ extern "C" int pcd_batch_destroy(pcd_batch *b) {
     if (b->d_tile_pre) (void)hipFree(b->d_tile_pre);
     if (b->d_gather) (void)hipFree(b->d_gather);
     if (b->d_pulls) (void)hipFree(b->d_pulls);
+    if (b->d_check) (void)hipFree(b->d_check);
+    if (b->d_bad) (void)hipFree(b->d_bad);
     if (b->h_state) (void)hipHostFree(b->h_state);
     if (b->h_gtiles) (void)hipHostFree(b->h_gtiles);
     if (b->h_items) (void)hipHostFree(b->h_items);
@@ -765,8 +794,10 @@ struct Prepared {
 };
 }  // namespace
 
+// check_rows false leaves the adjacency rows to the device check of
+// pcd_batch_add_many (k_b_check_rows after the upload).
 static void prepare_graph(int n, int m, const int32_t *indices, const int32_t *adj, const int32_t *labels,
-                          const pcd_job *job, bool parallel, Prepared &pr) {
+                          const pcd_job *job, bool parallel, bool check_rows, Prepared &pr) {
     auto reject = [&](int code, const char *msg) {
         pr.code = code;
         pr.msg = msg;
@@ -797,7 +828,8 @@ static void prepare_graph(int n, int m, const int32_t *indices, const int32_t *a
         return true;
     };
     bool ok = true;
-    if (parallel) {
+    if (!check_rows) {
+    } else if (parallel) {
         std::atomic<int> bad{0};
         parallel_for(n, [&](int64_t lo, int64_t hi) {
             if (!rows_ok(lo, hi)) bad = 1;
@@ -995,7 +1027,7 @@ extern "C" int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices,
     if (!b || !job || !slot_out) return bfail(PCD_E_INVALID, "null argument");
     *slot_out = -1;
     Prepared pr;
-    prepare_graph(n, m, indices, adj, labels, job, true, pr);
+    prepare_graph(n, m, indices, adj, labels, job, true, true, pr);
     if (pr.code != PCD_OK) return bfail(pr.code, pr.msg);
     PCD_HIP(hipSetDevice(b->device));
     int hint = 0, s = -1;
@@ -1034,7 +1066,7 @@ extern "C" int pcd_batch_add_many(pcd_batch *b, int count, const pcd_graph_ref *
         parallel_for(c1 - c0, [&](int64_t lo, int64_t hi) {
             for (int64_t i = lo; i < hi; i++) {
                 const pcd_graph_ref &g = graphs[c0 + i];
-                prepare_graph(g.n, g.m, g.indices, g.adj, g.labels, jobs + c0 + i, false, pr[i]);
+                prepare_graph(g.n, g.m, g.indices, g.adj, g.labels, jobs + c0 + i, false, false, pr[i]);
             }
         }, 1);
         for (int i = c0; i < c1; i++) {
@@ -1062,10 +1094,67 @@ extern "C" int pcd_batch_add_many(pcd_batch *b, int count, const pcd_graph_ref *
         }
         PCD_TRY(flush_pulls(b));  // this chunk's copies run while the next one is checked
     }
-    std::vector<BatchGraph> staging;
-    int rc = upload_records(b, admitted, staging);
-    if (rc != PCD_OK) return rc;
-    PCD_HIP(hipStreamSynchronize(b->stream));
+    // the rows of the admitted graphs, checked on the device after the upload
+    if (!admitted.empty()) {
+        std::vector<WorkItem> items;
+        for (int s : admitted)
+            for (int p = 0; p < b->slots[s].n; p += 256) items.push_back(WorkItem{s, p, std::min(b->slots[s].n, p + 256)});
+        if (items.size() > b->check_cap) {
+            if (b->d_check) PCD_HIP(hipFree(b->d_check));
+            b->d_check = nullptr;
+            b->check_cap = 0;
+            if (hipMalloc((void **)&b->d_check, sizeof(WorkItem) * items.size()) != hipSuccess) {
+                (void)hipGetLastError();
+                return bfail(PCD_E_NOMEM, "row check list");
+            }
+            b->check_cap = items.size();
+        }
+        if (!b->d_bad && hipMalloc((void **)&b->d_bad, sizeof(int32_t) * b->cap) != hipSuccess) {
+            (void)hipGetLastError();
+            return bfail(PCD_E_NOMEM, "row check flags");
+        }
+        std::vector<BatchGraph> staging;
+        PCD_TRY(upload_records(b, admitted, staging));
+        PCD_HIP(hipMemsetAsync(b->d_bad, 0, sizeof(int32_t) * b->cap, b->stream));
+        PCD_HIP(hipMemcpyAsync(b->d_check, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice,
+                               b->stream));
+        hipLaunchKernelGGL(k_b_check_rows, dim3(std::min<size_t>(items.size(), (size_t)b->cu_count * 8)), dim3(256), 0,
+                           b->stream, b->d_check, (int)items.size(), (const BatchGraph *)b->d_graphs, b->d_bad);
+        PCD_HIP(hipGetLastError());
+        std::vector<int32_t> bad((size_t)b->cap);
+        PCD_HIP(hipMemcpyAsync(bad.data(), b->d_bad, sizeof(int32_t) * b->cap, hipMemcpyDeviceToHost, b->stream));
+        PCD_HIP(hipStreamSynchronize(b->stream));
+        std::vector<char> rejected((size_t)b->cap, 0);
+        for (int s : admitted) rejected[s] = bad[s] != 0;
+        // a rejected graph gives its slot and arena space back; its state
+        // record goes back to free
+        std::vector<int> freed;
+        for (int i = 0; i < count; i++) {
+            const int s = slots_out[i];
+            if (s < 0 || !rejected[s]) continue;
+            Slot &sl = b->slots[s];
+            arena_free(b, sl.off, sl.bytes);
+            sl = Slot();
+            std::memset(&b->h_state[s], 0, sizeof(BatchState));
+            b->h_state[s].status = BG_FREE;
+            freed.push_back(s);
+            slots_out[i] = -1;
+            if (codes_out) codes_out[i] = PCD_E_FORMAT;
+            if (first_code == PCD_OK || i < first_index) {
+                first_code = PCD_E_FORMAT;
+                first_index = i;
+                first_msg = "adjacency rows must be ascending, loop-free, with ids in [0,n)";
+            }
+        }
+        if (!freed.empty()) {
+            auto &pa = b->pending_append;
+            pa.erase(std::remove_if(pa.begin(), pa.end(), [&](int s) { return rejected[s] != 0; }), pa.end());
+            for (int s : freed)
+                PCD_HIP(hipMemcpyAsync(b->d_state + s, b->h_state + s, sizeof(BatchState), hipMemcpyHostToDevice,
+                                       b->stream));
+            PCD_HIP(hipStreamSynchronize(b->stream));
+        }
+    }
     if (first_code != PCD_OK) return bfail(first_code, "graph " + std::to_string(first_index) + ": " + first_msg);
     return PCD_OK;
 }

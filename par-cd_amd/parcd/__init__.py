@@ -625,9 +625,10 @@ class Batch:
     def add_many(self, graphs, labels, jobs):
         """Admits graphs[i] = (n, m, indices, adj) with labels[i] and jobs[i]
         in one call (pcd_batch_add_many).  Returns the slots, None where a
-        graph was not admitted for lack of room (PCD_E_NOMEM); the codes are
-        kept in self.last_codes.  Raises PcdError when a graph was refused
-        for another reason (the admitted ones stay admitted)."""
+        graph was not admitted for lack of room (PCD_E_NOMEM); the codes and
+        slots are kept in self.last_codes and self.last_slots.  Raises
+        PcdError when a graph was refused for another reason (the admitted
+        ones stay admitted)."""
         count = len(graphs)
         refs = (GraphRef * max(count, 1))()
         keep = []
@@ -649,6 +650,7 @@ class Batch:
             else:
                 out.append(None)
         self.last_codes = [int(c) for c in codes[:count]]
+        self.last_slots = out
         if any(c not in (0, -3) for c in self.last_codes):
             raise PcdError(msg)
         return out

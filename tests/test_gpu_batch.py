@@ -253,6 +253,46 @@ def test_add_many_rejections_leave_the_rest_admitted():
             assert np.array_equal(l, ref[0])
 
 
+def test_add_many_checks_rows_on_the_device():
+    """pcd_batch_add_many checks the adjacency rows on the device after the
+    upload: a descending row, a self loop and an id out of range are refused
+    with PCD_E_FORMAT, and their slots and arena space come back."""
+    g = sbm(4, 50, 0.3, 0.01, seed=11)
+    lab = singleton_labels(g[0])
+    r0, r1 = int(g[2][5]), int(g[2][6])
+    assert r1 - r0 >= 2
+    broken = []
+    for kind_ in range(3):
+        adj = g[3].copy()
+        if kind_ == 0:
+            adj[r0], adj[r0 + 1] = adj[r0 + 1], adj[r0]
+        elif kind_ == 1:
+            adj[r0] = 5
+        else:
+            adj[r1 - 1] = g[0]
+        broken.append((g[0], g[1], g[2], adj))
+    ref = run_batch([(g, lab, parcd.job())])[0]
+    with parcd.Batch(0, max_graphs=4, arena_bytes=4 << 20) as b:
+        s0 = b.add(g[0], g[1], g[2], g[3], lab, parcd.job())
+        free0 = b.info()["free_bytes"]
+        with pytest.raises(parcd.PcdError):
+            b.add_many(broken, [lab] * 3, [parcd.job()] * 3)
+        assert b.last_codes == [-5, -5, -5]  # PCD_E_FORMAT
+        assert b.info()["free_bytes"] == free0 and b.info()["graphs"] == 1
+        with pytest.raises(parcd.PcdError):
+            b.add_many([g[:4], broken[1], g[:4]], [lab] * 3, [parcd.job()] * 3)
+        ok = b.last_slots
+        assert ok[1] is None and b.last_codes == [0, -5, 0]
+        while b.step(1):
+            pass
+        for s_ in (s0, ok[0], ok[2]):
+            assert np.array_equal(b.labels(s_), ref[0])
+    # the single-graph call still checks on the host
+    with parcd.Batch(0, max_graphs=2, arena_bytes=4 << 20) as b:
+        with pytest.raises(parcd.PcdError):
+            b.add(*broken[0], lab, parcd.job())
+
+
 @pytest.mark.parametrize("strategy", [0, 1, 2])
 def test_refine_call_equals_replay(strategy):
     g = sbm(6, 60, 0.25, 0.01, seed=21)
