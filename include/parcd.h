@@ -227,7 +227,10 @@ typedef struct pcd_graph_ref {
     const int32_t *indices, *adj, *labels;
 } pcd_graph_ref;
 /* Admits `count` graphs (graphs[i] with jobs[i]) in one call: the host
- * checks run on several threads and the stream is synchronised once.
+ * checks run on several threads, arrays in page-locked memory
+ * (pcd_host_alloc) are read by the device directly, the adjacency rows are
+ * checked on the device after the upload (a graph that fails is released
+ * and reports PCD_E_FORMAT), and the stream is synchronised once.
  * slots[i] receives graph i's slot or -1; codes[i] (optional) its
  * pcd_batch_add return code.  Returns PCD_OK when every graph was admitted,
  * else the code of the first one that was not (pcd_last_error names it);
