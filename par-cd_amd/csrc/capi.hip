@@ -175,7 +175,7 @@ static void free_graph_state(pcd_ctx *ctx) {
     ctx->d_indices = ctx->d_adj = ctx->d_order = nullptr;
     ctx->d_hub_items = nullptr;
     ctx->d_hub_item_off = nullptr;
-    ctx->d_hub_part_items = ctx->d_hub_info = nullptr;
+    ctx->d_hub_part_items = ctx->d_hub_info = ctx->d_hub_part_where = nullptr;
     ctx->d_hub_poff = ctx->d_hub_buf = ctx->d_hub_over = nullptr;
     ctx->d_hub_buf_base = nullptr;
     ctx->d_hub_partial = nullptr;
@@ -313,7 +313,7 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
     }
     // hub layout: chunks of HUB_CHUNK entries, P label partitions per hub
     const int hub_end = ctx->bins.start[BIN_HUB + 1];
-    std::vector<int4> hub_chunks, hub_parts, hub_info(hub_end);
+    std::vector<int4> hub_chunks, hub_parts, hub_where, hub_info(hub_end);
     std::vector<int32_t> chunk_off;
     std::vector<int64_t> buf_base(hub_end + 1, 0), toff(hub_end + 1, 0);
     std::vector<int32_t> tlog(hub_end + 1, 0);
@@ -331,7 +331,12 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
         }
         int nch = (int)hub_chunks.size() - first_chunk;
         hub_info[h] = make_int4(v, P, (int)hub_parts.size(), first_chunk);
-        for (int q = 0; q < P; q++) hub_parts.push_back(make_int4(h, q, first_chunk, nch));
+        for (int q = 0; q < P; q++) {
+            hub_parts.push_back(make_int4(h, q, first_chunk, nch));
+            // a partition's chunk offsets sit P + 1 apart from the first
+            // chunk's; the buffer base fits int32 (hub entries < E)
+            hub_where.push_back(make_int4(v, P, chunk_off[first_chunk], (int)buf_base[h]));
+        }
         buf_base[h + 1] = buf_base[h] + d;
         tlog[h] = std::max(6, ceil_log2_host(2 * (int64_t)d));
         toff[h + 1] = toff[h] + ((int64_t)1 << tlog[h]);
@@ -369,6 +374,7 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
         PCD_TRY(up(&ctx->d_hub_item_off, chunk_off));
         PCD_TRY(up(&ctx->d_hub_part_items, hub_parts));
         PCD_TRY(up(&ctx->d_hub_info, hub_info));
+        PCD_TRY(up(&ctx->d_hub_part_where, hub_where));
         PCD_TRY(up(&ctx->d_hub_buf_base, buf_base));
         PCD_TRY(up(&ctx->d_hub_table_off, toff));
         PCD_TRY(up(&ctx->d_hub_table_log, tlog));
@@ -660,6 +666,11 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.small_aux = value != 0;
     } else if (k == "block_aux") {
         ctx->tune.block_aux = value != 0;
+    } else if (k == "hub_aux") {
+        ctx->tune.hub_aux = value != 0;
+    } else if (k == "hub_nt") {
+        if (value != 256 && value != 512) return fail(PCD_E_INVALID, "hub_nt: 256 or 512");
+        ctx->tune.hub_nt = value;
     } else if (k == "stats_cache") {
         ctx->tune.stats_cache = value != 0;
     } else if (k == "inc_stats") {

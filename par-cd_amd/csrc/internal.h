@@ -52,6 +52,8 @@ struct Tuning {
     int stat_chunk = 2048;       // entries per statistics work item for deg > 64
     int small_aux = 0;           // small bins gather {label, aux} records (1) or labels (0)
     int block_aux = 0;           // wave/workgroup-per-vertex bins gather {label, aux} records
+    int hub_aux = 0;             // hub partitions count {label, aux} records (1) or labels (0)
+    int hub_nt = 512;            // hub partition workgroups: 512 threads, 4096 slots or 256, 2048
     int stats_cache = 1;         // statistics over long rows read the labels of the first
                                  // STATS_CACHE vertices (the highest degrees) from LDS
     int inc_stats = 1;           // a propagation step updates the statistics from the rows
@@ -111,6 +113,7 @@ struct pcd_ctx {
     int32_t *d_hub_item_off = nullptr;   // chunk -> offset of its (P+1) partition offsets
     int4 *d_hub_part_items = nullptr;    // {hub, part, first chunk, number of chunks}
     int4 *d_hub_info = nullptr;          // per hub: {v, P, first part item, first chunk}
+    int4 *d_hub_part_where = nullptr;    // per partition item: {v, P, poff of the first chunk, buffer base}
     int32_t *d_hub_poff = nullptr;       // partition offsets of all chunks
     int32_t *d_hub_buf = nullptr;        // partitioned {label, aux} records (sum of hub degrees)
     int64_t *d_hub_buf_base = nullptr;   // per hub: offset into d_hub_buf

@@ -6,12 +6,12 @@
 // the best of those (score, then tie key, then label), score_leave(v, own
 // count) and the move decision.
 //
-// Neighbour gathers read a per-vertex 8-byte record {label, aux} built at
-// the start of the step, where aux is the one community total the rule's
-// gains read (modularity: int_volume + ext_volume; CPM: size).  One random
-// gather per neighbour entry thus brings both the label and what its score
-// needs.  Rules that read more (WCC, conductance) gather the community
-// record per distinct label instead.
+// Neighbour gathers read labels; the community record of each distinct
+// label is read once per row when its gain is scored.  For modularity and
+// CPM, whose gains read one community total (int_volume + ext_volume, or
+// size), kernels can be tuned to gather an 8-byte {label, aux} record per
+// entry instead (built at the start of the step), so that one gather brings
+// both; WCC and conductance always read the whole record.
 //
 // Vertex selection (counter hash against frac) is evaluated inline.  The
 // next labels start as a copy of the current ones (one bulk copy before the
@@ -20,20 +20,24 @@
 //
 //   G4..G64   deg <= G: each wave compacts the active vertices of CP
 //             consecutive positions (ballot), then G lanes per vertex load
-//             one neighbour record each, bitonic-sort them by label across
+//             one neighbour label each, bitonic-sort them by label across
 //             the lanes (shuffles), find run heads by ballot and reduce the
 //             best candidate by butterfly.  No LDS table.
 //   W256      one wave per vertex, LDS hash table of 2 deg slots.
-//   B1024     one 256-thread workgroup per vertex, 2 deg slots.
-//   B4096     one 512-thread workgroup per vertex, 1.5 deg slots.
+//   B1024     one 256-thread workgroup per vertex (or, for degrees up to
+//             512, one wave), 2 deg slots.
+//   B4096     one 512-thread workgroup per vertex, 2 deg slots.
 //   HUB       deg > 4096.  Pass 1: each 4096-entry chunk of a hub row
-//             gathers its records and writes them grouped by label-hash
+//             gathers its labels and writes them grouped by label-hash
 //             partition (LDS counting sort).  Pass 2: one workgroup per
-//             (hub, partition) counts the partition in an LDS table and
-//             reduces its best candidate.  Pass 3: one wave per hub combines
-//             the partitions and decides.  A partition whose distinct labels
-//             overflow its table flags the hub, which is then recounted in a
-//             global-memory hash table (never full).
+//             (hub, partition) counts the partition in an LDS table, scans
+//             the slots it claimed and reduces its best candidate.  Pass 3:
+//             one wave per hub combines the partitions and decides.  A
+//             partition whose distinct labels overflow its table flags the
+//             hub, which is then recounted in a global-memory hash table
+//             (never full).
+#include <type_traits>
+
 #include "work.h"
 
 namespace pcd {
@@ -211,105 +215,15 @@ __global__ __launch_bounds__(256) void k_lp_small(Src src) {
 }
 
 // ------------------------------------------------------------- LDS hash table
-// T slots (any multiple of 64), linear probing from (hash * T) >> 32.
-struct Table {
-    int *keys, *cnts, *auxs;
-    int T;
-};
-
+// Hub partition tables: T slots (any multiple of 64), linear probing from
+// (hash * T) >> 32.
 __device__ inline int tslot(int key, int T) {
     return (int)(((uint64_t)((uint32_t)key * 0x9E3779B1u) * (uint32_t)T) >> 32);
 }
 
-// Inserts (key, aux) with count add; probes at most T slots and returns
-// false when the table is full.
-template <bool AUX>
-__device__ inline bool tinsert(const Table &t, int key, int aux, int add) {
-    int h = tslot(key, t.T);
-    for (int probe = 0; probe < t.T; probe++) {
-        int old = atomicCAS(&t.keys[h], EMPTY_KEY, key);
-        if (AUX && old == EMPTY_KEY) t.auxs[h] = aux;
-        if (old == EMPTY_KEY || old == key) {
-            atomicAdd(&t.cnts[h], add);
-            return true;
-        }
-        if (++h == t.T) h = 0;
-    }
-    return false;
-}
-
-__device__ inline void tclear(const Table &t, int lo, int stride) {
-    for (int s = lo; s < t.T; s += stride) { t.keys[s] = EMPTY_KEY; t.cnts[s] = 0; }
-}
-
-// Gathers the labels (or {label, aux} records) of row entries
-// [start + lo, start + deg) with stride `stride`, PF at a time (independent
-// loads in flight), and inserts them.
-template <int PF, bool AUX>
-__device__ inline void insert_row(const LPArgs &a, const Table &t, int start, int deg, int lo, int stride) {
-    for (int j0 = lo; j0 < deg; j0 += PF * stride) {
-        int nb[PF];
-#pragma unroll
-        for (int e = 0; e < PF; e++) {
-            int j = j0 + e * stride;
-            nb[e] = j < deg ? a.adj[start + j] : -1;
-        }
-        int2 r[PF];
-#pragma unroll
-        for (int e = 0; e < PF; e++) {
-            if (AUX) r[e] = nb[e] >= 0 ? a.lv[nb[e]] : make_int2(EMPTY_KEY, 0);
-            else r[e] = make_int2(nb[e] >= 0 ? a.labels[nb[e]] : EMPTY_KEY, 0);
-        }
-#pragma unroll
-        for (int e = 0; e < PF; e++)
-            if (r[e].x != EMPTY_KEY) tinsert<AUX>(t, r[e].x, r[e].y, 1);
-    }
-}
-
-// Scans table slots [lo, T) with stride `stride` for candidates, SB slots at
-// a time so that their community-record gathers overlap (tables without
-// aux records).
+// Listed-slot scans read SB slots at a time so that their community-record
+// gathers overlap.
 constexpr int SB = 4;
-template <bool AUX, unsigned KM>
-__device__ inline void scan_table(const LPArgs &a, const Table &t, int lo, int stride, int deg, int own, uint64_t tb,
-                                  Cand &c, int &own_cnt, int &distinct) {
-    for (int s0 = lo; s0 < t.T; s0 += SB * stride) {
-        int key[SB], cnt[SB];
-        CommStats st[SB];
-#pragma unroll
-        for (int e = 0; e < SB; e++) {
-            int s = s0 + e * stride;
-            key[e] = s < t.T ? t.keys[s] : EMPTY_KEY;
-            cnt[e] = 0;
-            if (key[e] != EMPTY_KEY) {
-                cnt[e] = t.cnts[s];
-                if (AUX) {
-                    int x = t.auxs[s];
-                    st[e] = CommStats{x, x, 0, 0};
-                }
-            }
-        }
-        if (!AUX) {
-#pragma unroll
-            for (int e = 0; e < SB; e++)
-                if (key[e] != EMPTY_KEY && key[e] != own) st[e] = a.stats[key[e]];
-        }
-#pragma unroll
-        for (int e = 0; e < SB; e++) {
-            if (key[e] == EMPTY_KEY) continue;
-            distinct++;
-            if (key[e] == own) {
-                own_cnt = cnt[e];
-            } else {
-                double sc = score_join<KM>(a.rule, deg, st[e], cnt[e]);
-                if (sc > 0) {
-                    Cand o{sc, tie_key(a.tie, tb, key[e]), key[e]};
-                    if (cand_better(o, c)) c = o;
-                }
-            }
-        }
-    }
-}
 
 __device__ inline void wave_reduce_cand(Cand &c, int &own_cnt) {
 #pragma unroll
@@ -559,6 +473,7 @@ struct HubArgs {
     const int4 *chunks;        // {hub, v, begin, end}
     const int32_t *chunk_off;  // chunk -> first of its P+1 offsets in poff
     const int4 *parts;         // {hub, part, first chunk, number of chunks}
+    const int4 *where;         // per part item: {v, P, poff of the hub's first chunk, buffer base}
     const int4 *info;          // per hub {v, P, first part item, first chunk}
     const int64_t *buf_base;   // per hub
     int32_t *poff;
@@ -584,11 +499,14 @@ __device__ inline int hub_part_of(int label, int P) {
 }
 
 constexpr int HUB1_NT = 512, HUB1_PER = HUB_CHUNK / HUB1_NT;
-constexpr int HUB2_NT = 512;
 static_assert(HUB1_NT * HUB1_PER == HUB_CHUNK, "hub chunk layout");
 
-// pass 1: gather the chunk's records and group them by partition
+// pass 1: gather the chunk's labels (with AUX its {label, aux} records) and
+// group them by partition
+template <bool AUX>
 __global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) {
+    using Rec = typename std::conditional<AUX, int2, int32_t>::type;
+    Rec *out = reinterpret_cast<Rec *>(h.buf);
     __shared__ int cnt[HUB_MAX_PARTS + 1];
     for (int it = blockIdx.x; it < h.nchunks; it += gridDim.x) {
         const int4 ck = h.chunks[it];
@@ -598,19 +516,25 @@ __global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) 
         for (int i = threadIdx.x; i <= P; i += HUB1_NT) cnt[i] = 0;
         __syncthreads();
         int nb[HUB1_PER], part[HUB1_PER], rank[HUB1_PER];
-        int2 rec[HUB1_PER];
+        Rec rec[HUB1_PER];
 #pragma unroll
         for (int e = 0; e < HUB1_PER; e++) {
             int j = ck.z + e * HUB1_NT + threadIdx.x;
             nb[e] = j < ck.w ? a.adj[j] : -1;
         }
 #pragma unroll
-        for (int e = 0; e < HUB1_PER; e++) rec[e] = nb[e] >= 0 ? a.lv[nb[e]] : make_int2(0, 0);
+        for (int e = 0; e < HUB1_PER; e++) {
+            if constexpr (AUX) rec[e] = nb[e] >= 0 ? a.lv[nb[e]] : make_int2(0, 0);
+            else rec[e] = nb[e] >= 0 ? a.labels[nb[e]] : 0;
+        }
 #pragma unroll
         for (int e = 0; e < HUB1_PER; e++) {
             part[e] = -1;
             if (nb[e] >= 0) {
-                part[e] = hub_part_of(rec[e].x, P);
+                int key;
+                if constexpr (AUX) key = rec[e].x;
+                else key = rec[e];
+                part[e] = hub_part_of(key, P);
                 rank[e] = atomicAdd(&cnt[part[e]], 1);
             }
         }
@@ -631,76 +555,160 @@ __global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) 
         for (int q = threadIdx.x; q <= P; q += HUB1_NT) po[q] = rel + cnt[q];
 #pragma unroll
         for (int e = 0; e < HUB1_PER; e++)
-            if (part[e] >= 0) h.buf[base + cnt[part[e]] + rank[e]] = rec[e];
+            if (part[e] >= 0) out[base + cnt[part[e]] + rank[e]] = rec[e];
         __syncthreads();
     }
 }
 
-// pass 2: count one partition of one hub in LDS, reduce its best candidate
-template <unsigned KM>
-__global__ __launch_bounds__(HUB2_NT) void k_hub_count(LPArgs a, HubArgs h) {
-    __shared__ int keys[HUB_TB];
-    __shared__ int cnts[HUB_TB];
-    __shared__ int auxs[HUB_TB];
-    __shared__ int seg[HUB2_NT + 1];
-    __shared__ int segbase[HUB2_NT];
-    __shared__ Cand s_c[HUB2_NT / WAVE];
-    __shared__ int s_i[2 * HUB2_NT / WAVE];
-    __shared__ int s_over;
-    const Table t{keys, cnts, auxs, h.table_slots};
-    for (int it = blockIdx.x; it < h.nparts; it += gridDim.x) {
-        const int4 pi = h.parts[it];
-        const int4 inf = h.info[pi.x];
-        const int v = inf.x, part = pi.y;
-        if (!selected(a.seed, a.step, v, a.f)) continue;
-        tclear(t, threadIdx.x, HUB2_NT);
-        if (threadIdx.x == 0) s_over = 0;
-        __syncthreads();
-        const int64_t hb = h.buf_base[pi.x];
-        for (int c0 = 0; c0 < pi.w; c0 += HUB2_NT) {
-            int nc = min(HUB2_NT, pi.w - c0);
-            int len = 0, b0 = 0;
-            if (threadIdx.x < nc) {
-                const int32_t *po = h.poff + h.chunk_off[pi.z + c0 + threadIdx.x];
-                b0 = po[part];
-                len = po[part + 1] - b0;
-            }
-            int tot;
-            int ex = block_excl_scan_n<HUB2_NT>(len, &tot, s_i);
-            if (threadIdx.x < nc) { seg[threadIdx.x] = ex; segbase[threadIdx.x] = b0; }
-            __syncthreads();
-            for (int i = threadIdx.x; i < tot; i += HUB2_NT) {
-                int lo = 0, hi = nc;  // last segment with seg[lo] <= i
-                while (hi - lo > 1) {
-                    int mid = (lo + hi) >> 1;
-                    if (seg[mid] <= i) lo = mid;
-                    else hi = mid;
+// Inserts (key, aux) with count 1 into a hub partition table of T slots,
+// probing at most T slots; false when the table is full.  claimed: this
+// lane took a free slot, whose index lands in `slot`.
+template <bool AUX>
+__device__ inline bool tclaim(int *keys, int *cnts, int *auxs, int T, int key, int aux, bool &claimed, int &slot) {
+    int hs = tslot(key, T);
+    for (int probe = 0; probe < T; probe++) {
+        int old = atomicCAS(&keys[hs], EMPTY_KEY, key);
+        if (AUX && old == EMPTY_KEY) auxs[hs] = aux;
+        if (old == EMPTY_KEY || old == key) {
+            atomicAdd(&cnts[hs], 1);
+            claimed = old == EMPTY_KEY;
+            slot = hs;
+            return true;
+        }
+        if (++hs == T) hs = 0;
+    }
+    return false;
+}
+
+// pass 2: one workgroup per (hub, partition) counts the partition in an LDS
+// table that lists the slots it claims, so the scan visits only the
+// partition's distinct labels (reading each one's community record) and
+// clears just those slots for the next partition.  A key that finds the
+// table full flags the hub for the global-table path; the listed slots are
+// cleared all the same.
+// NT threads per workgroup, TB table slots (the tuned table size is capped
+// at TB); the entries of a partition are gathered PF per thread at a time so
+// that their loads are in flight together.  With AUX the records are
+// {label, aux} and the table keeps aux (no community-record reads).
+// A partition's entries lie in one segment per chunk of its hub; each wave
+// builds its own copy of the segment table, 64 chunks at a time (wave scan,
+// no workgroup barrier), and the next partition item is loaded while the
+// current one is counted.
+template <unsigned KM, int NT, int TB, bool AUX>
+__global__ __launch_bounds__(NT) void k_hub_count_listed(LPArgs a, HubArgs h) {
+    constexpr int PF = 4;
+    constexpr int NW = NT / WAVE;
+    using Rec = typename std::conditional<AUX, int2, int32_t>::type;
+    __shared__ int keys[TB];
+    __shared__ int cnts[TB];
+    __shared__ int auxs[AUX ? TB : 1];
+    __shared__ uint16_t occ[TB];
+    __shared__ int s_seg[NW][WAVE];
+    __shared__ int s_segbase[NW][WAVE];
+    __shared__ Cand s_c[NW];
+    __shared__ int s_i[2 * NW];
+    __shared__ int s_over, s_nocc;
+    const int T = min(h.table_slots, TB);
+    const int lane = __lane_id(), w = threadIdx.x / WAVE;
+    int *seg = s_seg[w], *segbase = s_segbase[w];
+    for (int i = threadIdx.x; i < TB; i += NT) {
+        keys[i] = EMPTY_KEY;
+        cnts[i] = 0;
+    }
+    if (threadIdx.x == 0) { s_over = 0; s_nocc = 0; }
+    __syncthreads();
+    const Rec *buf = reinterpret_cast<const Rec *>(h.buf);
+    int it = blockIdx.x;
+    int4 pi = make_int4(0, 0, 0, 0), pw = make_int4(0, 0, 0, 0);
+    if (it < h.nparts) { pi = h.parts[it]; pw = h.where[it]; }
+    while (it < h.nparts) {
+        const int nxt = it + gridDim.x;
+        int4 pi_n = pi, pw_n = pw;
+        if (nxt < h.nparts) { pi_n = h.parts[nxt]; pw_n = h.where[nxt]; }
+        const int v = pw.x, P = pw.y, part = pi.y;
+        if (selected(a.seed, a.step, v, a.f)) {
+            for (int c0 = 0; c0 < pi.w; c0 += WAVE) {
+                const int nc = min(WAVE, pi.w - c0);
+                int len = 0, b0 = 0;
+                if (lane < nc) {
+                    const int32_t *po = h.poff + pw.z + (int64_t)(c0 + lane) * (P + 1);
+                    b0 = po[part];
+                    len = po[part + 1] - b0;
                 }
-                int2 r = h.buf[hb + segbase[lo] + (i - seg[lo])];
-                if (!tinsert<true>(t, r.x, r.y, 1)) s_over = 1;
+                const int inc = wave_incl_scan(len);
+                const int tot = __shfl(inc, WAVE - 1, WAVE);
+                seg[lane] = inc - len;
+                segbase[lane] = pw.w + b0;
+                wave_sync();
+                // the trip count is the same for every lane of the
+                // workgroup: occ_append is called by whole waves
+                for (int i0 = 0; i0 < tot; i0 += PF * NT) {
+                    int key[PF], aux[PF];
+#pragma unroll
+                    for (int e = 0; e < PF; e++) {
+                        const int i = i0 + e * NT + threadIdx.x;
+                        key[e] = EMPTY_KEY;
+                        aux[e] = 0;
+                        if (i < tot) {
+                            int lo = 0, hi = nc;  // last segment with seg[lo] <= i
+                            while (hi - lo > 1) {
+                                int mid = (lo + hi) >> 1;
+                                if (seg[mid] <= i) lo = mid;
+                                else hi = mid;
+                            }
+                            const Rec r = buf[(int64_t)segbase[lo] + (i - seg[lo])];
+                            if constexpr (AUX) { key[e] = r.x; aux[e] = r.y; }
+                            else key[e] = r;
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < PF; e++) {
+                        bool claimed = false;
+                        int slot = 0;
+                        if (key[e] != EMPTY_KEY && !tclaim<AUX>(keys, cnts, auxs, T, key[e], aux[e], claimed, slot))
+                            s_over = 1;
+                        occ_append(claimed, slot, occ, &s_nocc);
+                    }
+                }
+                wave_sync();
             }
             __syncthreads();
-        }
-        if (s_over) {
+            const int D = s_nocc;
+            const bool over = s_over != 0;
+            const int deg = a.indices[v + 1] - a.indices[v], own = a.labels[v];
+            Cand c = cand_none();
+            int own_cnt = 0, distinct = threadIdx.x == 0 ? D : 0;
+            if (over) {
+                for (int i = threadIdx.x; i < D; i += NT) {
+                    const int hs = occ[i];
+                    keys[hs] = EMPTY_KEY;
+                    cnts[hs] = 0;
+                }
+            } else {
+                uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
+                scan_listed<AUX, KM>(a, keys, cnts, auxs, occ, D, threadIdx.x, NT, deg, own, tb, c, own_cnt);
+            }
+            block_reduce_cand<NT>(c, own_cnt, distinct, s_c, s_i);
             if (threadIdx.x == 0) {
-                h.over[pi.x] = 1;
-                atomicAdd(h.over_count, 1);
+                if (over) {
+                    h.over[pi.x] = 1;
+                    atomicAdd(h.over_count, 1);
+                } else {
+                    h.partial[it] = HubPartial{c, own_cnt, distinct};
+                }
+                s_over = 0;
+                s_nocc = 0;
             }
             __syncthreads();
-            continue;
         }
-        const int deg = a.indices[v + 1] - a.indices[v], own = a.labels[v];
-        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
-        Cand c = cand_none();
-        int own_cnt = 0, distinct = 0;
-        scan_table<true, KM>(a, t, threadIdx.x, HUB2_NT, deg, own, tb, c, own_cnt, distinct);
-        block_reduce_cand<HUB2_NT>(c, own_cnt, distinct, s_c, s_i);
-        if (threadIdx.x == 0) h.partial[it] = HubPartial{c, own_cnt, distinct};
+        it = nxt;
+        pi = pi_n;
+        pw = pw_n;
     }
 }
 
 // pass 3: one wave per hub combines its partitions and decides
-template <unsigned KM>
+template <unsigned KM, bool AUX>
 __global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
     const int lane = __lane_id();
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
@@ -722,8 +730,12 @@ __global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
         distinct = wave_sum(distinct);
         if (lane == 0) {
             const int deg = a.indices[v + 1] - a.indices[v];
-            int2 own = a.lv[v];
-            a.new_labels[v] = lp_decide<KM>(a, deg, own.x, own.y, own_cnt, c);
+            if constexpr (AUX) {
+                int2 own = a.lv[v];
+                a.new_labels[v] = lp_decide<KM>(a, deg, own.x, own.y, own_cnt, c);
+            } else {
+                a.new_labels[v] = lp_decide_stats<KM>(a, deg, a.labels[v], own_cnt, c);
+            }
             if (a.counters) count3(a, 1, deg, distinct);
         }
     }
@@ -820,7 +832,11 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
                            ctx->stream));
     const bool small_aux = a.use_aux && ctx->tune.small_aux;
     const bool block_aux = a.use_aux && ctx->tune.block_aux;
-    if (a.use_aux && (ctx->num_hubs > 0 || small_aux || block_aux)) {
+    // hubs count {label, aux} records only when tuned so (and the rule's
+    // gains read aux alone); otherwise labels, with the community record read
+    // per distinct label
+    const bool hub_aux = a.use_aux && ctx->tune.hub_aux;
+    if (a.use_aux && ((ctx->num_hubs > 0 && hub_aux) || small_aux || block_aux)) {
         kt_begin(ctx, KS_LP_SELECT);
         hipLaunchKernelGGL(k_lp_records, dim3(grid_for(ctx, ctx->n, 256, 8)), dim3(256), 0, ctx->stream, ctx->n,
                            ctx->d_labels, ctx->d_stats, a.rule.kind, ctx->d_lv);
@@ -833,6 +849,7 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
         h.chunk_off = ctx->d_hub_item_off;
         h.parts = ctx->d_hub_part_items;
         h.info = ctx->d_hub_info;
+        h.where = ctx->d_hub_part_where;
         h.buf_base = ctx->d_hub_buf_base;
         h.poff = ctx->d_hub_poff;
         h.buf = (int2 *)ctx->d_hub_buf;
@@ -846,12 +863,35 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
         h.table_slots = std::min(HUB_TB, std::max(64, ctx->tune.hub_table_slots & ~63));
         LPArgs ah = args_for(BIN_HUB);
         kt_begin(ctx, KS_LP_HUB_INSERT);
-        hipLaunchKernelGGL(k_hub_partition, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0, ctx->stream, ah,
-                           h);
+        if (hub_aux)
+            hipLaunchKernelGGL(k_hub_partition<true>, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
+                               ctx->stream, ah, h);
+        else
+            hipLaunchKernelGGL(k_hub_partition<false>, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
+                               ctx->stream, ah, h);
         kt_end(ctx, KS_LP_HUB_INSERT);
         kt_begin(ctx, KS_LP_HUB_DECIDE);
-        hipLaunchKernelGGL(k_hub_count<KM>, dim3(grid_for(ctx, h.nparts, 1, 3)), dim3(HUB2_NT), 0, ctx->stream, ah, h);
-        hipLaunchKernelGGL(k_hub_final<KM>, dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0, ctx->stream, ah, h);
+        // 512 threads with 4096 slots (three workgroups per CU) or 256
+        // threads with 2048 slots (seven per CU; five with aux records)
+        if (hub_aux) {
+            if (ctx->tune.hub_nt == 256)
+                hipLaunchKernelGGL((k_hub_count_listed<KM, 256, 2048, true>), dim3(grid_for(ctx, h.nparts, 1, 5)),
+                                   dim3(256), 0, ctx->stream, ah, h);
+            else
+                hipLaunchKernelGGL((k_hub_count_listed<KM, 512, HUB_TB, true>), dim3(grid_for(ctx, h.nparts, 1, 2)),
+                                   dim3(512), 0, ctx->stream, ah, h);
+            hipLaunchKernelGGL((k_hub_final<KM, true>), dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0, ctx->stream,
+                               ah, h);
+        } else {
+            if (ctx->tune.hub_nt == 256)
+                hipLaunchKernelGGL((k_hub_count_listed<KM, 256, 2048, false>), dim3(grid_for(ctx, h.nparts, 1, 7)),
+                                   dim3(256), 0, ctx->stream, ah, h);
+            else
+                hipLaunchKernelGGL((k_hub_count_listed<KM, 512, HUB_TB, false>), dim3(grid_for(ctx, h.nparts, 1, 3)),
+                                   dim3(512), 0, ctx->stream, ah, h);
+            hipLaunchKernelGGL((k_hub_final<KM, false>), dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0,
+                               ctx->stream, ah, h);
+        }
         hipLaunchKernelGGL(k_hub_global_insert, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(512), 0, ctx->stream, ah,
                            h, ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts);
         hipLaunchKernelGGL(k_hub_global_decide<KM>, dim3(grid_for(ctx, h.nhubs, 1, 1)), dim3(1024), 0, ctx->stream, ah,

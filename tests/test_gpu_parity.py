@@ -198,23 +198,41 @@ def test_statistics_follow_propagation(ctx, inc):
     assert left > 0  # NEW_LABEL moves were exercised
 
 
-@pytest.mark.parametrize("slots,part_entries", [(4096, 2048), (64, 2048), (1024, 512), (64, 100000), (4096, 100000)])
-def test_propagation_hub_paths(ctx, slots, part_entries):
-    """Hub rows: partitioned LDS counting, and the global-table fallback when a
-    partition overflows its table (forced by tiny tables)."""
+HUB_CASES = [(4096, 2048, 512, 0), (64, 2048, 512, 0), (1024, 512, 512, 0), (64, 100000, 512, 0),
+             (4096, 100000, 512, 0), (2048, 1024, 256, 0), (64, 1024, 256, 0), (2048, 100000, 256, 0),
+             (4096, 2048, 512, 1), (64, 2048, 512, 1), (1024, 512, 512, 1), (64, 100000, 512, 1),
+             (2048, 1024, 256, 1), (64, 1024, 256, 1)]
+
+
+@pytest.mark.parametrize("slots,part_entries,hub_nt,hub_aux", HUB_CASES)
+def test_propagation_hub_paths(ctx, slots, part_entries, hub_nt, hub_aux):
+    """Hub rows: partitioned LDS counting of labels (hub_aux 0; workgroups
+    of 512 threads with 4096-slot tables or of 256 with 2048) or of
+    {label, aux} records (hub_aux 1, modularity and CPM), and the
+    global-table fallback when a partition overflows its table (forced by
+    tiny tables or huge partitions).  WCC and conductance gains read the
+    whole community record and always count labels."""
     g = GG.bin_zoo()
-    n = g[0]
+    n, m, ind, adj = g[:4]
+    cc = O.count_triangles(n, ind, adj)[3]
     rng = np.random.default_rng(6)
+    rules = [(O.MOD, 0.0, None), (O.CPM, 0.02, None)]
+    if hub_aux == 0:
+        rules += [(O.WCC, 0.0, cc), (O.COND, 0.0, None)]
     try:
         ctx.set_tuning("hub_part_entries", part_entries)
         ctx.set_tuning("hub_table_slots", slots)
+        ctx.set_tuning("hub_aux", hub_aux)
+        ctx.set_tuning("hub_nt", hub_nt)
         for lab in [GG.singleton_labels(n), rng.integers(0, 40, n), rng.integers(0, 3000, n)]:
-            for kind, lam in [(O.MOD, 0.0), (O.CPM, 0.02)]:
-                _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=4, step=3, tie=O.TIE_RANDOM)
-                _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=4, step=4, tie=O.TIE_LOWEST)
+            for kind, lam, c in rules:
+                _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=4, step=3, tie=O.TIE_RANDOM, cc=c)
+                _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=4, step=4, tie=O.TIE_LOWEST, cc=c)
     finally:
         ctx.set_tuning("hub_part_entries", 2048)
         ctx.set_tuning("hub_table_slots", 4096)
+        ctx.set_tuning("hub_aux", 0)
+        ctx.set_tuning("hub_nt", 512)
 
 
 def test_hub_partitions_fit_with_default_tuning():
