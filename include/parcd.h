@@ -164,9 +164,12 @@ int pcd_bin_counters(pcd_ctx *ctx, int bin, int64_t *active, int64_t *entries, i
  * partition table, multiple of 64 in [64, 4096]; takes effect at once),
  * "hub_part_entries" (target entries per hub label partition) and
  * "stat_chunk" (entries per statistics work item), which take effect at the
- * next pcd_graph_upload, and "small_aux" (0/1: the small-degree bins gather
- * {label, community total} records instead of labels).  Results do not
- * depend on them. */
+ * next pcd_graph_upload, "small_aux", "block_aux" and "hub_aux" (0/1: the
+ * small-degree, per-vertex table and hub kernels gather {label, community
+ * total} records instead of labels, for modularity and CPM) and "hub_nt"
+ * (512 or 256: hub partitions are counted by 512-thread workgroups with
+ * 4096-slot tables or 256-thread ones with 2048).  Results do not depend on
+ * them. */
 int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value);
 /* Hub label partitions whose distinct labels overflowed their LDS table
  * since the last call (those hubs were recounted in global memory); the
