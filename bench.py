@@ -96,7 +96,7 @@ ROCPROF_NAMES = {
     "lp_b4096": [r"pcd::k_lp_block<512, 8192, 1u, pcd::RangeSrc"],
     "lp_b1024": [r"pcd::k_lp_block<256, 2048, 1u, pcd::RangeSrc", r"pcd::k_lp_wave<1024, 1u, pcd::RangeSrc"],
     "lp_w256": [r"pcd::k_lp_wave<512, 1u, pcd::RangeSrc"],
-    "lp_hub": [r"pcd::k_hub_partition$", r"pcd::k_hub_count<1u>$"],
+    "lp_hub": [r"pcd::k_hub_partition<false>$", r"pcd::k_hub_count_listed<1u, 512, 4096, false>$"],
     "lp_g64": [r"pcd::k_lp_small<64, 8, false, 1u"], "lp_g32": [r"pcd::k_lp_small<32, 16, false, 1u"],
     "lp_g16": [r"pcd::k_lp_small<16, 32, false, 1u"], "lp_g8": [r"pcd::k_lp_small<8, 64, false, 1u"],
     "lp_g4": [r"pcd::k_lp_small<4, 64, false, 1u"]}
