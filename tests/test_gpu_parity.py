@@ -429,3 +429,72 @@ def test_rmat_large_properties(ctx):
     assert lab.min() >= -1
     rng = np.random.default_rng(1)
     _lp_compare(ctx, g, rng.integers(0, 5000, n), O.CPM, 0.01, 0.5, seed=2, step=5, tie=O.TIE_LOWEST)
+
+
+def test_rmat22_benchmark_iterations(ctx):
+    """The benchmark's own workload at full size (R-MAT scale 22 seed 2,
+    2.4 M vertices, 128 M entries): its first three refinement iterations from
+    the singleton partition (frac 0.5, seed 1, random ties).  Each
+    compression, each statistics table (recounted in the first iteration,
+    then updated from the moved rows and renumbered by the compression), each
+    modularity value and each propagation step equals the oracle's on the
+    same inputs."""
+    g = GG.rmat(22)
+    n, m, ind, adj = g[:4]
+    ctx.upload_graph(n, m, ind, adj)
+    prev = GG.singleton_labels(n)
+    ctx.upload_labels(prev)
+    for it in range(3):
+        k = ctx.compress()
+        lab_o, k_o = O.compress_labels(prev)
+        assert k == k_o
+        assert np.array_equal(ctx.labels(), lab_o)
+        assert ctx.statistics() == k_o
+        s, iv, ev = ctx.stats(k)
+        _, so, ivo, evo = O.collect_statistics(n, ind, adj, lab_o)
+        assert np.array_equal(s, so) and np.array_equal(iv, ivo) and np.array_equal(ev, evo), it
+        assert int(so.sum()) == n and int(ivo.sum() + evo.sum()) == 2 * m
+        q = ctx.metric(O.MOD, 0.0)
+        assert q == pytest.approx(O.calculate(O.MOD, 0.0, n, m, ind, adj, lab_o, so, ivo, evo), rel=1e-9, abs=1e-12)
+        ctx.propagate(O.MOD, 0.0, frac=0.5, seed=1, step=it, tie=O.TIE_RANDOM)
+        got = ctx.labels()
+        want = O.propagation_refine(O.MOD, 0.0, n, m, ind, adj, lab_o, so, ivo, evo, 0.0, 0.5, 1, 1, it, O.TIE_RANDOM)
+        assert np.array_equal(got, want), (it, int(np.count_nonzero(got != want)))
+        prev = got
+
+
+def test_chung_lu_wcc_benchmark_iterations(ctx):
+    """The benchmark's WCC workload at full size (Chung-Lu seed 3: 10^6
+    vertices, degrees up to 8,415, so rows in the hub path), run as
+    `-m wcc -i 1`: triangles, the triangle-based initial partition on the
+    device, then two refinement iterations whose compression, statistics,
+    WCC metric and propagation step each equal the oracle's."""
+    import parcd as P
+    g = P.make_graph("chung_lu", 3, n=1000000, exponent=2.3, dmin=4, dmax=5000, num_comms=32, cmin=100,
+                     cmax=50000, boost=20.0)
+    n, m, ind, adj = g[:4]
+    assert int(np.diff(ind).max()) > 4096  # hub rows present
+    ctx.upload_graph(n, m, ind, adj)
+    tri, td, cc = ctx.triangles()
+    et, tri_o, td_o, cc_o = O.count_triangles(n, ind, adj)
+    assert np.array_equal(tri, tri_o) and np.array_equal(td, td_o)
+    assert cc == pytest.approx(cc_o, rel=1e-12)
+    ctx.clustering()
+    prev = ctx.labels()
+    assert np.array_equal(prev, O.clustering_partition(n, ind, adj, tri_o))
+    for it in range(2):
+        k = ctx.compress()
+        lab_o, k_o = O.compress_labels(prev)
+        assert k == k_o and np.array_equal(ctx.labels(), lab_o)
+        ctx.statistics()
+        s, iv, ev = ctx.stats(k)
+        _, so, ivo, evo = O.collect_statistics(n, ind, adj, lab_o)
+        assert np.array_equal(s, so) and np.array_equal(iv, ivo) and np.array_equal(ev, evo), it
+        q = ctx.metric(O.WCC)
+        q_o = O.calculate(O.WCC, 0, n, m, ind, adj, lab_o, so, ivo, evo, tri_o, td_o)
+        assert q == pytest.approx(q_o, rel=1e-9, abs=1e-12)
+        ctx.propagate(O.WCC, 0.0, frac=0.5, seed=1, step=it, tie=O.TIE_RANDOM)
+        got = ctx.labels()
+        want = O.propagation_refine(O.WCC, 0.0, n, m, ind, adj, lab_o, so, ivo, evo, cc, 0.5, 1, 1, it, O.TIE_RANDOM)
+        assert np.array_equal(got, want), (it, int(np.count_nonzero(got != want)))
+        prev = got
