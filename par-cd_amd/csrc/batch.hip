@@ -969,6 +969,7 @@ static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, co
     g.lp.labels = d_cur;
     g.lp.order = d_order;
     g.lp.lv = nullptr;
+    g.lp.tot = nullptr;
     g.lp.stats = d_stats;
     g.lp.new_labels = d_next;
     g.lp.rule.kind = job->rule.kind;

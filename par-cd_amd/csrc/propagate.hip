@@ -80,6 +80,25 @@ __device__ inline CommStats comm_of(const LPArgs &a, int label, int aux) {
     return a.stats[label];
 }
 
+// Kernel builds for the batch read the 16-byte community records (a batch
+// keeps no per-community totals); an extra bit of the rule-kind mask.
+constexpr unsigned RK_REC = 1u << 8;
+constexpr unsigned RK_BATCH = RK_MOD | RK_CPM | RK_REC;
+
+// Community record of `label` as the gains read it.  Builds for modularity
+// or CPM alone read the 4-byte per-community total the step built (one
+// random read into a 4x smaller array than the records); the others read
+// the record.
+template <unsigned KM>
+__device__ inline CommStats comm_rec(const LPArgs &a, int label) {
+    if constexpr ((KM & (RK_WCC | RK_COND | RK_REC)) == 0) {
+        const int x = a.tot[label];
+        return CommStats{x, x, 0, 0};
+    } else {
+        return a.stats[label];
+    }
+}
+
 // refinement.cpp:211-221: best_score (0 if no join candidate beat 0) plus
 // score_leave decides between moving, leaving to a new singleton, staying.
 template <unsigned KM>
@@ -94,7 +113,7 @@ __device__ inline int lp_decide(const LPArgs &a, int deg, int own, int own_aux, 
 
 template <unsigned KM>
 __device__ inline int lp_decide_stats(const LPArgs &a, int deg, int own, int own_cnt, const Cand &best) {
-    double leave = score_leave<KM>(a.rule, deg, a.stats[own], own_cnt);
+    double leave = score_leave<KM>(a.rule, deg, comm_rec<KM>(a, own), own_cnt);
     double bs = best.label != INT32_MAX ? best.s : 0.0;
     bs += leave;
     if (bs > 0 && bs > leave) return best.label;
@@ -121,6 +140,16 @@ __global__ void k_lp_records(int n, const int32_t *labels, const CommStats *st, 
         CommStats c = st[l];
         int aux = kind == PCD_RULE_MODULARITY ? c.int_vol + c.ext_vol : c.size;
         lv[v] = make_int2(l, aux);
+    }
+}
+
+// Per community the one total the gains of modularity (int_volume +
+// ext_volume) or CPM (size) read: 4 bytes per community instead of the
+// 16-byte record, so the random reads of the step hit a 4x smaller array.
+__global__ void k_label_totals(int k, const CommStats *st, int kind, int32_t *tot) {
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) {
+        CommStats c = st[l];
+        tot[l] = kind == PCD_RULE_MODULARITY ? c.int_vol + c.ext_vol : c.size;
     }
 }
 
@@ -192,7 +221,7 @@ __global__ __launch_bounds__(256) void k_lp_small(Src src) {
                     own_cnt = cnt;
                 } else {
                     uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, u);
-                    double sc = score_join<KM>(a.rule, deg, AUX ? CommStats{xa, xa, 0, 0} : a.stats[x], cnt);
+                    double sc = score_join<KM>(a.rule, deg, AUX ? CommStats{xa, xa, 0, 0} : comm_rec<KM>(a, x), cnt);
                     if (sc > 0) c = Cand{sc, tie_key(a.tie, tb, x), x};
                 }
             }
@@ -347,7 +376,7 @@ __device__ inline void scan_listed(const LPArgs &a, int *keys, int *cnts, const 
         if (!AUX) {
 #pragma unroll
             for (int e = 0; e < SB; e++)
-                if (key[e] != EMPTY_KEY && key[e] != own) st[e] = a.stats[key[e]];
+                if (key[e] != EMPTY_KEY && key[e] != own) st[e] = comm_rec<KM>(a, key[e]);
         }
 #pragma unroll
         for (int e = 0; e < SB; e++) {
@@ -791,7 +820,7 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
             distinct++;
             if (key == own) own_cnt = cnt;
             else {
-                double sc = score_join<KM>(a.rule, deg, a.stats[key], cnt);
+                double sc = score_join<KM>(a.rule, deg, comm_rec<KM>(a, key), cnt);
                 if (sc > 0) {
                     Cand o{sc, tie_key(a.tie, tb, key), key};
                     if (cand_better(o, c)) c = o;
@@ -800,7 +829,7 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
         }
         block_reduce_cand<1024>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
-            double leave = score_leave<KM>(a.rule, deg, a.stats[own], own_cnt);
+            double leave = score_leave<KM>(a.rule, deg, comm_rec<KM>(a, own), own_cnt);
             double bs = c.label != INT32_MAX ? c.s : 0.0;
             bs += leave;
             a.new_labels[v] = (bs > 0 && bs > leave) ? c.label : (leave > 0 ? PCD_NEW_LABEL : own);
@@ -840,6 +869,12 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
         kt_begin(ctx, KS_LP_SELECT);
         hipLaunchKernelGGL(k_lp_records, dim3(grid_for(ctx, ctx->n, 256, 8)), dim3(256), 0, ctx->stream, ctx->n,
                            ctx->d_labels, ctx->d_stats, a.rule.kind, ctx->d_lv);
+        kt_end(ctx, KS_LP_SELECT);
+    }
+    if (a.use_aux) {  // the modularity and CPM builds read the totals
+        kt_begin(ctx, KS_LP_SELECT);
+        hipLaunchKernelGGL(k_label_totals, dim3(grid_for(ctx, ctx->k, 256, 8)), dim3(256), 0, ctx->stream, ctx->k,
+                           ctx->d_stats, a.rule.kind, ctx->d_tot);
         kt_end(ctx, KS_LP_SELECT);
     }
     int p0, p1;
@@ -983,6 +1018,7 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
     a.tie = tie;
     a.f = (int)(fr * 2147483647.0);  // refinement.cpp:158
     a.use_aux = rule.kind == PCD_RULE_MODULARITY || rule.kind == PCD_RULE_CPM;
+    a.tot = a.use_aux ? ctx->d_tot : nullptr;
     a.counters = nullptr;
     // kernels specialised for modularity or CPM hold fewer registers (more
     // waves per SIMD); WCC and conductance use the general build
@@ -1010,25 +1046,25 @@ int batch_propagation_launch(hipStream_t stream, const BatchLPLaunch &l) {
     const int cu = l.cu_count;
     auto src = [&](int v) { return ItemSrc{l.items[v], l.count[v], l.graphs, l.state}; };
     if (l.used[LPV_B512X8192])
-        hipLaunchKernelGGL((k_lp_block<512, 8192, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 2), dim3(512), 0, stream, src(LPV_B512X8192));
+        hipLaunchKernelGGL((k_lp_block<512, 8192, RK_BATCH, ItemSrc>), dim3(cu * 2), dim3(512), 0, stream, src(LPV_B512X8192));
     if (l.used[LPV_B512])
-        hipLaunchKernelGGL((k_lp_block<512, 2048, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 4), dim3(512), 0, stream, src(LPV_B512));
+        hipLaunchKernelGGL((k_lp_block<512, 2048, RK_BATCH, ItemSrc>), dim3(cu * 4), dim3(512), 0, stream, src(LPV_B512));
     if (l.used[LPV_B256])
-        hipLaunchKernelGGL((k_lp_block<256, 2048, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_B256));
+        hipLaunchKernelGGL((k_lp_block<256, 2048, RK_BATCH, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_B256));
     if (l.used[LPV_W1024])
-        hipLaunchKernelGGL((k_lp_wave<1024, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W1024));
+        hipLaunchKernelGGL((k_lp_wave<1024, RK_BATCH, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W1024));
     if (l.used[LPV_W512])
-        hipLaunchKernelGGL((k_lp_wave<512, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W512));
+        hipLaunchKernelGGL((k_lp_wave<512, RK_BATCH, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W512));
     if (l.used[LPV_S64])
-        hipLaunchKernelGGL((k_lp_small<64, 8, false, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S64));
+        hipLaunchKernelGGL((k_lp_small<64, 8, false, RK_BATCH, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S64));
     if (l.used[LPV_S32])
-        hipLaunchKernelGGL((k_lp_small<32, 16, false, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S32));
+        hipLaunchKernelGGL((k_lp_small<32, 16, false, RK_BATCH, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S32));
     if (l.used[LPV_S16])
-        hipLaunchKernelGGL((k_lp_small<16, 32, false, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S16));
+        hipLaunchKernelGGL((k_lp_small<16, 32, false, RK_BATCH, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S16));
     if (l.used[LPV_S8])
-        hipLaunchKernelGGL((k_lp_small<8, 64, false, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S8));
+        hipLaunchKernelGGL((k_lp_small<8, 64, false, RK_BATCH, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S8));
     if (l.used[LPV_S4])
-        hipLaunchKernelGGL((k_lp_small<4, 64, false, RK_MOD | RK_CPM, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S4));
+        hipLaunchKernelGGL((k_lp_small<4, 64, false, RK_BATCH, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_S4));
     PCD_HIP(hipGetLastError());
     return PCD_OK;
 }

@@ -18,6 +18,7 @@ struct LPArgs {
     const int32_t *indices, *adj, *labels, *order;
     const int2 *lv;  // per vertex {label, aux}
     const CommStats *stats;
+    const int32_t *tot;  // per community the one total the gains read (aux), or nullptr: read stats
     int32_t *new_labels;
     RuleParams rule;
     uint64_t seed;
