@@ -186,6 +186,7 @@ static void free_graph_state(pcd_ctx *ctx) {
     ctx->d_labels = ctx->d_labels_alt = nullptr;
     ctx->d_lv = nullptr;
     ctx->d_tot = nullptr;
+    ctx->d_moved_bits = nullptr;
     ctx->d_stats = ctx->d_stats_alt = nullptr;
     ctx->d_moved = nullptr;
     ctx->d_first = ctx->d_newid = ctx->d_active = ctx->d_hub_active = nullptr;
@@ -398,6 +399,7 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_labels_alt, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_lv, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_tot, n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_moved_bits, (size_t)(n + WAVE - 1) / WAVE + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_stats, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_stats_alt, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_moved, 8 + (size_t)n + ctx->num_stat_items + 1));

@@ -115,6 +115,7 @@ struct pcd_ctx {
     int4 *d_hub_info = nullptr;          // per hub: {v, P, first part item, first chunk}
     int4 *d_hub_part_where = nullptr;    // per partition item: {v, P, poff of the first chunk, buffer base}
     int32_t *d_hub_poff = nullptr;       // partition offsets of all chunks
+    uint64_t *d_moved_bits = nullptr;    // per vertex: label changed in the last step (statistics update)
     int32_t *d_tot = nullptr;            // per community the total the modularity / CPM gains read
     int32_t *d_hub_buf = nullptr;        // partitioned {label, aux} records (sum of hub degrees)
     int64_t *d_hub_buf_base = nullptr;   // per hub: offset into d_hub_buf

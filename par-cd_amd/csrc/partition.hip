@@ -288,10 +288,31 @@ struct MovedStore {
 };
 
 // G lanes per moved vertex of one small bin (degree <= 64)
+// Moved bitmap: bit v of M is set when vertex v's label changed in the step
+// (L[v] != N[v]).  One wave writes one 64-bit word.
+__global__ __launch_bounds__(256) void k_moved_bits(int n, const int32_t *L, const int32_t *N, uint64_t *M) {
+    const int lane = __lane_id();
+    const int nwords = (n + WAVE - 1) / WAVE;
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    for (int wd = wave_g; wd < nwords; wd += nwaves) {
+        const int v = wd * WAVE + lane;
+        const uint64_t bits = __ballot(v < n && L[v] != N[v]);
+        if (lane == 0) M[wd] = bits;
+    }
+}
+
+// A neighbour's label before the step: its label after the step (already
+// gathered) unless the bitmap says it moved, so unmoved neighbours cost one
+// random gather instead of two (the bitmap itself stays in L2).
+__device__ inline int label_before(const uint64_t *M, const int32_t *L, int w, int after) {
+    return ((M[w >> 6] >> (w & 63)) & 1) ? L[w] : after;
+}
+
 template <int G>
 __global__ __launch_bounds__(256) void k_delta_small(const int32_t *list_all, const int32_t *bounds,
                                                      const int32_t *indices, const int32_t *adj, const int32_t *L,
-                                                     const int32_t *N, CommStats *st) {
+                                                     const int32_t *N, const uint64_t *M, CommStats *st) {
     const int lane = __lane_id(), glane = lane & (G - 1);
     const int per_wave = WAVE / G;
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
@@ -312,7 +333,8 @@ __global__ __launch_bounds__(256) void k_delta_small(const int32_t *list_all, co
         int a = 0, b = 0;
         for (int j = glane; j < deg; j += G) {
             const int w = adj[start + j];
-            const int lw = L[w], nw = N[w];
+            const int nw = N[w];
+            const int lw = label_before(M, L, w, nw);
             const int f = lw != nw ? 1 : 2;
             a += lw == lo ? f : 0;
             b += nw == ln ? f : 0;
@@ -335,7 +357,7 @@ __global__ __launch_bounds__(256) void k_delta_small(const int32_t *list_all, co
 // chunks are spread over many waves.
 __global__ __launch_bounds__(256) void k_delta_items(const int32_t *list_all, const int32_t *bounds,
                                                      const int4 *items, const int32_t *adj, const int32_t *L,
-                                                     const int32_t *N, CommStats *st) {
+                                                     const int32_t *N, const uint64_t *M, CommStats *st) {
     constexpr int PF = 4;
     const int lane = __lane_id();
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
@@ -368,10 +390,9 @@ __global__ __launch_bounds__(256) void k_delta_items(const int32_t *list_all, co
                 }
                 int lw[PF], nw[PF];
 #pragma unroll
-                for (int e = 0; e < PF; e++) {
-                    lw[e] = nb[e] < 0 ? -2 : L[nb[e]];
-                    nw[e] = nb[e] < 0 ? -2 : N[nb[e]];
-                }
+                for (int e = 0; e < PF; e++) nw[e] = nb[e] < 0 ? -2 : N[nb[e]];
+#pragma unroll
+                for (int e = 0; e < PF; e++) lw[e] = nb[e] < 0 ? -2 : label_before(M, L, nb[e], nw[e]);
 #pragma unroll
                 for (int e = 0; e < PF; e++) {
                     if (nb[e] < 0) continue;
@@ -405,13 +426,16 @@ int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *aft
     int32_t *list = ctx->d_moved + 8;
     const int64_t len = (int64_t)(m.s[5] - m.s[0]) + m.nitems + 1;
     PCD_TRY(device_scan(ctx, len, MovedFlag{m}, MovedStore{m, bounds, list}, nullptr));
+    uint64_t *M = ctx->d_moved_bits;
+    hipLaunchKernelGGL(k_moved_bits, dim3(grid_for(ctx, (ctx->n + WAVE - 1) / WAVE, 4, 8)), dim3(256), 0, ctx->stream,
+                       ctx->n, before, after, M);
     if (m.nitems > 0)
         hipLaunchKernelGGL(k_delta_items, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, list, bounds + 5,
-                           ctx->d_stat_items, ctx->d_adj, before, after, ctx->d_stats);
+                           ctx->d_stat_items, ctx->d_adj, before, after, M, ctx->d_stats);
 #define PCD_DELTA(I, G)                                                                                             \
     if (m.s[I + 1] > m.s[I])                                                                                        \
         hipLaunchKernelGGL(k_delta_small<G>, dim3(ctx->cu_count * 4), dim3(256), 0, ctx->stream, list, bounds + I, \
-                           ctx->d_indices, ctx->d_adj, before, after, ctx->d_stats);
+                           ctx->d_indices, ctx->d_adj, before, after, M, ctx->d_stats);
     PCD_DELTA(0, 64)
     PCD_DELTA(1, 32)
     PCD_DELTA(2, 16)
