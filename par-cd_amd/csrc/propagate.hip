@@ -112,13 +112,19 @@ __device__ inline int lp_decide(const LPArgs &a, int deg, int own, int own_aux, 
 }
 
 template <unsigned KM>
-__device__ inline int lp_decide_stats(const LPArgs &a, int deg, int own, int own_cnt, const Cand &best) {
-    double leave = score_leave<KM>(a.rule, deg, comm_rec<KM>(a, own), own_cnt);
+__device__ inline int lp_decide_rec(const LPArgs &a, int deg, int own, const CommStats &own_rec, int own_cnt,
+                                    const Cand &best) {
+    double leave = score_leave<KM>(a.rule, deg, own_rec, own_cnt);
     double bs = best.label != INT32_MAX ? best.s : 0.0;
     bs += leave;
     if (bs > 0 && bs > leave) return best.label;
     if (leave > 0) return PCD_NEW_LABEL;
     return own;
+}
+
+template <unsigned KM>
+__device__ inline int lp_decide_stats(const LPArgs &a, int deg, int own, int own_cnt, const Cand &best) {
+    return lp_decide_rec<KM>(a, deg, own, comm_rec<KM>(a, own), own_cnt, best);
 }
 
 __device__ inline void count3(const LPArgs &a, long long act, long long ent, long long dist) {
@@ -177,16 +183,35 @@ __global__ __launch_bounds__(256) void k_lp_small(Src src) {
         int nact = __popcll(am);
         if (act) list[__popcll(am & lt)] = v;
         wave_sync();
+        // the row, own label and own community record of every active vertex
+        // of the unit, one lane each, loaded together (nact <= CP <= 64);
+        // each group takes its vertices' values by shuffle
+        int pu = 0, pstart = 0, pdeg = 0, pown = 0, paux = 0;
+        CommStats pst{0, 0, 0, 0};
+        if (lane < nact) {
+            pu = list[lane];
+            pstart = a.indices[pu];
+            pdeg = a.indices[pu + 1] - pstart;
+            if constexpr (AUX) {
+                const int2 o = a.lv[pu];
+                pown = o.x;
+                paux = o.y;
+            } else {
+                pown = a.labels[pu];
+                pst = comm_rec<KM>(a, pown);
+            }
+        }
         for (int gb = 0; gb < nact; gb += per_wave) {
             int gi = gb + lane / G;
             bool has = gi < nact;
-            int u = 0, start = 0, deg = 0;
-            int2 own = make_int2(0, 0);
-            if (has) {
-                u = list[gi];
-                start = a.indices[u];
-                deg = a.indices[u + 1] - start;
-                own = AUX ? a.lv[u] : make_int2(a.labels[u], 0);
+            const int sl = has ? gi : 0;
+            int u = __shfl(pu, sl, WAVE), start = __shfl(pstart, sl, WAVE), deg = __shfl(pdeg, sl, WAVE);
+            int2 own = make_int2(__shfl(pown, sl, WAVE), __shfl(paux, sl, WAVE));
+            const CommStats own_rec{__shfl(pst.size, sl, WAVE), __shfl(pst.int_vol, sl, WAVE),
+                                    __shfl(pst.ext_vol, sl, WAVE), 0};
+            if (!has) {
+                u = start = deg = 0;
+                own = make_int2(0, 0);
             }
             int2 r = make_int2(INT32_MAX, 0);
             if (has && glane < deg) {
@@ -233,7 +258,7 @@ __global__ __launch_bounds__(256) void k_lp_small(Src src) {
             }
             if (has && glane == 0)
                 a.new_labels[u] = AUX ? lp_decide<KM>(a, deg, own.x, own.y, own_cnt, c)
-                                      : lp_decide_stats<KM>(a, deg, own.x, own_cnt, c);
+                                      : lp_decide_rec<KM>(a, deg, own.x, own_rec, own_cnt, c);
             if (a.counters) {
                 int ent = wave_sum((has && glane < deg) ? 1 : 0);
                 if (lane == 0) count3(a, __popcll(__ballot(has && glane == 0)), ent, __popcll(heads));
