@@ -80,9 +80,9 @@ def run_iterations(ctx, parcd, kind, lam, init, steps, warmup, seed, timers=True
 
 def lp_bin_bytes(active, entries, distinct, S):
     """Algorithmic bytes of one propagation bin kernel: per active vertex two
-    offsets, its own label, its community record and the new label; per
+    offsets, its own label, its community total and the new label; per
     entry the neighbour id and its label; per distinct neighbour label its
-    community record (S bytes: what the rule reads)."""
+    community total (S bytes: what the rule reads)."""
     return active * (8 + 4 + S + 4) + 8 * entries + S * distinct
 
 
@@ -596,7 +596,7 @@ def main():
         replay_identical = bool(np.array_equal(ctx.labels(), labels_timed)) and scores2 == scores
     log("replay identical:", replay_identical, "counters:", counters)
 
-    S = 8  # modularity reads int_volume + ext_volume per community record
+    S = 4  # modularity reads one 4-byte total (int_volume + ext_volume) per community
     per_kernel = {}
     for b, (a_, e_, d_) in counters.items():
         slot = SLOT_OF_BIN[b]
