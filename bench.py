@@ -46,21 +46,25 @@ def libc_srand(seed):
 
 
 def run_iterations(ctx, parcd, kind, lam, init, steps, warmup, seed, timers=True, counters=False):
-    """Refinement iterations of main.cpp:226-266 (plain path, -s 0)."""
+    """Refinement iterations of main.cpp:226-266 (plain path, -s 0), one
+    pcd_iterate call per pass (compress, statistics, metric, stop rule,
+    propagation).  -n is set past the last pass, so every pass propagates."""
     ctx.upload_labels(init)
     ctx.enable_timing(timers)
     ctx.enable_counters(False)
+    jb = parcd.job(kind=kind, lam=lam, frac=0.5, seed=seed, tie=parcd.TIE_RANDOM, max_iter=warmup + steps + 1)
     scores = []
     it = 0
+    prev = 0.0
 
-    def one(it):
-        ctx.compress()
-        ctx.statistics()
-        scores.append(ctx.metric(kind, lam))
-        ctx.propagate(kind, lam, frac=0.5, seed=seed, step=it, tie=parcd.TIE_RANDOM)
+    def one(it, prev):
+        score, k, done = ctx.iterate(jb, it, prev)
+        assert not done
+        scores.append(score)
+        return score
 
     for _ in range(warmup):
-        one(it)
+        prev = one(it, prev)
         it += 1
     ctx.synchronize()
     ctx.kernel_timer_reset()
@@ -70,7 +74,7 @@ def run_iterations(ctx, parcd, kind, lam, init, steps, warmup, seed, timers=True
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        one(it)
+        prev = one(it, prev)
         it += 1
     ctx.synchronize()
     t1 = time.perf_counter()

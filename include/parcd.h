@@ -208,6 +208,17 @@ typedef struct pcd_job {
 #define PCD_BATCH_DONE 1
 #define PCD_BATCH_FREE 2
 
+/* One pass of the main.cpp:226-266 loop body (main.cpp:229-260) on the
+ * context's graph and labels: compress, statistics, metric (*score), the stop
+ * rule of -n / -t against prev_score (*done), and, unless done, one
+ * propagation (-s 0; -s 2 on even it) or merging step numbered `it`.  *k
+ * receives the communities after the compression.  The propagation kernels
+ * are queued behind the metric's before the score is read back, and their
+ * result is dropped when the stop rule ends the loop, so the labels and
+ * statistics on return are those of a loop that never ran the step. */
+int pcd_iterate(pcd_ctx *ctx, const pcd_job *job, int strategy, int rounds, int it, double prev_score, double *score,
+                int *k, int *done);
+
 /* The whole main.cpp:226-266 loop on the context's graph and current labels
  * (-s strategy 0/1/2, -r rounds): on return the context holds the final
  * compressed labels and statistics; *score receives the last score (what

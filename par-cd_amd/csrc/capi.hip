@@ -140,6 +140,7 @@ extern "C" int pcd_ctx_create(int device, pcd_ctx **out) {
         (void)hipEventCreate(&ctx->ev0[i]);
         (void)hipEventCreate(&ctx->ev1[i]);
     }
+    if (hipEventCreateWithFlags(&ctx->ev_metric, hipEventDisableTiming) != hipSuccess) ctx->ev_metric = nullptr;
     if (hipHostMalloc((void **)&ctx->h_result, 64, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&ctx->h_small, 256, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&ctx->h_small64, 256, hipHostMallocDefault) != hipSuccess) {
@@ -227,6 +228,7 @@ extern "C" int pcd_ctx_destroy(pcd_ctx *ctx) {
         }
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->kt_open) (void)hipEventDestroy(ctx->kt_open);
+    if (ctx->ev_metric) (void)hipEventDestroy(ctx->ev_metric);
     if (ctx->h_result) (void)hipHostFree(ctx->h_result);
     if (ctx->h_small) (void)hipHostFree(ctx->h_small);
     if (ctx->h_small64) (void)hipHostFree(ctx->h_small64);
@@ -598,6 +600,89 @@ extern "C" int pcd_merging_step(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_
     PCD_TRY(merging_dev(ctx, rule, rounds, seed, step, tie));
     phase_end(ctx, PCD_PHASE_MERGE);
     ctx->labels_have_new = false;
+    return PCD_OK;
+}
+
+// One pass of the main.cpp:226-266 loop body.  With a propagation step due,
+// its kernels are launched right after the metric's, before the score is
+// back on the host; the host then waits on an event behind the metric only,
+// applies the stop rule, and either commits the step (statistics update and
+// label swap, queued behind the kernels) or drops it (the kernels wrote only
+// the spare label buffer and scratch).  When the statistics follow the
+// labels, k also stays on the device until that wait, so the GPU runs the
+// iterations back to back.
+extern "C" int pcd_iterate(pcd_ctx *ctx, const pcd_job *job, int strategy, int rounds, int it, double prev_score,
+                           double *score, int *num_labels, int *done) {
+    if (!ctx || !job) return fail(PCD_E_INVALID, "null argument");
+    if (strategy < 0 || strategy > 2) return fail(PCD_E_INVALID, "strategy must be 0, 1 or 2");
+    PCD_TRY(check_labels(ctx));
+    PCD_TRY(check_rule(job->rule));
+    const pcd_rule rule = job->rule;
+    const bool prop = strategy == 0 || (strategy == 2 && it % 2 == 0);
+    if (prop && !(job->frac > 0.0)) return fail(PCD_E_INVALID, "frac must be > 0");
+    if (!prop && rounds <= 0) return fail(PCD_E_INVALID, "rounds must be positive");
+    if (job->tie != PCD_TIE_RANDOM && job->tie != PCD_TIE_LOWEST) return fail(PCD_E_INVALID, "unknown tie order");
+    const int cap = job->iter_cap > 0 ? job->iter_cap : 1000;
+    if (rule.kind == PCD_RULE_WCC && !ctx->has_triangles) {
+        phase_begin(ctx, PCD_PHASE_TRIANGLES);
+        PCD_TRY(triangles_dev(ctx));
+        phase_end(ctx, PCD_PHASE_TRIANGLES);
+    }
+    // k can stay on the device when the compression renumbers statistics
+    // that follow the labels (no recount) and the metric reads k there
+    const bool defer = rule.kind != PCD_RULE_WCC && ctx->ev_metric && (ctx->stats_pending || ctx->has_stats);
+    phase_begin(ctx, PCD_PHASE_COMPRESS);
+    PCD_TRY(compress_labels_dev(ctx, defer));
+    phase_end(ctx, PCD_PHASE_COMPRESS);
+    ctx->labels_have_new = false;
+    phase_begin(ctx, PCD_PHASE_STATS);
+    if (!ctx->has_stats) PCD_TRY(collect_statistics_dev(ctx));
+    phase_end(ctx, PCD_PHASE_STATS);
+    // -n and the iteration cap decide without the score
+    const bool known_done = (job->max_iter != -1 && it >= job->max_iter) || it >= cap;
+    double s = 0.0;
+    bool launched = false;
+    if (rule.kind == PCD_RULE_WCC || !ctx->ev_metric) {
+        phase_begin(ctx, PCD_PHASE_METRIC);
+        PCD_TRY(metric_dev(ctx, rule, &s));
+        phase_end(ctx, PCD_PHASE_METRIC);
+    } else {
+        phase_begin(ctx, PCD_PHASE_METRIC);
+        PCD_TRY(metric_enqueue(ctx, rule));
+        phase_end(ctx, PCD_PHASE_METRIC);
+        PCD_HIP(hipEventRecord(ctx->ev_metric, ctx->stream));
+        if (prop && !known_done) {
+            phase_begin(ctx, PCD_PHASE_PROPAGATE);
+            PCD_TRY(propagation_launch(ctx, rule, job->frac, job->seed, it, job->tie));
+            launched = true;
+        }
+        PCD_HIP(hipEventSynchronize(ctx->ev_metric));
+        s = ctx->h_result[0];
+    }
+    resolve_k(ctx);  // the event is behind k's read-back too
+    const bool d = known_done ||
+                   (job->max_iter == -1 && prev_score > 0 && s > 0 && s / prev_score - 1 <= job->threshold);
+    if (!d) {
+        if (prop) {
+            if (!launched) {
+                phase_begin(ctx, PCD_PHASE_PROPAGATE);
+                PCD_TRY(propagation_launch(ctx, rule, job->frac, job->seed, it, job->tie));
+            }
+            PCD_TRY(propagation_commit(ctx));
+            phase_end(ctx, PCD_PHASE_PROPAGATE);
+            ctx->labels_have_new = true;
+        } else {
+            phase_begin(ctx, PCD_PHASE_MERGE);
+            PCD_TRY(merging_dev(ctx, rule, rounds, job->seed, it, job->tie));
+            phase_end(ctx, PCD_PHASE_MERGE);
+            ctx->labels_have_new = false;
+        }
+    } else if (launched) {
+        phase_end(ctx, PCD_PHASE_PROPAGATE);  // dropped: labels and statistics unchanged
+    }
+    if (score) *score = s;
+    if (num_labels) *num_labels = ctx->k;
+    if (done) *done = d ? 1 : 0;
     return PCD_OK;
 }
 

@@ -157,6 +157,8 @@ struct pcd_ctx {
     int dacc_cap = 0;
     double *d_result = nullptr;
     double *h_result = nullptr;      // pinned
+    bool k_deferred = false;         // k is on the device; h_small[K_SLOT] receives it
+    hipEvent_t ev_metric = nullptr;  // recorded after the score's read-back
     int32_t *h_small = nullptr;      // pinned scratch ints
     int64_t *h_small64 = nullptr;    // pinned scratch int64
 
@@ -218,12 +220,23 @@ int ensure_scan_parts(pcd_ctx *ctx, int64_t n);
 int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int bits, uint64_t **out);
 
 // phases (partition.hip, propagate.hip, merge.hip, triangles.hip)
-int compress_labels_dev(pcd_ctx *ctx);
+// defer_k: k stays on the device (d_k) and its read-back lands in pinned
+// memory; resolve_k takes it once the stream has passed the copy.
+int compress_labels_dev(pcd_ctx *ctx, bool defer_k = false);
+void resolve_k(pcd_ctx *ctx);
 int collect_statistics_dev(pcd_ctx *ctx);
 int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *after);
 int clustering_labels_dev(pcd_ctx *ctx, int *num_communities);
 int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out);
+// modularity, CPM, conductance: kernels and the score's copy into
+// ctx->h_result, no wait
+int metric_enqueue(pcd_ctx *ctx, pcd_rule rule);
 int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie);
+// propagation_dev in two parts: the kernels that write the next labels into
+// d_labels_alt, then the statistics update and the swap that make them the
+// current labels (a launch that is not committed changes nothing else)
+int propagation_launch(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie);
+int propagation_commit(pcd_ctx *ctx);
 int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie);
 int contract_dev(pcd_ctx *ctx, int64_t *num_pairs);
 int contract_download(pcd_ctx *ctx, int64_t U, int32_t *l, int32_t *k, int32_t *links);

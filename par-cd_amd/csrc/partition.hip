@@ -54,7 +54,17 @@ __global__ void k_relabel(int n, const int32_t *labels, const int32_t *newid, in
 
 __global__ void k_copy_i32(int32_t *dst, const int32_t *src) { *dst = *src; }
 
-int compress_labels_dev(pcd_ctx *ctx) {
+// slot of the pinned scratch ints that receives a deferred k
+constexpr int K_SLOT = 8;
+
+void resolve_k(pcd_ctx *ctx) {
+    if (!ctx->k_deferred) return;
+    ctx->k = ctx->h_small[K_SLOT];
+    ctx->label_bound = ctx->k;
+    ctx->k_deferred = false;
+}
+
+int compress_labels_dev(pcd_ctx *ctx, bool defer_k) {
     const int n = ctx->n;
     kt_begin(ctx, KS_COMPRESS);
     if (ctx->label_bound > 0) PCD_TRY(fill_i32(ctx, ctx->d_first, INT32_MAX, ctx->label_bound));
@@ -73,12 +83,18 @@ int compress_labels_dev(pcd_ctx *ctx) {
     hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(1), 0, ctx->stream, ctx->d_k, total);
     kt_end(ctx, KS_COMPRESS);
     PCD_HIP(hipGetLastError());
-    PCD_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_k, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    if (defer_k) {
+        PCD_HIP(hipMemcpyAsync(ctx->h_small + K_SLOT, ctx->d_k, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->k_deferred = true;
+    } else {
+        PCD_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_k, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        PCD_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->k = ctx->h_small[0];
+        ctx->label_bound = ctx->k;
+        ctx->k_deferred = false;
+    }
     std::swap(ctx->d_labels, ctx->d_labels_alt);
     if (renumber) std::swap(ctx->d_stats, ctx->d_stats_alt);
-    ctx->k = ctx->h_small[0];
-    ctx->label_bound = ctx->k;
     ctx->labels_compressed = true;
     ctx->has_stats = renumber;
     ctx->stats_pending = false;
@@ -520,6 +536,13 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *part, int np
 
 int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out) {
     if (rule.kind == PCD_RULE_WCC) return wcc_metric_dev(ctx, out);
+    PCD_TRY(metric_enqueue(ctx, rule));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ctx->h_result[0];
+    return PCD_OK;
+}
+
+int metric_enqueue(pcd_ctx *ctx, pcd_rule rule) {
     kt_begin(ctx, KS_METRIC);
     if (rule.kind == PCD_RULE_CONDUCTANCE) {
         hipLaunchKernelGGL(k_cond_partial, dim3(FSUM_BLOCKS), dim3(256), 0, ctx->stream, ctx->d_stats, ctx->d_k,
@@ -529,7 +552,9 @@ int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out) {
                            (double)ctx->n, ctx->d_result);
     } else {
         PCD_TRY(fill_u64(ctx, (uint64_t *)ctx->d_acc, 0, 4));
-        hipLaunchKernelGGL(k_metric_sums, dim3(grid_for(ctx, ctx->k, 256 * 8, 2)), dim3(256), 0, ctx->stream,
+        // the kernel reads k from d_k; a deferred k is bounded by n + 1
+        const int kg = ctx->k_deferred ? ctx->n + 1 : ctx->k;
+        hipLaunchKernelGGL(k_metric_sums, dim3(grid_for(ctx, kg, 256 * 8, 2)), dim3(256), 0, ctx->stream,
                            ctx->d_stats, ctx->d_k, (unsigned long long *)ctx->d_acc);
         hipLaunchKernelGGL(k_metric_final, dim3(1), dim3(1), 0, ctx->stream, (const unsigned long long *)ctx->d_acc,
                            rule.kind, rule.lambda, ctx->m, ctx->d_result);
@@ -537,8 +562,6 @@ int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out) {
     kt_end(ctx, KS_METRIC);
     PCD_HIP(hipGetLastError());
     PCD_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    PCD_HIP(hipStreamSynchronize(ctx->stream));
-    *out = ctx->h_result[0];
     return PCD_OK;
 }
 

@@ -152,7 +152,8 @@ __global__ void k_lp_records(int n, const int32_t *labels, const CommStats *st, 
 // Per community the one total the gains of modularity (int_volume +
 // ext_volume) or CPM (size) read: 4 bytes per community instead of the
 // 16-byte record, so the random reads of the step hit a 4x smaller array.
-__global__ void k_label_totals(int k, const CommStats *st, int kind, int32_t *tot) {
+__global__ void k_label_totals(const int32_t *kp, const CommStats *st, int kind, int32_t *tot) {
+    const int k = *kp;
     for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) {
         CommStats c = st[l];
         tot[l] = kind == PCD_RULE_MODULARITY ? c.int_vol + c.ext_vol : c.size;
@@ -898,7 +899,9 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
     }
     if (a.use_aux) {  // the modularity and CPM builds read the totals
         kt_begin(ctx, KS_LP_SELECT);
-        hipLaunchKernelGGL(k_label_totals, dim3(grid_for(ctx, ctx->k, 256, 8)), dim3(256), 0, ctx->stream, ctx->k,
+        // k is read from d_k (a deferred k is bounded by n + 1)
+        const int kg = ctx->k_deferred ? ctx->n + 1 : ctx->k;
+        hipLaunchKernelGGL(k_label_totals, dim3(grid_for(ctx, kg, 256, 8)), dim3(256), 0, ctx->stream, ctx->d_k,
                            ctx->d_stats, a.rule.kind, ctx->d_tot);
         kt_end(ctx, KS_LP_SELECT);
     }
@@ -1028,6 +1031,11 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
 }
 
 int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie) {
+    PCD_TRY(propagation_launch(ctx, rule, frac, seed, step, tie));
+    return propagation_commit(ctx);
+}
+
+int propagation_launch(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie) {
     double fr = frac < 1.0 ? frac : 1.0;
     LPArgs a;
     a.indices = ctx->d_indices;
@@ -1050,6 +1058,14 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
     if (rule.kind == PCD_RULE_MODULARITY) PCD_TRY(launch_propagation<RK_MOD>(ctx, a));
     else if (rule.kind == PCD_RULE_CPM) PCD_TRY(launch_propagation<RK_CPM>(ctx, a));
     else PCD_TRY(launch_propagation<RK_ALL>(ctx, a));
+    return PCD_OK;
+}
+
+int propagation_commit(pcd_ctx *ctx) {
+    if (ctx->k_deferred) {  // label_bound below needs k on the host
+        PCD_HIP(hipStreamSynchronize(ctx->stream));
+        resolve_k(ctx);
+    }
     // the statistics follow the moved vertices; the next compression
     // renumbers them
     const bool follow = ctx->tune.inc_stats != 0;

@@ -1,7 +1,7 @@
-// The refinement loop of main.cpp:226-266 as one library call, over the
-// public entry points: compress, statistics, metric, stop rule, then one
-// propagation (-s 0), merging (-s 1) or alternating (-s 2) step.  One read-back
-// per phase as in pcd_main; no host allocation inside the loop.
+// The refinement loop of main.cpp:226-266 as one library call: pcd_iterate
+// per pass (compress, statistics, metric, stop rule, then one propagation
+// (-s 0), merging (-s 1) or alternating (-s 2) step).  No host allocation
+// inside the loop.
 #include "parcd.h"
 #include "hostutil.hpp"
 
@@ -15,20 +15,9 @@ extern "C" int pcd_refine(pcd_ctx *ctx, const pcd_job *job, int strategy, int ro
     double prev = 0.0, score = 0.0;
     int it = 0, k = 0;
     for (;;) {
-        int rc = pcd_compress_labels(ctx, &k);
-        if (rc == PCD_OK) rc = pcd_collect_statistics(ctx, &k);
-        if (rc == PCD_OK) rc = pcd_metric(ctx, job->rule, &score);
+        int done = 0;
+        int rc = pcd_iterate(ctx, job, strategy, rounds, it, prev, &score, &k, &done);
         if (rc != PCD_OK) return rc;
-        bool done = job->max_iter != -1 ? it >= job->max_iter
-                                        : (prev > 0 && score > 0 && score / prev - 1 <= job->threshold);
-        done = done || it >= cap;
-        if (!done) {
-            if (strategy == 0 || (strategy == 2 && it % 2 == 0))
-                rc = pcd_propagation_step(ctx, job->rule, job->frac, job->seed, it, job->tie);
-            else
-                rc = pcd_merging_step(ctx, job->rule, rounds, job->seed, it, job->tie);
-            if (rc != PCD_OK) return rc;
-        }
         it++;
         prev = score;
         if (done) break;

@@ -32,7 +32,7 @@ EXPORTS = [
     "pcd_kernel_timer_select",
     "pcd_batch_status", "pcd_batch_finished", "pcd_batch_labels", "pcd_batch_stats", "pcd_batch_release",
     "pcd_batch_info", "pcd_batch_map_count", "pcd_batch_map_name", "pcd_batch_set_map", "pcd_batch_autotune",
-    "pcd_gen_sbm_batch", "pcd_refine", "pcd_host_alloc", "pcd_host_free", "pcd_clustering_partition",
+    "pcd_gen_sbm_batch", "pcd_refine", "pcd_iterate", "pcd_host_alloc", "pcd_host_free", "pcd_clustering_partition",
     "pcd_server_create", "pcd_server_destroy", "pcd_server_submit", "pcd_server_wait", "pcd_server_result",
     "pcd_server_info", "pcd_gen_requests",
 ]
@@ -168,6 +168,7 @@ def lib():
         [C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.POINTER(C.c_int64)), C.POINTER(C.POINTER(C.c_int64)),
          C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.POINTER(C.c_double))]
     L.pcd_refine.argtypes = [vp, C.POINTER(Job), C.c_int, C.c_int, dp, ip, ip]
+    L.pcd_iterate.argtypes = [vp, C.POINTER(Job), C.c_int, C.c_int, C.c_int, C.c_double, dp, ip, ip]
     L.pcd_gen_sbm_batch.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + \
         [C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.POINTER(C.c_int64)), C.POINTER(C.POINTER(C.c_int64)),
          C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.POINTER(C.c_double))]
@@ -455,6 +456,14 @@ class Context:
         sc, it, k = C.c_double(), C.c_int(), C.c_int()
         check(lib().pcd_refine(self._ctx, C.byref(job_), strategy, rounds, C.byref(sc), C.byref(it), C.byref(k)))
         return sc.value, it.value, k.value
+
+    def iterate(self, job_, it, prev_score, strategy=0, rounds=1):
+        """pcd_iterate: one pass of the refinement loop numbered `it`:
+        (score, k, done)."""
+        sc, k, d = C.c_double(), C.c_int(), C.c_int()
+        check(lib().pcd_iterate(self._ctx, C.byref(job_), strategy, rounds, int(it), float(prev_score), C.byref(sc),
+                                C.byref(k), C.byref(d)))
+        return sc.value, k.value, bool(d.value)
 
     def merge(self, kind, lam=0.0, rounds=1, seed=0, step=0, tie=TIE_RANDOM):
         check(lib().pcd_merging_step(self._ctx, rule(kind, lam), rounds, seed, step, tie))

@@ -498,3 +498,48 @@ def test_chung_lu_wcc_benchmark_iterations(ctx):
         want = O.propagation_refine(O.WCC, 0.0, n, m, ind, adj, lab_o, so, ivo, evo, cc, 0.5, 1, 1, it, O.TIE_RANDOM)
         assert np.array_equal(got, want), (it, int(np.count_nonzero(got != want)))
         prev = got
+
+
+@pytest.mark.parametrize("kind,lam,max_iter", [(O.MOD, 0.0, -1), (O.CPM, 0.02, -1), (O.WCC, 0.0, -1),
+                                               (O.MOD, 0.0, 3)])
+def test_iterate_equals_stepwise_calls(kind, lam, max_iter):
+    """pcd_iterate (the loop body with the propagation queued before the
+    score is read back and dropped when the stop rule ends the loop) gives
+    the scores, k and stop decision of the separate compress / statistics /
+    metric / propagation calls, pass by pass; after the loop the labels and
+    statistics are those of the stepwise loop, and one more step from there
+    agrees too."""
+    for g in [GG.bin_zoo(), GOLDEN["golden_sbm200"]]:
+        n, m, ind, adj = g[:4]
+        cap = 25  # the iteration cap ends runs that keep improving (WCC here)
+        jb = parcd.job(kind=kind, lam=lam, frac=0.5, seed=5, tie=O.TIE_RANDOM, max_iter=max_iter, iter_cap=cap)
+        with parcd.Context(0) as a, parcd.Context(0) as b:
+            for c in (a, b):
+                c.upload_graph(n, m, ind, adj)
+                c.upload_labels(GG.singleton_labels(n))
+            if kind == O.WCC:
+                b.triangles()
+            prev_a = prev_b = 0.0
+            it = 0
+            while True:
+                sa, ka, da = a.iterate(jb, it, prev_a)
+                kb = b.compress()
+                b.statistics()
+                sb = b.metric(kind, lam)
+                db = (max_iter != -1 and it >= max_iter) or it >= cap or (
+                    max_iter == -1 and prev_b > 0 and sb > 0 and sb / prev_b - 1 <= 0.0)
+                assert (sa, ka, da) == (sb, kb, db), it
+                if not db:
+                    b.propagate(kind, lam, frac=0.5, seed=5, step=it, tie=O.TIE_RANDOM)
+                it += 1
+                prev_a, prev_b = sa, sb
+                if da:
+                    break
+            assert np.array_equal(a.labels(), b.labels())
+            k = a.statistics()
+            assert k == b.statistics()
+            for x, y in zip(a.stats(k), b.stats(k)):
+                assert np.array_equal(x, y)
+            for c in (a, b):
+                c.propagate(kind, lam, frac=0.5, seed=5, step=it, tie=O.TIE_RANDOM)
+            assert np.array_equal(a.labels(), b.labels())
