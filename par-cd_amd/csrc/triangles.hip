@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void k_triangles(int n, const int32_t *indices
 // flattened over the lanes (an exclusive scan of the per-r lengths in LDS,
 // a binary search of the lane's item in it), so a wave stays busy however
 // short the rows are.  Same checks and counts as k_triangles.
-constexpr int FLAT_TS = 512;
+constexpr int FLAT_TS = 256;  // 16 KB of LDS per workgroup: eight workgroups (32 waves) per CU
 
 template <int TS>
 __global__ __launch_bounds__(256) void k_triangles_flat(int n, const int32_t *indices, const int32_t *adj,
@@ -174,6 +174,44 @@ __global__ __launch_bounds__(256) void k_triangles_flat(int n, const int32_t *in
 
 // graph.cpp:189-202 (rules.cpp:273-287): per entry (v,u) with t > 0,
 // t is added to both endpoints and each endpoint counts one edge.
+// Prefixes of at most ACC_G * 8 entries: ACC_G lanes per vertex, 64 / ACC_G
+// vertices per wave; longer prefixes: k_tri_accumulate, one wave each.
+constexpr int ACC_G = 8, ACC_SMALL = ACC_G * 8;
+__global__ __launch_bounds__(256) void k_tri_accumulate_small(int n, const int32_t *indices, const int32_t *adj,
+                                                              const int32_t *lowpos, const int32_t *et, int32_t *tri,
+                                                              int32_t *tdeg) {
+    const int lane = __lane_id(), glane = lane & (ACC_G - 1);
+    const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / ACC_G;
+    const int ngroups = gridDim.x * blockDim.x / ACC_G;
+    // the lanes of a group share v, so the group's shuffles stay inside
+    // lanes that are active together
+    for (int v = gid; v < n; v += ngroups) {
+        const int lenv = lowpos[v];
+        const bool mine = lenv <= ACC_SMALL;
+        const int sv = mine ? indices[v] : 0;
+        int ts = 0, tc = 0;
+        for (int i = glane; mine && i < lenv; i += ACC_G) {
+            int t = et[sv + i];
+            if (t > 0) {
+                int u = adj[sv + i];
+                ts += t;
+                tc++;
+                atomicAdd(&tri[u], t);
+                atomicAdd(&tdeg[u], 1);
+            }
+        }
+#pragma unroll
+        for (int d = ACC_G / 2; d > 0; d >>= 1) {
+            ts += __shfl_xor(ts, d, WAVE);
+            tc += __shfl_xor(tc, d, WAVE);
+        }
+        if (glane == 0 && tc) {
+            atomicAdd(&tri[v], ts);
+            atomicAdd(&tdeg[v], tc);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_tri_accumulate(int n, const int32_t *indices, const int32_t *adj,
                                                         const int32_t *lowpos, const int32_t *et, int32_t *tri,
                                                         int32_t *tdeg) {
@@ -181,7 +219,9 @@ __global__ __launch_bounds__(256) void k_tri_accumulate(int n, const int32_t *in
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int nwaves = gridDim.x * blockDim.x / WAVE;
     for (int v = wave_g; v < n; v += nwaves) {
-        const int sv = indices[v], lenv = lowpos[v];
+        const int lenv = lowpos[v];
+        if (lenv <= ACC_SMALL) continue;  // k_tri_accumulate_small
+        const int sv = indices[v];
         int ts = 0, tc = 0;
         for (int i = lane; i < lenv; i += WAVE) {
             int t = et[sv + i];
@@ -286,10 +326,12 @@ static int count_pass(pcd_ctx *ctx, const int32_t *labels, int32_t *t, int32_t *
     PCD_TRY(fill_i32(ctx, ctx->d_et, 0, E));
     PCD_TRY(fill_i32(ctx, t, 0, n));
     PCD_TRY(fill_i32(ctx, td, 0, n));
-    hipLaunchKernelGGL(k_triangles_flat<FLAT_TS>, dim3(grid_for(ctx, n, 4, 4)), dim3(256), 0, ctx->stream, n,
+    hipLaunchKernelGGL(k_triangles_flat<FLAT_TS>, dim3(grid_for(ctx, n, 4, 8)), dim3(256), 0, ctx->stream, n,
                        ctx->d_indices, ctx->d_adj, ctx->d_lowpos, labels, ctx->d_et);
     hipLaunchKernelGGL(k_triangles, dim3(grid_for(ctx, n, 4, 2)), dim3(256), 0, ctx->stream, n, ctx->d_indices,
                        ctx->d_adj, ctx->d_lowpos, labels, ctx->d_et, FLAT_TS);
+    hipLaunchKernelGGL(k_tri_accumulate_small, dim3(grid_for(ctx, n, 256 / ACC_G, 8)), dim3(256), 0, ctx->stream, n,
+                       ctx->d_indices, ctx->d_adj, ctx->d_lowpos, ctx->d_et, t, td);
     hipLaunchKernelGGL(k_tri_accumulate, dim3(grid_for(ctx, n, 4, 8)), dim3(256), 0, ctx->stream, n, ctx->d_indices,
                        ctx->d_adj, ctx->d_lowpos, ctx->d_et, t, td);
     kt_end(ctx, KS_TRIANGLES);
