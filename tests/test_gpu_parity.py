@@ -431,14 +431,15 @@ def test_rmat_large_properties(ctx):
     _lp_compare(ctx, g, rng.integers(0, 5000, n), O.CPM, 0.01, 0.5, seed=2, step=5, tie=O.TIE_LOWEST)
 
 
-def test_rmat22_benchmark_iterations(ctx):
-    """The benchmark's own workload at full size (R-MAT scale 22 seed 2,
-    2.4 M vertices, 128 M entries): its first three refinement iterations from
-    the singleton partition (frac 0.5, seed 1, random ties).  Each
-    compression, each statistics table (recounted in the first iteration,
-    then updated from the moved rows and renumbered by the compression), each
-    modularity value and each propagation step equals the oracle's on the
-    same inputs."""
+@pytest.mark.parametrize("kind,lam", [(O.MOD, 0.0), (O.CPM, 0.01)])
+def test_rmat22_benchmark_iterations(ctx, kind, lam):
+    """The benchmark's own workloads at full size (R-MAT scale 22 seed 2,
+    2.4 M vertices, 128 M entries; modularity and CPM 0.01): their first three
+    refinement iterations from the singleton partition (frac 0.5, seed 1,
+    random ties).  Each compression, each statistics table (recounted in the
+    first iteration, then updated from the moved rows and renumbered by the
+    compression), each metric value and each propagation step equals the
+    oracle's on the same inputs."""
     g = GG.rmat(22)
     n, m, ind, adj = g[:4]
     ctx.upload_graph(n, m, ind, adj)
@@ -454,11 +455,11 @@ def test_rmat22_benchmark_iterations(ctx):
         _, so, ivo, evo = O.collect_statistics(n, ind, adj, lab_o)
         assert np.array_equal(s, so) and np.array_equal(iv, ivo) and np.array_equal(ev, evo), it
         assert int(so.sum()) == n and int(ivo.sum() + evo.sum()) == 2 * m
-        q = ctx.metric(O.MOD, 0.0)
-        assert q == pytest.approx(O.calculate(O.MOD, 0.0, n, m, ind, adj, lab_o, so, ivo, evo), rel=1e-9, abs=1e-12)
-        ctx.propagate(O.MOD, 0.0, frac=0.5, seed=1, step=it, tie=O.TIE_RANDOM)
+        q = ctx.metric(kind, lam)
+        assert q == pytest.approx(O.calculate(kind, lam, n, m, ind, adj, lab_o, so, ivo, evo), rel=1e-9, abs=1e-12)
+        ctx.propagate(kind, lam, frac=0.5, seed=1, step=it, tie=O.TIE_RANDOM)
         got = ctx.labels()
-        want = O.propagation_refine(O.MOD, 0.0, n, m, ind, adj, lab_o, so, ivo, evo, 0.0, 0.5, 1, 1, it, O.TIE_RANDOM)
+        want = O.propagation_refine(kind, lam, n, m, ind, adj, lab_o, so, ivo, evo, 0.0, 0.5, 1, 1, it, O.TIE_RANDOM)
         assert np.array_equal(got, want), (it, int(np.count_nonzero(got != want)))
         prev = got
 
