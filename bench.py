@@ -523,7 +523,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--serve", action="store_true",
-                    help="run the serving leg (held off by default: see DESIGN.md, serving form)")
+                    help="run the serving leg (off by default while an unexplained GPU fault in a run of the serving tests has no shown cause)")
     ap.add_argument("--no-serve", action="store_true", help=argparse.SUPPRESS)  # the default; kept for old commands
     ap.add_argument("--no-wcc", action="store_true", help="skip the WCC / Chung-Lu leg")
     ap.add_argument("--no-merge", action="store_true", help="skip the merging (-s 1) leg")

@@ -9,11 +9,11 @@ import pytest
 import oracle_lib as O
 from gpu_graphs import bin_zoo, rmat, sbm, singleton_labels
 
-# A GPU call that began with these tests ended with the box faulted before
-# any output (no cause found in the code): kept off the GPU
-# until a cause is shown.
+# A run that began with these tests ended with the machine's GPU faulted
+# before any output, and reading the serving code found no cause: the tests
+# stay skipped until a cause is shown.
 pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="serving path held off the GPU after an unexplained box fault")]
+              pytest.mark.skip(reason="serving path held off the GPU after an unexplained GPU fault")]
 parcd = pytest.importorskip("parcd")
 
 
