@@ -424,7 +424,7 @@ __device__ inline int ceil_log2(int x) { return x <= 1 ? 0 : 32 - __clz(x - 1); 
 
 // ------------------------------------------------------------- wave / vertex
 // The table starts empty and each row's scan clears the slots it used.
-template <int TW, unsigned KM, class Src, bool AUX = false>
+template <int TW, unsigned KM, class Src, bool AUX = false, int PF = 4>
 __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
     __shared__ int s_keys[256 / WAVE][TW];
     __shared__ int s_cnts[256 / WAVE][TW];
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
         int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
         int own = ownr.x;
         int log_t = max(6, ceil_log2(2 * deg));
-        insert_labels<4, AUX>(a, keys, cnts, auxs, occ, nocc, (1 << log_t) - 1, log_t, start, deg, lane, WAVE);
+        insert_labels<PF, AUX>(a, keys, cnts, auxs, occ, nocc, (1 << log_t) - 1, log_t, start, deg, lane, WAVE);
         wave_sync();
         const int D = *nocc;
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
 
 // ------------------------------------------------------------- block / vertex
 // Table slots: nextpow2(2 deg), at most TB.
-template <int NT, int TB, unsigned KM, class Src, bool AUX = false>
+template <int NT, int TB, unsigned KM, class Src, bool AUX = false, int PF = 4>
 __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
     __shared__ int keys[TB];
     __shared__ int cnts[TB];
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
         int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
         int own = ownr.x;
         int log_t = max(6, ceil_log2(2 * deg));
-        insert_labels<4, AUX>(a, keys, cnts, auxs, occ, &s_nocc, (1 << log_t) - 1, log_t, start, deg, threadIdx.x,
+        insert_labels<PF, AUX>(a, keys, cnts, auxs, occ, &s_nocc, (1 << log_t) - 1, log_t, start, deg, threadIdx.x,
                               NT);
         __syncthreads();
         const int D = s_nocc;
@@ -993,9 +993,9 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
             if (block_aux)
                 hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - ps, 4, 3)),
                                    dim3(256), 0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B1024)});
-            else
-                hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc>), dim3(grid_for(ctx, p1 - ps, 4, 4)), dim3(256),
-                                   0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B1024)});
+            else  // 4-8 entries per lane: all eight gathers in flight at once
+                hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8>), dim3(grid_for(ctx, p1 - ps, 4, 4)),
+                                   dim3(256), 0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B1024)});
         }
         kt_end(ctx, KS_LP_B1024);
     }
