@@ -544,3 +544,64 @@ def test_iterate_equals_stepwise_calls(kind, lam, max_iter):
             for c in (a, b):
                 c.propagate(kind, lam, frac=0.5, seed=5, step=it, tie=O.TIE_RANDOM)
             assert np.array_equal(a.labels(), b.labels())
+
+
+def _csr_direct(n, edges):
+    """CSR with the given vertex count: isolated vertices stay (what a PCD
+    file may hold; the edge-list converter drops them)."""
+    nb = [set() for _ in range(n)]
+    for a, b in edges:
+        if a != b:
+            nb[a].add(b)
+            nb[b].add(a)
+    ind = np.zeros(n + 1, np.int32)
+    for v in range(n):
+        ind[v + 1] = ind[v] + len(nb[v])
+    adj = np.array([u for v in range(n) for u in sorted(nb[v])], np.int32)
+    return n, int(ind[-1]) // 2, ind, adj, np.arange(n, dtype=np.int32)
+
+
+TINY = {
+    "no_edges": (6, []),
+    "single_vertex": (1, []),
+    "edge_and_isolated": (4, [(0, 1)]),
+    "path_and_isolated": (7, [(0, 1), (1, 2), (2, 3)]),
+    "star": (9, [(0, i) for i in range(1, 9)]),
+    "clique5": (5, [(i, j) for i in range(5) for j in range(i + 1, 5)]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(TINY))
+def test_degenerate_graphs(ctx, name):
+    """Graphs without edges, a single vertex, isolated vertices, a star and
+    a clique: compression, statistics, every metric (NaN and -inf where m or
+    a community volume is 0, as the reference divides), one propagation step
+    of every rule, the bounded -s 0 / -s 1 loops (stepwise and pcd_refine),
+    triangles and the -i 1 partition equal the oracle."""
+    g = _csr_direct(*TINY[name])
+    n, m, ind, adj = g[:4]
+    et, tri_o, td_o, cc_o = O.count_triangles(n, ind, adj)
+    rng = np.random.default_rng(1)
+    for lab in [GG.singleton_labels(n), np.zeros(n, np.int32), rng.integers(0, 3, n).astype(np.int32)]:
+        lab_o, so, ivo, evo = check_partition(ctx, g, lab)
+        got = ctx.metric(O.WCC)
+        want = O.calculate(O.WCC, 0, n, m, ind, adj, lab_o, so, ivo, evo, tri_o, td_o)
+        assert (np.isnan(got) and np.isnan(want)) or got == pytest.approx(want, rel=1e-9, abs=1e-12)
+        for kind, lam, c in [(O.MOD, 0.0, None), (O.CPM, 0.05, None), (O.WCC, 0.0, cc_o), (O.COND, 0.0, None)]:
+            _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=1, step=0, tie=O.TIE_RANDOM, cc=c)
+            _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=2, step=1, tie=O.TIE_LOWEST, cc=c)
+    upload(ctx, g)
+    tri, td, cc = ctx.triangles()
+    assert np.array_equal(tri, tri_o[:n]) and np.array_equal(td, td_o[:n]) and cc == pytest.approx(cc_o, abs=1e-12)
+    ctx.clustering()
+    assert np.array_equal(ctx.labels(), O.clustering_partition(n, ind, adj, tri_o))
+    for strategy in (0, 1):
+        init = GG.singleton_labels(n)
+        lab, score, iters, k, trace = gpu_run(ctx, g, O.MOD, 0.0, strategy, 4, 3, O.TIE_LOWEST, init=init)
+        lab_o, score_o, iters_o, k_o, trace_o = O.run(n, m, ind, adj, kind=O.MOD, strategy=strategy, mode=1, seed=3,
+                                                      tie=O.TIE_LOWEST, init_labels=init, max_iter=4)
+        assert np.array_equal(lab, lab_o) and iters == iters_o and k == k_o
+        upload(ctx, g)
+        ctx.upload_labels(init)
+        sc, it, kk = ctx.refine(parcd.job(kind=O.MOD, seed=3, tie=O.TIE_LOWEST, max_iter=4), strategy=strategy)
+        assert np.array_equal(ctx.labels(), lab_o) and it == iters_o and kk == k_o
