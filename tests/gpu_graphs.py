@@ -15,6 +15,32 @@ def csr_from_edges(a, b):
     return O.convert_edges(nn, a, b)
 
 
+def csr_direct(n, edges):
+    """CSR with the given vertex count: isolated vertices stay (what a PCD
+    file may hold; the edge-list converter drops them)."""
+    nb = [set() for _ in range(n)]
+    for a, b in edges:
+        if a != b:
+            nb[a].add(b)
+            nb[b].add(a)
+    ind = np.zeros(n + 1, np.int32)
+    for v in range(n):
+        ind[v + 1] = ind[v] + len(nb[v])
+    adj = np.array([u for v in range(n) for u in sorted(nb[v])], np.int32)
+    return n, int(ind[-1]) // 2, ind, adj, np.arange(n, dtype=np.int32)
+
+
+# small graphs at the edges of the input domain: (n, edge list)
+TINY = {
+    "no_edges": (6, []),
+    "single_vertex": (1, []),
+    "edge_and_isolated": (4, [(0, 1)]),
+    "path_and_isolated": (7, [(0, 1), (1, 2), (2, 3)]),
+    "star": (9, [(0, i) for i in range(1, 9)]),
+    "clique5": (5, [(i, j) for i in range(5) for j in range(i + 1, 5)]),
+}
+
+
 @functools.lru_cache(maxsize=None)
 def bin_zoo(seed=7):
     """Degrees spanning every bin: hubs above 4096, a 1024-4096 block range,

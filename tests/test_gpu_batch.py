@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from gpu_graphs import rmat, sbm, singleton_labels, unsorted_copy
+from gpu_graphs import TINY, csr_direct, rmat, sbm, singleton_labels, unsorted_copy
 
 pytestmark = pytest.mark.gpu
 parcd = pytest.importorskip("parcd")
@@ -308,3 +308,30 @@ def test_refine_call_equals_replay(strategy):
                                 init_labels=lab)
     assert np.array_equal(got, ol) and iters == oit and k == ok
     assert abs(score - osc) <= 1e-9 * max(1.0, abs(osc))
+
+
+def test_batch_tiny_graphs_with_isolated_vertices():
+    """Graphs with isolated vertices, a star and a clique end where they end
+    alone and in the replay; a graph without edges is refused (batches take
+    m >= 1) and leaves the others running."""
+    names = ["edge_and_isolated", "path_and_isolated", "star", "clique5"]
+    gs = [csr_direct(*TINY[k]) for k in names]
+    jobs = [dict(kind=O.MOD, seed=1, max_iter=4), dict(kind=O.CPM, lam=0.05, seed=2, max_iter=4),
+            dict(kind=O.MOD, seed=3, tie=O.TIE_LOWEST, max_iter=4), dict(kind=O.CPM, lam=0.3, seed=4, max_iter=4)]
+    items = [(g, singleton_labels(g[0]), parcd.job(**jb)) for g, jb in zip(gs, jobs)]
+    with parcd.Batch(0, max_graphs=8, arena_bytes=64 << 20) as b:
+        e = csr_direct(*TINY["no_edges"])
+        with pytest.raises(parcd.PcdError):
+            b.add(e[0], e[1], e[2], e[3], singleton_labels(e[0]), parcd.job(kind=O.MOD, max_iter=2))
+        slots = [b.add(g[0], g[1], g[2], g[3], lab, jb) for g, lab, jb in items]
+        while b.step(1):
+            pass
+        res = [(b.labels(s),) + b.status(s)[1:] + (b.status(s)[0],) for s in slots]
+    for (g, lab, jb), (bl, iters, k, score, status), jd in zip(items, res, jobs):
+        assert status == parcd.BATCH_DONE
+        pl, ps, pit, pk = plain(g, lab, jb)
+        assert np.array_equal(bl, pl) and iters == pit and k == pk and score == ps
+        n, m, ind, adj, _ = g
+        ol, osc, oit, ok, _ = O.run(n, m, ind, adj, kind=jd["kind"], lam=jd.get("lam", 0.0), max_iter=4, mode=1,
+                                    seed=jd["seed"], tie=jd.get("tie", O.TIE_RANDOM), init_labels=lab)
+        assert np.array_equal(bl, ol) and iters == oit and k == ok

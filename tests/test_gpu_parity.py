@@ -546,39 +546,14 @@ def test_iterate_equals_stepwise_calls(kind, lam, max_iter):
             assert np.array_equal(a.labels(), b.labels())
 
 
-def _csr_direct(n, edges):
-    """CSR with the given vertex count: isolated vertices stay (what a PCD
-    file may hold; the edge-list converter drops them)."""
-    nb = [set() for _ in range(n)]
-    for a, b in edges:
-        if a != b:
-            nb[a].add(b)
-            nb[b].add(a)
-    ind = np.zeros(n + 1, np.int32)
-    for v in range(n):
-        ind[v + 1] = ind[v] + len(nb[v])
-    adj = np.array([u for v in range(n) for u in sorted(nb[v])], np.int32)
-    return n, int(ind[-1]) // 2, ind, adj, np.arange(n, dtype=np.int32)
-
-
-TINY = {
-    "no_edges": (6, []),
-    "single_vertex": (1, []),
-    "edge_and_isolated": (4, [(0, 1)]),
-    "path_and_isolated": (7, [(0, 1), (1, 2), (2, 3)]),
-    "star": (9, [(0, i) for i in range(1, 9)]),
-    "clique5": (5, [(i, j) for i in range(5) for j in range(i + 1, 5)]),
-}
-
-
-@pytest.mark.parametrize("name", sorted(TINY))
+@pytest.mark.parametrize("name", sorted(GG.TINY))
 def test_degenerate_graphs(ctx, name):
     """Graphs without edges, a single vertex, isolated vertices, a star and
     a clique: compression, statistics, every metric (NaN and -inf where m or
     a community volume is 0, as the reference divides), one propagation step
     of every rule, the bounded -s 0 / -s 1 loops (stepwise and pcd_refine),
     triangles and the -i 1 partition equal the oracle."""
-    g = _csr_direct(*TINY[name])
+    g = GG.csr_direct(*GG.TINY[name])
     n, m, ind, adj = g[:4]
     et, tri_o, td_o, cc_o = O.count_triangles(n, ind, adj)
     rng = np.random.default_rng(1)
