@@ -326,13 +326,11 @@ __device__ inline int label_before(const uint64_t *M, const int32_t *L, int w, i
 }
 
 template <int G>
-__global__ __launch_bounds__(256) void k_delta_small(const int32_t *list_all, const int32_t *bounds,
-                                                     const int32_t *indices, const int32_t *adj, const int32_t *L,
-                                                     const int32_t *N, const uint64_t *M, CommStats *st) {
+__device__ inline void delta_small(int wave_g, int nwaves, const int32_t *list_all, const int32_t *bounds,
+                                   const int32_t *indices, const int32_t *adj, const int32_t *L, const int32_t *N,
+                                   const uint64_t *M, CommStats *st) {
     const int lane = __lane_id(), glane = lane & (G - 1);
     const int per_wave = WAVE / G;
-    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    const int nwaves = gridDim.x * blockDim.x / WAVE;
     const int32_t *list = list_all + bounds[0];
     const int cnt = bounds[1] - bounds[0];
     for (int base = wave_g * per_wave; base < cnt; base += nwaves * per_wave) {
@@ -371,13 +369,11 @@ __global__ __launch_bounds__(256) void k_delta_small(const int32_t *list_all, co
 // 64 list entries at once, lane j taking entry wave + (64 r + j) * waves in
 // round r, and runs their row chunks one after another; one long row's
 // chunks are spread over many waves.
-__global__ __launch_bounds__(256) void k_delta_items(const int32_t *list_all, const int32_t *bounds,
-                                                     const int4 *items, const int32_t *adj, const int32_t *L,
-                                                     const int32_t *N, const uint64_t *M, CommStats *st) {
+__device__ inline void delta_items(int wave_g, int nwaves, const int32_t *list_all, const int32_t *bounds,
+                                   const int4 *items, const int32_t *adj, const int32_t *L, const int32_t *N,
+                                   const uint64_t *M, CommStats *st) {
     constexpr int PF = 4;
     const int lane = __lane_id();
-    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    const int nwaves = gridDim.x * blockDim.x / WAVE;
     const int32_t *list = list_all + bounds[0];
     const int cnt = bounds[1] - bounds[0];
     for (int64_t r = 0; wave_g + r * WAVE * nwaves < cnt; r++) {
@@ -428,6 +424,27 @@ __global__ __launch_bounds__(256) void k_delta_items(const int32_t *list_all, co
     }
 }
 
+// All six moved lists in one launch: workgroup b works on list b % 6 (the
+// G64..G4 small-bin lists, then the long-row items), so the short
+// dependent-load chains of the lists overlap instead of running one kernel
+// after another.  The lists are disjoint and the updates are atomic adds.
+constexpr int DELTA_LISTS = 6;
+__global__ __launch_bounds__(256) void k_delta_all(const int32_t *list_all, const int32_t *bounds, const int4 *items,
+                                                   const int32_t *indices, const int32_t *adj, const int32_t *L,
+                                                   const int32_t *N, const uint64_t *M, CommStats *st) {
+    const int which = blockIdx.x % DELTA_LISTS;
+    const int wave_g = (blockIdx.x / DELTA_LISTS) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int nwaves = (gridDim.x / DELTA_LISTS) * (blockDim.x / WAVE);
+    switch (which) {
+    case 0: delta_small<64>(wave_g, nwaves, list_all, bounds + 0, indices, adj, L, N, M, st); break;
+    case 1: delta_small<32>(wave_g, nwaves, list_all, bounds + 1, indices, adj, L, N, M, st); break;
+    case 2: delta_small<16>(wave_g, nwaves, list_all, bounds + 2, indices, adj, L, N, M, st); break;
+    case 3: delta_small<8>(wave_g, nwaves, list_all, bounds + 3, indices, adj, L, N, M, st); break;
+    case 4: delta_small<4>(wave_g, nwaves, list_all, bounds + 4, indices, adj, L, N, M, st); break;
+    default: delta_items(wave_g, nwaves, list_all, bounds + 5, items, adj, L, N, M, st); break;
+    }
+}
+
 int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *after) {
     const BinLayout &b = ctx->bins;
     kt_begin(ctx, KS_STATS_DELTA);
@@ -445,19 +462,9 @@ int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *aft
     uint64_t *M = ctx->d_moved_bits;
     hipLaunchKernelGGL(k_moved_bits, dim3(grid_for(ctx, (ctx->n + WAVE - 1) / WAVE, 4, 8)), dim3(256), 0, ctx->stream,
                        ctx->n, before, after, M);
-    if (m.nitems > 0)
-        hipLaunchKernelGGL(k_delta_items, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, list, bounds + 5,
-                           ctx->d_stat_items, ctx->d_adj, before, after, M, ctx->d_stats);
-#define PCD_DELTA(I, G)                                                                                             \
-    if (m.s[I + 1] > m.s[I])                                                                                        \
-        hipLaunchKernelGGL(k_delta_small<G>, dim3(ctx->cu_count * 4), dim3(256), 0, ctx->stream, list, bounds + I, \
-                           ctx->d_indices, ctx->d_adj, before, after, M, ctx->d_stats);
-    PCD_DELTA(0, 64)
-    PCD_DELTA(1, 32)
-    PCD_DELTA(2, 16)
-    PCD_DELTA(3, 8)
-    PCD_DELTA(4, 4)
-#undef PCD_DELTA
+    // the lists of empty bins (and no items) are empty: their workgroups exit
+    hipLaunchKernelGGL(k_delta_all, dim3(DELTA_LISTS * ctx->cu_count * 4), dim3(256), 0, ctx->stream, list, bounds,
+                       ctx->d_stat_items, ctx->d_indices, ctx->d_adj, before, after, M, ctx->d_stats);
     kt_end(ctx, KS_STATS_DELTA);
     PCD_HIP(hipGetLastError());
     return PCD_OK;
