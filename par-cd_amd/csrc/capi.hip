@@ -611,8 +611,21 @@ extern "C" int pcd_merging_step(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_
 // the spare label buffer and scratch).  When the statistics follow the
 // labels, k also stays on the device until that wait, so the GPU runs the
 // iterations back to back.
+static int iterate_pass(pcd_ctx *ctx, const pcd_job *job, int strategy, int rounds, int it, double prev_score,
+                        double *score, int *num_labels, int *done);
+
 extern "C" int pcd_iterate(pcd_ctx *ctx, const pcd_job *job, int strategy, int rounds, int it, double prev_score,
                            double *score, int *num_labels, int *done) {
+    const int rc = iterate_pass(ctx, job, strategy, rounds, it, prev_score, score, num_labels, done);
+    if (ctx && ctx->k_deferred) {  // a failure after the compression: take k before returning
+        (void)hipStreamSynchronize(ctx->stream);
+        resolve_k(ctx);
+    }
+    return rc;
+}
+
+static int iterate_pass(pcd_ctx *ctx, const pcd_job *job, int strategy, int rounds, int it, double prev_score,
+                        double *score, int *num_labels, int *done) {
     if (!ctx || !job) return fail(PCD_E_INVALID, "null argument");
     if (strategy < 0 || strategy > 2) return fail(PCD_E_INVALID, "strategy must be 0, 1 or 2");
     PCD_TRY(check_labels(ctx));
