@@ -424,18 +424,51 @@ __device__ inline void delta_items(int wave_g, int nwaves, const int32_t *list_a
     }
 }
 
-// All six moved lists in one launch: workgroup b works on list b % 6 (the
-// G64..G4 small-bin lists, then the long-row items), so the short
-// dependent-load chains of the lists overlap instead of running one kernel
-// after another.  The lists are disjoint and the updates are atomic adds.
-constexpr int DELTA_LISTS = 6;
+// All six moved lists in one launch, so the short dependent-load chains of
+// the lists overlap instead of running one kernel after another.  Every
+// workgroup reads the six list lengths and computes the same split of the
+// grid into contiguous ranges, one per list, proportional to the list's
+// entries (at most G per small-bin vertex, DELTA_ITEM_COST per long-row
+// item), with at least one workgroup for a list that is not empty.  The
+// lists are disjoint and the updates are atomic adds.
+constexpr int DELTA_ITEM_COST = 512;
 __global__ __launch_bounds__(256) void k_delta_all(const int32_t *list_all, const int32_t *bounds, const int4 *items,
                                                    const int32_t *indices, const int32_t *adj, const int32_t *L,
                                                    const int32_t *N, const uint64_t *M, CommStats *st) {
-    const int which = blockIdx.x % DELTA_LISTS;
-    const int wave_g = (blockIdx.x / DELTA_LISTS) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    const int nwaves = (gridDim.x / DELTA_LISTS) * (blockDim.x / WAVE);
-    switch (which) {
+    __shared__ int s_lo[7];
+    if (threadIdx.x == 0) {
+        constexpr int cost[6] = {64, 32, 16, 8, 4, DELTA_ITEM_COST};
+        double w[6], tot = 0.0;
+        int nonempty = 0;
+        for (int l = 0; l < 6; l++) {
+            const int c = bounds[l + 1] - bounds[l];
+            w[l] = (double)c * cost[l];
+            tot += w[l];
+            nonempty += c > 0;
+        }
+        // one workgroup reserved per non-empty list, the rest by weight
+        const int spare = (int)gridDim.x - nonempty;
+        int at = 0;
+        double acc = 0.0;
+        for (int l = 0; l < 6; l++) {
+            s_lo[l] = at;
+            if (w[l] > 0) {
+                const int before = tot > 0 ? (int)(spare * (acc / tot)) : 0;
+                acc += w[l];
+                const int after = tot > 0 ? (int)(spare * (acc / tot)) : 0;
+                at += 1 + (after - before);
+            }
+        }
+        s_lo[6] = at;  // <= gridDim.x
+    }
+    __syncthreads();
+    int l = 0;
+    while (l < 6 && !(s_lo[l] <= (int)blockIdx.x && (int)blockIdx.x < s_lo[l + 1])) l++;
+    if (l == 6) return;  // beyond the last range (rounding) or no moved rows
+    const int wpb = blockDim.x / WAVE;
+    const int wave_g = ((int)blockIdx.x - s_lo[l]) * wpb + (int)threadIdx.x / WAVE;
+    const int nwaves = (s_lo[l + 1] - s_lo[l]) * wpb;
+    switch (l) {
     case 0: delta_small<64>(wave_g, nwaves, list_all, bounds + 0, indices, adj, L, N, M, st); break;
     case 1: delta_small<32>(wave_g, nwaves, list_all, bounds + 1, indices, adj, L, N, M, st); break;
     case 2: delta_small<16>(wave_g, nwaves, list_all, bounds + 2, indices, adj, L, N, M, st); break;
@@ -463,7 +496,7 @@ int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *aft
     hipLaunchKernelGGL(k_moved_bits, dim3(grid_for(ctx, (ctx->n + WAVE - 1) / WAVE, 4, 8)), dim3(256), 0, ctx->stream,
                        ctx->n, before, after, M);
     // the lists of empty bins (and no items) are empty: their workgroups exit
-    hipLaunchKernelGGL(k_delta_all, dim3(DELTA_LISTS * ctx->cu_count * 4), dim3(256), 0, ctx->stream, list, bounds,
+    hipLaunchKernelGGL(k_delta_all, dim3(ctx->cu_count * 24), dim3(256), 0, ctx->stream, list, bounds,
                        ctx->d_stat_items, ctx->d_indices, ctx->d_adj, before, after, M, ctx->d_stats);
     kt_end(ctx, KS_STATS_DELTA);
     PCD_HIP(hipGetLastError());
