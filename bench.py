@@ -103,7 +103,7 @@ ROCPROF_NAMES = {
     "lp_hub": [r"pcd::k_hub_partition<false>$", r"pcd::k_hub_count_listed<1u, 512, 4096, false>$"],
     "lp_g64": [r"pcd::k_lp_small<64, 8, false, 1u"], "lp_g32": [r"pcd::k_lp_small<32, 16, false, 1u"],
     "lp_g16": [r"pcd::k_lp_small<16, 32, false, 1u"], "lp_g8": [r"pcd::k_lp_small<8, 64, false, 1u"],
-    "lp_g4": [r"pcd::k_lp_small<4, 64, false, 1u"]}
+    "lp_g4": [r"pcd::k_lp_small<4, 64, false, 1u"], "lp_small": [r"pcd::k_lp_small_all<1u>"]}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "rmat22_mod_pmc.md")
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0  # stream-copy calibration of FETCH_SIZE / WRITE_SIZE
 
@@ -573,7 +573,7 @@ def main():
     probe_avg = {}
     for k_, (ms_, cnt_) in timers.items():
         name_ = "lp_hub" if k_ in ("lp_hub_partition", "lp_hub_count") else k_
-        if cnt_ and (name_ in SLOT_OF_BIN.values() or name_ == "stats_items"):
+        if cnt_ and (name_ in SLOT_OF_BIN.values() or name_ in ("stats_items", "lp_small")):
             probe_avg[name_] = probe_avg.get(name_, 0.0) + ms_ / cnt_
     dom_probe = max(probe_avg, key=probe_avg.get) if probe_avg else None
     dom_slots = (["lp_hub_partition", "lp_hub_count"] if dom_probe == "lp_hub" else [dom_probe]) if dom_probe else []
@@ -615,6 +615,14 @@ def main():
         per_kernel[slot] = {
             "bytes_per_launch": byt / launches, "avg_ms": tms / launches,
             "gbs": byt / (tms / 1000.0) / 1e9, "entries_per_launch": e_ / launches}
+    # the G64..G4 bins run as one launch (timer slot lp_small)
+    tms, launches = timers.get("lp_small", (0.0, 0))
+    if tms > 0 and launches:
+        small = [counters[b] for b in ("g64", "g32", "g16", "g8", "g4") if b in counters]
+        byt = sum(lp_bin_bytes(a_, e_, d_, S) for a_, e_, d_ in small)
+        ent = sum(e_ for _, e_, _ in small)
+        per_kernel["lp_small"] = {"bytes_per_launch": byt / launches, "avg_ms": tms / launches,
+                                  "gbs": byt / (tms / 1000.0) / 1e9, "entries_per_launch": ent / launches}
     # statistics over rows longer than 64 (work items): per entry the neighbour
     # id and its label, per row its extent and label and one 12-byte record
     # update

@@ -77,7 +77,7 @@ void kt_end(pcd_ctx *ctx, int slot) {
 static const char *kslot_names[NUM_KSLOTS] = {
     "compress", "stats_items", "stats_small", "metric", "lp_records", "lp_hub_partition", "lp_hub_count",
     "lp_b4096", "lp_b1024", "lp_w256", "lp_g64", "lp_g32", "lp_g16", "lp_g8", "lp_g4",
-    "merge_keys", "merge_sort", "merge_rle", "merge_rounds", "triangles", "stats_delta"};
+    "merge_keys", "merge_sort", "merge_rle", "merge_rounds", "triangles", "stats_delta", "lp_small"};
 
 static const char *bin_names[NUM_BINS] = {"hub", "b4096", "b1024", "w256", "g64", "g32", "g16", "g8", "g4"};
 
@@ -770,6 +770,8 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.block_aux = value != 0;
     } else if (k == "hub_aux") {
         ctx->tune.hub_aux = value != 0;
+    } else if (k == "small_fused") {
+        ctx->tune.small_fused = value != 0;
     } else if (k == "hub_nt") {
         if (value != 256 && value != 512) return fail(PCD_E_INVALID, "hub_nt: 256 or 512");
         ctx->tune.hub_nt = value;

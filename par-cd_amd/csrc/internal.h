@@ -54,6 +54,7 @@ struct Tuning {
     int block_aux = 0;           // wave/workgroup-per-vertex bins gather {label, aux} records
     int hub_aux = 0;             // hub partitions count {label, aux} records (1) or labels (0)
     int hub_nt = 512;            // hub partition workgroups: 512 threads, 4096 slots or 256, 2048
+    int small_fused = 1;         // the G64..G4 bins in one launch (1) or one launch each (0)
     int stats_cache = 1;         // statistics over long rows read the labels of the first
                                  // STATS_CACHE vertices (the highest degrees) from LDS
     int inc_stats = 1;           // a propagation step updates the statistics from the rows
@@ -78,7 +79,7 @@ enum KSlot {
     KS_COMPRESS = 0, KS_STATS_ITEMS, KS_STATS_SMALL, KS_METRIC,
     KS_LP_SELECT, KS_LP_HUB_INSERT, KS_LP_HUB_DECIDE, KS_LP_B4096, KS_LP_B1024, KS_LP_W256,
     KS_LP_G64, KS_LP_G32, KS_LP_G16, KS_LP_G8, KS_LP_G4,
-    KS_MERGE_KEYS, KS_MERGE_SORT, KS_MERGE_RLE, KS_MERGE_ROUNDS, KS_TRIANGLES, KS_STATS_DELTA,
+    KS_MERGE_KEYS, KS_MERGE_SORT, KS_MERGE_RLE, KS_MERGE_ROUNDS, KS_TRIANGLES, KS_STATS_DELTA, KS_LP_SMALL,
     NUM_KSLOTS
 };
 

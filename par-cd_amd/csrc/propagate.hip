@@ -162,15 +162,11 @@ __global__ void k_label_totals(const int32_t *kp, const CommStats *st, int kind,
 
 // ------------------------------------------------------------- G lanes / vertex
 template <int G, int CP, bool AUX, unsigned KM, class Src>
-__global__ __launch_bounds__(256) void k_lp_small(Src src) {
-    __shared__ int s_list[256 / WAVE][WAVE];
-    const int lane = __lane_id(), glane = lane & (G - 1), w = threadIdx.x / WAVE;
+__device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *list) {
+    const int lane = __lane_id(), glane = lane & (G - 1);
     const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
     const int per_wave = WAVE / G;
-    const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
-    const int nwaves = gridDim.x * blockDim.x / WAVE;
     const int nunits = src.units();
-    int *list = s_list[w];
     for (int u = wave_g; u < nunits; u += nwaves) {
         LPArgs a;
         int q0, q1;
@@ -266,6 +262,41 @@ __global__ __launch_bounds__(256) void k_lp_small(Src src) {
             }
         }
         wave_sync();
+    }
+}
+
+template <int G, int CP, bool AUX, unsigned KM, class Src>
+__global__ __launch_bounds__(256) void k_lp_small(Src src) {
+    __shared__ int s_list[256 / WAVE][WAVE];
+    const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+    lp_small<G, CP, AUX, KM>(src, wave_g, gridDim.x * blockDim.x / WAVE, s_list[threadIdx.x / WAVE]);
+}
+
+// The five small-degree bins in one launch: the grid is split into
+// contiguous workgroup ranges, one per bin, in proportion to the bin's
+// entries (at least one workgroup for a bin that is not empty), so the
+// bins' ramps and tails overlap instead of following one another.
+struct SmallBins {
+    RangeSrc src[5];  // G64, G32, G16, G8, G4
+    int lo[6];        // workgroup range of bin b: [lo[b], lo[b + 1])
+};
+
+template <unsigned KM>
+__global__ __launch_bounds__(256) void k_lp_small_all(SmallBins sb) {
+    __shared__ int s_list[256 / WAVE][WAVE];
+    int b = 0;
+    while (b < 5 && !(sb.lo[b] <= (int)blockIdx.x && (int)blockIdx.x < sb.lo[b + 1])) b++;
+    if (b == 5) return;
+    const int wpb = blockDim.x / WAVE;
+    const int wave_g = __builtin_amdgcn_readfirstlane(((int)blockIdx.x - sb.lo[b]) * wpb + (int)threadIdx.x / WAVE);
+    const int nwaves = (sb.lo[b + 1] - sb.lo[b]) * wpb;
+    int *list = s_list[threadIdx.x / WAVE];
+    switch (b) {
+    case 0: lp_small<64, 8, false, KM>(sb.src[0], wave_g, nwaves, list); break;
+    case 1: lp_small<32, 16, false, KM>(sb.src[1], wave_g, nwaves, list); break;
+    case 2: lp_small<16, 32, false, KM>(sb.src[2], wave_g, nwaves, list); break;
+    case 3: lp_small<8, 64, false, KM>(sb.src[3], wave_g, nwaves, list); break;
+    default: lp_small<4, 64, false, KM>(sb.src[4], wave_g, nwaves, list); break;
     }
 }
 
@@ -1020,11 +1051,44 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
                                dim3(256), 0, ctx->stream, RangeSrc{p0, p1, CP, args_for(BIN)});                \
         kt_end(ctx, SLOT);                                                                                     \
     }
-    PCD_SMALL(64, 8, BIN_G64, KS_LP_G64)
-    PCD_SMALL(32, 16, BIN_G32, KS_LP_G32)
-    PCD_SMALL(16, 32, BIN_G16, KS_LP_G16)
-    PCD_SMALL(8, 64, BIN_G8, KS_LP_G8)
-    PCD_SMALL(4, 64, BIN_G4, KS_LP_G4)
+    if (!small_aux && ctx->tune.small_fused) {
+        SmallBins sb;
+        const int cps[5] = {8, 16, 32, 64, 64};
+        int64_t tot = 0;
+        int nonempty = 0;
+        for (int i = 0; i < 5; i++) {
+            range(BIN_G64 + i, p0, p1);
+            sb.src[i] = RangeSrc{p0, p1, cps[i], args_for(BIN_G64 + i)};
+            tot += ctx->bin_entries[BIN_G64 + i];
+            nonempty += p1 > p0;
+        }
+        if (nonempty > 0) {
+            // one workgroup per non-empty bin, the rest by entries
+            const int spare = ctx->cu_count * 16 - nonempty;
+            int at = 0;
+            int64_t acc = 0;
+            for (int i = 0; i < 5; i++) {
+                sb.lo[i] = at;
+                if (sb.src[i].p1 > sb.src[i].p0) {
+                    const int64_t e = ctx->bin_entries[BIN_G64 + i];
+                    const int before = tot > 0 ? (int)(spare * acc / tot) : 0;
+                    acc += e;
+                    const int after = tot > 0 ? (int)(spare * acc / tot) : 0;
+                    at += 1 + (after - before);
+                }
+            }
+            sb.lo[5] = at;
+            kt_begin(ctx, KS_LP_SMALL);
+            hipLaunchKernelGGL(k_lp_small_all<KM>, dim3(at), dim3(256), 0, ctx->stream, sb);
+            kt_end(ctx, KS_LP_SMALL);
+        }
+    } else {
+        PCD_SMALL(64, 8, BIN_G64, KS_LP_G64)
+        PCD_SMALL(32, 16, BIN_G32, KS_LP_G32)
+        PCD_SMALL(16, 32, BIN_G16, KS_LP_G16)
+        PCD_SMALL(8, 64, BIN_G8, KS_LP_G8)
+        PCD_SMALL(4, 64, BIN_G4, KS_LP_G4)
+    }
 #undef PCD_SMALL
     PCD_HIP(hipGetLastError());
     return PCD_OK;
