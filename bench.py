@@ -615,10 +615,10 @@ def main():
         per_kernel[slot] = {
             "bytes_per_launch": byt / launches, "avg_ms": tms / launches,
             "gbs": byt / (tms / 1000.0) / 1e9, "entries_per_launch": e_ / launches}
-    # the G64..G4 bins run as one launch (timer slot lp_small)
+    # the W256 and G64..G4 bins run as one launch (timer slot lp_small)
     tms, launches = timers.get("lp_small", (0.0, 0))
     if tms > 0 and launches:
-        small = [counters[b] for b in ("g64", "g32", "g16", "g8", "g4") if b in counters]
+        small = [counters[b] for b in ("w256", "g64", "g32", "g16", "g8", "g4") if b in counters]
         byt = sum(lp_bin_bytes(a_, e_, d_, S) for a_, e_, d_ in small)
         ent = sum(e_ for _, e_, _ in small)
         per_kernel["lp_small"] = {"bytes_per_launch": byt / launches, "avg_ms": tms / launches,

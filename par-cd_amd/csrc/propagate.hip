@@ -272,34 +272,6 @@ __global__ __launch_bounds__(256) void k_lp_small(Src src) {
     lp_small<G, CP, AUX, KM>(src, wave_g, gridDim.x * blockDim.x / WAVE, s_list[threadIdx.x / WAVE]);
 }
 
-// The five small-degree bins in one launch: the grid is split into
-// contiguous workgroup ranges, one per bin, in proportion to the bin's
-// entries (at least one workgroup for a bin that is not empty), so the
-// bins' ramps and tails overlap instead of following one another.
-struct SmallBins {
-    RangeSrc src[5];  // G64, G32, G16, G8, G4
-    int lo[6];        // workgroup range of bin b: [lo[b], lo[b + 1])
-};
-
-template <unsigned KM>
-__global__ __launch_bounds__(256) void k_lp_small_all(SmallBins sb) {
-    __shared__ int s_list[256 / WAVE][WAVE];
-    int b = 0;
-    while (b < 5 && !(sb.lo[b] <= (int)blockIdx.x && (int)blockIdx.x < sb.lo[b + 1])) b++;
-    if (b == 5) return;
-    const int wpb = blockDim.x / WAVE;
-    const int wave_g = __builtin_amdgcn_readfirstlane(((int)blockIdx.x - sb.lo[b]) * wpb + (int)threadIdx.x / WAVE);
-    const int nwaves = (sb.lo[b + 1] - sb.lo[b]) * wpb;
-    int *list = s_list[threadIdx.x / WAVE];
-    switch (b) {
-    case 0: lp_small<64, 8, false, KM>(sb.src[0], wave_g, nwaves, list); break;
-    case 1: lp_small<32, 16, false, KM>(sb.src[1], wave_g, nwaves, list); break;
-    case 2: lp_small<16, 32, false, KM>(sb.src[2], wave_g, nwaves, list); break;
-    case 3: lp_small<8, 64, false, KM>(sb.src[3], wave_g, nwaves, list); break;
-    default: lp_small<4, 64, false, KM>(sb.src[4], wave_g, nwaves, list); break;
-    }
-}
-
 // ------------------------------------------------------------- LDS hash table
 // Hub partition tables: T slots (any multiple of 64), linear probing from
 // (hash * T) >> 32.
@@ -455,25 +427,18 @@ __device__ inline int ceil_log2(int x) { return x <= 1 ? 0 : 32 - __clz(x - 1); 
 
 // ------------------------------------------------------------- wave / vertex
 // The table starts empty and each row's scan clears the slots it used.
-template <int TW, unsigned KM, class Src, bool AUX = false, int PF = 4>
-__global__ __launch_bounds__(256) void k_lp_wave(Src src) {
-    __shared__ int s_keys[256 / WAVE][TW];
-    __shared__ int s_cnts[256 / WAVE][TW];
-    __shared__ int s_auxs[256 / WAVE][AUX ? TW : 1];
-    __shared__ uint16_t s_occ[256 / WAVE][TW / 2];
-    __shared__ int s_nocc[256 / WAVE];
-    const int lane = __lane_id(), w = threadIdx.x / WAVE;
-    int *keys = s_keys[w], *cnts = s_cnts[w], *auxs = s_auxs[w];
-    uint16_t *occ = s_occ[w];
-    int *nocc = &s_nocc[w];
+// One wave per vertex over the units of src, with this wave's LDS table
+// (TW slots of keys, counts and aux, a slot list of TW / 2 entries).
+template <int TW, unsigned KM, class Src, bool AUX, int PF>
+__device__ inline void lp_wave(const Src &src, int wave_g, int nwaves, int *keys, int *cnts, int *auxs, uint16_t *occ,
+                               int *nocc) {
+    const int lane = __lane_id();
     for (int i = lane; i < TW; i += WAVE) {
         keys[i] = EMPTY_KEY;
         cnts[i] = 0;
     }
     if (lane == 0) *nocc = 0;
     wave_sync();
-    const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
-    const int nwaves = gridDim.x * blockDim.x / WAVE;
     const int nunits = src.units();
     for (int u = wave_g; u < nunits; u += nwaves) {
       LPArgs a;
@@ -505,7 +470,56 @@ __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
     }
 }
 
-// ------------------------------------------------------------- block / vertex
+template <int TW, unsigned KM, class Src, bool AUX = false, int PF = 4>
+__global__ __launch_bounds__(256) void k_lp_wave(Src src) {
+    __shared__ int s_keys[256 / WAVE][TW];
+    __shared__ int s_cnts[256 / WAVE][TW];
+    __shared__ int s_auxs[256 / WAVE][AUX ? TW : 1];
+    __shared__ uint16_t s_occ[256 / WAVE][TW / 2];
+    __shared__ int s_nocc[256 / WAVE];
+    const int w = threadIdx.x / WAVE;
+    const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+    lp_wave<TW, KM, Src, AUX, PF>(src, wave_g, gridDim.x * blockDim.x / WAVE, s_keys[w], s_cnts[w], s_auxs[w],
+                                  s_occ[w], &s_nocc[w]);
+}
+
+// The W256 and G64..G4 bins in one launch: the grid is split into
+// contiguous workgroup ranges, one per bin, in proportion to the bin's
+// entries (at least one workgroup for a bin that is not empty), so the
+// bins' ramps and tails overlap instead of following one another.  The
+// W256 waves keep their 512-slot tables; the LDS of a workgroup is that
+// of a W256 workgroup either way.
+struct SmallBins {
+    RangeSrc src[6];  // W256, G64, G32, G16, G8, G4
+    int lo[7];        // workgroup range of bin b: [lo[b], lo[b + 1])
+};
+
+template <unsigned KM>
+__global__ __launch_bounds__(256) void k_lp_small_all(SmallBins sb) {
+    __shared__ int s_keys[256 / WAVE][512];
+    __shared__ int s_cnts[256 / WAVE][512];
+    __shared__ uint16_t s_occ[256 / WAVE][256];
+    __shared__ int s_nocc[256 / WAVE];
+    __shared__ int s_list[256 / WAVE][WAVE];
+    int b = 0;
+    while (b < 6 && !(sb.lo[b] <= (int)blockIdx.x && (int)blockIdx.x < sb.lo[b + 1])) b++;
+    if (b == 6) return;
+    const int wpb = blockDim.x / WAVE, w = threadIdx.x / WAVE;
+    const int wave_g = __builtin_amdgcn_readfirstlane(((int)blockIdx.x - sb.lo[b]) * wpb + w);
+    const int nwaves = (sb.lo[b + 1] - sb.lo[b]) * wpb;
+    int *list = s_list[w];
+    switch (b) {
+    case 0: lp_wave<512, KM, RangeSrc, false, 4>(sb.src[0], wave_g, nwaves, s_keys[w], s_cnts[w], nullptr, s_occ[w],
+                                                 &s_nocc[w]); break;
+    case 1: lp_small<64, 8, false, KM>(sb.src[1], wave_g, nwaves, list); break;
+    case 2: lp_small<32, 16, false, KM>(sb.src[2], wave_g, nwaves, list); break;
+    case 3: lp_small<16, 32, false, KM>(sb.src[3], wave_g, nwaves, list); break;
+    case 4: lp_small<8, 64, false, KM>(sb.src[4], wave_g, nwaves, list); break;
+    default: lp_small<4, 64, false, KM>(sb.src[5], wave_g, nwaves, list); break;
+    }
+}
+
+
 // Table slots: nextpow2(2 deg), at most TB.
 template <int NT, int TB, unsigned KM, class Src, bool AUX = false, int PF = 4>
 __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
@@ -1030,7 +1044,9 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
         }
         kt_end(ctx, KS_LP_B1024);
     }
-    if (range(BIN_W256, p0, p1)) {
+    // W256 and G64..G4 in one launch unless a record-gathering form is tuned on
+    const bool fused = !small_aux && !block_aux && ctx->tune.small_fused;
+    if (!fused && range(BIN_W256, p0, p1)) {
         kt_begin(ctx, KS_LP_W256);
         if (block_aux)
             hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256),
@@ -1051,15 +1067,17 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
                                dim3(256), 0, ctx->stream, RangeSrc{p0, p1, CP, args_for(BIN)});                \
         kt_end(ctx, SLOT);                                                                                     \
     }
-    if (!small_aux && ctx->tune.small_fused) {
+    if (fused) {
         SmallBins sb;
-        const int cps[5] = {8, 16, 32, 64, 64};
+        // bins W256, G64..G4 (consecutive bin numbers) and positions per unit
+        const int cps[6] = {1, 8, 16, 32, 64, 64};
+        static_assert(BIN_G64 == BIN_W256 + 1 && BIN_G4 == BIN_W256 + 5, "bin order");
         int64_t tot = 0;
         int nonempty = 0;
-        for (int i = 0; i < 5; i++) {
-            range(BIN_G64 + i, p0, p1);
-            sb.src[i] = RangeSrc{p0, p1, cps[i], args_for(BIN_G64 + i)};
-            tot += ctx->bin_entries[BIN_G64 + i];
+        for (int i = 0; i < 6; i++) {
+            range(BIN_W256 + i, p0, p1);
+            sb.src[i] = RangeSrc{p0, p1, cps[i], args_for(BIN_W256 + i)};
+            tot += ctx->bin_entries[BIN_W256 + i];
             nonempty += p1 > p0;
         }
         if (nonempty > 0) {
@@ -1067,17 +1085,17 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
             const int spare = ctx->cu_count * 16 - nonempty;
             int at = 0;
             int64_t acc = 0;
-            for (int i = 0; i < 5; i++) {
+            for (int i = 0; i < 6; i++) {
                 sb.lo[i] = at;
                 if (sb.src[i].p1 > sb.src[i].p0) {
-                    const int64_t e = ctx->bin_entries[BIN_G64 + i];
+                    const int64_t e = ctx->bin_entries[BIN_W256 + i];
                     const int before = tot > 0 ? (int)(spare * acc / tot) : 0;
                     acc += e;
                     const int after = tot > 0 ? (int)(spare * acc / tot) : 0;
                     at += 1 + (after - before);
                 }
             }
-            sb.lo[5] = at;
+            sb.lo[6] = at;
             kt_begin(ctx, KS_LP_SMALL);
             hipLaunchKernelGGL(k_lp_small_all<KM>, dim3(at), dim3(256), 0, ctx->stream, sb);
             kt_end(ctx, KS_LP_SMALL);
