@@ -335,6 +335,44 @@ int pcd_convert_edges(int num_nodes, int64_t num_edges, const int32_t *a, const 
                       int32_t **indices, int32_t **adj, int32_t **map);
 void pcd_free(void *p);
 
+/* ------------------------------------------------------------ graph sets */
+/* Many PCD graphs in one file, for batching and serving.  Each member holds
+ * the exact bytes of one PCD file (graph.cpp:40-105 reads them,
+ * convert.cpp:244-278 writes them).  Layout: "PCDSET1\0", int64 count, count
+ * x {int64 offset, int64 bytes}, then the members in table order, each
+ * starting 11 bytes before a 64-byte boundary (its arrays 4-byte aligned, its
+ * indices 64-byte aligned in a mapping), zero bytes between them, nothing
+ * after the last.  The layout is canonical: the same graphs always give the
+ * same bytes.  No reference counterpart (the reference reads one graph per
+ * run, main.cpp:170-225). */
+typedef struct pcd_graphset pcd_graphset;
+typedef struct pcd_graphset_writer pcd_graphset_writer;
+/* Maps the file read-only and checks its table and each member's header and
+ * length (PCD_E_FORMAT otherwise). */
+int pcd_graphset_open(const char *path, pcd_graphset **out);
+/* Unmaps the file: pointers from pcd_graphset_graph become invalid. */
+int pcd_graphset_close(pcd_graphset *s);
+int pcd_graphset_count(const pcd_graphset *s, int *count);
+/* Graph i as pointers into the mapping (valid until pcd_graphset_close; any
+ * output may be NULL).  The first call for a member makes pcd_read_pcd's
+ * structural checks on its arrays; a member that fails them reports
+ * PCD_E_FORMAT on every call. */
+int pcd_graphset_graph(pcd_graphset *s, int i, int *n, int *m, const int32_t **indices, const int32_t **adj,
+                       const int32_t **map);
+/* Writes member i as a PCD file: the same bytes the set holds for it. */
+int pcd_graphset_extract(pcd_graphset *s, int i, const char *path);
+/* Streaming writer: open with the number of graphs, add them in order (map
+ * may be NULL: vertex v keeps id v; a graph that fails the structural checks
+ * is refused with PCD_E_INVALID), close.  Closing after fewer than `count`
+ * graphs removes the file and returns PCD_E_STATE. */
+int pcd_graphset_writer_open(const char *path, int count, pcd_graphset_writer **out);
+int pcd_graphset_writer_add(pcd_graphset_writer *w, int n, int m, const int32_t *indices, const int32_t *adj,
+                            const int32_t *map);
+int pcd_graphset_writer_close(pcd_graphset_writer *w);
+/* Packs PCD files (read with pcd_read_pcd's checks) into a set, in order;
+ * on an error no file is left behind. */
+int pcd_graphset_pack(const char *path, int count, const char *const *pcd_paths);
+
 /* framework.cpp:14-24 create_singleton_partition: identity permutation
  * shuffled with glibc rand() (the caller's generator state, as there). */
 int pcd_singleton_partition(int n, int32_t *labels);

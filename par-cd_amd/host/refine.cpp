@@ -11,7 +11,7 @@ extern "C" int pcd_refine(pcd_ctx *ctx, const pcd_job *job, int strategy, int ro
                           int *iterations_out, int *k_out) {
     if (!ctx || !job) return fail(PCD_E_INVALID, "null argument");
     if (strategy < 0 || strategy > 2) return fail(PCD_E_INVALID, "strategy must be 0, 1 or 2");
-    const int cap = job->iter_cap > 0 ? job->iter_cap : 1000;
+    // pcd_iterate applies -n and the iteration cap (done on the pass that reaches them)
     double prev = 0.0, score = 0.0;
     int it = 0, k = 0;
     for (;;) {

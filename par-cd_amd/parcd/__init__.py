@@ -35,6 +35,8 @@ EXPORTS = [
     "pcd_gen_sbm_batch", "pcd_refine", "pcd_iterate", "pcd_host_alloc", "pcd_host_free", "pcd_clustering_partition",
     "pcd_server_create", "pcd_server_destroy", "pcd_server_submit", "pcd_server_wait", "pcd_server_result",
     "pcd_server_info", "pcd_gen_requests",
+    "pcd_graphset_open", "pcd_graphset_close", "pcd_graphset_count", "pcd_graphset_graph", "pcd_graphset_extract",
+    "pcd_graphset_writer_open", "pcd_graphset_writer_add", "pcd_graphset_writer_close", "pcd_graphset_pack",
 ]
 BATCH_RUNNING, BATCH_DONE, BATCH_FREE = 0, 1, 2
 
@@ -106,6 +108,16 @@ def lib():
     L.pcd_read_pcd.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(I32P),
                                C.POINTER(I32P), C.POINTER(I32P)]
     L.pcd_write_pcd.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, vp, vp]
+    L.pcd_graphset_open.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.pcd_graphset_close.argtypes = [vp]
+    L.pcd_graphset_count.argtypes = [vp, C.POINTER(C.c_int)]
+    L.pcd_graphset_graph.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(I32P),
+                                     C.POINTER(I32P), C.POINTER(I32P)]
+    L.pcd_graphset_extract.argtypes = [vp, C.c_int, C.c_char_p]
+    L.pcd_graphset_writer_open.argtypes = [C.c_char_p, C.c_int, C.POINTER(vp)]
+    L.pcd_graphset_writer_add.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+    L.pcd_graphset_writer_close.argtypes = [vp]
+    L.pcd_graphset_pack.argtypes = [C.c_char_p, C.c_int, vp]
     L.pcd_read_edge_text.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(I32P), C.POINTER(I32P),
                                      C.POINTER(C.c_int)]
     L.pcd_convert_edges.argtypes = [C.c_int, C.c_int64, vp, vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -215,6 +227,84 @@ def read_pcd(path):
 
 def write_pcd(path, n, m, indices, adj, mp):
     check(lib().pcd_write_pcd(path.encode(), n, m, _p(_i32(indices)), _p(_i32(adj)), _p(_i32(mp))))
+
+
+class GraphSet:
+    """A graph set file opened read-only (pcd_graphset_open): many PCD graphs
+    in one mapping.  graph(i) returns (n, m, indices, adj, map) as read-only
+    numpy views of the mapping; they keep the set open while they live."""
+
+    def __init__(self, path):
+        h = C.c_void_p()
+        check(lib().pcd_graphset_open(path.encode(), C.byref(h)))
+        self._h = h
+        c = C.c_int()
+        check(lib().pcd_graphset_count(h, C.byref(c)))
+        self.count = c.value
+
+    def __len__(self):
+        return self.count
+
+    def _view(self, ptr, count):
+        arr = (C.c_int32 * max(count, 1)).from_address(C.cast(ptr, C.c_void_p).value)
+        arr._owner = self  # the mapping stays while a view refers to it
+        out = np.frombuffer(arr, dtype=np.int32, count=count)
+        out.flags.writeable = False
+        return out
+
+    def graph(self, i):
+        if self._h is None:
+            raise PcdError("graph set is closed")
+        n, m = C.c_int(), C.c_int()
+        pi, pa, pm = I32P(), I32P(), I32P()
+        check(lib().pcd_graphset_graph(self._h, int(i), C.byref(n), C.byref(m), C.byref(pi), C.byref(pa),
+                                       C.byref(pm)))
+        n, m = n.value, m.value
+        return n, m, self._view(pi, n + 1), self._view(pa, 2 * m), self._view(pm, n)
+
+    def extract(self, i, path):
+        check(lib().pcd_graphset_extract(self._h, int(i), path.encode()))
+
+    def close(self):
+        """Unmaps the file; only call it when no view of it is in use."""
+        if self._h is not None:
+            lib().pcd_graphset_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_graph_set(path, graphs):
+    """Writes graphs, each (n, m, indices, adj) or (n, m, indices, adj, map),
+    as one graph set (pcd_graphset_writer_*)."""
+    graphs = list(graphs)
+    w = C.c_void_p()
+    check(lib().pcd_graphset_writer_open(path.encode(), len(graphs), C.byref(w)))
+    try:
+        for g in graphs:
+            mp = _i32(g[4]) if len(g) > 4 and g[4] is not None else None
+            rc = lib().pcd_graphset_writer_add(w, int(g[0]), int(g[1]), _p(_i32(g[2])), _p(_i32(g[3])),
+                                              _p(mp) if mp is not None else None)
+            check(rc)
+    finally:
+        rc = lib().pcd_graphset_writer_close(w)
+    check(rc)
+
+
+def pack_graph_set(path, pcd_paths):
+    """Packs PCD files into one graph set (pcd_graphset_pack)."""
+    arr = (C.c_char_p * max(len(pcd_paths), 1))(*[p.encode() for p in pcd_paths])
+    check(lib().pcd_graphset_pack(path.encode(), len(pcd_paths), C.cast(arr, C.c_void_p)))
 
 
 def read_edge_text(path):

@@ -335,3 +335,24 @@ def test_batch_tiny_graphs_with_isolated_vertices():
         ol, osc, oit, ok, _ = O.run(n, m, ind, adj, kind=jd["kind"], lam=jd.get("lam", 0.0), max_iter=4, mode=1,
                                     seed=jd["seed"], tie=jd.get("tie", O.TIE_RANDOM), init_labels=lab)
         assert np.array_equal(bl, ol) and iters == oit and k == ok
+
+
+def test_batch_admits_a_graph_set_in_place(tmp_path):
+    """Graphs read from a graph set file (pointers into its read-only
+    mapping, no host copies) are admitted with pcd_batch_add_many and end
+    exactly where the same graphs admitted from their own arrays end."""
+    gs = graphs()
+    path = str(tmp_path / "batch.pcds")
+    parcd.write_graph_set(path, [g[:4] for g in gs])
+    items = [(g, singleton_labels(g[0]), parcd.job(**JOBS[i % len(JOBS)])) for i, g in enumerate(gs)]
+    ref = run_batch(items)
+    with parcd.GraphSet(path) as s, parcd.Batch(0, max_graphs=16, arena_bytes=256 << 20) as b:
+        views = [s.graph(i)[:4] for i in range(len(s))]
+        for v, g in zip(views, gs):
+            assert v[:2] == g[:2] and np.array_equal(v[2], g[2]) and np.array_equal(v[3], g[3])
+        slots = b.add_many(views, [x[1] for x in items], [x[2] for x in items])
+        assert all(x is not None for x in slots)
+        while b.step(1):
+            pass
+        for i, sl in enumerate(slots):
+            assert np.array_equal(b.labels(sl), ref[i][0]) and b.status(sl)[1:] == ref[i][1:4], i
