@@ -103,7 +103,7 @@ ROCPROF_NAMES = {
     "lp_hub": [r"pcd::k_hub_partition<false>$", r"pcd::k_hub_count_listed<1u, 512, 4096, false>$"],
     "lp_g64": [r"pcd::k_lp_small<64, 8, false, 1u"], "lp_g32": [r"pcd::k_lp_small<32, 16, false, 1u"],
     "lp_g16": [r"pcd::k_lp_small<16, 32, false, 1u"], "lp_g8": [r"pcd::k_lp_small<8, 64, false, 1u"],
-    "lp_g4": [r"pcd::k_lp_small<4, 64, false, 1u"], "lp_small": [r"pcd::k_lp_small_all<1u>"]}
+    "lp_g4": [r"pcd::k_lp_small<4, 64, false, 1u"], "lp_small": [r"pcd::k_lp_small_all<1u, (1|8)>"]}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "rmat22_mod_pmc.md")
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0  # stream-copy calibration of FETCH_SIZE / WRITE_SIZE
 

@@ -772,6 +772,8 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.hub_aux = value != 0;
     } else if (k == "small_fused") {
         ctx->tune.small_fused = value != 0;
+    } else if (k == "small_occ8") {
+        ctx->tune.small_occ8 = value != 0;
     } else if (k == "hub_nt") {
         if (value != 256 && value != 512) return fail(PCD_E_INVALID, "hub_nt: 256 or 512");
         ctx->tune.hub_nt = value;

@@ -55,6 +55,8 @@ struct Tuning {
     int hub_aux = 0;             // hub partitions count {label, aux} records (1) or labels (0)
     int hub_nt = 512;            // hub partition workgroups: 512 threads, 4096 slots or 256, 2048
     int small_fused = 1;         // the G64..G4 bins in one launch (1) or one launch each (0)
+    int small_occ8 = 1;          // the fused small-bin launch compiled for 8 waves per SIMD
+                                 // (modularity and CPM builds; registers beyond spill to lanes)
     int stats_cache = 1;         // statistics over long rows read the labels of the first
                                  // STATS_CACHE vertices (the highest degrees) from LDS
     int inc_stats = 1;           // a propagation step updates the statistics from the rows

@@ -494,8 +494,11 @@ struct SmallBins {
     int lo[7];        // workgroup range of bin b: [lo[b], lo[b + 1])
 };
 
-template <unsigned KM>
-__global__ __launch_bounds__(256) void k_lp_small_all(SmallBins sb) {
+// WPE: waves per SIMD the registers are allocated for (1: the compiler's
+// choice).  At 8 the modularity and CPM builds fit 64 VGPRs without scratch;
+// the scalar registers beyond 96 spill to VGPR lanes.
+template <unsigned KM, int WPE = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_lp_small_all(SmallBins sb) {
     __shared__ int s_keys[256 / WAVE][512];
     __shared__ int s_cnts[256 / WAVE][512];
     __shared__ uint16_t s_occ[256 / WAVE][256];
@@ -1097,7 +1100,11 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
             }
             sb.lo[6] = at;
             kt_begin(ctx, KS_LP_SMALL);
-            hipLaunchKernelGGL(k_lp_small_all<KM>, dim3(at), dim3(256), 0, ctx->stream, sb);
+            if (ctx->tune.small_occ8 && (KM == RK_MOD || KM == RK_CPM))
+                hipLaunchKernelGGL((k_lp_small_all<KM, (KM == RK_MOD || KM == RK_CPM) ? 8 : 1>), dim3(at), dim3(256),
+                                   0, ctx->stream, sb);
+            else
+                hipLaunchKernelGGL(k_lp_small_all<KM>, dim3(at), dim3(256), 0, ctx->stream, sb);
             kt_end(ctx, KS_LP_SMALL);
         }
     } else {

@@ -166,19 +166,21 @@ def test_propagation_bin_zoo(ctx, kind, lam, tie, aux):
 
 
 @pytest.mark.parametrize("kind,lam", RULES)
-def test_propagation_small_bins_one_launch_each(ctx, kind, lam):
-    """The G64..G4 bins launched one kernel each (small_fused 0) agree with
-    the oracle as the single fused launch does."""
+@pytest.mark.parametrize("knob", ["small_fused", "small_occ8"])
+def test_propagation_small_bins_other_launch_forms(ctx, kind, lam, knob):
+    """The small-degree bins launched one kernel each (small_fused 0), and the
+    fused launch built for the compiler's own occupancy (small_occ8 0), agree
+    with the oracle as the default fused launch does."""
     g = GG.bin_zoo()
     n = g[0]
     rng = np.random.default_rng(9)
-    ctx.set_tuning("small_fused", 0)
+    ctx.set_tuning(knob, 0)
     try:
         for lab in [GG.singleton_labels(n), rng.integers(0, 200, n)]:
             _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=99, step=1, tie=O.TIE_RANDOM)
             _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=7, step=0, tie=O.TIE_LOWEST)
     finally:
-        ctx.set_tuning("small_fused", 1)
+        ctx.set_tuning(knob, 1)
 
 
 @pytest.mark.parametrize("inc", [1, 0])
