@@ -774,6 +774,8 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.small_fused = value != 0;
     } else if (k == "small_occ8") {
         ctx->tune.small_occ8 = value != 0;
+    } else if (k == "b1024_u16") {
+        ctx->tune.b1024_u16 = value != 0;
     } else if (k == "hub_nt") {
         if (value != 256 && value != 512) return fail(PCD_E_INVALID, "hub_nt: 256 or 512");
         ctx->tune.hub_nt = value;

@@ -63,6 +63,8 @@ struct Tuning {
                                  // of the vertices it moved (else a full recount)
     int b1024_wave = 1;          // degrees 257-512 of B1024 run one wave per vertex with a
                                  // 1024-slot table (twice the vertices in flight per CU)
+    int b1024_u16 = 1;           // those wave tables keep 16-bit counts (five workgroups per
+                                 // CU instead of four)
     int b4096_split = 0;         // degrees 1025-2048 of B4096 run 256-thread workgroups with a
                                  // 4096-slot table (measured slower on R-MAT-22: off)
     int cluster_rounds = 128;    // device rounds of the clustering partition before the

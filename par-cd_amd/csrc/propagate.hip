@@ -316,17 +316,26 @@ __device__ inline int round64(int x) { return (x + 63) & ~63; }
 // row's distinct labels, so probing always ends at a free or matching slot.
 __device__ inline uint32_t pslot(int key, int log_t) { return ((uint32_t)key * 0x9E3779B1u) >> (32 - log_t); }
 
+// Counts of a table: 32-bit, or 16-bit where a row's degree bounds them
+// below 2^16.  A 16-bit count is incremented by a 32-bit LDS atomic on the
+// word holding it (the other half cannot carry into it); it is read and
+// cleared with 16-bit accesses.
+__device__ inline void count_add(int *cnts, uint32_t h) { atomicAdd(&cnts[h], 1); }
+__device__ inline void count_add(uint16_t *cnts, uint32_t h) {
+    atomicAdd((unsigned *)(cnts + (h & ~1u)), 1u << ((h & 1u) * 16));
+}
+
 // Inserts `key` (with AUX its aux, stored by the lane that claims the slot)
 // with count 1; returns true when this lane claimed a free slot, whose
 // index lands in `slot`.
-template <bool AUX>
-__device__ inline bool pinsert(int *keys, int *cnts, int *auxs, int mask, int log_t, int key, int aux, int &slot) {
+template <bool AUX, class CT = int>
+__device__ inline bool pinsert(int *keys, CT *cnts, int *auxs, int mask, int log_t, int key, int aux, int &slot) {
     uint32_t h = pslot(key, log_t);
     while (true) {
         int old = atomicCAS(&keys[h], EMPTY_KEY, key);
         if (AUX && old == EMPTY_KEY) auxs[h] = aux;
         if (old == EMPTY_KEY || old == key) {
-            atomicAdd(&cnts[h], 1);
+            count_add(cnts, h);
             slot = (int)h;
             return old == EMPTY_KEY;
         }
@@ -352,8 +361,8 @@ __device__ inline void occ_append(bool claimed, int slot, uint16_t *occ, int *no
 // base + lo + e * stride, PF at a time (independent loads in flight), and
 // inserts them; claimed slots go to the slot list.  The loop bounds are the
 // same for every lane of the workgroup.
-template <int PF, bool AUX = false>
-__device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int *auxs, uint16_t *occ, int *nocc,
+template <int PF, bool AUX = false, class CT = int>
+__device__ inline void insert_labels(const LPArgs &a, int *keys, CT *cnts, int *auxs, uint16_t *occ, int *nocc,
                                      int mask, int log_t, int start, int deg, int lo, int stride) {
     for (int base = 0; base < deg; base += PF * stride) {
         int nb[PF];
@@ -370,7 +379,8 @@ __device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int 
         for (int e = 0; e < PF; e++) {
             bool claimed = false;
             int slot = 0;
-            if (rec[e].x != EMPTY_KEY) claimed = pinsert<AUX>(keys, cnts, auxs, mask, log_t, rec[e].x, rec[e].y, slot);
+            if (rec[e].x != EMPTY_KEY)
+                claimed = pinsert<AUX, CT>(keys, cnts, auxs, mask, log_t, rec[e].x, rec[e].y, slot);
             occ_append(claimed, slot, occ, nocc);
         }
     }
@@ -379,8 +389,8 @@ __device__ inline void insert_labels(const LPArgs &a, int *keys, int *cnts, int 
 // Scans the D listed slots [lo, D) with stride `stride` for candidates, SB
 // at a time so that their community-record gathers overlap, and clears each
 // slot it read (the table is empty again for the next row).
-template <bool AUX, unsigned KM>
-__device__ inline void scan_listed(const LPArgs &a, int *keys, int *cnts, const int *auxs, const uint16_t *occ, int D,
+template <bool AUX, unsigned KM, class CT = int>
+__device__ inline void scan_listed(const LPArgs &a, int *keys, CT *cnts, const int *auxs, const uint16_t *occ, int D,
                                    int lo, int stride, int deg, int own, uint64_t tb, Cand &c, int &own_cnt) {
     for (int i0 = lo; i0 < D; i0 += SB * stride) {
         int key[SB], cnt[SB];
@@ -429,8 +439,8 @@ __device__ inline int ceil_log2(int x) { return x <= 1 ? 0 : 32 - __clz(x - 1); 
 // The table starts empty and each row's scan clears the slots it used.
 // One wave per vertex over the units of src, with this wave's LDS table
 // (TW slots of keys, counts and aux, a slot list of TW / 2 entries).
-template <int TW, unsigned KM, class Src, bool AUX, int PF>
-__device__ inline void lp_wave(const Src &src, int wave_g, int nwaves, int *keys, int *cnts, int *auxs, uint16_t *occ,
+template <int TW, unsigned KM, class Src, bool AUX, int PF, class CT = int>
+__device__ inline void lp_wave(const Src &src, int wave_g, int nwaves, int *keys, CT *cnts, int *auxs, uint16_t *occ,
                                int *nocc) {
     const int lane = __lane_id();
     for (int i = lane; i < TW; i += WAVE) {
@@ -451,13 +461,13 @@ __device__ inline void lp_wave(const Src &src, int wave_g, int nwaves, int *keys
         int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
         int own = ownr.x;
         int log_t = max(6, ceil_log2(2 * deg));
-        insert_labels<PF, AUX>(a, keys, cnts, auxs, occ, nocc, (1 << log_t) - 1, log_t, start, deg, lane, WAVE);
+        insert_labels<PF, AUX, CT>(a, keys, cnts, auxs, occ, nocc, (1 << log_t) - 1, log_t, start, deg, lane, WAVE);
         wave_sync();
         const int D = *nocc;
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
         Cand c = cand_none();
         int own_cnt = 0;
-        scan_listed<AUX, KM>(a, keys, cnts, auxs, occ, D, lane, WAVE, deg, own, tb, c, own_cnt);
+        scan_listed<AUX, KM, CT>(a, keys, cnts, auxs, occ, D, lane, WAVE, deg, own, tb, c, own_cnt);
         wave_reduce_cand(c, own_cnt);
         if (lane == 0) {
             a.new_labels[v] = AUX ? lp_decide<KM>(a, deg, own, ownr.y, own_cnt, c)
@@ -470,17 +480,18 @@ __device__ inline void lp_wave(const Src &src, int wave_g, int nwaves, int *keys
     }
 }
 
-template <int TW, unsigned KM, class Src, bool AUX = false, int PF = 4>
+// CT = uint16_t needs degrees below 2^16 (the table's counts).
+template <int TW, unsigned KM, class Src, bool AUX = false, int PF = 4, class CT = int>
 __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
     __shared__ int s_keys[256 / WAVE][TW];
-    __shared__ int s_cnts[256 / WAVE][TW];
+    __shared__ __align__(16) CT s_cnts[256 / WAVE][TW];
     __shared__ int s_auxs[256 / WAVE][AUX ? TW : 1];
     __shared__ uint16_t s_occ[256 / WAVE][TW / 2];
     __shared__ int s_nocc[256 / WAVE];
     const int w = threadIdx.x / WAVE;
     const int wave_g = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
-    lp_wave<TW, KM, Src, AUX, PF>(src, wave_g, gridDim.x * blockDim.x / WAVE, s_keys[w], s_cnts[w], s_auxs[w],
-                                  s_occ[w], &s_nocc[w]);
+    lp_wave<TW, KM, Src, AUX, PF, CT>(src, wave_g, gridDim.x * blockDim.x / WAVE, s_keys[w], s_cnts[w], s_auxs[w],
+                                      s_occ[w], &s_nocc[w]);
 }
 
 // The W256 and G64..G4 bins in one launch: the grid is split into
@@ -1041,6 +1052,10 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
             if (block_aux)
                 hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - ps, 4, 3)),
                                    dim3(256), 0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B1024)});
+            else if (ctx->tune.b1024_u16)  // 16-bit counts (degree <= 512): 28.7 KB, five workgroups per CU
+                hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8, uint16_t>),
+                                   dim3(grid_for(ctx, p1 - ps, 4, 5)), dim3(256), 0, ctx->stream,
+                                   RangeSrc{ps, p1, 1, args_for(BIN_B1024)});
             else  // 4-8 entries per lane: all eight gathers in flight at once
                 hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8>), dim3(grid_for(ctx, p1 - ps, 4, 4)),
                                    dim3(256), 0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B1024)});

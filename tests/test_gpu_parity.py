@@ -166,21 +166,41 @@ def test_propagation_bin_zoo(ctx, kind, lam, tie, aux):
 
 
 @pytest.mark.parametrize("kind,lam", RULES)
-@pytest.mark.parametrize("knob", ["small_fused", "small_occ8"])
-def test_propagation_small_bins_other_launch_forms(ctx, kind, lam, knob):
-    """The small-degree bins launched one kernel each (small_fused 0), and the
-    fused launch built for the compiler's own occupancy (small_occ8 0), agree
-    with the oracle as the default fused launch does."""
+@pytest.mark.parametrize("knob", ["small_fused", "small_occ8", "b1024_u16"])
+def test_propagation_other_launch_forms(ctx, kind, lam, knob):
+    """The small-degree bins launched one kernel each (small_fused 0), the
+    fused launch built for the compiler's own occupancy (small_occ8 0), and
+    the degree 257-512 wave tables with 32-bit counts (b1024_u16 0) agree with
+    the oracle as the default launches do."""
     g = GG.bin_zoo()
     n = g[0]
     rng = np.random.default_rng(9)
+    # every degree 257-512 (SBM blocks of 600 at p_in 0.6): block labels,
+    # one label and two alternating ones put counts of up to the degree into
+    # both halves of the 16-bit count words
+    big = GG.sbm(4, 600, 0.6, 0.01, seed=21)
+    nb = big[0]
+    assert np.all((np.diff(big[2]) > 256) & (np.diff(big[2]) <= 512))
+    cases = [(g, [GG.singleton_labels(n), rng.integers(0, 200, n)]),
+             (big, [big[4] // 600, np.zeros(nb, np.int64), np.arange(nb) % 2])]
     ctx.set_tuning(knob, 0)
     try:
-        for lab in [GG.singleton_labels(n), rng.integers(0, 200, n)]:
-            _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=99, step=1, tie=O.TIE_RANDOM)
-            _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=7, step=0, tie=O.TIE_LOWEST)
+        for gr, labs in cases:
+            for lab in labs:
+                _lp_compare(ctx, gr, lab, kind, lam, 0.5, seed=99, step=1, tie=O.TIE_RANDOM)
+                _lp_compare(ctx, gr, lab, kind, lam, 1.0, seed=7, step=0, tie=O.TIE_LOWEST)
     finally:
         ctx.set_tuning(knob, 1)
+
+
+@pytest.mark.parametrize("kind,lam", RULES)
+def test_propagation_degree_257_512_rows(ctx, kind, lam):
+    """The same rows under the default launches (16-bit counts)."""
+    big = GG.sbm(4, 600, 0.6, 0.01, seed=21)
+    nb = big[0]
+    for lab in [big[4] // 600, np.zeros(nb, np.int64), np.arange(nb) % 2, GG.singleton_labels(nb)]:
+        _lp_compare(ctx, big, lab, kind, lam, 0.5, seed=99, step=1, tie=O.TIE_RANDOM)
+        _lp_compare(ctx, big, lab, kind, lam, 1.0, seed=7, step=0, tie=O.TIE_LOWEST)
 
 
 @pytest.mark.parametrize("inc", [1, 0])
