@@ -36,10 +36,13 @@ extern "C" {
 #define PCD_RULE_CPM 1
 #define PCD_RULE_WCC 2
 #define PCD_RULE_CONDUCTANCE 3
+/* a rule with no device form (the default of parcd::Rule::descriptor in the
+ * C++ mirror): refused with PCD_E_INVALID by every call that takes a rule */
+#define PCD_RULE_HOST (-1)
 
 /* Tie order among equal best gains.  RANDOM is a uniform choice keyed by a
  * counter hash of (seed, step, vertex|community, label) -- the counterpart of
- * the reservoir sampling at refinement.cpp:200-204,291-294; LOWEST takes the
+ * the reservoir sampling at refinement.cpp:158-162,249-252; LOWEST takes the
  * lowest label (the deterministic synchronous mode). */
 #define PCD_TIE_RANDOM 0
 #define PCD_TIE_LOWEST 1
@@ -91,7 +94,7 @@ int pcd_metric(pcd_ctx *ctx, pcd_rule rule, double *score);
 
 /* ------------------------------------------------------------- refinement */
 /* One label-propagation step, PropagationRefinement::refine
- * (refinement.cpp:156-224): each vertex is active with probability frac
+ * (refinement.cpp:114-182): each vertex is active with probability frac
  * (counter hash of (seed, step, v) against frac*RAND_MAX); an active vertex
  * joins the neighbour community of best score_join if best + score_leave > 0
  * and > score_leave, else leaves to a fresh singleton if score_leave > 0, else
@@ -101,7 +104,7 @@ int pcd_metric(pcd_ctx *ctx, pcd_rule rule, double *score);
 int pcd_propagation_step(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie);
 
 /* One parallel-merging step, MergingRefinement::refine
- * (refinement.cpp:227-330): per round every unmatched community picks the
+ * (refinement.cpp:185-288): per round every unmatched community picks the
  * neighbour community of best score_merge (>= 0), matched pairs are found by
  * handshake, and matched communities take min(l, partner).  Community-pair
  * link counts come from the contraction below.  Needs current statistics. */
@@ -109,12 +112,12 @@ int pcd_merging_step(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int
 
 /* Contraction: the community graph of the current labels as the list of
  * (l, k, links) for l != k, sorted by (l, k) -- the table MergingRefinement
- * builds at refinement.cpp:270-280 and framework-cuda.cu:744-807 computes by
+ * builds at refinement.cpp:228-238 and framework-cuda.cu:744-807 computes by
  * sort + reduce-by-key.  *num_pairs receives U; arrays (may be NULL) need U
  * entries; call first with NULL arrays to get U. */
 int pcd_contract(pcd_ctx *ctx, int64_t *num_pairs, int32_t *l, int32_t *k, int32_t *links);
 
-/* Graph::count_triangles (graph.cpp:150-218): per-vertex tri (twice the
+/* Graph::count_triangles (graph.cpp:107-175): per-vertex tri (twice the
  * triangles at v), tri_deg (incident edges on a triangle) and the mean local
  * clustering coefficient cc.  Results stay on the device for WCC; host
  * arrays (n entries) and cc may be NULL. */
@@ -400,13 +403,13 @@ int pcd_clustering_partition(int n, const int32_t *indices, const int32_t *adj, 
                              int32_t *labels);
 
 /* Normalized mutual information of two labelings of the n vertices
- * (common.cpp:170-176 compare_communities(g, a, b) with
- * common.cpp:75-128 norm_mutual_info; labels >= 0).  As in the reference
+ * (common.cpp:165-173 compare_communities(g, a, b) with
+ * common.cpp:75-127 norm_mutual_info; labels >= 0).  As in the reference
  * the count product n_i * n_j is int32 arithmetic: past 2^31 it wraps, and
  * the result can exceed 1 or be NaN; both labelings with one community
  * give 0/0 = NaN. */
 int pcd_nmi(int n, const int32_t *a, const int32_t *b, double *nmi);
-/* The same against a ground-truth file (common.cpp:130-168
+/* The same against a ground-truth file (common.cpp:129-163
  * compare_communities(g, labels, filename)): each line that is not empty and
  * does not start with '#' lists the vertices of one community; N stays n. */
 int pcd_nmi_file(int n, const int32_t *labels, const char *path, double *nmi);

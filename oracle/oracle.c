@@ -57,7 +57,7 @@ uint64_t orc_hash3(uint64_t seed, uint32_t a, uint32_t b) {
     return orc_mix64(seed + 0x9E3779B97F4A7C15ULL * x);
 }
 
-/* reference: refinement.cpp:158,171 `f = min(frac,1)*RAND_MAX; rand_r() > f -> skip` */
+/* reference: refinement.cpp:116,129 `f = min(frac,1)*RAND_MAX; rand_r() > f -> skip` */
 int orc_frac_threshold(double frac) {
     double f = frac < 1.0 ? frac : 1.0;
     return (int)(f * 2147483647.0);
@@ -286,7 +286,7 @@ int orc_read_pcd(const char *path, int *n_out, int *m_out, int **indices, int **
     int err = fread(ind, 4, (size_t)n + 1, fp) != (size_t)n + 1;
     err |= fread(a, 4, (size_t)2 * m, fp) != (size_t)2 * m;
     err |= fread(mp, 4, (size_t)n, fp) != (size_t)n;
-    if (!err && fgetc(fp) != EOF) err = 1; /* graph.cpp:133-135 trailing bytes */
+    if (!err && fgetc(fp) != EOF) err = 1; /* graph.cpp:90-93 trailing bytes */
     fclose(fp);
     if (err) { free(ind); free(a); free(mp); return -4; }
     *n_out = n; *m_out = m; *indices = ind; *adj = a; *map = mp;
@@ -294,14 +294,14 @@ int orc_read_pcd(const char *path, int *n_out, int *m_out, int **indices, int **
 }
 
 /* ------------------------------------------------------------------------ */
-/* triangles (graph.cpp:150-218 Graph::count_triangles)                     */
+/* triangles (graph.cpp:107-175 Graph::count_triangles)                     */
 /* ------------------------------------------------------------------------ */
 
 void orc_count_triangles(int n, const int *indices, const int *adj, int *et, int *tri, int *tri_deg, double *cc) {
     long E = indices[n];
     for (long i = 0; i < E; i++) et[i] = 0;
     for (int i = 0; i < n; i++) { tri[i] = 0; tri_deg[i] = 0; }
-    /* graph.cpp:160-187 */
+    /* graph.cpp:117-144 */
     for (int v = 0; v < n; v++) {
         for (int r = indices[v]; r < indices[v + 1]; r++) {
             int u = adj[r];
@@ -315,7 +315,7 @@ void orc_count_triangles(int n, const int *indices, const int *adj, int *et, int
             }
         }
     }
-    /* graph.cpp:189-202 */
+    /* graph.cpp:146-159 */
     for (int v = 0; v < n; v++) {
         for (int r = indices[v]; r < indices[v + 1]; r++) {
             int u = adj[r];
@@ -323,7 +323,7 @@ void orc_count_triangles(int n, const int *indices, const int *adj, int *et, int
             if (t > 0) { tri[v] += t; tri[u] += t; tri_deg[v]++; tri_deg[u]++; }
         }
     }
-    /* graph.cpp:204-212, num_threads = 1 (graph.hpp:23 default): sequential */
+    /* graph.cpp:161-169, num_threads = 1 (graph.hpp:23 default): sequential */
     double c = 0;
     for (int v = 0; v < n; v++) {
         int t = tri[v];
@@ -423,8 +423,8 @@ int orc_collect_statistics(int n, const int *indices, const int *adj, const int 
 /* rules (rules.cpp)                                                        */
 /* ------------------------------------------------------------------------ */
 
-/* rules.cpp:316-465 calculate_wcc_improvement; int32 sub-products wrap as
- * in the reference (rules.cpp:341,343,344,373,402-405,434). */
+/* rules.cpp:240-389 calculate_wcc_improvement; int32 sub-products wrap as
+ * in the reference (rules.cpp:243, 265, 267-268, 297, 326, 328-329, 358). */
 double orc_wcc_improvement(int n_x, int n_y, int c_x, int c_y, int m_x, int m_y, int m_xy,
                            int b_x, int b_y, double cc) {
     double p_x = n_x > 1 ? m_x / (n_x * (n_x - 1.0) / 2.0) : 0.0;
@@ -481,7 +481,7 @@ double orc_wcc_improvement(int n_x, int n_y, int c_x, int c_y, int m_x, int m_y,
 /* Scalar gain functions.  Arguments carry exactly the values the reference
  * reads from Graph/Partition. */
 
-/* rules.cpp:110-119 (mod), 212-215 (cpm), 479-493 (wcc), 169-178 (cond) */
+/* rules.cpp:34-43 (mod), 136-139 (cpm), 403-417 (wcc), 93-102 (cond) */
 double orc_score_join(int kind, double lambda, int m, double cc, int deg,
                       int size_c, int int_c, int ext_c, int num) {
     switch (kind) {
@@ -500,7 +500,7 @@ double orc_score_join(int kind, double lambda, int m, double cc, int deg,
     }
 }
 
-/* rules.cpp:121-132 (mod), 217-220 (cpm), 495-512 (wcc), 180-190 (cond) */
+/* rules.cpp:45-56 (mod), 141-144 (cpm), 419-436 (wcc), 104-114 (cond) */
 double orc_score_leave(int kind, double lambda, int m, double cc, int deg,
                        int size_own, int int_own, int ext_own, int num) {
     switch (kind) {
@@ -521,7 +521,7 @@ double orc_score_leave(int kind, double lambda, int m, double cc, int deg,
     }
 }
 
-/* rules.cpp:100-108 (mod), 207-210 (cpm), 467-477 (wcc), 152-167 (cond) */
+/* rules.cpp:24-32 (mod), 131-134 (cpm), 391-401 (wcc), 76-91 (cond) */
 double orc_score_merge(int kind, double lambda, int m, double cc,
                        int size_i, int int_i, int ext_i, int size_j, int int_j, int ext_j, int links) {
     switch (kind) {
@@ -545,7 +545,7 @@ double orc_score_merge(int kind, double lambda, int m, double cc,
     }
 }
 
-/* rules.cpp:226-311 WCCRule::calculate (sequential; the reference sums with
+/* rules.cpp:150-235 WCCRule::calculate (sequential; the reference sums with
  * an OpenMP reduction, so its last bits depend on the thread count) */
 double orc_wcc_calculate(int n, const int *indices, const int *adj, const int *labels,
                          const int *sizes, const int *tri, const int *tri_deg, int *it_out, int *itd_out) {
@@ -637,7 +637,7 @@ static double merge_v(const orc_view *g, int i, int j, int links) {
                            g->sizes[j], g->intv[j], g->extv[j], links);
 }
 
-/* refinement.cpp:156-224 PropagationRefinement::refine at one thread.
+/* refinement.cpp:114-182 PropagationRefinement::refine at one thread.
  * mode 0: libc RNG (one rand() per call for the thread seed, rand_r per
  *         vertex and per tie); mode 1: counter RNG + fixed tie order. */
 void orc_propagation_refine(int kind, double lambda, int n, int m, const int *indices, const int *adj,
@@ -684,7 +684,7 @@ void orc_propagation_refine(int kind, double lambda, int n, int m, const int *in
     ot_free(&t);
 }
 
-/* refinement.cpp:227-330 MergingRefinement::refine at one thread.
+/* refinement.cpp:185-288 MergingRefinement::refine at one thread.
  * k = number of labels (labels in [0,k)).  Optional outputs: best_out[k]
  * (partner, -1 for none) and found_out[k] after the last round. */
 void orc_merging_refine(int kind, double lambda, int n, int m, int k, const int *indices, const int *adj,
@@ -760,7 +760,7 @@ static int cmp_u64(const void *a, const void *b) {
 
 /* Community-pair link table (the integer content of contraction):
  * for each community l, the distinct neighbour communities k != l of its
- * members and their link counts (refinement.cpp:270-280 with no community
+ * members and their link counts (refinement.cpp:228-238 with no community
  * matched yet).  Output is sorted by (l, k).  Returns U, the number of
  * pairs; arrays must hold up to indices[n] entries. */
 long orc_community_pairs(int n, const int *indices, const int *adj, const int *labels, int k,
@@ -856,8 +856,8 @@ int orc_run(int n, int m, const int *indices, const int *adj, int kind, double l
 }
 
 /* ------------------------------------------------------------------ NMI
- * norm_mutual_info (common.cpp:75-128) over the confusion counts of
- * compare_communities (common.cpp:130-176).  The reference keeps the counts
+ * norm_mutual_info (common.cpp:75-127) over the confusion counts of
+ * compare_communities (common.cpp:129-173).  The reference keeps the counts
  * in a std::map keyed by (i, j): nom sums over pairs in ascending (i, j)
  * order, den over i ascending then j ascending; ni[i] * nj[j] is an int32
  * product (it wraps for products of 2^31 and more) converted to double
@@ -906,7 +906,7 @@ static double nmi_of_pairs(int n, uint64_t *keys, long cnt) {
     return -2 * nom / den;
 }
 
-/* compare_communities(g, a, b) (common.cpp:170-176); -1 on negative labels */
+/* compare_communities(g, a, b) (common.cpp:165-173); -1 on negative labels */
 int orc_nmi(int n, const int *a, const int *b, double *out) {
     uint64_t *keys = malloc(sizeof(uint64_t) * (n > 0 ? n : 1));
     for (int v = 0; v < n; v++) {
@@ -918,7 +918,7 @@ int orc_nmi(int n, const int *a, const int *b, double *out) {
     return 0;
 }
 
-/* compare_communities(g, labels, filename) (common.cpp:130-168): community
+/* compare_communities(g, labels, filename) (common.cpp:129-163): community
  * c (from 1, in line order; empty and '#' lines skipped) holds the integers
  * of its line up to the first token that is not one.  -1: unreadable file,
  * a vertex outside [0, n) or a negative label. */

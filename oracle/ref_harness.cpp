@@ -69,7 +69,7 @@ static const Graph *load(const char *path) {
     return g;
 }
 
-// graph <pcd> <out>: Graph::count_triangles (graph.cpp:150-218)
+// graph <pcd> <out>: Graph::count_triangles (graph.cpp:107-175)
 static int cmd_graph(int argc, char **argv) {
     const Graph *g = load(argv[2]);
     g->count_triangles();
@@ -161,7 +161,7 @@ static int cmd_part(int argc, char **argv) {
 }
 
 // synth <in> <out>: score functions on fabricated community statistics
-// (reaches the int32-overflow sub-expressions of rules.cpp:341-434).
+// (reaches the int32-overflow sub-expressions of rules.cpp:265-358).
 static int cmd_synth(int argc, char **argv) {
     map<string, Rec> in = read_recs(argv[2]);
     vector<int> &deg = in["deg"].i;
@@ -210,7 +210,7 @@ static int cmd_synth(int argc, char **argv) {
 }
 
 // refine <pcd> <in> <out>: one PropagationRefinement / MergingRefinement
-// step at one thread after srand(seed) (refinement.cpp:156-330).
+// step at one thread after srand(seed) (refinement.cpp:114-288).
 static int cmd_refine(int argc, char **argv) {
     const Graph *g = load(argv[2]);
     map<string, Rec> in = read_recs(argv[3]);
@@ -240,7 +240,7 @@ static int cmd_refine(int argc, char **argv) {
     return 0;
 }
 
-// nmi <pcd> <in> <out>: compare_communities(g, a, b) (common.cpp:170-176)
+// nmi <pcd> <in> <out>: compare_communities(g, a, b) (common.cpp:165-173)
 static int cmd_nmi(int argc, char **argv) {
     const Graph *g = load(argv[2]);
     map<string, Rec> in = read_recs(argv[3]);
@@ -255,7 +255,7 @@ static int cmd_nmi(int argc, char **argv) {
 }
 
 // nmifile <pcd> <in> <truth> <out>: compare_communities(g, labels, file)
-// (common.cpp:130-168)
+// (common.cpp:129-163)
 static int cmd_nmifile(int argc, char **argv) {
     if (argc < 6) return 2;
     const Graph *g = load(argv[2]);

@@ -979,7 +979,7 @@ static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, co
     g.lp.seed = job->seed;
     g.lp.step = 0;
     g.lp.tie = job->tie;
-    g.lp.f = (int)((job->frac < 1.0 ? job->frac : 1.0) * 2147483647.0);  // refinement.cpp:158
+    g.lp.f = (int)((job->frac < 1.0 ? job->frac : 1.0) * 2147483647.0);  // refinement.cpp:116
     g.lp.use_aux = 0;
     g.lp.counters = nullptr;
     g.next = d_next;

@@ -509,7 +509,7 @@ extern "C" int pcd_collect_statistics(pcd_ctx *ctx, int *num_labels) {
         return PCD_OK;
     }
     if (!ctx->labels_compressed) {
-        // framework.cpp:110,348: k = max label + 1
+        // framework.cpp:110: k = max label + 1
         PCD_TRY(fill_i32(ctx, ctx->d_k, 0, 1));
         hipLaunchKernelGGL(k_max_label, dim3(grid_for(ctx, ctx->n, 256, 4)), dim3(256), 0, ctx->stream, ctx->n,
                            ctx->d_labels, ctx->d_k);

@@ -1,19 +1,19 @@
-// Parallel merging (MergingRefinement::refine, refinement.cpp:227-330) and
+// Parallel merging (MergingRefinement::refine, refinement.cpp:185-288) and
 // contraction on gfx950.
 //
 // Contraction: every CSR entry (v, u) writes the 2b-bit key (l_v << b | l_u)
 // (or an all-ones sentinel when l_v == l_u) into a buffer indexed by the
 // entry itself, the buffer is radix-sorted, and a run-length reduce-by-key
 // (scan of run heads) gives the community graph as unique (l, k, links)
-// sorted by (l, k): the table refinement.cpp:270-280 hashes per community
+// sorted by (l, k): the table refinement.cpp:228-238 hashes per community
 // and framework-cuda.cu:744-807 sorts.
 //
 // Merging round: each unmatched, eligible community l takes the neighbour
-// community k (unmatched) of best score_merge >= 0 (refinement.cpp:282-302;
+// community k (unmatched) of best score_merge >= 0 (refinement.cpp:240-260;
 // a zero score is accepted, as there) using two order-independent
 // atomics per l (max of the order-preserving score bits, then min of
-// (tie key, k)); handshake (refinement.cpp:309-316); relabel to
-// min(l, partner) (refinement.cpp:319-328).
+// (tie key, k)); handshake (refinement.cpp:267-274); relabel to
+// min(l, partner) (refinement.cpp:277-286).
 #include "prims.h"
 
 namespace pcd {
@@ -180,7 +180,7 @@ __global__ void k_unpack_pairs(const uint64_t *ukeys, int64_t U, int half, int32
 
 // ---------------------------------------------------------------- rounds
 __device__ inline bool eligible(int l, const int32_t *found, const int32_t *bp) {
-    // refinement.cpp:266: skip matched communities and ones waiting on an
+    // refinement.cpp:224: skip matched communities and ones waiting on an
     // unmatched partner
     if (found[l]) return false;
     int b = bp[l];
@@ -379,7 +379,7 @@ int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step
         kt_end(ctx, KS_MERGE_ROUNDS);
         PCD_HIP(hipGetLastError());
         if (rnd + 1 < rounds) {
-            // refinement.cpp:262,306-316: stop once every community is matched
+            // refinement.cpp:220,264-274: stop once every community is matched
             PCD_HIP(hipMemcpyAsync(ctx->h_small, unmatched, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
             PCD_HIP(hipStreamSynchronize(ctx->stream));
             if (ctx->h_small[0] == 0) break;

@@ -1,4 +1,4 @@
-// Label propagation (PropagationRefinement::refine, refinement.cpp:156-224)
+// Label propagation (PropagationRefinement::refine, refinement.cpp:114-182)
 // on gfx950.
 //
 // Per active vertex v the step needs, for every distinct neighbour label c,
@@ -99,7 +99,7 @@ __device__ inline CommStats comm_rec(const LPArgs &a, int label) {
     }
 }
 
-// refinement.cpp:211-221: best_score (0 if no join candidate beat 0) plus
+// refinement.cpp:169-179: best_score (0 if no join candidate beat 0) plus
 // score_leave decides between moving, leaving to a new singleton, staying.
 template <unsigned KM>
 __device__ inline int lp_decide(const LPArgs &a, int deg, int own, int own_aux, int own_cnt, const Cand &best) {
@@ -257,8 +257,10 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
                 a.new_labels[u] = AUX ? lp_decide<KM>(a, deg, own.x, own.y, own_cnt, c)
                                       : lp_decide_rec<KM>(a, deg, own.x, own_rec, own_cnt, c);
             if (a.counters) {
+                // every lane votes: a wave holds up to 64 / G active vertices
+                const int act = __popcll(__ballot(has && glane == 0));
                 int ent = wave_sum((has && glane < deg) ? 1 : 0);
-                if (lane == 0) count3(a, __popcll(__ballot(has && glane == 0)), ent, __popcll(heads));
+                if (lane == 0) count3(a, act, ent, __popcll(heads));
             }
         }
         wave_sync();
@@ -1153,7 +1155,7 @@ int propagation_launch(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, 
     a.seed = seed;
     a.step = step;
     a.tie = tie;
-    a.f = (int)(fr * 2147483647.0);  // refinement.cpp:158
+    a.f = (int)(fr * 2147483647.0);  // refinement.cpp:116
     a.use_aux = rule.kind == PCD_RULE_MODULARITY || rule.kind == PCD_RULE_CPM;
     a.tot = a.use_aux ? ctx->d_tot : nullptr;
     a.counters = nullptr;

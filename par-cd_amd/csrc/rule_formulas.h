@@ -39,7 +39,7 @@ struct RuleParams {
 
 PCD_HD inline int wrap_mul(int a, int b) { return (int)((unsigned)a * (unsigned)b); }
 
-// rules.cpp:316-465 calculate_wcc_improvement
+// rules.cpp:240-389 calculate_wcc_improvement
 PCD_HD inline double wcc_improvement(int n_x, int n_y, int c_x, int c_y, int m_x, int m_y, int m_xy,
                                                   int b_x, int b_y, double cc) {
     double p_x = n_x > 1 ? m_x / (n_x * (n_x - 1.0) / 2.0) : 0.0;
@@ -98,7 +98,7 @@ constexpr unsigned RK_WCC = 1u << PCD_RULE_WCC;
 constexpr unsigned RK_COND = 1u << PCD_RULE_CONDUCTANCE;
 constexpr unsigned RK_ALL = RK_MOD | RK_CPM | RK_WCC | RK_COND;
 
-// Rule::score_join (rules.cpp:110-119, 212-215, 479-493, 169-178)
+// Rule::score_join (rules.cpp:34-43, 136-139, 403-417, 93-102)
 template <unsigned KM = RK_ALL>
 PCD_HD inline double score_join(const RuleParams &r, int deg, CommStats c, int num) {
     if ((KM & RK_MOD) && (KM == RK_MOD || r.kind == PCD_RULE_MODULARITY)) {
@@ -117,7 +117,7 @@ PCD_HD inline double score_join(const RuleParams &r, int deg, CommStats c, int n
     return 0.0;  // kind outside KM: never called so
 }
 
-// Rule::score_leave (rules.cpp:121-132, 217-220, 495-512, 180-190)
+// Rule::score_leave (rules.cpp:45-56, 141-144, 419-436, 104-114)
 template <unsigned KM = RK_ALL>
 PCD_HD inline double score_leave(const RuleParams &r, int deg, CommStats c, int num) {
     if ((KM & RK_MOD) && (KM == RK_MOD || r.kind == PCD_RULE_MODULARITY)) {
@@ -137,7 +137,7 @@ PCD_HD inline double score_leave(const RuleParams &r, int deg, CommStats c, int 
     return 0.0;
 }
 
-// Rule::score_merge (rules.cpp:100-108, 207-210, 467-477, 152-167)
+// Rule::score_merge (rules.cpp:24-32, 131-134, 391-401, 76-91)
 PCD_HD inline double score_merge(const RuleParams &r, CommStats a, CommStats b, int links) {
     switch (r.kind) {
     case PCD_RULE_MODULARITY: {
@@ -171,7 +171,7 @@ PCD_HD inline uint64_t hash3(uint64_t seed, uint32_t a, uint32_t b) {
     uint64_t x = (((uint64_t)a << 32) | (uint64_t)b) + 1ULL;
     return mix64(seed + 0x9E3779B97F4A7C15ULL * x);
 }
-// refinement.cpp:158,171: active iff rand <= (int)(min(frac,1) * RAND_MAX);
+// refinement.cpp:116,129: active iff rand <= (int)(min(frac,1) * RAND_MAX);
 // rand is the top 31 bits of hash3(seed, step, v).
 PCD_HD inline bool selected(uint64_t seed, int step, int v, int f) {
     return (int)(uint32_t)(hash3(seed, (uint32_t)step, (uint32_t)v) >> 33) <= f;

@@ -1,5 +1,5 @@
-// Triangle counting (Graph::count_triangles, graph.cpp:150-218) and the
-// internal-triangle pass of WCCRule::calculate (rules.cpp:226-311).
+// Triangle counting (Graph::count_triangles, graph.cpp:107-175) and the
+// internal-triangle pass of WCCRule::calculate (rules.cpp:150-235).
 //
 // For each vertex v and each neighbour u < v at row position r, the
 // reference intersects N(v) ∩ [0,u) -- which, rows being ascending, is the
@@ -10,7 +10,7 @@
 // take the entries of N(u) ∩ [0,u) and binary-search them in the staged
 // prefix.  Counters of row v stay in LDS and are added to memory once;
 // counters of (u,w) go to memory with atomics.  With labels given, only
-// edges and triangles inside one community count (rules.cpp:248,261).
+// edges and triangles inside one community count (rules.cpp:172,185).
 #include "prims.h"
 
 namespace pcd {
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void k_triangles_flat(int n, const int32_t *in
     }
 }
 
-// graph.cpp:189-202 (rules.cpp:273-287): per entry (v,u) with t > 0,
+// graph.cpp:146-159 (rules.cpp:197-211): per entry (v,u) with t > 0,
 // t is added to both endpoints and each endpoint counts one edge.
 // Prefixes of at most ACC_G * 8 entries: ACC_G lanes per vertex, 64 / ACC_G
 // vertices per wave; longer prefixes: k_tri_accumulate, one wave each.
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(256) void k_tri_accumulate(int n, const int32_t *in
 
 constexpr int TSUM_BLOCKS = 1024;
 
-// graph.cpp:204-212: cc = sum over v with deg > 1 of (tri/(deg(deg-1)))/n
+// graph.cpp:161-169: cc = sum over v with deg > 1 of (tri/(deg(deg-1)))/n
 __global__ __launch_bounds__(256) void k_cc_partial(int n, const int32_t *indices, const int32_t *tri, double *part) {
     __shared__ double lds[256 / WAVE];
     double acc = 0;
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_cc_partial(int n, const int32_t *indice
     }
 }
 
-// rules.cpp:289-304
+// rules.cpp:213-228
 __global__ __launch_bounds__(256) void k_wcc_partial(int n, const int32_t *labels, const CommStats *st,
                                                      const int32_t *tri, const int32_t *tdeg, const int32_t *it,
                                                      const int32_t *itd, double *part) {

@@ -24,7 +24,7 @@ struct LPArgs {
     uint64_t seed;
     int step;
     int tie;
-    int f;        // selection threshold (refinement.cpp:158)
+    int f;        // selection threshold (refinement.cpp:116)
     int use_aux;  // the rule's gains read only aux
     unsigned long long *counters;  // {active, entries, distinct} of the bin, or nullptr
 };

@@ -100,7 +100,7 @@ extern "C" int pcd_read_pcd(const char *path, int *n_out, int *m_out, int32_t **
     bool err = fread(ind, 4, (size_t)n + 1, fp) != (size_t)n + 1;
     err = err || fread(a, 4, (size_t)2 * m, fp) != (size_t)2 * m;
     err = err || fread(mp, 4, (size_t)n, fp) != (size_t)n;
-    if (!err && fgetc(fp) != EOF) err = true;  // graph.cpp:133-135: no trailing bytes
+    if (!err && fgetc(fp) != EOF) err = true;  // graph.cpp:90-93: no trailing bytes
     fclose(fp);
     if (!err) {
         // structural checks the kernels rely on (the reference trusts the file)
@@ -652,7 +652,7 @@ extern "C" int pcd_clustering_partition(int n, const int32_t *indices, const int
 }
 
 // ----------------------------------------------------------------- NMI
-// norm_mutual_info (common.cpp:75-128) over the confusion counts n_ij of two
+// norm_mutual_info (common.cpp:75-127) over the confusion counts n_ij of two
 // labelings: -2 sum n_ij log(n_ij N / (n_i n_j)) / (sum n_i log(n_i / N) +
 // sum n_j log(n_j / N)), N = the graph's vertex count.  The reference keeps
 // n_ij in a std::map, so the first sum runs over (i, j) ascending and the
@@ -695,7 +695,7 @@ static double nmi_sorted_pairs(int n, std::vector<uint64_t> &keys) {
     return -2 * nom / den;
 }
 
-// compare_communities(g, a, b) (common.cpp:170-176)
+// compare_communities(g, a, b) (common.cpp:165-173)
 extern "C" int pcd_nmi(int n, const int32_t *a, const int32_t *b, double *nmi) {
     if (n <= 0 || !a || !b || !nmi) return fail(PCD_E_INVALID, "bad arguments");
     std::vector<uint64_t> keys((size_t)n);
@@ -707,7 +707,7 @@ extern "C" int pcd_nmi(int n, const int32_t *a, const int32_t *b, double *nmi) {
     return PCD_OK;
 }
 
-// compare_communities(g, labels, filename) (common.cpp:130-168): line c
+// compare_communities(g, labels, filename) (common.cpp:129-163): line c
 // (from 1, empty and '#' lines not counted) lists the vertices of ground
 // truth community c, read as integers up to the first token that is not one.
 extern "C" int pcd_nmi_file(int n, const int32_t *labels, const char *path, double *nmi) {

@@ -82,7 +82,7 @@ class Graph {
         check(pcd_graph_upload(dev.get(), n, m, indices.data(), adj.data()));
     }
 
-    // graph.cpp:150-218 (on the device; num_threads is accepted and unused)
+    // graph.cpp:107-175 (on the device; num_threads is accepted and unused)
     void count_triangles(int num_threads = 1) {
         (void)num_threads;
         tri.resize(n);
@@ -125,9 +125,8 @@ inline void create_singleton_partition(const Graph &g, int32_t *labels) {
 }
 
 // framework.cpp:27-59: vertices by (-cc, -deg, v), each unlabelled vertex
-// opens a community with its unlabelled neighbours.  Reads g.tri, computed
-// on the device by count_triangles (deg*(deg-1) is an int32 product there).
-// framework.cpp:27-59, computed on the device (triangles included)
+// opens a community with its unlabelled neighbours; computed on the device,
+// triangles included (deg*(deg-1) is an int32 product there as well).
 inline void create_clustering_partition(const Graph &g, int32_t *labels) {
     int nc = 0;
     check(pcd_clustering_labels(g.ctx(), &nc));
@@ -154,17 +153,25 @@ inline void collect_statistics(const Graph &g, Partition &p, int num_threads = 1
     check(pcd_stats_download(g.ctx(), p.sizes.data(), p.int_volume.data(), p.ext_volume.data()));
 }
 
-// rules.hpp:9-19.  calculate runs on the device; score_* evaluate the same
-// formulas the kernels use.
+// rules.hpp:9-19.  The five methods keep the reference's signatures, so a
+// rule written against rules.hpp compiles here unchanged.  The device
+// evaluates a rule through its descriptor(): the four built-in rules below
+// return their kind (and are final, so no override of their formulas can be
+// silently bypassed by the device).  A rule of its own keeps the default
+// descriptor, PCD_RULE_HOST: its methods work on the host, and every call
+// that would run it on the device (refine, and calculate of the base class)
+// throws parcd::Error with PCD_E_INVALID, because the device cannot run code
+// it was not compiled with.
 class Rule {
   public:
     virtual ~Rule() {}
     virtual double calculate(const Graph &g, const Partition &p) const {
+        const pcd_rule d = device_rule();
         check(pcd_labels_upload(g.ctx(), p.labels.data()));
         int k = 0;
         check(pcd_collect_statistics(g.ctx(), &k));
         double s = 0;
-        check(pcd_metric(g.ctx(), descriptor(), &s));
+        check(pcd_metric(g.ctx(), d, &s));
         return s;
     }
     virtual double score_merge(const Graph &g, const Partition &p, int c1, int c2, int num_links) const {
@@ -177,11 +184,21 @@ class Rule {
         return pcd::score_leave(params(g), g.degree(v), stats(p, p.labels[v]), num_neighbors);
     }
     virtual std::string get_name() const = 0;
-    virtual pcd_rule descriptor() const = 0;
+    // the formula set the device evaluates for this rule
+    virtual pcd_rule descriptor() const { return pcd_rule{PCD_RULE_HOST, 0.0}; }
+    // descriptor(), or PCD_E_INVALID for a rule the device cannot run
+    pcd_rule device_rule() const {
+        const pcd_rule d = descriptor();
+        if (d.kind < PCD_RULE_MODULARITY || d.kind > PCD_RULE_CONDUCTANCE)
+            throw Error(PCD_E_INVALID, "rule '" + get_name() +
+                                           "' has no device form: the device runs modularity, CPM, WCC and "
+                                           "conductance");
+        return d;
+    }
 
   protected:
     pcd::RuleParams params(const Graph &g) const {
-        pcd_rule d = descriptor();
+        pcd_rule d = device_rule();
         pcd::RuleParams r;
         r.kind = d.kind;
         r.lambda = d.lambda;
@@ -199,22 +216,22 @@ class Rule {
     }
 };
 
-class ModularityRule : public Rule {
+class ModularityRule final : public Rule {
   public:
-    std::string get_name() const override { return "modularity"; }  // rules.cpp:148-150
+    std::string get_name() const override { return "modularity"; }  // rules.cpp:72-74
     pcd_rule descriptor() const override { return pcd_rule{PCD_RULE_MODULARITY, 0.0}; }
 };
 
-class ConductanceRule : public Rule {
+class ConductanceRule final : public Rule {
   public:
-    std::string get_name() const override { return "conductance"; }  // rules.cpp:192-194
+    std::string get_name() const override { return "conductance"; }  // rules.cpp:116-118
     pcd_rule descriptor() const override { return pcd_rule{PCD_RULE_CONDUCTANCE, 0.0}; }
 };
 
-class CPMRule : public Rule {
+class CPMRule final : public Rule {
   public:
     explicit CPMRule(double l) : lambda(l) {}
-    std::string get_name() const override {  // rules.cpp:222-224
+    std::string get_name() const override {  // rules.cpp:146-148
         std::ostringstream s;
         s << lambda;
         return "CPM (lambda=" + s.str() + ")";
@@ -225,9 +242,9 @@ class CPMRule : public Rule {
     double lambda;
 };
 
-class WCCRule : public Rule {
+class WCCRule final : public Rule {
   public:
-    std::string get_name() const override { return "WCC"; }  // rules.cpp:514-516
+    std::string get_name() const override { return "WCC"; }  // rules.cpp:438-440
     pcd_rule descriptor() const override { return pcd_rule{PCD_RULE_WCC, 0.0}; }
 };
 
@@ -246,8 +263,9 @@ class PropagationRefinement : public Refinement {
   public:
     explicit PropagationRefinement(double f, int n = 1) : frac(f), num_threads(n) {}
     void refine(const Graph &g, const Partition &p, const Rule &r, int32_t *new_labels) override {
+        const pcd_rule d = r.device_rule();
         collect_statistics(g, const_cast<Partition &>(p));
-        check(pcd_propagation_step(g.ctx(), r.descriptor(), frac, seed, step++, tie));
+        check(pcd_propagation_step(g.ctx(), d, frac, seed, step++, tie));
         check(pcd_labels_download(g.ctx(), new_labels));
     }
 
@@ -260,8 +278,9 @@ class MergingRefinement : public Refinement {
   public:
     explicit MergingRefinement(int r, int n = 1) : rounds(r), num_threads(n) {}
     void refine(const Graph &g, const Partition &p, const Rule &r, int32_t *new_labels) override {
+        const pcd_rule d = r.device_rule();
         collect_statistics(g, const_cast<Partition &>(p));
-        check(pcd_merging_step(g.ctx(), r.descriptor(), rounds, seed, step++, tie));
+        check(pcd_merging_step(g.ctx(), d, rounds, seed, step++, tie));
         check(pcd_labels_download(g.ctx(), new_labels));
     }
 

@@ -253,7 +253,7 @@ def main():
                                 trace=np.array(trace))
             print("run", name, args, it, k, score)
 
-        # fabricated statistics (int32 overflow region of rules.cpp:341-434)
+        # fabricated statistics (int32 overflow region of rules.cpp:265-358)
         K = 64
         sizes = np.concatenate([rng.integers(1, 60, K // 2), rng.integers(30000, 120000, K // 2)])
         intv = (sizes * rng.integers(1, 40, K)).astype(np.int64)
@@ -283,7 +283,7 @@ def main():
 
 
 def make_nmi():
-    """compare_communities (common.cpp:75-176) on seeded labelings of the
+    """compare_communities (common.cpp:75-173) on seeded labelings of the
     golden graphs and of a 150,000-vertex path, where the int32 product
     ni * nj of norm_mutual_info wraps (finite but shifted values, and NaN
     where the wrapped product is negative)."""

@@ -82,7 +82,7 @@ def test_read_pcd_rejects_bad_files(tmp_path):
     with pytest.raises(parcd.PcdError):
         parcd.read_pcd(str(p))
     d = load(golden("convert_sbm200.npz")[0])
-    p.write_bytes(d["pcd"].tobytes() + b"\0")   # trailing byte (graph.cpp:133-135)
+    p.write_bytes(d["pcd"].tobytes() + b"\0")   # trailing byte (graph.cpp:90-93)
     with pytest.raises(parcd.PcdError):
         parcd.read_pcd(str(p))
     p.write_bytes(d["pcd"].tobytes()[:-3])      # truncated

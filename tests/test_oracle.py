@@ -395,7 +395,7 @@ def _same_float(x, y):
 
 @pytest.mark.parametrize("path", golden("nmi_*.npz"))
 def test_nmi_matches_reference(path, tmp_path):
-    """compare_communities of the compiled reference (common.cpp:75-176),
+    """compare_communities of the compiled reference (common.cpp:75-173),
     including the int32 wrap of ni * nj (values above 1, and NaN)."""
     d = load(path)
     want = float(d["nmi"][0])
