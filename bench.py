@@ -90,20 +90,33 @@ def lp_bin_bytes(active, entries, distinct, S):
     return active * (8 + 4 + S + 4) + 8 * entries + S * distinct
 
 
-SLOT_OF_BIN = {"hub": "lp_hub", "b4096": "lp_b4096", "b1024": "lp_b1024", "w256": "lp_w256",
-               "g64": "lp_g64", "g32": "lp_g32", "g16": "lp_g16", "g8": "lp_g8", "g4": "lp_g4"}
+# Per-kernel work model: timer slot -> (degree bins whose work counters it
+# covers, algorithmic bytes of that work given the counters (active
+# vertices, entries, distinct neighbour labels) and S, the bytes of community
+# data the rule's gains read per label).  The hub rows run in two kernels:
+# the partition pass reads each entry's neighbour id and label and writes a
+# 4-byte partition record; the count pass reads the records back and one
+# community total per distinct label, plus each hub's own row data.
+KERNELS = {
+    "lp_small": (("w256", "g64", "g32", "g16", "g8", "g4"), lambda a, e, d, S: lp_bin_bytes(a, e, d, S)),
+    "lp_b512": (("b512",), lambda a, e, d, S: lp_bin_bytes(a, e, d, S)),
+    "lp_b1024": (("b1024",), lambda a, e, d, S: lp_bin_bytes(a, e, d, S)),
+    "lp_b4096": (("b4096",), lambda a, e, d, S: lp_bin_bytes(a, e, d, S)),
+    "lp_hub_partition": (("hub",), lambda a, e, d, S: 12 * e),
+    "lp_hub_count": (("hub",), lambda a, e, d, S: 4 * e + S * d + a * (8 + 4 + S + 4)),
+}
 
-# kernel-timer slot -> patterns of its kernel names in a rocprofv3 trace of
-# this build (modularity build, template argument 1u)
-ROCPROF_NAMES = {
-    "stats_items": [r"pcd::k_stats_items(_cached<\d+>)?$"],
-    "lp_b4096": [r"pcd::k_lp_block<512, 8192, 1u, pcd::RangeSrc"],
-    "lp_b1024": [r"pcd::k_lp_block<256, 2048, 1u, pcd::RangeSrc", r"pcd::k_lp_wave<1024, 1u, pcd::RangeSrc"],
-    "lp_w256": [r"pcd::k_lp_wave<512, 1u, pcd::RangeSrc"],
-    "lp_hub": [r"pcd::k_hub_partition<false>$", r"pcd::k_hub_count_listed<1u, 512, 4096, false>$"],
-    "lp_g64": [r"pcd::k_lp_small<64, 8, false, 1u"], "lp_g32": [r"pcd::k_lp_small<32, 16, false, 1u"],
-    "lp_g16": [r"pcd::k_lp_small<16, 32, false, 1u"], "lp_g8": [r"pcd::k_lp_small<8, 64, false, 1u"],
-    "lp_g4": [r"pcd::k_lp_small<4, 64, false, 1u"], "lp_small": [r"pcd::k_lp_small_all<1u, (1|8)>"]}
+# timer slot -> the kernel's name in a rocprofv3 trace of this build
+# (modularity build: rule-kind template argument 1u; default tuning)
+KERNEL_NAMES = {
+    "lp_small": "pcd::k_lp_small_all<1u, 8>",
+    "lp_b512": "pcd::k_lp_wave<1024, 1u, pcd::RangeSrc, false, 8, unsigned short>",
+    "lp_b1024": "pcd::k_lp_block<256, 2048, 1u, pcd::RangeSrc, false, 4>",
+    "lp_b4096": "pcd::k_lp_block<512, 8192, 1u, pcd::RangeSrc, false, 4>",
+    "lp_hub_partition": "pcd::k_hub_partition<false>",
+    "lp_hub_count": "pcd::k_hub_count_listed<1u, 512, 4096, false>",
+    "stats_items": "pcd::k_stats_items_cached<1024>",
+}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "rmat22_mod_pmc.md")
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0  # stream-copy calibration of FETCH_SIZE / WRITE_SIZE
 
@@ -112,24 +125,47 @@ def pmc_traffic(slot):
     """HBM-side traffic (bytes per dispatch) of one kernel from the committed
     rocprofv3 counter summary of this workload: FETCH_SIZE x 2.0 + WRITE_SIZE
     x 1.0 (KiB).  None when the summary or the kernel is absent."""
-    pats = ROCPROF_NAMES.get(slot)
-    if not pats or not os.path.exists(PMC_SUMMARY):
+    name = KERNEL_NAMES.get(slot)
+    if not name or not os.path.exists(PMC_SUMMARY):
         return None
-    rows = {}
     for line in open(PMC_SUMMARY):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) > 4 and cells[0].startswith("pcd::"):
+        if len(cells) > 4 and cells[0] == name:
             try:
-                rows[cells[0]] = float(cells[3]) * 1024 * FETCH_FACTOR + float(cells[4]) * 1024 * WRITE_FACTOR
+                return float(cells[3]) * 1024 * FETCH_FACTOR + float(cells[4]) * 1024 * WRITE_FACTOR
             except ValueError:
-                pass
-    total = 0.0
-    for pat in pats:
-        hit = [v for k, v in rows.items() if re.match(pat, k)]
-        if len(hit) != 1:
-            return None
-        total += hit[0]
-    return total
+                return None
+    return None
+
+
+def host_cores():
+    """Threads this process may run on (the box's CPU share)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def host_info():
+    """CPU model and topology from lscpu, and the affinity count."""
+    info = {"affinity_cpus": host_cores()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus", "NUMA node(s)": "numa_nodes"}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                if k.strip() in keys:
+                    info[keys[k.strip()]] = v.strip()
+    except Exception:
+        pass
+    return info
+
+
+def _median(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2] if xs else None
 
 
 def _ref_run(exe, pcd_path, kind_arg, iters, cores):
@@ -145,37 +181,73 @@ def _ref_run(exe, pcd_path, kind_arg, iters, cores):
     return [stats[i] + metric[i] + lp[i] for i in range(n_it)], lp[:n_it], time.time() - t0, out
 
 
-def cpu_baseline(pcd_path, E, kind_arg, first, count):
+def cpu_baseline(pcd_path, E, kind_arg, first, count, reps=3):
     """The reference OpenMP binary (oracle/_ref/main) on the same graph file:
     seconds of (statistics + metric + label propagation) per iteration over
-    the same iterations the GPU times (first .. first+count-1), on all host
-    cores (at most 16, OMP_PROC_BIND=close), plus one core for the first
-    iterations."""
+    the same iterations the GPU times (first .. first+count-1), on every
+    core of this process's affinity (OMP_PROC_BIND=close), median of `reps`
+    runs; and on one core (-p 1), the first iteration, median of `reps`."""
     exe = os.path.join(ROOT, "oracle", "_ref", "main")
     if not os.path.exists(exe):
         log("reference binary missing (oracle/_ref/main); cpu_baseline skipped")
         return None
-    cores = min(16, os.cpu_count() or 1)
-    per_it, lp, wall, out = _ref_run(exe, pcd_path, kind_arg, first + count, cores)
-    sel = per_it[first: first + count]
-    if not sel:
-        log("reference run produced too few iterations:", out[-400:])
-        return None
-    per_iter = sum(sel) / len(sel)
-    one, _, wall1, _ = _ref_run(exe, pcd_path, kind_arg, 1, 1)
-    cpu = ""
-    try:
-        cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
-    except Exception:
-        pass
+    cores = host_cores()
+    runs, walls = [], []
+    for _ in range(reps):
+        per_it, lp, wall, out = _ref_run(exe, pcd_path, kind_arg, first + count, cores)
+        sel = per_it[first: first + count]
+        if not sel:
+            log("reference run produced too few iterations:", out[-400:])
+            return None
+        runs.append(sum(sel) / len(sel))
+        walls.append(wall)
+    per_iter = _median(runs)
+    ones, wall1 = [], []
+    for _ in range(reps):
+        one, _, w1, _ = _ref_run(exe, pcd_path, kind_arg, 1, 1)
+        if one:
+            ones.append(one[0])
+        wall1.append(w1)
+    one_core = _median(ones)
     return {"value": E / per_iter / 1e9, "unit": "GTEPS", "cores": cores, "kind": "reference",
             "sample": "reference main (oracle/_ref/main, -O3 -fopenmp) on the same R-MAT-22 PCD file, "
-                      "-m %s -n %d -p %d: mean of (statistics + metric + label propagation) over iterations "
-                      "%d-%d from the singleton partition (the iterations the GPU times); wall %.1f s incl. load"
-                      % (kind_arg, first + count, cores, first + 1, first + count, wall),
-            "sec_per_iteration": per_iter, "per_iteration_sec": [round(x, 4) for x in per_it],
-            "one_core_sec_first_iteration": round(one[0], 3) if one else None,
-            "one_core_wall_s": round(wall1, 1), "host_cpu": cpu}
+                      "-m %s -n %d -p %d, OMP_PROC_BIND=close: mean of (statistics + metric + label propagation) "
+                      "over iterations %d-%d from the singleton partition (the iterations the GPU times), median of "
+                      "%d runs (%.1f s of CPU work each incl. load)" % (kind_arg, first + count, cores, first + 1,
+                                                                        first + count, reps, _median(walls)),
+            "sec_per_iteration": per_iter, "sec_per_iteration_runs": [round(x, 4) for x in runs],
+            "one_core_sec_first_iteration": round(one_core, 3) if one_core else None,
+            "one_core_runs": [round(x, 3) for x in ones], "one_core_gteps_first_iteration":
+                round(E / one_core / 1e9, 4) if one_core else None,
+            "one_core_wall_s": round(_median(wall1), 1), "host": host_info()}
+
+
+def cpu_reference_sbm1(pcd_path, reps=3):
+    """The reference binary on SBM-1 (config 1), -m mod, to convergence from
+    its own singleton partition: refinement-phase seconds and iterations at
+    every core of the affinity and at -p 1, median of `reps` runs each."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "main")
+    if not os.path.exists(exe):
+        return None
+    out = {}
+    for label, cores in (("all_cores", host_cores()), ("one_core", 1)):
+        secs, its, qs = [], [], []
+        for _ in range(reps):
+            env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(cores))
+            r = subprocess.run([exe, pcd_path, "-m", "mod", "-p", str(cores)], env=env, stdout=subprocess.PIPE,
+                               stderr=subprocess.DEVNULL, text=True, timeout=600)
+            t = re.findall(r"time refinement phase: ([0-9.e+-]+)", r.stdout)
+            it = re.findall(r"refinement iterations: (\d+)", r.stdout)
+            q = re.findall(r"^modularity: ([0-9.e+-]+)", r.stdout, re.M)
+            if t:
+                secs.append(float(t[0]))
+            if it:
+                its.append(int(it[0]))
+            if q:
+                qs.append(float(q[0]))
+        out[label] = {"cores": cores, "refine_s": _median(secs), "refine_s_runs": secs, "iterations": its,
+                      "final_modularity": qs}
+    return out
 
 
 ITER_CAP = 100
@@ -283,7 +355,7 @@ def cpu_reference_merge(pcd_path, E, first, count):
     exe = os.path.join(ROOT, "oracle", "_ref", "main")
     if not os.path.exists(exe):
         return None
-    cores = min(16, os.cpu_count() or 1)
+    cores = host_cores()
     env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(cores))
     t0 = time.time()
     r = subprocess.run([exe, pcd_path, "-m", "mod", "-s", "1", "-r", "1", "-n", str(first + count), "-p", str(cores)],
@@ -306,7 +378,7 @@ def cpu_reference_wcc(pcd_path, E, iters=3):
     exe = os.path.join(ROOT, "oracle", "_ref", "main")
     if not os.path.exists(exe):
         return None
-    cores = min(16, os.cpu_count() or 1)
+    cores = host_cores()
     env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(cores))
     t0 = time.time()
     r = subprocess.run([exe, pcd_path, "-m", "wcc", "-i", "1", "-n", str(iters), "-p", str(cores)], env=env,
@@ -323,6 +395,45 @@ def cpu_reference_wcc(pcd_path, E, iters=3):
     return {"count_triangles_sec": tri[0], "cores": cores, "sec_per_iteration": sum(per) / k,
             "gteps": E / (sum(per) / k) / 1e9, "iterations": k, "wall_s": round(time.time() - t0, 1),
             "note": "count_triangles is serial in the reference (graph.hpp:23)"}
+
+
+def pct_ms(xs):
+    """p50 / p90 / p99 / max of durations in seconds, as milliseconds."""
+    import numpy as np
+    if not xs:
+        return None
+    a = np.asarray(xs) * 1000.0
+    return {"p50": round(float(np.percentile(a, 50)), 3), "p90": round(float(np.percentile(a, 90)), 3),
+            "p99": round(float(np.percentile(a, 99)), 3), "max": round(float(a.max()), 3)}
+
+
+def sbm1_leg(parcd, np, seeds=5):
+    """Config 1 on the GPU: SBM-1 (100 blocks of 100, p_in 0.08, p_out
+    0.0005, generator seed 1), -m mod -s 0 to convergence from the
+    singleton partition ./main draws, pcd_refine (the whole main.cpp:226-266
+    loop in one call) for counter-hash seeds 1..`seeds`.  The graph fits in
+    L2, so this is a latency figure, not a roofline one."""
+    n, m, ind, adj, mp = parcd.make_graph("sbm", 1, block_sizes=[100] * 100, p_in=0.08, p_out=0.0005)
+    libc_srand(1)
+    init = parcd.singleton_partition(n)
+    secs, its, qs = [], [], []
+    with parcd.Context(0) as ctx:
+        ctx.upload_graph(n, m, ind, adj)
+        for sd in range(0, seeds + 1):
+            ctx.upload_labels(init)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            q, it, k = ctx.refine(parcd.job(kind=parcd.RULE_MODULARITY, seed=sd))
+            t1 = time.perf_counter()
+            if sd == 0:
+                continue  # warm-up
+            secs.append(t1 - t0)
+            its.append(it)
+            qs.append(q)
+    return {"graph": "SBM-1: n=%d m=%d" % (n, m), "refine_s_median": round(_median(secs), 5),
+            "refine_s_runs": [round(x, 5) for x in secs], "iterations": its,
+            "final_modularity": [round(x, 6) for x in qs],
+            "timing": "pcd_refine from labels on the device to the last score, host clock"}, (n, m, ind, adj, mp)
 
 
 def batch_leg(parcd, np, graphs_count, seed=4):
@@ -365,12 +476,24 @@ def batch_leg(parcd, np, graphs_count, seed=4):
     slots = b.add_many([g[:4] for g in gs], labs, jobs)
     assert all(s_ is not None for s_ in slots)
     t_adm = time.perf_counter()
+    slot_index = {s_: i for i, s_ in enumerate(slots)}
+    done_at = [None] * len(slots)
+
+    def note_done():
+        now = time.perf_counter()
+        for s_ in b.finished():
+            i = slot_index.get(s_)
+            if i is not None and done_at[i] is None:
+                done_at[i] = now
+
     running = b.step(1)
+    note_done()
     t_first = time.perf_counter()
     chosen, map_ms = b.autotune(reps=2)
     t_rounds0 = time.perf_counter()
     while running:
         running = b.step(1)
+        note_done()
     t_rounds1 = time.perf_counter()
     res_b = list(zip(b.labels_many(slots), [b.status(s_) for s_ in slots]))
     t1 = time.perf_counter()
@@ -384,13 +507,16 @@ def batch_leg(parcd, np, graphs_count, seed=4):
     ctx = parcd.Context(0)
     t2 = time.perf_counter()
     res_p = []
+    plain_done = []
     for i, (g, lab, jb) in enumerate(zip(gs, labs, jobs)):
         if i and i % 1024 == 0:
             log("plain per graph: %d of %d" % (i, len(gs)))
+        ta = time.perf_counter()
         ctx.upload_graph(g[0], g[1], g[2], g[3])
         ctx.upload_labels(lab)
         sc, it, k = ctx.refine(jb)
         res_p.append((ctx.labels(), sc, it, k))
+        plain_done.append((ta, time.perf_counter()))
     t3 = time.perf_counter()
     ctx.close()
     log("plain per graph: %.3f s" % (t3 - t2))
@@ -414,6 +540,13 @@ def batch_leg(parcd, np, graphs_count, seed=4):
         "iterations_min_mean_max": [int(min(iters)), round(float(np.mean(iters)), 2), int(max(iters))],
         "capped_at_%d_iterations" % ITER_CAP: int(sum(1 for x in iters if x > ITER_CAP)),
         "timing": "end to end, host CSR in page-locked memory to labels on the host, both paths",
+        "latency_ms": {
+            "batched_completion": pct_ms([d_ - t0 for d_ in done_at if d_ is not None]),
+            "plain_service": pct_ms([b_ - a_ for a_, b_ in plain_done]),
+            "plain_completion_one_after_another": pct_ms([b_ - t2 for _, b_ in plain_done]),
+            "note": "batched: from the start of the batch (before admission) to the poll after the round in "
+                    "which the graph stopped, all graphs submitted at once; plain: each graph's own upload + "
+                    "refine + download, and its completion when the graphs run one after another"},
     }
 
 
@@ -527,6 +660,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help=argparse.SUPPRESS)  # the default; kept for old commands
     ap.add_argument("--no-wcc", action="store_true", help="skip the WCC / Chung-Lu leg")
     ap.add_argument("--no-merge", action="store_true", help="skip the merging (-s 1) leg")
+    ap.add_argument("--no-sbm1", action="store_true", help="skip the SBM-1 (config 1) leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--serve-batch-max-edges", type=int, default=1 << 17,
@@ -570,18 +704,13 @@ def main():
     # stream, about 0.15 ms per iteration for all slots together)
     run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup, args.seed)
     timers = ctx.kernel_timers()
-    probe_avg = {}
-    for k_, (ms_, cnt_) in timers.items():
-        name_ = "lp_hub" if k_ in ("lp_hub_partition", "lp_hub_count") else k_
-        if cnt_ and (name_ in SLOT_OF_BIN.values() or name_ in ("stats_items", "lp_small")):
-            probe_avg[name_] = probe_avg.get(name_, 0.0) + ms_ / cnt_
+    probe_avg = {k_: ms_ / cnt_ for k_, (ms_, cnt_) in timers.items() if cnt_ and k_ in KERNELS}
     dom_probe = max(probe_avg, key=probe_avg.get) if probe_avg else None
-    dom_slots = (["lp_hub_partition", "lp_hub_count"] if dom_probe == "lp_hub" else [dom_probe]) if dom_probe else []
-    ctx.kernel_timer_select(dom_slots)
+    ctx.kernel_timer_select([dom_probe] if dom_probe else [])
     secs, scores = run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup, args.seed)
     timed_dom = ctx.kernel_timers()
     ctx.kernel_timer_select(None)
-    log("timed pass: events around %s only" % (", ".join(dom_slots) or "nothing"))
+    log("timed pass: events around %s only" % (dom_probe or "nothing"))
     labels_timed = ctx.labels()
     log("hub partitions overflowed:", ctx.hub_overflows())
     ms_per_step = secs * 1000.0 / args.steps
@@ -602,52 +731,40 @@ def main():
 
     S = 4  # modularity reads one 4-byte total (int_volume + ext_volume) per community
     per_kernel = {}
-    for b, (a_, e_, d_) in counters.items():
-        slot = SLOT_OF_BIN[b]
-        if b == "hub":
-            tms = timers.get("lp_hub_partition", (0, 0))[0] + timers.get("lp_hub_count", (0, 0))[0]
-            launches = timers.get("lp_hub_partition", (0, 0))[1]
-        else:
-            tms, launches = timers.get(slot, (0.0, 0))
-        if tms <= 0 or launches == 0:
+    for slot, (bins, model) in KERNELS.items():
+        tms, launches = timers.get(slot, (0.0, 0))
+        work = [counters[b_] for b_ in bins if b_ in counters]
+        if tms <= 0 or launches == 0 or not work:
             continue
-        byt = lp_bin_bytes(a_, e_, d_, S)
-        per_kernel[slot] = {
-            "bytes_per_launch": byt / launches, "avg_ms": tms / launches,
-            "gbs": byt / (tms / 1000.0) / 1e9, "entries_per_launch": e_ / launches}
-    # the W256 and G64..G4 bins run as one launch (timer slot lp_small)
-    tms, launches = timers.get("lp_small", (0.0, 0))
-    if tms > 0 and launches:
-        small = [counters[b] for b in ("w256", "g64", "g32", "g16", "g8", "g4") if b in counters]
-        byt = sum(lp_bin_bytes(a_, e_, d_, S) for a_, e_, d_ in small)
-        ent = sum(e_ for _, e_, _ in small)
-        per_kernel["lp_small"] = {"bytes_per_launch": byt / launches, "avg_ms": tms / launches,
-                                  "gbs": byt / (tms / 1000.0) / 1e9, "entries_per_launch": ent / launches}
-    # statistics over rows longer than 64 (work items): per entry the neighbour
-    # id and its label, per row its extent and label and one 12-byte record
-    # update
+        byt = sum(model(a_, e_, d_, S) for a_, e_, d_ in work)
+        per_kernel[slot] = {"kernel": KERNEL_NAMES[slot], "bytes_per_launch": byt / launches,
+                            "avg_ms": tms / launches, "entries_per_launch": sum(w[1] for w in work) / launches}
+    # statistics over rows longer than 64 (work items, the full count of the
+    # first iteration): per entry the neighbour id and its label, per row its
+    # extent and label and one 12-byte record update
     deg = np.diff(ind)
     big = deg > 64
     st_ms, st_launches = timers.get("stats_items", (0.0, 0))
     if st_ms > 0 and st_launches:
-        st_bytes = 8 * int(deg[big].sum()) + 20 * int(big.sum())
-        per_kernel["stats_items"] = {"bytes_per_launch": st_bytes, "avg_ms": st_ms / st_launches,
-                                     "gbs": st_bytes / (st_ms / st_launches / 1000.0) / 1e9,
-                                     "entries_per_launch": int(deg[big].sum())}
-    dom_name = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"]) if per_kernel else None
+        per_kernel["stats_items"] = {"kernel": KERNEL_NAMES["stats_items"],
+                                     "bytes_per_launch": 8 * int(deg[big].sum()) + 20 * int(big.sum()),
+                                     "avg_ms": st_ms / st_launches, "entries_per_launch": int(deg[big].sum())}
     # the roofline kernel's average launch comes from the timed pass
-    if dom_name and dom_name == dom_probe:
-        if dom_name == "lp_hub":
-            t_ms = sum(timed_dom.get(k_, (0.0, 0))[0] for k_ in dom_slots)
-            t_launches = timed_dom.get("lp_hub_partition", (0.0, 0))[1]
-        else:
-            t_ms, t_launches = timed_dom.get(dom_name, (0.0, 0))
+    if dom_probe in per_kernel:
+        t_ms, t_launches = timed_dom.get(dom_probe, (0.0, 0))
         if t_launches:
-            d_ = per_kernel[dom_name]
+            d_ = per_kernel[dom_probe]
             d_["avg_ms_probe_pass"] = d_["avg_ms"]
             d_["avg_ms"] = t_ms / t_launches
-            d_["gbs"] = d_["bytes_per_launch"] / (d_["avg_ms"] / 1000.0) / 1e9
             d_["measured_in"] = "timed pass"
+    for slot, d_ in per_kernel.items():
+        d_["gbs"] = d_["bytes_per_launch"] / (d_["avg_ms"] / 1000.0) / 1e9
+        d_["frac"] = round(d_["gbs"] / HBM_PEAK_GBS, 4)
+        tr = pmc_traffic(slot) if args.scale == 22 else None
+        d_["traffic_per_launch"] = int(tr) if tr else None
+        d_["traffic_over_algorithmic"] = round(tr / d_["bytes_per_launch"], 2) if tr else None
+    dom_name = max((k_ for k_ in per_kernel if k_ in KERNELS), key=lambda k_: per_kernel[k_]["avg_ms"],
+                   default=None)
     lp_ms = sum(v[0] for k, v in timers.items() if k.startswith("lp_"))
     tot_a = sum(c[0] for c in counters.values())
     tot_e = sum(c[1] for c in counters.values())
@@ -669,13 +786,11 @@ def main():
     roofline = None
     if dom_name:
         d = per_kernel[dom_name]
-        traffic = pmc_traffic(dom_name) if args.scale == 22 else None
-        roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(d["gbs"], 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
-                    "traffic": int(traffic) if traffic else None,
-                    "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if traffic else None,
+        roofline = {"bound": "hbm", "kernel": d["kernel"], "slot": dom_name, "achieved": round(d["gbs"], 2),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"], "traffic": d["traffic_per_launch"],
+                    "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if d["traffic_per_launch"] else None,
                     "bytes_per_launch": int(d["bytes_per_launch"]), "avg_launch_ms": round(d["avg_ms"], 4),
-                    "avg_launch_from": "HIP events around this kernel in the timed pass"
+                    "avg_launch_from": "HIP events around this kernel alone in the timed pass"
                     if d.get("measured_in") == "timed pass" else "HIP events of the probe pass"}
 
     # ---------------------------------------------------------- merging (-s 1)
@@ -699,6 +814,12 @@ def main():
     if not args.no_wcc:
         wcc, wcc_graph = wcc_leg(parcd, np, 3, 1)
         log("wcc:", json.dumps(wcc))
+
+    # ---------------------------------------------------------- SBM-1
+    sbm1 = sbm1_graph = None
+    if not args.no_sbm1:
+        sbm1, sbm1_graph = sbm1_leg(parcd, np)
+        log("sbm1:", json.dumps(sbm1))
 
     # ---------------------------------------------------------- batched form
     batch = None
@@ -727,6 +848,15 @@ def main():
                 wpath = os.path.join(tmp, "chung_lu.pcd")
                 parcd.write_pcd(wpath, wn, wm, wi, wa, wmp)
                 cpu["wcc_chung_lu"] = cpu_reference_wcc(wpath, 2 * wm)
+            if cpu and sbm1_graph:
+                sn, sm, si, sa, smp = sbm1_graph
+                spath = os.path.join(tmp, "sbm1.pcd")
+                parcd.write_pcd(spath, sn, sm, si, sa, smp)
+                cpu["sbm1"] = cpu_reference_sbm1(spath)
+                if sbm1 and cpu["sbm1"]:
+                    for k_ in ("all_cores", "one_core"):
+                        r_ = cpu["sbm1"][k_]["refine_s"]
+                        sbm1["speedup_vs_reference_" + k_] = round(r_ / sbm1["refine_s_median"], 2) if r_ else None
         if cpu:
             log("cpu baseline: %.4f GTEPS (%.3f s/iteration on %d cores)" % (cpu["value"], cpu["sec_per_iteration"],
                                                                              cpu["cores"]))
@@ -742,7 +872,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "stream_copy_gbs": round(copy_gbs, 1),
         "speedup_vs_cpu": round(gteps / cpu["value"], 2) if cpu else None,
         "phases": {"propagate": lp_phase, "statistics": stats_phase, "kernels": per_kernel},
-        "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm, "merge_s1": merge, "wcc_chung_lu": wcc, "batch": batch, "serve": serve,
+        "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm, "merge_s1": merge, "wcc_chung_lu": wcc, "sbm1": sbm1, "batch": batch, "serve": serve,
     }
     print(json.dumps(out), flush=True)
 

@@ -225,6 +225,21 @@ typedef struct pcd_job {
 int pcd_iterate(pcd_ctx *ctx, const pcd_job *job, int strategy, int rounds, int it, double prev_score, double *score,
                 int *k, int *done);
 
+/* Passes it, it+1, ..., it+count-1 of the main.cpp:226-266 loop, each as
+ * pcd_iterate would run it, stopping after the pass at which the stop rule
+ * (main.cpp:241-242, -n, the iteration cap) fires.  prev_score is the score
+ * of pass it-1 (0 for it == 0).  *passes receives the number of passes run,
+ * *score and *num_labels those of the last pass run, *done whether the rule
+ * fired.  Modularity and CPM with -s 0 run device-resident once the
+ * statistics follow the labels: one pass is captured as a hipGraph (stop
+ * rule, pass number and step number on the device) and replayed, and the
+ * host reads a small state record every few passes instead of a score per
+ * pass.  The labels and statistics of the last pass are then left in the
+ * compressed form pcd_iterate's next compression would give (the same
+ * partition).  Other rules and strategies run pcd_iterate per pass. */
+int pcd_iterate_many(pcd_ctx *ctx, const pcd_job *job, int strategy, int rounds, int it, int count,
+                     double prev_score, int *passes, double *score, int *num_labels, int *done);
+
 /* The whole main.cpp:226-266 loop on the context's graph and current labels
  * (-s strategy 0/1/2, -r rounds): on return the context holds the final
  * compressed labels and statistics; *score receives the last score (what

@@ -17,6 +17,7 @@ int dev_alloc(pcd_ctx *ctx, void **p, size_t bytes) {
         return fail(PCD_E_NOMEM, "hipMalloc(" + std::to_string(bytes) + " bytes) failed: " + hipGetErrorString(e));
     }
     ctx->allocations.push_back(*p);
+    ctx->alloc_gen++;
     return PCD_OK;
 }
 
@@ -25,6 +26,7 @@ void dev_free(pcd_ctx *ctx, void *p) {
     auto it = std::find(ctx->allocations.begin(), ctx->allocations.end(), p);
     if (it != ctx->allocations.end()) ctx->allocations.erase(it);
     (void)hipFree(p);
+    ctx->alloc_gen++;
 }
 
 RuleParams make_rule(pcd_ctx *ctx, pcd_rule rule) {
@@ -77,9 +79,10 @@ void kt_end(pcd_ctx *ctx, int slot) {
 static const char *kslot_names[NUM_KSLOTS] = {
     "compress", "stats_items", "stats_small", "metric", "lp_records", "lp_hub_partition", "lp_hub_count",
     "lp_b4096", "lp_b1024", "lp_w256", "lp_g64", "lp_g32", "lp_g16", "lp_g8", "lp_g4",
-    "merge_keys", "merge_sort", "merge_rle", "merge_rounds", "triangles", "stats_delta", "lp_small"};
+    "merge_keys", "merge_sort", "merge_rle", "merge_rounds", "triangles", "stats_delta", "lp_small",
+    "lp_b512", "lp_hub_final"};
 
-static const char *bin_names[NUM_BINS] = {"hub", "b4096", "b1024", "w256", "g64", "g32", "g16", "g8", "g4"};
+static const char *bin_names[NUM_BINS] = {"hub", "b4096", "b1024", "b512", "w256", "g64", "g32", "g16", "g8", "g4"};
 
 }  // namespace pcd
 
@@ -165,6 +168,7 @@ extern "C" int pcd_ctx_create(int device, pcd_ctx **out) {
 }
 
 static void free_graph_state(pcd_ctx *ctx) {
+    loop_release(ctx);  // its device buffers go with the others below
     // everything except the per-context scalars allocated at creation
     void *keep[] = {ctx->d_k, ctx->d_acc, ctx->d_dacc, ctx->d_result, ctx->d_counts, ctx->d_bin_counters};
     std::vector<void *> all = ctx->allocations;
@@ -199,6 +203,8 @@ static void free_graph_state(pcd_ctx *ctx) {
     ctx->d_pair_links = ctx->d_pair_head = nullptr;
     ctx->d_rs_hist = nullptr;
     ctx->rs_hist_cap = 0;
+    ctx->d_os = nullptr;
+    ctx->os_cap = 0;
     ctx->d_bp = ctx->d_found = nullptr;
     ctx->d_best_ord = ctx->d_best_key = nullptr;
     ctx->d_mcand = nullptr;
@@ -215,6 +221,7 @@ extern "C" int pcd_ctx_destroy(pcd_ctx *ctx) {
     if (!ctx) return PCD_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    loop_release(ctx);
     for (void *p : ctx->allocations) (void)hipFree(p);
     ctx->allocations.clear();
     for (int i = 0; i < PCD_PHASE_COUNT; i++) {
@@ -308,7 +315,6 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
     ctx->bins.start[0] = 0;
     for (int b = 1; b < NUM_BINS; b++) ctx->bins.start[b] = first_pos_le(BIN_MAX_DEG[b]);
     ctx->bins.start[NUM_BINS] = n;
-    ctx->bins.split512 = first_pos_le(512);
     ctx->bins.split2048 = first_pos_le(2048);
     for (int b = 0; b < NUM_BINS; b++) {
         int64_t e = 0;
@@ -790,12 +796,20 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
     } else if (k == "cluster_rounds") {
         if (value < 0) return fail(PCD_E_INVALID, "cluster_rounds must be >= 0");
         ctx->tune.cluster_rounds = value;
+    } else if (k == "onesweep") {
+        ctx->tune.onesweep = value != 0;
+    } else if (k == "graph_loop") {
+        ctx->tune.graph_loop = value != 0;
+    } else if (k == "graph_loop_min_entries") {
+        if (value < 0) return fail(PCD_E_INVALID, "graph_loop_min_entries must be >= 0");
+        ctx->tune.graph_loop_min_entries = value;
     } else if (k == "stat_chunk") {
         if (value < 64) return fail(PCD_E_INVALID, "stat_chunk must be >= 64");
         ctx->tune.stat_chunk = value;  // applies to the next graph upload
     } else {
         return fail(PCD_E_INVALID, "unknown tuning key " + k);
     }
+    ctx->tune_gen++;  // a captured pass holds the old kernel forms
     return PCD_OK;
 }
 

@@ -34,13 +34,12 @@ constexpr int EMPTY_KEY = -1;
 // Degree bins over vertex positions (positions are vertex ids when the
 // graph is degree-sorted descending, as `convert` writes it, else the
 // positions of a degree-descending permutation `order`).
-enum Bin { BIN_HUB = 0, BIN_B4096, BIN_B1024, BIN_W256, BIN_G64, BIN_G32, BIN_G16, BIN_G8, BIN_G4, NUM_BINS };
+enum Bin { BIN_HUB = 0, BIN_B4096, BIN_B1024, BIN_B512, BIN_W256, BIN_G64, BIN_G32, BIN_G16, BIN_G8, BIN_G4, NUM_BINS };
 // degree upper bound of each bin (HUB: unbounded)
-constexpr int BIN_MAX_DEG[NUM_BINS] = {0x7fffffff, 4096, 1024, 256, 64, 32, 16, 8, 4};
+constexpr int BIN_MAX_DEG[NUM_BINS] = {0x7fffffff, 4096, 1024, 512, 256, 64, 32, 16, 8, 4};
 
 struct BinLayout {
     int start[NUM_BINS + 1];  // position ranges, bin b = [start[b], start[b+1])
-    int split512;             // first position of degree <= 512 (inside B1024)
     int split2048;            // first position of degree <= 2048 (inside B4096)
 };
 
@@ -69,6 +68,11 @@ struct Tuning {
                                  // 4096-slot table (measured slower on R-MAT-22: off)
     int cluster_rounds = 128;    // device rounds of the clustering partition before the
                                  // remaining dependency chains are finished on the host
+    int onesweep = 1;            // radix sorts: one launch per pass with decoupled look-back (1)
+                                 // or tile histograms, a scan and a scatter per pass (0)
+    int graph_loop = 1;          // modularity / CPM with -s 0: passes replayed from one captured
+                                 // pass with the stop rule on the device (pcd_iterate_many)
+    int graph_loop_min_entries = 0;  // graphs with fewer CSR entries run pass by pass
 };
 constexpr int STATS_CACHE = 16384;
 constexpr int HUB_MAX_PARTS = 1024;
@@ -84,6 +88,7 @@ enum KSlot {
     KS_LP_SELECT, KS_LP_HUB_INSERT, KS_LP_HUB_DECIDE, KS_LP_B4096, KS_LP_B1024, KS_LP_W256,
     KS_LP_G64, KS_LP_G32, KS_LP_G16, KS_LP_G8, KS_LP_G4,
     KS_MERGE_KEYS, KS_MERGE_SORT, KS_MERGE_RLE, KS_MERGE_ROUNDS, KS_TRIANGLES, KS_STATS_DELTA, KS_LP_SMALL,
+    KS_LP_B512, KS_LP_HUB_FINAL,
     NUM_KSLOTS
 };
 
@@ -179,6 +184,15 @@ struct pcd_ctx {
     int32_t *d_pair_links = nullptr, *d_pair_head = nullptr;
     int32_t *d_rs_hist = nullptr;
     size_t rs_hist_cap = 0;
+    // device-resident loop (loop.hip): one captured pass, replayed
+    bool capturing = false;          // a pass is being captured: no host reads, no pointer swaps of stats
+    const int *graph_step = nullptr; // the captured pass reads its step number here
+    void *loop = nullptr;            // the captured pass and its device state (loop.hip)
+    int64_t alloc_gen = 0;           // counts device allocations and frees (a capture's buffers)
+    int64_t tune_gen = 0;            // counts tuning changes (a capture's kernel forms)
+    uint32_t *d_os = nullptr;  // one-sweep sort: look-back status words, histograms, tickets
+    size_t os_cap = 0;
+    int64_t os_fallbacks = 0;  // passes redone after a look-back gave up waiting
     int32_t *d_bp = nullptr, *d_found = nullptr;
     uint64_t *d_best_ord = nullptr, *d_best_key = nullptr;
     void *d_mcand = nullptr;         // merge candidates of runs crossing waves
@@ -229,6 +243,8 @@ int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int b
 // memory; resolve_k takes it once the stream has passed the copy.
 int compress_labels_dev(pcd_ctx *ctx, bool defer_k = false);
 void resolve_k(pcd_ctx *ctx);
+// drops the captured pass of the device-resident loop (loop.hip)
+void loop_release(pcd_ctx *ctx);
 int collect_statistics_dev(pcd_ctx *ctx);
 int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *after);
 int clustering_labels_dev(pcd_ctx *ctx, int *num_communities);

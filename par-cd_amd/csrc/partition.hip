@@ -54,6 +54,12 @@ __global__ void k_relabel(int n, const int32_t *labels, const int32_t *newid, in
 
 __global__ void k_copy_i32(int32_t *dst, const int32_t *src) { *dst = *src; }
 
+// the first *kp community records of src into dst
+__global__ void k_copy_stats(const int32_t *kp, const CommStats *src, CommStats *dst) {
+    const int k = *kp;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < k; c += gridDim.x * blockDim.x) dst[c] = src[c];
+}
+
 // slot of the pinned scratch ints that receives a deferred k
 constexpr int K_SLOT = 8;
 
@@ -83,7 +89,15 @@ int compress_labels_dev(pcd_ctx *ctx, bool defer_k) {
     hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(1), 0, ctx->stream, ctx->d_k, total);
     kt_end(ctx, KS_COMPRESS);
     PCD_HIP(hipGetLastError());
-    if (defer_k) {
+    if (ctx->capturing) {
+        // a captured pass: k stays on the device, and the renumbered
+        // statistics move back in place so that the replayed pass always
+        // reads and writes the same buffers
+        ctx->k_deferred = true;
+        if (renumber)
+            hipLaunchKernelGGL(k_copy_stats, dim3(grid_for(ctx, n + 1, 256, 8)), dim3(256), 0, ctx->stream,
+                               ctx->d_k, ctx->d_stats_alt, ctx->d_stats);
+    } else if (defer_k) {
         PCD_HIP(hipMemcpyAsync(ctx->h_small + K_SLOT, ctx->d_k, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         ctx->k_deferred = true;
     } else {
@@ -94,7 +108,7 @@ int compress_labels_dev(pcd_ctx *ctx, bool defer_k) {
         ctx->k_deferred = false;
     }
     std::swap(ctx->d_labels, ctx->d_labels_alt);
-    if (renumber) std::swap(ctx->d_stats, ctx->d_stats_alt);
+    if (renumber && !ctx->capturing) std::swap(ctx->d_stats, ctx->d_stats_alt);
     ctx->labels_compressed = true;
     ctx->has_stats = renumber;
     ctx->stats_pending = false;
@@ -601,7 +615,8 @@ int metric_enqueue(pcd_ctx *ctx, pcd_rule rule) {
     }
     kt_end(ctx, KS_METRIC);
     PCD_HIP(hipGetLastError());
-    PCD_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (!ctx->capturing)
+        PCD_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     return PCD_OK;
 }
 

@@ -163,11 +163,10 @@ struct RsStore {
     __device__ void operator()(int64_t i, int ex, int) const { h[i] = ex; }
 };
 
-// Sorts n keys whose significant bits are [0, bits).  Ping-pongs between
-// keys and tmp; *out receives the buffer holding the result.
-int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int bits, uint64_t **out) {
-    *out = keys;
-    if (n <= 1) return PCD_OK;
+// Sorts n keys whose significant bits are [0, bits) with the three-launch
+// passes above (tile histograms, a scan of all (digit, tile) counters, the
+// scatter).  Ping-pongs between src and dst from pass `shift0` on.
+static int lsd_passes(pcd_ctx *ctx, uint64_t *src, uint64_t *dst, int64_t n, int shift0, int bits, uint64_t **out) {
     int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
     size_t need = (size_t)ntiles * 256;
     if (2 * need > ctx->rs_hist_cap) {
@@ -177,14 +176,205 @@ int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int b
         ctx->rs_hist_cap = 2 * need;
     }
     int32_t *raw = ctx->d_rs_hist + need;
-    uint64_t *src = keys, *dst = tmp;
-    for (int shift = 0; shift < bits; shift += 8) {
+    for (int shift = shift0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, ctx->stream, src, n, shift, ntiles,
                            ctx->d_rs_hist, raw);
         PCD_TRY(device_scan(ctx, (int64_t)need, RsLoad{ctx->d_rs_hist}, RsStore{ctx->d_rs_hist}, nullptr));
         hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, ctx->stream, src, dst, n, shift, ntiles,
                            ctx->d_rs_hist, raw);
         PCD_HIP(hipGetLastError());
+        uint64_t *t = src; src = dst; dst = t;
+    }
+    *out = src;
+    return PCD_OK;
+}
+
+// ------------------------------------------------- one-sweep radix sort
+// LSD with 8-bit digits, one launch per pass.  A first kernel reads every
+// key once and builds the global histogram of every pass's digit; one
+// workgroup scans them into per-pass digit bases.  Each pass's scatter then
+// takes a tile ticket (tiles are numbered in the order workgroups start, so
+// every lower ticket belongs to a workgroup that is running or done),
+// counts its digits, publishes them as an aggregate, sorts its tile by
+// digit in LDS (the ranking of k_rs_scatter), and finds each digit's
+// offset among the earlier tiles by walking back over their published
+// words until an inclusive prefix (decoupled look-back); it publishes its
+// own inclusive prefix and writes each digit's run contiguously.  Per pass
+// the keys are read once and written once.
+//
+// Status word per (tile, digit): bits 31-30 the state (0 none, 1
+// aggregate, 2 inclusive prefix), bits 29-0 the count, so n < 2^30.  A
+// wait polls with a bounded number of sleeps; a workgroup that runs out
+// sets the error flag and moves on, and the host redoes that pass with the
+// three-launch kernels, so every wait ends and the result stays exact.
+constexpr int OS_MAX_PASSES = 8;
+constexpr uint32_t OS_AGG = 1u << 30, OS_INC = 2u << 30, OS_VAL = (1u << 30) - 1;
+constexpr int OS_SPIN_LIMIT = 1 << 20;
+
+__global__ __launch_bounds__(RS_THREADS) void k_os_hist(const uint64_t *keys, int64_t n, int passes,
+                                                        uint32_t *ghist) {
+    __shared__ uint32_t h[RS_THREADS / WAVE][OS_MAX_PASSES * 256];  // one copy per wave
+    const int t = threadIdx.x, w = t / WAVE;
+    for (int i = t; i < RS_THREADS / WAVE * OS_MAX_PASSES * 256; i += RS_THREADS) (&h[0][0])[i] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * RS_THREADS + t; i < n; i += (int64_t)gridDim.x * RS_THREADS) {
+        const uint64_t k = keys[i];
+        for (int p = 0; p < passes; p++) atomicAdd(&h[w][p * 256 + (int)((k >> (8 * p)) & 0xFF)], 1u);
+    }
+    __syncthreads();
+    for (int i = t; i < passes * 256; i += RS_THREADS) {
+        uint32_t c = 0;
+        for (int j = 0; j < RS_THREADS / WAVE; j++) c += h[j][i];
+        if (c) atomicAdd(&ghist[i], c);
+    }
+}
+
+// one workgroup: exclusive scan of each pass's 256 digit counts
+__global__ __launch_bounds__(RS_THREADS) void k_os_bases(int passes, const uint32_t *ghist, uint32_t *base) {
+    __shared__ int lds[RS_THREADS / WAVE + 1];
+    for (int p = 0; p < passes; p++) {
+        int tot;
+        const int ex = block_excl_scan((int)ghist[p * 256 + threadIdx.x], &tot, lds);
+        base[p * 256 + threadIdx.x] = (uint32_t)ex;
+    }
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_os_scatter(const uint64_t *keys, uint64_t *out, int64_t n, int shift,
+                                                           const uint32_t *digit_base, uint32_t *status, int *ticket,
+                                                           int *err) {
+    __shared__ uint64_t sk[RS_TILE];
+    __shared__ int cnt[256];
+    __shared__ int lstart[256];
+    __shared__ int run[256];
+    __shared__ int gout[256];
+    __shared__ int wcnt[RS_THREADS / WAVE][256];
+    __shared__ int scan_lds[RS_THREADS / WAVE + 1];
+    __shared__ int s_tile;
+    const int t = threadIdx.x, lane = __lane_id(), w = t / WAVE;
+    if (t == 0) s_tile = atomicAdd(ticket, 1);
+    cnt[t] = 0;
+    run[t] = 0;
+    for (int i = 0; i < RS_THREADS / WAVE; i++) wcnt[i][t] = 0;
+    __syncthreads();
+    const int tile = s_tile;
+    const int64_t base = (int64_t)tile * RS_TILE;
+    const int count = (int)min((int64_t)RS_TILE, n - base);
+    uint64_t kv[RS_ROUNDS];  // all of the thread's keys in flight at once
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        const int64_t i = base + (int64_t)r * RS_THREADS + t;
+        kv[r] = i < n ? keys[i] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++)
+        if (r * RS_THREADS + t < count) atomicAdd(&cnt[(int)((kv[r] >> shift) & 0xFF)], 1);
+    __syncthreads();
+    // publish this tile's counts as early as possible
+    uint32_t *my = status + (size_t)tile * 256;
+    __hip_atomic_store(&my[t], (tile == 0 ? OS_INC : OS_AGG) | (uint32_t)cnt[t], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    int tot;
+    lstart[t] = block_excl_scan(cnt[t], &tot, scan_lds);
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        const bool valid = r * RS_THREADS + t < count;
+        const uint64_t key = kv[r];
+        const int d = (int)((key >> shift) & 0xFF);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint64_t bb = __ballot(valid && ((d >> b) & 1));
+            peers &= ((d >> b) & 1) ? bb : ~bb;
+        }
+        const int rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int off = run[d] + rank;
+            for (int j = 0; j < w; j++) off += wcnt[j][d];
+            sk[lstart[d] + off] = key;
+        }
+        __syncthreads();
+        int add = 0;
+        for (int j = 0; j < RS_THREADS / WAVE; j++) { add += wcnt[j][t]; wcnt[j][t] = 0; }
+        run[t] += add;
+        __syncthreads();
+    }
+    // decoupled look-back: thread t owns digit t
+    uint32_t prefix = 0;
+    if (tile > 0) {
+        int j = tile - 1, spins = 0;
+        while (j >= 0) {
+            const uint32_t s =
+                __hip_atomic_load(status + (size_t)j * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((s & ~OS_VAL) == 0) {
+                if (++spins > OS_SPIN_LIMIT) {
+                    atomicOr(err, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+                continue;
+            }
+            prefix += s & OS_VAL;
+            if ((s & ~OS_VAL) == OS_INC) break;
+            j--;
+        }
+        __hip_atomic_store(&my[t], OS_INC | (prefix + (uint32_t)cnt[t]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    gout[t] = (int)(digit_base[t] + prefix) - lstart[t];
+    __syncthreads();
+    // the tile is sorted by digit: each digit's run goes to its global place
+    for (int i = t; i < count; i += RS_THREADS) {
+        const uint64_t key = sk[i];
+        out[gout[(int)((key >> shift) & 0xFF)] + i] = key;
+    }
+}
+
+// Sorts n keys whose significant bits are [0, bits).  Ping-pongs between
+// keys and tmp; *out receives the buffer holding the result.  Uses the
+// one-sweep passes (tunable "onesweep", on by default) when n < 2^30.
+int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int bits, uint64_t **out) {
+    *out = keys;
+    if (n <= 1) return PCD_OK;
+    const int passes = (bits + 7) / 8;
+    if (!ctx->tune.onesweep || n >= (int64_t)OS_VAL || passes > OS_MAX_PASSES)
+        return lsd_passes(ctx, keys, tmp, n, 0, bits, out);
+    const int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
+    // status words, then 2 x 256 x passes histogram / bases, tickets, error flag
+    const size_t words = (size_t)ntiles * 256 + 2 * 256 * OS_MAX_PASSES + OS_MAX_PASSES + 64;
+    if (words > ctx->os_cap) {
+        if (ctx->d_os) dev_free(ctx, ctx->d_os);
+        ctx->d_os = nullptr;
+        ctx->os_cap = 0;
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_os, words));
+        ctx->os_cap = words;
+    }
+    uint32_t *status = ctx->d_os;
+    uint32_t *ghist = status + (size_t)ntiles * 256;
+    uint32_t *dbase = ghist + 256 * OS_MAX_PASSES;
+    int *tickets = (int *)(dbase + 256 * OS_MAX_PASSES);
+    int *err = tickets + OS_MAX_PASSES;
+    PCD_TRY(fill_i32(ctx, (int32_t *)ghist, 0, 256 * OS_MAX_PASSES));
+    PCD_TRY(fill_i32(ctx, tickets, 0, OS_MAX_PASSES + 1));
+    hipLaunchKernelGGL(k_os_hist, dim3(grid_for(ctx, n, RS_THREADS * 16, 4)), dim3(RS_THREADS), 0, ctx->stream, keys,
+                       n, passes, ghist);
+    hipLaunchKernelGGL(k_os_bases, dim3(1), dim3(RS_THREADS), 0, ctx->stream, passes, ghist, dbase);
+    uint64_t *src = keys, *dst = tmp;
+    for (int p = 0; p < passes; p++) {
+        PCD_TRY(fill_i32(ctx, (int32_t *)status, 0, (int64_t)ntiles * 256));
+        hipLaunchKernelGGL(k_os_scatter, dim3(ntiles), dim3(RS_THREADS), 0, ctx->stream, src, dst, n, 8 * p,
+                           dbase + 256 * p, status, tickets + p, err);
+        PCD_HIP(hipGetLastError());
+        // a look-back that ran out of patience: redo this pass the other way
+        PCD_HIP(hipMemcpyAsync(ctx->h_small + 12, err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        PCD_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->h_small[12] != 0) {
+            ctx->os_fallbacks++;
+            PCD_TRY(fill_i32(ctx, err, 0, 1));
+            return lsd_passes(ctx, src, dst, n, 8 * p, bits, out);
+        }
         uint64_t *t = src; src = dst; dst = t;
     }
     *out = src;

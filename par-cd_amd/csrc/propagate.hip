@@ -175,7 +175,7 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
         int p = q0 + lane;
         bool valid = lane < CP && p < q1;
         int v = valid ? vertex_at(a, p) : 0;
-        bool act = valid && selected(a.seed, a.step, v, a.f);
+        bool act = valid && lp_selected(a, v);
         uint64_t am = __ballot(act);
         int nact = __popcll(am);
         if (act) list[__popcll(am & lt)] = v;
@@ -242,7 +242,7 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
                 if (x == own.x) {
                     own_cnt = cnt;
                 } else {
-                    uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, u);
+                    uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), u);
                     double sc = score_join<KM>(a.rule, deg, AUX ? CommStats{xa, xa, 0, 0} : comm_rec<KM>(a, x), cnt);
                     if (sc > 0) c = Cand{sc, tie_key(a.tie, tb, x), x};
                 }
@@ -458,7 +458,7 @@ __device__ inline void lp_wave(const Src &src, int wave_g, int nwaves, int *keys
       if (!src.unit(u, a, q0, q1)) continue;
       for (int p = q0; p < q1; p++) {
         int v = vertex_at(a, p);
-        if (!selected(a.seed, a.step, v, a.f)) continue;
+        if (!lp_selected(a, v)) continue;
         int start = a.indices[v], deg = a.indices[v + 1] - start;
         int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
         int own = ownr.x;
@@ -466,7 +466,7 @@ __device__ inline void lp_wave(const Src &src, int wave_g, int nwaves, int *keys
         insert_labels<PF, AUX, CT>(a, keys, cnts, auxs, occ, nocc, (1 << log_t) - 1, log_t, start, deg, lane, WAVE);
         wave_sync();
         const int D = *nocc;
-        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
+        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), v);
         Cand c = cand_none();
         int own_cnt = 0;
         scan_listed<AUX, KM, CT>(a, keys, cnts, auxs, occ, D, lane, WAVE, deg, own, tb, c, own_cnt);
@@ -559,7 +559,7 @@ __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
       if (!src.unit(u, a, q0, q1)) continue;
       for (int p = q0; p < q1; p++) {
         int v = vertex_at(a, p);
-        if (!selected(a.seed, a.step, v, a.f)) continue;
+        if (!lp_selected(a, v)) continue;
         int start = a.indices[v], deg = a.indices[v + 1] - start;
         int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
         int own = ownr.x;
@@ -568,7 +568,7 @@ __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
                               NT);
         __syncthreads();
         const int D = s_nocc;
-        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
+        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
         scan_listed<AUX, KM>(a, keys, cnts, auxs, occ, D, threadIdx.x, NT, deg, own, tb, c, own_cnt);
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) 
     for (int it = blockIdx.x; it < h.nchunks; it += gridDim.x) {
         const int4 ck = h.chunks[it];
         const int v = ck.y;
-        if (!selected(a.seed, a.step, v, a.f)) continue;
+        if (!lp_selected(a, v)) continue;
         const int P = h.info[ck.x].y;
         for (int i = threadIdx.x; i <= P; i += HUB1_NT) cnt[i] = 0;
         __syncthreads();
@@ -742,7 +742,7 @@ __global__ __launch_bounds__(NT) void k_hub_count_listed(LPArgs a, HubArgs h) {
         int4 pi_n = pi, pw_n = pw;
         if (nxt < h.nparts) { pi_n = h.parts[nxt]; pw_n = h.where[nxt]; }
         const int v = pw.x, P = pw.y, part = pi.y;
-        if (selected(a.seed, a.step, v, a.f)) {
+        if (lp_selected(a, v)) {
             for (int c0 = 0; c0 < pi.w; c0 += WAVE) {
                 const int nc = min(WAVE, pi.w - c0);
                 int len = 0, b0 = 0;
@@ -801,7 +801,7 @@ __global__ __launch_bounds__(NT) void k_hub_count_listed(LPArgs a, HubArgs h) {
                     cnts[hs] = 0;
                 }
             } else {
-                uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
+                uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), v);
                 scan_listed<AUX, KM>(a, keys, cnts, auxs, occ, D, threadIdx.x, NT, deg, own, tb, c, own_cnt);
             }
             block_reduce_cand<NT>(c, own_cnt, distinct, s_c, s_i);
@@ -832,7 +832,7 @@ __global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
     for (int hub = wave_g; hub < h.nhubs; hub += nwaves) {
         const int4 inf = h.info[hub];
         const int v = inf.x, P = inf.y;
-        if (!selected(a.seed, a.step, v, a.f)) continue;
+        if (!lp_selected(a, v)) continue;
         if (h.over[hub]) continue;  // recounted by the global-table path
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
@@ -895,7 +895,7 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
         const int deg = a.indices[v + 1] - a.indices[v], own = a.labels[v];
         const int64_t off = toff[hub];
         const int T = 1 << tlog[hub];
-        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, a.step, v);
+        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
         for (int s = threadIdx.x; s < T; s += 1024) {
@@ -1004,6 +1004,8 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
             else
                 hipLaunchKernelGGL((k_hub_count_listed<KM, 512, HUB_TB, true>), dim3(grid_for(ctx, h.nparts, 1, 2)),
                                    dim3(512), 0, ctx->stream, ah, h);
+            kt_end(ctx, KS_LP_HUB_DECIDE);
+            kt_begin(ctx, KS_LP_HUB_FINAL);
             hipLaunchKernelGGL((k_hub_final<KM, true>), dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0, ctx->stream,
                                ah, h);
         } else {
@@ -1013,6 +1015,8 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
             else
                 hipLaunchKernelGGL((k_hub_count_listed<KM, 512, HUB_TB, false>), dim3(grid_for(ctx, h.nparts, 1, 3)),
                                    dim3(512), 0, ctx->stream, ah, h);
+            kt_end(ctx, KS_LP_HUB_DECIDE);
+            kt_begin(ctx, KS_LP_HUB_FINAL);
             hipLaunchKernelGGL((k_hub_final<KM, false>), dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0,
                                ctx->stream, ah, h);
         }
@@ -1020,7 +1024,7 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
                            h, ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts);
         hipLaunchKernelGGL(k_hub_global_decide<KM>, dim3(grid_for(ctx, h.nhubs, 1, 1)), dim3(1024), 0, ctx->stream, ah,
                            h, ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts);
-        kt_end(ctx, KS_LP_HUB_DECIDE);
+        kt_end(ctx, KS_LP_HUB_FINAL);
     }
     if (range(BIN_B4096, p0, p1)) {
         // degrees 2049-4096: 512-thread workgroups with 8192-slot tables (two
@@ -1036,33 +1040,35 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
                                0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B4096)});
         kt_end(ctx, KS_LP_B4096);
     }
+    // degrees 513-1024: one workgroup per vertex (2048-slot table)
     if (range(BIN_B1024, p0, p1)) {
-        // degrees 513-1024: one workgroup per vertex (2048-slot table);
-        // 257-512: one wave per vertex with a 1024-slot table, so a CU holds
-        // 16 vertices in flight instead of 8
-        const int ps = ctx->tune.b1024_wave ? std::min(std::max(ctx->bins.split512, p0), p1) : p1;
         kt_begin(ctx, KS_LP_B1024);
-        if (ps > p0) {
-            if (block_aux)
-                hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc, true>), dim3(grid_for(ctx, ps - p0, 1, 8)),
-                                   dim3(256), 0, ctx->stream, RangeSrc{p0, ps, 1, args_for(BIN_B1024)});
-            else
-                hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid_for(ctx, ps - p0, 1, 8)),
-                                   dim3(256), 0, ctx->stream, RangeSrc{p0, ps, 1, args_for(BIN_B1024)});
-        }
-        if (p1 > ps) {
-            if (block_aux)
-                hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - ps, 4, 3)),
-                                   dim3(256), 0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B1024)});
-            else if (ctx->tune.b1024_u16)  // 16-bit counts (degree <= 512): 28.7 KB, five workgroups per CU
-                hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8, uint16_t>),
-                                   dim3(grid_for(ctx, p1 - ps, 4, 5)), dim3(256), 0, ctx->stream,
-                                   RangeSrc{ps, p1, 1, args_for(BIN_B1024)});
-            else  // 4-8 entries per lane: all eight gathers in flight at once
-                hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8>), dim3(grid_for(ctx, p1 - ps, 4, 4)),
-                                   dim3(256), 0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B1024)});
-        }
+        if (block_aux)
+            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - p0, 1, 8)),
+                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
+        else
+            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256),
+                               0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
         kt_end(ctx, KS_LP_B1024);
+    }
+    // degrees 257-512: one wave per vertex with a 1024-slot table, so a CU
+    // holds 16 vertices in flight instead of 8 (or, tuned off, workgroups as
+    // for 513-1024)
+    if (range(BIN_B512, p0, p1)) {
+        kt_begin(ctx, KS_LP_B512);
+        if (!ctx->tune.b1024_wave)
+            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256),
+                               0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B512)});
+        else if (block_aux)
+            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - p0, 4, 3)), dim3(256),
+                               0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B512)});
+        else if (ctx->tune.b1024_u16)  // 16-bit counts (degree <= 512): 28.7 KB, five workgroups per CU
+            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8, uint16_t>), dim3(grid_for(ctx, p1 - p0, 4, 5)),
+                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B512)});
+        else  // 4-8 entries per lane: all eight gathers in flight at once
+            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8>), dim3(grid_for(ctx, p1 - p0, 4, 4)),
+                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B512)});
+        kt_end(ctx, KS_LP_B512);
     }
     // W256 and G64..G4 in one launch unless a record-gathering form is tuned on
     const bool fused = !small_aux && !block_aux && ctx->tune.small_fused;
@@ -1091,7 +1097,7 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
         SmallBins sb;
         // bins W256, G64..G4 (consecutive bin numbers) and positions per unit
         const int cps[6] = {1, 8, 16, 32, 64, 64};
-        static_assert(BIN_G64 == BIN_W256 + 1 && BIN_G4 == BIN_W256 + 5, "bin order");
+        static_assert(BIN_G64 == BIN_W256 + 1 && BIN_G4 == BIN_W256 + 5 && BIN_W256 == BIN_B512 + 1, "bin order");
         int64_t tot = 0;
         int nonempty = 0;
         for (int i = 0; i < 6; i++) {
@@ -1159,6 +1165,7 @@ int propagation_launch(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, 
     a.use_aux = rule.kind == PCD_RULE_MODULARITY || rule.kind == PCD_RULE_CPM;
     a.tot = a.use_aux ? ctx->d_tot : nullptr;
     a.counters = nullptr;
+    a.step_ptr = ctx->graph_step;  // set while a pass is captured for replay
     // kernels specialised for modularity or CPM hold fewer registers (more
     // waves per SIMD); WCC and conductance use the general build
     if (rule.kind == PCD_RULE_MODULARITY) PCD_TRY(launch_propagation<RK_MOD>(ctx, a));
@@ -1168,7 +1175,7 @@ int propagation_launch(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, 
 }
 
 int propagation_commit(pcd_ctx *ctx) {
-    if (ctx->k_deferred) {  // label_bound below needs k on the host
+    if (ctx->k_deferred && !ctx->capturing) {  // label_bound below needs k on the host
         PCD_HIP(hipStreamSynchronize(ctx->stream));
         resolve_k(ctx);
     }
@@ -1180,7 +1187,8 @@ int propagation_commit(pcd_ctx *ctx) {
     ctx->labels_compressed = false;
     ctx->has_stats = false;
     ctx->stats_pending = follow;
-    ctx->label_bound = ctx->k;
+    // a captured pass keeps k on the device: labels stay below n
+    ctx->label_bound = ctx->capturing ? ctx->n : ctx->k;
     return PCD_OK;
 }
 

@@ -27,9 +27,23 @@ struct LPArgs {
     int f;        // selection threshold (refinement.cpp:116)
     int use_aux;  // the rule's gains read only aux
     unsigned long long *counters;  // {active, entries, distinct} of the bin, or nullptr
+    const int *step_ptr;  // the step number in device memory (replayed graphs), or nullptr: `step`
 };
 
 __device__ inline int vertex_at(const LPArgs &a, int p) { return a.order ? a.order[p] : p; }
+
+// the step number of the selection hash and the tie keys
+__device__ inline int step_of(const LPArgs &a) { return a.step_ptr ? *a.step_ptr : a.step; }
+
+// refinement.cpp:129: the vertex takes part in this step.  A replayed graph
+// whose loop has stopped sets its device step number to -1: no vertex does.
+__device__ inline bool lp_selected(const LPArgs &a, int v) {
+    if (a.step_ptr) {
+        const int st = *a.step_ptr;
+        return st >= 0 && selected(a.seed, st, v, a.f);
+    }
+    return selected(a.seed, a.step, v, a.f);
+}
 
 // positions [p0, p1) of one bin in units of `per` positions
 struct RangeSrc {

@@ -37,6 +37,7 @@ EXPORTS = [
     "pcd_server_info", "pcd_gen_requests",
     "pcd_graphset_open", "pcd_graphset_close", "pcd_graphset_count", "pcd_graphset_graph", "pcd_graphset_extract",
     "pcd_graphset_writer_open", "pcd_graphset_writer_add", "pcd_graphset_writer_close", "pcd_graphset_pack",
+    "pcd_iterate_many",
 ]
 BATCH_RUNNING, BATCH_DONE, BATCH_FREE = 0, 1, 2
 
@@ -93,6 +94,9 @@ def lib():
     L.pcd_labels_upload.argtypes = [vp, vp]
     L.pcd_labels_download.argtypes = [vp, vp]
     L.pcd_compress_labels.argtypes = [vp, C.POINTER(C.c_int)]
+    L.pcd_iterate_many.argtypes = [vp, C.POINTER(Job), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                   C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int)]
     L.pcd_collect_statistics.argtypes = [vp, C.POINTER(C.c_int)]
     L.pcd_stats_download.argtypes = [vp, vp, vp, vp]
     L.pcd_metric.argtypes = [vp, Rule, C.POINTER(C.c_double)]
@@ -546,6 +550,14 @@ class Context:
         sc, it, k = C.c_double(), C.c_int(), C.c_int()
         check(lib().pcd_refine(self._ctx, C.byref(job_), strategy, rounds, C.byref(sc), C.byref(it), C.byref(k)))
         return sc.value, it.value, k.value
+
+    def iterate_many(self, job_, it, count, prev_score, strategy=0, rounds=1):
+        """pcd_iterate_many: passes it .. it+count-1 (device-resident for
+        modularity and CPM with -s 0): (passes run, score, k, done)."""
+        p, sc, k, d = C.c_int(), C.c_double(), C.c_int(), C.c_int()
+        check(lib().pcd_iterate_many(self._ctx, C.byref(job_), strategy, rounds, int(it), int(count),
+                                     float(prev_score), C.byref(p), C.byref(sc), C.byref(k), C.byref(d)))
+        return p.value, sc.value, k.value, bool(d.value)
 
     def iterate(self, job_, it, prev_score, strategy=0, rounds=1):
         """pcd_iterate: one pass of the refinement loop numbered `it`:

@@ -618,3 +618,36 @@ def test_degenerate_graphs(ctx, name):
         ctx.upload_labels(init)
         sc, it, kk = ctx.refine(parcd.job(kind=O.MOD, seed=3, tie=O.TIE_LOWEST, max_iter=4), strategy=strategy)
         assert np.array_equal(ctx.labels(), lab_o) and it == iters_o and kk == k_o
+
+
+def test_wcc_int32_wrap_on_device(ctx):
+    """WCC gains of a community above 46,341 vertices: the reference's int32
+    products (n_x - 1) * (n_x - 2) / 2, (c_x - 1) * c_y, c_y * (c_y - 1) / 2
+    and c_x * c_y wrap there (rules.cpp:243, 265, 267-268, 297, 326, 328-329,
+    358).  One propagation step and one merging step from the planted
+    blocks (a 50,000-vertex community) equal the oracle, whose wrapping is
+    pinned against the reference's own gains."""
+    g = parcd.make_graph("sbm", 11, block_sizes=[50000, 8000], p_in=0.0005, p_out=0.00005)
+    n, m, ind, adj, mp = g
+    upload(ctx, g)
+    tri, td, cc = ctx.triangles()
+    et, tri_o, td_o, cc_o = O.count_triangles(n, ind, adj)
+    assert np.array_equal(tri, tri_o) and np.array_equal(td, td_o)
+    assert cc == pytest.approx(cc_o, rel=1e-12)
+    rng = np.random.default_rng(12)
+    lab = (mp >= 50000).astype(np.int32)
+    moved = rng.random(n) < 0.05        # a few vertices elsewhere, so moves go both ways
+    lab[moved] = rng.integers(2, 400, int(moved.sum()))
+    lab_o, so, ivo, evo = check_partition(ctx, g, lab)
+    assert so.max() >= 46342            # the wrapping sizes are reached
+    got = ctx.metric(O.WCC)
+    want = O.calculate(O.WCC, 0, n, m, ind, adj, lab_o, so, ivo, evo, tri_o, td_o)
+    assert got == pytest.approx(want, rel=1e-9, abs=1e-12)
+    for tie in (O.TIE_RANDOM, O.TIE_LOWEST):
+        _lp_compare(ctx, g, lab, O.WCC, 0.0, 1.0, seed=5, step=3, tie=tie, cc=cc)
+        lab_o, so, ivo, evo = check_partition(ctx, g, lab)
+        ctx.set_cc(cc)
+        ctx.merge(O.WCC, 0.0, rounds=2, seed=6, step=4, tie=tie)
+        got = ctx.labels()
+        want, bp, fp = O.merging_refine(O.WCC, 0.0, n, m, ind, adj, lab_o, so, ivo, evo, cc, 2, 1, 6, 4, tie)
+        assert np.array_equal(got, want)

@@ -3,35 +3,26 @@ SBM-1 (100 blocks of 100, p_in 0.08, p_out 0.0005, generator seed 1).
 
 The GPU draws vertex selection and ties from a counter hash; the reference
 draws them from glibc rand/rand_r, so single runs differ and only the
-distributions can be compared.  The reference side is the oracle in glibc
-mode (label-exact with the reference binary at -p 1), reset with srand(s)
-for seeds s = 1..SEEDS; the GPU side is pcd_refine with seeds 1..SEEDS.
-Final scores: the means must agree within 0.5 % of the reference mean, or
-within three standard errors of the difference where the runs' own spread
-is larger than that (label propagation alone, -s 0, spreads by ~2 %).
-Agreement with the planted blocks, as normalized mutual information
-(pcd_nmi, pinned to the reference's compare_communities), is held to the
-same bar."""
-import multiprocessing as mp
-import os
-
+distributions can be compared.  The reference side is the committed fixture
+tests/golden/quality_sbm1.npz (tests/golden/make_quality.py): the oracle in
+glibc mode -- label-exact with the reference binary at -p 1 -- reset with
+srand(s) for s = 1..256.  The GPU side is pcd_refine with seeds 1..256.  The
+mean final score must be within 0.5 % of the reference's mean for every
+metric and schedule, and so must the mean agreement with the planted blocks
+(normalized mutual information, pcd_nmi, pinned to the reference's
+compare_communities)."""
 import numpy as np
 import pytest
 
 import oracle_lib as O
+from conftest import golden, load
 from gpu_graphs import sbm, singleton_labels
 
 pytestmark = pytest.mark.gpu
 parcd = pytest.importorskip("parcd")
 
-SEEDS = 12
 CASES = [("modularity", O.MOD, 0.0, 0), ("cpm-0.01", O.CPM, 0.01, 0), ("wcc -i 1", O.WCC, 0.0, 1)]
-
-
-def _ref(a):
-    n, m, ind, adj, kind, lam, initial, strategy, seed = a
-    r = O.run(n, m, ind, adj, kind=kind, lam=lam, initial=initial, strategy=strategy, mode=0, seed=seed)
-    return r[1], r[0]
+BAR = 0.005
 
 
 def planted(g):
@@ -42,11 +33,11 @@ def planted(g):
 
 def same_bar(name, ours, ref):
     diff = ours.mean() - ref.mean()
-    se = np.sqrt(ours.var(ddof=1) / SEEDS + ref.var(ddof=1) / SEEDS)
+    se = np.sqrt(ours.var(ddof=1) / len(ours) + ref.var(ddof=1) / len(ref))
     print("%s: gpu %.6f +- %.6f, reference %.6f +- %.6f, rel diff %+.3f %%, diff/se %+.2f" % (
         name, ours.mean(), ours.std(), ref.mean(), ref.std(), 100 * diff / abs(ref.mean()),
         diff / se if se > 0 else 0.0))
-    assert abs(diff) <= max(0.005 * abs(ref.mean()), 3 * se)
+    assert abs(diff) <= BAR * abs(ref.mean())
 
 
 @pytest.fixture(scope="module")
@@ -56,15 +47,9 @@ def sbm1():
 
 @pytest.fixture(scope="module")
 def reference_scores(sbm1):
-    n, m, ind, adj, _ = sbm1
-    work = [(n, m, ind, adj, kind, lam, ini, st, sd) for _, kind, lam, ini in CASES for st in (0, 2)
-            for sd in range(1, SEEDS + 1)]
-    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
-        res = pool.map(_ref, work)
-    truth = planted(sbm1)
-    scores = np.array([r[0] for r in res]).reshape(len(CASES), 2, SEEDS)
-    nmis = np.array([parcd.nmi(r[1], truth) for r in res]).reshape(len(CASES), 2, SEEDS)
-    return scores, nmis
+    d = load(golden("quality_sbm1.npz")[0])
+    assert tuple(d["graph_n_m"]) == (sbm1[0], sbm1[1])
+    return d["scores"], d["nmi"], d["seeds"]
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))
@@ -72,17 +57,17 @@ def reference_scores(sbm1):
 def test_quality_matches_reference(sbm1, reference_scores, case, strategy):
     name, kind, lam, ini = CASES[case]
     n, m, ind, adj, _ = sbm1
+    scores, nmis, seeds = reference_scores
     ours, ours_nmi = [], []
     truth = planted(sbm1)
     with parcd.Context(0) as ctx:
         ctx.upload_graph(n, m, ind, adj)
         tri = ctx.triangles()[0]
-        for sd in range(1, SEEDS + 1):
-            lab = parcd.clustering_partition(n, ind, adj, tri) if ini else singleton_labels(n)
+        lab = parcd.clustering_partition(n, ind, adj, tri) if ini else singleton_labels(n)
+        for sd in seeds:
             ctx.upload_labels(lab)
             ours.append(ctx.refine(parcd.job(kind=kind, lam=lam, seed=sd), strategy=strategy)[0])
             ours_nmi.append(parcd.nmi(ctx.labels(), truth))
-    scores, nmis = reference_scores
     side = 0 if strategy == 0 else 1
     same_bar("%s -s %d score" % (name, strategy), np.array(ours), scores[case, side])
     same_bar("%s -s %d NMI vs planted" % (name, strategy), np.array(ours_nmi), nmis[case, side])

@@ -440,3 +440,61 @@ def test_nmi_file_matches_pairs(tmp_path):
     assert O.nmi_file(n, lab, str(p)) == pytest.approx(O.nmi(truth + 1, lab), rel=1e-13)
     p.write_text("1 2 %d\n" % n)
     assert O.nmi_file(n, lab, str(p)) is None
+
+
+def _distinct_size_partition(n, rng):
+    """Communities of pairwise different sizes 1, 2, 3, ... over a random
+    vertex order (the remainder joins the largest)."""
+    sizes, tot = [], 0
+    while tot + len(sizes) + 1 <= n:
+        sizes.append(len(sizes) + 1)
+        tot += sizes[-1]
+    sizes[-1] += n - tot
+    lab = np.repeat(np.arange(len(sizes)), sizes)[np.argsort(rng.permutation(n))]
+    return lab.astype(np.int32)
+
+
+@pytest.mark.parametrize("name", ["golden_gnp150", "golden_sbm200", "golden_rmat9", "golden_starclique"])
+def test_counter_mode_equals_libc_mode_without_ties(golden_graphs, name):
+    """At frac = 1 every vertex is selected in both modes, so the two
+    replays differ only in how they break equal gains.  With CPM at
+    resolution 0.0137 = 137/10^4 over communities of pairwise different
+    sizes, two candidates' gains k - lambda s (joins) or links - lambda s_i s_j
+    (merges) can only be equal for equal sizes, and none is 0: the inputs
+    have no ties, and counter mode (hash selection, tie keys, its own
+    comparison branches) must give the labels of libc mode, which the
+    reference's refine fixtures pin (refinement.cpp:114-182, 185-288)."""
+    n, m, ind, adj, _, _ = golden_graphs[name[7:]]
+    rng = np.random.default_rng(21)
+    lam = 0.0137
+    for trial in range(3):
+        lab, k = O.compress_labels(_distinct_size_partition(n, rng))
+        _, s, iv, ev = O.collect_statistics(n, ind, adj, lab)
+        assert len(set(s.tolist())) == len(s)
+        for tie in (O.TIE_RANDOM, O.TIE_LOWEST):
+            libc = O.propagation_refine(O.CPM, lam, n, m, ind, adj, lab, s, iv, ev, 0.0, 1.0, 0, 5 + trial, 0, tie)
+            ctr = O.propagation_refine(O.CPM, lam, n, m, ind, adj, lab, s, iv, ev, 0.0, 1.0, 1, 5 + trial, 0, tie)
+            assert np.array_equal(libc, ctr)
+            assert (libc != lab).any()
+            for rounds in (1, 3):
+                ml, _, _ = O.merging_refine(O.CPM, lam, n, m, ind, adj, lab, s, iv, ev, 0.0, rounds, 0, 7, 1, tie)
+                mc, _, _ = O.merging_refine(O.CPM, lam, n, m, ind, adj, lab, s, iv, ev, 0.0, rounds, 1, 7, 1, tie)
+                assert np.array_equal(ml, mc)
+
+
+def test_quality_fixture_is_the_oracle():
+    """tests/golden/quality_sbm1.npz holds the oracle's glibc-mode runs on
+    SBM-1; re-running a few seeds of every case reproduces it exactly."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_quality as MQ
+    d = load(golden("quality_sbm1.npz")[0])
+    g = MQ.sbm1()
+    n, m, ind, adj, _ = g
+    assert tuple(d["graph_n_m"]) == (n, m)
+    for c, (_, kind, lam, ini) in enumerate(MQ.CASES):
+        for side, st in enumerate((0, 2)):
+            for j in (0, 1, 255):
+                sd = int(d["seeds"][j])
+                lab, score, iters = MQ.reference_run(n, m, ind, adj, kind, lam, ini, st, sd)
+                assert score == d["scores"][c, side, j] and iters == d["iterations"][c, side, j]
