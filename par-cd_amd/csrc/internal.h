@@ -9,7 +9,7 @@
 #include <vector>
 
 #include "parcd.h"
-#include "rules_device.h"
+#include "rule_formulas.h"
 
 namespace pcd {
 
