@@ -81,6 +81,24 @@ int pcd_labels_download(pcd_ctx *ctx, int32_t *labels);
  * with the reference.  *num_labels (may be NULL) receives k. */
 int pcd_compress_labels(pcd_ctx *ctx, int *num_labels);
 
+/* No counterpart in the reference: an allocation-free compress_labels for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same result as
+ * pcd_compress_labels, bit for bit, on caller-owned DEVICE arrays: labels[n]
+ * (values in [0,n) or PCD_NEW_LABEL) is read, out_labels[n] receives the
+ * compressed labels and *num_labels (one device int32) receives k, or -1
+ * when a label lies outside [-1, n).  `stream` is a hipStream_t (NULL: the
+ * null stream); the call only enqueues kernels and returns.  labels and
+ * out_labels must be different arrays.  Scratch (first-occurrence and
+ * new-id tables, scan partials) is a file-scope device array of about
+ * 32 MiB that each device holds until the process ends: n is limited to
+ * PCD_NOALLOC_MAX_VERTICES (PCD_E_INVALID above).  Every call on one device
+ * shares that array, so calls that overlap in time on one device race.
+ * No hipMalloc, hipFree, hipMemset or allocating library routine is called. */
+#define PCD_NOALLOC_MAX_VERTICES 4194304
+int pcd_compress_labels_noalloc(int n, const int32_t *labels, int32_t *out_labels, int32_t *num_labels,
+                                void *stream);
+
 /* framework.cpp:103-143 collect_statistics: sizes, int_volume, ext_volume
  * for the current (compressed) labels.  k = max label + 1.  Bit-exact. */
 int pcd_collect_statistics(pcd_ctx *ctx, int *num_labels);

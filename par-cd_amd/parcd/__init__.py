@@ -37,7 +37,7 @@ EXPORTS = [
     "pcd_server_info", "pcd_gen_requests",
     "pcd_graphset_open", "pcd_graphset_close", "pcd_graphset_count", "pcd_graphset_graph", "pcd_graphset_extract",
     "pcd_graphset_writer_open", "pcd_graphset_writer_add", "pcd_graphset_writer_close", "pcd_graphset_pack",
-    "pcd_iterate_many",
+    "pcd_compress_labels_noalloc", "pcd_iterate_many",
 ]
 BATCH_RUNNING, BATCH_DONE, BATCH_FREE = 0, 1, 2
 
@@ -94,6 +94,7 @@ def lib():
     L.pcd_labels_upload.argtypes = [vp, vp]
     L.pcd_labels_download.argtypes = [vp, vp]
     L.pcd_compress_labels.argtypes = [vp, C.POINTER(C.c_int)]
+    L.pcd_compress_labels_noalloc.argtypes = [C.c_int, vp, vp, vp, vp]
     L.pcd_iterate_many.argtypes = [vp, C.POINTER(Job), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                    C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int)]
@@ -195,6 +196,18 @@ def lib():
 def check(rc):
     if rc != 0:
         raise PcdError("pcd error %d: %s" % (rc, lib().pcd_last_error().decode(errors="replace")))
+
+
+NOALLOC_MAX_VERTICES = 4194304
+
+
+def compress_labels_noalloc(n, labels_dev, out_dev, num_labels_dev, stream=None):
+    """pcd_compress_labels_noalloc on device addresses (ints) allocated with
+    the HIP runtime libparcd.so uses: enqueues the compression on `stream` (a
+    hipStream_t as an int, None for the null stream) and returns without
+    waiting."""
+    check(lib().pcd_compress_labels_noalloc(int(n), C.c_void_p(labels_dev), C.c_void_p(out_dev),
+                                            C.c_void_p(num_labels_dev), C.c_void_p(stream)))
 
 
 def _i32(a):
