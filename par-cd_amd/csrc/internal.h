@@ -252,6 +252,7 @@ int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out);
 // modularity, CPM, conductance: kernels and the score's copy into
 // ctx->h_result, no wait
 int metric_enqueue(pcd_ctx *ctx, pcd_rule rule);
+int wcc_metric_enqueue(pcd_ctx *ctx);
 int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie);
 // propagation_dev in two parts: the kernels that write the next labels into
 // d_labels_alt, then the statistics update and the swap that make them the

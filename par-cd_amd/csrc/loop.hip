@@ -1,8 +1,8 @@
 // The refinement loop of main.cpp:226-266 kept on the device.
 //
 // One pass (compress, metric, stop rule, propagation step with the
-// statistics update) of a modularity or CPM run with -s 0 is captured once
-// as a hipGraph and replayed.  The pass number, the previous score and the
+// statistics update) of a run with -s 0 is captured once as a hipGraph and
+// replayed (every rule; WCC once the graph's triangles are counted).  The pass number, the previous score and the
 // stop rule (main.cpp:241-242, -n, the iteration cap) live in a device
 // record; the captured propagation reads its step number from it, so every
 // replay is the next pass.  When the rule fires, or the requested number of
@@ -292,7 +292,7 @@ extern "C" int pcd_iterate_many(pcd_ctx *ctx, const pcd_job *job, int strategy, 
     // strategies the captured pass does not cover
     for (;;) {
         const bool device_ok = strategy == 0 &&
-                               (job->rule.kind == PCD_RULE_MODULARITY || job->rule.kind == PCD_RULE_CPM) &&
+                               (job->rule.kind != PCD_RULE_WCC || ctx->has_triangles) &&
                                ctx->tune.graph_loop && ctx->tune.inc_stats && ctx->ev_metric && ctx->has_labels &&
                                (ctx->stats_pending || ctx->has_stats) &&
                                2 * (int64_t)ctx->m >= ctx->tune.graph_loop_min_entries &&

@@ -597,6 +597,7 @@ int metric_dev(pcd_ctx *ctx, pcd_rule rule, double *out) {
 }
 
 int metric_enqueue(pcd_ctx *ctx, pcd_rule rule) {
+    if (rule.kind == PCD_RULE_WCC) return wcc_metric_enqueue(ctx);
     kt_begin(ctx, KS_METRIC);
     if (rule.kind == PCD_RULE_CONDUCTANCE) {
         hipLaunchKernelGGL(k_cond_partial, dim3(FSUM_BLOCKS), dim3(256), 0, ctx->stream, ctx->d_stats, ctx->d_k,

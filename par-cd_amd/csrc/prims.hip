@@ -210,6 +210,7 @@ static int lsd_passes(pcd_ctx *ctx, uint64_t *src, uint64_t *dst, int64_t n, int
 constexpr int OS_MAX_PASSES = 8;
 constexpr uint32_t OS_AGG = 1u << 30, OS_INC = 2u << 30, OS_VAL = (1u << 30) - 1;
 constexpr int OS_SPIN_LIMIT = 1 << 20;
+constexpr int OS_WIN = 16;  // look-back words read at once per digit
 
 __global__ __launch_bounds__(RS_THREADS) void k_os_hist(const uint64_t *keys, int64_t n, int passes,
                                                         uint32_t *ghist) {
@@ -302,24 +303,42 @@ __global__ __launch_bounds__(RS_THREADS) void k_os_scatter(const uint64_t *keys,
         run[t] += add;
         __syncthreads();
     }
-    // decoupled look-back: thread t owns digit t
+    // decoupled look-back: thread t owns digit t and reads the words of
+    // OS_WIN earlier tiles at once (nearest first), adding aggregates until
+    // an inclusive prefix; a word not yet published is read again
     uint32_t prefix = 0;
     if (tile > 0) {
         int j = tile - 1, spins = 0;
         while (j >= 0) {
-            const uint32_t s =
-                __hip_atomic_load(status + (size_t)j * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((s & ~OS_VAL) == 0) {
+            uint32_t s[OS_WIN];
+#pragma unroll
+            for (int e = 0; e < OS_WIN; e++)
+                s[e] = j - e >= 0 ? __hip_atomic_load(status + (size_t)(j - e) * 256 + t, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : OS_INC;  // never reached: tile 0 publishes an inclusive prefix
+            int used = 0;
+            bool found = false, waiting = false;
+#pragma unroll
+            for (int e = 0; e < OS_WIN; e++) {
+                if (found || waiting) continue;
+                const uint32_t fl = s[e] & ~OS_VAL;
+                if (fl == 0) {
+                    waiting = true;
+                } else {
+                    prefix += s[e] & OS_VAL;
+                    used++;
+                    if (fl == OS_INC) found = true;
+                }
+            }
+            if (found) break;
+            j -= used;
+            if (waiting) {
                 if (++spins > OS_SPIN_LIMIT) {
                     atomicOr(err, 1);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(2);
-                continue;
             }
-            prefix += s & OS_VAL;
-            if ((s & ~OS_VAL) == OS_INC) break;
-            j--;
         }
         __hip_atomic_store(&my[t], OS_INC | (prefix + (uint32_t)cnt[t]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }

@@ -354,15 +354,25 @@ int triangles_dev(pcd_ctx *ctx) {
     return PCD_OK;
 }
 
-int wcc_metric_dev(pcd_ctx *ctx, double *out) {
-    if (!ctx->has_triangles) PCD_TRY(triangles_dev(ctx));
+// WCCRule::calculate (rules.cpp:150-235) into d_result; the triangles of
+// the graph must be counted.  The score is copied to the host unless a pass
+// is being captured.
+int wcc_metric_enqueue(pcd_ctx *ctx) {
+    if (!ctx->has_triangles) return fail(PCD_E_STATE, "WCC metric needs the triangle counts");
     PCD_TRY(count_pass(ctx, ctx->d_labels, ctx->d_it, ctx->d_itd));
     hipLaunchKernelGGL(k_wcc_partial, dim3(TSUM_BLOCKS), dim3(256), 0, ctx->stream, ctx->n, ctx->d_labels,
                        ctx->d_stats, ctx->d_tri, ctx->d_tri_deg, ctx->d_it, ctx->d_itd, ctx->d_dacc);
     hipLaunchKernelGGL(k_sum_partials_plain, dim3(1), dim3(256), 0, ctx->stream, ctx->d_dacc, TSUM_BLOCKS,
                        ctx->d_result);
     PCD_HIP(hipGetLastError());
-    PCD_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (!ctx->capturing)
+        PCD_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return PCD_OK;
+}
+
+int wcc_metric_dev(pcd_ctx *ctx, double *out) {
+    if (!ctx->has_triangles) PCD_TRY(triangles_dev(ctx));
+    PCD_TRY(wcc_metric_enqueue(ctx));
     PCD_HIP(hipStreamSynchronize(ctx->stream));
     *out = ctx->h_result[0];
     return PCD_OK;

@@ -112,3 +112,28 @@ def test_stop_rule_fires_inside_the_replays():
     assert done and passes == oit and k == ok
     assert np.array_equal(got, ol)
     assert abs(score - osc) <= 1e-9 * max(1.0, abs(osc))
+
+
+@pytest.mark.parametrize("name", ["sbm", "bin_zoo"])
+def test_wcc_refine_equals_pass_by_pass_and_replay(name):
+    """WCC from the triangle-based initial partition (-i 1): the captured
+    pass runs the internal-triangle metric on the device too."""
+    g = graphs()[name]
+    n, m, ind, adj, _ = g
+    jb = parcd.job(kind=O.WCC, seed=7, iter_cap=25)
+    res = {}
+    for loop in (1, 0):
+        with parcd.Context(0) as ctx:
+            ctx.set_tuning("graph_loop", loop)
+            ctx.upload_graph(n, m, ind, adj)
+            cc = ctx.triangles()[2]
+            ctx.clustering()
+            score, iters, k = ctx.refine(jb)
+            res[loop] = (ctx.labels(), score, iters, k, cc)
+    a, b = res[1], res[0]
+    assert np.array_equal(a[0], b[0]) and a[1:] == b[1:]
+    # the device's clustering coefficient, as the gains read it
+    ol, osc, oit, ok, _ = O.run(n, m, ind, adj, kind=O.WCC, initial=1, mode=1, seed=7,
+                                max_iter=25 if b[2] >= 25 else -1, cc_override=a[4])
+    assert np.array_equal(a[0], ol) and (a[2], a[3]) == (oit, ok)
+    assert abs(a[1] - osc) <= 1e-9 * max(1.0, abs(osc))

@@ -502,6 +502,32 @@ def test_rmat22_benchmark_iterations(ctx, kind, lam):
         prev = got
 
 
+def test_rmat22_merging_first_iterations(ctx):
+    """The merging leg of the benchmark at full size (R-MAT scale 22 seed 2,
+    -s 1): in each of the first two iterations from the singleton partition,
+    the contraction's community pairs (l, k, links) -- 128 M of them at first,
+    sorted by the radix sort -- equal the oracle's table
+    (refinement.cpp:228-238), and the merging step's labels equal the
+    oracle's replay (refinement.cpp:185-288)."""
+    g = GG.rmat(22)
+    n, m, ind, adj = g[:4]
+    ctx.upload_graph(n, m, ind, adj)
+    prev = GG.singleton_labels(n)
+    for it in range(2):
+        lab_o, so, ivo, evo = check_partition(ctx, g, prev)
+        l, k, w = ctx.contract()
+        lo, ko, wo = O.community_pairs(n, ind, adj, lab_o, len(so))
+        assert len(l) == len(lo), it
+        assert np.array_equal(l, lo) and np.array_equal(k, ko) and np.array_equal(w, wo), it
+        del l, k, w, lo, ko, wo
+        ctx.merge(O.MOD, 0.0, rounds=1, seed=5, step=it, tie=O.TIE_RANDOM)
+        got = ctx.labels()
+        want, _, _ = O.merging_refine(O.MOD, 0.0, n, m, ind, adj, lab_o, so, ivo, evo, 0.0, 1, 1, 5, it,
+                                      O.TIE_RANDOM)
+        assert np.array_equal(got, want), (it, int(np.count_nonzero(got != want)))
+        prev = got
+
+
 def test_chung_lu_wcc_benchmark_iterations(ctx):
     """The benchmark's WCC workload at full size (Chung-Lu seed 3: 10^6
     vertices, degrees up to 8,415, so rows in the hub path), run as
