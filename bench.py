@@ -253,13 +253,8 @@ def cpu_reference_sbm1(pcd_path, reps=3):
 ITER_CAP = 100
 
 
-def merge_leg(parcd, np, ctx, n, m, init, steps, warmup, seed):
-    """Parallel merging (-s 1, one round) on R-MAT-22 from the singleton
-    partition: compress + statistics + metric + merging step per iteration,
-    the same counting as the propagation iterations.  The contraction inside
-    the step (community-pair keys, LSD radix sort, run-length reduce) is
-    timed per phase; its sort streams 16 bytes per key and pass."""
-    E = 2 * m
+def _merge_iterations(parcd, np, ctx, E, init, steps, warmup, seed, merge_sort):
+    ctx.set_tuning("merge_sort", merge_sort)
     ctx.upload_labels(init)
     ctx.enable_timing(True)
     ks = []
@@ -279,18 +274,48 @@ def merge_leg(parcd, np, ctx, n, m, init, steps, warmup, seed):
     ctx.synchronize()
     t1 = time.perf_counter()
     timers = ctx.kernel_timers()
+    ctx.set_tuning("merge_sort", 0)
     ms = (t1 - t0) * 1000 / steps
-    sort_ms = timers.get("merge_sort", (0.0, 0))[0] / steps
-    # LSD passes of 8 bits over 2 x ceil(log2(k+1)) key bits, every entry a key
-    passes = [((2 * max(1, int(np.ceil(np.log2(k + 2))))) + 7) // 8 for k in ks]
-    sort_bytes = sum(16 * E * p for p in passes) / steps
     return {
         "ms_per_iteration": round(ms, 3), "gteps": round(E / (ms / 1000) / 1e9, 2),
         "modularity_scores": [round(x, 6) for x in scores], "communities": ks,
-        "phase_ms_per_iteration": {k: round(v[0] / steps, 3) for k, v in timers.items() if k.startswith("merge")},
-        "sort_passes": passes, "sort_gbs": round(sort_bytes / (sort_ms / 1000) / 1e9, 1) if sort_ms else None,
-        "sort_hbm_frac": round(sort_bytes / (sort_ms / 1000) / 1e9 / HBM_PEAK_GBS, 3) if sort_ms else None,
+        "kernel_ms_per_iteration": {k: round(v[0] / steps, 3) for k, v in sorted(timers.items()) if v[0] > 0},
     }
+
+
+def merge_leg(parcd, np, ctx, n, m, init, steps, warmup, seed):
+    """Parallel merging (-s 1, one round) on R-MAT-22 from the singleton
+    partition: compress + statistics + metric + merging step per iteration,
+    the same counting as the propagation iterations.
+
+    The partner choice counts over community rows (the default): the
+    vertices sorted by label (members), every entry's label gathered into its
+    community's row (merge_keys: 4 bytes of adj, a 4-byte label gather and a
+    4-byte write per entry), and one round of the propagation kernels in
+    their merging form over the rows (the lp_* slots).  The sorted-pairs
+    form (community-pair keys, LSD radix sort of every entry, run-length
+    reduce, one pass over the unique pairs) runs the same iterations beside
+    it: both must give the same modularity trajectory."""
+    E = 2 * m
+    out = _merge_iterations(parcd, np, ctx, E, init, steps, warmup, seed, 0)
+    kt = out["kernel_ms_per_iteration"]
+    if kt.get("merge_keys"):
+        out["row_build_gbs"] = round(12 * E / (kt["merge_keys"] / 1000) / 1e9, 1)
+        out["row_build_hbm_frac"] = round(out["row_build_gbs"] / HBM_PEAK_GBS, 3)
+    pairs = _merge_iterations(parcd, np, ctx, E, init, steps, warmup, seed, 1)
+    pt = pairs["kernel_ms_per_iteration"]
+    sort_ms = pt.get("merge_sort", 0.0)
+    # LSD passes of 8 bits over 2 x ceil(log2(k+1)) key bits, every entry a key
+    passes = [((2 * max(1, int(np.ceil(np.log2(k + 2))))) + 7) // 8 for k in pairs["communities"]]
+    sort_bytes = sum(16 * E * p for p in passes) / steps
+    pairs["sort_passes"] = passes
+    pairs["sort_gbs"] = round(sort_bytes / (sort_ms / 1000) / 1e9, 1) if sort_ms else None
+    pairs["sort_hbm_frac"] = round(sort_bytes / (sort_ms / 1000) / 1e9 / HBM_PEAK_GBS, 3) if sort_ms else None
+    out["sorted_pairs"] = pairs
+    out["same_trajectory_as_sorted_pairs"] = (pairs["modularity_scores"] == out["modularity_scores"] and
+                                              pairs["communities"] == out["communities"])
+    out["speedup_vs_sorted_pairs"] = round(pairs["ms_per_iteration"] / out["ms_per_iteration"], 2)
+    return out
 
 
 def wcc_leg(parcd, np, steps, warmup):

@@ -169,6 +169,7 @@ extern "C" int pcd_ctx_create(int device, pcd_ctx **out) {
 
 static void free_graph_state(pcd_ctx *ctx) {
     loop_release(ctx);  // its device buffers go with the others below
+    merge_release(ctx);
     // everything except the per-context scalars allocated at creation
     void *keep[] = {ctx->d_k, ctx->d_acc, ctx->d_dacc, ctx->d_result, ctx->d_counts, ctx->d_bin_counters};
     std::vector<void *> all = ctx->allocations;
@@ -222,6 +223,7 @@ extern "C" int pcd_ctx_destroy(pcd_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     loop_release(ctx);
+    merge_release(ctx);
     for (void *p : ctx->allocations) (void)hipFree(p);
     ctx->allocations.clear();
     for (int i = 0; i < PCD_PHASE_COUNT; i++) {
@@ -255,6 +257,50 @@ static int ceil_log2_host(int64_t x) {
     while (((int64_t)1 << b) < x) b++;
     return b;
 }
+
+namespace pcd {
+// Chunks of HUB_CHUNK entries and P label partitions per HUB-bin row; the
+// partitioned-entry buffer holds the rows one after another, and each row
+// has a fallback global table of nextpow2(2 entries) slots.
+int build_hub_layout(const std::vector<int4> &rows, int part_entries, HubLayoutHost &L) {
+    const size_t nh = rows.size();
+    L.chunks.clear();
+    L.parts.clear();
+    L.where.clear();
+    L.chunk_off.clear();
+    L.info.assign(nh, make_int4(0, 0, 0, 0));
+    L.buf_base.assign(nh + 1, 0);
+    L.toff.assign(nh + 1, 0);
+    L.tlog.assign(nh + 1, 0);
+    int64_t poff_total = 0;
+    for (size_t h = 0; h < nh; h++) {
+        const int row = rows[h].x, b = rows[h].y, e = rows[h].z;
+        const int64_t d = (int64_t)e - b;
+        const int P = (int)std::min<int64_t>(HUB_MAX_PARTS,
+                                             std::max<int64_t>(1, (d + part_entries - 1) / part_entries));
+        const int first_chunk = (int)L.chunks.size();
+        for (int b0 = b; b0 < e; b0 += HUB_CHUNK) {
+            L.chunks.push_back(make_int4((int)h, row, b0, (int)std::min<int64_t>(e, (int64_t)b0 + HUB_CHUNK)));
+            L.chunk_off.push_back((int32_t)poff_total);
+            poff_total += P + 1;
+        }
+        const int nch = (int)L.chunks.size() - first_chunk;
+        L.info[h] = make_int4(row, P, (int)L.parts.size(), first_chunk);
+        for (int q = 0; q < P; q++) {
+            // a partition's chunk offsets sit P + 1 apart from the first
+            // chunk's; the buffer base fits int32 (hub entries < 2^31)
+            L.parts.push_back(make_int4((int)h, q, first_chunk, nch));
+            L.where.push_back(make_int4(row, P, nch > 0 ? L.chunk_off[first_chunk] : 0, (int)L.buf_base[h]));
+        }
+        L.buf_base[h + 1] = L.buf_base[h] + d;
+        L.tlog[h] = std::max(6, ceil_log2_host(2 * d));
+        L.toff[h + 1] = L.toff[h] + ((int64_t)1 << L.tlog[h]);
+        if (poff_total > INT32_MAX || L.buf_base[h + 1] > INT32_MAX) return fail(PCD_E_INVALID, "hub layout too large");
+    }
+    L.poff_total = poff_total;
+    return PCD_OK;
+}
+}  // namespace pcd
 
 // graph.hpp:6-40 layout; builds degree bins and static work items.
 extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indices, const int32_t *adj) {
@@ -323,35 +369,17 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
     }
     // hub layout: chunks of HUB_CHUNK entries, P label partitions per hub
     const int hub_end = ctx->bins.start[BIN_HUB + 1];
-    std::vector<int4> hub_chunks, hub_parts, hub_where, hub_info(hub_end);
-    std::vector<int32_t> chunk_off;
-    std::vector<int64_t> buf_base(hub_end + 1, 0), toff(hub_end + 1, 0);
-    std::vector<int32_t> tlog(hub_end + 1, 0);
-    int64_t poff_total = 0;
+    std::vector<int4> hub_rows(hub_end);
     for (int h = 0; h < hub_end; h++) {
-        int v = sorted ? h : order[h];
-        int d = deg[v];
-        int P = (int)std::min<int64_t>(HUB_MAX_PARTS, std::max<int64_t>(1, ((int64_t)d + ctx->tune.hub_part_entries - 1) /
-                                                                          ctx->tune.hub_part_entries));
-        int first_chunk = (int)hub_chunks.size();
-        for (int b0 = indices[v]; b0 < indices[v + 1]; b0 += HUB_CHUNK) {
-            hub_chunks.push_back(make_int4(h, v, b0, std::min(indices[v + 1], b0 + HUB_CHUNK)));
-            chunk_off.push_back((int32_t)poff_total);
-            poff_total += P + 1;
-        }
-        int nch = (int)hub_chunks.size() - first_chunk;
-        hub_info[h] = make_int4(v, P, (int)hub_parts.size(), first_chunk);
-        for (int q = 0; q < P; q++) {
-            hub_parts.push_back(make_int4(h, q, first_chunk, nch));
-            // a partition's chunk offsets sit P + 1 apart from the first
-            // chunk's; the buffer base fits int32 (hub entries < E)
-            hub_where.push_back(make_int4(v, P, chunk_off[first_chunk], (int)buf_base[h]));
-        }
-        buf_base[h + 1] = buf_base[h] + d;
-        tlog[h] = std::max(6, ceil_log2_host(2 * (int64_t)d));
-        toff[h + 1] = toff[h] + ((int64_t)1 << tlog[h]);
+        const int v = sorted ? h : order[h];
+        hub_rows[h] = make_int4(v, indices[v], indices[v + 1], 0);
     }
-    if (poff_total > INT32_MAX) return fail(PCD_E_INVALID, "hub layout too large");
+    HubLayoutHost hl;
+    PCD_TRY(build_hub_layout(hub_rows, ctx->tune.hub_part_entries, hl));
+    const std::vector<int4> &hub_chunks = hl.chunks, &hub_parts = hl.parts, &hub_where = hl.where, &hub_info = hl.info;
+    const std::vector<int32_t> &chunk_off = hl.chunk_off, &tlog = hl.tlog;
+    const std::vector<int64_t> &buf_base = hl.buf_base, &toff = hl.toff;
+    const int64_t poff_total = hl.poff_total;
     ctx->num_hubs = hub_end;
     ctx->num_hub_items = (int)hub_chunks.size();
     ctx->num_hub_parts = (int)hub_parts.size();
@@ -798,6 +826,8 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.cluster_rounds = value;
     } else if (k == "onesweep") {
         ctx->tune.onesweep = value != 0;
+    } else if (k == "merge_sort") {
+        ctx->tune.merge_sort = value != 0;
     } else if (k == "graph_loop") {
         ctx->tune.graph_loop = value != 0;
     } else if (k == "graph_loop_min_entries") {

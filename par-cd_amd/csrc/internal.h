@@ -43,6 +43,31 @@ struct BinLayout {
     int split2048;            // first position of degree <= 2048 (inside B4096)
 };
 
+// The rows one propagation launch counts: their positions in bin order (a
+// bin is a position range), each bin's entries, and the layout of the rows
+// in the HUB bin -- fixed-size chunks, label-hash partitions, the
+// partitioned-entry buffer, the partial results and the fallback global
+// tables.  Either the graph's vertices (fields of the context) or, in a
+// merging step, its communities.
+struct RowSet {
+    BinLayout bins{};
+    int64_t bin_entries[NUM_BINS]{};
+    int num_hubs = 0, num_hub_items = 0, num_hub_parts = 0;
+    const int4 *hub_items = nullptr;       // chunk: {hub, row, begin, end}
+    const int32_t *hub_item_off = nullptr; // chunk -> offset of its (P+1) partition offsets
+    const int4 *hub_part_items = nullptr;  // {hub, part, first chunk, number of chunks}
+    const int4 *hub_info = nullptr;        // per hub: {row, P, first part item, first chunk}
+    const int4 *hub_part_where = nullptr;  // per partition item: {row, P, poff of the first chunk, buffer base}
+    const int64_t *hub_buf_base = nullptr; // per hub: offset into hub_buf
+    int32_t *hub_poff = nullptr;
+    int32_t *hub_buf = nullptr;
+    void *hub_partial = nullptr;
+    int32_t *hub_over = nullptr;
+    const int64_t *hub_table_off = nullptr;  // per hub: fallback global table
+    const int32_t *hub_table_log = nullptr;
+    int32_t *hub_keys = nullptr, *hub_cnts = nullptr;
+};
+
 // Tunables (the autotuner rewrites these per graph shape).
 struct Tuning {
     int hub_chunk = 4096;        // entries per hub partition work item
@@ -68,11 +93,15 @@ struct Tuning {
                                  // 4096-slot table (measured slower on R-MAT-22: off)
     int cluster_rounds = 128;    // device rounds of the clustering partition before the
                                  // remaining dependency chains are finished on the host
-    int onesweep = 1;            // radix sorts: one launch per pass with decoupled look-back (1)
-                                 // or tile histograms, a scan and a scatter per pass (0)
+    int onesweep = 0;            // radix sorts: one launch per pass with decoupled look-back (1)
+                                 // or tile histograms, a scan and a scatter per pass (0).  Off:
+                                 // its scatter costs what the three-launch pass's does, and its
+                                 // all-digit histogram is slower on keys with long equal runs
     int graph_loop = 1;          // modularity / CPM with -s 0: passes replayed from one captured
                                  // pass with the stop rule on the device (pcd_iterate_many)
     int graph_loop_min_entries = 0;  // graphs with fewer CSR entries run pass by pass
+    int merge_sort = 0;          // merging picks partners from the sorted unique community pairs
+                                 // (1) or counts over community rows, no sort of the entries (0)
 };
 constexpr int STATS_CACHE = 16384;
 constexpr int HUB_MAX_PARTS = 1024;
@@ -96,6 +125,16 @@ struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
 };
+
+// Host-built layout of HUB-bin rows (see RowSet), from each row's {id,
+// first entry, end entry}.
+struct HubLayoutHost {
+    std::vector<int4> chunks, parts, where, info;
+    std::vector<int32_t> chunk_off, tlog;
+    std::vector<int64_t> buf_base, toff;  // per row, plus the total at the end
+    int64_t poff_total = 0;
+};
+int build_hub_layout(const std::vector<int4> &rows, int part_entries, HubLayoutHost &L);
 
 struct pcd_ctx_impl;
 
@@ -197,6 +236,7 @@ struct pcd_ctx {
     uint64_t *d_best_ord = nullptr, *d_best_key = nullptr;
     void *d_mcand = nullptr;         // merge candidates of runs crossing waves
     size_t mcand_cap = 0;
+    void *mrows = nullptr;           // community rows of the merging step (merge.hip)
 
     // measurement
     bool timing = false;
@@ -245,6 +285,9 @@ int compress_labels_dev(pcd_ctx *ctx, bool defer_k = false);
 void resolve_k(pcd_ctx *ctx);
 // drops the captured pass of the device-resident loop (loop.hip)
 void loop_release(pcd_ctx *ctx);
+// drops the merging step's row state (merge.hip); its device buffers belong
+// to the context's allocation list
+void merge_release(pcd_ctx *ctx);
 int collect_statistics_dev(pcd_ctx *ctx);
 int statistics_delta_dev(pcd_ctx *ctx, const int32_t *before, const int32_t *after);
 int clustering_labels_dev(pcd_ctx *ctx, int *num_communities);

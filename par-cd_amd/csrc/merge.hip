@@ -10,11 +10,23 @@
 //
 // Merging round: each unmatched, eligible community l takes the neighbour
 // community k (unmatched) of best score_merge >= 0 (refinement.cpp:240-260;
-// a zero score is accepted, as there) using two order-independent
-// atomics per l (max of the order-preserving score bits, then min of
-// (tie key, k)); handshake (refinement.cpp:267-274); relabel to
-// min(l, partner) (refinement.cpp:277-286).
-#include "prims.h"
+// a zero score is accepted, as there); handshake (refinement.cpp:267-274);
+// relabel to min(l, partner) (refinement.cpp:277-286).
+//
+// Two forms of the partner choice:
+//  * community rows (the default): as refinement.cpp:222-238 states it,
+//    each community counts the labels of its members' neighbours.  Once per
+//    step every entry's label is gathered into its community's row (rows in
+//    label order, members by a stable radix sort of the vertices by label),
+//    and each round runs the propagation kernels in their merging form over
+//    the rows, binned by length.  No sort of the entries.
+//  * sorted pairs (tunable merge_sort): the contraction above, then one pass
+//    over the unique pairs with two order-independent atomics per l (max of
+//    the order-preserving score bits, then min of (tie key, k)).
+// Both choose the same partners: (score, tie key, label) order is total.
+#include <type_traits>
+
+#include "work.h"
 
 namespace pcd {
 
@@ -318,6 +330,298 @@ __global__ void k_merge_relabel(int n, const int32_t *labels, const int32_t *fou
     }
 }
 
+// ------------------------------------------------------ community rows
+__global__ void k_member_keys(int n, const int32_t *labels, uint64_t *keys) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        keys[v] = ((uint64_t)(uint32_t)v << 32) | (uint32_t)labels[v];
+}
+
+// Community l's row starts at the volume (int_volume + ext_volume, the sum
+// of its members' degrees) of the communities below l; an empty community
+// has an empty row.
+struct CommVol {
+    const CommStats *st;
+    __device__ int operator()(int64_t l) const { return st[l].int_vol + st[l].ext_vol; }
+};
+struct CommStart {
+    int32_t *cind;
+    int64_t k;
+    __device__ void operator()(int64_t l, int ex, int vol) const {
+        cind[l] = ex;
+        if (l == k - 1) cind[k] = ex + vol;
+    }
+};
+
+// Member i of the label-sorted vertex list: an exclusive scan of the
+// members' degrees in that order puts each member's entries at its place in
+// its community's row.
+struct MemberDeg {
+    const uint64_t *members;
+    const int32_t *indices;
+    __device__ int operator()(int64_t i) const {
+        const int v = (int)(members[i] >> 32);
+        return indices[v + 1] - indices[v];
+    }
+};
+struct MemberDst {
+    const uint64_t *members;
+    int32_t *dst;
+    __device__ void operator()(int64_t i, int ex, int) const { dst[(int)(members[i] >> 32)] = ex; }
+};
+
+// every entry's label, written into the row of its vertex's community
+template <int G>
+__global__ __launch_bounds__(256) void k_member_rows_small(int p0, int p1, const int32_t *order, const int32_t *indices,
+                                                           const int32_t *adj, const int32_t *labels,
+                                                           const int32_t *dst, int32_t *ent) {
+    const int lane = __lane_id(), glane = lane & (G - 1);
+    const int per_wave = WAVE / G;
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    const int count = p1 - p0;
+    for (int base = wave_g * per_wave; base < count; base += nwaves * per_wave) {
+        const int gi = base + lane / G;
+        if (gi >= count) continue;
+        const int p = p0 + gi;
+        const int v = order ? order[p] : p;
+        const int start = indices[v], deg = indices[v + 1] - start, o = dst[v];
+        for (int j = glane; j < deg; j += G) ent[o + j] = labels[adj[start + j]];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_member_rows_items(int nitems, const int4 *items, const int32_t *indices,
+                                                           const int32_t *adj, const int32_t *labels,
+                                                           const int32_t *dst, int32_t *ent) {
+    const int lane = __lane_id();
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    for (int it = wave_g; it < nitems; it += nwaves) {
+        const int4 w = items[it];
+        const int o = dst[w.x] - indices[w.x];
+        for (int j = w.y + lane; j < w.z; j += WAVE) ent[o + j] = labels[adj[j]];
+    }
+}
+
+// bin of a row of `len` entries (the vertex bins' degree bounds)
+__device__ inline int row_bin(int len) {
+    if (len > 4096) return BIN_HUB;
+    if (len > 1024) return BIN_B4096;
+    if (len > 512) return BIN_B1024;
+    if (len > 256) return BIN_B512;
+    if (len > 64) return BIN_W256;
+    if (len > 32) return BIN_G64;
+    if (len > 16) return BIN_G32;
+    if (len > 8) return BIN_G16;
+    if (len > 4) return BIN_G8;
+    return BIN_G4;
+}
+
+// per community: its bin as the sort key (community id above), and the
+// rows and entries of each bin
+__global__ __launch_bounds__(256) void k_comm_bins(int k, const int32_t *cind, uint64_t *keys,
+                                                   unsigned long long *bcount) {
+    __shared__ unsigned long long rows[NUM_BINS], ents[NUM_BINS];
+    if (threadIdx.x < NUM_BINS) {
+        rows[threadIdx.x] = 0;
+        ents[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) {
+        const int len = cind[l + 1] - cind[l];
+        const int b = row_bin(len);
+        keys[l] = ((uint64_t)(uint32_t)l << 32) | (uint32_t)b;
+        atomicAdd(&rows[b], 1ULL);
+        atomicAdd(&ents[b], (unsigned long long)len);
+    }
+    __syncthreads();
+    if (threadIdx.x < NUM_BINS) {
+        if (rows[threadIdx.x]) atomicAdd(&bcount[threadIdx.x], rows[threadIdx.x]);
+        if (ents[threadIdx.x]) atomicAdd(&bcount[NUM_BINS + threadIdx.x], ents[threadIdx.x]);
+    }
+}
+
+__global__ void k_high_words(int k, const uint64_t *keys, int32_t *out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x)
+        out[i] = (int32_t)(keys[i] >> 32);
+}
+
+// the HUB-bin rows {community, first entry, end entry}
+__global__ void k_hub_rows(int nh, const int32_t *corder, const int32_t *cind, int4 *out) {
+    for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
+        const int l = corder[h];
+        out[h] = make_int4(l, cind[l], cind[l + 1], 0);
+    }
+}
+
+// A grow-only device buffer in the context's allocation list.
+template <typename T>
+struct GrowBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    // holds at least `need` elements afterwards (contents not kept); *grew
+    // tells whether it was reallocated
+    int reserve(pcd_ctx *ctx, size_t need, bool *grew = nullptr) {
+        if (grew) *grew = false;
+        if (p && cap >= need) return PCD_OK;
+        if (p) dev_free(ctx, p);
+        p = nullptr;
+        cap = 0;
+        const size_t c = need + need / 8 + 16;
+        PCD_TRY(dev_alloc_t(ctx, &p, c));
+        cap = c;
+        if (grew) *grew = true;
+        return PCD_OK;
+    }
+    int upload(pcd_ctx *ctx, const std::vector<T> &v) {
+        PCD_TRY(reserve(ctx, v.size()));
+        if (!v.empty())
+            PCD_HIP(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, ctx->stream));
+        return PCD_OK;
+    }
+};
+
+// Device state of the community rows and the row set of the current step.
+struct MergeRows {
+    GrowBuf<int32_t> ent, cind, corder, dst;
+    GrowBuf<uint64_t> mk0, mk1;
+    GrowBuf<unsigned long long> bcount;
+    GrowBuf<int4> hrows;
+    // hub layout
+    GrowBuf<int4> items, parts, info, where;
+    GrowBuf<int32_t> item_off, tlog, poff, hbuf, over;
+    GrowBuf<int64_t> buf_base, toff;
+    GrowBuf<char> partial;
+    GrowBuf<int32_t> gkeys, gcnts;  // fallback tables: all slots empty between uses
+    RowSet rows;
+};
+
+void merge_release(pcd_ctx *ctx) {
+    delete static_cast<MergeRows *>(ctx->mrows);
+    ctx->mrows = nullptr;
+}
+
+// Builds the community rows of the current compressed labels (k
+// communities): entries, offsets, bin order and hub layout.
+static int build_community_rows(pcd_ctx *ctx, MergeRows &R) {
+    const int n = ctx->n, k = ctx->k;
+    const int64_t E = (int64_t)2 * ctx->m;
+    PCD_TRY(R.ent.reserve(ctx, (size_t)E + 1));
+    PCD_TRY(R.cind.reserve(ctx, (size_t)n + 2));
+    PCD_TRY(R.corder.reserve(ctx, (size_t)n + 2));
+    PCD_TRY(R.dst.reserve(ctx, (size_t)n + 2));
+    PCD_TRY(R.mk0.reserve(ctx, (size_t)n + 2));
+    PCD_TRY(R.mk1.reserve(ctx, (size_t)n + 2));
+    PCD_TRY(R.bcount.reserve(ctx, 2 * NUM_BINS));
+    const int half = label_bits(k);
+    // members by label (stable: ascending vertex id inside a community)
+    kt_begin(ctx, KS_MERGE_SORT);
+    hipLaunchKernelGGL(k_member_keys, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_labels,
+                       R.mk0.p);
+    uint64_t *members = nullptr;
+    PCD_TRY(radix_sort_u64(ctx, R.mk0.p, R.mk1.p, n, half, &members));
+    PCD_TRY(device_scan(ctx, n, MemberDeg{members, ctx->d_indices}, MemberDst{members, R.dst.p}, nullptr));
+    PCD_TRY(device_scan(ctx, k, CommVol{ctx->d_stats}, CommStart{R.cind.p, k}, nullptr));
+    kt_end(ctx, KS_MERGE_SORT);
+    // the rows: every entry's label at its member's place
+    kt_begin(ctx, KS_MERGE_KEYS);
+    if (ctx->num_stat_items > 0)
+        hipLaunchKernelGGL(k_member_rows_items, dim3(grid_for(ctx, ctx->num_stat_items, 4, 16)), dim3(256), 0,
+                           ctx->stream, ctx->num_stat_items, ctx->d_stat_items, ctx->d_indices, ctx->d_adj,
+                           ctx->d_labels, R.dst.p, R.ent.p);
+    const BinLayout &vb = ctx->bins;
+    auto small = [&](auto g, int bin) {
+        constexpr int G = decltype(g)::value;
+        const int p0 = vb.start[bin], p1 = vb.start[bin + 1];
+        if (p1 > p0)
+            hipLaunchKernelGGL(k_member_rows_small<G>, dim3(grid_for(ctx, p1 - p0, 256 / G, 16)), dim3(256), 0,
+                               ctx->stream, p0, p1, ctx->d_order, ctx->d_indices, ctx->d_adj, ctx->d_labels, R.dst.p,
+                               R.ent.p);
+    };
+    small(std::integral_constant<int, 64>{}, BIN_G64);
+    small(std::integral_constant<int, 32>{}, BIN_G32);
+    small(std::integral_constant<int, 16>{}, BIN_G16);
+    small(std::integral_constant<int, 8>{}, BIN_G8);
+    small(std::integral_constant<int, 4>{}, BIN_G4);
+    kt_end(ctx, KS_MERGE_KEYS);
+    PCD_HIP(hipGetLastError());
+    // bins of the community rows by length: one radix pass over 4-bit bins
+    kt_begin(ctx, KS_MERGE_SORT);
+    PCD_TRY(fill_u64(ctx, (uint64_t *)R.bcount.p, 0, 2 * NUM_BINS));
+    hipLaunchKernelGGL(k_comm_bins, dim3(grid_for(ctx, k, 256, 4)), dim3(256), 0, ctx->stream, k, R.cind.p, R.mk0.p,
+                       R.bcount.p);
+    uint64_t *cs = nullptr;
+    PCD_TRY(radix_sort_u64(ctx, R.mk0.p, R.mk1.p, k, 4, &cs));
+    hipLaunchKernelGGL(k_high_words, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, cs, R.corder.p);
+    kt_end(ctx, KS_MERGE_SORT);
+    PCD_HIP(hipGetLastError());
+    PCD_HIP(hipMemcpyAsync(ctx->h_small64, R.bcount.p, sizeof(int64_t) * 2 * NUM_BINS, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    RowSet &rs = R.rows;
+    rs = RowSet{};
+    int64_t at = 0;
+    for (int b = 0; b < NUM_BINS; b++) {
+        rs.bins.start[b] = (int)at;
+        at += ctx->h_small64[b];
+        rs.bin_entries[b] = ctx->h_small64[NUM_BINS + b];
+    }
+    rs.bins.start[NUM_BINS] = (int)at;
+    rs.bins.split2048 = rs.bins.start[BIN_B4096 + 1];
+    if (at != k) return fail(PCD_E_STATE, "community bins do not cover the k communities");
+    // rows above 4096 entries: the hub layout, built on the host
+    const int nh = rs.bins.start[BIN_HUB + 1];
+    if (nh > 0) {
+        PCD_TRY(R.hrows.reserve(ctx, (size_t)nh));
+        hipLaunchKernelGGL(k_hub_rows, dim3(grid_for(ctx, nh, 256)), dim3(256), 0, ctx->stream, nh, R.corder.p,
+                           R.cind.p, R.hrows.p);
+        PCD_HIP(hipGetLastError());
+        std::vector<int4> hr(nh);
+        PCD_HIP(hipMemcpyAsync(hr.data(), R.hrows.p, sizeof(int4) * nh, hipMemcpyDeviceToHost, ctx->stream));
+        PCD_HIP(hipStreamSynchronize(ctx->stream));
+        HubLayoutHost L;
+        PCD_TRY(build_hub_layout(hr, ctx->tune.hub_part_entries, L));
+        PCD_TRY(R.items.upload(ctx, L.chunks));
+        PCD_TRY(R.item_off.upload(ctx, L.chunk_off));
+        PCD_TRY(R.parts.upload(ctx, L.parts));
+        PCD_TRY(R.where.upload(ctx, L.where));
+        PCD_TRY(R.info.upload(ctx, L.info));
+        PCD_TRY(R.buf_base.upload(ctx, L.buf_base));
+        PCD_TRY(R.toff.upload(ctx, L.toff));
+        PCD_TRY(R.tlog.upload(ctx, L.tlog));
+        PCD_TRY(R.partial.reserve(ctx, (size_t)HUB_PARTIAL_BYTES * L.parts.size()));
+        PCD_TRY(R.poff.reserve(ctx, (size_t)L.poff_total));
+        PCD_TRY(R.hbuf.reserve(ctx, (size_t)L.buf_base[nh]));
+        PCD_TRY(R.over.reserve(ctx, (size_t)nh));
+        PCD_TRY(fill_i32(ctx, R.over.p, 0, nh));
+        const size_t gslots = (size_t)L.toff[nh];
+        bool grew_k = false, grew_c = false;
+        PCD_TRY(R.gkeys.reserve(ctx, gslots, &grew_k));
+        PCD_TRY(R.gcnts.reserve(ctx, gslots, &grew_c));
+        if (grew_k) PCD_TRY(fill_i32(ctx, R.gkeys.p, EMPTY_KEY, (int64_t)R.gkeys.cap));
+        if (grew_c) PCD_TRY(fill_i32(ctx, R.gcnts.p, 0, (int64_t)R.gcnts.cap));
+        // the host vectors must outlive their copies
+        PCD_HIP(hipStreamSynchronize(ctx->stream));
+        rs.num_hubs = nh;
+        rs.num_hub_items = (int)L.chunks.size();
+        rs.num_hub_parts = (int)L.parts.size();
+        rs.hub_items = R.items.p;
+        rs.hub_item_off = R.item_off.p;
+        rs.hub_part_items = R.parts.p;
+        rs.hub_info = R.info.p;
+        rs.hub_part_where = R.where.p;
+        rs.hub_buf_base = R.buf_base.p;
+        rs.hub_poff = R.poff.p;
+        rs.hub_buf = R.hbuf.p;
+        rs.hub_partial = R.partial.p;
+        rs.hub_over = R.over.p;
+        rs.hub_table_off = R.toff.p;
+        rs.hub_table_log = R.tlog.p;
+        rs.hub_keys = R.gkeys.p;
+        rs.hub_cnts = R.gcnts.p;
+    }
+    return PCD_OK;
+}
+
 static int ensure_merge_state(pcd_ctx *ctx) {
     // sized for n communities at most (k <= n)
     if (ctx->d_bp) return PCD_OK;
@@ -328,7 +632,61 @@ static int ensure_merge_state(pcd_ctx *ctx) {
     return PCD_OK;
 }
 
-int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie) {
+// refinement.cpp:277-286: every vertex of a matched community with a partner
+// takes min(l, partner)
+static int merge_relabel(pcd_ctx *ctx, int k) {
+    hipLaunchKernelGGL(k_merge_relabel, dim3(grid_for(ctx, ctx->n, 256)), dim3(256), 0, ctx->stream, ctx->n,
+                       ctx->d_labels, ctx->d_found, ctx->d_bp, ctx->d_labels_alt);
+    PCD_HIP(hipGetLastError());
+    std::swap(ctx->d_labels, ctx->d_labels_alt);
+    ctx->labels_compressed = false;
+    ctx->has_stats = false;
+    ctx->stats_pending = false;
+    ctx->label_bound = k;
+    return PCD_OK;
+}
+
+// the partner choice over community rows (see the top of this file)
+static int merging_rows_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie) {
+    const int k = ctx->k;
+    if (!ctx->mrows) ctx->mrows = new MergeRows();
+    MergeRows &R = *static_cast<MergeRows *>(ctx->mrows);
+    PCD_TRY(build_community_rows(ctx, R));
+    PCD_TRY(ensure_merge_state(ctx));
+    PCD_TRY(fill_i32(ctx, ctx->d_bp, -1, k));
+    PCD_TRY(fill_i32(ctx, ctx->d_found, 0, k));
+    LPArgs a{};
+    a.indices = R.cind.p;  // community l's row: entries [cind[l], cind[l+1]) of ent
+    a.adj = R.ent.p;
+    a.order = R.corder.p;
+    a.stats = ctx->d_stats;
+    a.new_labels = ctx->d_bp;
+    a.found = ctx->d_found;
+    a.rule = make_rule(ctx, rule);
+    a.step = step;
+    a.tie = tie;
+    int32_t *unmatched = ctx->d_counts + NUM_BINS;
+    for (int rnd = 0; rnd < rounds; rnd++) {
+        a.seed = merge_seed(seed, rnd);
+        PCD_TRY(merge_rows_launch(ctx, R.rows, a));
+        kt_begin(ctx, KS_MERGE_ROUNDS);
+        PCD_TRY(fill_i32(ctx, unmatched, 0, 1));
+        hipLaunchKernelGGL(k_handshake, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_found,
+                           ctx->d_bp, unmatched);
+        kt_end(ctx, KS_MERGE_ROUNDS);
+        PCD_HIP(hipGetLastError());
+        if (rnd + 1 < rounds) {
+            // refinement.cpp:220,264-274: stop once every community is matched
+            PCD_HIP(hipMemcpyAsync(ctx->h_small, unmatched, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+            PCD_HIP(hipStreamSynchronize(ctx->stream));
+            if (ctx->h_small[0] == 0) break;
+        }
+    }
+    return merge_relabel(ctx, k);
+}
+
+// the partner choice over the sorted unique community pairs
+static int merging_pairs_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie) {
     const int k = ctx->k;
     int64_t U = 0;
     int half = 0;
@@ -385,15 +743,12 @@ int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step
             if (ctx->h_small[0] == 0) break;
         }
     }
-    hipLaunchKernelGGL(k_merge_relabel, dim3(grid_for(ctx, ctx->n, 256)), dim3(256), 0, ctx->stream, ctx->n,
-                       ctx->d_labels, ctx->d_found, ctx->d_bp, ctx->d_labels_alt);
-    PCD_HIP(hipGetLastError());
-    std::swap(ctx->d_labels, ctx->d_labels_alt);
-    ctx->labels_compressed = false;
-    ctx->has_stats = false;
-    ctx->stats_pending = false;
-    ctx->label_bound = k;
-    return PCD_OK;
+    return merge_relabel(ctx, k);
+}
+
+int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie) {
+    if (ctx->tune.merge_sort) return merging_pairs_dev(ctx, rule, rounds, seed, step, tie);
+    return merging_rows_dev(ctx, rule, rounds, seed, step, tie);
 }
 
 int contract_dev(pcd_ctx *ctx, int64_t *num_pairs) {

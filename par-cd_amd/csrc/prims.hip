@@ -353,7 +353,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_os_scatter(const uint64_t *keys,
 
 // Sorts n keys whose significant bits are [0, bits).  Ping-pongs between
 // keys and tmp; *out receives the buffer holding the result.  Uses the
-// one-sweep passes (tunable "onesweep", on by default) when n < 2^30.
+// one-sweep passes when tuned so (tunable "onesweep") and n < 2^30.
 int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int bits, uint64_t **out) {
     *out = keys;
     if (n <= 1) return PCD_OK;

@@ -85,6 +85,60 @@ __device__ inline CommStats comm_of(const LPArgs &a, int label, int aux) {
 constexpr unsigned RK_REC = 1u << 8;
 constexpr unsigned RK_BATCH = RK_MOD | RK_CPM | RK_REC;
 
+// Builds for the merging step (MergingRefinement::refine,
+// refinement.cpp:185-288): a row is a community l, its entries are the
+// labels of its members' neighbours (gathered into the row once per step),
+// its own label is l, a candidate is another community not yet matched,
+// scored by score_merge and accepted from 0 up, and the row's result is its
+// best partner or -1.  They read the community records.
+constexpr unsigned RK_MERGE = 1u << 9;
+constexpr unsigned RK_MERGE_REC = RK_MERGE | RK_REC;
+
+template <unsigned KM>
+__device__ inline bool lp_active(const LPArgs &a, int v) {
+    if constexpr ((KM & RK_MERGE) != 0) {
+        // refinement.cpp:224-226: matched communities, and those whose chosen
+        // partner is still unmatched, sit the round out
+        if (a.found[v]) return false;
+        const int b = a.new_labels[v];
+        return b == -1 || a.found[b] != 0;
+    } else {
+        return lp_selected(a, v);
+    }
+}
+
+// label of a neighbour entry (merging rows hold labels already) and of the row
+template <unsigned KM>
+__device__ inline int entry_label(const LPArgs &a, int nb) {
+    if constexpr ((KM & RK_MERGE) != 0) return nb;
+    else return a.labels[nb];
+}
+template <unsigned KM>
+__device__ inline int own_label(const LPArgs &a, int v) {
+    if constexpr ((KM & RK_MERGE) != 0) return v;
+    else return a.labels[v];
+}
+
+// Score of a candidate label other than the row's own, and whether it is a
+// candidate at all: score_join > 0 (refinement.cpp:152-166), or for a
+// merging row score_merge >= 0 (refinement.cpp:247-260: a zero score ties
+// with "no partner" and is taken).  A merging caller has already dropped
+// matched communities.
+template <unsigned KM>
+__device__ inline bool cand_score(const LPArgs &a, int deg, const CommStats &own_rec, const CommStats &rec, int cnt,
+                                  double &sc) {
+    if constexpr ((KM & RK_MERGE) != 0) {
+        sc = score_merge(a.rule, own_rec, rec, cnt);
+        return sc >= 0.0;
+    } else {
+        sc = score_join<KM>(a.rule, deg, rec, cnt);
+        return sc > 0;
+    }
+}
+
+// merging row's result (refinement.cpp:262): the best partner, or -1
+__device__ inline int merge_partner(const Cand &c) { return c.label != INT32_MAX ? c.label : -1; }
+
 // Community record of `label` as the gains read it.  Builds for modularity
 // or CPM alone read the 4-byte per-community total the step built (one
 // random read into a 4x smaller array than the records); the others read
@@ -175,7 +229,7 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
         int p = q0 + lane;
         bool valid = lane < CP && p < q1;
         int v = valid ? vertex_at(a, p) : 0;
-        bool act = valid && lp_selected(a, v);
+        bool act = valid && lp_active<KM>(a, v);
         uint64_t am = __ballot(act);
         int nact = __popcll(am);
         if (act) list[__popcll(am & lt)] = v;
@@ -194,7 +248,7 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
                 pown = o.x;
                 paux = o.y;
             } else {
-                pown = a.labels[pu];
+                pown = own_label<KM>(a, pu);
                 pst = comm_rec<KM>(a, pown);
             }
         }
@@ -213,7 +267,7 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
             int2 r = make_int2(INT32_MAX, 0);
             if (has && glane < deg) {
                 int nb = a.adj[start + glane];
-                r = AUX ? a.lv[nb] : make_int2(a.labels[nb], 0);
+                r = AUX ? a.lv[nb] : make_int2(entry_label<KM>(a, nb), 0);
             }
             int x = r.x, xa = r.y;
 #pragma unroll
@@ -241,10 +295,11 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
                 int cnt = min(nxt, gend) - lane;
                 if (x == own.x) {
                     own_cnt = cnt;
-                } else {
+                } else if ((KM & RK_MERGE) == 0 || !a.found[x]) {
                     uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), u);
-                    double sc = score_join<KM>(a.rule, deg, AUX ? CommStats{xa, xa, 0, 0} : comm_rec<KM>(a, x), cnt);
-                    if (sc > 0) c = Cand{sc, tie_key(a.tie, tb, x), x};
+                    double sc;
+                    if (cand_score<KM>(a, deg, own_rec, AUX ? CommStats{xa, xa, 0, 0} : comm_rec<KM>(a, x), cnt, sc))
+                        c = Cand{sc, tie_key(a.tie, tb, x), x};
                 }
             }
 #pragma unroll
@@ -253,9 +308,11 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
                 if (cand_better(o, c)) c = o;
                 own_cnt = max(own_cnt, __shfl_xor(own_cnt, d, WAVE));
             }
-            if (has && glane == 0)
-                a.new_labels[u] = AUX ? lp_decide<KM>(a, deg, own.x, own.y, own_cnt, c)
-                                      : lp_decide_rec<KM>(a, deg, own.x, own_rec, own_cnt, c);
+            if (has && glane == 0) {
+                if constexpr ((KM & RK_MERGE) != 0) a.new_labels[u] = merge_partner(c);
+                else a.new_labels[u] = AUX ? lp_decide<KM>(a, deg, own.x, own.y, own_cnt, c)
+                                           : lp_decide_rec<KM>(a, deg, own.x, own_rec, own_cnt, c);
+            }
             if (a.counters) {
                 // every lane votes: a wave holds up to 64 / G active vertices
                 const int act = __popcll(__ballot(has && glane == 0));
@@ -361,11 +418,13 @@ __device__ inline void occ_append(bool claimed, int slot, uint16_t *occ, int *no
 // Gathers the labels (with AUX the {label, aux} records: one 8-byte gather
 // brings what the gains read, and aux lands in the table) of row entries
 // base + lo + e * stride, PF at a time (independent loads in flight), and
-// inserts them; claimed slots go to the slot list.  The loop bounds are the
-// same for every lane of the workgroup.
-template <int PF, bool AUX = false, class CT = int>
+// inserts them; claimed slots go to the slot list.  A merging row (its
+// entries are labels) leaves out its own label, which is never a
+// candidate.  The loop bounds are the same for every lane of the workgroup.
+template <int PF, unsigned KM, bool AUX = false, class CT = int>
 __device__ inline void insert_labels(const LPArgs &a, int *keys, CT *cnts, int *auxs, uint16_t *occ, int *nocc,
-                                     int mask, int log_t, int start, int deg, int lo, int stride) {
+                                     int mask, int log_t, int start, int deg, int lo, int stride, int own) {
+    constexpr bool MERGE = (KM & RK_MERGE) != 0;
     for (int base = 0; base < deg; base += PF * stride) {
         int nb[PF];
 #pragma unroll
@@ -376,12 +435,13 @@ __device__ inline void insert_labels(const LPArgs &a, int *keys, CT *cnts, int *
         int2 rec[PF];
 #pragma unroll
         for (int e = 0; e < PF; e++)
-            rec[e] = nb[e] < 0 ? make_int2(EMPTY_KEY, 0) : (AUX ? a.lv[nb[e]] : make_int2(a.labels[nb[e]], 0));
+            rec[e] = nb[e] < 0 ? make_int2(EMPTY_KEY, 0)
+                               : (AUX ? a.lv[nb[e]] : make_int2(entry_label<KM>(a, nb[e]), 0));
 #pragma unroll
         for (int e = 0; e < PF; e++) {
             bool claimed = false;
             int slot = 0;
-            if (rec[e].x != EMPTY_KEY)
+            if (rec[e].x != EMPTY_KEY && (!MERGE || rec[e].x != own))
                 claimed = pinsert<AUX, CT>(keys, cnts, auxs, mask, log_t, rec[e].x, rec[e].y, slot);
             occ_append(claimed, slot, occ, nocc);
         }
@@ -394,6 +454,9 @@ __device__ inline void insert_labels(const LPArgs &a, int *keys, CT *cnts, int *
 template <bool AUX, unsigned KM, class CT = int>
 __device__ inline void scan_listed(const LPArgs &a, int *keys, CT *cnts, const int *auxs, const uint16_t *occ, int D,
                                    int lo, int stride, int deg, int own, uint64_t tb, Cand &c, int &own_cnt) {
+    constexpr bool MERGE = (KM & RK_MERGE) != 0;
+    CommStats orec{0, 0, 0, 0};
+    if constexpr (MERGE) orec = a.stats[own];
     for (int i0 = lo; i0 < D; i0 += SB * stride) {
         int key[SB], cnt[SB];
         CommStats st[SB];
@@ -414,6 +477,11 @@ __device__ inline void scan_listed(const LPArgs &a, int *keys, CT *cnts, const i
                 cnts[h] = 0;
             }
         }
+        if constexpr (MERGE) {  // matched communities are no candidates
+#pragma unroll
+            for (int e = 0; e < SB; e++)
+                if (key[e] != EMPTY_KEY && a.found[key[e]]) key[e] = EMPTY_KEY;
+        }
         if (!AUX) {
 #pragma unroll
             for (int e = 0; e < SB; e++)
@@ -425,8 +493,8 @@ __device__ inline void scan_listed(const LPArgs &a, int *keys, CT *cnts, const i
             if (key[e] == own) {
                 own_cnt = cnt[e];
             } else {
-                double sc = score_join<KM>(a.rule, deg, st[e], cnt[e]);
-                if (sc > 0) {
+                double sc;
+                if (cand_score<KM>(a, deg, orec, st[e], cnt[e], sc)) {
                     Cand o{sc, tie_key(a.tie, tb, key[e]), key[e]};
                     if (cand_better(o, c)) c = o;
                 }
@@ -458,12 +526,13 @@ __device__ inline void lp_wave(const Src &src, int wave_g, int nwaves, int *keys
       if (!src.unit(u, a, q0, q1)) continue;
       for (int p = q0; p < q1; p++) {
         int v = vertex_at(a, p);
-        if (!lp_selected(a, v)) continue;
+        if (!lp_active<KM>(a, v)) continue;
         int start = a.indices[v], deg = a.indices[v + 1] - start;
-        int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
+        int2 ownr = AUX ? a.lv[v] : make_int2(own_label<KM>(a, v), 0);
         int own = ownr.x;
         int log_t = max(6, ceil_log2(2 * deg));
-        insert_labels<PF, AUX, CT>(a, keys, cnts, auxs, occ, nocc, (1 << log_t) - 1, log_t, start, deg, lane, WAVE);
+        insert_labels<PF, KM, AUX, CT>(a, keys, cnts, auxs, occ, nocc, (1 << log_t) - 1, log_t, start, deg, lane, WAVE,
+                                       own);
         wave_sync();
         const int D = *nocc;
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), v);
@@ -472,8 +541,9 @@ __device__ inline void lp_wave(const Src &src, int wave_g, int nwaves, int *keys
         scan_listed<AUX, KM, CT>(a, keys, cnts, auxs, occ, D, lane, WAVE, deg, own, tb, c, own_cnt);
         wave_reduce_cand(c, own_cnt);
         if (lane == 0) {
-            a.new_labels[v] = AUX ? lp_decide<KM>(a, deg, own, ownr.y, own_cnt, c)
-                                  : lp_decide_stats<KM>(a, deg, own, own_cnt, c);
+            if constexpr ((KM & RK_MERGE) != 0) a.new_labels[v] = merge_partner(c);
+            else a.new_labels[v] = AUX ? lp_decide<KM>(a, deg, own, ownr.y, own_cnt, c)
+                                       : lp_decide_stats<KM>(a, deg, own, own_cnt, c);
             if (a.counters) count3(a, 1, deg, D);
             *nocc = 0;
         }
@@ -559,13 +629,13 @@ __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
       if (!src.unit(u, a, q0, q1)) continue;
       for (int p = q0; p < q1; p++) {
         int v = vertex_at(a, p);
-        if (!lp_selected(a, v)) continue;
+        if (!lp_active<KM>(a, v)) continue;
         int start = a.indices[v], deg = a.indices[v + 1] - start;
-        int2 ownr = AUX ? a.lv[v] : make_int2(a.labels[v], 0);
+        int2 ownr = AUX ? a.lv[v] : make_int2(own_label<KM>(a, v), 0);
         int own = ownr.x;
         int log_t = max(6, ceil_log2(2 * deg));
-        insert_labels<PF, AUX>(a, keys, cnts, auxs, occ, &s_nocc, (1 << log_t) - 1, log_t, start, deg, threadIdx.x,
-                              NT);
+        insert_labels<PF, KM, AUX>(a, keys, cnts, auxs, occ, &s_nocc, (1 << log_t) - 1, log_t, start, deg, threadIdx.x,
+                                  NT, own);
         __syncthreads();
         const int D = s_nocc;
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), v);
@@ -574,8 +644,9 @@ __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
         scan_listed<AUX, KM>(a, keys, cnts, auxs, occ, D, threadIdx.x, NT, deg, own, tb, c, own_cnt);
         block_reduce_cand<NT>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
-            a.new_labels[v] = AUX ? lp_decide<KM>(a, deg, own, ownr.y, own_cnt, c)
-                                  : lp_decide_stats<KM>(a, deg, own, own_cnt, c);
+            if constexpr ((KM & RK_MERGE) != 0) a.new_labels[v] = merge_partner(c);
+            else a.new_labels[v] = AUX ? lp_decide<KM>(a, deg, own, ownr.y, own_cnt, c)
+                                       : lp_decide_stats<KM>(a, deg, own, own_cnt, c);
             if (a.counters) count3(a, 1, deg, D);
             s_nocc = 0;
         }
@@ -618,16 +689,17 @@ constexpr int HUB1_NT = 512, HUB1_PER = HUB_CHUNK / HUB1_NT;
 static_assert(HUB1_NT * HUB1_PER == HUB_CHUNK, "hub chunk layout");
 
 // pass 1: gather the chunk's labels (with AUX its {label, aux} records) and
-// group them by partition
-template <bool AUX>
+// group them by partition; a merging row leaves its own label out
+template <unsigned KM, bool AUX>
 __global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) {
+    constexpr bool MERGE = (KM & RK_MERGE) != 0;
     using Rec = typename std::conditional<AUX, int2, int32_t>::type;
     Rec *out = reinterpret_cast<Rec *>(h.buf);
     __shared__ int cnt[HUB_MAX_PARTS + 1];
     for (int it = blockIdx.x; it < h.nchunks; it += gridDim.x) {
         const int4 ck = h.chunks[it];
         const int v = ck.y;
-        if (!lp_selected(a, v)) continue;
+        if (!lp_active<KM>(a, v)) continue;
         const int P = h.info[ck.x].y;
         for (int i = threadIdx.x; i <= P; i += HUB1_NT) cnt[i] = 0;
         __syncthreads();
@@ -641,7 +713,7 @@ __global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) 
 #pragma unroll
         for (int e = 0; e < HUB1_PER; e++) {
             if constexpr (AUX) rec[e] = nb[e] >= 0 ? a.lv[nb[e]] : make_int2(0, 0);
-            else rec[e] = nb[e] >= 0 ? a.labels[nb[e]] : 0;
+            else rec[e] = nb[e] >= 0 ? entry_label<KM>(a, nb[e]) : 0;
         }
 #pragma unroll
         for (int e = 0; e < HUB1_PER; e++) {
@@ -650,8 +722,10 @@ __global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) 
                 int key;
                 if constexpr (AUX) key = rec[e].x;
                 else key = rec[e];
-                part[e] = hub_part_of(key, P);
-                rank[e] = atomicAdd(&cnt[part[e]], 1);
+                if (!MERGE || key != v) {
+                    part[e] = hub_part_of(key, P);
+                    rank[e] = atomicAdd(&cnt[part[e]], 1);
+                }
             }
         }
         __syncthreads();
@@ -742,7 +816,7 @@ __global__ __launch_bounds__(NT) void k_hub_count_listed(LPArgs a, HubArgs h) {
         int4 pi_n = pi, pw_n = pw;
         if (nxt < h.nparts) { pi_n = h.parts[nxt]; pw_n = h.where[nxt]; }
         const int v = pw.x, P = pw.y, part = pi.y;
-        if (lp_selected(a, v)) {
+        if (lp_active<KM>(a, v)) {
             for (int c0 = 0; c0 < pi.w; c0 += WAVE) {
                 const int nc = min(WAVE, pi.w - c0);
                 int len = 0, b0 = 0;
@@ -791,7 +865,7 @@ __global__ __launch_bounds__(NT) void k_hub_count_listed(LPArgs a, HubArgs h) {
             __syncthreads();
             const int D = s_nocc;
             const bool over = s_over != 0;
-            const int deg = a.indices[v + 1] - a.indices[v], own = a.labels[v];
+            const int deg = a.indices[v + 1] - a.indices[v], own = own_label<KM>(a, v);
             Cand c = cand_none();
             int own_cnt = 0, distinct = threadIdx.x == 0 ? D : 0;
             if (over) {
@@ -832,7 +906,7 @@ __global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
     for (int hub = wave_g; hub < h.nhubs; hub += nwaves) {
         const int4 inf = h.info[hub];
         const int v = inf.x, P = inf.y;
-        if (!lp_selected(a, v)) continue;
+        if (!lp_active<KM>(a, v)) continue;
         if (h.over[hub]) continue;  // recounted by the global-table path
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
@@ -846,7 +920,9 @@ __global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
         distinct = wave_sum(distinct);
         if (lane == 0) {
             const int deg = a.indices[v + 1] - a.indices[v];
-            if constexpr (AUX) {
+            if constexpr ((KM & RK_MERGE) != 0) {
+                a.new_labels[v] = merge_partner(c);
+            } else if constexpr (AUX) {
                 int2 own = a.lv[v];
                 a.new_labels[v] = lp_decide<KM>(a, deg, own.x, own.y, own_cnt, c);
             } else {
@@ -862,6 +938,7 @@ __global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
 // records are read from the statistics.
 __device__ inline uint32_t gslot(int key, int log_t) { return ((uint32_t)key * 0x9E3779B1u) >> (32 - log_t); }
 
+template <unsigned KM>
 __global__ __launch_bounds__(512) void k_hub_global_insert(LPArgs a, HubArgs h, const int64_t *toff,
                                                            const int32_t *tlog, int32_t *gkeys, int32_t *gcnts) {
     for (int it = blockIdx.x; it < h.nchunks; it += gridDim.x) {
@@ -870,7 +947,7 @@ __global__ __launch_bounds__(512) void k_hub_global_insert(LPArgs a, HubArgs h, 
         const int64_t off = toff[ck.x];
         const int glog = tlog[ck.x], gmask = (1 << glog) - 1;
         for (int j = ck.z + threadIdx.x; j < ck.w; j += blockDim.x) {
-            int key = a.labels[a.adj[j]];
+            int key = entry_label<KM>(a, a.adj[j]);
             uint32_t s = gslot(key, glog);
             while (true) {
                 int old = atomicCAS(&gkeys[off + s], EMPTY_KEY, key);
@@ -892,7 +969,9 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
     for (int hub = blockIdx.x; hub < h.nhubs; hub += gridDim.x) {
         if (!h.over[hub]) continue;
         const int v = h.info[hub].x;
-        const int deg = a.indices[v + 1] - a.indices[v], own = a.labels[v];
+        const int deg = a.indices[v + 1] - a.indices[v], own = own_label<KM>(a, v);
+        CommStats orec{0, 0, 0, 0};
+        if constexpr ((KM & RK_MERGE) != 0) orec = a.stats[own];
         const int64_t off = toff[hub];
         const int T = 1 << tlog[hub];
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), v);
@@ -906,9 +985,9 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
             gcnts[off + s] = 0;
             distinct++;
             if (key == own) own_cnt = cnt;
-            else {
-                double sc = score_join<KM>(a.rule, deg, comm_rec<KM>(a, key), cnt);
-                if (sc > 0) {
+            else if ((KM & RK_MERGE) == 0 || !a.found[key]) {
+                double sc;
+                if (cand_score<KM>(a, deg, orec, comm_rec<KM>(a, key), cnt, sc)) {
                     Cand o{sc, tie_key(a.tie, tb, key), key};
                     if (cand_better(o, c)) c = o;
                 }
@@ -916,10 +995,14 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
         }
         block_reduce_cand<1024>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
-            double leave = score_leave<KM>(a.rule, deg, comm_rec<KM>(a, own), own_cnt);
-            double bs = c.label != INT32_MAX ? c.s : 0.0;
-            bs += leave;
-            a.new_labels[v] = (bs > 0 && bs > leave) ? c.label : (leave > 0 ? PCD_NEW_LABEL : own);
+            if constexpr ((KM & RK_MERGE) != 0) {
+                a.new_labels[v] = merge_partner(c);
+            } else {
+                double leave = score_leave<KM>(a.rule, deg, comm_rec<KM>(a, own), own_cnt);
+                double bs = c.label != INT32_MAX ? c.s : 0.0;
+                bs += leave;
+                a.new_labels[v] = (bs > 0 && bs > leave) ? c.label : (leave > 0 ? PCD_NEW_LABEL : own);
+            }
             if (a.counters) count3(a, 1, deg, distinct);
             h.over[hub] = 0;
         }
@@ -927,13 +1010,40 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
 }
 
 // ------------------------------------------------------------- driver
-// Launches the propagation kernels of one step, compiled for the rule kinds KM.
+RowSet graph_rows(pcd_ctx *ctx) {
+    RowSet r;
+    r.bins = ctx->bins;
+    for (int b = 0; b < NUM_BINS; b++) r.bin_entries[b] = ctx->bin_entries[b];
+    r.num_hubs = ctx->num_hubs;
+    r.num_hub_items = ctx->num_hub_items;
+    r.num_hub_parts = ctx->num_hub_parts;
+    r.hub_items = ctx->d_hub_items;
+    r.hub_item_off = ctx->d_hub_item_off;
+    r.hub_part_items = ctx->d_hub_part_items;
+    r.hub_info = ctx->d_hub_info;
+    r.hub_part_where = ctx->d_hub_part_where;
+    r.hub_buf_base = ctx->d_hub_buf_base;
+    r.hub_poff = ctx->d_hub_poff;
+    r.hub_buf = ctx->d_hub_buf;
+    r.hub_partial = ctx->d_hub_partial;
+    r.hub_over = ctx->d_hub_over;
+    r.hub_table_off = ctx->d_hub_table_off;
+    r.hub_table_log = ctx->d_hub_table_log;
+    r.hub_keys = ctx->d_hub_keys;
+    r.hub_cnts = ctx->d_hub_cnts;
+    return r;
+}
+
+// Launches the propagation kernels of one step over the rows `rs`, compiled
+// for the rule kinds KM (with RK_MERGE: one merging round over community
+// rows, which writes each active community's partner).
 template <unsigned KM>
-static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
-    const BinLayout &b = ctx->bins;
+static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
+    constexpr bool MERGE = (KM & RK_MERGE) != 0;
+    const BinLayout &b = rs.bins;
     auto args_for = [&](int bin) {
         LPArgs x = a;
-        if (ctx->counters) x.counters = ctx->d_bin_counters + 3 * bin;
+        if (ctx->counters && !MERGE) x.counters = ctx->d_bin_counters + 3 * bin;
         return x;
     };
     auto range = [&](int bin, int &p0, int &p1) {
@@ -943,16 +1053,19 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
     };
     // {label, aux} records are read by the hub path and, when tuned so, by
     // the small bins; rules without aux read the statistics instead
-    // inactive vertices keep their label: copy all, the kernels overwrite the active ones
-    PCD_HIP(hipMemcpyAsync(a.new_labels, a.labels, sizeof(int32_t) * (size_t)ctx->n, hipMemcpyDeviceToDevice,
-                           ctx->stream));
+    // inactive vertices keep their label: copy all, the kernels overwrite the
+    // active ones (a merging round writes the partners of active communities
+    // into the partner array, which keeps the others)
+    if (!MERGE)
+        PCD_HIP(hipMemcpyAsync(a.new_labels, a.labels, sizeof(int32_t) * (size_t)ctx->n, hipMemcpyDeviceToDevice,
+                               ctx->stream));
     const bool small_aux = a.use_aux && ctx->tune.small_aux;
     const bool block_aux = a.use_aux && ctx->tune.block_aux;
     // hubs count {label, aux} records only when tuned so (and the rule's
     // gains read aux alone); otherwise labels, with the community record read
     // per distinct label
     const bool hub_aux = a.use_aux && ctx->tune.hub_aux;
-    if (a.use_aux && ((ctx->num_hubs > 0 && hub_aux) || small_aux || block_aux)) {
+    if (a.use_aux && ((rs.num_hubs > 0 && hub_aux) || small_aux || block_aux)) {
         kt_begin(ctx, KS_LP_SELECT);
         hipLaunchKernelGGL(k_lp_records, dim3(grid_for(ctx, ctx->n, 256, 8)), dim3(256), 0, ctx->stream, ctx->n,
                            ctx->d_labels, ctx->d_stats, a.rule.kind, ctx->d_lv);
@@ -967,31 +1080,31 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
         kt_end(ctx, KS_LP_SELECT);
     }
     int p0, p1;
-    if (ctx->num_hubs > 0) {
+    if (rs.num_hubs > 0) {
         HubArgs h;
-        h.chunks = ctx->d_hub_items;
-        h.chunk_off = ctx->d_hub_item_off;
-        h.parts = ctx->d_hub_part_items;
-        h.info = ctx->d_hub_info;
-        h.where = ctx->d_hub_part_where;
-        h.buf_base = ctx->d_hub_buf_base;
-        h.poff = ctx->d_hub_poff;
-        h.buf = (int2 *)ctx->d_hub_buf;
-        h.partial = (HubPartial *)ctx->d_hub_partial;
-        h.over = ctx->d_hub_over;
+        h.chunks = rs.hub_items;
+        h.chunk_off = rs.hub_item_off;
+        h.parts = rs.hub_part_items;
+        h.info = rs.hub_info;
+        h.where = rs.hub_part_where;
+        h.buf_base = rs.hub_buf_base;
+        h.poff = rs.hub_poff;
+        h.buf = (int2 *)rs.hub_buf;
+        h.partial = (HubPartial *)rs.hub_partial;
+        h.over = rs.hub_over;
         h.over_count = ctx->d_counts + NUM_BINS + 2;
-        h.nchunks = ctx->num_hub_items;
-        h.nparts = ctx->num_hub_parts;
-        h.nhubs = ctx->num_hubs;
+        h.nchunks = rs.num_hub_items;
+        h.nparts = rs.num_hub_parts;
+        h.nhubs = rs.num_hubs;
         // the LDS table holds HUB_TB slots: never index past it
         h.table_slots = std::min(HUB_TB, std::max(64, ctx->tune.hub_table_slots & ~63));
         LPArgs ah = args_for(BIN_HUB);
         kt_begin(ctx, KS_LP_HUB_INSERT);
         if (hub_aux)
-            hipLaunchKernelGGL(k_hub_partition<true>, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
+            hipLaunchKernelGGL((k_hub_partition<KM, true>), dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
                                ctx->stream, ah, h);
         else
-            hipLaunchKernelGGL(k_hub_partition<false>, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
+            hipLaunchKernelGGL((k_hub_partition<KM, false>), dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
                                ctx->stream, ah, h);
         kt_end(ctx, KS_LP_HUB_INSERT);
         kt_begin(ctx, KS_LP_HUB_DECIDE);
@@ -1020,17 +1133,17 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
             hipLaunchKernelGGL((k_hub_final<KM, false>), dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0,
                                ctx->stream, ah, h);
         }
-        hipLaunchKernelGGL(k_hub_global_insert, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(512), 0, ctx->stream, ah,
-                           h, ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts);
+        hipLaunchKernelGGL(k_hub_global_insert<KM>, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(512), 0, ctx->stream,
+                           ah, h, rs.hub_table_off, rs.hub_table_log, rs.hub_keys, rs.hub_cnts);
         hipLaunchKernelGGL(k_hub_global_decide<KM>, dim3(grid_for(ctx, h.nhubs, 1, 1)), dim3(1024), 0, ctx->stream, ah,
-                           h, ctx->d_hub_table_off, ctx->d_hub_table_log, ctx->d_hub_keys, ctx->d_hub_cnts);
+                           h, rs.hub_table_off, rs.hub_table_log, rs.hub_keys, rs.hub_cnts);
         kt_end(ctx, KS_LP_HUB_FINAL);
     }
     if (range(BIN_B4096, p0, p1)) {
         // degrees 2049-4096: 512-thread workgroups with 8192-slot tables (two
         // per CU; a 12-byte-slot table of 8192 entries would leave one);
         // 1025-2048: 256-thread workgroups with 4096-slot tables (four per CU)
-        const int ps = ctx->tune.b4096_split ? std::min(std::max(ctx->bins.split2048, p0), p1) : p1;
+        const int ps = ctx->tune.b4096_split && !MERGE ? std::min(std::max(b.split2048, p0), p1) : p1;
         kt_begin(ctx, KS_LP_B4096);
         if (ps > p0)
             hipLaunchKernelGGL((k_lp_block<512, 8192, KM, RangeSrc>), dim3(grid_for(ctx, ps - p0, 1, 2)), dim3(512),
@@ -1103,7 +1216,7 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
         for (int i = 0; i < 6; i++) {
             range(BIN_W256 + i, p0, p1);
             sb.src[i] = RangeSrc{p0, p1, cps[i], args_for(BIN_W256 + i)};
-            tot += ctx->bin_entries[BIN_W256 + i];
+            tot += rs.bin_entries[BIN_W256 + i];
             nonempty += p1 > p0;
         }
         if (nonempty > 0) {
@@ -1114,7 +1227,7 @@ static int launch_propagation(pcd_ctx *ctx, const LPArgs &a) {
             for (int i = 0; i < 6; i++) {
                 sb.lo[i] = at;
                 if (sb.src[i].p1 > sb.src[i].p0) {
-                    const int64_t e = ctx->bin_entries[BIN_W256 + i];
+                    const int64_t e = rs.bin_entries[BIN_W256 + i];
                     const int before = tot > 0 ? (int)(spare * acc / tot) : 0;
                     acc += e;
                     const int after = tot > 0 ? (int)(spare * acc / tot) : 0;
@@ -1166,12 +1279,18 @@ int propagation_launch(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, 
     a.tot = a.use_aux ? ctx->d_tot : nullptr;
     a.counters = nullptr;
     a.step_ptr = ctx->graph_step;  // set while a pass is captured for replay
+    a.found = nullptr;
     // kernels specialised for modularity or CPM hold fewer registers (more
     // waves per SIMD); WCC and conductance use the general build
-    if (rule.kind == PCD_RULE_MODULARITY) PCD_TRY(launch_propagation<RK_MOD>(ctx, a));
-    else if (rule.kind == PCD_RULE_CPM) PCD_TRY(launch_propagation<RK_CPM>(ctx, a));
-    else PCD_TRY(launch_propagation<RK_ALL>(ctx, a));
+    const RowSet rs = graph_rows(ctx);
+    if (rule.kind == PCD_RULE_MODULARITY) PCD_TRY(launch_propagation<RK_MOD>(ctx, rs, a));
+    else if (rule.kind == PCD_RULE_CPM) PCD_TRY(launch_propagation<RK_CPM>(ctx, rs, a));
+    else PCD_TRY(launch_propagation<RK_ALL>(ctx, rs, a));
     return PCD_OK;
+}
+
+int merge_rows_launch(pcd_ctx *ctx, const RowSet &rows, const LPArgs &a) {
+    return launch_propagation<RK_MERGE_REC>(ctx, rows, a);
 }
 
 int propagation_commit(pcd_ctx *ctx) {

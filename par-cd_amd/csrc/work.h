@@ -28,6 +28,7 @@ struct LPArgs {
     int use_aux;  // the rule's gains read only aux
     unsigned long long *counters;  // {active, entries, distinct} of the bin, or nullptr
     const int *step_ptr;  // the step number in device memory (replayed graphs), or nullptr: `step`
+    const int32_t *found;  // merging rows: per community "matched" (refinement.cpp:206), else unused
 };
 
 __device__ inline int vertex_at(const LPArgs &a, int p) { return a.order ? a.order[p] : p; }
@@ -139,6 +140,13 @@ struct BatchLPLaunch {
 
 // launches the propagation kernels of a batch round (propagate.hip)
 int batch_propagation_launch(hipStream_t stream, const BatchLPLaunch &l);
+
+// the graph's vertices as rows of the propagation kernels (propagate.hip)
+RowSet graph_rows(pcd_ctx *ctx);
+// One merging round over community rows (propagate.hip): a.indices are the
+// rows' offsets into a.adj (labels), a.order the bin order of `rows`,
+// a.new_labels the partner array, a.found the matched flags.
+int merge_rows_launch(pcd_ctx *ctx, const RowSet &rows, const LPArgs &a);
 
 }  // namespace pcd
 

@@ -385,39 +385,90 @@ def test_wcc_metric(ctx):
             assert got == pytest.approx(want, rel=1e-9, abs=1e-12)
 
 
-def test_contraction_pairs(ctx):
-    cases = [GOLDEN["golden_sbm200"], GOLDEN["golden_rmat9"], GG.bin_zoo(), GG.rmat(13)]
+@pytest.mark.parametrize("onesweep", [0, 1])
+def test_contraction_pairs(ctx, onesweep):
+    """Both radix-sort forms: three launches per pass, and one launch per
+    pass with decoupled look-back (R-MAT-16: about 500 tiles per pass)."""
+    cases = [GOLDEN["golden_sbm200"], GOLDEN["golden_rmat9"], GG.bin_zoo(), GG.rmat(13), GG.rmat(16)]
     rng = np.random.default_rng(9)
-    for g in cases:
-        n, m, ind, adj = g[:4]
-        for lab in [GG.singleton_labels(n), rng.integers(0, max(2, n // 30), n)]:
-            lab_o, so, ivo, evo = check_partition(ctx, g, lab)
-            l, k, w = ctx.contract()
-            lo, ko, wo = O.community_pairs(n, ind, adj, lab_o, len(so))
-            assert np.array_equal(l, lo) and np.array_equal(k, ko) and np.array_equal(w, wo)
+    ctx.set_tuning("onesweep", onesweep)
+    try:
+        for g in cases:
+            n, m, ind, adj = g[:4]
+            for lab in [GG.singleton_labels(n), rng.integers(0, max(2, n // 30), n)]:
+                lab_o, so, ivo, evo = check_partition(ctx, g, lab)
+                l, k, w = ctx.contract()
+                lo, ko, wo = O.community_pairs(n, ind, adj, lab_o, len(so))
+                assert np.array_equal(l, lo) and np.array_equal(k, ko) and np.array_equal(w, wo)
+    finally:
+        ctx.set_tuning("onesweep", 0)
 
 
+@pytest.mark.parametrize("merge_sort", [0, 1])
 @pytest.mark.parametrize("tie", [O.TIE_RANDOM, O.TIE_LOWEST])
 @pytest.mark.parametrize("kind,lam", RULES + [(O.WCC, 0.0)])
-def test_merging(ctx, kind, lam, tie):
+def test_merging(ctx, kind, lam, tie, merge_sort):
+    """Both partner choices: counting over community rows (merge_sort 0)
+    and the sorted unique community pairs (1)."""
     cases = [GOLDEN["golden_sbm200"], GOLDEN["golden_rmat9"], GOLDEN["golden_starclique"], GG.bin_zoo()]
     rng = np.random.default_rng(10)
-    for g in cases:
-        n, m, ind, adj = g[:4]
-        cc = 0.0
-        if kind == O.WCC:
-            upload(ctx, g)
-            cc = ctx.triangles()[2]
-        for lab in [GG.singleton_labels(n), rng.integers(0, max(2, n // 5), n)]:
-            for rounds in (1, 3):
-                lab_o, so, ivo, evo = check_partition(ctx, g, lab)
-                if kind == O.WCC:
-                    ctx.set_cc(cc)
-                ctx.merge(kind, lam, rounds=rounds, seed=77, step=2, tie=tie)
-                got = ctx.labels()
-                want, bp, fp = O.merging_refine(kind, lam, n, m, ind, adj, lab_o, so, ivo, evo, cc, rounds, 1, 77, 2,
-                                                tie)
-                assert np.array_equal(got, want)
+    ctx.set_tuning("merge_sort", merge_sort)
+    try:
+        for g in cases:
+            n, m, ind, adj = g[:4]
+            cc = 0.0
+            if kind == O.WCC:
+                upload(ctx, g)
+                cc = ctx.triangles()[2]
+            for lab in [GG.singleton_labels(n), rng.integers(0, max(2, n // 5), n)]:
+                for rounds in (1, 3):
+                    lab_o, so, ivo, evo = check_partition(ctx, g, lab)
+                    if kind == O.WCC:
+                        ctx.set_cc(cc)
+                    ctx.merge(kind, lam, rounds=rounds, seed=77, step=2, tie=tie)
+                    got = ctx.labels()
+                    want, bp, fp = O.merging_refine(kind, lam, n, m, ind, adj, lab_o, so, ivo, evo, cc, rounds, 1, 77,
+                                                    2, tie)
+                    assert np.array_equal(got, want)
+    finally:
+        ctx.set_tuning("merge_sort", 0)
+
+
+@pytest.mark.parametrize("slots,part_entries", [(4096, 2048), (64, 100000), (1024, 512)])
+def test_merging_community_rows_hub_paths(ctx, slots, part_entries):
+    """Community rows above 4096 entries go through the hub path (chunks,
+    label partitions, LDS counting) and, when a partition overflows its
+    table (forced by tiny tables or huge partitions), the global-table
+    fallback.  Partitions of 40 communities make every row a hub row; of
+    3,000 communities, rows in every bin; uncompressed labels leave
+    communities empty."""
+    g = GG.bin_zoo()
+    n, m, ind, adj = g[:4]
+    rng = np.random.default_rng(12)
+    try:
+        ctx.set_tuning("hub_part_entries", part_entries)
+        ctx.set_tuning("hub_table_slots", slots)
+        for lab in [GG.singleton_labels(n), rng.integers(0, 40, n), rng.integers(0, 3000, n)]:
+            for kind, lam in [(O.MOD, 0.0), (O.CPM, 0.02)]:
+                for rounds in (1, 2):
+                    lab_o, so, ivo, evo = check_partition(ctx, g, lab)
+                    ctx.merge(kind, lam, rounds=rounds, seed=13, step=1, tie=O.TIE_RANDOM)
+                    want = O.merging_refine(kind, lam, n, m, ind, adj, lab_o, so, ivo, evo, 0.0, rounds, 1, 13, 1,
+                                            O.TIE_RANDOM)[0]
+                    assert np.array_equal(ctx.labels(), want)
+        # statistics of labels that skip ids (k = max label + 1, empty communities)
+        upload(ctx, g)
+        lab = 2 * rng.integers(0, 500, n)
+        ctx.upload_labels(lab)
+        k = ctx.statistics()
+        assert k == int(lab.max()) + 1
+        _, so, ivo, evo = O.collect_statistics(n, ind, adj, lab)
+        ctx.merge(O.MOD, 0.0, rounds=1, seed=14, step=0, tie=O.TIE_LOWEST)
+        want = O.merging_refine(O.MOD, 0.0, n, m, ind, adj, lab, so, ivo, evo, 0.0, 1, 1, 14, 0, O.TIE_LOWEST)[0]
+        assert np.array_equal(ctx.labels(), want)
+    finally:
+        ctx.set_tuning("hub_part_entries", 2048)
+        ctx.set_tuning("hub_table_slots", 4096)
 
 
 def gpu_run(ctx, g, kind, lam, strategy, max_iter, seed, tie, frac=0.5, rounds=1, init=None):
