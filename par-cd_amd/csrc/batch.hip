@@ -1210,6 +1210,26 @@ static void slot_fill(const BinMap &mp, const Slot &sl, int s, bool lp_only, Wor
 
 static size_t list_esz(int) { return sizeof(WorkItem); }
 
+// Host-side check of the work-item list layout: every list's region lies
+// inside the item buffer, after the previous list's, and holds its items;
+// the host counts match the lengths; the tile list fits the tile-scan
+// buffers.  Run after every rebuild and append, so that a kernel of the
+// captured round can only read items that were written.
+static int check_lists(const pcd_batch *b) {
+    size_t end = 0;
+    for (int l = 0; l < NUM_LISTS; l++) {
+        const size_t off = b->list_off[l], bytes = (size_t)b->list_cap[l] * list_esz(l);
+        if (b->list_len[l] < 0 || b->list_len[l] > b->list_cap[l] || b->h_counts[l] != b->list_len[l] ||
+            off < end || off + bytes > b->items_cap)
+            return bfail(PCD_E_STATE, "work item list " + std::to_string(l) + " out of its region (len " +
+                                          std::to_string(b->list_len[l]) + ", cap " + std::to_string(b->list_cap[l]) +
+                                          ", offset " + std::to_string(off) + ", buffer " + std::to_string(b->items_cap) + ")");
+        end = off + bytes;
+    }
+    if ((size_t)b->list_cap[L_TILES] > b->tile_cap) return bfail(PCD_E_STATE, "tile list exceeds the tile-scan buffers");
+    return PCD_OK;
+}
+
 // Rebuilds the work-item lists from the running graphs and uploads them:
 // one counting pass fixes the layout (every list with room to grow, so
 // later admissions append in place), a second writes the items straight
@@ -1313,7 +1333,7 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
     }
     // offsets inside the item buffer moved: the captured round is stale
     drop_graph(b);
-    return PCD_OK;
+    return check_lists(b);
 }
 
 // Appends the items of the graphs admitted since the last rebuild behind
@@ -1350,7 +1370,7 @@ static int append_items(pcd_batch *b) {
     PCD_HIP(hipMemcpyAsync(b->d_counts, b->h_counts, sizeof(int32_t) * NUM_LISTS, hipMemcpyHostToDevice, b->stream));
     PCD_HIP(hipStreamSynchronize(b->stream));
     b->pending_append.clear();
-    return PCD_OK;
+    return check_lists(b);
 }
 
 template <class T>
@@ -1427,6 +1447,35 @@ static int launch_round(pcd_batch *b) {
     return PCD_OK;
 }
 
+// Host-side check of a captured round before it is instantiated: only
+// kernel nodes (no memset or copy nodes), each with a kernel, a grid of at
+// most 64 workgroups per CU, 64-1024 threads per workgroup and no dynamic
+// LDS (every batch kernel declares its LDS statically).
+static int check_graph(pcd_batch *b, hipGraph_t g) {
+    size_t nn = 0;
+    PCD_HIP(hipGraphGetNodes(g, nullptr, &nn));
+    std::vector<hipGraphNode_t> nodes(nn);
+    if (nn) PCD_HIP(hipGraphGetNodes(g, nodes.data(), &nn));
+    for (size_t i = 0; i < nn; i++) {
+        hipGraphNodeType t;
+        PCD_HIP(hipGraphNodeGetType(nodes[i], &t));
+        if (t != hipGraphNodeTypeKernel)
+            return bfail(PCD_E_STATE, "captured round holds a node of type " + std::to_string((int)t));
+        hipKernelNodeParams kp;
+        std::memset(&kp, 0, sizeof(kp));
+        PCD_HIP(hipGraphKernelNodeGetParams(nodes[i], &kp));
+        const uint64_t blocks = (uint64_t)kp.gridDim.x * kp.gridDim.y * kp.gridDim.z;
+        const uint64_t threads = (uint64_t)kp.blockDim.x * kp.blockDim.y * kp.blockDim.z;
+        if (!kp.func || blocks == 0 || blocks > (uint64_t)b->cu_count * 64 || threads < 64 || threads > 1024 ||
+            kp.sharedMemBytes != 0)
+            return bfail(PCD_E_STATE, "captured kernel node " + std::to_string(i) + " has grid " + std::to_string(blocks) +
+                                          ", block " + std::to_string(threads) + ", dynamic LDS " +
+                                          std::to_string(kp.sharedMemBytes));
+    }
+    if (nn == 0) return bfail(PCD_E_STATE, "captured round is empty");
+    return PCD_OK;
+}
+
 static int ensure_graph(pcd_batch *b) {
     if (b->items_dirty) PCD_TRY(rebuild_items(b));
     else if (!b->pending_append.empty()) PCD_TRY(append_items(b));
@@ -1443,6 +1492,7 @@ static int ensure_graph(pcd_batch *b) {
     }
     if (e != hipSuccess) return bfail(PCD_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     b->graph = g;
+    PCD_TRY(check_graph(b, g));
     PCD_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
     return PCD_OK;
 }

@@ -80,7 +80,7 @@ static const char *kslot_names[NUM_KSLOTS] = {
     "compress", "stats_items", "stats_small", "metric", "lp_records", "lp_hub_partition", "lp_hub_count",
     "lp_b4096", "lp_b1024", "lp_w256", "lp_g64", "lp_g32", "lp_g16", "lp_g8", "lp_g4",
     "merge_keys", "merge_sort", "merge_rle", "merge_rounds", "triangles", "stats_delta", "lp_small",
-    "lp_b512", "lp_hub_final"};
+    "lp_b512", "lp_hub_final", "merge_stats"};
 
 static const char *bin_names[NUM_BINS] = {"hub", "b4096", "b1024", "b512", "w256", "g64", "g32", "g16", "g8", "g4"};
 

@@ -117,7 +117,7 @@ enum KSlot {
     KS_LP_SELECT, KS_LP_HUB_INSERT, KS_LP_HUB_DECIDE, KS_LP_B4096, KS_LP_B1024, KS_LP_W256,
     KS_LP_G64, KS_LP_G32, KS_LP_G16, KS_LP_G8, KS_LP_G4,
     KS_MERGE_KEYS, KS_MERGE_SORT, KS_MERGE_RLE, KS_MERGE_ROUNDS, KS_TRIANGLES, KS_STATS_DELTA, KS_LP_SMALL,
-    KS_LP_B512, KS_LP_HUB_FINAL,
+    KS_LP_B512, KS_LP_HUB_FINAL, KS_MERGE_STATS,
     NUM_KSLOTS
 };
 

@@ -395,10 +395,24 @@ __global__ __launch_bounds__(256) void k_member_rows_items(int nitems, const int
     const int lane = __lane_id();
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int nwaves = gridDim.x * blockDim.x / WAVE;
+    constexpr int PF = 8;  // neighbour-label gathers in flight per lane
     for (int it = wave_g; it < nitems; it += nwaves) {
         const int4 w = items[it];
         const int o = dst[w.x] - indices[w.x];
-        for (int j = w.y + lane; j < w.z; j += WAVE) ent[o + j] = labels[adj[j]];
+        for (int j0 = w.y + lane; j0 < w.z; j0 += PF * WAVE) {
+            int nb[PF];
+#pragma unroll
+            for (int e = 0; e < PF; e++) {
+                const int j = j0 + e * WAVE;
+                nb[e] = j < w.z ? adj[j] : -1;
+            }
+            int lab[PF];
+#pragma unroll
+            for (int e = 0; e < PF; e++) lab[e] = nb[e] >= 0 ? labels[nb[e]] : 0;
+#pragma unroll
+            for (int e = 0; e < PF; e++)
+                if (nb[e] >= 0) ent[o + j0 + e * WAVE] = lab[e];
+        }
     }
 }
 
@@ -480,9 +494,86 @@ struct GrowBuf {
     }
 };
 
+// ------------------------------------------------ statistics after merging
+// A matched pair (l, p), l < p, becomes the community l (refinement.cpp:
+// 277-286): size_l + size_p, and the links between them turn from external
+// into internal volume on both sides, so int_volume gains 2 links and
+// ext_volume loses 2 links.  links(l, p) = the entries of l's row that hold
+// p; the lower community of each pair counts them.
+__device__ inline int pair_low(int l, const int32_t *found, const int32_t *bp) {
+    if (!found[l]) return -1;
+    const int p = bp[l];
+    return p > l ? p : -1;  // the partner when l is the lower side of a pair
+}
+
+// rows of at most 64 entries, G lanes per row
+template <int G>
+__global__ __launch_bounds__(256) void k_pair_links_small(int p0, int p1, const int32_t *corder, const int32_t *cind,
+                                                          const int32_t *ent, const int32_t *found, const int32_t *bp,
+                                                          int32_t *links) {
+    const int lane = __lane_id(), glane = lane & (G - 1);
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int nt = gridDim.x * blockDim.x / G;
+    for (int base = t; base < p1 - p0; base += nt) {
+        const int l = corder[p0 + base];
+        const int p = pair_low(l, found, bp);
+        int c = 0;
+        if (p >= 0)
+            for (int j = cind[l] + glane; j < cind[l + 1]; j += G) c += ent[j] == p;
+#pragma unroll
+        for (int d = G / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, WAVE);
+        if (p >= 0 && glane == 0) links[l] = c;
+    }
+}
+
+// longer rows: one wave per row (up to 4096 entries), or per 4096-entry
+// chunk of a hub row (summed with atomics)
+__global__ __launch_bounds__(256) void k_pair_links_rows(int p0, int p1, const int32_t *corder, const int32_t *cind,
+                                                         const int32_t *ent, const int32_t *found, const int32_t *bp,
+                                                         int32_t *links) {
+    const int lane = __lane_id();
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    for (int q = p0 + wave_g; q < p1; q += nwaves) {
+        const int l = corder[q];
+        const int p = pair_low(l, found, bp);
+        if (p < 0) continue;
+        int c = 0;
+        for (int j = cind[l] + lane; j < cind[l + 1]; j += WAVE) c += ent[j] == p;
+        c = wave_sum(c);
+        if (lane == 0) links[l] = c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pair_links_hub(int nchunks, const int4 *chunks, const int32_t *ent,
+                                                        const int32_t *found, const int32_t *bp, int32_t *links) {
+    const int lane = __lane_id();
+    const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int nwaves = gridDim.x * blockDim.x / WAVE;
+    for (int it = wave_g; it < nchunks; it += nwaves) {
+        const int4 ck = chunks[it];  // {hub, community, begin, end}
+        const int p = pair_low(ck.y, found, bp);
+        if (p < 0) continue;
+        int c = 0;
+        for (int j = ck.z + lane; j < ck.w; j += WAVE) c += ent[j] == p;
+        c = wave_sum(c);
+        if (lane == 0 && c) atomicAdd(&links[ck.y], c);
+    }
+}
+
+__global__ void k_merged_stats(int k, const int32_t *found, const int32_t *bp, const int32_t *links, CommStats *st) {
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) {
+        const int p = pair_low(l, found, bp);
+        if (p < 0) continue;
+        const CommStats a = st[l], b = st[p];  // p's record is not written: p is no pair's lower side
+        const int x = links[l];
+        st[l] = CommStats{a.size + b.size, a.int_vol + b.int_vol + 2 * x, a.ext_vol + b.ext_vol - 2 * x, 0};
+    }
+}
+
 // Device state of the community rows and the row set of the current step.
 struct MergeRows {
-    GrowBuf<int32_t> ent, cind, corder, dst;
+    GrowBuf<int32_t> ent, cind, corder, dst, links;
     GrowBuf<uint64_t> mk0, mk1;
     GrowBuf<unsigned long long> bcount;
     GrowBuf<int4> hrows;
@@ -682,7 +773,35 @@ static int merging_rows_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t se
             if (ctx->h_small[0] == 0) break;
         }
     }
-    return merge_relabel(ctx, k);
+    const bool follow = ctx->tune.inc_stats != 0;
+    if (follow) {
+        // the statistics of the merged labels, from the records and the
+        // pairs' links (the next compression renumbers them)
+        kt_begin(ctx, KS_MERGE_STATS);
+        const RowSet &rs = R.rows;
+        PCD_TRY(R.links.reserve(ctx, (size_t)k + 1));
+        if (rs.num_hubs > 0) {
+            PCD_TRY(fill_i32(ctx, R.links.p, 0, k));
+            hipLaunchKernelGGL(k_pair_links_hub, dim3(grid_for(ctx, rs.num_hub_items, 4, 8)), dim3(256), 0,
+                               ctx->stream, rs.num_hub_items, rs.hub_items, R.ent.p, ctx->d_found, ctx->d_bp,
+                               R.links.p);
+        }
+        const int q0 = rs.bins.start[BIN_B4096], q1 = rs.bins.start[BIN_G64];
+        if (q1 > q0)
+            hipLaunchKernelGGL(k_pair_links_rows, dim3(grid_for(ctx, q1 - q0, 4, 8)), dim3(256), 0, ctx->stream, q0,
+                               q1, R.corder.p, R.cind.p, R.ent.p, ctx->d_found, ctx->d_bp, R.links.p);
+        const int s0 = rs.bins.start[BIN_G64], s1 = rs.bins.start[NUM_BINS];
+        if (s1 > s0)
+            hipLaunchKernelGGL(k_pair_links_small<8>, dim3(grid_for(ctx, s1 - s0, 32, 8)), dim3(256), 0, ctx->stream,
+                               s0, s1, R.corder.p, R.cind.p, R.ent.p, ctx->d_found, ctx->d_bp, R.links.p);
+        hipLaunchKernelGGL(k_merged_stats, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_found,
+                           ctx->d_bp, R.links.p, ctx->d_stats);
+        kt_end(ctx, KS_MERGE_STATS);
+        PCD_HIP(hipGetLastError());
+    }
+    PCD_TRY(merge_relabel(ctx, k));
+    ctx->stats_pending = follow;
+    return PCD_OK;
 }
 
 // the partner choice over the sorted unique community pairs

@@ -434,6 +434,37 @@ def test_merging(ctx, kind, lam, tie, merge_sort):
         ctx.set_tuning("merge_sort", 0)
 
 
+@pytest.mark.parametrize("inc", [1, 0])
+def test_statistics_follow_merging(ctx, inc):
+    """After a merging step, the statistics of the merged labels come from
+    the pairs' records and links (inc_stats 1; the compression renumbers
+    them) or from a full recount (0).  Either way, compress + statistics
+    equal the oracle's count of the new labels, over several merging steps
+    (hub rows included)."""
+    g = GG.bin_zoo()
+    n, m, ind, adj = g[:4]
+    rng = np.random.default_rng(15)
+    ctx.set_tuning("inc_stats", inc)
+    try:
+        for lab in [GG.singleton_labels(n), rng.integers(0, 40, n), rng.integers(0, 3000, n)]:
+            lab_o, so, ivo, evo = check_partition(ctx, g, lab)
+            for it in range(3):
+                ctx.merge(O.MOD, 0.0, rounds=1 + it % 2, seed=16, step=it, tie=O.TIE_RANDOM)
+                want = O.merging_refine(O.MOD, 0.0, n, m, ind, adj, lab_o, so, ivo, evo, 0.0, 1 + it % 2, 1, 16, it,
+                                        O.TIE_RANDOM)[0]
+                k = ctx.compress()
+                lab_o, k_o = O.compress_labels(want)
+                assert k == k_o and np.array_equal(ctx.labels(), lab_o)
+                assert ctx.statistics() == k_o
+                s_, iv, ev = ctx.stats(k)
+                _, so, ivo, evo = O.collect_statistics(n, ind, adj, lab_o)
+                assert np.array_equal(s_, so) and np.array_equal(iv, ivo) and np.array_equal(ev, evo), it
+                assert ctx.metric(O.MOD) == pytest.approx(O.calculate(O.MOD, 0.0, n, m, ind, adj, lab_o, so, ivo, evo),
+                                                          rel=1e-9, abs=1e-12)
+    finally:
+        ctx.set_tuning("inc_stats", 1)
+
+
 @pytest.mark.parametrize("slots,part_entries", [(4096, 2048), (64, 100000), (1024, 512)])
 def test_merging_community_rows_hub_paths(ctx, slots, part_entries):
     """Community rows above 4096 entries go through the hub path (chunks,
