@@ -561,14 +561,39 @@ __global__ __launch_bounds__(256) void k_pair_links_hub(int nchunks, const int4 
     }
 }
 
+// (also clears the records' matched flags, kept in their pad words during
+// the step; the other records of a pair are never read again)
 __global__ void k_merged_stats(int k, const int32_t *found, const int32_t *bp, const int32_t *links, CommStats *st) {
     for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) {
         const int p = pair_low(l, found, bp);
-        if (p < 0) continue;
+        if (p < 0) {
+            if (found[l]) st[l].pad = 0;
+            continue;
+        }
         const CommStats a = st[l], b = st[p];  // p's record is not written: p is no pair's lower side
         const int x = links[l];
         st[l] = CommStats{a.size + b.size, a.int_vol + b.int_vol + 2 * x, a.ext_vol + b.ext_vol - 2 * x, 0};
     }
+}
+
+// The matched flags of a merging step on community rows live in the
+// records' pad words too (the merging kernels read the record of every
+// candidate anyway): cleared at the start, set with `found`.
+__global__ void k_clear_pads(int k, CommStats *st) {
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) st[l].pad = 0;
+}
+
+__global__ void k_handshake_rec(int k, int32_t *found, const int32_t *bp, int32_t *unmatched, CommStats *st) {
+    int local = 0;
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) {
+        if (!found[l] && (bp[l] == -1 || bp[bp[l]] == l)) {
+            found[l] = 1;
+            st[l].pad = 1;
+        }
+        if (!found[l]) local++;
+    }
+    local = wave_sum(local);
+    if (__lane_id() == 0 && local) atomicAdd(unmatched, local);
 }
 
 // Device state of the community rows and the row set of the current step.
@@ -746,6 +771,7 @@ static int merging_rows_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t se
     PCD_TRY(ensure_merge_state(ctx));
     PCD_TRY(fill_i32(ctx, ctx->d_bp, -1, k));
     PCD_TRY(fill_i32(ctx, ctx->d_found, 0, k));
+    hipLaunchKernelGGL(k_clear_pads, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_stats);
     LPArgs a{};
     a.indices = R.cind.p;  // community l's row: entries [cind[l], cind[l+1]) of ent
     a.adj = R.ent.p;
@@ -762,8 +788,8 @@ static int merging_rows_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t se
         PCD_TRY(merge_rows_launch(ctx, R.rows, a));
         kt_begin(ctx, KS_MERGE_ROUNDS);
         PCD_TRY(fill_i32(ctx, unmatched, 0, 1));
-        hipLaunchKernelGGL(k_handshake, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_found,
-                           ctx->d_bp, unmatched);
+        hipLaunchKernelGGL(k_handshake_rec, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_found,
+                           ctx->d_bp, unmatched, ctx->d_stats);
         kt_end(ctx, KS_MERGE_ROUNDS);
         PCD_HIP(hipGetLastError());
         if (rnd + 1 < rounds) {
@@ -774,6 +800,8 @@ static int merging_rows_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t se
         }
     }
     const bool follow = ctx->tune.inc_stats != 0;
+    if (!follow)  // the records stay as they are for a caller that reads them
+        hipLaunchKernelGGL(k_clear_pads, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_stats);
     if (follow) {
         // the statistics of the merged labels, from the records and the
         // pairs' links (the next compression renumbers them)

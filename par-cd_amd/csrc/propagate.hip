@@ -90,7 +90,9 @@ constexpr unsigned RK_BATCH = RK_MOD | RK_CPM | RK_REC;
 // labels of its members' neighbours (gathered into the row once per step),
 // its own label is l, a candidate is another community not yet matched,
 // scored by score_merge and accepted from 0 up, and the row's result is its
-// best partner or -1.  They read the community records.
+// best partner or -1.  They read the community records, whose pad word
+// holds the community's "matched" flag during a merging step (one gather
+// per candidate brings both).
 constexpr unsigned RK_MERGE = 1u << 9;
 constexpr unsigned RK_MERGE_REC = RK_MERGE | RK_REC;
 
@@ -295,11 +297,13 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
                 int cnt = min(nxt, gend) - lane;
                 if (x == own.x) {
                     own_cnt = cnt;
-                } else if ((KM & RK_MERGE) == 0 || !a.found[x]) {
-                    uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), u);
-                    double sc;
-                    if (cand_score<KM>(a, deg, own_rec, AUX ? CommStats{xa, xa, 0, 0} : comm_rec<KM>(a, x), cnt, sc))
-                        c = Cand{sc, tie_key(a.tie, tb, x), x};
+                } else {
+                    const CommStats xr = AUX ? CommStats{xa, xa, 0, 0} : comm_rec<KM>(a, x);
+                    if ((KM & RK_MERGE) == 0 || !xr.pad) {  // merging: matched communities are no candidates
+                        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), u);
+                        double sc;
+                        if (cand_score<KM>(a, deg, own_rec, xr, cnt, sc)) c = Cand{sc, tie_key(a.tie, tb, x), x};
+                    }
                 }
             }
 #pragma unroll
@@ -477,11 +481,6 @@ __device__ inline void scan_listed(const LPArgs &a, int *keys, CT *cnts, const i
                 cnts[h] = 0;
             }
         }
-        if constexpr (MERGE) {  // matched communities are no candidates
-#pragma unroll
-            for (int e = 0; e < SB; e++)
-                if (key[e] != EMPTY_KEY && a.found[key[e]]) key[e] = EMPTY_KEY;
-        }
         if (!AUX) {
 #pragma unroll
             for (int e = 0; e < SB; e++)
@@ -490,6 +489,7 @@ __device__ inline void scan_listed(const LPArgs &a, int *keys, CT *cnts, const i
 #pragma unroll
         for (int e = 0; e < SB; e++) {
             if (key[e] == EMPTY_KEY) continue;
+            if (MERGE && key[e] != own && st[e].pad) continue;  // matched: no candidate
             if (key[e] == own) {
                 own_cnt = cnt[e];
             } else {
@@ -579,8 +579,11 @@ struct SmallBins {
 
 // WPE: waves per SIMD the registers are allocated for (1: the compiler's
 // choice).  At 8 the modularity and CPM builds fit 64 VGPRs without scratch;
-// the scalar registers beyond 96 spill to VGPR lanes.
-template <unsigned KM, int WPE = 1>
+// the scalar registers beyond 96 spill to VGPR lanes.  SAUX: the G bins
+// gather {label, aux} records (one 8-byte gather brings a neighbour's label
+// and its community's total, one dependent load fewer per row); W256 keeps
+// gathering labels.
+template <unsigned KM, int WPE = 1, bool SAUX = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_lp_small_all(SmallBins sb) {
     __shared__ int s_keys[256 / WAVE][512];
     __shared__ int s_cnts[256 / WAVE][512];
@@ -597,11 +600,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     switch (b) {
     case 0: lp_wave<512, KM, RangeSrc, false, 4>(sb.src[0], wave_g, nwaves, s_keys[w], s_cnts[w], nullptr, s_occ[w],
                                                  &s_nocc[w]); break;
-    case 1: lp_small<64, 8, false, KM>(sb.src[1], wave_g, nwaves, list); break;
-    case 2: lp_small<32, 16, false, KM>(sb.src[2], wave_g, nwaves, list); break;
-    case 3: lp_small<16, 32, false, KM>(sb.src[3], wave_g, nwaves, list); break;
-    case 4: lp_small<8, 64, false, KM>(sb.src[4], wave_g, nwaves, list); break;
-    default: lp_small<4, 64, false, KM>(sb.src[5], wave_g, nwaves, list); break;
+    case 1: lp_small<64, 8, SAUX, KM>(sb.src[1], wave_g, nwaves, list); break;
+    case 2: lp_small<32, 16, SAUX, KM>(sb.src[2], wave_g, nwaves, list); break;
+    case 3: lp_small<16, 32, SAUX, KM>(sb.src[3], wave_g, nwaves, list); break;
+    case 4: lp_small<8, 64, SAUX, KM>(sb.src[4], wave_g, nwaves, list); break;
+    default: lp_small<4, 64, SAUX, KM>(sb.src[5], wave_g, nwaves, list); break;
     }
 }
 
@@ -985,9 +988,10 @@ __global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h,
             gcnts[off + s] = 0;
             distinct++;
             if (key == own) own_cnt = cnt;
-            else if ((KM & RK_MERGE) == 0 || !a.found[key]) {
+            else {
+                const CommStats xr = comm_rec<KM>(a, key);
                 double sc;
-                if (cand_score<KM>(a, deg, orec, comm_rec<KM>(a, key), cnt, sc)) {
+                if (((KM & RK_MERGE) == 0 || !xr.pad) && cand_score<KM>(a, deg, orec, xr, cnt, sc)) {
                     Cand o{sc, tie_key(a.tie, tb, key), key};
                     if (cand_better(o, c)) c = o;
                 }
@@ -1183,8 +1187,9 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
                                dim3(256), 0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B512)});
         kt_end(ctx, KS_LP_B512);
     }
-    // W256 and G64..G4 in one launch unless a record-gathering form is tuned on
-    const bool fused = !small_aux && !block_aux && ctx->tune.small_fused;
+    // W256 and G64..G4 in one launch unless the wave / workgroup bins gather
+    // records (small_aux: the launch's G bins gather records)
+    const bool fused = !block_aux && ctx->tune.small_fused;
     if (!fused && range(BIN_W256, p0, p1)) {
         kt_begin(ctx, KS_LP_W256);
         if (block_aux)
@@ -1236,11 +1241,23 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
             }
             sb.lo[6] = at;
             kt_begin(ctx, KS_LP_SMALL);
-            if (ctx->tune.small_occ8 && (KM == RK_MOD || KM == RK_CPM))
-                hipLaunchKernelGGL((k_lp_small_all<KM, (KM == RK_MOD || KM == RK_CPM) ? 8 : 1>), dim3(at), dim3(256),
-                                   0, ctx->stream, sb);
-            else
-                hipLaunchKernelGGL(k_lp_small_all<KM>, dim3(at), dim3(256), 0, ctx->stream, sb);
+            constexpr int W8 = (KM == RK_MOD || KM == RK_CPM) ? 8 : 1;
+            bool launched = false;
+            if constexpr (KM == RK_MOD || KM == RK_CPM) {
+                if (small_aux) {  // records of the modularity / CPM totals
+                    if (ctx->tune.small_occ8)
+                        hipLaunchKernelGGL((k_lp_small_all<KM, W8, true>), dim3(at), dim3(256), 0, ctx->stream, sb);
+                    else
+                        hipLaunchKernelGGL((k_lp_small_all<KM, 1, true>), dim3(at), dim3(256), 0, ctx->stream, sb);
+                    launched = true;
+                }
+            }
+            if (!launched) {
+                if (ctx->tune.small_occ8 && (KM == RK_MOD || KM == RK_CPM))
+                    hipLaunchKernelGGL((k_lp_small_all<KM, W8>), dim3(at), dim3(256), 0, ctx->stream, sb);
+                else
+                    hipLaunchKernelGGL(k_lp_small_all<KM>, dim3(at), dim3(256), 0, ctx->stream, sb);
+            }
             kt_end(ctx, KS_LP_SMALL);
         }
     } else {
