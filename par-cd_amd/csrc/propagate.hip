@@ -130,7 +130,7 @@ template <unsigned KM>
 __device__ inline bool cand_score(const LPArgs &a, int deg, const CommStats &own_rec, const CommStats &rec, int cnt,
                                   double &sc) {
     if constexpr ((KM & RK_MERGE) != 0) {
-        sc = score_merge(a.rule, own_rec, rec, cnt);
+        sc = score_merge<KM & RK_ALL>(a.rule, own_rec, rec, cnt);
         return sc >= 0.0;
     } else {
         sc = score_join<KM>(a.rule, deg, rec, cnt);
@@ -1241,7 +1241,9 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
             }
             sb.lo[6] = at;
             kt_begin(ctx, KS_LP_SMALL);
-            constexpr int W8 = (KM == RK_MOD || KM == RK_CPM) ? 8 : 1;
+            // modularity and CPM builds (propagation or merging) at 8 waves per SIMD
+            constexpr bool ONE_KIND = (KM & RK_ALL) == RK_MOD || (KM & RK_ALL) == RK_CPM;
+            constexpr int W8 = ONE_KIND ? 8 : 1;
             bool launched = false;
             if constexpr (KM == RK_MOD || KM == RK_CPM) {
                 if (small_aux) {  // records of the modularity / CPM totals
@@ -1253,7 +1255,7 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
                 }
             }
             if (!launched) {
-                if (ctx->tune.small_occ8 && (KM == RK_MOD || KM == RK_CPM))
+                if (ctx->tune.small_occ8 && ONE_KIND)
                     hipLaunchKernelGGL((k_lp_small_all<KM, W8>), dim3(at), dim3(256), 0, ctx->stream, sb);
                 else
                     hipLaunchKernelGGL(k_lp_small_all<KM>, dim3(at), dim3(256), 0, ctx->stream, sb);
@@ -1306,8 +1308,12 @@ int propagation_launch(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, 
     return PCD_OK;
 }
 
+// builds per rule kind, as for propagation (modularity and CPM hold fewer
+// registers without the WCC and conductance formulas)
 int merge_rows_launch(pcd_ctx *ctx, const RowSet &rows, const LPArgs &a) {
-    return launch_propagation<RK_MERGE_REC>(ctx, rows, a);
+    if (a.rule.kind == PCD_RULE_MODULARITY) return launch_propagation<RK_MERGE_REC | RK_MOD>(ctx, rows, a);
+    if (a.rule.kind == PCD_RULE_CPM) return launch_propagation<RK_MERGE_REC | RK_CPM>(ctx, rows, a);
+    return launch_propagation<RK_MERGE_REC | RK_ALL>(ctx, rows, a);
 }
 
 int propagation_commit(pcd_ctx *ctx) {

@@ -137,28 +137,28 @@ PCD_HD inline double score_leave(const RuleParams &r, int deg, CommStats c, int 
     return 0.0;
 }
 
-// Rule::score_merge (rules.cpp:24-32, 131-134, 391-401, 76-91)
+// Rule::score_merge (rules.cpp:24-32, 131-134, 391-401, 76-91); KM as for
+// score_join: a build for one rule kind compiles that formula alone.
+template <unsigned KM = RK_ALL>
 PCD_HD inline double score_merge(const RuleParams &r, CommStats a, CommStats b, int links) {
-    switch (r.kind) {
-    case PCD_RULE_MODULARITY: {
+    if ((KM & RK_MOD) && (KM == RK_MOD || r.kind == PCD_RULE_MODULARITY)) {
         double e_ij = links / (double)r.m;
         double a_i = (a.int_vol + a.ext_vol) / (2.0 * r.m);
         double a_j = (b.int_vol + b.ext_vol) / (2.0 * r.m);
         return e_ij - 2 * a_i * a_j;
     }
-    case PCD_RULE_CPM:
-        return links - r.lambda * a.size * b.size;
-    case PCD_RULE_WCC: {
+    if ((KM & RK_CPM) && (KM == RK_CPM || r.kind == PCD_RULE_CPM)) return links - r.lambda * a.size * b.size;
+    if ((KM & RK_WCC) && (KM == RK_WCC || r.kind == PCD_RULE_WCC)) {
         int cx = links < a.size ? links : a.size;
         int cy = links < b.size ? links : b.size;
         return wcc_improvement(a.size, b.size, cx, cy, a.int_vol / 2, b.int_vol / 2, links, a.ext_vol, b.ext_vol, r.cc);
     }
-    default: {
+    if (KM & RK_COND) {
         int si = a.size, sj = b.size, ii = a.int_vol, ij = b.int_vol, vi = ii + a.ext_vol, vj = ij + b.ext_vol;
         return wrap_mul(si + sj, ii + ij + links) / (double)(vi + vj + links) - wrap_mul(si, ii) / (double)vi -
                wrap_mul(sj, ij) / (double)vj;
     }
-    }
+    return 0.0;  // kind outside KM: never called so
 }
 
 // ------------------------------------------------------ counter-based RNG
