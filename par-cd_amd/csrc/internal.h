@@ -66,7 +66,14 @@ struct RowSet {
     const int64_t *hub_table_off = nullptr;  // per hub: fallback global table
     const int32_t *hub_table_log = nullptr;
     int32_t *hub_keys = nullptr, *hub_cnts = nullptr;
+    // Laid out on the device (no host read-back): `layout` holds the bin
+    // starts (NUM_BINS + 1 positions) and the fused small-row launch's
+    // workgroup split (7 entries), `hub_counts` {chunks, partition items,
+    // hubs}; the launches then use fixed grids and read these.
+    const int *layout = nullptr;
+    const int *hub_counts = nullptr;
 };
+constexpr int ROW_LAYOUT_INTS = NUM_BINS + 1 + 7;
 
 // Tunables (the autotuner rewrites these per graph shape).
 struct Tuning {
@@ -277,6 +284,8 @@ int fill_i32(pcd_ctx *ctx, int32_t *p, int32_t v, int64_t count);
 int fill_u64(pcd_ctx *ctx, uint64_t *p, uint64_t v, int64_t count);
 int ensure_scan_parts(pcd_ctx *ctx, int64_t n);
 int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int bits, uint64_t **out);
+// the tile histograms of a radix sort of n keys (a captured sort may not allocate)
+int ensure_sort_space(pcd_ctx *ctx, int64_t n);
 
 // phases (partition.hip, propagate.hip, merge.hip, triangles.hip)
 // defer_k: k stays on the device (d_k) and its read-back lands in pinned
@@ -303,6 +312,11 @@ int propagation_dev(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int
 int propagation_launch(pcd_ctx *ctx, pcd_rule rule, double frac, uint64_t seed, int step, int tie);
 int propagation_commit(pcd_ctx *ctx);
 int merging_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie);
+// reserves the buffers of a merging step over community rows (a captured
+// step may not allocate)
+int merge_rows_reserve(pcd_ctx *ctx);
+// fails when a merging step since the last check left hub rows out
+int merge_rows_check(pcd_ctx *ctx);
 int contract_dev(pcd_ctx *ctx, int64_t *num_pairs);
 int contract_download(pcd_ctx *ctx, int64_t U, int32_t *l, int32_t *k, int32_t *links);
 int triangles_dev(pcd_ctx *ctx);

@@ -1,10 +1,13 @@
 // The refinement loop of main.cpp:226-266 kept on the device.
 //
-// One pass (compress, metric, stop rule, propagation step with the
-// statistics update) of a run with -s 0 is captured once as a hipGraph and
-// replayed (every rule; WCC once the graph's triangles are counted).  The pass number, the previous score and the
+// One pass (compress, metric, stop rule, then a propagation step or a
+// merging step, with the statistics update) is captured once as a hipGraph
+// per step kind and replayed in the order of the schedule: -s 0 propagates
+// every pass, -s 1 merges every pass, -s 2 alternates, starting with
+// propagation (main.cpp:247-258); every rule, WCC once the graph's
+// triangles are counted.  The pass number, the previous score and the
 // stop rule (main.cpp:241-242, -n, the iteration cap) live in a device
-// record; the captured propagation reads its step number from it, so every
+// record; the captured step reads its step number from it, so every
 // replay is the next pass.  When the rule fires, or the requested number of
 // passes is reached, the pass freezes the record and keeps a copy of the
 // compressed labels and the renumbered statistics; from then on the step
@@ -88,9 +91,18 @@ __global__ void k_loop_next(LoopState *s) {
     s->just = 0;
 }
 
-// The captured pass, its state and the buffers it was captured with.
+enum PassKind { PASS_PROPAGATE = 0, PASS_MERGE = 1 };
+
+// step kind of pass `it` under schedule `strategy` (main.cpp:247-258)
+static int pass_kind(int strategy, int it) {
+    if (strategy == 0) return PASS_PROPAGATE;
+    if (strategy == 1) return PASS_MERGE;
+    return it % 2 == 0 ? PASS_PROPAGATE : PASS_MERGE;
+}
+
+// The captured passes, their state and the buffers they were captured with.
 struct Loop {
-    hipGraphExec_t exec = nullptr;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};  // by PassKind
     LoopState *d_state = nullptr, *h_state = nullptr;
     int32_t *snap_labels = nullptr;
     CommStats *snap_stats = nullptr;
@@ -101,7 +113,7 @@ struct Loop {
     pcd_rule rule{};
     double frac = 0, threshold = 0;
     uint64_t seed = 0;
-    int tie = 0, max_iter = 0, cap = 0;
+    int tie = 0, max_iter = 0, cap = 0, rounds = 0;
     bool broken = false;  // capture failed once: this context runs pass by pass
 };
 
@@ -110,33 +122,41 @@ static Loop *loop_of(pcd_ctx *ctx) { return (Loop *)ctx->loop; }
 void loop_release(pcd_ctx *ctx) {
     Loop *L = loop_of(ctx);
     if (!L) return;
-    if (L->exec) (void)hipGraphExecDestroy(L->exec);
+    for (hipGraphExec_t &e : L->exec)
+        if (e) (void)hipGraphExecDestroy(e);
     if (L->h_state) (void)hipHostFree(L->h_state);
     // device buffers belong to the context's allocation list
     delete L;
     ctx->loop = nullptr;
 }
 
-static bool same_key(const Loop *L, pcd_ctx *ctx, const pcd_job *job, int cap) {
+// whether the captured passes of `kinds` (bit per PassKind) were captured
+// for these buffers, tuning, job and merging rounds
+static bool same_key(const Loop *L, pcd_ctx *ctx, const pcd_job *job, int cap, int rounds, unsigned kinds) {
     const void *p[4] = {ctx->d_labels, ctx->d_labels_alt, ctx->d_stats, ctx->d_stats_alt};
     for (int i = 0; i < 4; i++)
         if (p[i] != L->ptrs[i]) return false;
-    return L->exec && L->alloc_gen == ctx->alloc_gen && L->tune_gen == ctx->tune_gen &&
-           L->rule.kind == job->rule.kind && L->rule.lambda == job->rule.lambda && L->frac == job->frac &&
-           L->threshold == job->threshold && L->seed == job->seed && L->tie == job->tie &&
-           L->max_iter == job->max_iter && L->cap == cap;
+    for (int k = 0; k < 2; k++)
+        if (((kinds >> k) & 1) && !L->exec[k]) return false;
+    return L->alloc_gen == ctx->alloc_gen && L->tune_gen == ctx->tune_gen && L->rule.kind == job->rule.kind &&
+           L->rule.lambda == job->rule.lambda && L->frac == job->frac && L->threshold == job->threshold &&
+           L->seed == job->seed && L->tie == job->tie && L->max_iter == job->max_iter && L->cap == cap &&
+           (!((kinds >> PASS_MERGE) & 1) || L->rounds == rounds);
 }
 
-// Captures one pass.  Nothing runs: the host bookkeeping is put back as it
-// was after the capture.  Any failure leaves no graph behind.
-static int capture_pass(pcd_ctx *ctx, Loop *L, const pcd_job *job, int cap) {
-    if (L->exec) {
-        (void)hipGraphExecDestroy(L->exec);
-        L->exec = nullptr;
+// Captures one pass of step kind `kind`.  Nothing runs: the host
+// bookkeeping is put back as it was after the capture.  Any failure leaves
+// no graph behind.
+static int capture_pass(pcd_ctx *ctx, Loop *L, const pcd_job *job, int cap, int kind, int rounds) {
+    hipGraphExec_t &exec = L->exec[kind];
+    if (exec) {
+        (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
     }
     // buffers the pass may grow lazily are sized first (no allocation may
     // happen while the stream captures)
     PCD_TRY(ensure_scan_parts(ctx, (int64_t)ctx->n + ctx->num_stat_items + 64));
+    if (kind == PASS_MERGE) PCD_TRY(merge_rows_reserve(ctx));
     if (!L->d_state) PCD_TRY(dev_alloc_t(ctx, &L->d_state, 1));
     if (!L->h_state && hipHostMalloc((void **)&L->h_state, sizeof(LoopState), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
@@ -165,9 +185,13 @@ static int capture_pass(pcd_ctx *ctx, Loop *L, const pcd_job *job, int cap) {
         hipLaunchKernelGGL(k_loop_rule, dim3(1), dim3(1), 0, ctx->stream, L->d_state, ctx->d_result, ctx->d_k, lp);
         hipLaunchKernelGGL(k_loop_snapshot, dim3(grid_for(ctx, ctx->n + 1, 256, 8)), dim3(256), 0, ctx->stream,
                            L->d_state, ctx->n, ctx->d_labels, ctx->d_stats, L->snap_labels, L->snap_stats);
-        rc = propagation_launch(ctx, rule, job->frac, job->seed, 0, job->tie);
+        if (kind == PASS_PROPAGATE) {
+            rc = propagation_launch(ctx, rule, job->frac, job->seed, 0, job->tie);
+            if (rc == PCD_OK) rc = propagation_commit(ctx);
+        } else {
+            rc = merging_dev(ctx, rule, rounds, job->seed, 0, job->tie);
+        }
     }
-    if (rc == PCD_OK) rc = propagation_commit(ctx);
     if (rc == PCD_OK) hipLaunchKernelGGL(k_loop_next, dim3(1), dim3(1), 0, ctx->stream, L->d_state);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
@@ -187,9 +211,9 @@ static int capture_pass(pcd_ctx *ctx, Loop *L, const pcd_job *job, int cap) {
     (void)hipGetLastError();
     if (rc == PCD_OK && (e != hipSuccess || !g)) rc = fail(PCD_E_HIP, std::string("loop capture: ") + hipGetErrorString(e));
     if (rc == PCD_OK && !same_ptrs) rc = fail(PCD_E_STATE, "loop capture: the pass does not return its buffers");
-    if (rc == PCD_OK && hipGraphInstantiate(&L->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    if (rc == PCD_OK && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) {
         (void)hipGetLastError();
-        L->exec = nullptr;
+        exec = nullptr;
         rc = fail(PCD_E_HIP, "loop graph instantiation");
     }
     if (g) (void)hipGraphDestroy(g);
@@ -204,12 +228,13 @@ static int capture_pass(pcd_ctx *ctx, Loop *L, const pcd_job *job, int cap) {
     L->tie = job->tie;
     L->max_iter = job->max_iter;
     L->cap = cap;
+    if (kind == PASS_MERGE) L->rounds = rounds;
     return PCD_OK;
 }
 
 // Passes [it, it + count) on the device; see pcd_iterate_many.
-static int device_passes(pcd_ctx *ctx, Loop *L, const pcd_job *job, int cap, int it, int count, double prev_score,
-                         int *passes, double *score, int *num_labels, int *done) {
+static int device_passes(pcd_ctx *ctx, Loop *L, const pcd_job *job, int strategy, int cap, int it, int count,
+                         double prev_score, int *passes, double *score, int *num_labels, int *done) {
     LoopState &h = *L->h_state;
     std::memset(&h, 0, sizeof(h));
     h.it = it;
@@ -227,7 +252,9 @@ static int device_passes(pcd_ctx *ctx, Loop *L, const pcd_job *job, int cap, int
         const int left = h.it_end - h.it + 1;  // the pass at it_end freezes
         const int c = std::max(1, std::min(per, left));
         const auto t0 = std::chrono::steady_clock::now();
-        for (int r = 0; r < c; r++) PCD_HIP(hipGraphLaunch(L->exec, ctx->stream));
+        // replay r of this batch is pass h.it + r (once frozen, a replay
+        // of either kind changes nothing)
+        for (int r = 0; r < c; r++) PCD_HIP(hipGraphLaunch(L->exec[pass_kind(strategy, h.it + r)], ctx->stream));
         PCD_HIP(hipMemcpyAsync(&h, L->d_state, sizeof(LoopState), hipMemcpyDeviceToHost, ctx->stream));
         PCD_HIP(hipStreamSynchronize(ctx->stream));
         if (h.frozen) break;
@@ -290,8 +317,11 @@ extern "C" int pcd_iterate_many(pcd_ctx *ctx, const pcd_job *job, int strategy, 
     // pass by pass until the statistics follow the labels (the first pass
     // of labels from outside counts them in full), or for the rules and
     // strategies the captured pass does not cover
+    // captured step kinds the schedule replays (bit per PassKind); merging
+    // is captured in its community-row form
+    const unsigned kinds = strategy == 0 ? 1u : strategy == 1 ? 2u : 3u;
     for (;;) {
-        const bool device_ok = strategy == 0 &&
+        const bool device_ok = (strategy == 0 || !ctx->tune.merge_sort) &&
                                (job->rule.kind != PCD_RULE_WCC || ctx->has_triangles) &&
                                ctx->tune.graph_loop && ctx->tune.inc_stats && ctx->ev_metric && ctx->has_labels &&
                                (ctx->stats_pending || ctx->has_stats) &&
@@ -315,8 +345,12 @@ extern "C" int pcd_iterate_many(pcd_ctx *ctx, const pcd_job *job, int strategy, 
     PCD_HIP(hipSetDevice(ctx->device));
     if (!ctx->loop) ctx->loop = new Loop();
     Loop *L = loop_of(ctx);
-    if (!same_key(L, ctx, job, cap)) {
-        const int rc = capture_pass(ctx, L, job, cap);
+    if (!same_key(L, ctx, job, cap, rounds, kinds)) {
+        // everything a captured merging step uses is allocated first, so
+        // that neither capture allocates
+        int rc = (kinds & 2u) ? merge_rows_reserve(ctx) : PCD_OK;
+        for (int kd = 0; kd < 2 && rc == PCD_OK; kd++)
+            if ((kinds >> kd) & 1u) rc = capture_pass(ctx, L, job, cap, kd, rounds);
         if (rc != PCD_OK) {
             // this context runs pass by pass from now on
             L->broken = true;
@@ -332,7 +366,8 @@ extern "C" int pcd_iterate_many(pcd_ctx *ctx, const pcd_job *job, int strategy, 
     }
     int p2 = 0, d2 = 0, k2 = 0;
     double s2 = 0.0;
-    PCD_TRY(device_passes(ctx, L, job, cap, it + run, count - run, prev, &p2, &s2, &k2, &d2));
+    PCD_TRY(device_passes(ctx, L, job, strategy, cap, it + run, count - run, prev, &p2, &s2, &k2, &d2));
+    if (kinds & 2u) PCD_TRY(merge_rows_check(ctx));
     *passes = run + p2;
     if (p2) {
         *score = s2;

@@ -24,6 +24,7 @@
 //    over the unique pairs with two order-independent atomics per l (max of
 //    the order-preserving score bits, then min of (tie key, k)).
 // Both choose the same partners: (score, tie key, label) order is total.
+#include <algorithm>
 #include <type_traits>
 
 #include "work.h"
@@ -331,6 +332,12 @@ __global__ void k_merge_relabel(int n, const int32_t *labels, const int32_t *fou
 }
 
 // ------------------------------------------------------ community rows
+// A merging step over community rows is laid out by kernels: no size is
+// read back, so the step can be queued behind others or captured in a
+// device-resident pass.  Community ids lie below `bound` (the label bound:
+// k, or n + 1 while k stays on the device, when ids from *kp on are empty).
+__device__ inline int k_of(int bound, const int *kp) { return kp ? min(*kp, bound) : bound; }
+
 __global__ void k_member_keys(int n, const int32_t *labels, uint64_t *keys) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
         keys[v] = ((uint64_t)(uint32_t)v << 32) | (uint32_t)labels[v];
@@ -341,14 +348,18 @@ __global__ void k_member_keys(int n, const int32_t *labels, uint64_t *keys) {
 // has an empty row.
 struct CommVol {
     const CommStats *st;
-    __device__ int operator()(int64_t l) const { return st[l].int_vol + st[l].ext_vol; }
+    int bound;
+    const int *kp;
+    __device__ int operator()(int64_t l) const {
+        return (int)l < k_of(bound, kp) ? st[l].int_vol + st[l].ext_vol : 0;
+    }
 };
 struct CommStart {
     int32_t *cind;
-    int64_t k;
+    int64_t bound;
     __device__ void operator()(int64_t l, int ex, int vol) const {
         cind[l] = ex;
-        if (l == k - 1) cind[k] = ex + vol;
+        if (l == bound - 1) cind[bound] = ex + vol;
     }
 };
 
@@ -430,9 +441,10 @@ __device__ inline int row_bin(int len) {
     return BIN_G4;
 }
 
-// per community: its bin as the sort key (community id above), and the
-// rows and entries of each bin
-__global__ __launch_bounds__(256) void k_comm_bins(int k, const int32_t *cind, uint64_t *keys,
+// Per community id below `bound`: its bin as the sort key (the id above);
+// the ids from k on get bin NUM_BINS and sort after every bin.  The rows
+// and entries of each bin are counted.
+__global__ __launch_bounds__(256) void k_comm_bins(int bound, const int *kp, const int32_t *cind, uint64_t *keys,
                                                    unsigned long long *bcount) {
     __shared__ unsigned long long rows[NUM_BINS], ents[NUM_BINS];
     if (threadIdx.x < NUM_BINS) {
@@ -440,12 +452,15 @@ __global__ __launch_bounds__(256) void k_comm_bins(int k, const int32_t *cind, u
         ents[threadIdx.x] = 0;
     }
     __syncthreads();
-    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) {
+    const int k = k_of(bound, kp);
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < bound; l += gridDim.x * blockDim.x) {
         const int len = cind[l + 1] - cind[l];
-        const int b = row_bin(len);
+        const int b = l < k ? row_bin(len) : NUM_BINS;
         keys[l] = ((uint64_t)(uint32_t)l << 32) | (uint32_t)b;
-        atomicAdd(&rows[b], 1ULL);
-        atomicAdd(&ents[b], (unsigned long long)len);
+        if (b < NUM_BINS) {
+            atomicAdd(&rows[b], 1ULL);
+            atomicAdd(&ents[b], (unsigned long long)len);
+        }
     }
     __syncthreads();
     if (threadIdx.x < NUM_BINS) {
@@ -459,11 +474,189 @@ __global__ void k_high_words(int k, const uint64_t *keys, int32_t *out) {
         out[i] = (int32_t)(keys[i] >> 32);
 }
 
-// the HUB-bin rows {community, first entry, end entry}
-__global__ void k_hub_rows(int nh, const int32_t *corder, const int32_t *cind, int4 *out) {
-    for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
+// One thread: the bin starts (positions in the sorted order) and the
+// fused small-row launch's split of cu16 workgroups (one per non-empty bin,
+// the rest in proportion to the bins' entries), as the host computes them
+// for the vertex bins.
+__global__ void k_row_layout(const unsigned long long *bcount, int cu16, int *layout) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    long long at = 0;
+    for (int b = 0; b < NUM_BINS; b++) {
+        layout[b] = (int)at;
+        at += (long long)bcount[b];
+    }
+    layout[NUM_BINS] = (int)at;
+    int *lo = layout + NUM_BINS + 1;
+    int nonempty = 0;
+    long long tot = 0;
+    for (int i = 0; i < 6; i++) {
+        nonempty += bcount[BIN_W256 + i] > 0;
+        tot += (long long)bcount[NUM_BINS + BIN_W256 + i];
+    }
+    const long long spare = cu16 - nonempty;
+    int a = 0;
+    long long acc = 0;
+    for (int i = 0; i < 6; i++) {
+        lo[i] = a;
+        if (bcount[BIN_W256 + i] > 0) {
+            const int before = tot > 0 ? (int)(spare * acc / tot) : 0;
+            acc += (long long)bcount[NUM_BINS + BIN_W256 + i];
+            const int after = tot > 0 ? (int)(spare * acc / tot) : 0;
+            a += 1 + (after - before);
+        }
+    }
+    lo[6] = a;
+}
+
+// The hub layout of the rows above 4096 entries (positions [0, nh) of the
+// sorted order), built on the device the way build_hub_layout builds the
+// vertex hubs' on the host: per hub ceil(len / HUB_CHUNK) chunks, P =
+// ceil(len / part_entries) label partitions (at most HUB_MAX_PARTS), P + 1
+// partition offsets per chunk, its entries in the partition buffer and a
+// fallback table of nextpow2(2 len) slots (at least 64).
+struct HubDev {
+    int4 *chunks, *parts, *where, *info;
+    int32_t *chunk_off, *tlog, *over;
+    int32_t *hc, *hq, *hpo;  // per hub: first chunk, first partition item, first partition offset (nh + 1)
+    int64_t *buf_base, *toff;
+    int *counts;             // {chunks, partition items, hubs}
+    int *err;                // set when a total would exceed its room (the hubs are then left out)
+    int64_t cap_hubs, cap_chunks, cap_parts, cap_poff, cap_buf, cap_slots;
+};
+
+constexpr int HL_NT = 1024;
+
+__device__ inline int ceil_log2_ll(long long x) {
+    int b = 0;
+    while ((1LL << b) < x) b++;
+    return b;
+}
+
+// inclusive scan over the workgroup (Hillis-Steele in LDS)
+__device__ inline long long hl_scan(long long x, long long *buf) {
+    const int t = threadIdx.x;
+    buf[t] = x;
+    __syncthreads();
+    for (int d = 1; d < HL_NT; d <<= 1) {
+        const long long y = t >= d ? buf[t - d] : 0;
+        __syncthreads();
+        buf[t] += y;
+        __syncthreads();
+    }
+    const long long r = buf[t];
+    __syncthreads();
+    return r;
+}
+
+struct HubSize {
+    long long nch, P, poff, len, slots;
+    int tl;
+};
+__device__ inline HubSize hub_size(long long len, int part_entries) {
+    HubSize s;
+    s.len = len;
+    s.nch = (len + HUB_CHUNK - 1) / HUB_CHUNK;
+    s.P = min((long long)HUB_MAX_PARTS, max(1LL, (len + part_entries - 1) / part_entries));
+    s.poff = s.nch * (s.P + 1);
+    s.tl = max(6, ceil_log2_ll(2 * len));
+    s.slots = 1LL << s.tl;
+    return s;
+}
+
+// one workgroup of HL_NT threads: the hubs' sizes, their prefix sums and the
+// per-hub records
+__global__ __launch_bounds__(HL_NT) void k_hub_sizes(const int *layout, const int32_t *corder, const int32_t *cind,
+                                                     int part_entries, HubDev d) {
+    __shared__ long long buf[HL_NT];
+    __shared__ int s_fits;
+    const int nh = layout[BIN_HUB + 1];
+    const int t = threadIdx.x;
+    const int per = (nh + HL_NT - 1) / HL_NT;
+    const int h0 = min(nh, t * per), h1 = min(nh, h0 + per);
+    long long s[5] = {0, 0, 0, 0, 0};
+    for (int h = h0; h < h1; h++) {
         const int l = corder[h];
-        out[h] = make_int4(l, cind[l], cind[l + 1], 0);
+        const HubSize z = hub_size(cind[l + 1] - cind[l], part_entries);
+        s[0] += z.nch;
+        s[1] += z.P;
+        s[2] += z.poff;
+        s[3] += z.len;
+        s[4] += z.slots;
+    }
+    long long ex[5], tot[5];
+    for (int i = 0; i < 5; i++) {
+        const long long inc = hl_scan(s[i], buf);
+        ex[i] = inc - s[i];
+        if (t == HL_NT - 1) buf[0] = inc;
+        __syncthreads();
+        tot[i] = buf[0];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const bool fits = nh <= d.cap_hubs && tot[0] <= d.cap_chunks && tot[1] <= d.cap_parts && tot[2] <= d.cap_poff &&
+                          tot[3] <= d.cap_buf && tot[4] <= d.cap_slots;
+        s_fits = fits;
+        d.counts[0] = fits ? (int)tot[0] : 0;
+        d.counts[1] = fits ? (int)tot[1] : 0;
+        d.counts[2] = fits ? nh : 0;
+        if (!fits) *d.err = 1;
+        if (fits) {
+            d.hc[nh] = (int)tot[0];
+            d.hq[nh] = (int)tot[1];
+            d.hpo[nh] = (int)tot[2];
+            d.buf_base[nh] = tot[3];
+            d.toff[nh] = tot[4];
+        }
+    }
+    __syncthreads();
+    if (!s_fits) return;
+    for (int h = h0; h < h1; h++) {
+        const int l = corder[h];
+        const HubSize z = hub_size(cind[l + 1] - cind[l], part_entries);
+        d.hc[h] = (int)ex[0];
+        d.hq[h] = (int)ex[1];
+        d.hpo[h] = (int)ex[2];
+        d.buf_base[h] = ex[3];
+        d.toff[h] = ex[4];
+        d.tlog[h] = z.tl;
+        d.over[h] = 0;
+        d.info[h] = make_int4(l, (int)z.P, (int)ex[1], (int)ex[0]);
+        ex[0] += z.nch;
+        ex[1] += z.P;
+        ex[2] += z.poff;
+        ex[3] += z.len;
+        ex[4] += z.slots;
+    }
+}
+
+// last h in [0, n) with a[h] <= x (a non-decreasing, a[0] <= x)
+__device__ inline int last_le(const int32_t *a, int n, int x) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// the chunk and partition items of the hubs
+__global__ void k_hub_expand(const int32_t *cind, HubDev d) {
+    const int nc = d.counts[0], np = d.counts[1], nh = d.counts[2];
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += stride) {
+        const int h = last_le(d.hc, nh, i);
+        const int4 inf = d.info[h];
+        const int c = i - d.hc[h];
+        const int begin = cind[inf.x] + c * HUB_CHUNK;
+        d.chunks[i] = make_int4(h, inf.x, begin, min(cind[inf.x + 1], begin + HUB_CHUNK));
+        d.chunk_off[i] = d.hpo[h] + c * (inf.y + 1);
+    }
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < np; j += stride) {
+        const int h = last_le(d.hq, nh, j);
+        const int4 inf = d.info[h];
+        d.parts[j] = make_int4(h, j - d.hq[h], d.hc[h], d.hc[h + 1] - d.hc[h]);
+        d.where[j] = make_int4(inf.x, inf.y, d.hpo[h], (int)d.buf_base[h]);
     }
 }
 
@@ -486,12 +679,6 @@ struct GrowBuf {
         if (grew) *grew = true;
         return PCD_OK;
     }
-    int upload(pcd_ctx *ctx, const std::vector<T> &v) {
-        PCD_TRY(reserve(ctx, v.size()));
-        if (!v.empty())
-            PCD_HIP(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, ctx->stream));
-        return PCD_OK;
-    }
 };
 
 // ------------------------------------------------ statistics after merging
@@ -506,14 +693,16 @@ __device__ inline int pair_low(int l, const int32_t *found, const int32_t *bp) {
     return p > l ? p : -1;  // the partner when l is the lower side of a pair
 }
 
-// rows of at most 64 entries, G lanes per row
+// rows of at most 64 entries (positions [layout[b0], layout[b1])), G lanes
+// per row
 template <int G>
-__global__ __launch_bounds__(256) void k_pair_links_small(int p0, int p1, const int32_t *corder, const int32_t *cind,
-                                                          const int32_t *ent, const int32_t *found, const int32_t *bp,
-                                                          int32_t *links) {
+__global__ __launch_bounds__(256) void k_pair_links_small(const int *layout, int b0, int b1, const int32_t *corder,
+                                                          const int32_t *cind, const int32_t *ent, const int32_t *found,
+                                                          const int32_t *bp, int32_t *links) {
     const int lane = __lane_id(), glane = lane & (G - 1);
     const int t = (blockIdx.x * blockDim.x + threadIdx.x) / G;
     const int nt = gridDim.x * blockDim.x / G;
+    const int p0 = layout[b0], p1 = layout[b1];
     for (int base = t; base < p1 - p0; base += nt) {
         const int l = corder[p0 + base];
         const int p = pair_low(l, found, bp);
@@ -526,15 +715,16 @@ __global__ __launch_bounds__(256) void k_pair_links_small(int p0, int p1, const 
     }
 }
 
-// longer rows: one wave per row (up to 4096 entries), or per 4096-entry
-// chunk of a hub row (summed with atomics)
-__global__ __launch_bounds__(256) void k_pair_links_rows(int p0, int p1, const int32_t *corder, const int32_t *cind,
-                                                         const int32_t *ent, const int32_t *found, const int32_t *bp,
-                                                         int32_t *links) {
+// longer rows (positions [layout[b0], layout[b1])): one wave per row (up to
+// 4096 entries), or per 4096-entry chunk of a hub row (summed with atomics)
+__global__ __launch_bounds__(256) void k_pair_links_rows(const int *layout, int b0, int b1, const int32_t *corder,
+                                                         const int32_t *cind, const int32_t *ent, const int32_t *found,
+                                                         const int32_t *bp, int32_t *links) {
     const int lane = __lane_id();
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int nwaves = gridDim.x * blockDim.x / WAVE;
-    for (int q = p0 + wave_g; q < p1; q += nwaves) {
+    const int p1 = layout[b1];
+    for (int q = layout[b0] + wave_g; q < p1; q += nwaves) {
         const int l = corder[q];
         const int p = pair_low(l, found, bp);
         if (p < 0) continue;
@@ -545,11 +735,12 @@ __global__ __launch_bounds__(256) void k_pair_links_rows(int p0, int p1, const i
     }
 }
 
-__global__ __launch_bounds__(256) void k_pair_links_hub(int nchunks, const int4 *chunks, const int32_t *ent,
+__global__ __launch_bounds__(256) void k_pair_links_hub(const int *counts, const int4 *chunks, const int32_t *ent,
                                                         const int32_t *found, const int32_t *bp, int32_t *links) {
     const int lane = __lane_id();
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int nwaves = gridDim.x * blockDim.x / WAVE;
+    const int nchunks = counts[0];
     for (int it = wave_g; it < nchunks; it += nwaves) {
         const int4 ck = chunks[it];  // {hub, community, begin, end}
         const int p = pair_low(ck.y, found, bp);
@@ -563,7 +754,9 @@ __global__ __launch_bounds__(256) void k_pair_links_hub(int nchunks, const int4 
 
 // (also clears the records' matched flags, kept in their pad words during
 // the step; the other records of a pair are never read again)
-__global__ void k_merged_stats(int k, const int32_t *found, const int32_t *bp, const int32_t *links, CommStats *st) {
+__global__ void k_merged_stats(int bound, const int *kp, const int32_t *found, const int32_t *bp, const int32_t *links,
+                               CommStats *st) {
+    const int k = k_of(bound, kp);
     for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) {
         const int p = pair_low(l, found, bp);
         if (p < 0) {
@@ -579,11 +772,15 @@ __global__ void k_merged_stats(int k, const int32_t *found, const int32_t *bp, c
 // The matched flags of a merging step on community rows live in the
 // records' pad words too (the merging kernels read the record of every
 // candidate anyway): cleared at the start, set with `found`.
-__global__ void k_clear_pads(int k, CommStats *st) {
+__global__ void k_clear_pads(int bound, const int *kp, CommStats *st) {
+    const int k = k_of(bound, kp);
     for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) st[l].pad = 0;
 }
 
-__global__ void k_handshake_rec(int k, int32_t *found, const int32_t *bp, int32_t *unmatched, CommStats *st) {
+// refinement.cpp:264-274 for the communities below k
+__global__ void k_handshake_rec(int bound, const int *kp, int32_t *found, const int32_t *bp, int32_t *unmatched,
+                                CommStats *st) {
+    const int k = k_of(bound, kp);
     int local = 0;
     for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < k; l += gridDim.x * blockDim.x) {
         if (!found[l] && (bp[l] == -1 || bp[bp[l]] == l)) {
@@ -596,18 +793,20 @@ __global__ void k_handshake_rec(int k, int32_t *found, const int32_t *bp, int32_
     if (__lane_id() == 0 && local) atomicAdd(unmatched, local);
 }
 
-// Device state of the community rows and the row set of the current step.
+// Device state of the community rows and the row set of the merging step
+// (every buffer sized for the largest step the graph allows).
 struct MergeRows {
     GrowBuf<int32_t> ent, cind, corder, dst, links;
     GrowBuf<uint64_t> mk0, mk1;
     GrowBuf<unsigned long long> bcount;
-    GrowBuf<int4> hrows;
+    GrowBuf<int> layout, counts;
     // hub layout
     GrowBuf<int4> items, parts, info, where;
-    GrowBuf<int32_t> item_off, tlog, poff, hbuf, over;
+    GrowBuf<int32_t> item_off, tlog, poff, hbuf, over, hc, hq, hpo;
     GrowBuf<int64_t> buf_base, toff;
     GrowBuf<char> partial;
     GrowBuf<int32_t> gkeys, gcnts;  // fallback tables: all slots empty between uses
+    HubDev hd{};
     RowSet rows;
 };
 
@@ -616,19 +815,100 @@ void merge_release(pcd_ctx *ctx) {
     ctx->mrows = nullptr;
 }
 
-// Builds the community rows of the current compressed labels (k
-// communities): entries, offsets, bin order and hub layout.
-static int build_community_rows(pcd_ctx *ctx, MergeRows &R) {
-    const int n = ctx->n, k = ctx->k;
-    const int64_t E = (int64_t)2 * ctx->m;
-    PCD_TRY(R.ent.reserve(ctx, (size_t)E + 1));
-    PCD_TRY(R.cind.reserve(ctx, (size_t)n + 2));
-    PCD_TRY(R.corder.reserve(ctx, (size_t)n + 2));
-    PCD_TRY(R.dst.reserve(ctx, (size_t)n + 2));
-    PCD_TRY(R.mk0.reserve(ctx, (size_t)n + 2));
-    PCD_TRY(R.mk1.reserve(ctx, (size_t)n + 2));
+// Reserves every buffer for the largest merging step of the graph (E
+// entries, n + 1 communities): rows above 4096 entries number at most
+// E / 4097; their chunks at most 2E / 4096 + 1; their partition items at
+// most E / part_entries + hubs; their partition offsets at most
+// (chunks of a row) x (P + 1) <= half its entries; their fallback tables at
+// most 4 E + 64 per hub slots.
+static int reserve_rows(pcd_ctx *ctx, MergeRows &R) {
+    const size_t n = (size_t)ctx->n, E = (size_t)2 * ctx->m;
+    const size_t pe = (size_t)std::max(64, ctx->tune.hub_part_entries);
+    const size_t hubs = std::min(n + 1, E / 4097 + 1);
+    const size_t chunks = 2 * E / HUB_CHUNK + 2, parts = E / pe + hubs + 1, poff = E / 2 + 1024;
+    const size_t slots = 4 * E + 64 * hubs + 64;
+    if (slots > (size_t)INT32_MAX * 4) return fail(PCD_E_NOMEM, "graph too large for the merging step's hub tables");
+    PCD_TRY(R.ent.reserve(ctx, E + 1));
+    PCD_TRY(R.cind.reserve(ctx, n + 3));
+    PCD_TRY(R.corder.reserve(ctx, n + 2));
+    PCD_TRY(R.dst.reserve(ctx, n + 2));
+    PCD_TRY(R.links.reserve(ctx, n + 2));
+    PCD_TRY(R.mk0.reserve(ctx, n + 2));
+    PCD_TRY(R.mk1.reserve(ctx, n + 2));
     PCD_TRY(R.bcount.reserve(ctx, 2 * NUM_BINS));
-    const int half = label_bits(k);
+    PCD_TRY(R.layout.reserve(ctx, ROW_LAYOUT_INTS));
+    bool grew_n = false;
+    PCD_TRY(R.counts.reserve(ctx, 4, &grew_n));
+    if (grew_n) PCD_TRY(fill_i32(ctx, R.counts.p, 0, (int64_t)R.counts.cap));
+    PCD_TRY(R.items.reserve(ctx, chunks));
+    PCD_TRY(R.item_off.reserve(ctx, chunks));
+    PCD_TRY(R.parts.reserve(ctx, parts));
+    PCD_TRY(R.where.reserve(ctx, parts));
+    PCD_TRY(R.partial.reserve(ctx, (size_t)HUB_PARTIAL_BYTES * parts));
+    PCD_TRY(R.poff.reserve(ctx, poff));
+    PCD_TRY(R.hbuf.reserve(ctx, E + 1));
+    PCD_TRY(R.info.reserve(ctx, hubs + 1));
+    PCD_TRY(R.tlog.reserve(ctx, hubs + 1));
+    PCD_TRY(R.over.reserve(ctx, hubs + 1));
+    PCD_TRY(R.hc.reserve(ctx, hubs + 1));
+    PCD_TRY(R.hq.reserve(ctx, hubs + 1));
+    PCD_TRY(R.hpo.reserve(ctx, hubs + 1));
+    PCD_TRY(R.buf_base.reserve(ctx, hubs + 1));
+    PCD_TRY(R.toff.reserve(ctx, hubs + 1));
+    bool grew_k = false, grew_c = false;
+    PCD_TRY(R.gkeys.reserve(ctx, slots, &grew_k));
+    PCD_TRY(R.gcnts.reserve(ctx, slots, &grew_c));
+    if (grew_k) PCD_TRY(fill_i32(ctx, R.gkeys.p, EMPTY_KEY, (int64_t)R.gkeys.cap));
+    if (grew_c) PCD_TRY(fill_i32(ctx, R.gcnts.p, 0, (int64_t)R.gcnts.cap));
+    HubDev &d = R.hd;
+    d.chunks = R.items.p;
+    d.parts = R.parts.p;
+    d.where = R.where.p;
+    d.info = R.info.p;
+    d.chunk_off = R.item_off.p;
+    d.tlog = R.tlog.p;
+    d.over = R.over.p;
+    d.hc = R.hc.p;
+    d.hq = R.hq.p;
+    d.hpo = R.hpo.p;
+    d.buf_base = R.buf_base.p;
+    d.toff = R.toff.p;
+    d.counts = R.counts.p;
+    d.err = R.counts.p + 3;
+    d.cap_hubs = (int64_t)std::min({R.info.cap, R.tlog.cap, R.over.cap, R.hc.cap, R.hq.cap, R.hpo.cap,
+                                     R.buf_base.cap, R.toff.cap}) - 1;
+    d.cap_chunks = (int64_t)std::min(R.items.cap, R.item_off.cap);
+    d.cap_parts = (int64_t)std::min(std::min(R.parts.cap, R.where.cap), R.partial.cap / HUB_PARTIAL_BYTES);
+    d.cap_poff = (int64_t)R.poff.cap;
+    d.cap_buf = (int64_t)R.hbuf.cap;
+    d.cap_slots = (int64_t)std::min(R.gkeys.cap, R.gcnts.cap);
+    RowSet &rs = R.rows;
+    rs = RowSet{};
+    rs.layout = R.layout.p;
+    rs.hub_counts = R.counts.p;
+    rs.hub_items = R.items.p;
+    rs.hub_item_off = R.item_off.p;
+    rs.hub_part_items = R.parts.p;
+    rs.hub_info = R.info.p;
+    rs.hub_part_where = R.where.p;
+    rs.hub_buf_base = R.buf_base.p;
+    rs.hub_poff = R.poff.p;
+    rs.hub_buf = R.hbuf.p;
+    rs.hub_partial = R.partial.p;
+    rs.hub_over = R.over.p;
+    rs.hub_table_off = R.toff.p;
+    rs.hub_table_log = R.tlog.p;
+    rs.hub_keys = R.gkeys.p;
+    rs.hub_cnts = R.gcnts.p;
+    return PCD_OK;
+}
+
+// Builds the community rows of the current labels (ids below `bound`, k on
+// the device when kp is set): entries, offsets, bin order and hub layout,
+// all on the stream, nothing read back.
+static int build_community_rows(pcd_ctx *ctx, MergeRows &R, int bound, const int *kp) {
+    const int n = ctx->n;
+    const int half = label_bits(bound);
     // members by label (stable: ascending vertex id inside a community)
     kt_begin(ctx, KS_MERGE_SORT);
     hipLaunchKernelGGL(k_member_keys, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_labels,
@@ -636,7 +916,7 @@ static int build_community_rows(pcd_ctx *ctx, MergeRows &R) {
     uint64_t *members = nullptr;
     PCD_TRY(radix_sort_u64(ctx, R.mk0.p, R.mk1.p, n, half, &members));
     PCD_TRY(device_scan(ctx, n, MemberDeg{members, ctx->d_indices}, MemberDst{members, R.dst.p}, nullptr));
-    PCD_TRY(device_scan(ctx, k, CommVol{ctx->d_stats}, CommStart{R.cind.p, k}, nullptr));
+    PCD_TRY(device_scan(ctx, bound, CommVol{ctx->d_stats, bound, kp}, CommStart{R.cind.p, bound}, nullptr));
     kt_end(ctx, KS_MERGE_SORT);
     // the rows: every entry's label at its member's place
     kt_begin(ctx, KS_MERGE_KEYS);
@@ -660,81 +940,22 @@ static int build_community_rows(pcd_ctx *ctx, MergeRows &R) {
     small(std::integral_constant<int, 4>{}, BIN_G4);
     kt_end(ctx, KS_MERGE_KEYS);
     PCD_HIP(hipGetLastError());
-    // bins of the community rows by length: one radix pass over 4-bit bins
+    // bins of the community rows by length (one radix pass over 4-bit bin
+    // keys), the launch layout, and the hub layout
     kt_begin(ctx, KS_MERGE_SORT);
     PCD_TRY(fill_u64(ctx, (uint64_t *)R.bcount.p, 0, 2 * NUM_BINS));
-    hipLaunchKernelGGL(k_comm_bins, dim3(grid_for(ctx, k, 256, 4)), dim3(256), 0, ctx->stream, k, R.cind.p, R.mk0.p,
-                       R.bcount.p);
+    hipLaunchKernelGGL(k_comm_bins, dim3(grid_for(ctx, bound, 256, 4)), dim3(256), 0, ctx->stream, bound, kp, R.cind.p,
+                       R.mk0.p, R.bcount.p);
     uint64_t *cs = nullptr;
-    PCD_TRY(radix_sort_u64(ctx, R.mk0.p, R.mk1.p, k, 4, &cs));
-    hipLaunchKernelGGL(k_high_words, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, cs, R.corder.p);
+    PCD_TRY(radix_sort_u64(ctx, R.mk0.p, R.mk1.p, bound, 4, &cs));
+    hipLaunchKernelGGL(k_high_words, dim3(grid_for(ctx, bound, 256)), dim3(256), 0, ctx->stream, bound, cs,
+                       R.corder.p);
+    hipLaunchKernelGGL(k_row_layout, dim3(1), dim3(64), 0, ctx->stream, R.bcount.p, ctx->cu_count * 16, R.layout.p);
+    hipLaunchKernelGGL(k_hub_sizes, dim3(1), dim3(HL_NT), 0, ctx->stream, R.layout.p, R.corder.p, R.cind.p,
+                       std::max(64, ctx->tune.hub_part_entries), R.hd);
+    hipLaunchKernelGGL(k_hub_expand, dim3(ctx->cu_count * 2), dim3(256), 0, ctx->stream, R.cind.p, R.hd);
     kt_end(ctx, KS_MERGE_SORT);
     PCD_HIP(hipGetLastError());
-    PCD_HIP(hipMemcpyAsync(ctx->h_small64, R.bcount.p, sizeof(int64_t) * 2 * NUM_BINS, hipMemcpyDeviceToHost,
-                           ctx->stream));
-    PCD_HIP(hipStreamSynchronize(ctx->stream));
-    RowSet &rs = R.rows;
-    rs = RowSet{};
-    int64_t at = 0;
-    for (int b = 0; b < NUM_BINS; b++) {
-        rs.bins.start[b] = (int)at;
-        at += ctx->h_small64[b];
-        rs.bin_entries[b] = ctx->h_small64[NUM_BINS + b];
-    }
-    rs.bins.start[NUM_BINS] = (int)at;
-    rs.bins.split2048 = rs.bins.start[BIN_B4096 + 1];
-    if (at != k) return fail(PCD_E_STATE, "community bins do not cover the k communities");
-    // rows above 4096 entries: the hub layout, built on the host
-    const int nh = rs.bins.start[BIN_HUB + 1];
-    if (nh > 0) {
-        PCD_TRY(R.hrows.reserve(ctx, (size_t)nh));
-        hipLaunchKernelGGL(k_hub_rows, dim3(grid_for(ctx, nh, 256)), dim3(256), 0, ctx->stream, nh, R.corder.p,
-                           R.cind.p, R.hrows.p);
-        PCD_HIP(hipGetLastError());
-        std::vector<int4> hr(nh);
-        PCD_HIP(hipMemcpyAsync(hr.data(), R.hrows.p, sizeof(int4) * nh, hipMemcpyDeviceToHost, ctx->stream));
-        PCD_HIP(hipStreamSynchronize(ctx->stream));
-        HubLayoutHost L;
-        PCD_TRY(build_hub_layout(hr, ctx->tune.hub_part_entries, L));
-        PCD_TRY(R.items.upload(ctx, L.chunks));
-        PCD_TRY(R.item_off.upload(ctx, L.chunk_off));
-        PCD_TRY(R.parts.upload(ctx, L.parts));
-        PCD_TRY(R.where.upload(ctx, L.where));
-        PCD_TRY(R.info.upload(ctx, L.info));
-        PCD_TRY(R.buf_base.upload(ctx, L.buf_base));
-        PCD_TRY(R.toff.upload(ctx, L.toff));
-        PCD_TRY(R.tlog.upload(ctx, L.tlog));
-        PCD_TRY(R.partial.reserve(ctx, (size_t)HUB_PARTIAL_BYTES * L.parts.size()));
-        PCD_TRY(R.poff.reserve(ctx, (size_t)L.poff_total));
-        PCD_TRY(R.hbuf.reserve(ctx, (size_t)L.buf_base[nh]));
-        PCD_TRY(R.over.reserve(ctx, (size_t)nh));
-        PCD_TRY(fill_i32(ctx, R.over.p, 0, nh));
-        const size_t gslots = (size_t)L.toff[nh];
-        bool grew_k = false, grew_c = false;
-        PCD_TRY(R.gkeys.reserve(ctx, gslots, &grew_k));
-        PCD_TRY(R.gcnts.reserve(ctx, gslots, &grew_c));
-        if (grew_k) PCD_TRY(fill_i32(ctx, R.gkeys.p, EMPTY_KEY, (int64_t)R.gkeys.cap));
-        if (grew_c) PCD_TRY(fill_i32(ctx, R.gcnts.p, 0, (int64_t)R.gcnts.cap));
-        // the host vectors must outlive their copies
-        PCD_HIP(hipStreamSynchronize(ctx->stream));
-        rs.num_hubs = nh;
-        rs.num_hub_items = (int)L.chunks.size();
-        rs.num_hub_parts = (int)L.parts.size();
-        rs.hub_items = R.items.p;
-        rs.hub_item_off = R.item_off.p;
-        rs.hub_part_items = R.parts.p;
-        rs.hub_info = R.info.p;
-        rs.hub_part_where = R.where.p;
-        rs.hub_buf_base = R.buf_base.p;
-        rs.hub_poff = R.poff.p;
-        rs.hub_buf = R.hbuf.p;
-        rs.hub_partial = R.partial.p;
-        rs.hub_over = R.over.p;
-        rs.hub_table_off = R.toff.p;
-        rs.hub_table_log = R.tlog.p;
-        rs.hub_keys = R.gkeys.p;
-        rs.hub_cnts = R.gcnts.p;
-    }
     return PCD_OK;
 }
 
@@ -762,16 +983,48 @@ static int merge_relabel(pcd_ctx *ctx, int k) {
     return PCD_OK;
 }
 
-// the partner choice over community rows (see the top of this file)
-static int merging_rows_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie) {
-    const int k = ctx->k;
+// Buffers a merging step over community rows needs, reserved ahead (a
+// captured step may not allocate).
+int merge_rows_reserve(pcd_ctx *ctx) {
+    PCD_TRY(ensure_merge_state(ctx));
     if (!ctx->mrows) ctx->mrows = new MergeRows();
     MergeRows &R = *static_cast<MergeRows *>(ctx->mrows);
-    PCD_TRY(build_community_rows(ctx, R));
-    PCD_TRY(ensure_merge_state(ctx));
-    PCD_TRY(fill_i32(ctx, ctx->d_bp, -1, k));
-    PCD_TRY(fill_i32(ctx, ctx->d_found, 0, k));
-    hipLaunchKernelGGL(k_clear_pads, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_stats);
+    PCD_TRY(reserve_rows(ctx, R));
+    PCD_TRY(ensure_scan_parts(ctx, (int64_t)ctx->n + 2));
+    PCD_TRY(ensure_sort_space(ctx, (int64_t)ctx->n + 2));
+    return PCD_OK;
+}
+
+// Fails if a merging step queued since the last check left hubs out (a hub
+// layout larger than its reserved room; reached only beyond the bounds of
+// reserve_rows).
+int merge_rows_check(pcd_ctx *ctx) {
+    MergeRows *R = static_cast<MergeRows *>(ctx->mrows);
+    if (!R || !R->counts.p) return PCD_OK;
+    PCD_HIP(hipMemcpyAsync(ctx->h_small + 9, R->counts.p + 3, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    PCD_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_small[9] != 0) {
+        PCD_TRY(fill_i32(ctx, R->counts.p + 3, 0, 1));
+        return fail(PCD_E_NOMEM, "merging step: hub layout exceeds its reserved room");
+    }
+    return PCD_OK;
+}
+
+// the partner choice over community rows (see the top of this file); k on
+// the device while a pass is captured or k is deferred
+static int merging_rows_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t seed, int step, int tie) {
+    const bool dev_k = ctx->capturing || ctx->k_deferred;
+    const int bound = dev_k ? ctx->n + 1 : ctx->k;
+    const int *kp = dev_k ? ctx->d_k : nullptr;
+    if (!ctx->capturing) PCD_TRY(merge_rows_reserve(ctx));
+    if (!ctx->mrows) return fail(PCD_E_STATE, "merging rows not reserved before a capture");
+    MergeRows &R = *static_cast<MergeRows *>(ctx->mrows);
+    if (bound <= 0) return PCD_OK;
+    PCD_TRY(build_community_rows(ctx, R, bound, kp));
+    PCD_TRY(fill_i32(ctx, ctx->d_bp, -1, bound));
+    PCD_TRY(fill_i32(ctx, ctx->d_found, 0, bound));
+    hipLaunchKernelGGL(k_clear_pads, dim3(grid_for(ctx, bound, 256)), dim3(256), 0, ctx->stream, bound, kp,
+                       ctx->d_stats);
     LPArgs a{};
     a.indices = R.cind.p;  // community l's row: entries [cind[l], cind[l+1]) of ent
     a.adj = R.ent.p;
@@ -781,6 +1034,7 @@ static int merging_rows_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t se
     a.found = ctx->d_found;
     a.rule = make_rule(ctx, rule);
     a.step = step;
+    a.step_ptr = ctx->graph_step;  // a replayed pass reads its step (or -1) there
     a.tie = tie;
     int32_t *unmatched = ctx->d_counts + NUM_BINS;
     for (int rnd = 0; rnd < rounds; rnd++) {
@@ -788,47 +1042,45 @@ static int merging_rows_dev(pcd_ctx *ctx, pcd_rule rule, int rounds, uint64_t se
         PCD_TRY(merge_rows_launch(ctx, R.rows, a));
         kt_begin(ctx, KS_MERGE_ROUNDS);
         PCD_TRY(fill_i32(ctx, unmatched, 0, 1));
-        hipLaunchKernelGGL(k_handshake_rec, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_found,
-                           ctx->d_bp, unmatched, ctx->d_stats);
+        hipLaunchKernelGGL(k_handshake_rec, dim3(grid_for(ctx, bound, 256)), dim3(256), 0, ctx->stream, bound, kp,
+                           ctx->d_found, ctx->d_bp, unmatched, ctx->d_stats);
         kt_end(ctx, KS_MERGE_ROUNDS);
         PCD_HIP(hipGetLastError());
-        if (rnd + 1 < rounds) {
+        if (rnd + 1 < rounds && !ctx->capturing) {
             // refinement.cpp:220,264-274: stop once every community is matched
+            // (a captured pass runs every round: once all are matched, a
+            // round changes nothing)
             PCD_HIP(hipMemcpyAsync(ctx->h_small, unmatched, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
             PCD_HIP(hipStreamSynchronize(ctx->stream));
             if (ctx->h_small[0] == 0) break;
         }
     }
     const bool follow = ctx->tune.inc_stats != 0;
-    if (!follow)  // the records stay as they are for a caller that reads them
-        hipLaunchKernelGGL(k_clear_pads, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_stats);
     if (follow) {
         // the statistics of the merged labels, from the records and the
         // pairs' links (the next compression renumbers them)
         kt_begin(ctx, KS_MERGE_STATS);
-        const RowSet &rs = R.rows;
-        PCD_TRY(R.links.reserve(ctx, (size_t)k + 1));
-        if (rs.num_hubs > 0) {
-            PCD_TRY(fill_i32(ctx, R.links.p, 0, k));
-            hipLaunchKernelGGL(k_pair_links_hub, dim3(grid_for(ctx, rs.num_hub_items, 4, 8)), dim3(256), 0,
-                               ctx->stream, rs.num_hub_items, rs.hub_items, R.ent.p, ctx->d_found, ctx->d_bp,
-                               R.links.p);
-        }
-        const int q0 = rs.bins.start[BIN_B4096], q1 = rs.bins.start[BIN_G64];
-        if (q1 > q0)
-            hipLaunchKernelGGL(k_pair_links_rows, dim3(grid_for(ctx, q1 - q0, 4, 8)), dim3(256), 0, ctx->stream, q0,
-                               q1, R.corder.p, R.cind.p, R.ent.p, ctx->d_found, ctx->d_bp, R.links.p);
-        const int s0 = rs.bins.start[BIN_G64], s1 = rs.bins.start[NUM_BINS];
-        if (s1 > s0)
-            hipLaunchKernelGGL(k_pair_links_small<8>, dim3(grid_for(ctx, s1 - s0, 32, 8)), dim3(256), 0, ctx->stream,
-                               s0, s1, R.corder.p, R.cind.p, R.ent.p, ctx->d_found, ctx->d_bp, R.links.p);
-        hipLaunchKernelGGL(k_merged_stats, dim3(grid_for(ctx, k, 256)), dim3(256), 0, ctx->stream, k, ctx->d_found,
-                           ctx->d_bp, R.links.p, ctx->d_stats);
+        PCD_TRY(fill_i32(ctx, R.links.p, 0, bound));
+        hipLaunchKernelGGL(k_pair_links_hub, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, R.counts.p, R.items.p,
+                           R.ent.p, ctx->d_found, ctx->d_bp, R.links.p);
+        hipLaunchKernelGGL(k_pair_links_rows, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, R.layout.p,
+                           (int)BIN_B4096, (int)BIN_G64, R.corder.p, R.cind.p, R.ent.p, ctx->d_found, ctx->d_bp,
+                           R.links.p);
+        hipLaunchKernelGGL(k_pair_links_small<8>, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, R.layout.p,
+                           (int)BIN_G64, (int)NUM_BINS, R.corder.p, R.cind.p, R.ent.p, ctx->d_found, ctx->d_bp,
+                           R.links.p);
+        hipLaunchKernelGGL(k_merged_stats, dim3(grid_for(ctx, bound, 256)), dim3(256), 0, ctx->stream, bound, kp,
+                           ctx->d_found, ctx->d_bp, R.links.p, ctx->d_stats);
         kt_end(ctx, KS_MERGE_STATS);
         PCD_HIP(hipGetLastError());
+    } else {  // the records stay as they are for a caller that reads them
+        hipLaunchKernelGGL(k_clear_pads, dim3(grid_for(ctx, bound, 256)), dim3(256), 0, ctx->stream, bound, kp,
+                           ctx->d_stats);
     }
-    PCD_TRY(merge_relabel(ctx, k));
+    PCD_TRY(merge_relabel(ctx, bound));
     ctx->stats_pending = follow;
+    // a hub layout that did not fit its room would have left hubs out
+    if (!ctx->capturing) PCD_TRY(merge_rows_check(ctx));
     return PCD_OK;
 }
 

@@ -163,18 +163,27 @@ struct RsStore {
     __device__ void operator()(int64_t i, int ex, int) const { h[i] = ex; }
 };
 
+// the tile histograms of a three-launch sort of n keys
+int ensure_sort_space(pcd_ctx *ctx, int64_t n) {
+    const int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
+    const size_t need = (size_t)ntiles * 256;
+    if (2 * need > ctx->rs_hist_cap) {
+        if (ctx->d_rs_hist) dev_free(ctx, ctx->d_rs_hist);
+        ctx->d_rs_hist = nullptr;
+        ctx->rs_hist_cap = 0;
+        PCD_TRY(dev_alloc_t(ctx, &ctx->d_rs_hist, 2 * need));
+        ctx->rs_hist_cap = 2 * need;
+    }
+    return PCD_OK;
+}
+
 // Sorts n keys whose significant bits are [0, bits) with the three-launch
 // passes above (tile histograms, a scan of all (digit, tile) counters, the
 // scatter).  Ping-pongs between src and dst from pass `shift0` on.
 static int lsd_passes(pcd_ctx *ctx, uint64_t *src, uint64_t *dst, int64_t n, int shift0, int bits, uint64_t **out) {
     int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
     size_t need = (size_t)ntiles * 256;
-    if (2 * need > ctx->rs_hist_cap) {
-        if (ctx->d_rs_hist) dev_free(ctx, ctx->d_rs_hist);
-        ctx->d_rs_hist = nullptr;
-        PCD_TRY(dev_alloc_t(ctx, &ctx->d_rs_hist, 2 * need));
-        ctx->rs_hist_cap = 2 * need;
-    }
+    PCD_TRY(ensure_sort_space(ctx, n));
     int32_t *raw = ctx->d_rs_hist + need;
     for (int shift = shift0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, ctx->stream, src, n, shift, ntiles,
@@ -358,7 +367,9 @@ int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int b
     *out = keys;
     if (n <= 1) return PCD_OK;
     const int passes = (bits + 7) / 8;
-    if (!ctx->tune.onesweep || n >= (int64_t)OS_VAL || passes > OS_MAX_PASSES)
+    // (a captured sort takes the three-launch passes: the one-sweep passes
+    // read an error flag back after each pass)
+    if (!ctx->tune.onesweep || ctx->capturing || n >= (int64_t)OS_VAL || passes > OS_MAX_PASSES)
         return lsd_passes(ctx, keys, tmp, n, 0, bits, out);
     const int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
     // status words, then 2 x 256 x passes histogram / bases, tickets, error flag

@@ -99,6 +99,8 @@ constexpr unsigned RK_MERGE_REC = RK_MERGE | RK_REC;
 template <unsigned KM>
 __device__ inline bool lp_active(const LPArgs &a, int v) {
     if constexpr ((KM & RK_MERGE) != 0) {
+        // a replayed merging pass whose loop has stopped (step -1) merges nothing
+        if (a.step_ptr && *a.step_ptr < 0) return false;
         // refinement.cpp:224-226: matched communities, and those whose chosen
         // partner is still unmatched, sit the round out
         if (a.found[v]) return false;
@@ -575,6 +577,8 @@ __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
 struct SmallBins {
     RangeSrc src[6];  // W256, G64, G32, G16, G8, G4
     int lo[7];        // workgroup range of bin b: [lo[b], lo[b + 1])
+    const int *dlo;   // the same seven bounds in device memory (bins laid out on the device), or nullptr
+    __device__ int lo_of(int b) const { return dlo ? dlo[b] : lo[b]; }
 };
 
 // WPE: waves per SIMD the registers are allocated for (1: the compiler's
@@ -591,11 +595,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     __shared__ int s_nocc[256 / WAVE];
     __shared__ int s_list[256 / WAVE][WAVE];
     int b = 0;
-    while (b < 6 && !(sb.lo[b] <= (int)blockIdx.x && (int)blockIdx.x < sb.lo[b + 1])) b++;
+    while (b < 6 && !(sb.lo_of(b) <= (int)blockIdx.x && (int)blockIdx.x < sb.lo_of(b + 1))) b++;
     if (b == 6) return;
     const int wpb = blockDim.x / WAVE, w = threadIdx.x / WAVE;
-    const int wave_g = __builtin_amdgcn_readfirstlane(((int)blockIdx.x - sb.lo[b]) * wpb + w);
-    const int nwaves = (sb.lo[b + 1] - sb.lo[b]) * wpb;
+    const int b0 = sb.lo_of(b);
+    const int wave_g = __builtin_amdgcn_readfirstlane(((int)blockIdx.x - b0) * wpb + w);
+    const int nwaves = (sb.lo_of(b + 1) - b0) * wpb;
     int *list = s_list[w];
     switch (b) {
     case 0: lp_wave<512, KM, RangeSrc, false, 4>(sb.src[0], wave_g, nwaves, s_keys[w], s_cnts[w], nullptr, s_occ[w],
@@ -673,7 +678,17 @@ struct HubArgs {
     int32_t *over_count;  // partitions that overflowed (diagnostic)
     int nchunks, nparts, nhubs;
     int table_slots;  // LDS slots per partition (multiple of 64, <= HUB_TB)
+    const int *counts;  // device {nchunks, nparts, nhubs} (a layout built on the device), or nullptr
 };
+
+__device__ inline HubArgs hub_args(HubArgs h) {
+    if (h.counts) {
+        h.nchunks = h.counts[0];
+        h.nparts = h.counts[1];
+        h.nhubs = h.counts[2];
+    }
+    return h;
+}
 
 // Partition of a label: murmur3's 32-bit finaliser, independent of the
 // multiplicative slot hash, so one partition's keys still spread over the
@@ -694,7 +709,8 @@ static_assert(HUB1_NT * HUB1_PER == HUB_CHUNK, "hub chunk layout");
 // pass 1: gather the chunk's labels (with AUX its {label, aux} records) and
 // group them by partition; a merging row leaves its own label out
 template <unsigned KM, bool AUX>
-__global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs h) {
+__global__ __launch_bounds__(HUB1_NT) void k_hub_partition(LPArgs a, HubArgs hin) {
+    const HubArgs h = hub_args(hin);
     constexpr bool MERGE = (KM & RK_MERGE) != 0;
     using Rec = typename std::conditional<AUX, int2, int32_t>::type;
     Rec *out = reinterpret_cast<Rec *>(h.buf);
@@ -788,7 +804,8 @@ __device__ inline bool tclaim(int *keys, int *cnts, int *auxs, int T, int key, i
 // no workgroup barrier), and the next partition item is loaded while the
 // current one is counted.
 template <unsigned KM, int NT, int TB, bool AUX>
-__global__ __launch_bounds__(NT) void k_hub_count_listed(LPArgs a, HubArgs h) {
+__global__ __launch_bounds__(NT) void k_hub_count_listed(LPArgs a, HubArgs hin) {
+    const HubArgs h = hub_args(hin);
     constexpr int PF = 4;
     constexpr int NW = NT / WAVE;
     using Rec = typename std::conditional<AUX, int2, int32_t>::type;
@@ -902,7 +919,8 @@ __global__ __launch_bounds__(NT) void k_hub_count_listed(LPArgs a, HubArgs h) {
 
 // pass 3: one wave per hub combines its partitions and decides
 template <unsigned KM, bool AUX>
-__global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
+__global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs hin) {
+    const HubArgs h = hub_args(hin);
     const int lane = __lane_id();
     const int wave_g = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const int nwaves = gridDim.x * blockDim.x / WAVE;
@@ -942,8 +960,9 @@ __global__ __launch_bounds__(256) void k_hub_final(LPArgs a, HubArgs h) {
 __device__ inline uint32_t gslot(int key, int log_t) { return ((uint32_t)key * 0x9E3779B1u) >> (32 - log_t); }
 
 template <unsigned KM>
-__global__ __launch_bounds__(512) void k_hub_global_insert(LPArgs a, HubArgs h, const int64_t *toff,
+__global__ __launch_bounds__(512) void k_hub_global_insert(LPArgs a, HubArgs hin, const int64_t *toff,
                                                            const int32_t *tlog, int32_t *gkeys, int32_t *gcnts) {
+    const HubArgs h = hub_args(hin);
     for (int it = blockIdx.x; it < h.nchunks; it += gridDim.x) {
         const int4 ck = h.chunks[it];
         if (!h.over[ck.x]) continue;
@@ -965,8 +984,9 @@ __global__ __launch_bounds__(512) void k_hub_global_insert(LPArgs a, HubArgs h, 
 }
 
 template <unsigned KM>
-__global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs h, const int64_t *toff,
+__global__ __launch_bounds__(1024) void k_hub_global_decide(LPArgs a, HubArgs hin, const int64_t *toff,
                                                             const int32_t *tlog, int32_t *gkeys, int32_t *gcnts) {
+    const HubArgs h = hub_args(hin);
     __shared__ Cand s_c[1024 / WAVE];
     __shared__ int s_i[2 * 1024 / WAVE];
     for (int hub = blockIdx.x; hub < h.nhubs; hub += gridDim.x) {
@@ -1050,10 +1070,19 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         if (ctx->counters && !MERGE) x.counters = ctx->d_bin_counters + 3 * bin;
         return x;
     };
+    // bins laid out on the device: every bin is launched with a fixed grid
+    // and reads its range there
+    const bool dev = rs.layout != nullptr;
     auto range = [&](int bin, int &p0, int &p1) {
-        p0 = b.start[bin];
-        p1 = b.start[bin + 1];
-        return p1 > p0;
+        p0 = dev ? 0 : b.start[bin];
+        p1 = dev ? 0 : b.start[bin + 1];
+        return dev || p1 > p0;
+    };
+    auto src = [&](int bin, int q0, int q1, int per) {
+        return RangeSrc{q0, q1, per, args_for(bin), dev ? rs.layout + bin : nullptr};
+    };
+    auto grid = [&](int64_t items, int per_block, int max_per_cu) {
+        return dev ? ctx->cu_count * max_per_cu : grid_for(ctx, items, per_block, max_per_cu);
     };
     // {label, aux} records are read by the hub path and, when tuned so, by
     // the small bins; rules without aux read the statistics instead
@@ -1084,7 +1113,7 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         kt_end(ctx, KS_LP_SELECT);
     }
     int p0, p1;
-    if (rs.num_hubs > 0) {
+    if (rs.num_hubs > 0 || rs.hub_counts) {
         HubArgs h;
         h.chunks = rs.hub_items;
         h.chunk_off = rs.hub_item_off;
@@ -1100,15 +1129,20 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         h.nchunks = rs.num_hub_items;
         h.nparts = rs.num_hub_parts;
         h.nhubs = rs.num_hubs;
+        h.counts = rs.hub_counts;
+        const bool hdev = rs.hub_counts != nullptr;
+        auto hgrid = [&](int64_t items, int per_block, int max_per_cu) {
+            return hdev ? ctx->cu_count * max_per_cu : grid_for(ctx, items, per_block, max_per_cu);
+        };
         // the LDS table holds HUB_TB slots: never index past it
         h.table_slots = std::min(HUB_TB, std::max(64, ctx->tune.hub_table_slots & ~63));
         LPArgs ah = args_for(BIN_HUB);
         kt_begin(ctx, KS_LP_HUB_INSERT);
         if (hub_aux)
-            hipLaunchKernelGGL((k_hub_partition<KM, true>), dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
+            hipLaunchKernelGGL((k_hub_partition<KM, true>), dim3(hgrid(h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
                                ctx->stream, ah, h);
         else
-            hipLaunchKernelGGL((k_hub_partition<KM, false>), dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
+            hipLaunchKernelGGL((k_hub_partition<KM, false>), dim3(hgrid(h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
                                ctx->stream, ah, h);
         kt_end(ctx, KS_LP_HUB_INSERT);
         kt_begin(ctx, KS_LP_HUB_DECIDE);
@@ -1116,30 +1150,30 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         // threads with 2048 slots (seven per CU; five with aux records)
         if (hub_aux) {
             if (ctx->tune.hub_nt == 256)
-                hipLaunchKernelGGL((k_hub_count_listed<KM, 256, 2048, true>), dim3(grid_for(ctx, h.nparts, 1, 5)),
+                hipLaunchKernelGGL((k_hub_count_listed<KM, 256, 2048, true>), dim3(hgrid(h.nparts, 1, 5)),
                                    dim3(256), 0, ctx->stream, ah, h);
             else
-                hipLaunchKernelGGL((k_hub_count_listed<KM, 512, HUB_TB, true>), dim3(grid_for(ctx, h.nparts, 1, 2)),
+                hipLaunchKernelGGL((k_hub_count_listed<KM, 512, HUB_TB, true>), dim3(hgrid(h.nparts, 1, 2)),
                                    dim3(512), 0, ctx->stream, ah, h);
             kt_end(ctx, KS_LP_HUB_DECIDE);
             kt_begin(ctx, KS_LP_HUB_FINAL);
-            hipLaunchKernelGGL((k_hub_final<KM, true>), dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0, ctx->stream,
+            hipLaunchKernelGGL((k_hub_final<KM, true>), dim3(hgrid(h.nhubs, 4, 4)), dim3(256), 0, ctx->stream,
                                ah, h);
         } else {
             if (ctx->tune.hub_nt == 256)
-                hipLaunchKernelGGL((k_hub_count_listed<KM, 256, 2048, false>), dim3(grid_for(ctx, h.nparts, 1, 7)),
+                hipLaunchKernelGGL((k_hub_count_listed<KM, 256, 2048, false>), dim3(hgrid(h.nparts, 1, 7)),
                                    dim3(256), 0, ctx->stream, ah, h);
             else
-                hipLaunchKernelGGL((k_hub_count_listed<KM, 512, HUB_TB, false>), dim3(grid_for(ctx, h.nparts, 1, 3)),
+                hipLaunchKernelGGL((k_hub_count_listed<KM, 512, HUB_TB, false>), dim3(hgrid(h.nparts, 1, 3)),
                                    dim3(512), 0, ctx->stream, ah, h);
             kt_end(ctx, KS_LP_HUB_DECIDE);
             kt_begin(ctx, KS_LP_HUB_FINAL);
-            hipLaunchKernelGGL((k_hub_final<KM, false>), dim3(grid_for(ctx, h.nhubs, 4, 4)), dim3(256), 0,
+            hipLaunchKernelGGL((k_hub_final<KM, false>), dim3(hgrid(h.nhubs, 4, 4)), dim3(256), 0,
                                ctx->stream, ah, h);
         }
-        hipLaunchKernelGGL(k_hub_global_insert<KM>, dim3(grid_for(ctx, h.nchunks, 1, 4)), dim3(512), 0, ctx->stream,
+        hipLaunchKernelGGL(k_hub_global_insert<KM>, dim3(hgrid(h.nchunks, 1, 4)), dim3(512), 0, ctx->stream,
                            ah, h, rs.hub_table_off, rs.hub_table_log, rs.hub_keys, rs.hub_cnts);
-        hipLaunchKernelGGL(k_hub_global_decide<KM>, dim3(grid_for(ctx, h.nhubs, 1, 1)), dim3(1024), 0, ctx->stream, ah,
+        hipLaunchKernelGGL(k_hub_global_decide<KM>, dim3(hgrid(h.nhubs, 1, 1)), dim3(1024), 0, ctx->stream, ah,
                            h, rs.hub_table_off, rs.hub_table_log, rs.hub_keys, rs.hub_cnts);
         kt_end(ctx, KS_LP_HUB_FINAL);
     }
@@ -1147,11 +1181,11 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         // degrees 2049-4096: 512-thread workgroups with 8192-slot tables (two
         // per CU; a 12-byte-slot table of 8192 entries would leave one);
         // 1025-2048: 256-thread workgroups with 4096-slot tables (four per CU)
-        const int ps = ctx->tune.b4096_split && !MERGE ? std::min(std::max(b.split2048, p0), p1) : p1;
+        const int ps = ctx->tune.b4096_split && !MERGE && !dev ? std::min(std::max(b.split2048, p0), p1) : p1;
         kt_begin(ctx, KS_LP_B4096);
-        if (ps > p0)
-            hipLaunchKernelGGL((k_lp_block<512, 8192, KM, RangeSrc>), dim3(grid_for(ctx, ps - p0, 1, 2)), dim3(512),
-                               0, ctx->stream, RangeSrc{p0, ps, 1, args_for(BIN_B4096)});
+        if (ps > p0 || dev)
+            hipLaunchKernelGGL((k_lp_block<512, 8192, KM, RangeSrc>), dim3(grid(ps - p0, 1, 2)), dim3(512), 0,
+                               ctx->stream, src(BIN_B4096, p0, ps, 1));
         if (p1 > ps)
             hipLaunchKernelGGL((k_lp_block<256, 4096, KM, RangeSrc>), dim3(grid_for(ctx, p1 - ps, 1, 4)), dim3(256),
                                0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B4096)});
@@ -1161,11 +1195,11 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
     if (range(BIN_B1024, p0, p1)) {
         kt_begin(ctx, KS_LP_B1024);
         if (block_aux)
-            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - p0, 1, 8)),
-                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
+            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc, true>), dim3(grid(p1 - p0, 1, 8)), dim3(256), 0,
+                               ctx->stream, src(BIN_B1024, p0, p1, 1));
         else
-            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256),
-                               0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B1024)});
+            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid(p1 - p0, 1, 8)), dim3(256), 0,
+                               ctx->stream, src(BIN_B1024, p0, p1, 1));
         kt_end(ctx, KS_LP_B1024);
     }
     // degrees 257-512: one wave per vertex with a 1024-slot table, so a CU
@@ -1174,17 +1208,17 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
     if (range(BIN_B512, p0, p1)) {
         kt_begin(ctx, KS_LP_B512);
         if (!ctx->tune.b1024_wave)
-            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 1, 8)), dim3(256),
-                               0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B512)});
+            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid(p1 - p0, 1, 8)), dim3(256), 0,
+                               ctx->stream, src(BIN_B512, p0, p1, 1));
         else if (block_aux)
-            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - p0, 4, 3)), dim3(256),
-                               0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B512)});
+            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, true>), dim3(grid(p1 - p0, 4, 3)), dim3(256), 0,
+                               ctx->stream, src(BIN_B512, p0, p1, 1));
         else if (ctx->tune.b1024_u16)  // 16-bit counts (degree <= 512): 28.7 KB, five workgroups per CU
-            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8, uint16_t>), dim3(grid_for(ctx, p1 - p0, 4, 5)),
-                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B512)});
+            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8, uint16_t>), dim3(grid(p1 - p0, 4, 5)),
+                               dim3(256), 0, ctx->stream, src(BIN_B512, p0, p1, 1));
         else  // 4-8 entries per lane: all eight gathers in flight at once
-            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8>), dim3(grid_for(ctx, p1 - p0, 4, 4)),
-                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_B512)});
+            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8>), dim3(grid(p1 - p0, 4, 4)), dim3(256), 0,
+                               ctx->stream, src(BIN_B512, p0, p1, 1));
         kt_end(ctx, KS_LP_B512);
     }
     // W256 and G64..G4 in one launch unless the wave / workgroup bins gather
@@ -1193,22 +1227,22 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
     if (!fused && range(BIN_W256, p0, p1)) {
         kt_begin(ctx, KS_LP_W256);
         if (block_aux)
-            hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc, true>), dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256),
-                               0, ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_W256)});
+            hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc, true>), dim3(grid(p1 - p0, 4, 8)), dim3(256), 0,
+                               ctx->stream, src(BIN_W256, p0, p1, 1));
         else
-            hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4, 8)), dim3(256), 0,
-                               ctx->stream, RangeSrc{p0, p1, 1, args_for(BIN_W256)});
+            hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc>), dim3(grid(p1 - p0, 4, 8)), dim3(256), 0, ctx->stream,
+                               src(BIN_W256, p0, p1, 1));
         kt_end(ctx, KS_LP_W256);
     }
 #define PCD_SMALL(G, CP, BIN, SLOT)                                                                            \
     if (range(BIN, p0, p1)) {                                                                                  \
         kt_begin(ctx, SLOT);                                                                                         \
         if (small_aux)                                                                                         \
-            hipLaunchKernelGGL((k_lp_small<G, CP, true, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)),  \
-                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, CP, args_for(BIN)});                \
+            hipLaunchKernelGGL((k_lp_small<G, CP, true, KM, RangeSrc>), dim3(grid(p1 - p0, 4 * CP, 16)),           \
+                               dim3(256), 0, ctx->stream, src(BIN, p0, p1, CP));                                \
         else                                                                                                   \
-            hipLaunchKernelGGL((k_lp_small<G, CP, false, KM, RangeSrc>), dim3(grid_for(ctx, p1 - p0, 4 * CP, 16)), \
-                               dim3(256), 0, ctx->stream, RangeSrc{p0, p1, CP, args_for(BIN)});                \
+            hipLaunchKernelGGL((k_lp_small<G, CP, false, KM, RangeSrc>), dim3(grid(p1 - p0, 4 * CP, 16)),          \
+                               dim3(256), 0, ctx->stream, src(BIN, p0, p1, CP));                                \
         kt_end(ctx, SLOT);                                                                                     \
     }
     if (fused) {
@@ -1220,11 +1254,12 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         int nonempty = 0;
         for (int i = 0; i < 6; i++) {
             range(BIN_W256 + i, p0, p1);
-            sb.src[i] = RangeSrc{p0, p1, cps[i], args_for(BIN_W256 + i)};
+            sb.src[i] = src(BIN_W256 + i, p0, p1, cps[i]);
             tot += rs.bin_entries[BIN_W256 + i];
             nonempty += p1 > p0;
         }
-        if (nonempty > 0) {
+        sb.dlo = dev ? rs.layout + NUM_BINS + 1 : nullptr;
+        if (nonempty > 0 || dev) {
             // one workgroup per non-empty bin, the rest by entries
             const int spare = ctx->cu_count * 16 - nonempty;
             int at = 0;
@@ -1240,6 +1275,7 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
                 }
             }
             sb.lo[6] = at;
+            if (dev) at = ctx->cu_count * 16;  // the split the device computed covers this grid
             kt_begin(ctx, KS_LP_SMALL);
             // modularity and CPM builds (propagation or merging) at 8 waves per SIMD
             constexpr bool ONE_KIND = (KM & RK_ALL) == RK_MOD || (KM & RK_ALL) == RK_CPM;

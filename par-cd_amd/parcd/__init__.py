@@ -565,8 +565,9 @@ class Context:
         return sc.value, it.value, k.value
 
     def iterate_many(self, job_, it, count, prev_score, strategy=0, rounds=1):
-        """pcd_iterate_many: passes it .. it+count-1 (device-resident for
-        modularity and CPM with -s 0): (passes run, score, k, done)."""
+        """pcd_iterate_many: passes it .. it+count-1 (device-resident once the
+        statistics follow the labels, for every rule and schedule): (passes
+        run, score, k, done)."""
         p, sc, k, d = C.c_int(), C.c_double(), C.c_int(), C.c_int()
         check(lib().pcd_iterate_many(self._ctx, C.byref(job_), strategy, rounds, int(it), int(count),
                                      float(prev_score), C.byref(p), C.byref(sc), C.byref(k), C.byref(d)))
