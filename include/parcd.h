@@ -248,13 +248,17 @@ int pcd_iterate(pcd_ctx *ctx, const pcd_job *job, int strategy, int rounds, int 
  * (main.cpp:241-242, -n, the iteration cap) fires.  prev_score is the score
  * of pass it-1 (0 for it == 0).  *passes receives the number of passes run,
  * *score and *num_labels those of the last pass run, *done whether the rule
- * fired.  Modularity and CPM with -s 0 run device-resident once the
- * statistics follow the labels: one pass is captured as a hipGraph (stop
- * rule, pass number and step number on the device) and replayed, and the
+ * fired.  Once the statistics follow the labels (after the first pass on
+ * labels from outside), the passes run device-resident: one propagation pass
+ * and one merging pass are captured as hipGraphs (stop rule, pass number and
+ * step number on the device) and replayed in the order of the schedule
+ * (-s 0 propagates, -s 1 merges, -s 2 alternates from propagation), and the
  * host reads a small state record every few passes instead of a score per
- * pass.  The labels and statistics of the last pass are then left in the
- * compressed form pcd_iterate's next compression would give (the same
- * partition).  Other rules and strategies run pcd_iterate per pass. */
+ * pass.  Every rule (WCC once the graph's triangles are counted).  The labels
+ * and statistics of the last pass are then left in the compressed form
+ * pcd_iterate's next compression would give (the same partition).  With the
+ * tunables graph_loop 0, inc_stats 0 or (for -s 1/2) merge_sort 1, every
+ * pass runs as pcd_iterate. */
 int pcd_iterate_many(pcd_ctx *ctx, const pcd_job *job, int strategy, int rounds, int it, int count,
                      double prev_score, int *passes, double *score, int *num_labels, int *done);
 
