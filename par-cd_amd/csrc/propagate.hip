@@ -574,12 +574,17 @@ __global__ __launch_bounds__(256) void k_lp_wave(Src src) {
 // bins' ramps and tails overlap instead of following one another.  The
 // W256 waves keep their 512-slot tables; the LDS of a workgroup is that
 // of a W256 workgroup either way.
-struct SmallBins {
-    RangeSrc src[6];  // W256, G64, G32, G16, G8, G4
+template <class Src>
+struct SmallBinsT {
+    Src src[6];       // W256, G64, G32, G16, G8, G4
     int lo[7];        // workgroup range of bin b: [lo[b], lo[b + 1])
-    const int *dlo;   // the same seven bounds in device memory (bins laid out on the device), or nullptr
-    __device__ int lo_of(int b) const { return dlo ? dlo[b] : lo[b]; }
+    const int *dlo;   // with DevRangeSrc: the seven bounds in device memory (laid out on the device)
+    __device__ int lo_of(int b) const {
+        if constexpr (std::is_same<Src, DevRangeSrc>::value) return dlo[b];
+        else return lo[b];
+    }
 };
+using SmallBins = SmallBinsT<RangeSrc>;
 
 // WPE: waves per SIMD the registers are allocated for (1: the compiler's
 // choice).  At 8 the modularity and CPM builds fit 64 VGPRs without scratch;
@@ -587,8 +592,8 @@ struct SmallBins {
 // gather {label, aux} records (one 8-byte gather brings a neighbour's label
 // and its community's total, one dependent load fewer per row); W256 keeps
 // gathering labels.
-template <unsigned KM, int WPE = 1, bool SAUX = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_lp_small_all(SmallBins sb) {
+template <unsigned KM, int WPE = 1, bool SAUX = false, class Src = RangeSrc>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_lp_small_all(SmallBinsT<Src> sb) {
     __shared__ int s_keys[256 / WAVE][512];
     __shared__ int s_cnts[256 / WAVE][512];
     __shared__ uint16_t s_occ[256 / WAVE][256];
@@ -603,8 +608,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const int nwaves = (sb.lo_of(b + 1) - b0) * wpb;
     int *list = s_list[w];
     switch (b) {
-    case 0: lp_wave<512, KM, RangeSrc, false, 4>(sb.src[0], wave_g, nwaves, s_keys[w], s_cnts[w], nullptr, s_occ[w],
-                                                 &s_nocc[w]); break;
+    case 0: lp_wave<512, KM, Src, false, 4>(sb.src[0], wave_g, nwaves, s_keys[w], s_cnts[w], nullptr, s_occ[w],
+                                            &s_nocc[w]); break;
     case 1: lp_small<64, 8, SAUX, KM>(sb.src[1], wave_g, nwaves, list); break;
     case 2: lp_small<32, 16, SAUX, KM>(sb.src[2], wave_g, nwaves, list); break;
     case 3: lp_small<16, 32, SAUX, KM>(sb.src[3], wave_g, nwaves, list); break;
@@ -1070,16 +1075,20 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         if (ctx->counters && !MERGE) x.counters = ctx->d_bin_counters + 3 * bin;
         return x;
     };
-    // bins laid out on the device: every bin is launched with a fixed grid
-    // and reads its range there
+    // Bins laid out on the device (the merging rows): every bin is launched
+    // with a fixed grid and reads its range there; the propagation rows'
+    // bins are host ranges.
     const bool dev = rs.layout != nullptr;
+    if (dev != MERGE) return fail(PCD_E_STATE, "row set layout does not match the kernel build");
+    using Src = typename std::conditional<MERGE, DevRangeSrc, RangeSrc>::type;
     auto range = [&](int bin, int &p0, int &p1) {
         p0 = dev ? 0 : b.start[bin];
         p1 = dev ? 0 : b.start[bin + 1];
         return dev || p1 > p0;
     };
     auto src = [&](int bin, int q0, int q1, int per) {
-        return RangeSrc{q0, q1, per, args_for(bin), dev ? rs.layout + bin : nullptr};
+        if constexpr (MERGE) return DevRangeSrc{rs.layout + bin, per, args_for(bin)};
+        else return RangeSrc{q0, q1, per, args_for(bin)};
     };
     auto grid = [&](int64_t items, int per_block, int max_per_cu) {
         return dev ? ctx->cu_count * max_per_cu : grid_for(ctx, items, per_block, max_per_cu);
@@ -1184,21 +1193,23 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         const int ps = ctx->tune.b4096_split && !MERGE && !dev ? std::min(std::max(b.split2048, p0), p1) : p1;
         kt_begin(ctx, KS_LP_B4096);
         if (ps > p0 || dev)
-            hipLaunchKernelGGL((k_lp_block<512, 8192, KM, RangeSrc>), dim3(grid(ps - p0, 1, 2)), dim3(512), 0,
+            hipLaunchKernelGGL((k_lp_block<512, 8192, KM, Src>), dim3(grid(ps - p0, 1, 2)), dim3(512), 0,
                                ctx->stream, src(BIN_B4096, p0, ps, 1));
-        if (p1 > ps)
-            hipLaunchKernelGGL((k_lp_block<256, 4096, KM, RangeSrc>), dim3(grid_for(ctx, p1 - ps, 1, 4)), dim3(256),
-                               0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B4096)});
+        if constexpr (!MERGE) {
+            if (p1 > ps)
+                hipLaunchKernelGGL((k_lp_block<256, 4096, KM, RangeSrc>), dim3(grid_for(ctx, p1 - ps, 1, 4)),
+                                   dim3(256), 0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B4096)});
+        }
         kt_end(ctx, KS_LP_B4096);
     }
     // degrees 513-1024: one workgroup per vertex (2048-slot table)
     if (range(BIN_B1024, p0, p1)) {
         kt_begin(ctx, KS_LP_B1024);
         if (block_aux)
-            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc, true>), dim3(grid(p1 - p0, 1, 8)), dim3(256), 0,
+            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, Src, true>), dim3(grid(p1 - p0, 1, 8)), dim3(256), 0,
                                ctx->stream, src(BIN_B1024, p0, p1, 1));
         else
-            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid(p1 - p0, 1, 8)), dim3(256), 0,
+            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, Src>), dim3(grid(p1 - p0, 1, 8)), dim3(256), 0,
                                ctx->stream, src(BIN_B1024, p0, p1, 1));
         kt_end(ctx, KS_LP_B1024);
     }
@@ -1208,16 +1219,16 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
     if (range(BIN_B512, p0, p1)) {
         kt_begin(ctx, KS_LP_B512);
         if (!ctx->tune.b1024_wave)
-            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, RangeSrc>), dim3(grid(p1 - p0, 1, 8)), dim3(256), 0,
+            hipLaunchKernelGGL((k_lp_block<256, 2048, KM, Src>), dim3(grid(p1 - p0, 1, 8)), dim3(256), 0,
                                ctx->stream, src(BIN_B512, p0, p1, 1));
         else if (block_aux)
-            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, true>), dim3(grid(p1 - p0, 4, 3)), dim3(256), 0,
+            hipLaunchKernelGGL((k_lp_wave<1024, KM, Src, true>), dim3(grid(p1 - p0, 4, 3)), dim3(256), 0,
                                ctx->stream, src(BIN_B512, p0, p1, 1));
         else if (ctx->tune.b1024_u16)  // 16-bit counts (degree <= 512): 28.7 KB, five workgroups per CU
-            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8, uint16_t>), dim3(grid(p1 - p0, 4, 5)),
-                               dim3(256), 0, ctx->stream, src(BIN_B512, p0, p1, 1));
+            hipLaunchKernelGGL((k_lp_wave<1024, KM, Src, false, 8, uint16_t>), dim3(grid(p1 - p0, 4, 5)), dim3(256),
+                               0, ctx->stream, src(BIN_B512, p0, p1, 1));
         else  // 4-8 entries per lane: all eight gathers in flight at once
-            hipLaunchKernelGGL((k_lp_wave<1024, KM, RangeSrc, false, 8>), dim3(grid(p1 - p0, 4, 4)), dim3(256), 0,
+            hipLaunchKernelGGL((k_lp_wave<1024, KM, Src, false, 8>), dim3(grid(p1 - p0, 4, 4)), dim3(256), 0,
                                ctx->stream, src(BIN_B512, p0, p1, 1));
         kt_end(ctx, KS_LP_B512);
     }
@@ -1227,10 +1238,10 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
     if (!fused && range(BIN_W256, p0, p1)) {
         kt_begin(ctx, KS_LP_W256);
         if (block_aux)
-            hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc, true>), dim3(grid(p1 - p0, 4, 8)), dim3(256), 0,
+            hipLaunchKernelGGL((k_lp_wave<512, KM, Src, true>), dim3(grid(p1 - p0, 4, 8)), dim3(256), 0,
                                ctx->stream, src(BIN_W256, p0, p1, 1));
         else
-            hipLaunchKernelGGL((k_lp_wave<512, KM, RangeSrc>), dim3(grid(p1 - p0, 4, 8)), dim3(256), 0, ctx->stream,
+            hipLaunchKernelGGL((k_lp_wave<512, KM, Src>), dim3(grid(p1 - p0, 4, 8)), dim3(256), 0, ctx->stream,
                                src(BIN_W256, p0, p1, 1));
         kt_end(ctx, KS_LP_W256);
     }
@@ -1238,27 +1249,29 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
     if (range(BIN, p0, p1)) {                                                                                  \
         kt_begin(ctx, SLOT);                                                                                         \
         if (small_aux)                                                                                         \
-            hipLaunchKernelGGL((k_lp_small<G, CP, true, KM, RangeSrc>), dim3(grid(p1 - p0, 4 * CP, 16)),           \
+            hipLaunchKernelGGL((k_lp_small<G, CP, true, KM, Src>), dim3(grid(p1 - p0, 4 * CP, 16)),                \
                                dim3(256), 0, ctx->stream, src(BIN, p0, p1, CP));                                \
         else                                                                                                   \
-            hipLaunchKernelGGL((k_lp_small<G, CP, false, KM, RangeSrc>), dim3(grid(p1 - p0, 4 * CP, 16)),          \
+            hipLaunchKernelGGL((k_lp_small<G, CP, false, KM, Src>), dim3(grid(p1 - p0, 4 * CP, 16)),               \
                                dim3(256), 0, ctx->stream, src(BIN, p0, p1, CP));                                \
         kt_end(ctx, SLOT);                                                                                     \
     }
     if (fused) {
-        SmallBins sb;
+        SmallBinsT<Src> sb;
         // bins W256, G64..G4 (consecutive bin numbers) and positions per unit
         const int cps[6] = {1, 8, 16, 32, 64, 64};
         static_assert(BIN_G64 == BIN_W256 + 1 && BIN_G4 == BIN_W256 + 5 && BIN_W256 == BIN_B512 + 1, "bin order");
         int64_t tot = 0;
         int nonempty = 0;
+        bool has[6];
         for (int i = 0; i < 6; i++) {
             range(BIN_W256 + i, p0, p1);
             sb.src[i] = src(BIN_W256 + i, p0, p1, cps[i]);
             tot += rs.bin_entries[BIN_W256 + i];
-            nonempty += p1 > p0;
+            has[i] = p1 > p0;
+            nonempty += has[i];
         }
-        sb.dlo = dev ? rs.layout + NUM_BINS + 1 : nullptr;
+        sb.dlo = MERGE ? rs.layout + NUM_BINS + 1 : nullptr;
         if (nonempty > 0 || dev) {
             // one workgroup per non-empty bin, the rest by entries
             const int spare = ctx->cu_count * 16 - nonempty;
@@ -1266,7 +1279,7 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
             int64_t acc = 0;
             for (int i = 0; i < 6; i++) {
                 sb.lo[i] = at;
-                if (sb.src[i].p1 > sb.src[i].p0) {
+                if (has[i]) {
                     const int64_t e = rs.bin_entries[BIN_W256 + i];
                     const int before = tot > 0 ? (int)(spare * acc / tot) : 0;
                     acc += e;
@@ -1284,17 +1297,19 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
             if constexpr (KM == RK_MOD || KM == RK_CPM) {
                 if (small_aux) {  // records of the modularity / CPM totals
                     if (ctx->tune.small_occ8)
-                        hipLaunchKernelGGL((k_lp_small_all<KM, W8, true>), dim3(at), dim3(256), 0, ctx->stream, sb);
+                        hipLaunchKernelGGL((k_lp_small_all<KM, W8, true, Src>), dim3(at), dim3(256), 0, ctx->stream,
+                                           sb);
                     else
-                        hipLaunchKernelGGL((k_lp_small_all<KM, 1, true>), dim3(at), dim3(256), 0, ctx->stream, sb);
+                        hipLaunchKernelGGL((k_lp_small_all<KM, 1, true, Src>), dim3(at), dim3(256), 0, ctx->stream,
+                                           sb);
                     launched = true;
                 }
             }
             if (!launched) {
                 if (ctx->tune.small_occ8 && ONE_KIND)
-                    hipLaunchKernelGGL((k_lp_small_all<KM, W8>), dim3(at), dim3(256), 0, ctx->stream, sb);
+                    hipLaunchKernelGGL((k_lp_small_all<KM, W8, false, Src>), dim3(at), dim3(256), 0, ctx->stream, sb);
                 else
-                    hipLaunchKernelGGL(k_lp_small_all<KM>, dim3(at), dim3(256), 0, ctx->stream, sb);
+                    hipLaunchKernelGGL((k_lp_small_all<KM, 1, false, Src>), dim3(at), dim3(256), 0, ctx->stream, sb);
             }
             kt_end(ctx, KS_LP_SMALL);
         }

@@ -46,19 +46,29 @@ __device__ inline bool lp_selected(const LPArgs &a, int v) {
     return selected(a.seed, a.step, v, a.f);
 }
 
-// positions [p0, p1) of one bin in units of `per` positions; with `bounds`
-// set, the range is [bounds[0], bounds[1]) in device memory instead (bins
-// laid out by kernels, for launches that are captured or not waited on)
+// positions [p0, p1) of one bin in units of `per` positions
 struct RangeSrc {
     int p0, p1, per;
     LPArgs a;
-    const int *bounds = nullptr;
-    __device__ int first() const { return bounds ? bounds[0] : p0; }
-    __device__ int last() const { return bounds ? bounds[1] : p1; }
-    __device__ int units() const { return (last() - first() + per - 1) / per; }
+    __device__ int units() const { return (p1 - p0 + per - 1) / per; }
     __device__ bool unit(int u, LPArgs &out, int &q0, int &q1) const {
-        q0 = first() + u * per;
-        q1 = min(last(), q0 + per);
+        q0 = p0 + u * per;
+        q1 = min(p1, q0 + per);
+        out = a;
+        return true;
+    }
+};
+
+// the same with the range [bounds[0], bounds[1]) in device memory (bins laid
+// out by kernels, for launches that are captured or not waited on)
+struct DevRangeSrc {
+    const int *bounds;
+    int per;
+    LPArgs a;
+    __device__ int units() const { return (bounds[1] - bounds[0] + per - 1) / per; }
+    __device__ bool unit(int u, LPArgs &out, int &q0, int &q1) const {
+        q0 = bounds[0] + u * per;
+        q1 = min(bounds[1], q0 + per);
         out = a;
         return true;
     }
