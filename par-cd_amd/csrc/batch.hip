@@ -974,7 +974,7 @@ static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, co
     g.lp.new_labels = d_next;
     g.lp.rule.kind = job->rule.kind;
     g.lp.rule.lambda = job->rule.lambda;
-    g.lp.rule.m = m;
+    set_edges(g.lp.rule, m);
     g.lp.rule.cc = 0.0;
     g.lp.seed = job->seed;
     g.lp.step = 0;

@@ -33,7 +33,7 @@ RuleParams make_rule(pcd_ctx *ctx, pcd_rule rule) {
     RuleParams r;
     r.kind = rule.kind;
     r.lambda = rule.lambda;
-    r.m = ctx->m;
+    set_edges(r, ctx->m);
     r.cc = ctx->cc;
     return r;
 }

@@ -35,7 +35,38 @@ struct RuleParams {
     double lambda;
     int m;       // Graph::m
     double cc;   // Graph::clustering_coef (WCC)
+    double inv_m;  // RN(1/m): device code's quotients by m read it (set_edges sets both)
 };
+
+PCD_HD inline void set_edges(RuleParams &r, int m) {
+    r.m = m;
+    r.inv_m = 1.0 / (double)m;
+}
+
+// x / m, rounded as the IEEE division rounds it, for an integer x and
+// m = r.m (x and m below 2^31 here; the argument needs |x| < 2^53, m < 2^50).
+// Device code replaces the division (a reciprocal iteration with scaling and
+// fix-up steps, about ten instructions around a quarter-rate reciprocal) by
+// one multiply and two fma, with y = RN(1/m):
+//  * q0 = RN(x y) lies within 2 ulp of t = x/m;
+//  * the residual x - q0 m is an integer multiple of ulp(q0) below 4m of
+//    them, so the fma computes it exactly;
+//  * q1 = RN(q0 + res y) = RN(t + eta) with |eta| < 2^-52 ulp(t).
+// RN(t + eta) = RN(t): a rounding midpoint has 54 significant bits, which
+// x/m cannot have (the odd 54-bit part would divide x < 2^53), and a point
+// of the 2^-54 ulp(t) grid other than t lies at least 2^-54 ulp(t) / m away
+// from t, farther than eta.  With m = 0 (no edges) every numerator is 0 and
+// both forms give NaN.  Host code divides.
+PCD_HD inline double div_m(const RuleParams &r, int x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double xd = (double)x, md = (double)r.m;
+    const double q0 = xd * r.inv_m;
+    const double res = __builtin_fma(-q0, md, xd);
+    return __builtin_fma(res, r.inv_m, q0);
+#else
+    return x / (double)r.m;
+#endif
+}
 
 PCD_HD inline int wrap_mul(int a, int b) { return (int)((unsigned)a * (unsigned)b); }
 
@@ -102,9 +133,10 @@ constexpr unsigned RK_ALL = RK_MOD | RK_CPM | RK_WCC | RK_COND;
 template <unsigned KM = RK_ALL>
 PCD_HD inline double score_join(const RuleParams &r, int deg, CommStats c, int num) {
     if ((KM & RK_MOD) && (KM == RK_MOD || r.kind == PCD_RULE_MODULARITY)) {
-        double e_ij = num / (double)r.m;
-        double a_i = (c.int_vol + c.ext_vol) / (2.0 * r.m);
-        double a_j = deg / (2.0 * r.m);
+        // x / (2m) is RN(x / m) / 2 exactly (no quotient here is subnormal)
+        double e_ij = div_m(r, num);
+        double a_i = div_m(r, c.int_vol + c.ext_vol) * 0.5;
+        double a_j = div_m(r, deg) * 0.5;
         return e_ij - 2 * a_i * a_j;
     }
     if ((KM & RK_CPM) && (KM == RK_CPM || r.kind == PCD_RULE_CPM)) return num - r.lambda * c.size;
@@ -121,9 +153,9 @@ PCD_HD inline double score_join(const RuleParams &r, int deg, CommStats c, int n
 template <unsigned KM = RK_ALL>
 PCD_HD inline double score_leave(const RuleParams &r, int deg, CommStats c, int num) {
     if ((KM & RK_MOD) && (KM == RK_MOD || r.kind == PCD_RULE_MODULARITY)) {
-        double e_ij = num / (double)r.m;
-        double a_i = (c.int_vol + c.ext_vol - deg) / (2.0 * r.m);
-        double a_j = deg / (2.0 * r.m);
+        double e_ij = div_m(r, num);
+        double a_i = div_m(r, c.int_vol + c.ext_vol - deg) * 0.5;
+        double a_j = div_m(r, deg) * 0.5;
         return -(e_ij - 2 * a_i * a_j);
     }
     if ((KM & RK_CPM) && (KM == RK_CPM || r.kind == PCD_RULE_CPM)) return -(num - r.lambda * (c.size - 1));
@@ -142,9 +174,9 @@ PCD_HD inline double score_leave(const RuleParams &r, int deg, CommStats c, int 
 template <unsigned KM = RK_ALL>
 PCD_HD inline double score_merge(const RuleParams &r, CommStats a, CommStats b, int links) {
     if ((KM & RK_MOD) && (KM == RK_MOD || r.kind == PCD_RULE_MODULARITY)) {
-        double e_ij = links / (double)r.m;
-        double a_i = (a.int_vol + a.ext_vol) / (2.0 * r.m);
-        double a_j = (b.int_vol + b.ext_vol) / (2.0 * r.m);
+        double e_ij = div_m(r, links);
+        double a_i = div_m(r, a.int_vol + a.ext_vol) * 0.5;
+        double a_j = div_m(r, b.int_vol + b.ext_vol) * 0.5;
         return e_ij - 2 * a_i * a_j;
     }
     if ((KM & RK_CPM) && (KM == RK_CPM || r.kind == PCD_RULE_CPM)) return links - r.lambda * a.size * b.size;

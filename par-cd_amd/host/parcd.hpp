@@ -202,7 +202,7 @@ class Rule {
         pcd::RuleParams r;
         r.kind = d.kind;
         r.lambda = d.lambda;
-        r.m = g.m;
+        pcd::set_edges(r, g.m);
         r.cc = g.clustering_coef;
         return r;
     }
