@@ -57,12 +57,53 @@ __device__ inline bool cand_better(const Cand &a, const Cand &b) {
     return a.label < b.label;
 }
 
-__device__ inline Cand cand_shfl_xor(const Cand &c, int d) {
+// x of lane ^ J, every lane of the wave active: a DPP quad permute for
+// J = 1 and 2 (no LDS access), ds_swizzle's bit mode (and 0x1f, xor J,
+// within 32 lanes; no address register) for J = 4..16, ds_bpermute for 32.
+template <int J>
+__device__ inline int xor_shfl(int x) {
+    static_assert(J >= 1 && J <= 32 && (J & (J - 1)) == 0, "lane distance");
+    if constexpr (J == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);       // quad_perm [1,0,3,2]
+    else if constexpr (J == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    else if constexpr (J <= 16) return __builtin_amdgcn_ds_swizzle(x, 0x1F | (J << 10));
+    else return __shfl_xor(x, J, WAVE);
+}
+
+// x of lane - 1 (DPP wave_shr:1, every lane active); lane 0's result is
+// undefined
+__device__ inline int prev_lane(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xF, 0xF, false); }
+
+template <int J>
+__device__ inline Cand cand_xor(const Cand &c) {
     Cand o;
-    o.s = __shfl_xor(c.s, d, WAVE);
-    o.key = (uint32_t)__shfl_xor((int)c.key, d, WAVE);
-    o.label = __shfl_xor(c.label, d, WAVE);
+    const long long b = __double_as_longlong(c.s);
+    const int lo = xor_shfl<J>((int)(uint32_t)b), hi = xor_shfl<J>((int)(b >> 32));
+    o.s = __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+    o.key = (uint32_t)xor_shfl<J>((int)c.key);
+    o.label = xor_shfl<J>(c.label);
     return o;
+}
+
+// best candidate and largest own count over aligned groups of 2D lanes
+template <int D>
+__device__ inline void group_reduce_cand(Cand &c, int &own_cnt) {
+    const Cand o = cand_xor<D>(c);
+    if (cand_better(o, c)) c = o;
+    own_cnt = max(own_cnt, xor_shfl<D>(own_cnt));
+    if constexpr (D > 1) group_reduce_cand<D / 2>(c, own_cnt);
+}
+
+// bitonic sort of each aligned group of G lanes, ascending: one
+// compare-exchange per step, the lane keeping the smaller or the larger of
+// its value and its partner's (lane ^ J)
+template <int G, int K = 2, int J = 1>
+__device__ inline int group_sort(int x, int glane) {
+    const bool take_min = ((glane & K) == 0) == ((glane & J) == 0);
+    const int y = xor_shfl<J>(x);
+    x = take_min ? min(x, y) : max(x, y);
+    if constexpr (J > 1) return group_sort<G, K, J / 2>(x, glane);
+    else if constexpr (K < G) return group_sort<G, 2 * K, K>(x, glane);
+    else return x;
 }
 
 // partial result of one hub partition
@@ -274,20 +315,22 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
                 r = AUX ? a.lv[nb] : make_int2(entry_label<KM>(a, nb), 0);
             }
             int x = r.x, xa = r.y;
+            if constexpr (AUX) {
 #pragma unroll
-            for (int k = 2; k <= G; k <<= 1) {
+                for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    int y = __shfl_xor(x, j, WAVE);
-                    int ya = AUX ? __shfl_xor(xa, j, WAVE) : 0;
-                    bool up = (glane & k) == 0;
-                    bool lower = (glane & j) == 0;
-                    bool take_min = lower == up;
-                    bool other = take_min ? (y < x) : (y > x);
-                    if (other) { x = y; xa = ya; }
+                    for (int j = k >> 1; j > 0; j >>= 1) {
+                        const bool take_min = ((glane & k) == 0) == ((glane & j) == 0);
+                        const int y = __shfl_xor(x, j, WAVE), ya = __shfl_xor(xa, j, WAVE);
+                        const bool other = take_min ? (y < x) : (y > x);
+                        x = other ? y : x;
+                        xa = other ? ya : xa;
+                    }
                 }
+            } else {
+                x = group_sort<G>(x, glane);
             }
-            int prev = __shfl(x, lane == 0 ? 0 : lane - 1, WAVE);
+            const int prev = prev_lane(x);  // read only where glane > 0
             bool head = x != INT32_MAX && (glane == 0 || x != prev);
             uint64_t heads = __ballot(head);
             Cand c = cand_none();
@@ -308,12 +351,7 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
                     }
                 }
             }
-#pragma unroll
-            for (int d = G / 2; d > 0; d >>= 1) {
-                Cand o = cand_shfl_xor(c, d);
-                if (cand_better(o, c)) c = o;
-                own_cnt = max(own_cnt, __shfl_xor(own_cnt, d, WAVE));
-            }
+            group_reduce_cand<G / 2>(c, own_cnt);
             if (has && glane == 0) {
                 if constexpr ((KM & RK_MERGE) != 0) a.new_labels[u] = merge_partner(c);
                 else a.new_labels[u] = AUX ? lp_decide<KM>(a, deg, own.x, own.y, own_cnt, c)
@@ -348,14 +386,7 @@ __device__ inline int tslot(int key, int T) {
 // gathers overlap.
 constexpr int SB = 4;
 
-__device__ inline void wave_reduce_cand(Cand &c, int &own_cnt) {
-#pragma unroll
-    for (int d = WAVE / 2; d > 0; d >>= 1) {
-        Cand o = cand_shfl_xor(c, d);
-        if (cand_better(o, c)) c = o;
-        own_cnt = max(own_cnt, __shfl_xor(own_cnt, d, WAVE));
-    }
-}
+__device__ inline void wave_reduce_cand(Cand &c, int &own_cnt) { group_reduce_cand<WAVE / 2>(c, own_cnt); }
 
 template <int NT>
 __device__ inline void block_reduce_cand(Cand &c, int &own_cnt, int &distinct, Cand *s_c, int *s_i) {
