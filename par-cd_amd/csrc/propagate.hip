@@ -284,6 +284,7 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
         // each group takes its vertices' values by shuffle
         int pu = 0, pstart = 0, pdeg = 0, pown = 0, paux = 0;
         CommStats pst{0, 0, 0, 0};
+        uint64_t ptb = 0;  // the vertex's tie-key base (one hash per vertex, not per pass)
         if (lane < nact) {
             pu = list[lane];
             pstart = a.indices[pu];
@@ -296,12 +297,15 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
                 pown = own_label<KM>(a, pu);
                 pst = comm_rec<KM>(a, pown);
             }
+            if (a.tie != PCD_TIE_LOWEST) ptb = tie_base(a.seed, step_of(a), pu);
         }
         for (int gb = 0; gb < nact; gb += per_wave) {
             int gi = gb + lane / G;
             bool has = gi < nact;
             const int sl = has ? gi : 0;
             int u = __shfl(pu, sl, WAVE), start = __shfl(pstart, sl, WAVE), deg = __shfl(pdeg, sl, WAVE);
+            const uint64_t tb = ((uint64_t)(uint32_t)__shfl((int)(ptb >> 32), sl, WAVE) << 32) |
+                                (uint32_t)__shfl((int)(uint32_t)ptb, sl, WAVE);
             int2 own = make_int2(__shfl(pown, sl, WAVE), __shfl(paux, sl, WAVE));
             const CommStats own_rec{__shfl(pst.size, sl, WAVE), __shfl(pst.int_vol, sl, WAVE),
                                     __shfl(pst.ext_vol, sl, WAVE), 0};
@@ -345,7 +349,6 @@ __device__ inline void lp_small(const Src &src, int wave_g, int nwaves, int *lis
                 } else {
                     const CommStats xr = AUX ? CommStats{xa, xa, 0, 0} : comm_rec<KM>(a, x);
                     if ((KM & RK_MERGE) == 0 || !xr.pad) {  // merging: matched communities are no candidates
-                        uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), u);
                         double sc;
                         if (cand_score<KM>(a, deg, own_rec, xr, cnt, sc)) c = Cand{sc, tie_key(a.tie, tb, x), x};
                     }
