@@ -11,10 +11,13 @@ synchronised on both sides.  Run once for modularity (the reported value)
 and once for CPM with resolution 0.01.
 
 value = 2m CSR entries per iteration / seconds per iteration / 1e9 (GTEPS,
-entries convention: 2 per undirected edge).  The roofline entry is the
-dominant propagation kernel: its algorithmic bytes (from exact per-bin work
-counters of a deterministic replay of the same iterations) over its HIP-event
-duration.  cpu_baseline is the reference OpenMP binary (built from the
+entries convention: 2 per undirected edge), from a timed pass without
+kernel events (the hub rows and degrees 1025-4096 run on side streams beside
+the other bins, as in any untimed run).  The roofline entry is the dominant
+propagation kernel: its algorithmic bytes (from exact per-bin work counters
+of a deterministic replay of the same iterations) over its HIP-event
+duration, timed alone in a pass over the same iterations with every bin on
+one stream.  cpu_baseline is the reference OpenMP binary (built from the
 reference sources into oracle/_ref) on the same graph file, same metric,
 timed over 2 iterations on the host cores.
 
@@ -713,9 +716,12 @@ def main():
     init = parcd.singleton_partition(n)
 
     ctx = parcd.Context(0)
+    bin_streams = 1  # the library's default
     for kv in args.tune:
         k_, v_ = kv.split("=")
         ctx.set_tuning(k_, int(v_))
+        if k_ == "bin_streams":
+            bin_streams = int(v_)
     copy_gbs = ctx.copy_probe(1 << 30, 10)
     log("stream copy: %.0f GB/s (1 GiB, read+write)" % copy_gbs)
     ctx.upload_graph(n, m, ind, adj)
@@ -724,19 +730,27 @@ def main():
 
     # ---------------------------------------------------------- modularity
     # probe pass (not timed): every kernel slot timed, to find the kernel
-    # with the longest average launch; the timed pass then brackets only
-    # that kernel with events (each timed launch adds two events to the
-    # stream, about 0.15 ms per iteration for all slots together)
+    # with the longest average launch (timed slots keep every bin on the
+    # context's stream, so each kernel is timed alone).  The timed pass then
+    # runs with no kernel events, the hub rows and degrees 1025-4096 on side
+    # streams as in any untimed run; the roofline pass brackets only the
+    # dominant kernel with events, every bin on one stream, so that kernel's
+    # launches are timed alone.
     run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup, args.seed)
     timers = ctx.kernel_timers()
     probe_avg = {k_: ms_ / cnt_ for k_, (ms_, cnt_) in timers.items() if cnt_ and k_ in KERNELS}
     dom_probe = max(probe_avg, key=probe_avg.get) if probe_avg else None
+    secs, scores = run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup, args.seed,
+                                  timers=False)
+    labels_timed = ctx.labels()
+    ctx.set_tuning("bin_streams", 0)
     ctx.kernel_timer_select([dom_probe] if dom_probe else [])
-    secs, scores = run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup, args.seed)
+    _, scores_rf = run_iterations(ctx, parcd, parcd.RULE_MODULARITY, 0.0, init, args.steps, args.warmup, args.seed)
     timed_dom = ctx.kernel_timers()
     ctx.kernel_timer_select(None)
-    log("timed pass: events around %s only" % (dom_probe or "nothing"))
-    labels_timed = ctx.labels()
+    ctx.set_tuning("bin_streams", bin_streams)
+    assert scores_rf == scores and np.array_equal(ctx.labels(), labels_timed)
+    log("roofline pass: events around %s only, bins on one stream" % (dom_probe or "nothing"))
     log("hub partitions overflowed:", ctx.hub_overflows())
     ms_per_step = secs * 1000.0 / args.steps
     gteps = E / (secs / args.steps) / 1e9
@@ -774,14 +788,14 @@ def main():
         per_kernel["stats_items"] = {"kernel": KERNEL_NAMES["stats_items"],
                                      "bytes_per_launch": 8 * int(deg[big].sum()) + 20 * int(big.sum()),
                                      "avg_ms": st_ms / st_launches, "entries_per_launch": int(deg[big].sum())}
-    # the roofline kernel's average launch comes from the timed pass
+    # the roofline kernel's average launch comes from the roofline pass
     if dom_probe in per_kernel:
         t_ms, t_launches = timed_dom.get(dom_probe, (0.0, 0))
         if t_launches:
             d_ = per_kernel[dom_probe]
             d_["avg_ms_probe_pass"] = d_["avg_ms"]
             d_["avg_ms"] = t_ms / t_launches
-            d_["measured_in"] = "timed pass"
+            d_["measured_in"] = "roofline pass"
     for slot, d_ in per_kernel.items():
         d_["gbs"] = d_["bytes_per_launch"] / (d_["avg_ms"] / 1000.0) / 1e9
         d_["frac"] = round(d_["gbs"] / HBM_PEAK_GBS, 4)
@@ -815,8 +829,9 @@ def main():
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"], "traffic": d["traffic_per_launch"],
                     "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if d["traffic_per_launch"] else None,
                     "bytes_per_launch": int(d["bytes_per_launch"]), "avg_launch_ms": round(d["avg_ms"], 4),
-                    "avg_launch_from": "HIP events around this kernel alone in the timed pass"
-                    if d.get("measured_in") == "timed pass" else "HIP events of the probe pass"}
+                    "avg_launch_from": "HIP events around this kernel alone in the roofline pass (the timed "
+                    "pass's iterations again, every bin on one stream)"
+                    if d.get("measured_in") == "roofline pass" else "HIP events of the probe pass"}
 
     # ---------------------------------------------------------- merging (-s 1)
     merge = None
@@ -896,7 +911,9 @@ def main():
                    "step": "compress+statistics+metric+label propagation", "teps": "2m entries per iteration"},
         "roofline": roofline, "cpu_baseline": cpu, "stream_copy_gbs": round(copy_gbs, 1),
         "speedup_vs_cpu": round(gteps / cpu["value"], 2) if cpu else None,
-        "phases": {"propagate": lp_phase, "statistics": stats_phase, "kernels": per_kernel},
+        "phases": {"propagate": lp_phase, "statistics": stats_phase, "kernels": per_kernel,
+                   "kernel_times": "probe pass: every kernel timed alone, every bin on one stream",
+                   "bin_streams": bin_streams},
         "modularity_scores": scores, "replay_identical": replay_identical, "cpm_0.01": cpm, "merge_s1": merge, "wcc_chung_lu": wcc, "sbm1": sbm1, "batch": batch, "serve": serve,
     }
     print(json.dumps(out), flush=True)

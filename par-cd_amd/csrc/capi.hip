@@ -144,6 +144,13 @@ extern "C" int pcd_ctx_create(int device, pcd_ctx **out) {
         (void)hipEventCreate(&ctx->ev1[i]);
     }
     if (hipEventCreateWithFlags(&ctx->ev_metric, hipEventDisableTiming) != hipSuccess) ctx->ev_metric = nullptr;
+    {
+        bool ok = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) == hipSuccess;
+        for (int i = 0; i < 2 && ok; i++)
+            ok = hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&ctx->ev_join[i], hipEventDisableTiming) == hipSuccess;
+        if (!ok) (void)hipGetLastError();  // without them, bins run on `stream` alone
+    }
     if (hipHostMalloc((void **)&ctx->h_result, 64, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&ctx->h_small, 256, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&ctx->h_small64, 256, hipHostMallocDefault) != hipSuccess) {
@@ -222,6 +229,8 @@ extern "C" int pcd_ctx_destroy(pcd_ctx *ctx) {
     if (!ctx) return PCD_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 2; i++)
+        if (ctx->side[i]) (void)hipStreamSynchronize(ctx->side[i]);
     loop_release(ctx);
     merge_release(ctx);
     for (void *p : ctx->allocations) (void)hipFree(p);
@@ -238,6 +247,11 @@ extern "C" int pcd_ctx_destroy(pcd_ctx *ctx) {
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->kt_open) (void)hipEventDestroy(ctx->kt_open);
     if (ctx->ev_metric) (void)hipEventDestroy(ctx->ev_metric);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    for (int i = 0; i < 2; i++) {
+        if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
+        if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
+    }
     if (ctx->h_result) (void)hipHostFree(ctx->h_result);
     if (ctx->h_small) (void)hipHostFree(ctx->h_small);
     if (ctx->h_small64) (void)hipHostFree(ctx->h_small64);
@@ -806,6 +820,8 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.hub_aux = value != 0;
     } else if (k == "small_fused") {
         ctx->tune.small_fused = value != 0;
+    } else if (k == "bin_streams") {
+        ctx->tune.bin_streams = value != 0;
     } else if (k == "small_occ8") {
         ctx->tune.small_occ8 = value != 0;
     } else if (k == "b1024_u16") {

@@ -86,6 +86,8 @@ struct Tuning {
     int hub_aux = 0;             // hub partitions count {label, aux} records (1) or labels (0)
     int hub_nt = 512;            // hub partition workgroups: 512 threads, 4096 slots or 256, 2048
     int small_fused = 1;         // the G64..G4 bins in one launch (1) or one launch each (0)
+    int bin_streams = 1;         // a propagation step runs the hub rows and degrees 1025-4096 on
+                                 // two side streams, beside the other bins (0: one stream)
     int small_occ8 = 1;          // the fused small-bin launch compiled for 8 waves per SIMD
                                  // (modularity and CPM builds; registers beyond spill to lanes)
     int stats_cache = 1;         // statistics over long rows read the labels of the first
@@ -151,6 +153,11 @@ struct pcd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int cu_count = 256;
+    // streams a propagation step forks independent degree bins onto (tuning
+    // bin_streams), and the events that fork them from `stream` and join them
+    // back; all at the default priority
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
 
     // graph
     int n = 0, m = 0, max_degree = 0;

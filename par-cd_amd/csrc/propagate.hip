@@ -1155,6 +1155,23 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
                            ctx->d_stats, a.rule.kind, ctx->d_tot);
         kt_end(ctx, KS_LP_SELECT);
     }
+    // Hub rows on one side stream and degrees 1025-4096 on another, beside
+    // the other bins here (tuning bin_streams): bins write disjoint vertices
+    // and read the same step inputs, so one kernel's tail overlaps another's
+    // work (R-MAT-22 modularity 1.255 -> 1.083 ms per iteration; sending
+    // degrees 513-1024 or 257-512 to the side streams as well, or to a third
+    // one, measured slower).  Kept on one stream while a timer brackets a side
+    // section (its events are recorded on `stream`).
+    constexpr uint32_t SIDE_SLOTS = (1u << KS_LP_HUB_INSERT) | (1u << KS_LP_HUB_DECIDE) | (1u << KS_LP_HUB_FINAL) |
+                                    (1u << KS_LP_B4096);
+    const bool fork = !MERGE && ctx->tune.bin_streams && ctx->side[1] &&
+                      !(ctx->timing && (ctx->timer_mask & SIDE_SLOTS));
+    const int nside = fork ? 2 : 0;
+    hipStream_t hs = fork ? ctx->side[0] : ctx->stream, bs = fork ? ctx->side[1] : ctx->stream;
+    if (fork) {
+        PCD_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+        for (int i = 0; i < nside; i++) PCD_HIP(hipStreamWaitEvent(ctx->side[i], ctx->ev_fork, 0));
+    }
     int p0, p1;
     if (rs.num_hubs > 0 || rs.hub_counts) {
         HubArgs h;
@@ -1183,10 +1200,10 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         kt_begin(ctx, KS_LP_HUB_INSERT);
         if (hub_aux)
             hipLaunchKernelGGL((k_hub_partition<KM, true>), dim3(hgrid(h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
-                               ctx->stream, ah, h);
+                               hs, ah, h);
         else
             hipLaunchKernelGGL((k_hub_partition<KM, false>), dim3(hgrid(h.nchunks, 1, 4)), dim3(HUB1_NT), 0,
-                               ctx->stream, ah, h);
+                               hs, ah, h);
         kt_end(ctx, KS_LP_HUB_INSERT);
         kt_begin(ctx, KS_LP_HUB_DECIDE);
         // 512 threads with 4096 slots (three workgroups per CU) or 256
@@ -1194,29 +1211,29 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         if (hub_aux) {
             if (ctx->tune.hub_nt == 256)
                 hipLaunchKernelGGL((k_hub_count_listed<KM, 256, 2048, true>), dim3(hgrid(h.nparts, 1, 5)),
-                                   dim3(256), 0, ctx->stream, ah, h);
+                                   dim3(256), 0, hs, ah, h);
             else
                 hipLaunchKernelGGL((k_hub_count_listed<KM, 512, HUB_TB, true>), dim3(hgrid(h.nparts, 1, 2)),
-                                   dim3(512), 0, ctx->stream, ah, h);
+                                   dim3(512), 0, hs, ah, h);
             kt_end(ctx, KS_LP_HUB_DECIDE);
             kt_begin(ctx, KS_LP_HUB_FINAL);
-            hipLaunchKernelGGL((k_hub_final<KM, true>), dim3(hgrid(h.nhubs, 4, 4)), dim3(256), 0, ctx->stream,
+            hipLaunchKernelGGL((k_hub_final<KM, true>), dim3(hgrid(h.nhubs, 4, 4)), dim3(256), 0, hs,
                                ah, h);
         } else {
             if (ctx->tune.hub_nt == 256)
                 hipLaunchKernelGGL((k_hub_count_listed<KM, 256, 2048, false>), dim3(hgrid(h.nparts, 1, 7)),
-                                   dim3(256), 0, ctx->stream, ah, h);
+                                   dim3(256), 0, hs, ah, h);
             else
                 hipLaunchKernelGGL((k_hub_count_listed<KM, 512, HUB_TB, false>), dim3(hgrid(h.nparts, 1, 3)),
-                                   dim3(512), 0, ctx->stream, ah, h);
+                                   dim3(512), 0, hs, ah, h);
             kt_end(ctx, KS_LP_HUB_DECIDE);
             kt_begin(ctx, KS_LP_HUB_FINAL);
             hipLaunchKernelGGL((k_hub_final<KM, false>), dim3(hgrid(h.nhubs, 4, 4)), dim3(256), 0,
-                               ctx->stream, ah, h);
+                               hs, ah, h);
         }
-        hipLaunchKernelGGL(k_hub_global_insert<KM>, dim3(hgrid(h.nchunks, 1, 4)), dim3(512), 0, ctx->stream,
+        hipLaunchKernelGGL(k_hub_global_insert<KM>, dim3(hgrid(h.nchunks, 1, 4)), dim3(512), 0, hs,
                            ah, h, rs.hub_table_off, rs.hub_table_log, rs.hub_keys, rs.hub_cnts);
-        hipLaunchKernelGGL(k_hub_global_decide<KM>, dim3(hgrid(h.nhubs, 1, 1)), dim3(1024), 0, ctx->stream, ah,
+        hipLaunchKernelGGL(k_hub_global_decide<KM>, dim3(hgrid(h.nhubs, 1, 1)), dim3(1024), 0, hs, ah,
                            h, rs.hub_table_off, rs.hub_table_log, rs.hub_keys, rs.hub_cnts);
         kt_end(ctx, KS_LP_HUB_FINAL);
     }
@@ -1228,11 +1245,11 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         kt_begin(ctx, KS_LP_B4096);
         if (ps > p0 || dev)
             hipLaunchKernelGGL((k_lp_block<512, 8192, KM, Src>), dim3(grid(ps - p0, 1, 2)), dim3(512), 0,
-                               ctx->stream, src(BIN_B4096, p0, ps, 1));
+                               bs, src(BIN_B4096, p0, ps, 1));
         if constexpr (!MERGE) {
             if (p1 > ps)
                 hipLaunchKernelGGL((k_lp_block<256, 4096, KM, RangeSrc>), dim3(grid_for(ctx, p1 - ps, 1, 4)),
-                                   dim3(256), 0, ctx->stream, RangeSrc{ps, p1, 1, args_for(BIN_B4096)});
+                                   dim3(256), 0, bs, RangeSrc{ps, p1, 1, args_for(BIN_B4096)});
         }
         kt_end(ctx, KS_LP_B4096);
     }
@@ -1355,6 +1372,12 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         PCD_SMALL(4, 64, BIN_G4, KS_LP_G4)
     }
 #undef PCD_SMALL
+    if (fork) {
+        for (int i = 0; i < nside; i++) {
+            PCD_HIP(hipEventRecord(ctx->ev_join[i], ctx->side[i]));
+            PCD_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join[i], 0));
+        }
+    }
     PCD_HIP(hipGetLastError());
     return PCD_OK;
 }

@@ -64,6 +64,29 @@ def test_refine_equals_pass_by_pass_and_replay(name, j):
     assert abs(a[1] - osc) <= 1e-9 * max(1.0, abs(osc))
 
 
+@pytest.mark.parametrize("name", ["bin_zoo", "rmat12"])
+@pytest.mark.parametrize("j", [0, 1])
+def test_bin_streams_equal_one_stream(name, j):
+    """Propagation with the hub rows and degrees 1025-4096 on side streams
+    (bin_streams 1): the captured loop, whose passes then hold parallel
+    branches, and pass-by-pass calls both equal the one-stream run."""
+    g = graphs()[name]
+    n, m, ind, adj, _ = g
+    jb = parcd.job(**JOBS[j])
+    lab = GG.singleton_labels(n)
+    res = []
+    for st, loop in ((0, 1), (1, 1), (1, 0)):
+        with parcd.Context(0) as ctx:
+            ctx.set_tuning("bin_streams", st)
+            ctx.set_tuning("graph_loop", loop)
+            ctx.upload_graph(n, m, ind, adj)
+            ctx.upload_labels(lab)
+            score, iters, k = ctx.refine(jb)
+            res.append((ctx.labels(), score, iters, k))
+    for r in res[1:]:
+        assert np.array_equal(r[0], res[0][0]) and r[1:] == res[0][1:]
+
+
 @pytest.mark.parametrize("name", ["sbm", "bin_zoo"])
 def test_ranges_of_passes_then_pass_by_pass(name):
     """pcd_iterate_many over a range that ends before the stop rule, then

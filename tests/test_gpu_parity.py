@@ -37,6 +37,7 @@ def graphs():
 
 GOLDEN = graphs()
 RULES = [(O.MOD, 0.0), (O.CPM, 0.05), (O.CPM, 0.01)]
+BIN_STREAMS_DEFAULT = 1  # the library's default of the bin_streams tuning key
 
 
 def upload(ctx, g):
@@ -271,6 +272,25 @@ def test_propagation_hub_paths(ctx, slots, part_entries, hub_nt, hub_aux):
         ctx.set_tuning("hub_table_slots", 4096)
         ctx.set_tuning("hub_aux", 0)
         ctx.set_tuning("hub_nt", 512)
+
+
+@pytest.mark.parametrize("streams", [0, 1])
+@pytest.mark.parametrize("kind,lam", [(O.MOD, 0.0), (O.CPM, 0.02), (O.WCC, 0.0)])
+def test_propagation_bin_streams(ctx, kind, lam, streams):
+    """Hub rows and degrees 1025-4096 forked onto side streams while the other
+    bins run on the context's stream (bin_streams 1, the default), and every
+    bin on one stream (0): the step equals the oracle's either way."""
+    g = GG.bin_zoo()
+    n, m, ind, adj = g[:4]
+    cc = O.count_triangles(n, ind, adj)[3] if kind == O.WCC else None
+    rng = np.random.default_rng(8)
+    ctx.set_tuning("bin_streams", streams)
+    try:
+        for lab in [GG.singleton_labels(n), rng.integers(0, 40, n), rng.integers(0, 3000, n)]:
+            _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=4, step=3, tie=O.TIE_RANDOM, cc=cc)
+            _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=4, step=4, tie=O.TIE_LOWEST, cc=cc)
+    finally:
+        ctx.set_tuning("bin_streams", BIN_STREAMS_DEFAULT)
 
 
 def test_hub_partitions_fit_with_default_tuning():
