@@ -150,6 +150,9 @@ extern "C" int pcd_ctx_create(int device, pcd_ctx **out) {
             ok = hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking) == hipSuccess &&
                  hipEventCreateWithFlags(&ctx->ev_join[i], hipEventDisableTiming) == hipSuccess;
         if (!ok) (void)hipGetLastError();  // without them, bins run on `stream` alone
+        if (hipStreamCreateWithFlags(&ctx->mstream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->ev_mfork, hipEventDisableTiming) != hipSuccess)
+            (void)hipGetLastError();  // the metric then runs on `stream`
     }
     if (hipHostMalloc((void **)&ctx->h_result, 64, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&ctx->h_small, 256, hipHostMallocDefault) != hipSuccess ||
@@ -231,6 +234,7 @@ extern "C" int pcd_ctx_destroy(pcd_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int i = 0; i < 2; i++)
         if (ctx->side[i]) (void)hipStreamSynchronize(ctx->side[i]);
+    if (ctx->mstream) (void)hipStreamSynchronize(ctx->mstream);
     loop_release(ctx);
     merge_release(ctx);
     for (void *p : ctx->allocations) (void)hipFree(p);
@@ -248,6 +252,8 @@ extern "C" int pcd_ctx_destroy(pcd_ctx *ctx) {
     if (ctx->kt_open) (void)hipEventDestroy(ctx->kt_open);
     if (ctx->ev_metric) (void)hipEventDestroy(ctx->ev_metric);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_mfork) (void)hipEventDestroy(ctx->ev_mfork);
+    if (ctx->mstream) (void)hipStreamDestroy(ctx->mstream);
     for (int i = 0; i < 2; i++) {
         if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
         if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
@@ -708,15 +714,32 @@ static int iterate_pass(pcd_ctx *ctx, const pcd_job *job, int strategy, int roun
         PCD_TRY(metric_dev(ctx, rule, &s));
         phase_end(ctx, PCD_PHASE_METRIC);
     } else {
+        // The metric on its own stream, forked after the compression: the
+        // propagation does not read it and starts at once; phase timing and
+        // kernel timers keep it on `stream` (their events are recorded there).
+        // The statistics update waits for it on the device below.
+        const bool mfork = ctx->mstream && ctx->ev_mfork && !ctx->timing;
+        hipStream_t main_stream = ctx->stream;
+        if (mfork) {
+            PCD_HIP(hipEventRecord(ctx->ev_mfork, ctx->stream));
+            PCD_HIP(hipStreamWaitEvent(ctx->mstream, ctx->ev_mfork, 0));
+            ctx->stream = ctx->mstream;  // the metric's launches and copy go there
+        }
         phase_begin(ctx, PCD_PHASE_METRIC);
-        PCD_TRY(metric_enqueue(ctx, rule));
+        const int mrc = metric_enqueue(ctx, rule);
         phase_end(ctx, PCD_PHASE_METRIC);
-        PCD_HIP(hipEventRecord(ctx->ev_metric, ctx->stream));
+        const hipError_t erc = hipEventRecord(ctx->ev_metric, ctx->stream);
+        ctx->stream = main_stream;
+        PCD_TRY(mrc);
+        PCD_HIP(erc);
         if (prop && !known_done) {
             phase_begin(ctx, PCD_PHASE_PROPAGATE);
             PCD_TRY(propagation_launch(ctx, rule, job->frac, job->seed, it, job->tie));
             launched = true;
         }
+        // what follows on `stream` (the statistics update, a merging step, the
+        // next compression) waits for the metric's reads of the statistics
+        if (mfork) PCD_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_metric, 0));
         PCD_HIP(hipEventSynchronize(ctx->ev_metric));
         s = ctx->h_result[0];
     }

@@ -158,6 +158,10 @@ struct pcd_ctx {
     // back; all at the default priority
     hipStream_t side[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+    // stream pcd_iterate runs the metric on, beside the propagation (which
+    // does not read it), and the event that forks it from `stream`
+    hipStream_t mstream = nullptr;
+    hipEvent_t ev_mfork = nullptr;
 
     // graph
     int n = 0, m = 0, max_degree = 0;
