@@ -112,7 +112,7 @@ KERNELS = {
 # timer slot -> the kernel's name in a rocprofv3 trace of this build
 # (modularity build: rule-kind template argument 1u; default tuning)
 KERNEL_NAMES = {
-    "lp_small": "pcd::k_lp_small_all<1u, 8>",
+    "lp_small": "pcd::k_lp_small_all<1u, 8, false, pcd::RangeSrc>",
     "lp_b512": "pcd::k_lp_wave<1024, 1u, pcd::RangeSrc, false, 8, unsigned short>",
     "lp_b1024": "pcd::k_lp_block<256, 2048, 1u, pcd::RangeSrc, false, 4>",
     "lp_b4096": "pcd::k_lp_block<512, 8192, 1u, pcd::RangeSrc, false, 4>",
