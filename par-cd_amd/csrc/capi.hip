@@ -845,6 +845,9 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.small_fused = value != 0;
     } else if (k == "bin_streams") {
         ctx->tune.bin_streams = value != 0;
+    } else if (k == "bin_streams_min_edges") {
+        if (value < 0) return fail(PCD_E_INVALID, "bin_streams_min_edges must be >= 0");
+        ctx->tune.bin_streams_min_edges = value;
     } else if (k == "small_occ8") {
         ctx->tune.small_occ8 = value != 0;
     } else if (k == "b1024_u16") {

@@ -88,6 +88,7 @@ struct Tuning {
     int small_fused = 1;         // the G64..G4 bins in one launch (1) or one launch each (0)
     int bin_streams = 1;         // a propagation step runs the hub rows and degrees 1025-4096 on
                                  // two side streams, beside the other bins (0: one stream)
+    int bin_streams_min_edges = 1 << 19;  // ... for graphs with at least this many edges
     int small_occ8 = 1;          // the fused small-bin launch compiled for 8 waves per SIMD
                                  // (modularity and CPM builds; registers beyond spill to lanes)
     int stats_cache = 1;         // statistics over long rows read the labels of the first

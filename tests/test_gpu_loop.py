@@ -78,6 +78,7 @@ def test_bin_streams_equal_one_stream(name, j):
     for st, loop in ((0, 1), (1, 1), (1, 0)):
         with parcd.Context(0) as ctx:
             ctx.set_tuning("bin_streams", st)
+            ctx.set_tuning("bin_streams_min_edges", 0)  # the graphs are below the default threshold
             ctx.set_tuning("graph_loop", loop)
             ctx.upload_graph(n, m, ind, adj)
             ctx.upload_labels(lab)

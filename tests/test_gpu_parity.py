@@ -285,12 +285,14 @@ def test_propagation_bin_streams(ctx, kind, lam, streams):
     cc = O.count_triangles(n, ind, adj)[3] if kind == O.WCC else None
     rng = np.random.default_rng(8)
     ctx.set_tuning("bin_streams", streams)
+    ctx.set_tuning("bin_streams_min_edges", 0)  # the graph is below the default threshold
     try:
         for lab in [GG.singleton_labels(n), rng.integers(0, 40, n), rng.integers(0, 3000, n)]:
             _lp_compare(ctx, g, lab, kind, lam, 1.0, seed=4, step=3, tie=O.TIE_RANDOM, cc=cc)
             _lp_compare(ctx, g, lab, kind, lam, 0.5, seed=4, step=4, tie=O.TIE_LOWEST, cc=cc)
     finally:
         ctx.set_tuning("bin_streams", BIN_STREAMS_DEFAULT)
+        ctx.set_tuning("bin_streams_min_edges", 1 << 19)
 
 
 def test_hub_partitions_fit_with_default_tuning():
