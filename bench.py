@@ -326,8 +326,9 @@ def wcc_leg(parcd, np, steps, warmup):
     degrees from a power law with exponent 2.3 on [4, 5000], 32 planted
     communities of 100-50,000 vertices boosted x20), as `main -m wcc -i 1`:
     triangle counting, the triangle-based initial partition, then refinement
-    iterations (compress, statistics, WCC metric with its internal-triangle
-    pass, propagation).  Returns the graph arrays too (for the reference run)."""
+    iterations, one pcd_iterate call each (compress, statistics, WCC metric
+    with its internal-triangle pass, propagation).  Returns the graph arrays
+    too (for the reference run)."""
     t = time.time()
     n, m, ind, adj, mp = parcd.make_graph("chung_lu", 3, n=1000000, exponent=2.3, dmin=4, dmax=5000, num_comms=32,
                                           cmin=100, cmax=50000, boost=20.0)
@@ -348,21 +349,36 @@ def wcc_leg(parcd, np, steps, warmup):
     nc = ctx.clustering()  # the same partition on the device (the run uses it)
     t2 = time.perf_counter()
     same_partition = bool(np.array_equal(ctx.labels(), lab))
-    scores = []
-    for it in range(warmup):
-        ctx.compress(); ctx.statistics(); scores.append(ctx.metric(parcd.RULE_WCC))
-        ctx.propagate(parcd.RULE_WCC, frac=0.5, seed=1, step=it)
-    ctx.synchronize()
-    ctx.kernel_timer_reset()
-    t3 = time.perf_counter()
-    for it in range(warmup, warmup + steps):
-        ctx.compress(); ctx.statistics(); scores.append(ctx.metric(parcd.RULE_WCC))
-        ctx.propagate(parcd.RULE_WCC, frac=0.5, seed=1, step=it)
-    ctx.synchronize()
-    t4 = time.perf_counter()
+    init = ctx.labels()
+    jb = parcd.job(kind=parcd.RULE_WCC, frac=0.5, seed=1, tie=parcd.TIE_RANDOM, max_iter=warmup + steps + 1)
+
+    def passes(timed):
+        """pcd_iterate passes from the clustering partition (main.cpp's loop):
+        the timed pass runs without kernel timers (the metric's
+        internal-triangle pass then runs beside the propagation), the probe
+        pass with them."""
+        ctx.upload_labels(init)
+        ctx.enable_timing(timed)
+        sc, prev = [], 0.0
+        for it in range(warmup):
+            prev, _, done = ctx.iterate(jb, it, prev)
+            sc.append(prev)
+        ctx.synchronize()
+        ctx.kernel_timer_reset()
+        ta = time.perf_counter()
+        for it in range(warmup, warmup + steps):
+            prev, _, done = ctx.iterate(jb, it, prev)
+            sc.append(prev)
+            assert not done
+        ctx.synchronize()
+        return time.perf_counter() - ta, sc
+
+    secs, scores = passes(False)
+    _, scores_probe = passes(True)
     timers = ctx.kernel_timers()
     ctx.close()
-    it_ms = (t4 - t3) * 1000 / steps
+    assert scores == scores_probe
+    it_ms = secs * 1000 / steps
     return {
         "graph": "Chung-Lu seed 3, n=%d m=%d" % (n, m), "E": int(E), "cc": round(cc, 6),
         "triangles_ms": round((t1 - t0) * 1000, 3), "triangle_kernels_ms": round(tri_ms, 3),

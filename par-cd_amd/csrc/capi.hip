@@ -709,15 +709,17 @@ static int iterate_pass(pcd_ctx *ctx, const pcd_job *job, int strategy, int roun
     const bool known_done = (job->max_iter != -1 && it >= job->max_iter) || it >= cap;
     double s = 0.0;
     bool launched = false;
-    if (rule.kind == PCD_RULE_WCC || !ctx->ev_metric) {
+    if (!ctx->ev_metric) {
         phase_begin(ctx, PCD_PHASE_METRIC);
         PCD_TRY(metric_dev(ctx, rule, &s));
         phase_end(ctx, PCD_PHASE_METRIC);
     } else {
         // The metric on its own stream, forked after the compression: the
-        // propagation does not read it and starts at once; phase timing and
-        // kernel timers keep it on `stream` (their events are recorded there).
-        // The statistics update waits for it on the device below.
+        // propagation does not read it and starts at once (under WCC the
+        // metric's internal-triangle pass, its own scratch arrays, overlaps the
+        // whole propagation); phase timing and kernel timers keep it on
+        // `stream` (their events are recorded there).  The statistics update
+        // waits for it on the device below.
         const bool mfork = ctx->mstream && ctx->ev_mfork && !ctx->timing;
         hipStream_t main_stream = ctx->stream;
         if (mfork) {
