@@ -206,6 +206,7 @@ static void free_graph_state(pcd_ctx *ctx) {
     ctx->d_stats = ctx->d_stats_alt = nullptr;
     ctx->d_moved = nullptr;
     ctx->d_first = ctx->d_newid = ctx->d_active = ctx->d_hub_active = nullptr;
+    ctx->d_firstv = nullptr;
     ctx->d_scan_part = nullptr;
     ctx->scan_part_cap = 0;
     ctx->d_tri = ctx->d_tri_deg = ctx->d_et = ctx->d_lowpos = ctx->d_it = ctx->d_itd = nullptr;
@@ -462,6 +463,8 @@ extern "C" int pcd_graph_upload(pcd_ctx *ctx, int n, int m, const int32_t *indic
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_active, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_first, n + 1));
     PCD_TRY(dev_alloc_t(ctx, &ctx->d_newid, n + 1));
+    PCD_TRY(dev_alloc_t(ctx, &ctx->d_firstv, n + 1));
+    PCD_TRY(fill_i32(ctx, ctx->d_firstv, -1, n + 1));
     ctx->first_cap = n + 1;
     PCD_HIP(hipStreamSynchronize(ctx->stream));
     ctx->has_graph = true;

@@ -214,6 +214,10 @@ struct pcd_ctx {
     pcd::CommStats *d_stats_alt = nullptr;  // compression writes the renumbered statistics here
     int32_t *d_moved = nullptr;  // 8 list bounds, then the moved small-bin vertices and stats items
     int32_t *d_first = nullptr, *d_newid = nullptr;
+    // per compressed label, the vertex the last compression found first
+    // (n + 1 entries, each a vertex id or -1): the next compression's
+    // first-occurrence table starts from it where that vertex kept the label
+    int32_t *d_firstv = nullptr;
     int32_t *d_active = nullptr;   // per-bin active lists (bin ranges)
     int32_t *d_counts = nullptr;   // per-bin active counts + misc device counters
     int32_t *d_hub_active = nullptr;

@@ -13,6 +13,17 @@ namespace pcd {
 // A plain read of first[l] skips the atomic when it already holds a smaller
 // vertex: the value only decreases, so a stale read costs at most an
 // atomic, and large communities do not serialise every member's update.
+// The table's start: the vertex the last compression found first for label
+// c, if it still has label c (an upper bound of the new first occurrence, so
+// most members' reads skip their atomic, large communities included), else
+// INT32_MAX.  firstv holds vertex ids or -1.
+__global__ void k_first_init(int bound, const int32_t *labels, const int32_t *firstv, int32_t *first) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < bound; c += gridDim.x * blockDim.x) {
+        const int w = firstv[c];
+        first[c] = (w >= 0 && labels[w] == c) ? w : INT32_MAX;
+    }
+}
+
 __global__ void k_first_occurrence(int n, const int32_t *labels, int32_t *first) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         int l = labels[v];
@@ -36,9 +47,11 @@ struct FirstStore {
     const CommStats *st_in;
     CommStats *st_out;
     const int32_t *indices;
+    int32_t *firstv;  // firstv[new id] = its first vertex
     __device__ void operator()(int64_t v, int ex, int flag) const {
         if (!flag) return;
         int l = labels[v];
+        firstv[ex] = (int)v;
         if (l < 0) out[v] = ex;
         else newid[l] = ex;
         if (st_in) st_out[ex] = l < 0 ? CommStats{1, 0, indices[v + 1] - indices[v], 0} : st_in[l];
@@ -73,7 +86,11 @@ void resolve_k(pcd_ctx *ctx) {
 int compress_labels_dev(pcd_ctx *ctx, bool defer_k) {
     const int n = ctx->n;
     kt_begin(ctx, KS_COMPRESS);
-    if (ctx->label_bound > 0) PCD_TRY(fill_i32(ctx, ctx->d_first, INT32_MAX, ctx->label_bound));
+    if (ctx->label_bound > 0 && ctx->label_bound <= n + 1)
+        hipLaunchKernelGGL(k_first_init, dim3(grid_for(ctx, ctx->label_bound, 256)), dim3(256), 0, ctx->stream,
+                           ctx->label_bound, ctx->d_labels, ctx->d_firstv, ctx->d_first);
+    else if (ctx->label_bound > 0)
+        PCD_TRY(fill_i32(ctx, ctx->d_first, INT32_MAX, ctx->label_bound));
     hipLaunchKernelGGL(k_first_occurrence, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_labels,
                        ctx->d_first);
     // statistics of these labels (full count, or updated by the last
@@ -82,7 +99,7 @@ int compress_labels_dev(pcd_ctx *ctx, bool defer_k) {
     int32_t *total = nullptr;
     PCD_TRY(device_scan(ctx, n, FirstFlag{ctx->d_labels, ctx->d_first},
                         FirstStore{ctx->d_labels, ctx->d_labels_alt, ctx->d_newid, renumber ? ctx->d_stats : nullptr,
-                                   ctx->d_stats_alt, ctx->d_indices},
+                                   ctx->d_stats_alt, ctx->d_indices, ctx->d_firstv},
                         &total));
     hipLaunchKernelGGL(k_relabel, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_labels,
                        ctx->d_newid, ctx->d_labels_alt);
