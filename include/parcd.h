@@ -3,7 +3,10 @@
  *
  * Plain C types only: int32 ids and offsets, host pointers that are borrowed
  * for the duration of a call, an opaque context that owns all device memory
- * and one HIP stream.  Every function returns PCD_OK (0) or a negative
+ * and its HIP streams: one that orders every call, plus side streams that a
+ * propagation step and pcd_iterate's metric fork onto and join back into it
+ * with events, so results are ordered as on one stream.  A context is used
+ * by one thread at a time.  Every function returns PCD_OK (0) or a negative
  * PCD_E* status; pcd_last_error() gives the message of the last failure on
  * the calling thread.  Nothing here exits the process or throws.
  *
