@@ -257,28 +257,38 @@ ITER_CAP = 100
 
 
 def _merge_iterations(parcd, np, ctx, E, init, steps, warmup, seed, merge_sort):
+    """-s 1 passes, one pcd_iterate call each (compress, statistics, metric,
+    stop rule, one merging round): a timed pass without kernel timers, then a
+    probe pass from the same partition with them (its per-kernel times); both
+    must give the same scores and community counts."""
     ctx.set_tuning("merge_sort", merge_sort)
-    ctx.upload_labels(init)
-    ctx.enable_timing(True)
-    ks = []
-    it = 0
-    for _ in range(warmup):
-        ctx.compress(); ctx.statistics(); ctx.metric(parcd.RULE_MODULARITY)
-        ctx.merge(parcd.RULE_MODULARITY, rounds=1, seed=seed, step=it)
-        it += 1
-    ctx.synchronize()
-    ctx.kernel_timer_reset()
-    scores = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        ks.append(ctx.compress()); ctx.statistics(); scores.append(ctx.metric(parcd.RULE_MODULARITY))
-        ctx.merge(parcd.RULE_MODULARITY, rounds=1, seed=seed, step=it)
-        it += 1
-    ctx.synchronize()
-    t1 = time.perf_counter()
+    jb = parcd.job(kind=parcd.RULE_MODULARITY, frac=0.5, seed=seed, tie=parcd.TIE_RANDOM,
+                   max_iter=warmup + steps + 1)
+
+    def passes(timed):
+        ctx.upload_labels(init)
+        ctx.enable_timing(timed)
+        ks, sc, prev = [], [], 0.0
+        for it in range(warmup):
+            prev, _, _ = ctx.iterate(jb, it, prev, strategy=1, rounds=1)
+        ctx.synchronize()
+        ctx.kernel_timer_reset()
+        t0 = time.perf_counter()
+        for it in range(warmup, warmup + steps):
+            prev, k, done = ctx.iterate(jb, it, prev, strategy=1, rounds=1)
+            assert not done
+            sc.append(prev)
+            ks.append(k)
+        ctx.synchronize()
+        return time.perf_counter() - t0, sc, ks
+
+    secs, scores, ks = passes(False)
+    _, scores_p, ks_p = passes(True)
     timers = ctx.kernel_timers()
+    ctx.enable_timing(False)
     ctx.set_tuning("merge_sort", 0)
-    ms = (t1 - t0) * 1000 / steps
+    assert scores == scores_p and ks == ks_p
+    ms = secs * 1000 / steps
     return {
         "ms_per_iteration": round(ms, 3), "gteps": round(E / (ms / 1000) / 1e9, 2),
         "modularity_scores": [round(x, 6) for x in scores], "communities": ks,
@@ -288,8 +298,8 @@ def _merge_iterations(parcd, np, ctx, E, init, steps, warmup, seed, merge_sort):
 
 def merge_leg(parcd, np, ctx, n, m, init, steps, warmup, seed):
     """Parallel merging (-s 1, one round) on R-MAT-22 from the singleton
-    partition: compress + statistics + metric + merging step per iteration,
-    the same counting as the propagation iterations.
+    partition: compress + statistics + metric + merging step per iteration
+    (one pcd_iterate call), the same counting as the propagation iterations.
 
     The partner choice counts over community rows (the default): the
     vertices sorted by label (members), every entry's label gathered into its
