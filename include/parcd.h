@@ -194,8 +194,9 @@ int pcd_bin_counters(pcd_ctx *ctx, int bin, int64_t *active, int64_t *entries, i
  * (512 or 256: hub partitions are counted by 512-thread workgroups with
  * 4096-slot tables or 256-thread ones with 2048), "small_fused" (0/1: the
  * small-degree bins in one launch), "small_occ8" (0/1: that launch built
- * for 8 waves per SIMD, modularity and CPM), "b1024_u16" (0/1: the
- * degree 257-512 wave tables keep 16-bit counts), "bin_streams" (1, the
+ * for 8 waves per SIMD, modularity and CPM), "b1024_u16" and "b4096_u16"
+ * (0/1: the degree 257-512 and 1025-4096 tables keep 16-bit counts),
+ * "bin_streams" (1, the
  * default: a propagation step runs the hub rows and degrees 1025-4096 on two
  * side streams of the context, beside the other bins; 0: one stream) and
  * "bin_streams_min_edges" (graphs with fewer edges keep one stream; default

@@ -866,6 +866,8 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.inc_stats = value != 0;
     } else if (k == "b1024_wave") {
         ctx->tune.b1024_wave = value != 0;
+    } else if (k == "b4096_u16") {
+        ctx->tune.b4096_u16 = value != 0;
     } else if (k == "b4096_split") {
         ctx->tune.b4096_split = value != 0;
     } else if (k == "cluster_rounds") {

@@ -99,6 +99,8 @@ struct Tuning {
                                  // 1024-slot table (twice the vertices in flight per CU)
     int b1024_u16 = 1;           // those wave tables keep 16-bit counts (five workgroups per
                                  // CU instead of four)
+    int b4096_u16 = 1;           // degrees 1025-4096: tables with 16-bit counts (56 instead of
+                                 // 72 KB of LDS: beside the other bins, 1.012 -> 1.005 ms)
     int b4096_split = 0;         // degrees 1025-2048 of B4096 run 256-thread workgroups with a
                                  // 4096-slot table (measured slower on R-MAT-22: off)
     int cluster_rounds = 128;    // device rounds of the clustering partition before the

@@ -654,10 +654,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
 
 
 // Table slots: nextpow2(2 deg), at most TB.
-template <int NT, int TB, unsigned KM, class Src, bool AUX = false, int PF = 4>
+// CT = uint16_t needs rows below 2^16 entries (the table's counts).
+template <int NT, int TB, unsigned KM, class Src, bool AUX = false, int PF = 4, class CT = int>
 __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
     __shared__ int keys[TB];
-    __shared__ int cnts[TB];
+    __shared__ __align__(16) CT cnts[TB];
     __shared__ int auxs[AUX ? TB : 1];
     __shared__ uint16_t occ[TB / 2];
     __shared__ int s_nocc;
@@ -681,14 +682,14 @@ __global__ __launch_bounds__(NT) void k_lp_block(Src src) {
         int2 ownr = AUX ? a.lv[v] : make_int2(own_label<KM>(a, v), 0);
         int own = ownr.x;
         int log_t = max(6, ceil_log2(2 * deg));
-        insert_labels<PF, KM, AUX>(a, keys, cnts, auxs, occ, &s_nocc, (1 << log_t) - 1, log_t, start, deg, threadIdx.x,
-                                  NT, own);
+        insert_labels<PF, KM, AUX, CT>(a, keys, cnts, auxs, occ, &s_nocc, (1 << log_t) - 1, log_t, start, deg,
+                                      threadIdx.x, NT, own);
         __syncthreads();
         const int D = s_nocc;
         uint64_t tb = a.tie == PCD_TIE_LOWEST ? 0 : tie_base(a.seed, step_of(a), v);
         Cand c = cand_none();
         int own_cnt = 0, distinct = 0;
-        scan_listed<AUX, KM>(a, keys, cnts, auxs, occ, D, threadIdx.x, NT, deg, own, tb, c, own_cnt);
+        scan_listed<AUX, KM, CT>(a, keys, cnts, auxs, occ, D, threadIdx.x, NT, deg, own, tb, c, own_cnt);
         block_reduce_cand<NT>(c, own_cnt, distinct, s_c, s_i);
         if (threadIdx.x == 0) {
             if constexpr ((KM & RK_MERGE) != 0) a.new_labels[v] = merge_partner(c);
@@ -1248,7 +1249,12 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         // 1025-2048: 256-thread workgroups with 4096-slot tables (four per CU)
         const int ps = ctx->tune.b4096_split && !MERGE && !dev ? std::min(std::max(b.split2048, p0), p1) : p1;
         kt_begin(ctx, KS_LP_B4096);
-        if (ps > p0 || dev)
+        // 16-bit counts (rows of at most 4096 entries): 56 instead of 72 KB of
+        // LDS, room on a CU for workgroups of the bins running beside it
+        if ((ps > p0 || dev) && ctx->tune.b4096_u16)
+            hipLaunchKernelGGL((k_lp_block<512, 8192, KM, Src, false, 4, uint16_t>), dim3(grid(ps - p0, 1, 2)),
+                               dim3(512), 0, bs, src(BIN_B4096, p0, ps, 1));
+        else if (ps > p0 || dev)
             hipLaunchKernelGGL((k_lp_block<512, 8192, KM, Src>), dim3(grid(ps - p0, 1, 2)), dim3(512), 0,
                                bs, src(BIN_B4096, p0, ps, 1));
         if constexpr (!MERGE) {

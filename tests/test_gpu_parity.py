@@ -167,12 +167,12 @@ def test_propagation_bin_zoo(ctx, kind, lam, tie, aux):
 
 
 @pytest.mark.parametrize("kind,lam", RULES)
-@pytest.mark.parametrize("knob", ["small_fused", "small_occ8", "b1024_u16"])
+@pytest.mark.parametrize("knob", ["small_fused", "small_occ8", "b1024_u16", "b4096_u16"])
 def test_propagation_other_launch_forms(ctx, kind, lam, knob):
     """The small-degree bins launched one kernel each (small_fused 0), the
     fused launch built for the compiler's own occupancy (small_occ8 0), and
-    the degree 257-512 wave tables with 32-bit counts (b1024_u16 0) agree with
-    the oracle as the default launches do."""
+    the degree 257-512 (b1024_u16 0) and 1025-4096 (b4096_u16 0) tables with
+    32-bit counts agree with the oracle as the default launches do."""
     g = GG.bin_zoo()
     n = g[0]
     rng = np.random.default_rng(9)
