@@ -114,7 +114,7 @@ KERNELS = {
 KERNEL_NAMES = {
     "lp_small": "pcd::k_lp_small_all<1u, 8, false, pcd::RangeSrc>",
     "lp_b512": "pcd::k_lp_wave<1024, 1u, pcd::RangeSrc, false, 8, unsigned short>",
-    "lp_b1024": "pcd::k_lp_block<256, 2048, 1u, pcd::RangeSrc, false, 4>",
+    "lp_b1024": "pcd::k_lp_block<256, 2048, 1u, pcd::RangeSrc, false, 4, int>",
     "lp_b4096": "pcd::k_lp_block<512, 8192, 1u, pcd::RangeSrc, false, 4, unsigned short>",
     "lp_hub_partition": "pcd::k_hub_partition<1u, false>",
     "lp_hub_count": "pcd::k_hub_count_listed<1u, 512, 4096, false>",
