@@ -802,8 +802,17 @@ def main():
         if tms <= 0 or launches == 0 or not work:
             continue
         byt = sum(model(a_, e_, d_, S) for a_, e_, d_ in work)
+        # random gathers: each active entry's label, each distinct label's
+        # total (the hub partition pass gathers the labels, the count pass the
+        # totals)
+        gathers = sum(w[1] + w[2] for w in work)
+        if slot == "lp_hub_partition":
+            gathers = sum(w[1] for w in work)
+        elif slot == "lp_hub_count":
+            gathers = sum(w[2] for w in work)
         per_kernel[slot] = {"kernel": KERNEL_NAMES[slot], "bytes_per_launch": byt / launches,
-                            "avg_ms": tms / launches, "entries_per_launch": sum(w[1] for w in work) / launches}
+                            "avg_ms": tms / launches, "entries_per_launch": sum(w[1] for w in work) / launches,
+                            "gathers_per_launch": gathers / launches}
     # statistics over rows longer than 64 (work items, the full count of the
     # first iteration): per entry the neighbour id and its label, per row its
     # extent and label and one 12-byte record update
@@ -822,9 +831,19 @@ def main():
             d_["avg_ms_probe_pass"] = d_["avg_ms"]
             d_["avg_ms"] = t_ms / t_launches
             d_["measured_in"] = "roofline pass"
+    # the gather roof of this graph: random label gathers over its own
+    # adjacency with nothing between them (one level, and a dependent second
+    # one by community id, as the gains read the community totals)
+    gather_roof = {"one_level_gathers_per_s": ctx.gather_probe(1, 10), "two_level_gathers_per_s":
+                   ctx.gather_probe(2, 10), "method": "pcd_gather_probe over the R-MAT-22 adjacency with the labels "
+                   "after the timed iterations, 8 neighbour ids in flight per lane"}
+    log("gather roof:", gather_roof)
     for slot, d_ in per_kernel.items():
         d_["gbs"] = d_["bytes_per_launch"] / (d_["avg_ms"] / 1000.0) / 1e9
         d_["frac"] = round(d_["gbs"] / HBM_PEAK_GBS, 4)
+        if d_.get("gathers_per_launch"):
+            d_["gathers_per_s"] = d_["gathers_per_launch"] / (d_["avg_ms"] / 1000.0)
+            d_["of_gather_roof"] = round(d_["gathers_per_s"] / gather_roof["two_level_gathers_per_s"], 3)
         tr = pmc_traffic(slot) if args.scale == 22 else None
         d_["traffic_per_launch"] = int(tr) if tr else None
         d_["traffic_over_algorithmic"] = round(tr / d_["bytes_per_launch"], 2) if tr else None
@@ -937,6 +956,7 @@ def main():
                    "step": "compress+statistics+metric+label propagation", "teps": "2m entries per iteration"},
         "roofline": roofline, "cpu_baseline": cpu, "stream_copy_gbs": round(copy_gbs, 1),
         "speedup_vs_cpu": round(gteps / cpu["value"], 2) if cpu else None,
+        "gather_roof": gather_roof,
         "phases": {"propagate": lp_phase, "statistics": stats_phase, "kernels": per_kernel,
                    "kernel_times": "probe pass: every kernel timed alone, every bin on one stream",
                    "bin_streams": bin_streams},

@@ -209,6 +209,15 @@ int pcd_hub_overflows(pcd_ctx *ctx, int64_t *count);
 /* Streaming device-to-device copy of `bytes` (16-byte accesses), `reps`
  * times: *gbs receives the attained read+write bandwidth in GB/s. */
 int pcd_copy_probe(pcd_ctx *ctx, int64_t bytes, int reps, double *gbs);
+/* Random 4-byte gathers over the uploaded graph's own adjacency, `reps`
+ * times: every entry's neighbour id (coalesced) and its label, with
+ * levels = 2 that label's label as well (a dependent gather by community id,
+ * as the gains read community totals).  *gathers_per_s receives the rate
+ * (E x levels gathers per rep): the gather roof the propagation kernels run
+ * against, with no hashing or scoring between the gathers.  Needs labels;
+ * labels outside [0, n) are not followed.  No counterpart in the reference:
+ * a measurement probe, like pcd_copy_probe. */
+int pcd_gather_probe(pcd_ctx *ctx, int levels, int reps, double *gathers_per_s);
 /* Vertices per degree bin of the uploaded graph (bin layout). */
 int pcd_bin_size(pcd_ctx *ctx, int bin, int64_t *vertices, int64_t *entries);
 

@@ -995,6 +995,40 @@ extern "C" int pcd_copy_probe(pcd_ctx *ctx, int64_t bytes, int reps, double *gbs
     return PCD_OK;
 }
 
+namespace pcd {
+__global__ void k_gather_probe(int64_t E, const int32_t *adj, const int32_t *labels, int n, int levels,
+                               unsigned long long *sink);
+}
+
+extern "C" int pcd_gather_probe(pcd_ctx *ctx, int levels, int reps, double *gathers_per_s) {
+    PCD_TRY(check_labels(ctx));
+    if ((levels != 1 && levels != 2) || reps < 1 || !gathers_per_s)
+        return fail(PCD_E_INVALID, "bad gather probe arguments");
+    const int64_t E = (int64_t)2 * ctx->m;
+    unsigned long long *sink = nullptr;
+    PCD_TRY(dev_alloc_t(ctx, &sink, 1));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int grid = ctx->cu_count * 8;
+    hipLaunchKernelGGL(k_gather_probe, dim3(grid), dim3(256), 0, ctx->stream, E, ctx->d_adj, ctx->d_labels, ctx->n,
+                       levels, sink);  // warm-up
+    (void)hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_gather_probe, dim3(grid), dim3(256), 0, ctx->stream, E, ctx->d_adj, ctx->d_labels,
+                           ctx->n, levels, sink);
+    (void)hipEventRecord(e1, ctx->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dev_free(ctx, sink);
+    if (e != hipSuccess) return fail(PCD_E_HIP, std::string("gather probe: ") + hipGetErrorString(e));
+    *gathers_per_s = (double)E * levels * reps / (ms / 1000.0);
+    return PCD_OK;
+}
+
 extern "C" int pcd_host_alloc(int64_t bytes, void **p) {
     if (!p || bytes <= 0) return fail(PCD_E_INVALID, "bad arguments");
     *p = nullptr;

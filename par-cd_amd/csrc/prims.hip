@@ -49,6 +49,37 @@ __global__ __launch_bounds__(256) void k_stream_copy(const int4 *__restrict__ sr
         dst[i] = src[i];
 }
 
+// Random gathers over a graph's own adjacency: every entry's neighbour id
+// (coalesced, PF in flight per lane as the propagation kernels keep them),
+// its label, and with two levels that label's label (a dependent gather into
+// an array indexed by community id, as the gains' community totals are).
+// Labels outside [0, n) are not followed.  The sum only keeps the gathers
+// from being dropped.
+__global__ __launch_bounds__(256) void k_gather_probe(int64_t E, const int32_t *adj, const int32_t *labels, int n,
+                                                      int levels, unsigned long long *sink) {
+    constexpr int PF = 8;
+    unsigned acc = 0;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x * PF;
+    for (int64_t e0 = (int64_t)blockIdx.x * blockDim.x * PF + threadIdx.x; e0 < E; e0 += step) {
+        int x[PF];
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int64_t e = e0 + (int64_t)k * blockDim.x;
+            x[k] = e < E ? adj[e] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < PF; k++) x[k] = (x[k] >= 0 && x[k] < n) ? labels[x[k]] : -1;
+        if (levels == 2) {
+#pragma unroll
+            for (int k = 0; k < PF; k++) x[k] = (x[k] >= 0 && x[k] < n) ? labels[x[k]] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < PF; k++) acc += (unsigned)x[k];
+    }
+    acc = (unsigned)wave_sum((int)acc);
+    if (lane_id() == 0) atomicAdd(sink, (unsigned long long)acc);
+}
+
 int ensure_scan_parts(pcd_ctx *ctx, int64_t n) {
     int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (tiles < 1) tiles = 1;

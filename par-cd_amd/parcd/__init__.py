@@ -26,7 +26,7 @@ EXPORTS = [
     "pcd_convert_edges", "pcd_free", "pcd_gen_sbm", "pcd_gen_rmat", "pcd_gen_chung_lu",
     "pcd_kernel_slot_count", "pcd_kernel_slot_name", "pcd_kernel_timer_reset", "pcd_kernel_timer",
     "pcd_bin_count", "pcd_bin_name", "pcd_bin_counters_reset", "pcd_bin_counters", "pcd_bin_size",
-    "pcd_singleton_partition", "pcd_set_tuning", "pcd_hub_overflows", "pcd_copy_probe",
+    "pcd_singleton_partition", "pcd_set_tuning", "pcd_hub_overflows", "pcd_copy_probe", "pcd_gather_probe",
     "pcd_batch_create", "pcd_batch_destroy", "pcd_batch_footprint", "pcd_batch_add", "pcd_batch_add_many",
     "pcd_batch_labels_many", "pcd_batch_step", "pcd_nmi", "pcd_nmi_file", "pcd_clustering_labels",
     "pcd_kernel_timer_select",
@@ -149,6 +149,7 @@ def lib():
     L.pcd_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
     L.pcd_hub_overflows.argtypes = [vp, C.POINTER(C.c_int64)]
     L.pcd_copy_probe.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    L.pcd_gather_probe.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
     ip, dp, lp = C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int64)
     L.pcd_batch_create.argtypes = [C.c_int, C.c_int, C.c_int64, C.POINTER(vp)]
     L.pcd_batch_destroy.argtypes = [vp]
@@ -627,6 +628,13 @@ class Context:
     def copy_probe(self, nbytes=1 << 30, reps=10):
         g = C.c_double()
         check(lib().pcd_copy_probe(self._ctx, int(nbytes), int(reps), C.byref(g)))
+        return g.value
+
+    def gather_probe(self, levels=1, reps=10):
+        """Random gathers per second over the graph's adjacency (labels, and
+        with levels=2 the labels of those labels)."""
+        g = C.c_double()
+        check(lib().pcd_gather_probe(self._ctx, int(levels), int(reps), C.byref(g)))
         return g.value
 
     def hub_overflows(self):

@@ -295,6 +295,21 @@ def test_propagation_bin_streams(ctx, kind, lam, streams):
         ctx.set_tuning("bin_streams_min_edges", 1 << 19)
 
 
+def test_gather_probe(ctx):
+    """pcd_gather_probe runs over the uploaded graph at one and two levels
+    (labels outside [0, n), NEW_LABEL here, are not followed) and refuses
+    other levels."""
+    g = GG.bin_zoo()
+    n, m, ind, adj = g[:4]
+    ctx.upload_graph(n, m, ind, adj)
+    lab = GG.singleton_labels(n)
+    lab[::7] = -1
+    ctx.upload_labels(lab)
+    assert ctx.gather_probe(1, 2) > 0 and ctx.gather_probe(2, 2) > 0
+    with pytest.raises(Exception):
+        ctx.gather_probe(3, 2)
+
+
 def test_hub_partitions_fit_with_default_tuning():
     """A fresh context's default tuning counts hub partitions in LDS (no
     overflow into the global-table fallback) on hubs of 5k-14k entries."""
