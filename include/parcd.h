@@ -247,6 +247,11 @@ typedef struct pcd_job {
 #define PCD_BATCH_RUNNING 0
 #define PCD_BATCH_DONE 1
 #define PCD_BATCH_FREE 2
+/* admitted by pcd_batch_add_many_async, uploads not yet joined */
+#define PCD_BATCH_PENDING 3
+/* admitted by pcd_batch_add_many_async, rows failed the device check
+ * (PCD_E_FORMAT); the slot holds no arena space until released */
+#define PCD_BATCH_REJECTED 4
 
 /* One pass of the main.cpp:226-266 loop body (main.cpp:229-260) on the
  * context's graph and labels: compress, statistics, metric (*score), the stop
@@ -311,8 +316,25 @@ typedef struct pcd_graph_ref {
  * is, one reference main.cpp run per graph (main.cpp:170-225). */
 int pcd_batch_add_many(pcd_batch *b, int count, const pcd_graph_ref *graphs, const pcd_job *jobs, int *slots,
                        int *codes);
+/* pcd_batch_add_many without the wait, so that admission overlaps the
+ * rounds of graphs already running.  The host checks run and the slots are
+ * given out before the call returns (slots[i], codes[i] and the return code
+ * as pcd_batch_add_many's for everything checked on the host); the uploads
+ * and the device row check are queued on the batch's admission stream, 256
+ * graphs to a chunk, and the graphs are PCD_BATCH_PENDING.  A later
+ * pcd_batch_step joins every chunk whose uploads are done (waiting for the
+ * first when no graph is running): a joined graph runs from the next round
+ * on (PCD_BATCH_RUNNING), or is PCD_BATCH_REJECTED when its rows failed the
+ * device check.  The arrays of every pending graph must stay valid and
+ * unchanged until it has left PCD_BATCH_PENDING.  Labels (and the degree
+ * order the host computes) from pageable memory are staged in page-locked
+ * memory by the call; indices and adj are read by the device directly when
+ * they are page-locked (pcd_host_alloc), else copied on the admission
+ * stream.  A second call first joins the chunks of the previous one. */
+int pcd_batch_add_many_async(pcd_batch *b, int count, const pcd_graph_ref *graphs, const pcd_job *jobs, int *slots,
+                             int *codes);
 /* Runs `rounds` rounds; *running receives the number of graphs still
- * running afterwards. */
+ * running afterwards, pending ones (pcd_batch_add_many_async) included. */
 int pcd_batch_step(pcd_batch *b, int rounds, int *running);
 /* State after the last step: PCD_BATCH_*, iterations (main.cpp's final
  * `it`), communities and the last score (the value main.cpp:273 prints). */
@@ -325,7 +347,8 @@ int pcd_batch_labels(pcd_batch *b, int slot, int32_t *labels);
  * of `slots` (graph i's n labels follow graph i-1's). */
 int pcd_batch_labels_many(pcd_batch *b, int count, const int *slots, int32_t *labels);
 int pcd_batch_stats(pcd_batch *b, int slot, int32_t *sizes, int32_t *int_volume, int32_t *ext_volume);
-/* Frees a slot and its arena space (a running graph is abandoned). */
+/* Frees a slot and its arena space (a running graph is abandoned; a
+ * pending one is waited for first). */
 int pcd_batch_release(pcd_batch *b, int slot);
 int pcd_batch_info(pcd_batch *b, int64_t *free_bytes, int *graphs, int *running, int64_t *rounds);
 /* Bin maps: which propagation kernel serves which degree range. */

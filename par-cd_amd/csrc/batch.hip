@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <vector>
 
@@ -490,7 +491,19 @@ __global__ __launch_bounds__(256) void k_b_pull(const PullPiece *pieces, int cou
             int4 *d4 = (int4 *)pc.dst;
             for (int64_t w = threadIdx.x; w < pc.words / 4; w += blockDim.x) d4[w] = s4[w];
         } else {
-            for (int64_t w = threadIdx.x; w < pc.words; w += blockDim.x) pc.dst[w] = pc.src[w];
+            // unaligned pieces: four loads in flight per lane before the stores
+            const int32_t *__restrict__ src = pc.src;
+            int32_t *__restrict__ dst = pc.dst;
+            const int64_t st = blockDim.x;
+            int64_t w = threadIdx.x;
+            for (; w + 3 * st < pc.words; w += 4 * st) {
+                const int32_t a0 = src[w], a1 = src[w + st], a2 = src[w + 2 * st], a3 = src[w + 3 * st];
+                dst[w] = a0;
+                dst[w + st] = a1;
+                dst[w + 2 * st] = a2;
+                dst[w + 3 * st] = a3;
+            }
+            for (; w < pc.words; w += st) dst[w] = src[w];
         }
     }
 }
@@ -623,6 +636,32 @@ struct pcd_batch {
     WorkItem *d_check = nullptr;   // row-check items of pcd_batch_add_many
     size_t check_cap = 0;
     int32_t *d_bad = nullptr;      // per slot: rows failed the check
+    // asynchronous admission (pcd_batch_add_many_async): uploads and row
+    // checks on their own stream, one event per chunk of graphs; the
+    // buffers belong to the last call and are reused once its chunks joined
+    struct AChunk {
+        int first = 0, count = 0;  // call-local graph indices
+        hipEvent_t ev = nullptr;
+    };
+    hipStream_t astream = nullptr;
+    std::vector<hipEvent_t> aevents;
+    std::deque<AChunk> achunks;  // queued, not yet joined
+    std::vector<int> aslots;     // call-local graph -> slot (-1: refused on the host)
+    int apending = 0;            // graphs in BG_PENDING
+    PullPiece *d_apulls = nullptr;
+    size_t apulls_cap = 0;
+    WorkItem *d_acheck = nullptr;
+    size_t acheck_cap = 0;
+    BatchGraph *d_arecs = nullptr;  // row-check records, call-local
+    size_t arecs_cap = 0;
+    int32_t *d_abad = nullptr, *h_abad = nullptr;
+    size_t abad_cap = 0;
+    int32_t *h_astage = nullptr;  // page-locked staging of pageable labels and degree orders
+    size_t astage_cap = 0;
+    // buffers replaced while admission chunks were queued: hipFree and
+    // hipHostFree wait for the whole device, the queued uploads included,
+    // so they are freed once the queue has drained
+    std::vector<void *> retired_dev, retired_host;
     bool items_dirty = true;
     int64_t listed_done_vertices = 0, listed_vertices = 0;
     // bin map and captured round
@@ -681,6 +720,26 @@ static void drop_graph(pcd_batch *b) {
     b->graph = nullptr;
 }
 
+// Frees a buffer the batch no longer uses, or holds it back while
+// admission chunks are queued (see retired_dev).
+static void free_device(pcd_batch *b, void *p) {
+    if (!p) return;
+    if (b->achunks.empty()) (void)hipFree(p);
+    else b->retired_dev.push_back(p);
+}
+static void free_host(pcd_batch *b, void *p) {
+    if (!p) return;
+    if (b->achunks.empty()) (void)hipHostFree(p);
+    else b->retired_host.push_back(p);
+}
+static void free_retired(pcd_batch *b) {
+    if (!b->achunks.empty()) return;
+    for (void *p : b->retired_dev) (void)hipFree(p);
+    for (void *p : b->retired_host) (void)hipHostFree(p);
+    b->retired_dev.clear();
+    b->retired_host.clear();
+}
+
 extern "C" int pcd_batch_map_count(void) { return NUM_MAPS; }
 extern "C" const char *pcd_batch_map_name(int map) { return (map >= 0 && map < NUM_MAPS) ? MAPS[map].name : ""; }
 
@@ -693,9 +752,20 @@ extern "C" int pcd_batch_footprint(int n, int m, int64_t *bytes) {
 extern "C" int pcd_batch_destroy(pcd_batch *b) {
     if (!b) return PCD_OK;
     (void)hipSetDevice(b->device);
+    if (b->astream) (void)hipStreamSynchronize(b->astream);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
     if (b->arena) (void)hipFree(b->arena);
+    if (b->d_apulls) (void)hipFree(b->d_apulls);
+    if (b->d_acheck) (void)hipFree(b->d_acheck);
+    if (b->d_arecs) (void)hipFree(b->d_arecs);
+    if (b->d_abad) (void)hipFree(b->d_abad);
+    if (b->h_abad) (void)hipHostFree(b->h_abad);
+    if (b->h_astage) (void)hipHostFree(b->h_astage);
+    for (void *p : b->retired_dev) (void)hipFree(p);
+    for (void *p : b->retired_host) (void)hipHostFree(p);
+    for (hipEvent_t e : b->aevents) (void)hipEventDestroy(e);
+    if (b->astream) (void)hipStreamDestroy(b->astream);
     if (b->d_graphs) (void)hipFree(b->d_graphs);
     if (b->d_state) (void)hipFree(b->d_state);
     if (b->d_gtiles) (void)hipFree(b->d_gtiles);
@@ -877,17 +947,30 @@ static bool device_readable(const void *p) {
 }
 
 // Queues a host-to-arena copy of `words` int32: a pull piece (flush_pulls
-// launches them) for device-readable sources, else a stream copy.
+// launches them) for device-readable sources, else a copy on `stream`.
 constexpr size_t PULL_WORDS = 1 << 18;  // 1 MiB pieces
-static int upload_words(pcd_batch *b, int32_t *dst, const int32_t *src, size_t words) {
+static void pull_pieces(pcd_batch *b, int32_t *dst, const int32_t *src, size_t words) {
+    for (size_t o = 0; o < words; o += PULL_WORDS)
+        b->pulls.push_back(PullPiece{src + o, dst + o, (int64_t)std::min(PULL_WORDS, words - o)});
+}
+static int upload_words(pcd_batch *b, int32_t *dst, const int32_t *src, size_t words, hipStream_t stream) {
     if (words == 0) return PCD_OK;
     if (device_readable(src) && device_readable(src + words - 1)) {
-        for (size_t o = 0; o < words; o += PULL_WORDS)
-            b->pulls.push_back(PullPiece{src + o, dst + o, (int64_t)std::min(PULL_WORDS, words - o)});
+        pull_pieces(b, dst, src, words);
         return PCD_OK;
     }
-    PCD_HIP(hipMemcpyAsync(dst, src, 4 * words, hipMemcpyHostToDevice, b->stream));
+    PCD_HIP(hipMemcpyAsync(dst, src, 4 * words, hipMemcpyHostToDevice, stream));
     return PCD_OK;
+}
+
+// Copies `words` int32 into the page-locked staging buffer at *cursor and
+// queues a pull piece from there (asynchronous admission of pageable
+// labels and of the degree orders the host computes).
+static void stage_words(pcd_batch *b, int32_t *dst, const int32_t *src, size_t words, int32_t **cursor) {
+    if (words == 0) return;
+    std::memcpy(*cursor, src, 4 * words);
+    pull_pieces(b, dst, *cursor, words);
+    *cursor += words;
 }
 
 // Launches the queued pull pieces on the batch stream.  The host sources
@@ -915,12 +998,31 @@ static int flush_pulls(pcd_batch *b) {
     return PCD_OK;
 }
 
+// A slot's graph starts: its state record (host copy) is reset to running
+// and it is listed for the next round.
+static void start_graph(pcd_batch *b, int s) {
+    BatchState &st = b->h_state[s];
+    std::memset(&st, 0, sizeof(st));
+    st.status = BG_RUNNING;
+    st.label_bound = b->slots[s].n;
+    // items of the slot's previous graph may still be listed: rebuild then
+    if (b->items_cap == 0 || b->slot_listed[s]) b->items_dirty = true;
+    if (!b->items_dirty) b->pending_append.push_back(s);
+}
+
 // Places a prepared graph in the first free slot at or after *hint and
 // queues its uploads on the batch stream.  The slot's records (graph and
 // state) are written on the host only; upload_records sends them.  The
 // caller synchronises the stream before the host buffers may change.
+// With `stage` (asynchronous admission) the copies go to the admission
+// stream, the degree order and, with stage_labels, the labels through the
+// staging buffer; the slot is left BG_PENDING, outside the item lists and
+// with its state record untouched, until join_chunk takes it in.
 static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, const int32_t *adj,
-                          const int32_t *labels, const pcd_job *job, Prepared &pr, int *hint, int *slot_out) {
+                          const int32_t *labels, const pcd_job *job, Prepared &pr, int *hint, int *slot_out,
+                          int32_t **stage = nullptr, bool stage_labels = false) {
+    const bool async = stage != nullptr;
+    hipStream_t us = async ? b->astream : b->stream;
     int s = -1;
     for (int i = *hint; i < b->cap; i++)
         if (!b->slots[i].used) { s = i; break; }
@@ -934,7 +1036,7 @@ static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, co
     Slot &sl = b->slots[s];
     sl = Slot();
     sl.used = true;
-    sl.status = BG_RUNNING;
+    sl.status = async ? BG_PENDING : BG_RUNNING;
     sl.n = n;
     sl.m = m;
     sl.off = off;
@@ -958,10 +1060,14 @@ static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, co
     int32_t *d_newid = (int32_t *)(p + 3 * nb);
     CommStats *d_stats = (CommStats *)(p + 4 * nb);
     CommStats *d_stats_alt = (CommStats *)(p + 4 * nb + align256(16 * ((size_t)n + 1)));
-    PCD_TRY(upload_words(b, d_ind, indices, (size_t)n + 1));
-    PCD_TRY(upload_words(b, d_adj, adj, (size_t)E));
-    if (!sorted) PCD_HIP(hipMemcpyAsync(d_order, pr.order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, b->stream));
-    PCD_TRY(upload_words(b, d_next, labels, (size_t)n));
+    PCD_TRY(upload_words(b, d_ind, indices, (size_t)n + 1, us));
+    PCD_TRY(upload_words(b, d_adj, adj, (size_t)E, us));
+    if (!sorted) {
+        if (async) stage_words(b, d_order, pr.order.data(), (size_t)n, stage);
+        else PCD_HIP(hipMemcpyAsync(d_order, pr.order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    }
+    if (async && stage_labels) stage_words(b, d_next, labels, (size_t)n, stage);
+    else PCD_TRY(upload_words(b, d_next, labels, (size_t)n, us));
     BatchGraph &g = sl.rec;
     std::memset(&g, 0, sizeof(g));
     g.lp.indices = d_ind;
@@ -992,13 +1098,7 @@ static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, co
     g.max_iter = job->max_iter;
     g.threshold = job->threshold;
     g.iter_cap = job->iter_cap > 0 ? job->iter_cap : 1000;
-    BatchState &st = b->h_state[s];
-    std::memset(&st, 0, sizeof(st));
-    st.status = BG_RUNNING;
-    st.label_bound = n;
-    // items of the slot's previous graph may still be listed: rebuild then
-    if (b->items_cap == 0 || b->slot_listed[s]) b->items_dirty = true;
-    if (!b->items_dirty) b->pending_append.push_back(s);
+    if (!async) start_graph(b, s);
     *slot_out = s;
     return PCD_OK;
 }
@@ -1160,6 +1260,228 @@ extern "C" int pcd_batch_add_many(pcd_batch *b, int count, const pcd_graph_ref *
     return PCD_OK;
 }
 
+// Takes in one chunk of an asynchronous admission once its uploads and row
+// check are done: a graph whose rows failed is rejected (its arena space
+// goes back, the slot stays taken until released), the others start and
+// their records go to the device ahead of the next round.
+static int join_chunk(pcd_batch *b, const pcd_batch::AChunk &c) {
+    std::vector<int> joined;
+    for (int i = c.first; i < c.first + c.count; i++) {
+        const int s = b->aslots[i];
+        if (s < 0) continue;
+        Slot &sl = b->slots[s];
+        if (!sl.used || sl.status != BG_PENDING) continue;
+        b->apending--;
+        if (b->h_abad[i]) {
+            arena_free(b, sl.off, sl.bytes);
+            sl.off = sl.bytes = 0;
+            sl.status = BG_REJECTED;
+            continue;
+        }
+        sl.status = BG_RUNNING;
+        sl.admitted = b->rounds;
+        start_graph(b, s);
+        joined.push_back(s);
+    }
+    if (joined.empty()) return PCD_OK;
+    std::sort(joined.begin(), joined.end());
+    std::vector<BatchGraph> staging;
+    PCD_TRY(upload_records(b, joined, staging));
+    PCD_HIP(hipStreamSynchronize(b->stream));
+    return PCD_OK;
+}
+
+// Joins the queued chunks in order: every chunk whose event has passed, all
+// of them with wait_all, and at least the first with wait_one.
+static int join_pending(pcd_batch *b, bool wait_all, bool wait_one) {
+    bool waited = false;
+    while (!b->achunks.empty()) {
+        const pcd_batch::AChunk c = b->achunks.front();
+        if (wait_all || (wait_one && !waited)) {
+            PCD_HIP(hipEventSynchronize(c.ev));
+            waited = true;
+        } else {
+            const hipError_t e = hipEventQuery(c.ev);
+            if (e == hipErrorNotReady) {
+                (void)hipGetLastError();
+                break;
+            }
+            if (e != hipSuccess) return bfail(PCD_E_HIP, std::string("admission event: ") + hipGetErrorString(e));
+        }
+        b->achunks.pop_front();
+        PCD_TRY(join_chunk(b, c));
+    }
+    free_retired(b);
+    return PCD_OK;
+}
+
+// Grows a device buffer of `count` elements of T (contents dropped).
+template <class T>
+static int grow_device(T **p, size_t *cap, size_t count, const char *what) {
+    if (count <= *cap) return PCD_OK;
+    if (*p) PCD_HIP(hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    const size_t c = count + count / 2 + 64;
+    if (hipMalloc((void **)p, sizeof(T) * c) != hipSuccess) {
+        (void)hipGetLastError();
+        return bfail(PCD_E_NOMEM, what);
+    }
+    *cap = c;
+    return PCD_OK;
+}
+
+extern "C" int pcd_batch_add_many_async(pcd_batch *b, int count, const pcd_graph_ref *graphs, const pcd_job *jobs,
+                                        int *slots_out, int *codes_out) {
+    if (!b || count < 0 || (count > 0 && (!graphs || !jobs || !slots_out)))
+        return bfail(PCD_E_INVALID, "bad arguments");
+    for (int i = 0; i < count; i++) {
+        slots_out[i] = -1;
+        if (codes_out) codes_out[i] = PCD_OK;
+    }
+    if (count == 0) return PCD_OK;
+    PCD_HIP(hipSetDevice(b->device));
+    // the previous call's buffers are reused: its chunks join first
+    PCD_TRY(join_pending(b, true, false));
+    if (!b->astream && hipStreamCreateWithFlags(&b->astream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        b->astream = nullptr;
+        return bfail(PCD_E_HIP, "admission stream");
+    }
+    // host checks of every graph (several threads), then the staging need
+    std::vector<Prepared> pr((size_t)count);
+    parallel_for(count, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; i++) {
+            const pcd_graph_ref &g = graphs[i];
+            prepare_graph(g.n, g.m, g.indices, g.adj, g.labels, jobs + i, false, false, pr[i]);
+        }
+    }, 1);
+    std::vector<char> stage_lab((size_t)count, 0);
+    size_t stage_words_needed = 0;
+    for (int i = 0; i < count; i++) {
+        if (pr[i].code != PCD_OK) continue;
+        const pcd_graph_ref &g = graphs[i];
+        stage_lab[i] = !(device_readable(g.labels) && device_readable(g.labels + g.n - 1));
+        stage_words_needed += (stage_lab[i] ? (size_t)g.n : 0) + (pr[i].sorted ? 0 : (size_t)g.n);
+    }
+    if (stage_words_needed > b->astage_cap) {
+        if (b->h_astage) PCD_HIP(hipHostFree(b->h_astage));
+        b->h_astage = nullptr;
+        b->astage_cap = 0;
+        const size_t c = stage_words_needed + stage_words_needed / 4 + 1024;
+        if (hipHostMalloc((void **)&b->h_astage, 4 * c, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            b->h_astage = nullptr;
+            return bfail(PCD_E_NOMEM, "admission staging");
+        }
+        b->astage_cap = c;
+    }
+    // slots, arena space and copies, chunk by chunk
+    constexpr int CHUNK = 256;
+    struct Span {
+        size_t p0, p1, i0, i1;
+    };
+    std::vector<Span> spans;
+    std::vector<PullPiece> pieces;
+    std::vector<WorkItem> items;
+    std::vector<BatchGraph> recs((size_t)count);
+    std::memset(recs.data(), 0, sizeof(BatchGraph) * recs.size());
+    b->aslots.assign((size_t)count, -1);
+    b->pulls.clear();
+    int32_t *cursor = b->h_astage;
+    int first_code = PCD_OK, first_index = -1;
+    std::string first_msg;
+    int hint = 0;
+    for (int c0 = 0; c0 < count; c0 += CHUNK) {
+        const int c1 = std::min(count, c0 + CHUNK);
+        Span sp;
+        sp.p0 = pieces.size();
+        sp.i0 = items.size();
+        for (int i = c0; i < c1; i++) {
+            const pcd_graph_ref &g = graphs[i];
+            int rc = pr[i].code;
+            std::string msg = pr[i].msg;
+            if (rc == PCD_OK) {
+                int s = -1;
+                rc = admit_prepared(b, g.n, g.m, g.indices, g.adj, g.labels, jobs + i, pr[i], &hint, &s, &cursor,
+                                    stage_lab[i] != 0);
+                if (rc == PCD_OK) {
+                    slots_out[i] = s;
+                    b->aslots[i] = s;
+                    b->apending++;
+                    recs[i] = b->slots[s].rec;
+                    for (int p = 0; p < g.n; p += 256) items.push_back(WorkItem{i, p, std::min(g.n, p + 256)});
+                } else {
+                    msg = pcd_last_error();
+                }
+            }
+            if (rc == PCD_E_HIP) return bfail(rc, "graph " + std::to_string(i) + ": " + msg);
+            if (codes_out) codes_out[i] = rc;
+            if (rc != PCD_OK && first_code == PCD_OK) {
+                first_code = rc;
+                first_index = i;
+                first_msg = msg;
+            }
+            pr[i].order = std::vector<int32_t>();
+        }
+        pieces.insert(pieces.end(), b->pulls.begin(), b->pulls.end());
+        b->pulls.clear();
+        sp.p1 = pieces.size();
+        sp.i1 = items.size();
+        spans.push_back(sp);
+    }
+    // device copies of the piece, item and record lists, then per chunk:
+    // pulls, row check, flags back to the host, event
+    PCD_TRY(grow_device(&b->d_apulls, &b->apulls_cap, std::max<size_t>(pieces.size(), 1), "admission copy list"));
+    PCD_TRY(grow_device(&b->d_acheck, &b->acheck_cap, std::max<size_t>(items.size(), 1), "row check list"));
+    PCD_TRY(grow_device(&b->d_arecs, &b->arecs_cap, (size_t)count, "row check records"));
+    if ((size_t)count > b->abad_cap) {
+        if (b->d_abad) PCD_HIP(hipFree(b->d_abad));
+        if (b->h_abad) PCD_HIP(hipHostFree(b->h_abad));
+        b->d_abad = b->h_abad = nullptr;
+        b->abad_cap = 0;
+        const size_t c = (size_t)count + count / 2 + 64;
+        if (hipMalloc((void **)&b->d_abad, 4 * c) != hipSuccess ||
+            hipHostMalloc((void **)&b->h_abad, 4 * c, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            if (b->d_abad) (void)hipFree(b->d_abad);
+            b->d_abad = nullptr;
+            b->h_abad = nullptr;
+            return bfail(PCD_E_NOMEM, "row check flags");
+        }
+        b->abad_cap = c;
+    }
+    hipStream_t as = b->astream;
+    if (!pieces.empty())
+        PCD_HIP(hipMemcpyAsync(b->d_apulls, pieces.data(), sizeof(PullPiece) * pieces.size(), hipMemcpyHostToDevice, as));
+    if (!items.empty())
+        PCD_HIP(hipMemcpyAsync(b->d_acheck, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice, as));
+    PCD_HIP(hipMemcpyAsync(b->d_arecs, recs.data(), sizeof(BatchGraph) * count, hipMemcpyHostToDevice, as));
+    PCD_HIP(hipMemsetAsync(b->d_abad, 0, 4 * (size_t)count, as));
+    for (size_t c = 0; c < spans.size(); c++) {
+        const Span &sp = spans[c];
+        const int first = (int)c * CHUNK, cnt = std::min(count - first, CHUNK);
+        if (sp.p1 > sp.p0)
+            hipLaunchKernelGGL(k_b_pull, dim3(std::min<size_t>(sp.p1 - sp.p0, (size_t)b->cu_count / 4)), dim3(256), 0, as,
+                               (const PullPiece *)(b->d_apulls + sp.p0), (int)(sp.p1 - sp.p0));
+        if (sp.i1 > sp.i0)
+            hipLaunchKernelGGL(k_b_check_rows, dim3(std::min<size_t>(sp.i1 - sp.i0, (size_t)b->cu_count * 8)), dim3(256),
+                               0, as, (const WorkItem *)(b->d_acheck + sp.i0), (int)(sp.i1 - sp.i0),
+                               (const BatchGraph *)b->d_arecs, b->d_abad);
+        PCD_HIP(hipGetLastError());
+        PCD_HIP(hipMemcpyAsync(b->h_abad + first, b->d_abad + first, 4 * (size_t)cnt, hipMemcpyDeviceToHost, as));
+        if (b->aevents.size() <= c) {
+            hipEvent_t e = nullptr;
+            PCD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            b->aevents.push_back(e);
+        }
+        PCD_HIP(hipEventRecord(b->aevents[c], as));
+        b->achunks.push_back(pcd_batch::AChunk{first, cnt, b->aevents[c]});
+    }
+    if (first_code != PCD_OK) return bfail(first_code, "graph " + std::to_string(first_index) + ": " + first_msg);
+    return PCD_OK;
+}
+
 // Items of one graph: vertex tiles, statistics classes (degree in (G/2, G],
 // G4 takes 0..4), long rows, and the propagation ranges of the bin map.
 static void lp_range(const BinMap &mp, const Slot &sl, int i, int &p0, int &p1) {
@@ -1244,11 +1566,13 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
         return sl.used && sl.status == BG_RUNNING && !(settled_only && sl.admitted >= b->rounds) &&
                (stride == 1 || (&sl - b->slots.data()) % stride == 0);
     };
-    // pass 1: counts
-    int64_t cnt[NUM_LISTS] = {0};
+    // pass 1: counts (room also for the graphs of queued admission chunks,
+    // so that they join by appending, without a new capture)
+    int64_t cnt[NUM_LISTS] = {0}, room[NUM_LISTS] = {0};
     int64_t listed = 0;
     for (int s = 0; s < b->cap; s++) {
         const Slot &sl = b->slots[s];
+        if (sl.used && sl.status == BG_PENDING && stride == 1) slot_counts(mp, sl, lp_only, room);
         if (!listed_slot(sl)) continue;
         listed += sl.n;
         slot_counts(mp, sl, lp_only, cnt);
@@ -1260,8 +1584,8 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
     size_t list_off[NUM_LISTS];
     int64_t list_cap[NUM_LISTS];
     for (int l = first_list; l < NUM_LISTS; l++) {
-        if (cnt[l] > INT32_MAX / 2) return bfail(PCD_E_NOMEM, "work item list too long");
-        list_cap[l] = cnt[l] + std::max<int64_t>(cnt[l] / 2, 1024);
+        if (cnt[l] + room[l] > INT32_MAX / 2) return bfail(PCD_E_NOMEM, "work item list too long");
+        list_cap[l] = cnt[l] + room[l] + std::max<int64_t>(cnt[l] / 2, 1024);
         list_off[l] = off;
         off = align256(off + list_cap[l] * list_esz(l));
     }
@@ -1270,8 +1594,8 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
     if (off > b->items_cap) {
         if (lp_only) return rebuild_items(b, settled_only, false, stride);
         drop_graph(b);
-        if (b->d_items) (void)hipFree(b->d_items);
-        if (b->h_items) (void)hipHostFree(b->h_items);
+        free_device(b, b->d_items);
+        free_host(b, b->h_items);
         b->d_items = b->h_items = nullptr;
         size_t cap = off + off / 4 + 4096;
         if (hipMalloc(&b->d_items, cap) != hipSuccess || hipHostMalloc(&b->h_items, cap, hipHostMallocDefault) != hipSuccess) {
@@ -1283,8 +1607,8 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
     }
     if (ntiles > b->tile_cap) {
         drop_graph(b);
-        if (b->d_tile_sum) (void)hipFree(b->d_tile_sum);
-        if (b->d_tile_pre) (void)hipFree(b->d_tile_pre);
+        free_device(b, b->d_tile_sum);
+        free_device(b, b->d_tile_pre);
         size_t cap = ntiles + 64;
         if (hipMalloc((void **)&b->d_tile_sum, 4 * cap) != hipSuccess ||
             hipMalloc((void **)&b->d_tile_pre, 4 * cap) != hipSuccess) {
@@ -1503,7 +1827,7 @@ static int poll_state(pcd_batch *b, int *running) {
     int r = 0;
     for (int s = 0; s < b->cap; s++) {
         Slot &sl = b->slots[s];
-        if (!sl.used) continue;
+        if (!sl.used || sl.status == BG_PENDING || sl.status == BG_REJECTED) continue;
         int now = b->h_state[s].status;
         if (sl.status == BG_RUNNING && now != BG_RUNNING) b->listed_done_vertices += sl.n;
         sl.status = now;
@@ -1519,10 +1843,17 @@ extern "C" int pcd_batch_step(pcd_batch *b, int rounds, int *running) {
     if (!b) return bfail(PCD_E_INVALID, "null batch");
     if (rounds <= 0) return bfail(PCD_E_INVALID, "rounds must be positive");
     PCD_HIP(hipSetDevice(b->device));
+    if (!b->achunks.empty()) {
+        bool any = false;
+        for (const Slot &sl : b->slots) any |= sl.used && sl.status == BG_RUNNING;
+        PCD_TRY(join_pending(b, false, !any));
+    }
     PCD_TRY(ensure_graph(b));
     for (int r = 0; r < rounds; r++) PCD_HIP(hipGraphLaunch(b->exec, b->stream));
     b->rounds += rounds;
-    return poll_state(b, running);
+    PCD_TRY(poll_state(b, running));
+    if (running) *running += b->apending;
+    return PCD_OK;
 }
 
 extern "C" int pcd_batch_status(pcd_batch *b, int slot, int *status, int *iterations, int *k, double *score) {
@@ -1579,7 +1910,7 @@ extern "C" int pcd_batch_labels_many(pcd_batch *b, int count, const int *slots, 
     const size_t need = hbytes + 4 * (size_t)total;
     PCD_HIP(hipSetDevice(b->device));
     if (need > b->gather_cap) {
-        if (b->d_gather) PCD_HIP(hipFree(b->d_gather));
+        free_device(b, b->d_gather);
         b->d_gather = nullptr;
         b->gather_cap = 0;
         if (hipMalloc((void **)&b->d_gather, need) != hipSuccess) {
@@ -1620,9 +1951,10 @@ extern "C" int pcd_batch_stats(pcd_batch *b, int slot, int32_t *sizes, int32_t *
 
 extern "C" int pcd_batch_release(pcd_batch *b, int slot) {
     if (!b || slot < 0 || slot >= b->cap) return bfail(PCD_E_INVALID, "bad slot");
-    Slot &sl = b->slots[slot];
-    if (!sl.used) return bfail(PCD_E_STATE, "slot is free");
+    if (!b->slots[slot].used) return bfail(PCD_E_STATE, "slot is free");
     PCD_HIP(hipSetDevice(b->device));
+    if (b->slots[slot].status == BG_PENDING) PCD_TRY(join_pending(b, true, false));
+    Slot &sl = b->slots[slot];
     BatchState &st = b->h_state[slot];
     std::memset(&st, 0, sizeof(st));
     st.status = BG_FREE;
@@ -1630,7 +1962,7 @@ extern "C" int pcd_batch_release(pcd_batch *b, int slot) {
     PCD_HIP(hipMemcpyAsync(b->d_state + slot, &st, sizeof(BatchState), hipMemcpyHostToDevice, b->stream));
     PCD_HIP(hipStreamSynchronize(b->stream));
     if (sl.status == BG_RUNNING) b->items_dirty = true;
-    arena_free(b, sl.off, sl.bytes);
+    if (sl.bytes) arena_free(b, sl.off, sl.bytes);
     sl = Slot();
     return PCD_OK;
 }
@@ -1643,7 +1975,7 @@ extern "C" int pcd_batch_info(pcd_batch *b, int64_t *free_bytes, int *graphs, in
     for (auto &sl : b->slots) {
         if (!sl.used) continue;
         g++;
-        if (sl.status == BG_RUNNING) r++;
+        if (sl.status == BG_RUNNING || sl.status == BG_PENDING) r++;
     }
     if (free_bytes) *free_bytes = (int64_t)fb;
     if (graphs) *graphs = g;

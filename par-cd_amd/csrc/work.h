@@ -81,7 +81,10 @@ struct WorkItem {
     int g, p0, p1;
 };
 
-enum BatchStatus { BG_RUNNING = 0, BG_DONE = 1, BG_FREE = 2 };
+// BG_PENDING and BG_REJECTED are host-side slot states of asynchronous
+// admission; a device state record never holds them (it stays BG_FREE
+// until the graph joins).
+enum BatchStatus { BG_RUNNING = 0, BG_DONE = 1, BG_FREE = 2, BG_PENDING = 3, BG_REJECTED = 4 };
 
 // Static description of one graph of a batch (written at admission).
 // Labels live in two buffers with fixed roles: propagation reads `cur`

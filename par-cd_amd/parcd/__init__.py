@@ -37,9 +37,9 @@ EXPORTS = [
     "pcd_server_info", "pcd_gen_requests",
     "pcd_graphset_open", "pcd_graphset_close", "pcd_graphset_count", "pcd_graphset_graph", "pcd_graphset_extract",
     "pcd_graphset_writer_open", "pcd_graphset_writer_add", "pcd_graphset_writer_close", "pcd_graphset_pack",
-    "pcd_compress_labels_noalloc", "pcd_iterate_many",
+    "pcd_compress_labels_noalloc", "pcd_iterate_many", "pcd_batch_add_many_async",
 ]
-BATCH_RUNNING, BATCH_DONE, BATCH_FREE = 0, 1, 2
+BATCH_RUNNING, BATCH_DONE, BATCH_FREE, BATCH_PENDING, BATCH_REJECTED = 0, 1, 2, 3, 4
 
 
 class PcdError(RuntimeError):
@@ -156,6 +156,7 @@ def lib():
     L.pcd_batch_footprint.argtypes = [C.c_int, C.c_int, lp]
     L.pcd_batch_add.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.POINTER(Job), ip]
     L.pcd_batch_add_many.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+    L.pcd_batch_add_many_async.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     L.pcd_batch_labels_many.argtypes = [vp, C.c_int, vp, vp]
     L.pcd_nmi.argtypes = [C.c_int, vp, vp, dp]
     L.pcd_clustering_labels.argtypes = [vp, ip]
@@ -725,11 +726,13 @@ class Batch:
         self._b = C.c_void_p()
         check(lib().pcd_batch_create(device, int(max_graphs), int(arena_bytes), C.byref(self._b)))
         self.sizes = {}
+        self._held = []  # host arrays of asynchronously admitted graphs
 
     def close(self):
         if self._b:
             lib().pcd_batch_destroy(self._b)
             self._b = C.c_void_p()
+        self._held = []
 
     def __enter__(self):
         return self
@@ -755,13 +758,16 @@ class Batch:
         self.sizes[slot.value] = int(n)
         return slot.value
 
-    def add_many(self, graphs, labels, jobs):
+    def add_many(self, graphs, labels, jobs, wait=True):
         """Admits graphs[i] = (n, m, indices, adj) with labels[i] and jobs[i]
         in one call (pcd_batch_add_many).  Returns the slots, None where a
         graph was not admitted for lack of room (PCD_E_NOMEM); the codes and
         slots are kept in self.last_codes and self.last_slots.  Raises
         PcdError when a graph was refused for another reason (the admitted
-        ones stay admitted)."""
+        ones stay admitted).  wait=False admits through
+        pcd_batch_add_many_async: the graphs are BATCH_PENDING until a later
+        step() joins them, and the batch holds on to their arrays until it is
+        closed."""
         count = len(graphs)
         refs = (GraphRef * max(count, 1))()
         keep = []
@@ -772,8 +778,10 @@ class Batch:
         jarr = (Job * max(count, 1))(*jobs)
         slots = np.full(max(count, 1), -1, np.int32)
         codes = np.zeros(max(count, 1), np.int32)
-        rc = lib().pcd_batch_add_many(self._b, count, C.cast(refs, C.c_void_p), C.cast(jarr, C.c_void_p),
-                                      _p(slots), _p(codes))
+        fn = lib().pcd_batch_add_many if wait else lib().pcd_batch_add_many_async
+        rc = fn(self._b, count, C.cast(refs, C.c_void_p), C.cast(jarr, C.c_void_p), _p(slots), _p(codes))
+        if not wait:
+            self._held.append(keep)
         msg = "pcd error %d: %s" % (rc, lib().pcd_last_error().decode(errors="replace")) if rc else ""
         out = []
         for i in range(count):

@@ -293,6 +293,94 @@ def test_add_many_checks_rows_on_the_device():
             b.add(*broken[0], lab, parcd.job())
 
 
+def _pinned(a):
+    out = parcd.pinned_empty(len(a), np.int32)
+    out[:] = a
+    return out
+
+
+def _run_to_end(b):
+    while b.step(1):
+        pass
+
+
+def test_add_many_async_equals_add_many():
+    """pcd_batch_add_many_async: graphs are pending until a step joins them,
+    then end exactly where pcd_batch_add_many's do, over several chunks of
+    256 graphs, from page-locked or pageable arrays (labels and degree
+    orders staged), admitted before the first round or while others run."""
+    base = graphs() + [sbm(2, 3000, 0.1, 0.001, seed=31)]
+    gs = (base * 90)[:620]  # three chunks
+    jobs = [parcd.job(**JOBS[i % len(JOBS)]) for i in range(len(gs))]
+    labs = [singleton_labels(g[0]) for g in gs]
+    arena = sum(parcd.batch_footprint(g[0], g[1]) for g in gs) + (64 << 20)
+    with parcd.Batch(0, max_graphs=len(gs), arena_bytes=arena) as b:
+        slots = b.add_many([g[:4] for g in gs], labs, jobs)
+        _run_to_end(b)
+        ref = [(b.labels(s_), b.status(s_)) for s_ in slots]
+    pinned_csr = ([(g[0], g[1], _pinned(g[2]), _pinned(g[3])) for g in base] * 90)[:620]
+    for case in ("pageable", "pinned", "pinned_all", "while_running"):
+        print("async admission:", case, flush=True)
+        with parcd.Batch(0, max_graphs=len(gs), arena_bytes=arena) as b:
+            csr = [g[:4] for g in gs] if case == "pageable" else pinned_csr
+            ll = [_pinned(l) for l in labs] if case == "pinned_all" else labs
+            if case == "while_running":
+                head = b.add_many(csr[:18], ll[:18], jobs[:18])
+                b.step(1)
+                b.step(1)
+                tail = b.add_many(csr[18:], ll[18:], jobs[18:], wait=False)
+                assert all(b.status(s_)[0] == parcd.BATCH_PENDING for s_ in tail)
+                slots = head + tail
+            else:
+                slots = b.add_many(csr, ll, jobs, wait=False)
+                assert all(s_ is not None for s_ in slots)
+                assert all(b.status(s_)[0] == parcd.BATCH_PENDING for s_ in slots)
+                assert b.info()["running"] == len(gs)
+            _run_to_end(b)
+            for i, s_ in enumerate(slots):
+                la, sa = b.labels(s_), b.status(s_)
+                assert np.array_equal(la, ref[i][0]) and sa == ref[i][1], (case, i)
+
+
+def test_add_many_async_rejections_and_release():
+    """Host refusals come back from the call; rows that fail the device
+    check leave their graph PCD_BATCH_REJECTED at the join, without arena
+    space; a pending slot can be released (its uploads are waited for); a
+    second call joins the first's chunks."""
+    g = sbm(4, 50, 0.3, 0.01, seed=11)
+    lab = singleton_labels(g[0])
+    r0 = int(g[2][5])
+    adj = g[3].copy()
+    adj[r0], adj[r0 + 1] = adj[r0 + 1], adj[r0]
+    broken = (g[0], g[1], g[2], adj)
+    ref = run_batch([(g, lab, parcd.job())])[0]
+    with parcd.Batch(0, max_graphs=8, arena_bytes=16 << 20) as b:
+        free0 = b.info()["free_bytes"]
+        with pytest.raises(parcd.PcdError):
+            b.add_many([g[:4], g[:4], broken, g[:4]], [lab] * 4,
+                       [parcd.job(), parcd.job(kind=O.WCC), parcd.job(), parcd.job()], wait=False)
+        sl = b.last_slots
+        assert b.last_codes == [0, -1, 0, 0] and sl[1] is None
+        assert [b.status(sl[i])[0] for i in (0, 2, 3)] == [parcd.BATCH_PENDING] * 3
+        b.release(sl[3])  # pending: waited for, then freed
+        assert b.info()["graphs"] == 2
+        # a second call joins the first call's chunks before it queues
+        more = b.add_many([g[:4]], [lab], [parcd.job()], wait=False)
+        assert b.status(sl[0])[0] == parcd.BATCH_RUNNING
+        assert b.status(sl[2])[0] == parcd.BATCH_REJECTED
+        assert b.status(more[0])[0] == parcd.BATCH_PENDING
+        _run_to_end(b)
+        assert b.status(sl[2])[0] == parcd.BATCH_REJECTED
+        for s_ in (sl[0], more[0]):
+            assert b.status(s_)[0] == parcd.BATCH_DONE
+            assert np.array_equal(b.labels(s_), ref[0])
+        with pytest.raises(parcd.PcdError):
+            b.labels(sl[2])
+        for s_ in (sl[0], sl[2], more[0]):
+            b.release(s_)
+        assert b.info()["graphs"] == 0 and b.info()["free_bytes"] == free0
+
+
 @pytest.mark.parametrize("strategy", [0, 1, 2])
 def test_refine_call_equals_replay(strategy):
     g = sbm(6, 60, 0.25, 0.01, seed=21)
