@@ -508,6 +508,16 @@ __global__ __launch_bounds__(256) void k_b_pull(const PullPiece *pieces, int cou
     }
 }
 
+// Word copy between device memory and page-locked host memory by the
+// device (the batch stream's record, state and item-list uploads and its
+// state read-back): a copy-engine transfer would wait behind the copies an
+// asynchronous admission queued.
+__global__ __launch_bounds__(256) void k_b_copy(const int32_t *__restrict__ src, int32_t *__restrict__ dst,
+                                                int64_t words) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // Format check of admitted graphs' rows in the arena (what prepare_graph
 // checks on the host for a single graph): ascending, loop-free, ids in
 // [0, n).  One wave per item of up to 256 rows; bad[slot] = 1 on failure.
@@ -615,6 +625,7 @@ struct pcd_batch {
     BatchGraph *d_graphs = nullptr;
     BatchState *d_state = nullptr;
     BatchState *h_state = nullptr;  // pinned
+    BatchGraph *h_graphs = nullptr; // pinned: graph records on their way to d_graphs
     int2 *d_gtiles = nullptr;
     int2 *h_gtiles = nullptr;       // pinned
     // item lists
@@ -654,6 +665,8 @@ struct pcd_batch {
     size_t acheck_cap = 0;
     BatchGraph *d_arecs = nullptr;  // row-check records, call-local
     size_t arecs_cap = 0;
+    char *d_adma = nullptr;  // device staging of one chunk's copy-engine transfers
+    size_t adma_cap = 0;
     int32_t *d_abad = nullptr, *h_abad = nullptr;
     size_t abad_cap = 0;
     int32_t *h_astage = nullptr;  // page-locked staging of pageable labels and degree orders
@@ -740,6 +753,20 @@ static void free_retired(pcd_batch *b) {
     b->retired_host.clear();
 }
 
+// Queues k_b_copy on the batch stream: `bytes` (a multiple of 4) between
+// page-locked host memory and the device, either way.  The host keeps a
+// host source unchanged, and reads a host destination only, after the
+// stream has passed the launch.
+static int kcopy(pcd_batch *b, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return PCD_OK;
+    if (bytes % 4) return bfail(PCD_E_STATE, "kcopy: size is not a multiple of 4");
+    const int64_t words = (int64_t)(bytes / 4);
+    const int blocks = (int)std::min<int64_t>(std::max<int64_t>(1, words / 2048), (int64_t)b->cu_count);
+    hipLaunchKernelGGL(k_b_copy, dim3(blocks), dim3(256), 0, b->stream, (const int32_t *)src, (int32_t *)dst, words);
+    PCD_HIP(hipGetLastError());
+    return PCD_OK;
+}
+
 extern "C" int pcd_batch_map_count(void) { return NUM_MAPS; }
 extern "C" const char *pcd_batch_map_name(int map) { return (map >= 0 && map < NUM_MAPS) ? MAPS[map].name : ""; }
 
@@ -759,6 +786,7 @@ extern "C" int pcd_batch_destroy(pcd_batch *b) {
     if (b->d_apulls) (void)hipFree(b->d_apulls);
     if (b->d_acheck) (void)hipFree(b->d_acheck);
     if (b->d_arecs) (void)hipFree(b->d_arecs);
+    if (b->d_adma) (void)hipFree(b->d_adma);
     if (b->d_abad) (void)hipFree(b->d_abad);
     if (b->h_abad) (void)hipHostFree(b->h_abad);
     if (b->h_astage) (void)hipHostFree(b->h_astage);
@@ -778,6 +806,7 @@ extern "C" int pcd_batch_destroy(pcd_batch *b) {
     if (b->d_check) (void)hipFree(b->d_check);
     if (b->d_bad) (void)hipFree(b->d_bad);
     if (b->h_state) (void)hipHostFree(b->h_state);
+    if (b->h_graphs) (void)hipHostFree(b->h_graphs);
     if (b->h_gtiles) (void)hipHostFree(b->h_gtiles);
     if (b->h_items) (void)hipHostFree(b->h_items);
     if (b->h_counts) (void)hipHostFree(b->h_counts);
@@ -828,9 +857,11 @@ extern "C" int pcd_batch_create(int device, int max_graphs, int64_t arena_bytes,
         (void)hipGetLastError();
         return bad(PCD_E_NOMEM, "batch tables");
     }
-    if (hipHostMalloc((void **)&b->h_state, sizeof(BatchState) * max_graphs, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&b->h_gtiles, sizeof(int2) * max_graphs, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&b->h_counts, sizeof(int32_t) * NUM_LISTS, hipHostMallocDefault) != hipSuccess)
+    // coherent: k_b_copy reads and writes these mirrors on the device
+    if (hipHostMalloc((void **)&b->h_state, sizeof(BatchState) * max_graphs, hipHostMallocCoherent) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_graphs, sizeof(BatchGraph) * max_graphs, hipHostMallocCoherent) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_gtiles, sizeof(int2) * max_graphs, hipHostMallocCoherent) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_counts, sizeof(int32_t) * NUM_LISTS, hipHostMallocCoherent) != hipSuccess)
         return bad(PCD_E_NOMEM, "pinned batch tables");
     for (int i = 0; i < max_graphs; i++) {
         std::memset(&b->h_state[i], 0, sizeof(BatchState));
@@ -1103,21 +1134,17 @@ static int admit_prepared(pcd_batch *b, int n, int m, const int32_t *indices, co
     return PCD_OK;
 }
 
-// Uploads the graph and state records of `slots` (ascending), one copy per
-// run of consecutive slots.  `staging` holds the graph records until the
-// caller has synchronised the stream.
-static int upload_records(pcd_batch *b, const std::vector<int> &slots, std::vector<BatchGraph> &staging) {
-    staging.resize(slots.size());
-    for (size_t i = 0; i < slots.size(); i++) staging[i] = b->slots[slots[i]].rec;
+// Uploads the graph and state records of `slots` (ascending), one launch
+// per run of consecutive slots, through the page-locked record mirrors.
+static int upload_records(pcd_batch *b, const std::vector<int> &slots) {
+    for (int s : slots) b->h_graphs[s] = b->slots[s].rec;
     for (size_t i = 0; i < slots.size();) {
         size_t j = i + 1;
         while (j < slots.size() && slots[j] == slots[j - 1] + 1) j++;
         const int s0 = slots[i];
         const size_t cnt = j - i;
-        PCD_HIP(hipMemcpyAsync(b->d_graphs + s0, staging.data() + i, sizeof(BatchGraph) * cnt, hipMemcpyHostToDevice,
-                               b->stream));
-        PCD_HIP(hipMemcpyAsync(b->d_state + s0, b->h_state + s0, sizeof(BatchState) * cnt, hipMemcpyHostToDevice,
-                               b->stream));
+        PCD_TRY(kcopy(b, b->d_graphs + s0, b->h_graphs + s0, sizeof(BatchGraph) * cnt));
+        PCD_TRY(kcopy(b, b->d_state + s0, b->h_state + s0, sizeof(BatchState) * cnt));
         i = j;
     }
     return PCD_OK;
@@ -1135,8 +1162,7 @@ extern "C" int pcd_batch_add(pcd_batch *b, int n, int m, const int32_t *indices,
     int rc = admit_prepared(b, n, m, indices, adj, labels, job, pr, &hint, &s);
     if (rc != PCD_OK) return rc;
     PCD_TRY(flush_pulls(b));
-    std::vector<BatchGraph> staging;
-    rc = upload_records(b, std::vector<int>{s}, staging);
+    rc = upload_records(b, std::vector<int>{s});
     if (rc != PCD_OK) return rc;
     PCD_HIP(hipStreamSynchronize(b->stream));
     *slot_out = s;
@@ -1214,8 +1240,7 @@ extern "C" int pcd_batch_add_many(pcd_batch *b, int count, const pcd_graph_ref *
             (void)hipGetLastError();
             return bfail(PCD_E_NOMEM, "row check flags");
         }
-        std::vector<BatchGraph> staging;
-        PCD_TRY(upload_records(b, admitted, staging));
+        PCD_TRY(upload_records(b, admitted));
         PCD_HIP(hipMemsetAsync(b->d_bad, 0, sizeof(int32_t) * b->cap, b->stream));
         PCD_HIP(hipMemcpyAsync(b->d_check, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice,
                                b->stream));
@@ -1285,8 +1310,7 @@ static int join_chunk(pcd_batch *b, const pcd_batch::AChunk &c) {
     }
     if (joined.empty()) return PCD_OK;
     std::sort(joined.begin(), joined.end());
-    std::vector<BatchGraph> staging;
-    PCD_TRY(upload_records(b, joined, staging));
+    PCD_TRY(upload_records(b, joined));
     PCD_HIP(hipStreamSynchronize(b->stream));
     return PCD_OK;
 }
@@ -1328,6 +1352,60 @@ static int grow_device(T **p, size_t *cap, size_t count, const char *what) {
         return bfail(PCD_E_NOMEM, what);
     }
     *cap = c;
+    return PCD_OK;
+}
+
+static void slot_counts(const BinMap &mp, const Slot &sl, bool lp_only, int64_t *cnt);
+static size_t list_esz(int);
+
+// Sizes the item lists and tile-scan buffers for every running and pending
+// graph under every bin map, as rebuild_items lays them out, so that no
+// rebuild while admission copies are queued has to allocate: an
+// allocation waits for the copy engines, which the queued copies hold.
+static int reserve_lists(pcd_batch *b) {
+    size_t need = 0;
+    int64_t tiles = 0;
+    for (int mp = 0; mp < NUM_MAPS; mp++) {
+        int64_t cnt[NUM_LISTS] = {0};
+        for (const Slot &sl : b->slots)
+            if (sl.used && (sl.status == BG_RUNNING || sl.status == BG_PENDING)) slot_counts(MAPS[mp], sl, false, cnt);
+        size_t off = 0;
+        for (int l = 0; l < NUM_LISTS; l++)
+            off = align256(off + (cnt[l] + std::max<int64_t>(cnt[l] / 2, 1024)) * list_esz(l));
+        need = std::max(need, off);
+        tiles = std::max(tiles, cnt[L_TILES] + std::max<int64_t>(cnt[L_TILES] / 2, 1024));
+    }
+    if (need > b->items_cap) {
+        drop_graph(b);
+        PCD_HIP(hipStreamSynchronize(b->stream));
+        free_device(b, b->d_items);
+        free_host(b, b->h_items);
+        b->d_items = b->h_items = nullptr;
+        b->items_cap = 0;
+        b->items_dirty = true;
+        const size_t cap = need + need / 4 + 4096;
+        if (hipMalloc(&b->d_items, cap) != hipSuccess || hipHostMalloc(&b->h_items, cap, hipHostMallocCoherent) != hipSuccess) {
+            (void)hipGetLastError();
+            return bfail(PCD_E_NOMEM, "work item lists");
+        }
+        b->items_cap = cap;
+    }
+    if ((size_t)tiles > b->tile_cap) {
+        drop_graph(b);
+        PCD_HIP(hipStreamSynchronize(b->stream));
+        free_device(b, b->d_tile_sum);
+        free_device(b, b->d_tile_pre);
+        b->d_tile_sum = b->d_tile_pre = nullptr;
+        b->tile_cap = 0;
+        b->items_dirty = true;
+        const size_t cap = (size_t)tiles + 64;
+        if (hipMalloc((void **)&b->d_tile_sum, 4 * cap) != hipSuccess ||
+            hipMalloc((void **)&b->d_tile_pre, 4 * cap) != hipSuccess) {
+            (void)hipGetLastError();
+            return bfail(PCD_E_NOMEM, "tile scan buffers");
+        }
+        b->tile_cap = cap;
+    }
     return PCD_OK;
 }
 
@@ -1430,8 +1508,77 @@ extern "C" int pcd_batch_add_many_async(pcd_batch *b, int count, const pcd_graph
         sp.i1 = items.size();
         spans.push_back(sp);
     }
+    PCD_TRY(reserve_lists(b));
+    // Kernel reads of host memory slow the rounds running beside them (each
+    // waiting read holds the memory pipeline); copy-engine transfers do not.
+    // So per chunk the pieces are sorted by source address, pieces that
+    // continue one another inside one page-locked allocation form one copy
+    // into device staging, and the chunk's launch then moves each piece from
+    // there to its place.  A piece whose allocation the runtime does not
+    // report is read from host memory by that launch, as before.
+    struct Run {
+        const char *src;
+        size_t off, bytes;
+    };
+    std::vector<Run> runs;
+    std::vector<size_t> run_first(spans.size() + 1, 0), soff(pieces.size(), SIZE_MAX);
+    std::vector<const char *> pbase(pieces.size(), nullptr);
+    {
+        const char *cb = nullptr, *ce = nullptr;  // the last allocation found
+        const char *sb = (const char *)b->h_astage, *se = sb + 4 * b->astage_cap;
+        for (size_t k = 0; k < pieces.size(); k++) {
+            const char *p = (const char *)pieces[k].src, *q = p + 4 * (size_t)pieces[k].words;
+            if (sb && p >= sb && q <= se) {
+                pbase[k] = sb;
+                continue;
+            }
+            if (!(cb && p >= cb && q <= ce)) {
+                hipDeviceptr_t base = nullptr;
+                size_t sz = 0;
+                if (hipMemGetAddressRange(&base, &sz, (hipDeviceptr_t)p) == hipSuccess && base &&
+                    (const char *)base <= p && q <= (const char *)base + sz) {
+                    cb = (const char *)base;
+                    ce = cb + sz;
+                } else {
+                    (void)hipGetLastError();
+                    cb = ce = nullptr;
+                }
+            }
+            pbase[k] = cb;
+        }
+    }
+    size_t stage_bytes = 0;
+    for (size_t c = 0; c < spans.size(); c++) {
+        run_first[c] = runs.size();
+        std::vector<size_t> ord;
+        for (size_t k = spans[c].p0; k < spans[c].p1; k++)
+            if (pbase[k]) ord.push_back(k);
+        std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return pieces[x].src < pieces[y].src; });
+        size_t off = 0;
+        const char *run_base = nullptr;
+        for (size_t k : ord) {
+            const char *p = (const char *)pieces[k].src;
+            const size_t nb = 4 * (size_t)pieces[k].words;
+            if (runs.size() > run_first[c] && pbase[k] == run_base && p == runs.back().src + runs.back().bytes) {
+                soff[k] = runs.back().off + runs.back().bytes;
+                runs.back().bytes += nb;
+            } else {
+                off = ((off + 15) & ~(size_t)15) + ((uintptr_t)p & 15);  // same alignment as the source
+                runs.push_back(Run{p, off, nb});
+                run_base = pbase[k];
+                soff[k] = off;
+            }
+            off = runs.back().off + runs.back().bytes;
+        }
+        stage_bytes = std::max(stage_bytes, off);
+    }
+    run_first[spans.size()] = runs.size();
+    PCD_TRY(grow_device(&b->d_adma, &b->adma_cap, std::max<size_t>(stage_bytes, 16), "admission staging"));
+    for (size_t k = 0; k < pieces.size(); k++)
+        if (soff[k] != SIZE_MAX) pieces[k].src = (const int32_t *)(b->d_adma + soff[k]);
     // device copies of the piece, item and record lists, then per chunk:
-    // pulls, row check, flags back to the host, event
+    // copies into staging, the pieces to their places, row check, flags back
+    // to the host, event
     PCD_TRY(grow_device(&b->d_apulls, &b->apulls_cap, std::max<size_t>(pieces.size(), 1), "admission copy list"));
     PCD_TRY(grow_device(&b->d_acheck, &b->acheck_cap, std::max<size_t>(items.size(), 1), "row check list"));
     PCD_TRY(grow_device(&b->d_arecs, &b->arecs_cap, (size_t)count, "row check records"));
@@ -1461,8 +1608,15 @@ extern "C" int pcd_batch_add_many_async(pcd_batch *b, int count, const pcd_graph
     for (size_t c = 0; c < spans.size(); c++) {
         const Span &sp = spans[c];
         const int first = (int)c * CHUNK, cnt = std::min(count - first, CHUNK);
+        // at most 16 MiB per transfer: a copy on the batch stream waits for
+        // the transfer in flight on the engine, not for the queue behind it
+        constexpr size_t DMA_BYTES = 16 << 20;
+        for (size_t r = run_first[c]; r < run_first[c + 1]; r++)
+            for (size_t o = 0; o < runs[r].bytes; o += DMA_BYTES)
+                PCD_HIP(hipMemcpyAsync(b->d_adma + runs[r].off + o, runs[r].src + o,
+                                       std::min(DMA_BYTES, runs[r].bytes - o), hipMemcpyHostToDevice, as));
         if (sp.p1 > sp.p0)
-            hipLaunchKernelGGL(k_b_pull, dim3(std::min<size_t>(sp.p1 - sp.p0, (size_t)b->cu_count / 4)), dim3(256), 0, as,
+            hipLaunchKernelGGL(k_b_pull, dim3(std::min<size_t>(sp.p1 - sp.p0, (size_t)b->cu_count * 2)), dim3(256), 0, as,
                                (const PullPiece *)(b->d_apulls + sp.p0), (int)(sp.p1 - sp.p0));
         if (sp.i1 > sp.i0)
             hipLaunchKernelGGL(k_b_check_rows, dim3(std::min<size_t>(sp.i1 - sp.i0, (size_t)b->cu_count * 8)), dim3(256),
@@ -1598,7 +1752,7 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
         free_host(b, b->h_items);
         b->d_items = b->h_items = nullptr;
         size_t cap = off + off / 4 + 4096;
-        if (hipMalloc(&b->d_items, cap) != hipSuccess || hipHostMalloc(&b->h_items, cap, hipHostMallocDefault) != hipSuccess) {
+        if (hipMalloc(&b->d_items, cap) != hipSuccess || hipHostMalloc(&b->h_items, cap, hipHostMallocCoherent) != hipSuccess) {
             (void)hipGetLastError();
             b->items_cap = 0;
             return bfail(PCD_E_NOMEM, "work item lists");
@@ -1644,10 +1798,9 @@ static int rebuild_items(pcd_batch *b, bool settled_only = false, bool lp_only =
         slot_fill(mp, sl, s, lp_only, wp);
     }
     const size_t up0 = lp_only ? b->list_off[L_LP0] : 0;
-    PCD_HIP(hipMemcpyAsync((char *)b->d_items + up0, hb + up0, off - up0, hipMemcpyHostToDevice, b->stream));
-    PCD_HIP(hipMemcpyAsync(b->d_counts, b->h_counts, sizeof(int32_t) * NUM_LISTS, hipMemcpyHostToDevice, b->stream));
-    if (!lp_only)
-        PCD_HIP(hipMemcpyAsync(b->d_gtiles, b->h_gtiles, sizeof(int2) * b->cap, hipMemcpyHostToDevice, b->stream));
+    PCD_TRY(kcopy(b, (char *)b->d_items + up0, hb + up0, off - up0));
+    PCD_TRY(kcopy(b, b->d_counts, b->h_counts, sizeof(int32_t) * NUM_LISTS));
+    if (!lp_only) PCD_TRY(kcopy(b, b->d_gtiles, b->h_gtiles, sizeof(int2) * b->cap));
     PCD_HIP(hipStreamSynchronize(b->stream));
     if (!lp_only) {
         b->items_dirty = false;
@@ -1674,6 +1827,7 @@ static int append_items(pcd_batch *b) {
     WorkItem *wp[NUM_LISTS];
     for (int l = 0; l < NUM_LISTS; l++) wp[l] = (WorkItem *)(hb + b->list_off[l] + b->list_len[l] * list_esz(l));
     int ntile = b->list_len[L_TILES];
+    int s_lo = b->cap, s_hi = -1;
     for (int s : b->pending_append) {
         const Slot &sl = b->slots[s];
         const int nt = (sl.n + VT - 1) / VT;
@@ -1682,16 +1836,19 @@ static int append_items(pcd_batch *b) {
         slot_fill(mp, sl, s, false, wp);
         b->slot_listed[s] = 1;
         b->listed_vertices += sl.n;
-        PCD_HIP(hipMemcpyAsync(b->d_gtiles + s, b->h_gtiles + s, sizeof(int2), hipMemcpyHostToDevice, b->stream));
+        s_lo = std::min(s_lo, s);
+        s_hi = std::max(s_hi, s);
     }
+    // the tile ranges of the slots in between are unchanged on both sides
+    if (s_hi >= s_lo) PCD_TRY(kcopy(b, b->d_gtiles + s_lo, b->h_gtiles + s_lo, sizeof(int2) * (size_t)(s_hi - s_lo + 1)));
     for (int l = 0; l < NUM_LISTS; l++) {
         if (!add[l]) continue;
         size_t o = b->list_off[l] + (size_t)b->list_len[l] * list_esz(l);
-        PCD_HIP(hipMemcpyAsync((char *)b->d_items + o, hb + o, add[l] * list_esz(l), hipMemcpyHostToDevice, b->stream));
+        PCD_TRY(kcopy(b, (char *)b->d_items + o, hb + o, add[l] * list_esz(l)));
         b->list_len[l] += (int)add[l];
         b->h_counts[l] = b->list_len[l];
     }
-    PCD_HIP(hipMemcpyAsync(b->d_counts, b->h_counts, sizeof(int32_t) * NUM_LISTS, hipMemcpyHostToDevice, b->stream));
+    PCD_TRY(kcopy(b, b->d_counts, b->h_counts, sizeof(int32_t) * NUM_LISTS));
     PCD_HIP(hipStreamSynchronize(b->stream));
     b->pending_append.clear();
     return check_lists(b);
@@ -1822,7 +1979,7 @@ static int ensure_graph(pcd_batch *b) {
 }
 
 static int poll_state(pcd_batch *b, int *running) {
-    PCD_HIP(hipMemcpyAsync(b->h_state, b->d_state, sizeof(BatchState) * b->cap, hipMemcpyDeviceToHost, b->stream));
+    PCD_TRY(kcopy(b, b->h_state, b->d_state, sizeof(BatchState) * b->cap));
     PCD_HIP(hipStreamSynchronize(b->stream));
     int r = 0;
     for (int s = 0; s < b->cap; s++) {
@@ -1959,7 +2116,7 @@ extern "C" int pcd_batch_release(pcd_batch *b, int slot) {
     std::memset(&st, 0, sizeof(st));
     st.status = BG_FREE;
     // stream order: earlier rounds finish before the slot is reused
-    PCD_HIP(hipMemcpyAsync(b->d_state + slot, &st, sizeof(BatchState), hipMemcpyHostToDevice, b->stream));
+    PCD_TRY(kcopy(b, b->d_state + slot, &st, sizeof(BatchState)));
     PCD_HIP(hipStreamSynchronize(b->stream));
     if (sl.status == BG_RUNNING) b->items_dirty = true;
     if (sl.bytes) arena_free(b, sl.off, sl.bytes);
