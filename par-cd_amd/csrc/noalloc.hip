@@ -136,10 +136,11 @@ extern "C" int pcd_compress_labels_noalloc(int n, const int32_t *labels, int32_t
         const int tiles = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
         const NaFlag load{n, labels, first};
         const NaStore store{labels, out_labels, newid};
-        hipLaunchKernelGGL(k_scan_tiles<NaFlag>, dim3(tiles), dim3(SCAN_THREADS), 0, s, (int64_t)n, load, parts);
+        hipLaunchKernelGGL(k_scan_tiles<NaFlag>, dim3(tiles), dim3(SCAN_THREADS), 0, s, (int64_t)n, load, parts,
+                           SCAN_ROUNDS);
         hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(SCAN_THREADS), 0, s, tiles, parts);
         hipLaunchKernelGGL((k_scan_apply<NaFlag, NaStore>), dim3(tiles), dim3(SCAN_THREADS), 0, s, (int64_t)n, load,
-                           store, parts);
+                           store, parts, SCAN_ROUNDS);
         hipLaunchKernelGGL(k_na_relabel, dim3(grid), dim3(256), 0, s, n, labels, newid, out_labels);
         hipLaunchKernelGGL(k_na_finish, dim3(1), dim3(1), 0, s, parts + tiles, flags, num_labels);
     } else {

@@ -94,14 +94,14 @@ __device__ inline int block_excl_scan_n(int x, int *total, int *lds /* >= NT/WAV
     return base + inc - x;
 }
 
-// Phase 1: per-tile sums.
+// Phase 1: per-tile sums (a tile is `rounds` x SCAN_THREADS values).
 template <class Load>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(int64_t n, Load load, int32_t *parts) {
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(int64_t n, Load load, int32_t *parts, int rounds) {
     __shared__ int lds[SCAN_THREADS / WAVE + 1];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    const int64_t base = (int64_t)blockIdx.x * rounds * SCAN_THREADS;
     int acc = 0;
 #pragma unroll 4
-    for (int r = 0; r < SCAN_ROUNDS; r++) {
+    for (int r = 0; r < rounds; r++) {
         int64_t i = base + (int64_t)r * SCAN_THREADS + threadIdx.x;
         if (i < n) acc += load(i);
     }
@@ -120,11 +120,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_parts(int nparts, int32_t
 
 // Phase 3: rescan each tile with its offset; store(i, exclusive, value).
 template <class Load, class Store>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int64_t n, Load load, Store store, const int32_t *parts) {
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int64_t n, Load load, Store store, const int32_t *parts,
+                                                             int rounds) {
     __shared__ int lds[SCAN_THREADS / WAVE + 1];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    const int64_t base = (int64_t)blockIdx.x * rounds * SCAN_THREADS;
     int run = parts[blockIdx.x];
-    for (int r = 0; r < SCAN_ROUNDS; r++) {
+    for (int r = 0; r < rounds; r++) {
         int64_t i = base + (int64_t)r * SCAN_THREADS + threadIdx.x;
         int x = i < n ? load(i) : 0;
         int tot;
@@ -134,19 +135,31 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int64_t n, Load loa
     }
 }
 
+// Rounds per scan tile for n values: SCAN_ROUNDS while that leaves at
+// least two tiles per CU, fewer for smaller n, so that a small scan is a
+// few short rounds on many workgroups rather than 16 serial rounds on a
+// few (its time is then latency, not work).
+inline int scan_rounds(const pcd_ctx *ctx, int64_t n) {
+    int r = SCAN_ROUNDS;
+    while (r > 1 && (n + (int64_t)r * SCAN_THREADS - 1) / ((int64_t)r * SCAN_THREADS) < 2 * (int64_t)ctx->cu_count) r /= 2;
+    return r;
+}
+
 // Exclusive scan over [0, n): load(i) gives the value, store(i, prefix,
 // value) consumes it.  The total lands in ctx->d_scan_part[num_tiles].
 // Returns the device address of the total.
 template <class Load, class Store>
 int device_scan(pcd_ctx *ctx, int64_t n, Load load, Store store, int32_t **total_dev) {
     PCD_TRY(ensure_scan_parts(ctx, n));
-    int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const int rounds = scan_rounds(ctx, n);
+    int64_t tiles = (n + (int64_t)rounds * SCAN_THREADS - 1) / ((int64_t)rounds * SCAN_THREADS);
     if (tiles < 1) tiles = 1;
+    if (tiles + 1 > ctx->scan_part_cap) return fail(PCD_E_STATE, "scan tile sums exceed their buffer");
     hipLaunchKernelGGL(k_scan_tiles<Load>, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, ctx->stream, n, load,
-                       ctx->d_scan_part);
+                       ctx->d_scan_part, rounds);
     hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, (int)tiles, ctx->d_scan_part);
     hipLaunchKernelGGL((k_scan_apply<Load, Store>), dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, ctx->stream, n,
-                       load, store, ctx->d_scan_part);
+                       load, store, ctx->d_scan_part, rounds);
     PCD_HIP(hipGetLastError());
     if (total_dev) *total_dev = ctx->d_scan_part + tiles;
     return PCD_OK;

@@ -81,7 +81,8 @@ __global__ __launch_bounds__(256) void k_gather_probe(int64_t E, const int32_t *
 }
 
 int ensure_scan_parts(pcd_ctx *ctx, int64_t n) {
-    int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    // room for the tile sums at any tile size scan_rounds picks for up to n
+    int64_t tiles = (n + SCAN_THREADS - 1) / SCAN_THREADS;
     if (tiles < 1) tiles = 1;
     if (tiles + 1 <= ctx->scan_part_cap) return PCD_OK;
     if (ctx->d_scan_part) dev_free(ctx, ctx->d_scan_part);

@@ -1335,8 +1335,10 @@ static int launch_propagation(pcd_ctx *ctx, const RowSet &rs, const LPArgs &a) {
         }
         sb.dlo = MERGE ? rs.layout + NUM_BINS + 1 : nullptr;
         if (nonempty > 0 || dev) {
-            // one workgroup per non-empty bin, the rest by entries
-            const int spare = ctx->cu_count * 16 - nonempty;
+            // one workgroup per non-empty bin, the rest by entries: up to
+            // 16 per CU, and no more than one per 2048 entries, so that a
+            // small graph's launch is not mostly empty workgroups
+            const int spare = (int)std::min<int64_t>(ctx->cu_count * 16 - nonempty, tot / 2048);
             int at = 0;
             int64_t acc = 0;
             for (int i = 0; i < 6; i++) {
