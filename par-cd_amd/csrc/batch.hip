@@ -1355,6 +1355,26 @@ static int grow_device(T **p, size_t *cap, size_t count, const char *what) {
     return PCD_OK;
 }
 
+// Gives back the slots and arena space of an asynchronous admission's
+// graphs from call-local index `from` on, whose chunks were not queued
+// because an error ended the call: they are reported refused with `code`.
+static void abandon_async(pcd_batch *b, int from, int count, int code, int *slots_out, int *codes_out) {
+    if (b->astream) (void)hipStreamSynchronize(b->astream);
+    for (int i = from; i < count; i++) {
+        const int s = b->aslots[i];
+        if (s < 0) continue;
+        Slot &sl = b->slots[s];
+        if (sl.used && sl.status == BG_PENDING) {
+            arena_free(b, sl.off, sl.bytes);
+            sl = Slot();
+            b->apending--;
+        }
+        b->aslots[i] = -1;
+        slots_out[i] = -1;
+        if (codes_out) codes_out[i] = code;
+    }
+}
+
 static void slot_counts(const BinMap &mp, const Slot &sl, bool lp_only, int64_t *cnt);
 static size_t list_esz(int);
 
@@ -1493,7 +1513,10 @@ extern "C" int pcd_batch_add_many_async(pcd_batch *b, int count, const pcd_graph
                     msg = pcd_last_error();
                 }
             }
-            if (rc == PCD_E_HIP) return bfail(rc, "graph " + std::to_string(i) + ": " + msg);
+            if (rc == PCD_E_HIP) {
+                abandon_async(b, 0, count, rc, slots_out, codes_out);
+                return bfail(rc, "graph " + std::to_string(i) + ": " + msg);
+            }
             if (codes_out) codes_out[i] = rc;
             if (rc != PCD_OK && first_code == PCD_OK) {
                 first_code = rc;
@@ -1508,129 +1531,142 @@ extern "C" int pcd_batch_add_many_async(pcd_batch *b, int count, const pcd_graph
         sp.i1 = items.size();
         spans.push_back(sp);
     }
-    PCD_TRY(reserve_lists(b));
-    // Kernel reads of host memory slow the rounds running beside them (each
-    // waiting read holds the memory pipeline); copy-engine transfers do not.
-    // So per chunk the pieces are sorted by source address, pieces that
-    // continue one another inside one page-locked allocation form one copy
-    // into device staging, and the chunk's launch then moves each piece from
-    // there to its place.  A piece whose allocation the runtime does not
-    // report is read from host memory by that launch, as before.
-    struct Run {
-        const char *src;
-        size_t off, bytes;
-    };
-    std::vector<Run> runs;
-    std::vector<size_t> run_first(spans.size() + 1, 0), soff(pieces.size(), SIZE_MAX);
-    std::vector<const char *> pbase(pieces.size(), nullptr);
-    {
-        const char *cb = nullptr, *ce = nullptr;  // the last allocation found
-        const char *sb = (const char *)b->h_astage, *se = sb + 4 * b->astage_cap;
-        for (size_t k = 0; k < pieces.size(); k++) {
-            const char *p = (const char *)pieces[k].src, *q = p + 4 * (size_t)pieces[k].words;
-            if (sb && p >= sb && q <= se) {
-                pbase[k] = sb;
-                continue;
-            }
-            if (!(cb && p >= cb && q <= ce)) {
-                hipDeviceptr_t base = nullptr;
-                size_t sz = 0;
-                if (hipMemGetAddressRange(&base, &sz, (hipDeviceptr_t)p) == hipSuccess && base &&
-                    (const char *)base <= p && q <= (const char *)base + sz) {
-                    cb = (const char *)base;
-                    ce = cb + sz;
-                } else {
-                    (void)hipGetLastError();
-                    cb = ce = nullptr;
+    // everything below queues work; on an error the graphs of the chunks
+    // not yet queued give their slots back
+    int queued = 0;  // graphs whose chunk has its event
+    auto queue_all = [&]() -> int {
+        PCD_TRY(reserve_lists(b));
+        // Kernel reads of host memory slow the rounds running beside them (each
+        // waiting read holds the memory pipeline); copy-engine transfers do not.
+        // So per chunk the pieces are sorted by source address, pieces that
+        // continue one another inside one page-locked allocation form one copy
+        // into device staging, and the chunk's launch then moves each piece from
+        // there to its place.  A piece whose allocation the runtime does not
+        // report is read from host memory by that launch.
+        struct Run {
+            const char *src;
+            size_t off, bytes;
+        };
+        std::vector<Run> runs;
+        std::vector<size_t> run_first(spans.size() + 1, 0), soff(pieces.size(), SIZE_MAX);
+        std::vector<const char *> pbase(pieces.size(), nullptr);
+        {
+            const char *cb = nullptr, *ce = nullptr;  // the last allocation found
+            const char *sb = (const char *)b->h_astage, *se = sb + 4 * b->astage_cap;
+            for (size_t k = 0; k < pieces.size(); k++) {
+                const char *p = (const char *)pieces[k].src, *q = p + 4 * (size_t)pieces[k].words;
+                if (sb && p >= sb && q <= se) {
+                    pbase[k] = sb;
+                    continue;
                 }
+                if (!(cb && p >= cb && q <= ce)) {
+                    hipDeviceptr_t base = nullptr;
+                    size_t sz = 0;
+                    if (hipMemGetAddressRange(&base, &sz, (hipDeviceptr_t)p) == hipSuccess && base &&
+                        (const char *)base <= p && q <= (const char *)base + sz) {
+                        cb = (const char *)base;
+                        ce = cb + sz;
+                    } else {
+                        (void)hipGetLastError();
+                        cb = ce = nullptr;
+                    }
+                }
+                pbase[k] = cb;
             }
-            pbase[k] = cb;
         }
-    }
-    size_t stage_bytes = 0;
-    for (size_t c = 0; c < spans.size(); c++) {
-        run_first[c] = runs.size();
-        std::vector<size_t> ord;
-        for (size_t k = spans[c].p0; k < spans[c].p1; k++)
-            if (pbase[k]) ord.push_back(k);
-        std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return pieces[x].src < pieces[y].src; });
-        size_t off = 0;
-        const char *run_base = nullptr;
-        for (size_t k : ord) {
-            const char *p = (const char *)pieces[k].src;
-            const size_t nb = 4 * (size_t)pieces[k].words;
-            if (runs.size() > run_first[c] && pbase[k] == run_base && p == runs.back().src + runs.back().bytes) {
-                soff[k] = runs.back().off + runs.back().bytes;
-                runs.back().bytes += nb;
-            } else {
-                off = ((off + 15) & ~(size_t)15) + ((uintptr_t)p & 15);  // same alignment as the source
-                runs.push_back(Run{p, off, nb});
-                run_base = pbase[k];
-                soff[k] = off;
+        size_t stage_bytes = 0;
+        for (size_t c = 0; c < spans.size(); c++) {
+            run_first[c] = runs.size();
+            std::vector<size_t> ord;
+            for (size_t k = spans[c].p0; k < spans[c].p1; k++)
+                if (pbase[k]) ord.push_back(k);
+            std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return pieces[x].src < pieces[y].src; });
+            size_t off = 0;
+            const char *run_base = nullptr;
+            for (size_t k : ord) {
+                const char *p = (const char *)pieces[k].src;
+                const size_t nb = 4 * (size_t)pieces[k].words;
+                if (runs.size() > run_first[c] && pbase[k] == run_base && p == runs.back().src + runs.back().bytes) {
+                    soff[k] = runs.back().off + runs.back().bytes;
+                    runs.back().bytes += nb;
+                } else {
+                    off = ((off + 15) & ~(size_t)15) + ((uintptr_t)p & 15);  // same alignment as the source
+                    runs.push_back(Run{p, off, nb});
+                    run_base = pbase[k];
+                    soff[k] = off;
+                }
+                off = runs.back().off + runs.back().bytes;
             }
-            off = runs.back().off + runs.back().bytes;
+            stage_bytes = std::max(stage_bytes, off);
         }
-        stage_bytes = std::max(stage_bytes, off);
-    }
-    run_first[spans.size()] = runs.size();
-    PCD_TRY(grow_device(&b->d_adma, &b->adma_cap, std::max<size_t>(stage_bytes, 16), "admission staging"));
-    for (size_t k = 0; k < pieces.size(); k++)
-        if (soff[k] != SIZE_MAX) pieces[k].src = (const int32_t *)(b->d_adma + soff[k]);
-    // device copies of the piece, item and record lists, then per chunk:
-    // copies into staging, the pieces to their places, row check, flags back
-    // to the host, event
-    PCD_TRY(grow_device(&b->d_apulls, &b->apulls_cap, std::max<size_t>(pieces.size(), 1), "admission copy list"));
-    PCD_TRY(grow_device(&b->d_acheck, &b->acheck_cap, std::max<size_t>(items.size(), 1), "row check list"));
-    PCD_TRY(grow_device(&b->d_arecs, &b->arecs_cap, (size_t)count, "row check records"));
-    if ((size_t)count > b->abad_cap) {
-        if (b->d_abad) PCD_HIP(hipFree(b->d_abad));
-        if (b->h_abad) PCD_HIP(hipHostFree(b->h_abad));
-        b->d_abad = b->h_abad = nullptr;
-        b->abad_cap = 0;
-        const size_t c = (size_t)count + count / 2 + 64;
-        if (hipMalloc((void **)&b->d_abad, 4 * c) != hipSuccess ||
-            hipHostMalloc((void **)&b->h_abad, 4 * c, hipHostMallocDefault) != hipSuccess) {
-            (void)hipGetLastError();
-            if (b->d_abad) (void)hipFree(b->d_abad);
-            b->d_abad = nullptr;
-            b->h_abad = nullptr;
-            return bfail(PCD_E_NOMEM, "row check flags");
+        run_first[spans.size()] = runs.size();
+        PCD_TRY(grow_device(&b->d_adma, &b->adma_cap, std::max<size_t>(stage_bytes, 16), "admission staging"));
+        for (size_t k = 0; k < pieces.size(); k++)
+            if (soff[k] != SIZE_MAX) pieces[k].src = (const int32_t *)(b->d_adma + soff[k]);
+        // device copies of the piece, item and record lists, then per chunk:
+        // copies into staging, the pieces to their places, row check, flags back
+        // to the host, event
+        PCD_TRY(grow_device(&b->d_apulls, &b->apulls_cap, std::max<size_t>(pieces.size(), 1), "admission copy list"));
+        PCD_TRY(grow_device(&b->d_acheck, &b->acheck_cap, std::max<size_t>(items.size(), 1), "row check list"));
+        PCD_TRY(grow_device(&b->d_arecs, &b->arecs_cap, (size_t)count, "row check records"));
+        if ((size_t)count > b->abad_cap) {
+            if (b->d_abad) PCD_HIP(hipFree(b->d_abad));
+            if (b->h_abad) PCD_HIP(hipHostFree(b->h_abad));
+            b->d_abad = b->h_abad = nullptr;
+            b->abad_cap = 0;
+            const size_t c = (size_t)count + count / 2 + 64;
+            if (hipMalloc((void **)&b->d_abad, 4 * c) != hipSuccess ||
+                hipHostMalloc((void **)&b->h_abad, 4 * c, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                if (b->d_abad) (void)hipFree(b->d_abad);
+                b->d_abad = nullptr;
+                b->h_abad = nullptr;
+                return bfail(PCD_E_NOMEM, "row check flags");
+            }
+            b->abad_cap = c;
         }
-        b->abad_cap = c;
-    }
-    hipStream_t as = b->astream;
-    if (!pieces.empty())
-        PCD_HIP(hipMemcpyAsync(b->d_apulls, pieces.data(), sizeof(PullPiece) * pieces.size(), hipMemcpyHostToDevice, as));
-    if (!items.empty())
-        PCD_HIP(hipMemcpyAsync(b->d_acheck, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice, as));
-    PCD_HIP(hipMemcpyAsync(b->d_arecs, recs.data(), sizeof(BatchGraph) * count, hipMemcpyHostToDevice, as));
-    PCD_HIP(hipMemsetAsync(b->d_abad, 0, 4 * (size_t)count, as));
-    for (size_t c = 0; c < spans.size(); c++) {
-        const Span &sp = spans[c];
-        const int first = (int)c * CHUNK, cnt = std::min(count - first, CHUNK);
-        // at most 16 MiB per transfer: a copy on the batch stream waits for
-        // the transfer in flight on the engine, not for the queue behind it
-        constexpr size_t DMA_BYTES = 16 << 20;
-        for (size_t r = run_first[c]; r < run_first[c + 1]; r++)
-            for (size_t o = 0; o < runs[r].bytes; o += DMA_BYTES)
-                PCD_HIP(hipMemcpyAsync(b->d_adma + runs[r].off + o, runs[r].src + o,
-                                       std::min(DMA_BYTES, runs[r].bytes - o), hipMemcpyHostToDevice, as));
-        if (sp.p1 > sp.p0)
-            hipLaunchKernelGGL(k_b_pull, dim3(std::min<size_t>(sp.p1 - sp.p0, (size_t)b->cu_count * 2)), dim3(256), 0, as,
-                               (const PullPiece *)(b->d_apulls + sp.p0), (int)(sp.p1 - sp.p0));
-        if (sp.i1 > sp.i0)
-            hipLaunchKernelGGL(k_b_check_rows, dim3(std::min<size_t>(sp.i1 - sp.i0, (size_t)b->cu_count * 8)), dim3(256),
-                               0, as, (const WorkItem *)(b->d_acheck + sp.i0), (int)(sp.i1 - sp.i0),
-                               (const BatchGraph *)b->d_arecs, b->d_abad);
-        PCD_HIP(hipGetLastError());
-        PCD_HIP(hipMemcpyAsync(b->h_abad + first, b->d_abad + first, 4 * (size_t)cnt, hipMemcpyDeviceToHost, as));
-        if (b->aevents.size() <= c) {
-            hipEvent_t e = nullptr;
-            PCD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            b->aevents.push_back(e);
+        hipStream_t as = b->astream;
+        if (!pieces.empty())
+            PCD_HIP(hipMemcpyAsync(b->d_apulls, pieces.data(), sizeof(PullPiece) * pieces.size(), hipMemcpyHostToDevice, as));
+        if (!items.empty())
+            PCD_HIP(hipMemcpyAsync(b->d_acheck, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice, as));
+        PCD_HIP(hipMemcpyAsync(b->d_arecs, recs.data(), sizeof(BatchGraph) * count, hipMemcpyHostToDevice, as));
+        PCD_HIP(hipMemsetAsync(b->d_abad, 0, 4 * (size_t)count, as));
+        for (size_t c = 0; c < spans.size(); c++) {
+            const Span &sp = spans[c];
+            const int first = (int)c * CHUNK, cnt = std::min(count - first, CHUNK);
+            // at most 16 MiB per transfer: a copy on the batch stream waits for
+            // the transfer in flight on the engine, not for the queue behind it
+            constexpr size_t DMA_BYTES = 16 << 20;
+            for (size_t r = run_first[c]; r < run_first[c + 1]; r++)
+                for (size_t o = 0; o < runs[r].bytes; o += DMA_BYTES)
+                    PCD_HIP(hipMemcpyAsync(b->d_adma + runs[r].off + o, runs[r].src + o,
+                                           std::min(DMA_BYTES, runs[r].bytes - o), hipMemcpyHostToDevice, as));
+            if (sp.p1 > sp.p0)
+                hipLaunchKernelGGL(k_b_pull, dim3(std::min<size_t>(sp.p1 - sp.p0, (size_t)b->cu_count * 2)), dim3(256), 0, as,
+                                   (const PullPiece *)(b->d_apulls + sp.p0), (int)(sp.p1 - sp.p0));
+            if (sp.i1 > sp.i0)
+                hipLaunchKernelGGL(k_b_check_rows, dim3(std::min<size_t>(sp.i1 - sp.i0, (size_t)b->cu_count * 8)), dim3(256),
+                                   0, as, (const WorkItem *)(b->d_acheck + sp.i0), (int)(sp.i1 - sp.i0),
+                                   (const BatchGraph *)b->d_arecs, b->d_abad);
+            PCD_HIP(hipGetLastError());
+            PCD_HIP(hipMemcpyAsync(b->h_abad + first, b->d_abad + first, 4 * (size_t)cnt, hipMemcpyDeviceToHost, as));
+            if (b->aevents.size() <= c) {
+                hipEvent_t e = nullptr;
+                PCD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                b->aevents.push_back(e);
+            }
+            PCD_HIP(hipEventRecord(b->aevents[c], as));
+            b->achunks.push_back(pcd_batch::AChunk{first, cnt, b->aevents[c]});
+            queued = first + cnt;
         }
-        PCD_HIP(hipEventRecord(b->aevents[c], as));
-        b->achunks.push_back(pcd_batch::AChunk{first, cnt, b->aevents[c]});
+        return PCD_OK;
+    };
+    const int qrc = queue_all();
+    if (qrc != PCD_OK) {
+        const std::string msg = pcd_last_error();
+        abandon_async(b, queued, count, qrc, slots_out, codes_out);
+        return bfail(qrc, msg);
     }
     if (first_code != PCD_OK) return bfail(first_code, "graph " + std::to_string(first_index) + ": " + first_msg);
     return PCD_OK;
