@@ -200,12 +200,20 @@ int pcd_bin_counters(pcd_ctx *ctx, int bin, int64_t *active, int64_t *entries, i
  * default: a propagation step runs the hub rows and degrees 1025-4096 on two
  * side streams of the context, beside the other bins; 0: one stream) and
  * "bin_streams_min_edges" (graphs with fewer edges keep one stream; default
- * 2^19).  Results do not depend on them. */
+ * 2^19), "onesweep" (0/1: radix sorts by one launch per pass with decoupled
+ * look-back instead of three) and "os_spin_limit" (polls a look-back waits
+ * before it gives up and the sort is finished by three-launch passes;
+ * default 2^20; 0 makes the second pass of every sort of more than one tile
+ * give up, for tests of that fallback).  Results do not depend on them. */
 int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value);
 /* Hub label partitions whose distinct labels overflowed their LDS table
  * since the last call (those hubs were recounted in global memory); the
  * counter is reset by the call. */
 int pcd_hub_overflows(pcd_ctx *ctx, int64_t *count);
+/* One-sweep radix sorts (tunable "onesweep") whose look-back gave up
+ * waiting, so that three-launch passes finished them, since the last call;
+ * the counter is reset by the call. */
+int pcd_sort_fallbacks(pcd_ctx *ctx, int64_t *count);
 /* Streaming device-to-device copy of `bytes` (16-byte accesses), `reps`
  * times: *gbs receives the attained read+write bandwidth in GB/s. */
 int pcd_copy_probe(pcd_ctx *ctx, int64_t bytes, int reps, double *gbs);

@@ -875,6 +875,9 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.cluster_rounds = value;
     } else if (k == "onesweep") {
         ctx->tune.onesweep = value != 0;
+    } else if (k == "os_spin_limit") {
+        if (value < 0) return fail(PCD_E_INVALID, "os_spin_limit must be >= 0");
+        ctx->tune.os_spin_limit = value;
     } else if (k == "merge_sort") {
         ctx->tune.merge_sort = value != 0;
     } else if (k == "graph_loop") {
@@ -948,6 +951,13 @@ extern "C" int pcd_kernel_timer(pcd_ctx *ctx, int slot, double *total_ms, int64_
     }
     if (total_ms) *total_ms = t;
     if (launches) *launches = (int64_t)ctx->ktimes[slot].size();
+    return PCD_OK;
+}
+
+extern "C" int pcd_sort_fallbacks(pcd_ctx *ctx, int64_t *count) {
+    PCD_TRY(check_ctx(ctx));
+    if (count) *count = ctx->os_fallbacks;
+    ctx->os_fallbacks = 0;
     return PCD_OK;
 }
 

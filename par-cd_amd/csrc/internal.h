@@ -109,6 +109,7 @@ struct Tuning {
                                  // or tile histograms, a scan and a scatter per pass (0).  Off:
                                  // its scatter costs what the three-launch pass's does, and its
                                  // all-digit histogram is slower on keys with long equal runs
+    int os_spin_limit = 1 << 20; // polls a one-sweep look-back waits before the sort falls back
     int graph_loop = 1;          // modularity / CPM with -s 0: passes replayed from one captured
                                  // pass with the stop rule on the device (pcd_iterate_many)
     int graph_loop_min_entries = 0;  // graphs with fewer CSR entries run pass by pass

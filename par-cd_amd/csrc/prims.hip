@@ -246,11 +246,14 @@ static int lsd_passes(pcd_ctx *ctx, uint64_t *src, uint64_t *dst, int64_t n, int
 // Status word per (tile, digit): bits 31-30 the state (0 none, 1
 // aggregate, 2 inclusive prefix), bits 29-0 the count, so n < 2^30.  A
 // wait polls with a bounded number of sleeps; a workgroup that runs out
-// sets the error flag and moves on, and the host redoes that pass with the
-// three-launch kernels, so every wait ends and the result stays exact.
+// records its pass (plus one) in the error flag and moves on, and the
+// host, which reads the flag after each pass, redoes the sort from that
+// pass with the three-launch kernels, so every wait ends and the result
+// stays exact.  (Reading the flag once after the last pass instead, with
+// later passes skipping their work, measured slower: the scatter launches
+// took twice as long back to back.)
 constexpr int OS_MAX_PASSES = 8;
 constexpr uint32_t OS_AGG = 1u << 30, OS_INC = 2u << 30, OS_VAL = (1u << 30) - 1;
-constexpr int OS_SPIN_LIMIT = 1 << 20;
 constexpr int OS_WIN = 16;  // look-back words read at once per digit
 
 __global__ __launch_bounds__(RS_THREADS) void k_os_hist(const uint64_t *keys, int64_t n, int passes,
@@ -259,9 +262,33 @@ __global__ __launch_bounds__(RS_THREADS) void k_os_hist(const uint64_t *keys, in
     const int t = threadIdx.x, w = t / WAVE;
     for (int i = t; i < RS_THREADS / WAVE * OS_MAX_PASSES * 256; i += RS_THREADS) (&h[0][0])[i] = 0;
     __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * RS_THREADS + t; i < n; i += (int64_t)gridDim.x * RS_THREADS) {
-        const uint64_t k = keys[i];
-        for (int p = 0; p < passes; p++) atomicAdd(&h[w][p * 256 + (int)((k >> (8 * p)) & 0xFF)], 1u);
+    // Keys of neighbouring entries often share their high digits (pair keys
+    // of one community row share the community), and 64 lanes adding to one
+    // LDS counter serialise.  So per pass, up to four times, the lanes with
+    // the digit of the first lane still to count add it with one atomic;
+    // the lanes left (digits spread out) add one each.  Whole waves iterate
+    // together, so every lane takes part in the ballots and the shuffle.
+    const int lane = __lane_id();
+    for (int64_t b0 = (int64_t)blockIdx.x * RS_THREADS; b0 < n; b0 += (int64_t)gridDim.x * RS_THREADS) {
+        const int64_t i = b0 + t;
+        const bool valid = i < n;
+        const uint64_t k = valid ? keys[i] : 0;
+        for (int p = 0; p < passes; p++) {
+            const int d = (int)((k >> (8 * p)) & 0xFF);
+            bool todo = valid;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint64_t rem = __ballot(todo);
+                if (rem == 0) break;
+                const int leader = __ffsll((unsigned long long)rem) - 1;
+                const int dl = __shfl(d, leader, WAVE);
+                const bool mine = todo && d == dl;
+                const uint64_t same = __ballot(mine);
+                if (lane == leader) atomicAdd(&h[w][p * 256 + dl], (uint32_t)__popcll(same));
+                todo = todo && !mine;
+            }
+            if (todo) atomicAdd(&h[w][p * 256 + d], 1u);
+        }
     }
     __syncthreads();
     for (int i = t; i < passes * 256; i += RS_THREADS) {
@@ -283,7 +310,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_os_bases(int passes, const uint3
 
 __global__ __launch_bounds__(RS_THREADS) void k_os_scatter(const uint64_t *keys, uint64_t *out, int64_t n, int shift,
                                                            const uint32_t *digit_base, uint32_t *status, int *ticket,
-                                                           int *err) {
+                                                           int *err, int pass, int spin_limit) {
     __shared__ uint64_t sk[RS_TILE];
     __shared__ int cnt[256];
     __shared__ int lstart[256];
@@ -348,6 +375,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_os_scatter(const uint64_t *keys,
     // OS_WIN earlier tiles at once (nearest first), adding aggregates until
     // an inclusive prefix; a word not yet published is read again
     uint32_t prefix = 0;
+    // spin limit 0 (a host setting for tests of the fallback): in pass 1
+    // every tile after the first gives up as a look-back that ran out of
+    // patience would
+    if (spin_limit == 0 && pass == 1 && tile > 0) atomicCAS(err, 0, pass + 1);
     if (tile > 0) {
         int j = tile - 1, spins = 0;
         while (j >= 0) {
@@ -374,8 +405,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_os_scatter(const uint64_t *keys,
             if (found) break;
             j -= used;
             if (waiting) {
-                if (++spins > OS_SPIN_LIMIT) {
-                    atomicOr(err, 1);
+                if (++spins > spin_limit) {
+                    atomicCAS(err, 0, pass + 1);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(2);
@@ -427,12 +458,13 @@ int radix_sort_u64(pcd_ctx *ctx, uint64_t *keys, uint64_t *tmp, int64_t n, int b
     for (int p = 0; p < passes; p++) {
         PCD_TRY(fill_i32(ctx, (int32_t *)status, 0, (int64_t)ntiles * 256));
         hipLaunchKernelGGL(k_os_scatter, dim3(ntiles), dim3(RS_THREADS), 0, ctx->stream, src, dst, n, 8 * p,
-                           dbase + 256 * p, status, tickets + p, err);
+                           dbase + 256 * p, status, tickets + p, err, p, ctx->tune.os_spin_limit);
         PCD_HIP(hipGetLastError());
         // a look-back that ran out of patience: redo this pass the other way
         PCD_HIP(hipMemcpyAsync(ctx->h_small + 12, err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         PCD_HIP(hipStreamSynchronize(ctx->stream));
         if (ctx->h_small[12] != 0) {
+            if (ctx->h_small[12] != p + 1) return fail(PCD_E_STATE, "one-sweep sort: bad failed-pass record");
             ctx->os_fallbacks++;
             PCD_TRY(fill_i32(ctx, err, 0, 1));
             return lsd_passes(ctx, src, dst, n, 8 * p, bits, out);

@@ -37,7 +37,7 @@ EXPORTS = [
     "pcd_server_info", "pcd_gen_requests",
     "pcd_graphset_open", "pcd_graphset_close", "pcd_graphset_count", "pcd_graphset_graph", "pcd_graphset_extract",
     "pcd_graphset_writer_open", "pcd_graphset_writer_add", "pcd_graphset_writer_close", "pcd_graphset_pack",
-    "pcd_compress_labels_noalloc", "pcd_iterate_many", "pcd_batch_add_many_async",
+    "pcd_compress_labels_noalloc", "pcd_iterate_many", "pcd_batch_add_many_async", "pcd_sort_fallbacks",
 ]
 BATCH_RUNNING, BATCH_DONE, BATCH_FREE, BATCH_PENDING, BATCH_REJECTED = 0, 1, 2, 3, 4
 
@@ -148,6 +148,7 @@ def lib():
     L.pcd_singleton_partition.argtypes = [C.c_int, vp]
     L.pcd_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
     L.pcd_hub_overflows.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.pcd_sort_fallbacks.argtypes = [vp, C.POINTER(C.c_int64)]
     L.pcd_copy_probe.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
     L.pcd_gather_probe.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
     ip, dp, lp = C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int64)
@@ -641,6 +642,12 @@ class Context:
     def hub_overflows(self):
         c = C.c_int64()
         check(lib().pcd_hub_overflows(self._ctx, C.byref(c)))
+        return c.value
+
+    def sort_fallbacks(self):
+        """One-sweep sorts finished by three-launch passes since the last call."""
+        c = C.c_int64()
+        check(lib().pcd_sort_fallbacks(self._ctx, C.byref(c)))
         return c.value
 
     def kernel_timer_select(self, names=None):

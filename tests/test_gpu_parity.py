@@ -441,6 +441,30 @@ def test_contraction_pairs(ctx, onesweep):
         ctx.set_tuning("onesweep", 0)
 
 
+def test_contraction_pairs_one_sweep_fallback(ctx):
+    """One-sweep sort whose second pass gives up (spin limit 0, set from the
+    host: every wait still ends): the pass records itself, and three-launch
+    passes redo the sort from that pass's input (the second buffer), so the
+    pairs stay exact."""
+    cases = [GG.rmat(13), GG.rmat(16)]
+    rng = np.random.default_rng(19)
+    ctx.set_tuning("onesweep", 1)
+    ctx.set_tuning("os_spin_limit", 0)
+    try:
+        ctx.sort_fallbacks()
+        for g in cases:
+            n, m, ind, adj = g[:4]
+            for lab in [GG.singleton_labels(n), rng.integers(0, max(2, n // 30), n)]:
+                lab_o, so, ivo, evo = check_partition(ctx, g, lab)
+                l, k, w = ctx.contract()
+                lo, ko, wo = O.community_pairs(n, ind, adj, lab_o, len(so))
+                assert np.array_equal(l, lo) and np.array_equal(k, ko) and np.array_equal(w, wo)
+        assert ctx.sort_fallbacks() > 0
+    finally:
+        ctx.set_tuning("os_spin_limit", 1 << 20)
+        ctx.set_tuning("onesweep", 0)
+
+
 @pytest.mark.parametrize("merge_sort", [0, 1])
 @pytest.mark.parametrize("tie", [O.TIE_RANDOM, O.TIE_LOWEST])
 @pytest.mark.parametrize("kind,lam", RULES + [(O.WCC, 0.0)])
