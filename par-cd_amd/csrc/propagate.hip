@@ -1469,8 +1469,12 @@ int batch_propagation_launch(hipStream_t stream, const BatchLPLaunch &l) {
         hipLaunchKernelGGL((k_lp_block<512, 2048, RK_BATCH, ItemSrc>), dim3(cu * 4), dim3(512), 0, stream, src(LPV_B512));
     if (l.used[LPV_B256])
         hipLaunchKernelGGL((k_lp_block<256, 2048, RK_BATCH, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_B256));
+    // the plain path's form of the degree 257-512 wave kernel: all of a
+    // lane's entries gathered at once, 16-bit counts (batch degrees are at
+    // most 4096)
     if (l.used[LPV_W1024])
-        hipLaunchKernelGGL((k_lp_wave<1024, RK_BATCH, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W1024));
+        hipLaunchKernelGGL((k_lp_wave<1024, RK_BATCH, ItemSrc, false, 8, uint16_t>), dim3(cu * 8), dim3(256), 0, stream,
+                           src(LPV_W1024));
     if (l.used[LPV_W512])
         hipLaunchKernelGGL((k_lp_wave<512, RK_BATCH, ItemSrc>), dim3(cu * 8), dim3(256), 0, stream, src(LPV_W512));
     if (l.used[LPV_S64])
