@@ -204,7 +204,9 @@ int pcd_bin_counters(pcd_ctx *ctx, int bin, int64_t *active, int64_t *entries, i
  * look-back instead of three) and "os_spin_limit" (polls a look-back waits
  * before it gives up and the sort is finished by three-launch passes;
  * default 2^20; 0 makes the second pass of every sort of more than one tile
- * give up, for tests of that fallback).  Results do not depend on them. */
+ * give up, for tests of that fallback) and "scan_tiles_per_cu" (scans use
+ * shorter tiles until there are this many per CU; default 8, in [1, 64]).
+ * Results do not depend on them. */
 int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value);
 /* Hub label partitions whose distinct labels overflowed their LDS table
  * since the last call (those hubs were recounted in global memory); the

@@ -875,6 +875,9 @@ extern "C" int pcd_set_tuning(pcd_ctx *ctx, const char *key, int value) {
         ctx->tune.cluster_rounds = value;
     } else if (k == "onesweep") {
         ctx->tune.onesweep = value != 0;
+    } else if (k == "scan_tiles_per_cu") {
+        if (value < 1 || value > 64) return fail(PCD_E_INVALID, "scan_tiles_per_cu must be in [1, 64]");
+        ctx->tune.scan_tiles_per_cu = value;
     } else if (k == "os_spin_limit") {
         if (value < 0) return fail(PCD_E_INVALID, "os_spin_limit must be >= 0");
         ctx->tune.os_spin_limit = value;

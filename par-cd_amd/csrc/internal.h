@@ -110,6 +110,8 @@ struct Tuning {
                                  // its scatter costs what the three-launch pass's does, and its
                                  // all-digit histogram is slower on keys with long equal runs
     int os_spin_limit = 1 << 20; // polls a one-sweep look-back waits before the sort falls back
+    int scan_tiles_per_cu = 8;   // scans: fewer rounds per tile until there are this many tiles per CU
+                                 // (R-MAT-22 iteration 1.120 ms at 2, 1.102 at 8, 1.131 at 32)
     int graph_loop = 1;          // modularity / CPM with -s 0: passes replayed from one captured
                                  // pass with the stop rule on the device (pcd_iterate_many)
     int graph_loop_min_entries = 0;  // graphs with fewer CSR entries run pass by pass

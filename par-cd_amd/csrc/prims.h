@@ -136,12 +136,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int64_t n, Load loa
 }
 
 // Rounds per scan tile for n values: SCAN_ROUNDS while that leaves at
-// least two tiles per CU, fewer for smaller n, so that a small scan is a
-// few short rounds on many workgroups rather than 16 serial rounds on a
-// few (its time is then latency, not work).
+// least scan_tiles_per_cu tiles per CU, fewer for smaller n, so that a
+// scan is a few short rounds on many workgroups rather than 16 serial
+// rounds on a few (its time is then latency, not work).
 inline int scan_rounds(const pcd_ctx *ctx, int64_t n) {
+    const int64_t want = (int64_t)ctx->tune.scan_tiles_per_cu * ctx->cu_count;
     int r = SCAN_ROUNDS;
-    while (r > 1 && (n + (int64_t)r * SCAN_THREADS - 1) / ((int64_t)r * SCAN_THREADS) < 2 * (int64_t)ctx->cu_count) r /= 2;
+    while (r > 1 && (n + (int64_t)r * SCAN_THREADS - 1) / ((int64_t)r * SCAN_THREADS) < want) r /= 2;
     return r;
 }
 
