@@ -135,15 +135,25 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int64_t n, Load loa
     }
 }
 
-// Rounds per scan tile for n values: SCAN_ROUNDS while that leaves at
-// least scan_tiles_per_cu tiles per CU, fewer for smaller n, so that a
-// scan is a few short rounds on many workgroups rather than 16 serial
-// rounds on a few (its time is then latency, not work).
+// Rounds per scan tile for n values: of 16, 8, 4, 2 and 1, the one whose
+// tile count is nearest (by ratio) to scan_tiles_per_cu tiles per CU, the
+// smaller on a tie (a small n is one tile whatever the rounds).  A scan is then a few short rounds on many workgroups
+// rather than 16 serial rounds on a few (its time is latency, not work),
+// without so many tiles that the one-workgroup scan of their sums grows.
 inline int scan_rounds(const pcd_ctx *ctx, int64_t n) {
-    const int64_t want = (int64_t)ctx->tune.scan_tiles_per_cu * ctx->cu_count;
-    int r = SCAN_ROUNDS;
-    while (r > 1 && (n + (int64_t)r * SCAN_THREADS - 1) / ((int64_t)r * SCAN_THREADS) < want) r /= 2;
-    return r;
+    const double want = (double)ctx->tune.scan_tiles_per_cu * ctx->cu_count;
+    int best = SCAN_ROUNDS;
+    double best_d = 1e300;
+    for (int r = SCAN_ROUNDS; r >= 1; r /= 2) {
+        const int64_t tiles = (n + (int64_t)r * SCAN_THREADS - 1) / ((int64_t)r * SCAN_THREADS);
+        const double ratio = tiles > 0 ? (double)tiles / want : 1e-300;
+        const double d = ratio >= 1.0 ? ratio : 1.0 / ratio;
+        if (d <= best_d) {
+            best_d = d;
+            best = r;
+        }
+    }
+    return best;
 }
 
 // Exclusive scan over [0, n): load(i) gives the value, store(i, prefix,
